@@ -1,0 +1,1070 @@
+/*
+ * devil_oracle.c -- FP64 CPU restatement of devil's per-gene NB-GLM fit.
+ *
+ * TEST INFRASTRUCTURE ONLY (the oracle).  Only tests/, the smoke check and
+ * the benchmark's cpu_baseline leg may load this library, and only as the
+ * checker or the timed CPU baseline.  The product never calls it.
+ *
+ * What each part follows (paths relative to the devil 0.99.0 tree):
+ *   orc_beta_fit            src/beta.cpp:25-51
+ *   orc_mom_dispersion      src/mom_overdispersion.cpp:6-56
+ *   orc_compute_hessian     src/variance.cpp:22-47
+ *   orc_clustered_meat      src/variance.cpp:186-228, R/main.R:481-492
+ *   orc_meat_hc0            src/variance.cpp:64-85,157-178
+ *   orc_make_table          src/overdispersion.cpp:32-46
+ *   orc_nb_loglik/score/dscore  src/overdispersion.cpp:53-88,95-178,186-242
+ *   orc_fit_dispersion      R/overdispersion.R:25-115 (special cases, start)
+ *   orc_estimate_dispersion R/overdispersion.R:129-135
+ *   orc_init_beta_qr        R/beta.R:13-20
+ *   orc_init_beta_rough     R/main.R:404-406
+ *   orc_init_beta_univariate src/beta_init_univariate.cpp:13-95
+ *   orc_cpu_fit             R/main.R:375-523
+ *   orc_test_de             R/test.R:85-134 (stats::pt, stats::p.adjust "BH")
+ *
+ * How each part is pinned (tests/test_oracle_*.py):
+ *   special functions       scipy.special digamma/polygamma/betaln/betainc,
+ *                           scipy.stats.t.sf, mpmath at 50 digits
+ *   orc_beta_fit            closed-form fixed points (intercept-only ->
+ *                           log(sum y / sum s); two-group -> group means) with
+ *                           eps -> 0; an independent numpy replay of the
+ *                           iteration on tiny inputs; iteration counts
+ *   orc_mom_dispersion      its defining formula on tiny inputs (numpy)
+ *   bread / meat            Poisson closed form (X'diag(mu)X)^-1; one cell
+ *                           per cluster => n/(n-1) * HC0; symmetry; clustered
+ *                           >= unclustered on the reference test's case
+ *   NB score / dscore       central finite differences of orc_nb_loglik
+ *                           (score = d ll / d log theta, dscore = d score /
+ *                           d log theta), table branch == per-cell branch
+ *   estimate_dispersion     R/test-overdispersion.R:1-45 formula, 0.01
+ *                           truncation, all-zero gene
+ *   init_beta_qr            normal equations at 1e-10 (test-beta.R:41-77),
+ *                           intercept-only = colMeans(log1p) (test-beta.R:
+ *                           19-39), column scaling (test-beta.R:79-100)
+ *   test_de                 scipy t.sf; BH against a direct restatement of
+ *                           p.adjust; lfc clipping (test-test.R:32-62)
+ *   orc_fit_dispersion      PARITY UNPINNED against the reference: R's
+ *                           nlminb(rel.tol = tolerance) trajectory cannot be
+ *                           run here.  What is pinned: the 0 special cases
+ *                           (test-overdispersion.R:47-65), score(root) = 0
+ *                           and the root maximising orc_nb_loglik.
+ */
+#include "devil_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define PMAX 16
+
+/* ===================================================================== */
+/* special functions                                                      */
+/* ===================================================================== */
+
+/* digamma for x > 0: upward recurrence to x >= 10, then the asymptotic
+ * series psi(x) ~ ln x - 1/(2x) - sum B_2k / (2k x^2k)  (A&S 6.3.18). */
+double orc_digamma(double x) {
+  if (isnan(x)) return x;
+  if (x <= 0.0) return NAN;
+  double r = 0.0;
+  while (x < 10.0) {
+    r -= 1.0 / x;
+    x += 1.0;
+  }
+  double f = 1.0 / (x * x);
+  double t = f * (1.0 / 12 - f * (1.0 / 120 - f * (1.0 / 252 - f * (1.0 / 240 -
+             f * (1.0 / 132 - f * (691.0 / 32760 - f * (1.0 / 12)))))));
+  return r + log(x) - 0.5 / x - t;
+}
+
+/* trigamma for x > 0: recurrence to x >= 10, then
+ * psi1(x) ~ 1/x + 1/(2x^2) + sum B_2k / x^(2k+1)  (A&S 6.4.12). */
+double orc_trigamma(double x) {
+  if (isnan(x)) return x;
+  if (x <= 0.0) return NAN;
+  double r = 0.0;
+  while (x < 10.0) {
+    r += 1.0 / (x * x);
+    x += 1.0;
+  }
+  double t = 1.0 / x, f = t * t;
+  double s = t + 0.5 * f +
+             t * f * (1.0 / 6 - f * (1.0 / 30 - f * (1.0 / 42 - f * (1.0 / 30 -
+             f * (5.0 / 66 - f * (691.0 / 2730 - f * (7.0 / 6)))))));
+  return r + s;
+}
+
+/* Stirling remainder lgamma(x) - [(x-1/2) ln x - x + ln(2 pi)/2], x >= 10 */
+static double lgcor(double x) {
+  double f = 1.0 / (x * x);
+  return (1.0 / 12 - f * (1.0 / 360 - f * (1.0 / 1260 - f * (1.0 / 1680 -
+         f * (1.0 / 1188 - f * (691.0 / 360360 - f * (1.0 / 156))))))) / x;
+}
+
+static double lgam(double x) {
+  int sg;
+  return lgamma_r(x, &sg);
+}
+
+/* log B(a,b) with the large-argument forms that avoid the cancellation of
+ * lgamma(a) against lgamma(a+b). */
+double orc_lbeta(double a, double b) {
+  double p = a < b ? a : b, q = a < b ? b : a;
+  if (!(p > 0)) return NAN;
+  const double half_log_2pi = 0.91893853320467274178;
+  if (p >= 10.0) {
+    double corr = lgcor(p) + lgcor(q) - lgcor(p + q);
+    return -0.5 * log(q) + half_log_2pi + corr + (p - 0.5) * log(p / (p + q)) +
+           q * log1p(-p / (p + q));
+  } else if (q >= 10.0) {
+    double corr = lgcor(q) - lgcor(p + q);
+    return lgam(p) + corr + p - p * log(p + q) + (q - 0.5) * log1p(-p / (p + q));
+  }
+  return lgam(p) + lgam(q) - lgam(p + q);
+}
+
+/* continued fraction of I_x(a,b) (modified Lentz) */
+static double betacf(double a, double b, double x) {
+  const double tiny = 1e-300, epsr = 1e-16;
+  double qab = a + b, qap = a + 1.0, qam = a - 1.0;
+  double c = 1.0, d = 1.0 - qab * x / qap;
+  if (fabs(d) < tiny) d = tiny;
+  d = 1.0 / d;
+  double h = d;
+  for (int m = 1; m <= 100000; ++m) {
+    double m2 = 2.0 * m;
+    double aa = m * (b - m) * x / ((qam + m2) * (a + m2));
+    d = 1.0 + aa * d;
+    if (fabs(d) < tiny) d = tiny;
+    c = 1.0 + aa / c;
+    if (fabs(c) < tiny) c = tiny;
+    d = 1.0 / d;
+    h *= d * c;
+    aa = -(a + m) * (qab + m) * x / ((a + m2) * (qap + m2));
+    d = 1.0 + aa * d;
+    if (fabs(d) < tiny) d = tiny;
+    c = 1.0 + aa / c;
+    if (fabs(c) < tiny) c = tiny;
+    d = 1.0 / d;
+    double del = d * c;
+    h *= del;
+    if (fabs(del - 1.0) < epsr) break;
+  }
+  return h;
+}
+
+/* log I_x(a,b); x and 1-x are passed separately so neither loses digits. */
+double orc_log_ibeta(double a, double b, double x, double omx) {
+  if (isnan(x) || isnan(omx)) return NAN;
+  if (x <= 0.0) return -INFINITY;
+  if (omx <= 0.0) return 0.0;
+  double lfront = a * log(x) + b * log(omx) - orc_lbeta(a, b);
+  if (x < (a + 1.0) / (a + b + 2.0)) {
+    return lfront + log(betacf(a, b, x)) - log(a);
+  }
+  double lc = lfront + log(betacf(b, a, omx)) - log(b);
+  return log1p(-exp(lc));
+}
+
+/* 2 * P(T_df > |t|) = I_{df/(df+t^2)}(df/2, 1/2). */
+double orc_pt2_upper(double t, double df, double *logp) {
+  double lp;
+  if (isnan(t) || isnan(df) || !(df > 0)) {
+    lp = NAN;
+  } else {
+    t = fabs(t);
+    if (isinf(t)) {
+      lp = -INFINITY;
+    } else if (t == 0.0) {
+      lp = 0.0;
+    } else {
+      /* log x = log df - log(df + t^2), log(1-x) = 2 log t - log(df + t^2) */
+      double lt = log(t), ldf = log(df);
+      double l_sum = (t * t > df) ? 2.0 * lt + log1p(df / t / t)
+                                  : ldf + log1p(t / df * t);
+      double lx = ldf - l_sum, lomx = 2.0 * lt - l_sum;
+      double x = exp(lx), omx = exp(lomx);
+      double a = 0.5 * df, b = 0.5;
+      double lfront = a * lx + b * lomx - orc_lbeta(a, b);
+      if (x < (a + 1.0) / (a + b + 2.0)) {
+        lp = lfront + log(betacf(a, b, x)) - log(a);
+      } else {
+        double lc = lfront + log(betacf(b, a, omx)) - log(b);
+        lp = log1p(-exp(lc));
+      }
+    }
+  }
+  if (logp) *logp = lp;
+  return exp(lp);
+}
+
+/* ===================================================================== */
+/* small dense linear algebra (column-major p x p)                       */
+/* ===================================================================== */
+
+/* LU with partial pivoting, first maximal |pivot| chosen (LAPACK idamax /
+ * Eigen PartialPivLU).  perm[k] = row swapped with row k at step k.
+ * Returns 1 if an exactly zero pivot occurred. */
+static int lu_pp(double *A, int p, int *perm) {
+  int sing = 0;
+  for (int k = 0; k < p; ++k) {
+    int r = k;
+    double best = fabs(A[k + k * p]);
+    for (int i = k + 1; i < p; ++i) {
+      double v = fabs(A[i + k * p]);
+      if (v > best) { best = v; r = i; }
+    }
+    perm[k] = r;
+    if (best == 0.0 || isnan(best)) {
+      if (best == 0.0) sing = 1;
+      if (isnan(best)) sing = 1;
+      continue;
+    }
+    if (r != k)
+      for (int j = 0; j < p; ++j) {
+        double tmp = A[k + j * p];
+        A[k + j * p] = A[r + j * p];
+        A[r + j * p] = tmp;
+      }
+    double piv = A[k + k * p];
+    for (int i = k + 1; i < p; ++i) A[i + k * p] /= piv;
+    for (int j = k + 1; j < p; ++j) {
+      double akj = A[k + j * p];
+      for (int i = k + 1; i < p; ++i) A[i + j * p] -= A[i + k * p] * akj;
+    }
+  }
+  return sing;
+}
+
+int orc_inverse(const double *A, int p, double *Ainv) {
+  double LU[PMAX * PMAX];
+  int perm[PMAX];
+  memcpy(LU, A, sizeof(double) * p * p);
+  int sing = lu_pp(LU, p, perm);
+  for (int c = 0; c < p; ++c) {
+    double v[PMAX];
+    for (int i = 0; i < p; ++i) v[i] = (i == c) ? 1.0 : 0.0;
+    for (int k = 0; k < p; ++k)
+      if (perm[k] != k) {
+        double tmp = v[k];
+        v[k] = v[perm[k]];
+        v[perm[k]] = tmp;
+      }
+    for (int i = 0; i < p; ++i)
+      for (int k = 0; k < i; ++k) v[i] -= LU[i + k * p] * v[k];
+    for (int i = p - 1; i >= 0; --i) {
+      for (int k = i + 1; k < p; ++k) v[i] -= LU[i + k * p] * v[k];
+      v[i] /= LU[i + i * p];
+    }
+    for (int i = 0; i < p; ++i) Ainv[i + c * p] = v[i];
+  }
+  return sing;
+}
+
+/* Cholesky-based inverse of an SPD matrix (armadillo inv_sympd).  Returns 1
+ * (and fills NaN) if the matrix is not positive definite. */
+static int inv_sympd(const double *A, int p, double *Ainv) {
+  double L[PMAX * PMAX];
+  memset(L, 0, sizeof(L));
+  for (int j = 0; j < p; ++j) {
+    double s = A[j + j * p];
+    for (int k = 0; k < j; ++k) s -= L[j + k * p] * L[j + k * p];
+    if (!(s > 0.0)) {
+      for (int i = 0; i < p * p; ++i) Ainv[i] = NAN;
+      return 1;
+    }
+    double d = sqrt(s);
+    L[j + j * p] = d;
+    for (int i = j + 1; i < p; ++i) {
+      double t = A[i + j * p];
+      for (int k = 0; k < j; ++k) t -= L[i + k * p] * L[j + k * p];
+      L[i + j * p] = t / d;
+    }
+  }
+  for (int c = 0; c < p; ++c) {
+    double v[PMAX];
+    for (int i = 0; i < p; ++i) v[i] = (i == c) ? 1.0 : 0.0;
+    for (int i = 0; i < p; ++i) {
+      for (int k = 0; k < i; ++k) v[i] -= L[i + k * p] * v[k];
+      v[i] /= L[i + i * p];
+    }
+    for (int i = p - 1; i >= 0; --i) {
+      for (int k = i + 1; k < p; ++k) v[i] -= L[k + i * p] * v[k];
+      v[i] /= L[i + i * p];
+    }
+    for (int i = 0; i < p; ++i) Ainv[i + c * p] = v[i];
+  }
+  return 0;
+}
+
+static void matmul(const double *A, const double *B, int p, double *C) {
+  for (int j = 0; j < p; ++j)
+    for (int i = 0; i < p; ++i) {
+      double s = 0.0;
+      for (int k = 0; k < p; ++k) s += A[i + k * p] * B[k + j * p];
+      C[i + j * p] = s;
+    }
+}
+
+/* ===================================================================== */
+/* per-gene restatements                                                  */
+/* ===================================================================== */
+
+/* src/beta.cpp:25-51.  k is a float argument; k = 1.0/k is computed in
+ * double and stored back into the float (line 28); eps is a float compared
+ * after promotion (line 43). */
+int orc_beta_fit(const double *y, int64_t n, const double *X, int p,
+                 double *beta, const double *off, float kf, int max_iter,
+                 float epsf) {
+  kf = (float)(1.0 / (double)kf);
+  const double k = (double)kf;
+  const double eps = (double)epsf;
+  int iter = 0, converged = 0;
+  while (!converged && iter < max_iter) {
+    double A[PMAX * PMAX], g[PMAX], Z[PMAX * PMAX], delta[PMAX];
+    memset(A, 0, sizeof(A));
+    memset(g, 0, sizeof(g));
+    for (int64_t i = 0; i < n; ++i) {
+      double xb = 0.0;
+      for (int j = 0; j < p; ++j) xb += X[i + j * n] * beta[j];
+      double w = exp(-xb - off[i]);               /* line 38 */
+      double m = (k + y[i]) / (1.0 + k * w);      /* line 39 */
+      double mw = m * w;
+      for (int a = 0; a < p; ++a) {
+        double xa = X[i + a * n];
+        g[a] += xa * (mw - 1.0);                  /* line 41 */
+        for (int b = a; b < p; ++b) A[a + b * p] += xa * X[i + b * n] * mw; /* 40 */
+      }
+    }
+    for (int a = 0; a < p; ++a)
+      for (int b = a; b < p; ++b) {
+        A[a + b * p] *= k;
+        A[b + a * p] = A[a + b * p];
+      }
+    orc_inverse(A, p, Z);                         /* line 40 .inverse() */
+    for (int a = 0; a < p; ++a) {
+      double s = 0.0;
+      for (int b = 0; b < p; ++b) s += Z[a + b * p] * (k * g[b]);
+      delta[a] = s;
+    }
+    double mx = 0.0;
+    int has_nan = 0;
+    for (int a = 0; a < p; ++a) {
+      beta[a] += delta[a];                        /* line 42 */
+      double d = fabs(delta[a]);
+      if (isnan(d)) has_nan = 1;
+      else if (d > mx) mx = d;
+    }
+    converged = (!has_nan) && (mx < eps);        /* line 43 */
+    iter++;
+    if (delta[0] != delta[0]) converged = 1;      /* line 45 */
+  }
+  return iter;
+}
+
+/* src/mom_overdispersion.cpp:24-53 for one gene. */
+double orc_mom_dispersion(const double *y, int64_t n, const double *X, int p,
+                          const double *beta, const double *sf) {
+  double corr = (double)n / (double)(n - p);
+  double num = 0.0, den = 0.0;
+  for (int64_t j = 0; j < n; ++j) {
+    double eta = 0.0;
+    for (int k = 0; k < p; ++k) eta += X[j + k * n] * beta[k];
+    double mu = sf[j] * exp(eta);
+    double diff = y[j] - mu;
+    num += diff * diff - mu;
+    den += mu * mu;
+  }
+  double th = 0.0;
+  if (den > 0.0) {
+    th = corr * num / den;
+    if (th < 0.0) th = 0.0;
+  }
+  return th;
+}
+
+/* src/variance.cpp:22-47 */
+int orc_compute_hessian(const double *beta, double alpha, const double *y,
+                        const double *X, int64_t n, int p, const double *sf,
+                        double *bread) {
+  double H[PMAX * PMAX];
+  memset(H, 0, sizeof(H));
+  for (int64_t i = 0; i < n; ++i) {
+    double eta = 0.0;
+    for (int j = 0; j < p; ++j) eta += X[i + j * n] * beta[j];
+    double mu = sf[i] * exp(eta);
+    double denom = 1.0 + alpha * mu;
+    double s = (y[i] * alpha + 1.0) * mu / (denom * denom);
+    for (int a = 0; a < p; ++a)
+      for (int b = a; b < p; ++b) H[a + b * p] += X[i + a * n] * s * X[i + b * n];
+  }
+  for (int a = 0; a < p; ++a)
+    for (int b = a + 1; b < p; ++b) H[b + a * p] = H[a + b * p];
+  return orc_inverse(H, p, bread);
+}
+
+/* src/variance.cpp:186-228 */
+void orc_clustered_meat(const double *X, const double *y, const double *beta,
+                        double alpha, const double *sf, int64_t n, int p,
+                        const int32_t *ends, int C, double *meat) {
+  double *S = (double *)calloc((size_t)C * p, sizeof(double));
+  int64_t start = 0;
+  for (int c = 0; c < C; ++c) {
+    int64_t end = ends[c];
+    for (int64_t i = start; i < end; ++i) {
+      double eta = 0.0;
+      for (int j = 0; j < p; ++j) eta += X[i + j * n] * beta[j];
+      double mu = sf[i] * exp(eta);
+      double wr = (y[i] - mu) / (1.0 + mu * alpha);
+      for (int a = 0; a < p; ++a) S[c * p + a] += X[i + a * n] * wr;
+    }
+    start = end;
+  }
+  double adj = (C > 1) ? (double)C / (C - 1.0) : 1.0;
+  for (int a = 0; a < p; ++a)
+    for (int b = 0; b < p; ++b) {
+      double s = 0.0;
+      for (int c = 0; c < C; ++c) s += S[c * p + a] * S[c * p + b];
+      meat[a + b * p] = (adj / (double)n) * s;
+    }
+  free(S);
+}
+
+/* src/variance.cpp:157-178 with compute_scores (64-85) written in the
+ * compute_hessian convention: score_i = x_i (y_i - mu_i) / (1 + mu_i alpha). */
+void orc_meat_hc0(const double *X, const double *y, const double *beta,
+                  double alpha, const double *sf, int64_t n, int p,
+                  double *meat) {
+  double M[PMAX * PMAX];
+  memset(M, 0, sizeof(M));
+  for (int64_t i = 0; i < n; ++i) {
+    double eta = 0.0;
+    for (int j = 0; j < p; ++j) eta += X[i + j * n] * beta[j];
+    double mu = sf[i] * exp(eta);
+    double wr = (y[i] - mu) / (1.0 + mu * alpha);
+    for (int a = 0; a < p; ++a)
+      for (int b = 0; b < p; ++b)
+        M[a + b * p] += (X[i + a * n] * wr) * (X[i + b * n] * wr);
+  }
+  for (int a = 0; a < p * p; ++a) meat[a] = (1.0 / (double)n) * M[a];
+}
+
+static int cmp_double(const void *a, const void *b) {
+  double x = *(const double *)a, y = *(const double *)b;
+  return (x > y) - (x < y);
+}
+
+/* src/overdispersion.cpp:32-46 (keys returned ascending) */
+int64_t orc_make_table(const double *y, int64_t n, int64_t stop_if_larger,
+                       double *keys, double *freqs) {
+  double *s = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+  for (int64_t i = 0; i < n; ++i) s[i] = (double)(long)y[i];
+  qsort(s, (size_t)n, sizeof(double), cmp_double);
+  int64_t nu = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    if (nu == 0 || s[i] != keys[nu - 1]) {
+      if (nu + 1 > stop_if_larger) {
+        free(s);
+        return -1;
+      }
+      keys[nu] = s[i];
+      freqs[nu] = 0.0;
+      nu++;
+    }
+    freqs[nu - 1] += 1.0;
+  }
+  free(s);
+  return nu;
+}
+
+/* X' diag(w) X for w given per cell (p x p, full). */
+static void gram(const double *X, const double *w, int64_t n, int p, double *B) {
+  memset(B, 0, sizeof(double) * p * p);
+  for (int64_t i = 0; i < n; ++i)
+    for (int a = 0; a < p; ++a) {
+      double xa = X[i + a * n] * w[i];
+      for (int b = a; b < p; ++b) B[a + b * p] += xa * X[i + b * n];
+    }
+  for (int a = 0; a < p; ++a)
+    for (int b = a + 1; b < p; ++b) B[b + a * p] = B[a + b * p];
+}
+
+static double trace_prod(const double *A, const double *B, int p) {
+  double s = 0.0;
+  for (int a = 0; a < p; ++a)
+    for (int b = 0; b < p; ++b) s += A[a + b * p] * B[b + a * p];
+  return s;
+}
+
+static const double cr_correction_factor = 0.99; /* overdispersion.cpp:25 */
+
+/* src/overdispersion.cpp:53-88 */
+double orc_nb_loglik(const double *y, const double *mu, int64_t n,
+                     double log_theta, const double *X, int p, int cr,
+                     const double *uniq, const double *freq, int64_t nu) {
+  double theta = exp(log_theta);
+  double cr_term = 0.0;
+  if (cr) {
+    double *w = (double *)malloc(sizeof(double) * (size_t)n);
+    for (int64_t i = 0; i < n; ++i) w[i] = 1.0 / (1.0 / mu[i] + theta);
+    double b[PMAX * PMAX];
+    int perm[PMAX];
+    gram(X, w, n, p, b);
+    free(w);
+    lu_pp(b, p, perm);
+    double ld = 0.0; /* log diag(L) = 0 for unit-diagonal L */
+    for (int a = 0; a < p; ++a) {
+      double e = b[a + a * p];
+      ld += e < 1e-50 ? log(1e-50) : log(e);
+    }
+    cr_term = -0.5 * ld * cr_correction_factor;
+  }
+  double tn1 = 1.0 / theta;
+  double lgamma_term = 0.0;
+  if (nu > 0) {
+    for (int64_t u = 0; u < nu; ++u) lgamma_term += freq[u] * lgam(uniq[u] + tn1);
+  } else {
+    for (int64_t i = 0; i < n; ++i) lgamma_term += lgam(y[i] + tn1);
+  }
+  lgamma_term -= (double)n * lgam(tn1);
+  double ll_part = 0.0;
+  for (int64_t i = 0; i < n; ++i) ll_part += (-y[i] - tn1) * log(mu[i] + tn1);
+  ll_part -= (double)n * tn1 * log(theta);
+  return lgamma_term + ll_part + cr_term;
+}
+
+/* src/overdispersion.cpp:95-178 */
+double orc_nb_score(const double *y, const double *mu, int64_t n,
+                    double log_theta, const double *X, int p, int cr,
+                    const double *uniq, const double *freq, int64_t nu) {
+  double theta = exp(log_theta);
+  double tn1 = 1.0 / theta;
+  double cr_term = 0.0;
+  if (cr) {
+    double *w = (double *)malloc(sizeof(double) * (size_t)n * 2);
+    double *dw = w + n;
+    for (int64_t i = 0; i < n; ++i) {
+      w[i] = 1.0 / (1.0 / mu[i] + theta);
+      dw[i] = -1.0 * w[i] * w[i];
+    }
+    double b[PMAX * PMAX], db[PMAX * PMAX], binv[PMAX * PMAX];
+    gram(X, w, n, p, b);
+    gram(X, dw, n, p, db);
+    free(w);
+    for (int a = 0; a < p; ++a) b[a + a * p] += 1e-6;
+    inv_sympd(b, p, binv);
+    cr_term = -0.5 * trace_prod(binv, db, p) * cr_correction_factor;
+  }
+  double digamma_term = 0.0, max_y = 0.0, sum_y = 0.0, sum_prod_y = 0.0;
+  if (nu > 0) {
+    for (int64_t u = 0; u < nu; ++u) {
+      digamma_term += freq[u] * orc_digamma(uniq[u] + tn1);
+      sum_y += freq[u] * uniq[u];
+      sum_prod_y += freq[u] * (uniq[u] - 1) * uniq[u];
+      max_y = fmax(max_y, uniq[u]);
+    }
+  } else {
+    for (int64_t i = 0; i < n; ++i) {
+      digamma_term += orc_digamma(y[i] + tn1);
+      sum_y += y[i];
+      sum_prod_y += (y[i] - 1) * y[i];
+      max_y = fmax(max_y, y[i]);
+    }
+  }
+  double corr = tn1 > 1e5 ? sum_prod_y / (2 * tn1) : 0.0;
+  if (max_y * 1e6 < tn1) {
+    digamma_term = sum_y - corr;
+  } else {
+    digamma_term -= (double)n * orc_digamma(tn1);
+    digamma_term *= tn1;
+    digamma_term = fmin(digamma_term, sum_y - corr);
+  }
+  double ll_part = 0.0;
+  for (int64_t i = 0; i < n; ++i) {
+    double mt = mu[i] * theta;
+    if (mt < 1e-10) {
+      ll_part += mt * mt * (1 / (1 + mt) - 0.5);
+    } else if (mt < 1e-4) {
+      double inv = 1 / (1 + mt);
+      double ub = mt * mt * inv;
+      double lb = mt * mt * (inv - 0.5);
+      double sug = (log(1 + mt) - mu[i] / (mu[i] + tn1));
+      ll_part += fmax(fmin(sug, ub), lb);
+    } else {
+      ll_part += log(1 + mt) - mu[i] / (mu[i] + tn1);
+    }
+    ll_part += y[i] / (mu[i] + tn1);
+  }
+  ll_part *= tn1;
+  return ll_part - digamma_term + cr_term * theta;
+}
+
+/* src/overdispersion.cpp:186-242 */
+double orc_nb_dscore(const double *y, const double *mu, int64_t n,
+                     double log_theta, const double *X, int p, int cr,
+                     const double *uniq, const double *freq, int64_t nu) {
+  double theta = exp(log_theta);
+  double cr_term = 0.0, cr_term2 = 0.0;
+  if (cr) {
+    double *w = (double *)malloc(sizeof(double) * (size_t)n * 3);
+    double *dw = w + n, *d2w = w + 2 * n;
+    for (int64_t i = 0; i < n; ++i) {
+      w[i] = 1 / (1 / mu[i] + theta);
+      dw[i] = -1 * w[i] * w[i];
+      d2w[i] = -2 * dw[i] * w[i];
+    }
+    double b[PMAX * PMAX], db[PMAX * PMAX], d2b[PMAX * PMAX], binv[PMAX * PMAX];
+    double didb[PMAX * PMAX], sq[PMAX * PMAX];
+    gram(X, w, n, p, b);
+    gram(X, dw, n, p, db);
+    gram(X, d2w, n, p, d2b);
+    free(w);
+    for (int a = 0; a < p; ++a) b[a + a * p] += 1e-6;
+    inv_sympd(b, p, binv);
+    matmul(binv, db, p, didb);
+    double ddetb = 0.0;
+    for (int a = 0; a < p; ++a) ddetb += didb[a + a * p];
+    matmul(didb, didb, p, sq);
+    double tr_sq = 0.0;
+    for (int a = 0; a < p; ++a) tr_sq += sq[a + a * p];
+    double d2detb = (ddetb * ddetb - tr_sq + trace_prod(binv, d2b, p));
+    cr_term = (0.5 * ddetb * ddetb - 0.5 * d2detb) * cr_correction_factor;
+    cr_term2 = -0.5 * ddetb * cr_correction_factor;
+  }
+  double tn1 = 1.0 / theta;
+  double tn2 = 1.0 / (theta * theta);
+  double digamma_term = 0.0, trigamma_term = 0.0;
+  if (nu > 0) {
+    for (int64_t u = 0; u < nu; ++u) {
+      digamma_term += freq[u] * orc_digamma(uniq[u] + tn1);
+      trigamma_term += freq[u] * orc_trigamma(uniq[u] + tn1);
+    }
+    trigamma_term *= tn2;
+    digamma_term -= (double)n * orc_digamma(tn1);
+    trigamma_term -= tn2 * (double)n * orc_trigamma(tn1);
+  } else {
+    for (int64_t i = 0; i < n; ++i) digamma_term += orc_digamma(y[i] + tn1);
+    digamma_term -= (double)n * orc_digamma(tn1);
+    double s = 0.0;
+    for (int64_t i = 0; i < n; ++i) s += orc_trigamma(y[i] + tn1);
+    trigamma_term = tn2 * s;
+    trigamma_term -= tn2 * (double)n * orc_trigamma(tn1);
+  }
+  double l1 = 0.0, l2 = 0.0;
+  for (int64_t i = 0; i < n; ++i) {
+    l1 += log(1 + mu[i] * theta) + (y[i] - mu[i]) / (mu[i] + tn1);
+    l2 += (mu[i] * mu[i] * theta + y[i]) / (1 + mu[i] * theta) / (1 + mu[i] * theta);
+  }
+  double ll_part = -2 * tn1 * (l1 - digamma_term) + (l2 + trigamma_term);
+  return ll_part + cr_term * (theta * theta) + (l1 - digamma_term) * tn1 +
+         cr_term2 * theta;
+}
+
+/* R/overdispersion.R:25-115.  Special cases and start value follow the
+ * reference; the optimiser is the safeguarded Newton iteration on
+ * score(log theta) = 0 that the HIP library runs:
+ *   bracket [lo, hi] = [log 1e-8, log 1e16], x0 = clamp(log start, lo, hi)
+ *   each pass: s = score(x), d = dscore(x); shrink the bracket by sign(s);
+ *   x' = x - s/d if d < 0 else x +- 1 (towards sign(s)); bisect if x'
+ *   leaves the open bracket; stop when |x' - x| < 1e-9 or s == 0. */
+double orc_fit_dispersion(const double *beta, const double *X, const double *y,
+                          const double *off, int64_t n, int p, double tol,
+                          int max_iter, int cr, int *iters) {
+  (void)tol;
+  *iters = 0;
+  double *keys = (double *)malloc(sizeof(double) * (size_t)(n + 1) * 2);
+  double *freqs = keys + n + 1;
+  int64_t nu = orc_make_table(y, n, (int64_t)((double)n / 2.0), keys, freqs);
+  if (nu < 0) nu = 0;
+  int allzero = 1;
+  for (int64_t i = 0; i < n; ++i)
+    if (y[i] != 0) { allzero = 0; break; }
+  if (allzero) {
+    free(keys);
+    return 0.0;
+  }
+  double *mu = (double *)malloc(sizeof(double) * (size_t)n);
+  double ybar = 0.0;
+  for (int64_t i = 0; i < n; ++i) {
+    double eta = 0.0;
+    for (int j = 0; j < p; ++j) eta += X[i + j * n] * beta[j];
+    mu[i] = exp(eta + off[i]);
+    if (mu[i] == 0.0) mu[i] = 1e-6;
+    ybar += y[i];
+  }
+  ybar /= (double)n;
+  double v = 0.0;
+  for (int64_t i = 0; i < n; ++i) v += (y[i] - ybar) * (y[i] - ybar);
+  v /= (double)(n - 1);
+  double start = (v - ybar) / (ybar * ybar);
+  if (isnan(start) || start <= 0) start = 0.5;
+  const double *uq = nu > 0 ? keys : NULL, *fq = nu > 0 ? freqs : NULL;
+  double far_left = orc_nb_score(y, mu, n, log(1e-8), X, p, cr, uq, fq, nu);
+  *iters = 1;
+  if (far_left < 0) {
+    free(mu);
+    free(keys);
+    return 0.0;
+  }
+  double lo = log(1e-8), hi = log(1e16);
+  double x = log(start);
+  if (x < lo) x = lo;
+  if (x > hi) x = hi;
+  for (int it = 0; it < max_iter; ++it) {
+    double s = orc_nb_score(y, mu, n, x, X, p, cr, uq, fq, nu);
+    double d = orc_nb_dscore(y, mu, n, x, X, p, cr, uq, fq, nu);
+    (*iters)++;
+    if (!isfinite(s) || !isfinite(d)) break;
+    if (s == 0.0) break;
+    if (s > 0) lo = fmax(lo, x);
+    else hi = fmin(hi, x);
+    double xn = (d < 0) ? x - s / d : (s > 0 ? x + 1.0 : x - 1.0);
+    if (!(xn > lo && xn < hi)) xn = 0.5 * (lo + hi);
+    double step = xn - x;
+    x = xn;
+    if (fabs(step) < 1e-9) break;
+  }
+  free(mu);
+  free(keys);
+  return exp(x);
+}
+
+/* ===================================================================== */
+/* matrix-level restatements                                              */
+/* ===================================================================== */
+
+/* R/overdispersion.R:129-135: rowVars (n-1 denominator, two-pass) and
+ * rowMeans2; NaN or negative -> 0.01. */
+void orc_estimate_dispersion(const double *Y, int64_t G, int64_t n, int64_t ld,
+                             const double *off, double *alpha0) {
+  double se = 0.0;
+  for (int64_t i = 0; i < n; ++i) se += exp(off[i]);
+  double moi = 1.0 / (se / (double)n);
+#pragma omp parallel for schedule(static)
+  for (int64_t g = 0; g < G; ++g) {
+    const double *y = Y + g * ld;
+    double m = 0.0;
+    for (int64_t i = 0; i < n; ++i) m += y[i];
+    m /= (double)n;
+    double v = 0.0;
+    for (int64_t i = 0; i < n; ++i) v += (y[i] - m) * (y[i] - m);
+    v /= (double)(n - 1);
+    double d = (v - moi * m) / (m * m);
+    alpha0[g] = (isnan(d) || d < 0) ? 0.01 : d;
+  }
+}
+
+/* Householder QR of X (n x p): R (p x p upper) and the thin Q (n x p). */
+static int householder_qr(const double *X, int64_t n, int p, double *Q, double *R) {
+  double *A = (double *)malloc(sizeof(double) * (size_t)n * p);
+  double *V = (double *)malloc(sizeof(double) * (size_t)n * p);
+  double tau[PMAX];
+  memcpy(A, X, sizeof(double) * (size_t)n * p);
+  int rankdef = 0;
+  for (int k = 0; k < p; ++k) {
+    double nrm = 0.0;
+    for (int64_t i = k; i < n; ++i) nrm += A[i + k * n] * A[i + k * n];
+    nrm = sqrt(nrm);
+    if (nrm == 0.0) {
+      rankdef = 1;
+      tau[k] = 0.0;
+      for (int64_t i = 0; i < n; ++i) V[i + k * n] = 0.0;
+      continue;
+    }
+    double alpha = A[k + k * n] > 0 ? -nrm : nrm;
+    for (int64_t i = 0; i < n; ++i) V[i + k * n] = (i < k) ? 0.0 : A[i + k * n];
+    V[k + k * n] -= alpha;
+    double vn = 0.0;
+    for (int64_t i = k; i < n; ++i) vn += V[i + k * n] * V[i + k * n];
+    tau[k] = vn > 0 ? 2.0 / vn : 0.0;
+    for (int j = k; j < p; ++j) {
+      double s = 0.0;
+      for (int64_t i = k; i < n; ++i) s += V[i + k * n] * A[i + j * n];
+      s *= tau[k];
+      for (int64_t i = k; i < n; ++i) A[i + j * n] -= s * V[i + k * n];
+    }
+  }
+  for (int a = 0; a < p; ++a)
+    for (int b = 0; b < p; ++b) R[a + b * p] = (a <= b) ? A[a + b * n] : 0.0;
+  /* Q = H_0 H_1 ... H_{p-1} [I_p; 0] */
+  for (int j = 0; j < p; ++j)
+    for (int64_t i = 0; i < n; ++i) Q[i + j * n] = (i == j) ? 1.0 : 0.0;
+  for (int k = p - 1; k >= 0; --k) {
+    if (tau[k] == 0.0) continue;
+    for (int j = 0; j < p; ++j) {
+      double s = 0.0;
+      for (int64_t i = k; i < n; ++i) s += V[i + k * n] * Q[i + j * n];
+      s *= tau[k];
+      for (int64_t i = k; i < n; ++i) Q[i + j * n] -= s * V[i + k * n];
+    }
+  }
+  for (int k = 0; k < p; ++k) {
+    double dmax = 0.0;
+    for (int a = 0; a < p; ++a) dmax = fmax(dmax, fabs(R[a + a * p]));
+    if (fabs(R[k + k * p]) <= 1e-7 * dmax) rankdef = 1;
+  }
+  free(A);
+  free(V);
+  return rankdef;
+}
+
+/* R/beta.R:13-20: beta0 = R^-1 Q' log1p(y / exp(off)) */
+int orc_init_beta_qr(const double *Y, int64_t G, int64_t n, int64_t ld,
+                     const double *X, int p, const double *off, double *beta) {
+  double *Q = (double *)malloc(sizeof(double) * (size_t)n * p);
+  double R[PMAX * PMAX];
+  int rd = householder_qr(X, n, p, Q, R);
+  if (rd) {
+    free(Q);
+    return 1;
+  }
+  double *eo = (double *)malloc(sizeof(double) * (size_t)n);
+  for (int64_t i = 0; i < n; ++i) eo[i] = exp(off[i]);
+#pragma omp parallel for schedule(static)
+  for (int64_t g = 0; g < G; ++g) {
+    const double *y = Y + g * ld;
+    double w[PMAX];
+    for (int j = 0; j < p; ++j) w[j] = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+      double z = log1p(y[i] / eo[i]);
+      for (int j = 0; j < p; ++j) w[j] += Q[i + j * n] * z;
+    }
+    for (int a = p - 1; a >= 0; --a) {
+      double s = w[a];
+      for (int b = a + 1; b < p; ++b) s -= R[a + b * p] * w[b];
+      w[a] = s / R[a + a * p];
+    }
+    for (int j = 0; j < p; ++j) beta[g + j * G] = w[j];
+  }
+  free(eo);
+  free(Q);
+  return 0;
+}
+
+/* R/main.R:404-406 */
+void orc_init_beta_rough(const double *Y, int64_t G, int64_t n, int64_t ld,
+                         int p, double *beta) {
+  for (int64_t g = 0; g < G; ++g) {
+    const double *y = Y + g * ld;
+    double m = 0.0;
+    for (int64_t i = 0; i < n; ++i) m += y[i];
+    m /= (double)n;
+    beta[g] = log1p(m);
+    for (int j = 1; j < p; ++j) beta[g + j * G] = 0.0;
+  }
+}
+
+/* src/beta_init_univariate.cpp:13-95 */
+void orc_init_beta_univariate(const double *Y, int64_t G, int64_t n, int64_t ld,
+                              const double *X, int p, const double *sf,
+                              double *beta) {
+  int intercept_first = 0;
+  if (p > 0) {
+    double mx = 0.0;
+    for (int64_t i = 0; i < n; ++i) mx = fmax(mx, fabs(X[i] - 1.0));
+    intercept_first = mx < 1e-10;
+  }
+  double sum_sf = 0.0;
+  for (int64_t i = 0; i < n; ++i) sum_sf += sf[i];
+  double xmean[PMAX], denom[PMAX];
+  for (int j = 0; j < p; ++j) {
+    double s = 0.0;
+    for (int64_t i = 0; i < n; ++i) s += X[i + j * n];
+    xmean[j] = s / (double)n;
+    double d = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+      double c = X[i + j * n] - xmean[j];
+      d += c * c;
+    }
+    denom[j] = d <= 0.0 ? 1e-12 : d;
+  }
+  for (int64_t g = 0; g < G; ++g) {
+    const double *y = Y + g * ld;
+    for (int j = 0; j < p; ++j) beta[g + j * G] = 0.0;
+    if (intercept_first) {
+      double s = 0.0;
+      for (int64_t i = 0; i < n; ++i) s += y[i];
+      beta[g] = log((s + 1e-8) / (sum_sf + 1e-8));
+    }
+    for (int j = 0; j < p; ++j) {
+      if (j == 0 && intercept_first) continue;
+      double num = 0.0;
+      for (int64_t i = 0; i < n; ++i)
+        num += (X[i + j * n] - xmean[j]) * (log(y[i] + 0.5) - log(sf[i]));
+      beta[g + j * G] = num / denom[j];
+    }
+  }
+}
+
+/* ===================================================================== */
+/* R/main.R:375-523 cpu_fit                                               */
+/* ===================================================================== */
+int orc_cpu_fit(const orc_fit_args *a, orc_fit_out *o) {
+  const int64_t n = a->n;
+  const int p = a->p;
+  const int64_t Gs = a->genes ? a->n_genes_subset : a->G;
+  double *eo = (double *)malloc(sizeof(double) * (size_t)n);
+  for (int64_t i = 0; i < n; ++i) eo[i] = exp(a->offset[i]);   /* :390 */
+
+  /* dispersion_init (:396-400) and beta_0 (:404-409) for the genes used */
+  double *alpha0 = (double *)malloc(sizeof(double) * (size_t)Gs);
+  double *beta0 = (double *)malloc(sizeof(double) * (size_t)Gs * p);
+  double *Ysub = NULL;
+  const double *Yuse = a->Y;
+  int64_t lduse = a->ld;
+  if (a->genes) {
+    /* gather the subset rows so the matrix-level helpers see a G_s x n matrix */
+    Ysub = (double *)malloc(sizeof(double) * (size_t)Gs * n);
+    for (int64_t s = 0; s < Gs; ++s)
+      memcpy(Ysub + s * n, a->Y + a->genes[s] * a->ld, sizeof(double) * (size_t)n);
+    Yuse = Ysub;
+    lduse = n;
+  }
+#ifdef _OPENMP
+  int saved_threads = omp_get_max_threads();
+  if (a->n_threads > 0) omp_set_num_threads(a->n_threads);
+#endif
+  int rc = 0;
+  if (a->disp_init) {
+    for (int64_t s = 0; s < Gs; ++s)
+      alpha0[s] = a->disp_init[a->genes ? a->genes[s] : s];
+  } else {
+    orc_estimate_dispersion(Yuse, Gs, n, lduse, a->offset, alpha0);
+  }
+  if (a->init_mode == ORC_INIT_ROUGH) {
+    orc_init_beta_rough(Yuse, Gs, n, lduse, p, beta0);
+  } else if (a->init_mode == ORC_INIT_UNIVARIATE) {
+    orc_init_beta_univariate(Yuse, Gs, n, lduse, a->X, p, eo, beta0);
+  } else {
+    rc = orc_init_beta_qr(Yuse, Gs, n, lduse, a->X, p, a->offset, beta0);
+  }
+  if (rc == 0) {
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t s = 0; s < Gs; ++s) {
+      const double *y = Yuse + s * lduse;
+      double beta[PMAX];
+      for (int j = 0; j < p; ++j) beta[j] = beta0[s + j * Gs];
+      int it = orc_beta_fit(y, n, a->X, p, beta, a->offset, (float)alpha0[s],
+                            a->max_iter, (float)a->tolerance);
+      double theta = alpha0[s];
+      int th_it = 0;
+      if (a->od_mode == ORC_OD_MLE) {
+        theta = orc_fit_dispersion(beta, a->X, y, a->offset, n, p, a->tolerance,
+                                   a->max_iter, 1, &th_it);
+      } else if (a->od_mode == ORC_OD_MOM) {
+        theta = orc_mom_dispersion(y, n, a->X, p, beta, eo);
+      } else {
+        theta = 0.0;
+      }
+      double bread[PMAX * PMAX];
+      orc_compute_hessian(beta, theta, y, a->X, n, p, eo, bread);
+      for (int j = 0; j < p; ++j) o->beta[s + j * Gs] = beta[j];
+      o->theta[s] = theta;
+      o->beta_iters[s] = it;
+      o->theta_iters[s] = th_it;
+      memcpy(o->bread + s * p * p, bread, sizeof(double) * p * p);
+      if (o->sandwich) {
+        if (a->cluster_ends && a->n_clusters > 0) {
+          double meat[PMAX * PMAX], t1[PMAX * PMAX], t2[PMAX * PMAX];
+          orc_clustered_meat(a->X, y, beta, theta, eo, n, p, a->cluster_ends,
+                             a->n_clusters, meat);
+          matmul(bread, meat, p, t1);
+          matmul(t1, bread, p, t2);
+          for (int q = 0; q < p * p; ++q) o->sandwich[s * p * p + q] = t2[q] * (double)n;
+        } else {
+          for (int q = 0; q < p * p; ++q) o->sandwich[s * p * p + q] = NAN;
+        }
+      }
+      if (o->alpha0) o->alpha0[s] = alpha0[s];
+      if (o->beta0)
+        for (int j = 0; j < p; ++j) o->beta0[s + j * Gs] = beta0[s + j * Gs];
+    }
+  }
+#ifdef _OPENMP
+  omp_set_num_threads(saved_threads);
+#endif
+  free(Ysub);
+  free(alpha0);
+  free(beta0);
+  free(eo);
+  return rc;
+}
+
+/* ===================================================================== */
+/* R/test.R:85-134                                                        */
+/* ===================================================================== */
+typedef struct {
+  double p;
+  int64_t i;
+} pidx;
+
+static int cmp_desc(const void *a, const void *b) {
+  const pidx *x = (const pidx *)a, *y = (const pidx *)b;
+  if (x->p > y->p) return -1;
+  if (x->p < y->p) return 1;
+  return (x->i > y->i) - (x->i < y->i);
+}
+
+/* stats::p.adjust(method = "BH"): NaN entries are removed first, so n is the
+ * number of non-NaN p-values; o = order(p, decreasing = TRUE);
+ * adj[o] = pmin(1, cummin(n / (n:1) * p[o])). */
+void orc_p_adjust_bh(const double *p, int64_t G, double *adj) {
+  pidx *v = (pidx *)malloc(sizeof(pidx) * (size_t)(G > 0 ? G : 1));
+  int64_t m = 0;
+  for (int64_t g = 0; g < G; ++g) {
+    if (isnan(p[g])) adj[g] = NAN;
+    else v[m++] = (pidx){p[g], g};
+  }
+  qsort(v, (size_t)m, sizeof(pidx), cmp_desc);
+  double run = INFINITY;
+  for (int64_t r = 0; r < m; ++r) {
+    double i = (double)(m - r);
+    double val = (double)m / i * v[r].p;
+    if (val < run) run = val;
+    adj[v[r].i] = run < 1.0 ? run : 1.0;
+  }
+  free(v);
+}
+
+void orc_test_de(const double *beta, const double *bread, const double *sandwich,
+                 int64_t G, int p, int64_t n, const double *contrast,
+                 double max_lfc, double *pval, double *log_pval,
+                 double *adj_pval, double *lfc) {
+  const double df = (double)n - 2.0;
+  for (int64_t g = 0; g < G; ++g) {
+    double l = 0.0;
+    for (int j = 0; j < p; ++j) l += beta[g + j * G] * contrast[j];
+    const double *H = bread + g * p * p;
+    double v0 = 0.0;
+    for (int a = 0; a < p; ++a)
+      for (int b = 0; b < p; ++b) v0 += contrast[a] * H[a + b * p] * contrast[b];
+    double lp0;
+    double p0 = orc_pt2_upper(fabs(l) / sqrt(v0), df, &lp0);
+    double pv = p0, lpv = lp0;
+    if (sandwich) {
+      const double *S = sandwich + g * p * p;
+      double v1 = 0.0;
+      for (int a = 0; a < p; ++a)
+        for (int b = 0; b < p; ++b) v1 += contrast[a] * S[a + b * p] * contrast[b];
+      double lp1;
+      double p1 = orc_pt2_upper(fabs(l) / sqrt(v1), df, &lp1);
+      if (isnan(p0) || isnan(p1)) {
+        pv = NAN;
+        lpv = NAN;
+      } else if (lp1 > lp0) {
+        pv = p1;
+        lpv = lp1;
+      }
+    }
+    pval[g] = pv;
+    if (log_pval) log_pval[g] = lpv;
+    double lf = l / log(2.0);
+    if (lf >= max_lfc) lf = max_lfc;
+    if (lf <= -max_lfc) lf = -max_lfc;
+    lfc[g] = lf;
+  }
+  orc_p_adjust_bh(pval, G, adj_pval);
+}

@@ -1,0 +1,167 @@
+/*
+ * devil_oracle.h -- FP64 CPU restatement of devil's per-gene NB-GLM fit.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the parity oracle: tests/, the smoke
+ * check and the benchmark's cpu_baseline leg call it to check (or time
+ * beside) the HIP library.  The product (devil_amd, libdevil_hip.so) never
+ * links, loads or calls anything here.
+ *
+ * Every function restates one reference function; the citation is given
+ * next to each declaration (paths relative to the devil 0.99.0 tree).
+ *
+ * Conventions (shared with the C ABI in include/devil_hip.h):
+ *   counts  gene-major: y of gene g is Y[g*ld + 0 .. g*ld + n-1]
+ *   X       n x p, column-major (R layout): X[i + j*n]
+ *   beta    G x p, column-major (R layout): beta[g + j*G]
+ *   p x p matrices column-major, one per gene, contiguous: M[g*p*p + a + b*p]
+ *
+ * Parity status of each part is stated in the header comment of
+ * devil_oracle.c.
+ */
+#ifndef DEVIL_ORACLE_H
+#define DEVIL_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------- special functions ---------------- */
+double orc_digamma(double x);                 /* R nmath Rf_digamma, x > 0 */
+double orc_trigamma(double x);                /* R nmath Rf_trigamma, x > 0 */
+double orc_lbeta(double a, double b);         /* log B(a,b), a,b > 0 */
+/* log I_x(a,b), regularized incomplete beta, given x and 1-x separately */
+double orc_log_ibeta(double a, double b, double x, double one_minus_x);
+/* 2*pt(|t|, df, lower.tail=FALSE) (R/test.R:108-116); *logp receives its log */
+double orc_pt2_upper(double t, double df, double *logp);
+
+/* ---------------- small dense linear algebra ---------------- */
+/* Eigen MatrixXd::inverse() (PartialPivLU) as used at src/beta.cpp:40 and
+ * src/variance.cpp:46.  A is p x p column-major, overwritten by nothing;
+ * Ainv receives the inverse.  Returns 0, or 1 if a pivot is exactly zero
+ * (Ainv then holds inf/nan as Eigen would). */
+int orc_inverse(const double *A, int p, double *Ainv);
+
+/* ---------------- per-gene restatements ---------------- */
+/* src/beta.cpp:25-51 beta_fit.  kf and epsf are the float arguments of the
+ * reference signature; beta is updated in place; returns the iteration
+ * count. */
+int orc_beta_fit(const double *y, int64_t n, const double *X, int p,
+                 double *beta, const double *off, float kf, int max_iter,
+                 float epsf);
+
+/* src/mom_overdispersion.cpp:6-56 for one gene (count_matrix 1 x n). */
+double orc_mom_dispersion(const double *y, int64_t n, const double *X, int p,
+                          const double *beta, const double *sf);
+
+/* src/variance.cpp:22-47 compute_hessian -> bread (p x p). Returns 0, or 1
+ * if the p x p inverse met an exactly zero pivot. */
+int orc_compute_hessian(const double *beta, double alpha, const double *y,
+                        const double *X, int64_t n, int p, const double *sf,
+                        double *bread);
+
+/* src/variance.cpp:186-228 compute_clustered_meat_fast.  ends = cumulative
+ * 1-based block ends (R/main.R:193), C blocks. */
+void orc_clustered_meat(const double *X, const double *y, const double *beta,
+                        double alpha, const double *sf, int64_t n, int p,
+                        const int32_t *ends, int C, double *meat);
+
+/* src/variance.cpp:64-85 + 157-178 compute_meat (HC0) with the
+ * compute_hessian dispersion convention (score = x (y-mu)/(1+mu*alpha)). */
+void orc_meat_hc0(const double *X, const double *y, const double *beta,
+                  double alpha, const double *sf, int64_t n, int p,
+                  double *meat);
+
+/* src/overdispersion.cpp:32-46 make_table_if_small.  Fills keys/freqs with
+ * the distinct values of y (ascending) and returns their number, or -1 if
+ * more than stop_if_larger distinct values exist. */
+int64_t orc_make_table(const double *y, int64_t n, int64_t stop_if_larger,
+                       double *keys, double *freqs);
+
+/* src/overdispersion.cpp:53-88, 95-178, 186-242.  uniq/freq may be NULL
+ * (nu = 0) for the per-cell branch. */
+double orc_nb_loglik(const double *y, const double *mu, int64_t n,
+                     double log_theta, const double *X, int p, int cr,
+                     const double *uniq, const double *freq, int64_t nu);
+double orc_nb_score(const double *y, const double *mu, int64_t n,
+                    double log_theta, const double *X, int p, int cr,
+                    const double *uniq, const double *freq, int64_t nu);
+double orc_nb_dscore(const double *y, const double *mu, int64_t n,
+                     double log_theta, const double *X, int p, int cr,
+                     const double *uniq, const double *freq, int64_t nu);
+
+/* R/overdispersion.R:25-115 fit_dispersion: the special cases and start
+ * value of the reference, then the safeguarded Newton root of the score in
+ * log(theta) that the HIP library also runs (the nlminb trajectory itself is
+ * not restated).  *iters receives the number of score evaluations (passes
+ * over the counts); 0 for the early returns. */
+double orc_fit_dispersion(const double *beta, const double *X, const double *y,
+                          const double *off, int64_t n, int p, double tol,
+                          int max_iter, int cr, int *iters);
+
+/* ---------------- matrix-level restatements ---------------- */
+/* R/overdispersion.R:129-135 estimate_dispersion */
+void orc_estimate_dispersion(const double *Y, int64_t G, int64_t n, int64_t ld,
+                             const double *off, double *alpha0);
+/* R/beta.R:13-20 init_beta (QR least squares of log1p(y/exp(off)) on X).
+ * Returns 0, or 1 if X is rank deficient. */
+int orc_init_beta_qr(const double *Y, int64_t G, int64_t n, int64_t ld,
+                     const double *X, int p, const double *off, double *beta);
+/* R/main.R:404-406 rough init: (log1p(rowMeans), 0, ...) */
+void orc_init_beta_rough(const double *Y, int64_t G, int64_t n, int64_t ld,
+                         int p, double *beta);
+/* src/beta_init_univariate.cpp:13-95 */
+void orc_init_beta_univariate(const double *Y, int64_t G, int64_t n, int64_t ld,
+                              const double *X, int p, const double *sf,
+                              double *beta);
+
+/* ---------------- pipeline: R/main.R:375-523 cpu_fit ---------------- */
+enum { ORC_OD_NONE = 0, ORC_OD_MOM = 1, ORC_OD_MLE = 2 };
+enum { ORC_INIT_QR = 0, ORC_INIT_ROUGH = 1, ORC_INIT_UNIVARIATE = 2 };
+
+typedef struct {
+  int64_t G, n, ld;
+  int32_t p;
+  const double *Y;            /* G x n gene-major counts as doubles */
+  const double *X;            /* n x p column-major */
+  const double *offset;       /* n */
+  const int32_t *cluster_ends;/* cumulative 1-based ends, or NULL */
+  int32_t n_clusters;
+  const double *disp_init;    /* G, or NULL -> estimate_dispersion */
+  int32_t init_mode;          /* ORC_INIT_* */
+  int32_t od_mode;            /* ORC_OD_* */
+  int32_t max_iter;
+  double tolerance;
+  int32_t n_threads;          /* OpenMP threads, <=0: runtime default */
+  const int64_t *genes;       /* optional subset of gene indices (NULL: all) */
+  int64_t n_genes_subset;
+} orc_fit_args;
+
+typedef struct {
+  double *beta;        /* Gs x p column-major (Gs = G or subset size) */
+  double *theta;       /* Gs */
+  int32_t *beta_iters; /* Gs */
+  int32_t *theta_iters;/* Gs */
+  double *bread;       /* Gs x p x p */
+  double *sandwich;    /* Gs x p x p, or NULL */
+  double *alpha0;      /* Gs (optional, may be NULL): dispersion_init */
+  double *beta0;       /* Gs x p (optional, may be NULL): beta_0 */
+} orc_fit_out;
+
+/* Returns 0, or 1 on a rank-deficient design. */
+int orc_cpu_fit(const orc_fit_args *a, orc_fit_out *o);
+
+/* ---------------- R/test.R:85-134 test_de ---------------- */
+/* sandwich may be NULL (no clusters).  contrast has p entries.  Outputs:
+ * pval, log_pval (natural log), adj_pval (BH), lfc (log2, clipped). */
+void orc_test_de(const double *beta, const double *bread, const double *sandwich,
+                 int64_t G, int p, int64_t n, const double *contrast,
+                 double max_lfc, double *pval, double *log_pval,
+                 double *adj_pval, double *lfc);
+/* stats::p.adjust(p, "BH") with NA (NaN) entries left NaN */
+void orc_p_adjust_bh(const double *p, int64_t G, double *adj);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
