@@ -1,0 +1,168 @@
+"""Benchmark: gene x cell fits per second of the per-gene NB-GLM fit on one MI355X.
+
+One step = the whole fitting path over a synthetic count matrix already
+resident in HBM: init pass (alpha0, beta0), IRLS, overdispersion, bread (+
+clustered meat when the workload has patients) and the Wald test with BH.
+Default workload: G=20000 genes x n=100000 cells int32 counts, p=4 design,
+MLE overdispersion (the plain FP64 path).
+
+Prints one JSON line with the throughput, a roofline entry for the IRLS
+kernel (algorithmic bytes 4*n*sum_g iters_g over its HIP-event duration) and
+a CPU baseline: the FP64 oracle timed on a gene subsample on the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "gene×cell fits/s: mixed-precision vs FP64 GPU path at ≤1e-4 β/p rel err"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(w, Yd, fit, n_threads):
+    """The FP64 oracle on every ceil(G/512)-th gene, timed on the host."""
+    import oracle
+    stride = math.ceil(w.G / 512)
+    genes = np.arange(0, w.G, stride)
+    Ys = Yd[genes].cpu().numpy().astype(np.float64)
+    od = {"MOM": oracle.OD_MOM, "MLE": oracle.OD_MLE}[w.od]
+    ends = w.cluster_ends
+    log(f"[bench] cpu baseline: {len(genes)} genes x {w.n} cells, {n_threads} threads")
+    t0 = time.perf_counter()
+    ref = oracle.cpu_fit(Ys, w.X, w.offset, clusters_ends=ends, od_mode=od,
+                         n_threads=n_threads)
+    t = time.perf_counter() - t0
+    beta = fit["beta"].cpu().numpy()[genes]
+    it = fit["iterations"]["beta_iters"].cpu().numpy()[genes]
+    rel = np.abs(beta - ref["beta"]) / np.maximum(np.abs(ref["beta"]), 1e-6)
+    th = fit["overdispersion"].cpu().numpy()[genes]
+    rel_th = np.abs(th - ref["theta"]) / np.maximum(np.abs(ref["theta"]), 1e-8)
+    check = {"genes": int(len(genes)), "beta_max_rel_err": float(np.nanmax(rel)),
+             "theta_max_rel_err": float(np.nanmax(rel_th)),
+             "iter_mismatch": int(np.sum(it != ref["beta_iters"]))}
+    return {"value": len(genes) * w.n / t, "unit": "gene*cell fits/s", "cores": n_threads,
+            "kind": "port",
+            "sample": f"FP64 oracle cpu_fit on every {stride}th gene ({len(genes)} genes) x "
+                      f"all {w.n} cells of the same workload, {t:.1f} s"}, check
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="C2")
+    ap.add_argument("--genes", type=int, default=None)
+    ap.add_argument("--cells", type=int, default=None)
+    ap.add_argument("--precision", default="fp64")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the fit shards by genes with no communication; see the design notes)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import devil_amd
+    from devil_amd import synth
+
+    dev = torch.device("cuda", 0)
+    w = synth.make_workload(args.config, G=args.genes, n=args.cells)
+    log(f"[bench] generating {args.config}: G={w.G} n={w.n} p={w.p} od={w.od} on the GPU")
+    t0 = time.perf_counter()
+    Yd = synth.counts_device(w, device=dev)
+    torch.cuda.synchronize()
+    zero_frac = float((Yd == 0).float().mean().item())
+    log(f"[bench] counts ready in {time.perf_counter() - t0:.1f} s, zero fraction {zero_frac:.3f}")
+    contrast = np.zeros(w.p)
+    contrast[1] = 1.0
+    ctx = devil_amd.context(0)
+    ctx.set_timing(True)
+
+    def step():
+        fit = devil_amd.fit_devil(Yd, w.X, clusters=w.clusters, overdispersion=w.od,
+                                  size_factors=w.sf, precision=args.precision)
+        de = devil_amd.test_de(fit, contrast)
+        return fit, de
+
+    for i in range(args.warmup):
+        t1 = time.perf_counter()
+        fit, de = step()
+        torch.cuda.synchronize()
+        log(f"[bench] warmup {i}: {time.perf_counter() - t1:.3f} s")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    stage_sum = {k: 0.0 for k in devil_amd._lib.STAGES}
+    for i in range(args.steps):
+        fit, de = step()
+        for k, v in ctx.stage_times().items():
+            stage_sum[k] += v
+    torch.cuda.synchronize()
+    T = time.perf_counter() - t0
+    ms_step = 1e3 * T / args.steps
+    log(f"[bench] {args.steps} steps: {ms_step:.1f} ms/step")
+    stages = {k: v / args.steps for k, v in stage_sum.items()}
+    iters = fit["iterations"]["beta_iters"].cpu().numpy().astype(np.int64)
+    th_it = fit["iterations"]["theta_iters"].cpu().numpy().astype(np.int64)
+    sum_it = int(iters.sum())
+    irls_bytes = 4.0 * w.n * sum_it
+    irls_gbs = irls_bytes / (stages["irls"] * 1e-3) / 1e9
+    # whole-fit algorithmic bytes: init (2 reads: mean, variance) + IRLS +
+    # dispersion passes + bread/meat, 4 bytes per count per pass
+    disp_passes = int(th_it.sum()) if w.od == "MLE" else w.G
+    fit_bytes = 4.0 * w.n * (2 * w.G + sum_it + disp_passes + w.G)
+    fit_ms = sum(stages[k] for k in ("init", "irls", "dispersion", "bread_meat"))
+    value = w.G * w.n * args.steps / T
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "gene*cell fits/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{w.name}: G={w.G} genes x n={w.n} cells int32 NB counts "
+                               f"(zero fraction {zero_frac:.2f}), p={w.p}, {w.od} "
+                               "overdispersion, bread, Wald test + BH",
+                   "G": w.G, "n": w.n, "p": w.p, "precision": args.precision,
+                   "overdispersion": w.od},
+        "roofline": {"bound": "hbm", "kernel": f"k_irls<{w.p}> (IRLS, FP64)",
+                     "achieved": irls_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": irls_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes": irls_bytes, "avg_launch_ms": stages["irls"]},
+        "stages_ms": stages,
+        "fit_alg_GBps": fit_bytes / (fit_ms * 1e-3) / 1e9,
+        "sum_beta_iters": sum_it,
+        "sum_theta_passes": int(th_it.sum()),
+        "max_beta_iters": int(iters.max()),
+    }
+    if not args.no_cpu_baseline:
+        n_threads = max(1, min(16, os.cpu_count() or 1))
+        cb, check = cpu_baseline(w, Yd, fit, n_threads)
+        out["cpu_baseline"] = cb
+        out["cpu_check"] = check
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,150 @@
+"""ctypes binding of libdevil_hip.so (include/devil_hip.h).
+
+There is no CPU fallback: if the library is missing, or no GPU is present,
+calls raise instead of computing anything on the host.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdevil_hip.so")
+
+DEVIL_OK = 0
+MEM_HOST, MEM_DEVICE = 0, 1
+COUNTS_I32, COUNTS_F32, COUNTS_F64 = 0, 1, 2
+INIT_QR, INIT_ROUGH, INIT_UNIVARIATE = 0, 1, 2
+OD_NONE, OD_MOM, OD_MLE = 0, 1, 2
+PREC_FP64, PREC_MIXED = 0, 1
+PMAX = 10
+
+GENE_MAXITER, GENE_NAN, GENE_SINGULAR, GENE_MOM_DEN0, GENE_MLE_ZERO = 1, 2, 4, 8, 16
+
+_vp = C.c_void_p
+
+
+class Problem(C.Structure):
+    _fields_ = [
+        ("G", C.c_int64), ("n", C.c_int64), ("p", C.c_int32), ("mem", C.c_int32),
+        ("counts", _vp), ("counts_dtype", C.c_int32), ("counts_ld", C.c_int64),
+        ("X", _vp), ("offset", _vp), ("cluster_ends", _vp), ("n_clusters", C.c_int32),
+        ("disp_init", _vp), ("beta_init", _vp), ("init_mode", C.c_int32),
+        ("od_mode", C.c_int32), ("max_iter", C.c_int32), ("tolerance", C.c_double),
+        ("precision", C.c_int32),
+    ]
+
+
+class Outputs(C.Structure):
+    _fields_ = [
+        ("beta", _vp), ("theta", _vp), ("beta_iters", _vp), ("theta_iters", _vp),
+        ("bread", _vp), ("sandwich", _vp), ("status", _vp),
+    ]
+
+
+class DevilError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"devil_hip error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Load libdevil_hip.so; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -m devil_amd.build` "
+            "(devil_amd has no CPU fallback)")
+    L = C.CDLL(LIB_PATH)
+    i32, i64, d = C.c_int32, C.c_int64, C.c_double
+    sigs = {
+        "devil_create": (C.c_int, [C.POINTER(_vp), C.c_int]),
+        "devil_destroy": (None, [_vp]),
+        "devil_strerror": (C.c_char_p, [C.c_int]),
+        "devil_last_error": (C.c_char_p, [_vp]),
+        "devil_version": (C.c_char_p, []),
+        "devil_fit": (C.c_int, [_vp, C.POINTER(Problem), C.POINTER(Outputs)]),
+        "devil_estimate_dispersion": (C.c_int, [_vp, _vp, i32, i64, i64, i64, _vp, _vp]),
+        "devil_init_beta": (C.c_int, [_vp, _vp, i32, i64, i64, i64, _vp, i32, _vp, i32, _vp]),
+        "devil_beta_fit": (C.c_int, [_vp, _vp, i32, i64, i64, i64, _vp, i32, _vp, _vp, _vp,
+                                     i32, d, i32, _vp, _vp]),
+        "devil_mom_dispersion": (C.c_int, [_vp, _vp, i32, i64, i64, i64, _vp, i32, _vp, _vp,
+                                           _vp]),
+        "devil_fit_dispersion_mle": (C.c_int, [_vp, _vp, i32, i64, i64, i64, _vp, i32, _vp,
+                                               _vp, i32, _vp, _vp]),
+        "devil_bread_meat": (C.c_int, [_vp, _vp, i32, i64, i64, i64, _vp, i32, _vp, _vp, _vp,
+                                       _vp, i32, _vp, _vp, _vp]),
+        "devil_test_de": (C.c_int, [_vp, _vp, _vp, _vp, i64, i32, i64, _vp, d, _vp, _vp, _vp,
+                                    _vp]),
+        "devil_set_timing": (C.c_int, [_vp, C.c_int]),
+        "devil_stage_times": (C.c_int, [_vp, C.POINTER(C.c_double)]),
+    }
+    for name, (res, args) in sigs.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+EXPORTED = ["devil_create", "devil_destroy", "devil_strerror", "devil_last_error",
+            "devil_version", "devil_fit", "devil_estimate_dispersion", "devil_init_beta",
+            "devil_beta_fit", "devil_mom_dispersion", "devil_fit_dispersion_mle",
+            "devil_bread_meat", "devil_test_de", "devil_set_timing", "devil_stage_times"]
+
+STAGES = ["init", "irls", "dispersion", "bread_meat", "test_de"]
+
+
+class Context:
+    """Owns one devil_ctx (a HIP stream on one device)."""
+
+    def __init__(self, device: int = 0):
+        L = lib()
+        h = _vp()
+        rc = L.devil_create(C.byref(h), int(device))
+        if rc != DEVIL_OK:
+            raise DevilError(rc, L.devil_strerror(rc).decode())
+        self.h = h
+        self.device = device
+
+    def check(self, rc: int):
+        if rc != DEVIL_OK:
+            L = lib()
+            raise DevilError(rc, f"{L.devil_strerror(rc).decode()}: "
+                                 f"{L.devil_last_error(self.h).decode()}")
+
+    def set_timing(self, enable: bool = True):
+        self.check(lib().devil_set_timing(self.h, int(bool(enable))))
+
+    def stage_times(self) -> dict:
+        """Device ms per stage of the last fit / test_de (HIP events)."""
+        arr = (C.c_double * len(STAGES))()
+        self.check(lib().devil_stage_times(self.h, arr))
+        return dict(zip(STAGES, list(arr)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().devil_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_ctx_cache: dict[int, Context] = {}
+
+
+def context(device: int = 0) -> Context:
+    ctx = _ctx_cache.get(device)
+    if ctx is None:
+        ctx = Context(device)
+        _ctx_cache[device] = ctx
+    return ctx

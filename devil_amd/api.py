@@ -1,0 +1,216 @@
+"""Python mirror of devil's user API for the fitting path.
+
+``fit_devil`` and ``test_de`` keep the argument names, meanings and error
+behaviour of the R functions (R/main.R:164-273, R/test.R:85-134), and run the
+fit through libdevil_hip.so on an MI355X.  Counts may be a numpy array (host;
+the library streams them to HBM in gene batches) or a torch tensor already on
+the GPU (nothing is copied).  There is no CPU fitting path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Any
+
+import numpy as np
+
+from . import _lib as L
+
+_OD_MODES = {"MOM": L.OD_MOM, "MLE": L.OD_MLE, "old": L.OD_MLE}
+
+
+def _is_torch(x) -> bool:
+    return type(x).__module__.startswith("torch")
+
+
+def _counts_dtype(dt) -> int:
+    s = str(dt).replace("torch.", "")
+    if s == "int32":
+        return L.COUNTS_I32
+    if s == "float32":
+        return L.COUNTS_F32
+    if s == "float64":
+        return L.COUNTS_F64
+    raise TypeError(f"counts dtype {dt} not supported (int32, float32, float64)")
+
+
+def cluster_blocks(clusters, n: int) -> np.ndarray:
+    """R/main.R:190-196: codes by first appearance, must be sorted, then the
+    cumulative run-length ends (1-based)."""
+    lab = np.asarray(clusters)
+    if lab.shape[0] != n:
+        raise ValueError("clusters must have one entry per cell")
+    _, first = np.unique(lab, return_index=True)
+    order = np.argsort(first)
+    code_of = {lab[first[k]]: i + 1 for i, k in enumerate(order)}
+    codes = np.array([code_of[v] for v in lab], dtype=np.int64)
+    if np.any(np.diff(codes) < 0):
+        raise ValueError("Input data should be grouped in blocks of patients. "
+                         "Use the `group_data` function to group them.")
+    change = np.flatnonzero(np.diff(codes) != 0)
+    ends = np.concatenate([change + 1, [n]]).astype(np.int32)
+    return ends
+
+
+def compute_offset_vector(off, n: int, size_factors) -> np.ndarray:
+    """R/size_factors.R:172-184: rep(off, n) + log(size_factors)."""
+    v = np.full(n, float(off))
+    if size_factors is not None:
+        v = v + np.log(np.asarray(size_factors, dtype=np.float64))
+    return v
+
+
+def fit_devil(x, design_matrix, clusters=None, overdispersion: Any = "MOM",
+              init_overdispersion=None, init_beta_rough: bool = False, offset: float = 0,
+              size_factors=None, verbose: bool = False, max_iter: int = 200,
+              tolerance: float = 1e-3, CUDA: bool = True, batch_size: int = 1024,
+              precision: str = "fp64", device: int = 0, init_mode: str | None = None):
+    """Fit the per-gene NB GLM (R/main.R:164-273 with cpu_fit semantics).
+
+    ``x`` is genes x cells.  ``overdispersion`` is "MOM", "MLE"/"old" or
+    False.  ``batch_size`` is accepted for signature compatibility; the
+    library sizes its own gene batches.  ``precision`` is "fp64".
+    """
+    if not CUDA:
+        raise NotImplementedError("devil_amd only implements the GPU fit (CUDA=True)")
+    ctx = L.context(device)
+    X = np.asfortranarray(np.asarray(design_matrix, dtype=np.float64))
+    if X.ndim != 2:
+        raise ValueError("design_matrix must be a matrix")
+    n, p = X.shape
+    G = int(x.shape[0])
+    if int(x.shape[1]) != n:
+        raise ValueError("ncol(x) must equal nrow(design_matrix)")
+    if overdispersion is False:
+        od = L.OD_NONE
+    elif overdispersion in _OD_MODES:
+        od = _OD_MODES[overdispersion]
+    else:
+        raise ValueError(f"Unknown overdispersion mode: {overdispersion}")
+    if isinstance(size_factors, str):
+        raise NotImplementedError("size-factor estimation is outside the fitted path; "
+                                  "pass numeric size_factors")
+    sf = np.ones(n) if size_factors is None else np.asarray(size_factors, dtype=np.float64)
+    offset_vector = compute_offset_vector(offset, n, sf)
+    ends = None if clusters is None else cluster_blocks(clusters, n)
+    if init_mode is None:
+        init = L.INIT_ROUGH if init_beta_rough else L.INIT_QR
+    else:
+        init = {"qr": L.INIT_QR, "rough": L.INIT_ROUGH, "univariate": L.INIT_UNIVARIATE}[init_mode]
+    prec = {"fp64": L.PREC_FP64, "mixed": L.PREC_MIXED}[precision]
+    disp = None
+    if init_overdispersion is not None:
+        disp = np.broadcast_to(np.asarray(init_overdispersion, dtype=np.float64),
+                               (G,)).copy()
+
+    on_device = _is_torch(x) and x.is_cuda
+    if on_device:
+        import torch
+        dev = x.device
+        if not x.is_contiguous() and x.stride(1) != 1:
+            x = x.contiguous()
+        ld = x.stride(0)
+        t = lambda a, dt=torch.float64: torch.as_tensor(a, dtype=dt, device=dev)
+        Xd = t(X.T.copy().reshape(-1))  # column-major n x p
+        offd = t(offset_vector)
+        endsd = None if ends is None else t(ends, torch.int32)
+        dispd = None if disp is None else t(disp)
+        beta = torch.empty(G * p, dtype=torch.float64, device=dev)
+        theta = torch.empty(G, dtype=torch.float64, device=dev)
+        bit = torch.empty(G, dtype=torch.int32, device=dev)
+        tit = torch.empty(G, dtype=torch.int32, device=dev)
+        bread = torch.empty(G * p * p, dtype=torch.float64, device=dev)
+        sw = torch.empty(G * p * p, dtype=torch.float64, device=dev) if ends is not None else None
+        st = torch.empty(G, dtype=torch.int32, device=dev)
+        ptr = lambda a: None if a is None else a.data_ptr()
+        prob = L.Problem(G=G, n=n, p=p, mem=L.MEM_DEVICE, counts=x.data_ptr(),
+                         counts_dtype=_counts_dtype(x.dtype), counts_ld=ld, X=ptr(Xd),
+                         offset=ptr(offd), cluster_ends=ptr(endsd),
+                         n_clusters=0 if ends is None else len(ends), disp_init=ptr(dispd),
+                         beta_init=None, init_mode=init, od_mode=od, max_iter=int(max_iter),
+                         tolerance=float(tolerance), precision=prec)
+        outs = L.Outputs(beta=ptr(beta), theta=ptr(theta), beta_iters=ptr(bit),
+                         theta_iters=ptr(tit), bread=ptr(bread), sandwich=ptr(sw),
+                         status=ptr(st))
+        ctx.check(L.lib().devil_fit(ctx.h, C.byref(prob), C.byref(outs)))
+        res = dict(beta=beta.view(p, G).T, theta=theta, beta_iters=bit, theta_iters=tit,
+                   bread=bread.view(G, p, p).transpose(1, 2),
+                   sandwich=None if sw is None else sw.view(G, p, p).transpose(1, 2),
+                   status=st)
+    else:
+        Y = np.asarray(x)
+        if not Y.flags.c_contiguous:
+            Y = np.ascontiguousarray(Y)
+        if Y.dtype not in (np.int32, np.float32, np.float64):
+            Y = Y.astype(np.float64)
+        beta = np.zeros((G, p), order="F")
+        theta = np.zeros(G)
+        bit = np.zeros(G, np.int32)
+        tit = np.zeros(G, np.int32)
+        bread = np.zeros((G, p, p))
+        sw = np.zeros((G, p, p)) if ends is not None else None
+        st = np.zeros(G, np.int32)
+        ptr = lambda a: None if a is None else a.ctypes.data
+        prob = L.Problem(G=G, n=n, p=p, mem=L.MEM_HOST, counts=ptr(Y),
+                         counts_dtype=_counts_dtype(Y.dtype), counts_ld=n, X=ptr(X),
+                         offset=ptr(offset_vector), cluster_ends=ptr(ends),
+                         n_clusters=0 if ends is None else len(ends), disp_init=ptr(disp),
+                         beta_init=None, init_mode=init, od_mode=od, max_iter=int(max_iter),
+                         tolerance=float(tolerance), precision=prec)
+        outs = L.Outputs(beta=ptr(beta), theta=ptr(theta), beta_iters=ptr(bit),
+                         theta_iters=ptr(tit), bread=ptr(bread), sandwich=ptr(sw),
+                         status=ptr(st))
+        ctx.check(L.lib().devil_fit(ctx.h, C.byref(prob), C.byref(outs)))
+        # per-gene blocks are column-major: [g, a, b] = M[a + b p]
+        res = dict(beta=beta, theta=theta, beta_iters=bit, theta_iters=tit,
+                   bread=bread.transpose(0, 2, 1), sandwich=None if sw is None else
+                   sw.transpose(0, 2, 1), status=st)
+    if od == L.OD_MLE:
+        # cpu_fit reports theta_iters = NA for the MLE mode (R/main.R:432)
+        res["theta_passes"] = res["theta_iters"]
+    return {
+        "beta": res["beta"],
+        "beta_sandwiches_null": res["bread"],
+        "beta_sandwiches": res["sandwich"],
+        "overdispersion": res["theta"],
+        "iterations": {"beta_iters": res["beta_iters"], "theta_iters": res["theta_iters"]},
+        "size_factors": sf,
+        "offset_vector": offset_vector,
+        "design_matrix": design_matrix,
+        "input_matrix": x,
+        "input_parameters": {"max_iter": max_iter, "tolerance": tolerance},
+        "status": res["status"],
+    }
+
+
+def test_de(devil_fit, contrast, clusters=None, pval_adjust_method: str = "BH",
+            max_lfc: float = 10, device: int = 0):
+    """R/test.R:85-134 on the GPU: returns dict(name, pval, adj_pval, lfc,
+    log_pval).  ``clusters`` is accepted and unused, as in the reference."""
+    import torch
+    if "beta_sandwiches_null" not in devil_fit:
+        raise ValueError("The fit object was created with an older version of devil and is "
+                         "no longer supported. Please re-fit your model with the current "
+                         "version of fit_devil().")
+    if pval_adjust_method not in ("BH", "fdr"):
+        raise NotImplementedError("only the BH adjustment is implemented")
+    ctx = L.context(device)
+    dev = torch.device("cuda", device)
+    f64 = lambda a: torch.as_tensor(np.asarray(a.cpu() if _is_torch(a) else a,
+                                               dtype=np.float64), device=dev)
+    beta = f64(devil_fit["beta"])
+    G, p = beta.shape
+    n = int(np.asarray(devil_fit["design_matrix"]).shape[0])
+    bcol = beta.T.contiguous()  # column-major G x p
+    bread = f64(devil_fit["beta_sandwiches_null"]).transpose(1, 2).contiguous()
+    sw = devil_fit.get("beta_sandwiches")
+    swd = None if sw is None else f64(sw).transpose(1, 2).contiguous()
+    c = f64(np.asarray(contrast, dtype=np.float64).reshape(-1))
+    if c.numel() != p:
+        raise ValueError("contrast length must match the number of coefficients")
+    pval, logp, adj, lfc = (torch.empty(G, dtype=torch.float64, device=dev) for _ in range(4))
+    ptr = lambda a: None if a is None else a.data_ptr()
+    ctx.check(L.lib().devil_test_de(ctx.h, ptr(bcol), ptr(bread), ptr(swd), G, p, n, ptr(c),
+                                    float(max_lfc), ptr(pval), ptr(logp), ptr(adj), ptr(lfc)))
+    name = getattr(devil_fit["beta"], "index", None)
+    return {"name": name, "pval": pval.cpu().numpy(), "adj_pval": adj.cpu().numpy(),
+            "lfc": lfc.cpu().numpy(), "log_pval": logp.cpu().numpy()}

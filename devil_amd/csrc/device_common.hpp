@@ -1,0 +1,287 @@
+// Device helpers shared by the gfx950 kernels: wavefront reductions, count
+// loads, register-resident p x p LU (partial pivoting) and the special
+// functions the dispersion and test kernels need.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dv {
+
+constexpr int kWave = 64;
+
+// Butterfly sum over the 64 lanes; every lane ends with the same bits
+// (IEEE addition is commutative), so the result is deterministic.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o, kWave));
+  return v;
+}
+
+// Count element i of a gene row stored as int32 (0), float (1) or double (2).
+__device__ __forceinline__ double load_count(const void* row, int dtype, int64_t i) {
+  if (dtype == 0) return (double)static_cast<const int32_t*>(row)[i];
+  if (dtype == 1) return (double)static_cast<const float*>(row)[i];
+  return static_cast<const double*>(row)[i];
+}
+
+__device__ __forceinline__ const void* count_row(const void* base, int dtype, int64_t g,
+                                                 int64_t ld) {
+  const size_t es = dtype == 0 ? 4 : dtype == 1 ? 4 : 8;
+  return static_cast<const char*>(base) + (size_t)g * (size_t)ld * es;
+}
+
+// Packed upper triangle index of (a, b), a <= b, for a P x P symmetric matrix.
+template <int P>
+__host__ __device__ constexpr int tri(int a, int b) {
+  return a * P - a * (a - 1) / 2 + (b - a);
+}
+
+// LU factorisation with partial pivoting in registers.  The pivot is the
+// first entry of largest magnitude (LAPACK idamax / Eigen PartialPivLU);
+// an exactly zero pivot column is left unscaled and reported, as Eigen does.
+template <int P>
+__device__ __forceinline__ int lu_factor(double (&A)[P][P], int (&perm)[P]) {
+  int sing = 0;
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    int r = k;
+    double best = fabs(A[k][k]);
+#pragma unroll
+    for (int i = k + 1; i < P; ++i) {
+      const double v = fabs(A[i][k]);
+      if (v > best) {
+        best = v;
+        r = i;
+      }
+    }
+    perm[k] = r;
+#pragma unroll
+    for (int i = k + 1; i < P; ++i) {
+      if (i == r) {
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+          const double t = A[k][j];
+          A[k][j] = A[i][j];
+          A[i][j] = t;
+        }
+      }
+    }
+    if (best == 0.0) {
+      sing = 1;
+    } else {
+      const double piv = A[k][k];
+#pragma unroll
+      for (int i = k + 1; i < P; ++i) A[i][k] = A[i][k] / piv;
+    }
+#pragma unroll
+    for (int i = k + 1; i < P; ++i)
+#pragma unroll
+      for (int j = k + 1; j < P; ++j) A[i][j] -= A[i][k] * A[k][j];
+  }
+  return sing;
+}
+
+template <int P>
+__device__ __forceinline__ void lu_solve(const double (&LU)[P][P], const int (&perm)[P],
+                                         double (&v)[P]) {
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+#pragma unroll
+    for (int i = k + 1; i < P; ++i) {
+      if (perm[k] == i) {
+        const double t = v[k];
+        v[k] = v[i];
+        v[i] = t;
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < P; ++i)
+#pragma unroll
+    for (int k = 0; k < i; ++k) v[i] -= LU[i][k] * v[k];
+#pragma unroll
+  for (int i = P - 1; i >= 0; --i) {
+#pragma unroll
+    for (int k = i + 1; k < P; ++k) v[i] -= LU[i][k] * v[k];
+    v[i] = v[i] / LU[i][i];
+  }
+}
+
+// Inverse through the LU factors (Eigen's PartialPivLU::inverse).
+template <int P>
+__device__ __forceinline__ int lu_inverse(double (&A)[P][P], double (&Ainv)[P][P]) {
+  int perm[P];
+  const int sing = lu_factor<P>(A, perm);
+#pragma unroll
+  for (int c = 0; c < P; ++c) {
+    double v[P];
+#pragma unroll
+    for (int i = 0; i < P; ++i) v[i] = (i == c) ? 1.0 : 0.0;
+    lu_solve<P>(A, perm, v);
+#pragma unroll
+    for (int i = 0; i < P; ++i) Ainv[i][c] = v[i];
+  }
+  return sing;
+}
+
+// Inverse of a symmetric positive definite matrix by Cholesky (armadillo
+// inv_sympd).  Returns 1 and NaNs when a pivot is not positive.
+template <int P>
+__device__ __forceinline__ int chol_inverse(const double (&A)[P][P], double (&Ainv)[P][P]) {
+  double L[P][P];
+  int bad = 0;
+#pragma unroll
+  for (int j = 0; j < P; ++j) {
+#pragma unroll
+    for (int i = 0; i < P; ++i) L[i][j] = 0.0;
+  }
+#pragma unroll
+  for (int j = 0; j < P; ++j) {
+    double s = A[j][j];
+#pragma unroll
+    for (int k = 0; k < j; ++k) s -= L[j][k] * L[j][k];
+    if (!(s > 0.0)) bad = 1;
+    const double d = sqrt(s);
+    L[j][j] = d;
+#pragma unroll
+    for (int i = j + 1; i < P; ++i) {
+      double t = A[i][j];
+#pragma unroll
+      for (int k = 0; k < j; ++k) t -= L[i][k] * L[j][k];
+      L[i][j] = t / d;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < P; ++c) {
+    double v[P];
+#pragma unroll
+    for (int i = 0; i < P; ++i) v[i] = (i == c) ? 1.0 : 0.0;
+#pragma unroll
+    for (int i = 0; i < P; ++i) {
+#pragma unroll
+      for (int k = 0; k < i; ++k) v[i] -= L[i][k] * v[k];
+      v[i] = v[i] / L[i][i];
+    }
+#pragma unroll
+    for (int i = P - 1; i >= 0; --i) {
+#pragma unroll
+      for (int k = i + 1; k < P; ++k) v[i] -= L[k][i] * v[k];
+      v[i] = v[i] / L[i][i];
+    }
+#pragma unroll
+    for (int i = 0; i < P; ++i) Ainv[i][c] = bad ? __builtin_nan("") : v[i];
+  }
+  return bad;
+}
+
+// ---------------------------------------------------------------------------
+// special functions (x > 0)
+// ---------------------------------------------------------------------------
+
+// digamma: shift to x >= 10, then ln x - 1/(2x) - sum_k B_2k / (2k x^2k)
+// (Abramowitz & Stegun 6.3.18), seven Bernoulli terms.
+__device__ inline double digamma(double x) {
+  if (!(x > 0.0)) return __builtin_nan("");
+  double r = 0.0;
+  while (x < 10.0) {
+    r -= 1.0 / x;
+    x += 1.0;
+  }
+  const double f = 1.0 / (x * x);
+  const double t = f * (1.0 / 12 - f * (1.0 / 120 - f * (1.0 / 252 - f * (1.0 / 240 -
+                   f * (1.0 / 132 - f * (691.0 / 32760 - f * (1.0 / 12)))))));
+  return r + log(x) - 0.5 / x - t;
+}
+
+// trigamma: shift to x >= 10, then 1/x + 1/(2x^2) + sum_k B_2k / x^(2k+1)
+// (A&S 6.4.12).
+__device__ inline double trigamma(double x) {
+  if (!(x > 0.0)) return __builtin_nan("");
+  double r = 0.0;
+  while (x < 10.0) {
+    r += 1.0 / (x * x);
+    x += 1.0;
+  }
+  const double t = 1.0 / x, f = t * t;
+  return r + t + 0.5 * f +
+         t * f * (1.0 / 6 - f * (1.0 / 30 - f * (1.0 / 42 - f * (1.0 / 30 -
+         f * (5.0 / 66 - f * (691.0 / 2730 - f * (7.0 / 6)))))));
+}
+
+// Stirling remainder of lgamma for x >= 10.
+__device__ inline double lgamma_corr(double x) {
+  const double f = 1.0 / (x * x);
+  return (1.0 / 12 - f * (1.0 / 360 - f * (1.0 / 1260 - f * (1.0 / 1680 -
+          f * (1.0 / 1188 - f * (691.0 / 360360 - f * (1.0 / 156))))))) / x;
+}
+
+// log B(a, b) with the large-argument forms that avoid cancellation.
+__device__ inline double lbeta(double a, double b) {
+  const double p = fmin(a, b), q = fmax(a, b);
+  const double half_log_2pi = 0.91893853320467274178;
+  if (p >= 10.0) {
+    const double corr = lgamma_corr(p) + lgamma_corr(q) - lgamma_corr(p + q);
+    return -0.5 * log(q) + half_log_2pi + corr + (p - 0.5) * log(p / (p + q)) +
+           q * log1p(-p / (p + q));
+  } else if (q >= 10.0) {
+    const double corr = lgamma_corr(q) - lgamma_corr(p + q);
+    return lgamma(p) + corr + p - p * log(p + q) + (q - 0.5) * log1p(-p / (p + q));
+  }
+  return lgamma(p) + lgamma(q) - lgamma(p + q);
+}
+
+// Continued fraction of the regularised incomplete beta (modified Lentz).
+__device__ inline double betacf(double a, double b, double x) {
+  const double tiny = 1e-300, epsr = 1e-16;
+  const double qab = a + b, qap = a + 1.0, qam = a - 1.0;
+  double c = 1.0, d = 1.0 - qab * x / qap;
+  if (fabs(d) < tiny) d = tiny;
+  d = 1.0 / d;
+  double h = d;
+  for (int m = 1; m <= 100000; ++m) {
+    const double m2 = 2.0 * m;
+    double aa = m * (b - m) * x / ((qam + m2) * (a + m2));
+    d = 1.0 + aa * d;
+    if (fabs(d) < tiny) d = tiny;
+    c = 1.0 + aa / c;
+    if (fabs(c) < tiny) c = tiny;
+    d = 1.0 / d;
+    h *= d * c;
+    aa = -(a + m) * (qab + m) * x / ((a + m2) * (qap + m2));
+    d = 1.0 + aa * d;
+    if (fabs(d) < tiny) d = tiny;
+    c = 1.0 + aa / c;
+    if (fabs(c) < tiny) c = tiny;
+    d = 1.0 / d;
+    const double del = d * c;
+    h *= del;
+    if (fabs(del - 1.0) < epsr) break;
+  }
+  return h;
+}
+
+// log of 2*P(T_df > |t|) = log I_{df/(df+t^2)}(df/2, 1/2); exact in log space
+// where the probability underflows.
+__device__ inline double log_pt2_upper(double t, double df) {
+  if (isnan(t) || isnan(df) || !(df > 0.0)) return __builtin_nan("");
+  t = fabs(t);
+  if (isinf(t)) return -__builtin_huge_val();
+  if (t == 0.0) return 0.0;
+  const double lt = log(t), ldf = log(df);
+  const double l_sum = (t * t > df) ? 2.0 * lt + log1p(df / t / t) : ldf + log1p(t / df * t);
+  const double lx = ldf - l_sum, lomx = 2.0 * lt - l_sum;
+  const double x = exp(lx), omx = exp(lomx);
+  const double a = 0.5 * df, b = 0.5;
+  const double lfront = a * lx + b * lomx - lbeta(a, b);
+  if (x < (a + 1.0) / (a + b + 2.0)) return lfront + log(betacf(a, b, x)) - log(a);
+  const double lc = lfront + log(betacf(b, a, omx)) - log(b);
+  return log1p(-exp(lc));
+}
+
+}  // namespace dv

@@ -1,0 +1,710 @@
+// C ABI of the gfx950 NB-GLM fit (include/devil_hip.h).
+//
+// devil_fit runs R/main.R:375-523 (cpu_fit) on the GPU:
+//   init pass (alpha0, beta0) -> k = float(1/float(alpha0)) -> IRLS ->
+//   dispersion (MOM | MLE | none) -> bread + clustered meat pass.
+// Per-gene intermediates live on the device in gene-major layout; host-memory
+// problems stream their counts in gene batches sized to the free HBM.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/devil_hip.h"
+#include "kernels.hpp"
+
+struct devil_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int timing = 0;
+  double stage_ms[DEVIL_N_STAGES] = {0, 0, 0, 0, 0};
+};
+
+namespace {
+
+int fail(devil_ctx* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+#define DV_HIP(call)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(ctx, DEVIL_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// Device allocation owned by the caller's scope.
+struct DevBuf {
+  void* p = nullptr;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  hipError_t alloc(size_t bytes) {
+    if (p) {
+      (void)hipFree(p);
+      p = nullptr;
+    }
+    if (bytes == 0) bytes = 8;
+    return hipMalloc(&p, bytes);
+  }
+  template <typename T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+#define DV_ALLOC(buf, bytes)                                                             \
+  do {                                                                                   \
+    hipError_t e_ = (buf).alloc(bytes);                                                  \
+    if (e_ != hipSuccess)                                                                \
+      return fail(ctx, DEVIL_E_NOMEM, std::string("hipMalloc ") + std::to_string(bytes) + \
+                                          " bytes: " + hipGetErrorString(e_));           \
+  } while (0)
+
+size_t count_elem_size(int dtype) { return dtype == DEVIL_COUNTS_F64 ? 8 : 4; }
+
+// Householder QR of the n x p column-major X: thin Q (n x p) and upper R.
+// Returns false when a diagonal of R is below 1e-7 of the largest, the rank
+// tolerance R's qr() uses (LINPACK dqrdc2 tol = 1e-7).
+bool householder_qr(const double* X, int64_t n, int p, std::vector<double>& Q,
+                    std::vector<double>& R) {
+  std::vector<double> A(X, X + (size_t)n * p), V((size_t)n * p, 0.0);
+  std::vector<double> tau(p, 0.0);
+  bool ok = true;
+  for (int k = 0; k < p; ++k) {
+    double nrm = 0.0;
+    for (int64_t i = k; i < n; ++i) nrm += A[i + k * n] * A[i + k * n];
+    nrm = std::sqrt(nrm);
+    if (nrm == 0.0) {
+      ok = false;
+      continue;
+    }
+    const double alpha = A[k + k * n] > 0 ? -nrm : nrm;
+    for (int64_t i = k; i < n; ++i) V[i + k * n] = A[i + k * n];
+    V[k + k * n] -= alpha;
+    double vn = 0.0;
+    for (int64_t i = k; i < n; ++i) vn += V[i + k * n] * V[i + k * n];
+    tau[k] = vn > 0 ? 2.0 / vn : 0.0;
+    for (int j = k; j < p; ++j) {
+      double s = 0.0;
+      for (int64_t i = k; i < n; ++i) s += V[i + k * n] * A[i + j * n];
+      s *= tau[k];
+      for (int64_t i = k; i < n; ++i) A[i + j * n] -= s * V[i + k * n];
+    }
+  }
+  R.assign((size_t)p * p, 0.0);
+  for (int a = 0; a < p; ++a)
+    for (int b = a; b < p; ++b) R[a + b * p] = A[a + b * n];
+  Q.assign((size_t)n * p, 0.0);
+  for (int j = 0; j < p; ++j) Q[j + j * n] = 1.0;
+  for (int k = p - 1; k >= 0; --k) {
+    if (tau[k] == 0.0) continue;
+    for (int j = 0; j < p; ++j) {
+      double s = 0.0;
+      for (int64_t i = k; i < n; ++i) s += V[i + k * n] * Q[i + j * n];
+      s *= tau[k];
+      for (int64_t i = k; i < n; ++i) Q[i + j * n] -= s * V[i + k * n];
+    }
+  }
+  double dmax = 0.0;
+  for (int a = 0; a < p; ++a) dmax = std::max(dmax, std::fabs(R[a + a * p]));
+  for (int a = 0; a < p; ++a)
+    if (!(std::fabs(R[a + a * p]) > 1e-7 * dmax)) ok = false;
+  return ok;
+}
+
+// Host copies of the small per-cell inputs plus what the init pass needs.
+struct Design {
+  int64_t n = 0;
+  int p = 0;
+  std::vector<double> X, off, eo, logsf, Q, R, xmean, denom;
+  double moi = 0.0, sum_sf = 0.0;
+  int intercept_first = 0;
+};
+
+// Device copies of the design, offsets and cluster ends.
+struct DeviceDesign {
+  DevBuf X, off, eo, Q, R, logsf, xmean, denom, ends;
+};
+
+int prepare_design(devil_ctx* ctx, const double* X, const double* off, int64_t n, int p,
+                   int mem, int init_mode, Design& d) {
+  d.n = n;
+  d.p = p;
+  d.X.resize((size_t)n * p);
+  d.off.resize(n);
+  if (mem == DEVIL_MEM_DEVICE) {
+    DV_HIP(hipMemcpy(d.X.data(), X, sizeof(double) * n * p, hipMemcpyDeviceToHost));
+    DV_HIP(hipMemcpy(d.off.data(), off, sizeof(double) * n, hipMemcpyDeviceToHost));
+  } else {
+    std::memcpy(d.X.data(), X, sizeof(double) * n * p);
+    std::memcpy(d.off.data(), off, sizeof(double) * n);
+  }
+  d.eo.resize(n);
+  double se = 0.0;
+  for (int64_t i = 0; i < n; ++i) {
+    d.eo[i] = std::exp(d.off[i]);  // exp_offset, R/main.R:390
+    se += d.eo[i];
+  }
+  d.moi = 1.0 / (se / (double)n);  // R/overdispersion.R:130
+  if (init_mode == DEVIL_INIT_QR) {
+    if (!householder_qr(d.X.data(), n, p, d.Q, d.R))
+      return fail(ctx, DEVIL_E_RANK, "design matrix is rank deficient (QR init, R/beta.R:14)");
+  } else if (init_mode == DEVIL_INIT_UNIVARIATE) {
+    // src/beta_init_univariate.cpp:48-84
+    double mx = 0.0;
+    for (int64_t i = 0; i < n; ++i) mx = std::max(mx, std::fabs(d.X[i] - 1.0));
+    d.intercept_first = mx < 1e-10;
+    d.logsf.resize(n);
+    d.sum_sf = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+      d.logsf[i] = std::log(d.eo[i]);
+      d.sum_sf += d.eo[i];
+    }
+    d.xmean.assign(p, 0.0);
+    d.denom.assign(p, 0.0);
+    for (int j = 0; j < p; ++j) {
+      double s = 0.0;
+      for (int64_t i = 0; i < n; ++i) s += d.X[i + j * n];
+      d.xmean[j] = s / (double)n;
+      double q = 0.0;
+      for (int64_t i = 0; i < n; ++i) {
+        const double c = d.X[i + j * n] - d.xmean[j];
+        q += c * c;
+      }
+      d.denom[j] = q <= 0.0 ? 1e-12 : q;
+    }
+  }
+  return DEVIL_OK;
+}
+
+template <typename T>
+int upload(devil_ctx* ctx, DevBuf& b, const std::vector<T>& v) {
+  DV_ALLOC(b, sizeof(T) * v.size());
+  if (!v.empty())
+    DV_HIP(hipMemcpyAsync(b.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice,
+                          ctx->stream));
+  return DEVIL_OK;
+}
+
+int upload_design(devil_ctx* ctx, const Design& d, DeviceDesign& dd) {
+  int rc;
+  if ((rc = upload(ctx, dd.X, d.X))) return rc;
+  if ((rc = upload(ctx, dd.off, d.off))) return rc;
+  if ((rc = upload(ctx, dd.eo, d.eo))) return rc;
+  if (!d.Q.empty()) {
+    if ((rc = upload(ctx, dd.Q, d.Q))) return rc;
+    if ((rc = upload(ctx, dd.R, d.R))) return rc;
+  }
+  if (!d.logsf.empty()) {
+    if ((rc = upload(ctx, dd.logsf, d.logsf))) return rc;
+    if ((rc = upload(ctx, dd.xmean, d.xmean))) return rc;
+    if ((rc = upload(ctx, dd.denom, d.denom))) return rc;
+  }
+  return DEVIL_OK;
+}
+
+dv::InitParams init_params(const Design& d, const DeviceDesign& dd, int want_alpha,
+                           int init_mode) {
+  dv::InitParams ip{};
+  ip.want_alpha = want_alpha;
+  ip.moi = d.moi;
+  ip.init_mode = init_mode;
+  ip.eo = dd.eo.as<double>();
+  ip.Q = dd.Q.as<double>();
+  ip.R = dd.R.as<double>();
+  ip.X = dd.X.as<double>();
+  ip.logsf = dd.logsf.as<double>();
+  ip.xmean = dd.xmean.as<double>();
+  ip.denom = dd.denom.as<double>();
+  ip.intercept_first = d.intercept_first;
+  ip.sum_sf = d.sum_sf;
+  return ip;
+}
+
+int check_clusters(devil_ctx* ctx, const int32_t* ends, int C, int64_t n) {
+  int64_t prev = 0;
+  for (int c = 0; c < C; ++c) {
+    if (ends[c] < prev)
+      return fail(ctx, DEVIL_E_CLUSTERS,
+                  "cluster ends must be non-decreasing (R/main.R:192-193)");
+    prev = ends[c];
+  }
+  if (C > 0 && prev != n)
+    return fail(ctx, DEVIL_E_CLUSTERS, "last cluster end must equal n");
+  return DEVIL_OK;
+}
+
+int check_counts_args(devil_ctx* ctx, const void* counts, int dtype, int64_t G, int64_t n,
+                      int64_t ld) {
+  if (G < 0 || n < 2) return fail(ctx, DEVIL_E_ARG, "need G >= 0 and n >= 2");
+  if (G > 0 && !counts) return fail(ctx, DEVIL_E_ARG, "counts is NULL");
+  if (dtype < DEVIL_COUNTS_I32 || dtype > DEVIL_COUNTS_F64)
+    return fail(ctx, DEVIL_E_ARG, "unknown counts dtype");
+  if (ld < n) return fail(ctx, DEVIL_E_ARG, "counts_ld < n");
+  if (G > 0x7fffffffLL) return fail(ctx, DEVIL_E_ARG, "G above 2^31-1");
+  return DEVIL_OK;
+}
+
+int check_p(devil_ctx* ctx, int p) {
+  if (p < 1 || p > DEVIL_PMAX)
+    return fail(ctx, DEVIL_E_ARG, "p must be in [1, " + std::to_string(DEVIL_PMAX) + "]");
+  return DEVIL_OK;
+}
+
+// Records HIP events around stages on the context stream; elapsed times are
+// summed into ctx->stage_ms once the stream has been synchronised.
+struct StageTimer {
+  devil_ctx* ctx;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> marks;
+  explicit StageTimer(devil_ctx* c) : ctx(c) {}
+  ~StageTimer() {
+    for (auto& m : marks) {
+      (void)hipEventDestroy(m.second.first);
+      (void)hipEventDestroy(m.second.second);
+    }
+  }
+  hipEvent_t begin(int stage) {
+    if (!ctx->timing) return nullptr;
+    hipEvent_t a = nullptr, b = nullptr;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return nullptr;
+    (void)hipEventRecord(a, ctx->stream);
+    marks.push_back({stage, {a, b}});
+    return b;
+  }
+  void end(hipEvent_t b) {
+    if (b) (void)hipEventRecord(b, ctx->stream);
+  }
+  // call after the stream is synchronised
+  void collect() {
+    for (auto& m : marks) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, m.second.first, m.second.second) == hipSuccess)
+        ctx->stage_ms[m.first] += ms;
+    }
+  }
+};
+
+int set_device(devil_ctx* ctx) {
+  if (!ctx) return DEVIL_E_ARG;
+  DV_HIP(hipSetDevice(ctx->device));
+  return DEVIL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* devil_version(void) { return "devil_hip 0.1.0 gfx950"; }
+
+const char* devil_strerror(int code) {
+  switch (code) {
+    case DEVIL_OK: return "success";
+    case DEVIL_E_ARG: return "invalid argument";
+    case DEVIL_E_HIP: return "HIP runtime error";
+    case DEVIL_E_NOMEM: return "out of memory";
+    case DEVIL_E_RANK: return "design matrix is rank deficient";
+    case DEVIL_E_CLUSTERS: return "invalid cluster ends";
+    case DEVIL_E_NODEVICE: return "no HIP device";
+    case DEVIL_E_UNSUPPORTED: return "unsupported request";
+    default: return "unknown error";
+  }
+}
+
+const char* devil_last_error(const devil_ctx* ctx) { return ctx ? ctx->err.c_str() : ""; }
+
+int devil_create(devil_ctx** out, int hip_device) {
+  if (!out) return DEVIL_E_ARG;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return DEVIL_E_NODEVICE;
+  if (hip_device < 0 || hip_device >= count) return DEVIL_E_NODEVICE;
+  if (hipSetDevice(hip_device) != hipSuccess) return DEVIL_E_HIP;
+  devil_ctx* ctx = new (std::nothrow) devil_ctx();
+  if (!ctx) return DEVIL_E_NOMEM;
+  ctx->device = hip_device;
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return DEVIL_E_HIP;
+  }
+  *out = ctx;
+  return DEVIL_OK;
+}
+
+void devil_destroy(devil_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) {
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamDestroy(ctx->stream);
+  }
+  delete ctx;
+}
+
+int devil_fit(devil_ctx* ctx, const devil_problem* pr, devil_outputs* out) {
+  int rc;
+  if ((rc = set_device(ctx))) return rc;
+  if (!pr || !out) return fail(ctx, DEVIL_E_ARG, "NULL problem or outputs");
+  const int64_t G = pr->G, n = pr->n;
+  const int p = pr->p;
+  if ((rc = check_p(ctx, p))) return rc;
+  if ((rc = check_counts_args(ctx, pr->counts, pr->counts_dtype, G, n, pr->counts_ld)))
+    return rc;
+  if (pr->mem != DEVIL_MEM_HOST && pr->mem != DEVIL_MEM_DEVICE)
+    return fail(ctx, DEVIL_E_ARG, "mem must be DEVIL_MEM_HOST or DEVIL_MEM_DEVICE");
+  if (!pr->X || !pr->offset) return fail(ctx, DEVIL_E_ARG, "X or offset is NULL");
+  if (pr->init_mode < DEVIL_INIT_QR || pr->init_mode > DEVIL_INIT_UNIVARIATE)
+    return fail(ctx, DEVIL_E_ARG, "unknown init_mode");
+  if (pr->od_mode < DEVIL_OD_NONE || pr->od_mode > DEVIL_OD_MLE)
+    return fail(ctx, DEVIL_E_ARG, "unknown od_mode (R/main.R:467)");
+  if (pr->max_iter < 0) return fail(ctx, DEVIL_E_ARG, "max_iter < 0");
+  if (pr->precision != DEVIL_PREC_FP64 && pr->precision != DEVIL_PREC_MIXED)
+    return fail(ctx, DEVIL_E_ARG, "unknown precision");
+  if (pr->precision == DEVIL_PREC_MIXED)
+    return fail(ctx, DEVIL_E_UNSUPPORTED, "mixed precision path not built yet");
+  if (pr->n_clusters < 0 || (pr->n_clusters > 0 && !pr->cluster_ends))
+    return fail(ctx, DEVIL_E_ARG, "n_clusters > 0 needs cluster_ends");
+  if (G > 0 && (!out->beta || !out->theta || !out->beta_iters || !out->theta_iters ||
+                !out->bread))
+    return fail(ctx, DEVIL_E_ARG, "an output pointer is NULL");
+  if (G == 0) return DEVIL_OK;
+  const int mem = pr->mem;
+  const int C = pr->n_clusters;
+
+  std::vector<int32_t> ends_h(C);
+  if (C > 0) {
+    if (mem == DEVIL_MEM_DEVICE)
+      DV_HIP(hipMemcpy(ends_h.data(), pr->cluster_ends, sizeof(int32_t) * C,
+                       hipMemcpyDeviceToHost));
+    else
+      std::memcpy(ends_h.data(), pr->cluster_ends, sizeof(int32_t) * C);
+    if ((rc = check_clusters(ctx, ends_h.data(), C, n))) return rc;
+  }
+  const int init_mode = pr->beta_init ? dv::kInitNone : pr->init_mode;
+  Design d;
+  if ((rc = prepare_design(ctx, pr->X, pr->offset, n, p, mem, init_mode, d))) return rc;
+  DeviceDesign dd;
+  if ((rc = upload_design(ctx, d, dd))) return rc;
+  if (C > 0 && (rc = upload(ctx, dd.ends, ends_h))) return rc;
+
+  // per-gene device state (gene-major)
+  DevBuf alpha0, kvec, beta, theta, iters, thit, status, bread, sandwich;
+  DV_ALLOC(alpha0, sizeof(double) * G);
+  DV_ALLOC(kvec, sizeof(double) * G);
+  DV_ALLOC(beta, sizeof(double) * G * p);
+  DV_ALLOC(theta, sizeof(double) * G);
+  DV_ALLOC(iters, sizeof(int32_t) * G);
+  DV_ALLOC(thit, sizeof(int32_t) * G);
+  DV_ALLOC(status, sizeof(int32_t) * G);
+  DV_ALLOC(bread, sizeof(double) * G * p * p);
+  const bool want_sw = C > 0 && out->sandwich;
+  if (want_sw) DV_ALLOC(sandwich, sizeof(double) * G * p * p);
+
+  hipStream_t s = ctx->stream;
+  if (pr->disp_init) {
+    DV_HIP(hipMemcpyAsync(alpha0.p, pr->disp_init, sizeof(double) * G,
+                          mem == DEVIL_MEM_DEVICE ? hipMemcpyDeviceToDevice
+                                                  : hipMemcpyHostToDevice,
+                          s));
+  }
+  if (pr->beta_init) {
+    DevBuf tmp;
+    DV_ALLOC(tmp, sizeof(double) * G * p);
+    DV_HIP(hipMemcpyAsync(tmp.p, pr->beta_init, sizeof(double) * G * p,
+                          mem == DEVIL_MEM_DEVICE ? hipMemcpyDeviceToDevice
+                                                  : hipMemcpyHostToDevice,
+                          s));
+    DV_HIP(dv::launch_transpose_gp(s, tmp.as<double>(), G, p, 1, beta.as<double>()));
+    DV_HIP(hipStreamSynchronize(s));
+  }
+
+  // counts: device-resident as given, or streamed in gene batches
+  const size_t es = count_elem_size(pr->counts_dtype);
+  int64_t batch = G;
+  DevBuf cbuf;
+  if (mem == DEVIL_MEM_HOST) {
+    size_t freeb = 0, totalb = 0;
+    DV_HIP(hipMemGetInfo(&freeb, &totalb));
+    const size_t row_bytes = (size_t)pr->counts_ld * es;
+    const size_t budget = std::min<size_t>(freeb / 2, (size_t)8 << 30);
+    batch = std::max<int64_t>(1, std::min<int64_t>(G, (int64_t)(budget / row_bytes)));
+    DV_ALLOC(cbuf, row_bytes * batch);
+  }
+  const double eps = (double)(float)pr->tolerance;  // src/beta.cpp:25 float eps
+  const dv::InitParams ip = init_params(d, dd, pr->disp_init ? 0 : 1, init_mode);
+  for (int k = 0; k < DEVIL_STAGE_TEST_DE; ++k) ctx->stage_ms[k] = 0.0;
+  StageTimer tm(ctx);
+  hipEvent_t ev;
+  for (int64_t g0 = 0; g0 < G; g0 += batch) {
+    const int64_t Gb = std::min(batch, G - g0);
+    dv::Counts c;
+    c.dtype = pr->counts_dtype;
+    c.ld = pr->counts_ld;
+    if (mem == DEVIL_MEM_HOST) {
+      DV_HIP(hipMemcpyAsync(cbuf.p,
+                            static_cast<const char*>(pr->counts) + (size_t)g0 * pr->counts_ld * es,
+                            (size_t)Gb * pr->counts_ld * es, hipMemcpyHostToDevice, s));
+      c.ptr = cbuf.p;
+    } else {
+      c.ptr = static_cast<const char*>(pr->counts) + (size_t)g0 * pr->counts_ld * es;
+    }
+    double* a0 = alpha0.as<double>() + g0;
+    double* b0 = beta.as<double>() + g0 * p;
+    ev = tm.begin(DEVIL_STAGE_INIT);
+    DV_HIP(dv::launch_init(s, c, Gb, n, p, ip, a0, b0));
+    DV_HIP(dv::launch_k_from_alpha(s, a0, Gb, kvec.as<double>() + g0));
+    tm.end(ev);
+    ev = tm.begin(DEVIL_STAGE_IRLS);
+    DV_HIP(dv::launch_irls_fp64(s, c, Gb, n, p, dd.X.as<double>(), dd.off.as<double>(),
+                                kvec.as<double>() + g0, b0, pr->max_iter, eps,
+                                iters.as<int32_t>() + g0, status.as<int32_t>() + g0));
+    tm.end(ev);
+    double* th = theta.as<double>() + g0;
+    ev = tm.begin(DEVIL_STAGE_DISPERSION);
+    if (pr->od_mode == DEVIL_OD_MOM) {
+      DV_HIP(dv::launch_mom(s, c, Gb, n, p, dd.X.as<double>(), b0, dd.eo.as<double>(), th,
+                            status.as<int32_t>() + g0));
+      DV_HIP(dv::launch_fill_i32(s, thit.as<int32_t>() + g0, Gb, 0));
+    } else if (pr->od_mode == DEVIL_OD_MLE) {
+      DV_HIP(dv::launch_mle(s, c, Gb, n, p, dd.X.as<double>(), b0, dd.off.as<double>(),
+                            pr->max_iter, th, thit.as<int32_t>() + g0,
+                            status.as<int32_t>() + g0));
+    } else {
+      DV_HIP(hipMemsetAsync(th, 0, sizeof(double) * Gb, s));
+      DV_HIP(dv::launch_fill_i32(s, thit.as<int32_t>() + g0, Gb, 0));
+    }
+    tm.end(ev);
+    ev = tm.begin(DEVIL_STAGE_BREAD_MEAT);
+    DV_HIP(dv::launch_bread_meat(s, c, Gb, n, p, dd.X.as<double>(), b0, th,
+                                 dd.eo.as<double>(), C > 0 ? dd.ends.as<int32_t>() : nullptr,
+                                 C, bread.as<double>() + g0 * p * p,
+                                 want_sw ? sandwich.as<double>() + g0 * p * p : nullptr,
+                                 status.as<int32_t>() + g0));
+    tm.end(ev);
+  }
+
+  // outputs: beta back to R's column-major layout
+  DevBuf bcol;
+  DV_ALLOC(bcol, sizeof(double) * G * p);
+  DV_HIP(dv::launch_transpose_gp(s, beta.as<double>(), G, p, 0, bcol.as<double>()));
+  const hipMemcpyKind kind =
+      mem == DEVIL_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+  DV_HIP(hipMemcpyAsync(out->beta, bcol.p, sizeof(double) * G * p, kind, s));
+  DV_HIP(hipMemcpyAsync(out->theta, theta.p, sizeof(double) * G, kind, s));
+  DV_HIP(hipMemcpyAsync(out->beta_iters, iters.p, sizeof(int32_t) * G, kind, s));
+  DV_HIP(hipMemcpyAsync(out->theta_iters, thit.p, sizeof(int32_t) * G, kind, s));
+  DV_HIP(hipMemcpyAsync(out->bread, bread.p, sizeof(double) * G * p * p, kind, s));
+  if (want_sw)
+    DV_HIP(hipMemcpyAsync(out->sandwich, sandwich.p, sizeof(double) * G * p * p, kind, s));
+  if (out->status)
+    DV_HIP(hipMemcpyAsync(out->status, status.p, sizeof(int32_t) * G, kind, s));
+  DV_HIP(hipStreamSynchronize(s));
+  tm.collect();
+  return DEVIL_OK;
+}
+
+int devil_set_timing(devil_ctx* ctx, int enable) {
+  if (!ctx) return DEVIL_E_ARG;
+  ctx->timing = enable ? 1 : 0;
+  return DEVIL_OK;
+}
+
+int devil_stage_times(const devil_ctx* ctx, double* ms) {
+  if (!ctx || !ms) return DEVIL_E_ARG;
+  for (int k = 0; k < DEVIL_N_STAGES; ++k) ms[k] = ctx->stage_ms[k];
+  return DEVIL_OK;
+}
+
+// ---------------------------------------------------------------------------
+// stage entry points (device pointers)
+// ---------------------------------------------------------------------------
+
+int devil_estimate_dispersion(devil_ctx* ctx, const void* counts, int32_t dtype, int64_t G,
+                              int64_t n, int64_t ld, const double* offset, double* alpha0) {
+  int rc;
+  if ((rc = set_device(ctx))) return rc;
+  if ((rc = check_counts_args(ctx, counts, dtype, G, n, ld))) return rc;
+  if (!offset || (G > 0 && !alpha0)) return fail(ctx, DEVIL_E_ARG, "NULL pointer");
+  std::vector<double> off(n);
+  DV_HIP(hipMemcpy(off.data(), offset, sizeof(double) * n, hipMemcpyDeviceToHost));
+  double se = 0.0;
+  for (int64_t i = 0; i < n; ++i) se += std::exp(off[i]);
+  dv::InitParams ip{};
+  ip.want_alpha = 1;
+  ip.moi = 1.0 / (se / (double)n);
+  ip.init_mode = dv::kInitNone;
+  dv::Counts c{counts, dtype, ld};
+  DV_HIP(dv::launch_init(ctx->stream, c, G, n, 1, ip, alpha0, nullptr));
+  DV_HIP(hipStreamSynchronize(ctx->stream));
+  return DEVIL_OK;
+}
+
+int devil_init_beta(devil_ctx* ctx, const void* counts, int32_t dtype, int64_t G, int64_t n,
+                    int64_t ld, const double* X, int32_t p, const double* offset,
+                    int32_t init_mode, double* beta0) {
+  int rc;
+  if ((rc = set_device(ctx))) return rc;
+  if ((rc = check_p(ctx, p))) return rc;
+  if ((rc = check_counts_args(ctx, counts, dtype, G, n, ld))) return rc;
+  if (!X || !offset || (G > 0 && !beta0)) return fail(ctx, DEVIL_E_ARG, "NULL pointer");
+  if (init_mode < DEVIL_INIT_QR || init_mode > DEVIL_INIT_UNIVARIATE)
+    return fail(ctx, DEVIL_E_ARG, "unknown init_mode");
+  Design d;
+  if ((rc = prepare_design(ctx, X, offset, n, p, DEVIL_MEM_DEVICE, init_mode, d))) return rc;
+  DeviceDesign dd;
+  if ((rc = upload_design(ctx, d, dd))) return rc;
+  DevBuf tmp;
+  DV_ALLOC(tmp, sizeof(double) * G * p);
+  dv::Counts c{counts, dtype, ld};
+  const dv::InitParams ip = init_params(d, dd, 0, init_mode);
+  DV_HIP(dv::launch_init(ctx->stream, c, G, n, p, ip, nullptr, tmp.as<double>()));
+  DV_HIP(dv::launch_transpose_gp(ctx->stream, tmp.as<double>(), G, p, 0, beta0));
+  DV_HIP(hipStreamSynchronize(ctx->stream));
+  return DEVIL_OK;
+}
+
+int devil_beta_fit(devil_ctx* ctx, const void* counts, int32_t dtype, int64_t G, int64_t n,
+                   int64_t ld, const double* X, int32_t p, double* beta, const double* offset,
+                   const double* alpha0, int32_t max_iter, double tolerance, int32_t precision,
+                   int32_t* iters, int32_t* status) {
+  int rc;
+  if ((rc = set_device(ctx))) return rc;
+  if ((rc = check_p(ctx, p))) return rc;
+  if ((rc = check_counts_args(ctx, counts, dtype, G, n, ld))) return rc;
+  if (!X || !offset || (G > 0 && (!beta || !alpha0 || !iters)))
+    return fail(ctx, DEVIL_E_ARG, "NULL pointer");
+  if (max_iter < 0) return fail(ctx, DEVIL_E_ARG, "max_iter < 0");
+  if (precision != DEVIL_PREC_FP64)
+    return fail(ctx, DEVIL_E_UNSUPPORTED, "mixed precision path not built yet");
+  if (G == 0) return DEVIL_OK;
+  hipStream_t s = ctx->stream;
+  DevBuf bg, kv;
+  DV_ALLOC(bg, sizeof(double) * G * p);
+  DV_ALLOC(kv, sizeof(double) * G);
+  DV_HIP(dv::launch_transpose_gp(s, beta, G, p, 1, bg.as<double>()));
+  DV_HIP(dv::launch_k_from_alpha(s, alpha0, G, kv.as<double>()));
+  dv::Counts c{counts, dtype, ld};
+  DV_HIP(dv::launch_irls_fp64(s, c, G, n, p, X, offset, kv.as<double>(), bg.as<double>(),
+                              max_iter, (double)(float)tolerance, iters, status));
+  DV_HIP(dv::launch_transpose_gp(s, bg.as<double>(), G, p, 0, beta));
+  DV_HIP(hipStreamSynchronize(s));
+  return DEVIL_OK;
+}
+
+int devil_mom_dispersion(devil_ctx* ctx, const void* counts, int32_t dtype, int64_t G,
+                         int64_t n, int64_t ld, const double* X, int32_t p, const double* beta,
+                         const double* offset, double* theta) {
+  int rc;
+  if ((rc = set_device(ctx))) return rc;
+  if ((rc = check_p(ctx, p))) return rc;
+  if ((rc = check_counts_args(ctx, counts, dtype, G, n, ld))) return rc;
+  if (!X || !offset || (G > 0 && (!beta || !theta)))
+    return fail(ctx, DEVIL_E_ARG, "NULL pointer");
+  if (G == 0) return DEVIL_OK;
+  hipStream_t s = ctx->stream;
+  std::vector<double> off(n);
+  DV_HIP(hipMemcpy(off.data(), offset, sizeof(double) * n, hipMemcpyDeviceToHost));
+  for (auto& v : off) v = std::exp(v);
+  DevBuf eo, bg;
+  if ((rc = upload(ctx, eo, off))) return rc;
+  DV_ALLOC(bg, sizeof(double) * G * p);
+  DV_HIP(dv::launch_transpose_gp(s, beta, G, p, 1, bg.as<double>()));
+  dv::Counts c{counts, dtype, ld};
+  DV_HIP(dv::launch_mom(s, c, G, n, p, X, bg.as<double>(), eo.as<double>(), theta, nullptr));
+  DV_HIP(hipStreamSynchronize(s));
+  return DEVIL_OK;
+}
+
+int devil_fit_dispersion_mle(devil_ctx* ctx, const void* counts, int32_t dtype, int64_t G,
+                             int64_t n, int64_t ld, const double* X, int32_t p,
+                             const double* beta, const double* offset, int32_t max_iter,
+                             double* theta, int32_t* iters) {
+  int rc;
+  if ((rc = set_device(ctx))) return rc;
+  if ((rc = check_p(ctx, p))) return rc;
+  if ((rc = check_counts_args(ctx, counts, dtype, G, n, ld))) return rc;
+  if (!X || !offset || (G > 0 && (!beta || !theta || !iters)))
+    return fail(ctx, DEVIL_E_ARG, "NULL pointer");
+  if (G == 0) return DEVIL_OK;
+  hipStream_t s = ctx->stream;
+  DevBuf bg;
+  DV_ALLOC(bg, sizeof(double) * G * p);
+  DV_HIP(dv::launch_transpose_gp(s, beta, G, p, 1, bg.as<double>()));
+  dv::Counts c{counts, dtype, ld};
+  DV_HIP(dv::launch_mle(s, c, G, n, p, X, bg.as<double>(), offset, max_iter, theta, iters,
+                        nullptr));
+  DV_HIP(hipStreamSynchronize(s));
+  return DEVIL_OK;
+}
+
+int devil_bread_meat(devil_ctx* ctx, const void* counts, int32_t dtype, int64_t G, int64_t n,
+                     int64_t ld, const double* X, int32_t p, const double* beta,
+                     const double* theta, const double* offset, const int32_t* cluster_ends,
+                     int32_t n_clusters, double* bread, double* sandwich, int32_t* status) {
+  int rc;
+  if ((rc = set_device(ctx))) return rc;
+  if ((rc = check_p(ctx, p))) return rc;
+  if ((rc = check_counts_args(ctx, counts, dtype, G, n, ld))) return rc;
+  if (!X || !offset || (G > 0 && (!beta || !theta || !bread)))
+    return fail(ctx, DEVIL_E_ARG, "NULL pointer");
+  if (n_clusters < 0 || (n_clusters > 0 && !cluster_ends))
+    return fail(ctx, DEVIL_E_ARG, "n_clusters > 0 needs cluster_ends");
+  if (G == 0) return DEVIL_OK;
+  hipStream_t s = ctx->stream;
+  if (n_clusters > 0) {
+    std::vector<int32_t> e(n_clusters);
+    DV_HIP(hipMemcpy(e.data(), cluster_ends, sizeof(int32_t) * n_clusters,
+                     hipMemcpyDeviceToHost));
+    if ((rc = check_clusters(ctx, e.data(), n_clusters, n))) return rc;
+  }
+  std::vector<double> off(n);
+  DV_HIP(hipMemcpy(off.data(), offset, sizeof(double) * n, hipMemcpyDeviceToHost));
+  for (auto& v : off) v = std::exp(v);
+  DevBuf eo, bg;
+  if ((rc = upload(ctx, eo, off))) return rc;
+  DV_ALLOC(bg, sizeof(double) * G * p);
+  DV_HIP(dv::launch_transpose_gp(s, beta, G, p, 1, bg.as<double>()));
+  dv::Counts c{counts, dtype, ld};
+  DV_HIP(dv::launch_bread_meat(s, c, G, n, p, X, bg.as<double>(), theta, eo.as<double>(),
+                               cluster_ends, n_clusters, bread,
+                               n_clusters > 0 ? sandwich : nullptr, status));
+  DV_HIP(hipStreamSynchronize(s));
+  return DEVIL_OK;
+}
+
+int devil_test_de(devil_ctx* ctx, const double* beta, const double* bread,
+                  const double* sandwich, int64_t G, int32_t p, int64_t n,
+                  const double* contrast, double max_lfc, double* pval, double* log_pval,
+                  double* adj_pval, double* lfc) {
+  int rc;
+  if ((rc = set_device(ctx))) return rc;
+  if ((rc = check_p(ctx, p))) return rc;
+  if (G < 0 || (G > 0 && (!beta || !bread || !contrast || !pval || !adj_pval || !lfc)))
+    return fail(ctx, DEVIL_E_ARG, "NULL pointer or G < 0");
+  if (G == 0) return DEVIL_OK;
+  hipStream_t s = ctx->stream;
+  DevBuf bg, rank;
+  DV_ALLOC(bg, sizeof(double) * G * p);
+  DV_ALLOC(rank, sizeof(int32_t) * G);
+  ctx->stage_ms[DEVIL_STAGE_TEST_DE] = 0.0;
+  StageTimer tm(ctx);
+  hipEvent_t ev = tm.begin(DEVIL_STAGE_TEST_DE);
+  DV_HIP(dv::launch_transpose_gp(s, beta, G, p, 1, bg.as<double>()));
+  DV_HIP(dv::launch_test_de(s, bg.as<double>(), bread, sandwich, G, p, (double)n - 2.0,
+                            contrast, max_lfc, pval, log_pval, lfc, rank.as<int32_t>(),
+                            adj_pval));
+  tm.end(ev);
+  DV_HIP(hipStreamSynchronize(s));
+  tm.collect();
+  return DEVIL_OK;
+}
+
+}  // extern "C"

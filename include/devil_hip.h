@@ -1,0 +1,188 @@
+/*
+ * devil_hip.h -- C ABI of the MI355X (gfx950) per-gene NB-GLM fit.
+ *
+ * This is the drop-in boundary for devil's fitting hot path.  Each entry
+ * point replaces one reference interface (paths relative to the devil
+ * 0.99.0 tree):
+ *
+ *   devil_fit                 gpu_fit / cpu_fit          R/main.R:277-371, 375-523
+ *                             via Rcpp beta_fit_gpu      src/beta.cpp:82-87
+ *                             -> beta_fit_gpu_external   cuda/src/batch.hpp:18-27
+ *   devil_estimate_dispersion estimate_dispersion        R/overdispersion.R:129-135
+ *   devil_init_beta           init_beta (QR)             R/beta.R:13-20
+ *                             rough init                 R/main.R:404-406
+ *                             univariate init            src/beta_init_univariate.cpp:13-95
+ *   devil_beta_fit            beta_fit, over all genes   src/beta.cpp:25-51
+ *   devil_mom_dispersion      estimate_mom_dispersion_cpp src/mom_overdispersion.cpp:6-56
+ *   devil_fit_dispersion_mle  fit_dispersion             R/overdispersion.R:25-115
+ *                             (score/dscore of           src/overdispersion.cpp:95-242)
+ *   devil_bread_meat          compute_hessian            src/variance.cpp:22-47
+ *                             compute_clustered_meat_fast src/variance.cpp:186-228
+ *                             sandwich assembly          R/main.R:481-492
+ *   devil_test_de             test_de                    R/test.R:85-134
+ *
+ * Conventions
+ *   - Every function returns DEVIL_OK (0) or a DEVIL_E_* code; nothing calls
+ *     exit() or abort().  devil_last_error(ctx) gives the message of the last
+ *     failure on that context.
+ *   - counts are gene-major: gene g's n counts start at element g*counts_ld.
+ *   - X is n x p column-major (R's matrix layout, zero-copy from a REALSXP).
+ *   - beta is G x p column-major (R layout).  Per-gene p x p matrices are
+ *     column-major and contiguous: M[g*p*p + a + b*p].
+ *   - devil_fit takes host or device arrays (devil_problem.mem); the stage
+ *     entry points take device pointers only.
+ *   - A context is not thread-safe; distinct contexts may be used from
+ *     distinct host threads.  Calls are synchronous: outputs are complete on
+ *     return.
+ */
+#ifndef DEVIL_HIP_H
+#define DEVIL_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* return codes */
+#define DEVIL_OK 0
+#define DEVIL_E_ARG 1          /* invalid argument (pointer, size, mode, p out of range) */
+#define DEVIL_E_HIP 2          /* HIP runtime error */
+#define DEVIL_E_NOMEM 3        /* device or host allocation failed */
+#define DEVIL_E_RANK 4         /* design matrix rank deficient (QR init) */
+#define DEVIL_E_CLUSTERS 5     /* cluster ends not increasing or last end != n */
+#define DEVIL_E_NODEVICE 6     /* no HIP device / device index out of range */
+#define DEVIL_E_UNSUPPORTED 7  /* valid request this build does not implement */
+
+/* per-gene status bits (devil_outputs.status) */
+#define DEVIL_GENE_MAXITER 1   /* beta loop stopped at max_iter (src/beta.cpp:37) */
+#define DEVIL_GENE_NAN 2       /* beta loop stopped on a NaN step (src/beta.cpp:45) */
+#define DEVIL_GENE_SINGULAR 4  /* a p x p factorisation met an exactly zero pivot */
+#define DEVIL_GENE_MOM_DEN0 8  /* MOM denominator <= 0, theta = 0 */
+#define DEVIL_GENE_MLE_ZERO 16 /* MLE special case returned 0 (all-zero gene or score(1e-8) < 0) */
+
+#define DEVIL_PMAX 10          /* largest design width this build supports */
+
+enum { DEVIL_MEM_HOST = 0, DEVIL_MEM_DEVICE = 1 };
+enum { DEVIL_COUNTS_I32 = 0, DEVIL_COUNTS_F32 = 1, DEVIL_COUNTS_F64 = 2 };
+enum { DEVIL_INIT_QR = 0, DEVIL_INIT_ROUGH = 1, DEVIL_INIT_UNIVARIATE = 2 };
+enum { DEVIL_OD_NONE = 0, DEVIL_OD_MOM = 1, DEVIL_OD_MLE = 2 };
+enum { DEVIL_PREC_FP64 = 0, DEVIL_PREC_MIXED = 1 };
+
+typedef struct devil_ctx devil_ctx;
+
+/* One context per host thread: owns a HIP stream and device workspaces. */
+int devil_create(devil_ctx **ctx, int hip_device);
+void devil_destroy(devil_ctx *ctx);
+const char *devil_strerror(int code);
+const char *devil_last_error(const devil_ctx *ctx);
+/* Library version string, e.g. "devil_hip 0.1.0 gfx950". */
+const char *devil_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* Whole fit: R/main.R:375-523 cpu_fit semantics (the numerical contract),   */
+/* called where gpu_fit (R/main.R:277-371) calls beta_fit_gpu.              */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+  int64_t G, n;
+  int32_t p;
+  int32_t mem;                  /* DEVIL_MEM_* : where every array below lives */
+  const void *counts;           /* G x n, gene-major */
+  int32_t counts_dtype;         /* DEVIL_COUNTS_* */
+  int64_t counts_ld;            /* elements between consecutive genes (>= n) */
+  const double *X;              /* n x p column-major */
+  const double *offset;         /* n: off + log(size_factors) (R/size_factors.R:172-184) */
+  const int32_t *cluster_ends;  /* cumulative 1-based ends (R/main.R:193), or NULL */
+  int32_t n_clusters;           /* 0 = no clustered sandwich */
+  const double *disp_init;      /* G (init_overdispersion), or NULL -> estimate_dispersion */
+  const double *beta_init;      /* G x p column-major, or NULL -> init_mode */
+  int32_t init_mode;            /* DEVIL_INIT_* (R/main.R:404-409) */
+  int32_t od_mode;              /* DEVIL_OD_* (R/main.R:431-468) */
+  int32_t max_iter;             /* R/main.R:175 */
+  double tolerance;             /* R/main.R:176; used as a float, src/beta.cpp:25 */
+  int32_t precision;            /* DEVIL_PREC_* */
+} devil_problem;
+
+typedef struct {                /* caller-allocated, same memory space as the problem */
+  double *beta;                 /* G x p column-major */
+  double *theta;                /* G: overdispersion (MOM / MLE / 0) */
+  int32_t *beta_iters;          /* G */
+  int32_t *theta_iters;         /* G: MLE score passes; 0 for MOM / none */
+  double *bread;                /* G x p x p: beta_sandwiches_null */
+  double *sandwich;             /* G x p x p, or NULL: beta_sandwiches (clusters only) */
+  int32_t *status;              /* G, or NULL: DEVIL_GENE_* bits */
+} devil_outputs;
+
+int devil_fit(devil_ctx *ctx, const devil_problem *prob, devil_outputs *out);
+
+/* ------------------------------------------------------------------------ */
+/* Stage entry points (device pointers; counts as in devil_problem).         */
+/* ------------------------------------------------------------------------ */
+
+/* R/overdispersion.R:129-135: alpha0[g] = (var_g - ybar_g/mean(exp(off))) / ybar_g^2,
+ * var with n-1; NaN or negative -> 0.01. */
+int devil_estimate_dispersion(devil_ctx *ctx, const void *counts, int32_t counts_dtype,
+                              int64_t G, int64_t n, int64_t counts_ld,
+                              const double *offset, double *alpha0);
+
+/* R/beta.R:13-20 (mode QR), R/main.R:404-406 (ROUGH),
+ * src/beta_init_univariate.cpp:13-95 (UNIVARIATE, with sf = exp(offset)). */
+int devil_init_beta(devil_ctx *ctx, const void *counts, int32_t counts_dtype, int64_t G,
+                    int64_t n, int64_t counts_ld, const double *X, int32_t p,
+                    const double *offset, int32_t init_mode, double *beta0);
+
+/* src/beta.cpp:25-51 for every gene: beta (G x p col-major) is updated in place;
+ * alpha0 holds the per-gene dispersion the reference passes as `float k`. */
+int devil_beta_fit(devil_ctx *ctx, const void *counts, int32_t counts_dtype, int64_t G,
+                   int64_t n, int64_t counts_ld, const double *X, int32_t p,
+                   double *beta, const double *offset, const double *alpha0,
+                   int32_t max_iter, double tolerance, int32_t precision,
+                   int32_t *iters, int32_t *status);
+
+/* src/mom_overdispersion.cpp:6-56 with sf = exp(offset). */
+int devil_mom_dispersion(devil_ctx *ctx, const void *counts, int32_t counts_dtype,
+                         int64_t G, int64_t n, int64_t counts_ld, const double *X,
+                         int32_t p, const double *beta, const double *offset,
+                         double *theta);
+
+/* R/overdispersion.R:25-115 with Cox-Reid adjustment: special cases and start
+ * value as the reference; root of the score in log(theta) by safeguarded
+ * Newton (the reference drives nlminb).  iters: score passes per gene. */
+int devil_fit_dispersion_mle(devil_ctx *ctx, const void *counts, int32_t counts_dtype,
+                             int64_t G, int64_t n, int64_t counts_ld, const double *X,
+                             int32_t p, const double *beta, const double *offset,
+                             int32_t max_iter, double *theta, int32_t *iters);
+
+/* src/variance.cpp:22-47 (bread) and :186-228 (meat) in one pass; sandwich =
+ * bread * meat * bread * n (R/main.R:491) when cluster_ends != NULL. */
+int devil_bread_meat(devil_ctx *ctx, const void *counts, int32_t counts_dtype, int64_t G,
+                     int64_t n, int64_t counts_ld, const double *X, int32_t p,
+                     const double *beta, const double *theta, const double *offset,
+                     const int32_t *cluster_ends, int32_t n_clusters, double *bread,
+                     double *sandwich, int32_t *status);
+
+/* R/test.R:85-134: p = 2*pt(|c'b| / sqrt(c'Vc), n-2, upper); clustered:
+ * max(p_sandwich, p_bread); BH over non-NaN p; lfc = c'b/log(2) clipped to
+ * +-max_lfc.  sandwich may be NULL.  log_pval (natural log, exact where p
+ * underflows) may be NULL. */
+int devil_test_de(devil_ctx *ctx, const double *beta, const double *bread,
+                  const double *sandwich, int64_t G, int32_t p, int64_t n,
+                  const double *contrast, double max_lfc, double *pval,
+                  double *log_pval, double *adj_pval, double *lfc);
+
+/* ------------------------------------------------------------------------ */
+/* Instrumentation (no counterpart in the reference).                        */
+/* ------------------------------------------------------------------------ */
+enum { DEVIL_STAGE_INIT = 0, DEVIL_STAGE_IRLS = 1, DEVIL_STAGE_DISPERSION = 2,
+       DEVIL_STAGE_BREAD_MEAT = 3, DEVIL_STAGE_TEST_DE = 4, DEVIL_N_STAGES = 5 };
+
+/* Enable (1) or disable (0) per-stage timing with HIP events recorded on the
+ * context's stream around each stage's kernels. */
+int devil_set_timing(devil_ctx *ctx, int enable);
+/* Device milliseconds per stage of the last devil_fit (stages 0-3) and the
+ * last devil_test_de (stage 4); ms must hold DEVIL_N_STAGES doubles. */
+int devil_stage_times(const devil_ctx *ctx, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

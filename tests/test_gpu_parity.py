@@ -1,0 +1,272 @@
+"""GPU parity: the HIP fit (through the C ABI) against the FP64 oracle on the
+same seeded inputs.
+
+Bars (FP64 path): beta within 1e-8 relative (floor 1e-6), identical per-gene
+IRLS iteration counts, theta / bread / sandwich within 1e-8 relative, and
+p-values within 1e-6 relative.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def relerr(a, b, floor):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    both_nan = np.isnan(a) & np.isnan(b)
+    e = np.abs(a - b) / np.maximum(np.abs(b), floor)
+    e[both_nan] = 0.0
+    return e
+
+
+def assert_rel(a, b, tol, floor, what):
+    e = relerr(a, b, floor)
+    worst = np.nanmax(e) if e.size else 0.0
+    assert np.all(np.isfinite(e)) and worst <= tol, f"{what}: max rel err {worst:.3e} > {tol}"
+
+
+@pytest.fixture(scope="module")
+def dv():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import devil_amd
+    devil_amd.lib()
+    return devil_amd
+
+
+def run_both(dv, orc, w, Y, *, od, clusters=None, init="qr", max_iter=200, tol=1e-3,
+             disp=None):
+    say(f"[case] {w.name} G={Y.shape[0]} n={Y.shape[1]} p={w.p} od={od} init={init} "
+        f"dtype={Y.dtype} clusters={'yes' if clusters is not None else 'no'}")
+    fit = dv.fit_devil(Y, w.X, clusters=clusters, overdispersion=od, size_factors=w.sf,
+                       max_iter=max_iter, tolerance=tol, init_mode=init,
+                       init_overdispersion=disp)
+    ends = None if clusters is None else dv.cluster_blocks(clusters, Y.shape[1])
+    odm = {"MOM": orc.OD_MOM, "MLE": orc.OD_MLE, False: orc.OD_NONE}[od]
+    im = {"qr": orc.INIT_QR, "rough": orc.INIT_ROUGH, "univariate": orc.INIT_UNIVARIATE}[init]
+    ref = orc.cpu_fit(Y.astype(np.float64), w.X, w.offset, clusters_ends=ends, od_mode=odm,
+                      init_mode=im, max_iter=max_iter, tolerance=tol,
+                      disp_init=None if disp is None else np.full(Y.shape[0], disp))
+    return fit, ref
+
+
+def check_fit(fit, ref, *, theta_tol=1e-8):
+    np.testing.assert_array_equal(fit["iterations"]["beta_iters"], ref["beta_iters"])
+    assert_rel(fit["beta"], ref["beta"], 1e-8, 1e-6, "beta")
+    assert_rel(fit["overdispersion"], ref["theta"], theta_tol, 1e-8, "theta")
+    assert_rel(fit["beta_sandwiches_null"], ref["bread"], 1e-8, 1e-12, "bread")
+    if ref["sandwich"] is not None:
+        assert_rel(fit["beta_sandwiches"], ref["sandwich"], 1e-8, 1e-12, "sandwich")
+
+
+def test_fit_c1(dv, orc):
+    from devil_amd import synth
+    w = synth.make_workload("C1", G=400, n=5000)
+    Y = synth.counts_host(w)
+    fit, ref = run_both(dv, orc, w, Y, od="MOM")
+    check_fit(fit, ref)
+    st = fit["status"]
+    hit = fit["iterations"]["beta_iters"] == 200
+    np.testing.assert_array_equal((st & 1) != 0, hit)
+
+
+def test_fit_clustered_c4(dv, orc):
+    from devil_amd import synth
+    w = synth.make_workload("C4", G=120, n=6000)
+    Y = synth.counts_host(w)
+    fit, ref = run_both(dv, orc, w, Y, od="MOM", clusters=w.clusters)
+    check_fit(fit, ref)
+
+
+def test_fit_mle_c2(dv, orc):
+    from devil_amd import synth
+    w = synth.make_workload("C2", G=80, n=4000)
+    Y = synth.counts_host(w)
+    fit, ref = run_both(dv, orc, w, Y, od="MLE")
+    check_fit(fit, ref, theta_tol=1e-8)
+    same = np.mean(fit["iterations"]["theta_iters"] == ref["theta_iters"])
+    assert same >= 0.95
+
+
+@pytest.mark.parametrize("init", ["rough", "univariate"])
+def test_fit_init_modes(dv, orc, init):
+    from devil_amd import synth
+    w = synth.make_workload("C1", G=150, n=3000, seed=11)
+    Y = synth.counts_host(w)
+    fit, ref = run_both(dv, orc, w, Y, od="MOM", init=init)
+    check_fit(fit, ref)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_fit_count_dtypes(dv, orc, dtype):
+    from devil_amd import synth
+    w = synth.make_workload("C1", G=100, n=2500, seed=12)
+    Y = synth.counts_host(w, dtype=dtype)
+    fit, ref = run_both(dv, orc, w, Y, od="MOM")
+    check_fit(fit, ref)
+
+
+@pytest.mark.parametrize("p", [1, 3, 10])
+def test_fit_design_widths(dv, orc, p):
+    from devil_amd import synth
+    rng = np.random.default_rng(100 + p)
+    G, n = 60, 3000
+    X = np.column_stack([np.ones(n)] + [rng.normal(size=n) for _ in range(p - 1)])
+    w = synth.Workload("wide", G, n, p, X, np.exp(rng.normal(0, 0.3, n)),
+                       np.column_stack([rng.normal(0, 1, G)] +
+                                       [rng.normal(0, 0.1, G) for _ in range(p - 1)]),
+                       np.exp(rng.normal(0, 1, G)), seed=99 + p)
+    Y = synth.counts_host(w)
+    fit, ref = run_both(dv, orc, w, Y, od="MOM", clusters=np.repeat([1, 2, 3], n // 3))
+    check_fit(fit, ref)
+
+
+def test_fit_edge_cases(dv, orc):
+    from devil_amd import synth
+    w = synth.make_workload("C1", G=50, n=2000, seed=13)
+    Y = synth.counts_host(w)
+    Y[3] = 0                       # all-zero gene
+    Y[7] = 0
+    Y[7, 5] = 40                   # a single non-zero count
+    fit, ref = run_both(dv, orc, w, Y, od="MOM", max_iter=5)  # many genes hit max_iter
+    check_fit(fit, ref)
+    fit2, ref2 = run_both(dv, orc, w, Y, od="MLE", max_iter=200)
+    check_fit(fit2, ref2)
+    # alpha0 = 0 -> k = inf -> NaN step after one iteration (src/beta.cpp:45)
+    fit3, ref3 = run_both(dv, orc, w, Y, od=False, disp=0.0)
+    assert np.all(fit3["iterations"]["beta_iters"] == 1)
+    assert np.all(np.isnan(fit3["beta"]))
+    np.testing.assert_array_equal(fit3["iterations"]["beta_iters"], ref3["beta_iters"])
+
+
+def test_fit_device_resident_and_deterministic(dv, orc):
+    from devil_amd import synth
+    w = synth.make_workload("C1", G=300, n=4000, seed=14)
+    Y = synth.counts_host(w)
+    say("[case] device-resident counts, two runs")
+    host = dv.fit_devil(Y, w.X, size_factors=w.sf)
+    Yd = torch.as_tensor(Y, device="cuda")
+    d1 = dv.fit_devil(Yd, w.X, size_factors=w.sf)
+    d2 = dv.fit_devil(Yd, w.X, size_factors=w.sf)
+    for key in ("beta", "overdispersion", "beta_sandwiches_null"):
+        a = d1[key].cpu().numpy()
+        np.testing.assert_array_equal(a, d2[key].cpu().numpy())
+        np.testing.assert_array_equal(a, host[key])
+
+
+def test_test_de_parity(dv, orc):
+    from devil_amd import synth
+    w = synth.make_workload("C4", G=200, n=5000, seed=15)
+    Y = synth.counts_host(w)
+    say("[case] test_de on a clustered fit")
+    fit = dv.fit_devil(Y, w.X, clusters=w.clusters, size_factors=w.sf)
+    res = dv.test_de(fit, contrast=[0, 1, 0, 0, 0, 0], max_lfc=2)
+    ref = orc.de_test(fit["beta"], fit["beta_sandwiches_null"], fit["beta_sandwiches"], w.n,
+                      [0, 1, 0, 0, 0, 0], max_lfc=2)
+    assert_rel(res["log_pval"], ref["log_pval"], 1e-10, 1e-3, "log p")
+    assert_rel(res["pval"], ref["pval"], 1e-6, 1e-300, "p")
+    assert_rel(res["adj_pval"], ref["adj_pval"], 1e-6, 1e-300, "adj p")
+    np.testing.assert_allclose(res["lfc"], ref["lfc"], rtol=1e-15, atol=0)
+    # unclustered fit: p from the bread only
+    fit0 = dv.fit_devil(Y, w.X, size_factors=w.sf)
+    r0 = dv.test_de(fit0, contrast=[0, 1, 0, 0, 0, 0])
+    f0 = orc.de_test(fit0["beta"], fit0["beta_sandwiches_null"], None, w.n, [0, 1, 0, 0, 0, 0])
+    assert_rel(r0["pval"], f0["pval"], 1e-6, 1e-300, "p (bread)")
+    np.testing.assert_array_equal(np.isnan(r0["adj_pval"]), np.isnan(f0["adj_pval"]))
+
+
+def test_stage_entry_points(dv, orc):
+    """Each C-ABI stage against the oracle function it restates."""
+    import ctypes as C
+    from devil_amd import _lib as L
+    from devil_amd import synth
+    say("[case] stage entry points")
+    w = synth.make_workload("C2", G=64, n=3000, seed=16)
+    Y = synth.counts_host(w)
+    ctx = dv.context(0)
+    lib = dv.lib()
+    dev = "cuda"
+    t = lambda a, dt=torch.float64: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=dev)
+    Yd = t(Y, torch.int32)
+    Xd = t(w.X.T.reshape(-1))
+    off = t(w.offset)
+    G, n, p = Y.shape[0], Y.shape[1], w.p
+    Yf = Y.astype(np.float64)
+    a0 = torch.empty(G, dtype=torch.float64, device=dev)
+    ctx.check(lib.devil_estimate_dispersion(ctx.h, Yd.data_ptr(), L.COUNTS_I32, G, n, n,
+                                            off.data_ptr(), a0.data_ptr()))
+    assert_rel(a0.cpu().numpy(), orc.estimate_dispersion(Yf, w.offset), 1e-12, 1e-12, "alpha0")
+    b0 = torch.empty(G * p, dtype=torch.float64, device=dev)
+    ctx.check(lib.devil_init_beta(ctx.h, Yd.data_ptr(), L.COUNTS_I32, G, n, n, Xd.data_ptr(), p,
+                                  off.data_ptr(), L.INIT_QR, b0.data_ptr()))
+    b0n = b0.cpu().numpy().reshape(p, G).T
+    assert_rel(b0n, orc.init_beta_qr(Yf, w.X, w.offset), 1e-10, 1e-8, "beta0")
+    beta = b0.clone()
+    it = torch.empty(G, dtype=torch.int32, device=dev)
+    ctx.check(lib.devil_beta_fit(ctx.h, Yd.data_ptr(), L.COUNTS_I32, G, n, n, Xd.data_ptr(), p,
+                                 beta.data_ptr(), off.data_ptr(), a0.data_ptr(), 200, 1e-3,
+                                 L.PREC_FP64, it.data_ptr(), None))
+    bn = beta.cpu().numpy().reshape(p, G).T
+    a0n = a0.cpu().numpy()
+    for g in range(G):
+        rb, ri = orc.beta_fit(Yf[g], w.X, b0n[g], w.offset, a0n[g], 200, 1e-3)
+        assert ri == int(it[g])
+        assert_rel(bn[g], rb, 1e-8, 1e-6, f"beta gene {g}")
+    th = torch.empty(G, dtype=torch.float64, device=dev)
+    ctx.check(lib.devil_mom_dispersion(ctx.h, Yd.data_ptr(), L.COUNTS_I32, G, n, n,
+                                       Xd.data_ptr(), p, beta.data_ptr(), off.data_ptr(),
+                                       th.data_ptr()))
+    thn = th.cpu().numpy()
+    ref_th = [orc.mom_dispersion(Yf[g], w.X, bn[g], w.sf) for g in range(G)]
+    assert_rel(thn, ref_th, 1e-9, 1e-9, "theta MOM")
+    thm = torch.empty(G, dtype=torch.float64, device=dev)
+    itm = torch.empty(G, dtype=torch.int32, device=dev)
+    ctx.check(lib.devil_fit_dispersion_mle(ctx.h, Yd.data_ptr(), L.COUNTS_I32, G, n, n,
+                                           Xd.data_ptr(), p, beta.data_ptr(), off.data_ptr(),
+                                           200, thm.data_ptr(), itm.data_ptr()))
+    ref_mle = [orc.fit_dispersion(bn[g], w.X, Yf[g], w.offset)[0] for g in range(G)]
+    assert_rel(thm.cpu().numpy(), ref_mle, 1e-8, 1e-8, "theta MLE")
+    ends = t(np.array([1000, 2000, 3000], np.int32), torch.int32)
+    br = torch.empty(G * p * p, dtype=torch.float64, device=dev)
+    sw = torch.empty(G * p * p, dtype=torch.float64, device=dev)
+    st = torch.zeros(G, dtype=torch.int32, device=dev)
+    ctx.check(lib.devil_bread_meat(ctx.h, Yd.data_ptr(), L.COUNTS_I32, G, n, n, Xd.data_ptr(),
+                                   p, beta.data_ptr(), th.data_ptr(), off.data_ptr(),
+                                   ends.data_ptr(), 3, br.data_ptr(), sw.data_ptr(),
+                                   st.data_ptr()))
+    brn = br.cpu().numpy().reshape(G, p, p).transpose(0, 2, 1)
+    swn = sw.cpu().numpy().reshape(G, p, p).transpose(0, 2, 1)
+    for g in range(G):
+        rb = orc.compute_hessian(bn[g], thn[g], Yf[g], w.X, w.sf)
+        assert_rel(brn[g], rb, 1e-9, 1e-12, "bread")
+        m = orc.clustered_meat(w.X, Yf[g], bn[g], thn[g], w.sf, [1000, 2000, 3000])
+        assert_rel(swn[g], rb @ m @ rb * n, 1e-8, 1e-12, "sandwich")
+
+
+def test_abi_errors_on_gpu(dv):
+    """Argument errors come back as codes, never as a crash."""
+    import ctypes as C
+    from devil_amd import _lib as L
+    ctx = dv.context(0)
+    lib = dv.lib()
+    rc = lib.devil_estimate_dispersion(ctx.h, None, 0, 5, 10, 10, None, None)
+    assert rc == 1
+    prob = L.Problem(G=2, n=10, p=11, mem=0)
+    outs = L.Outputs()
+    assert lib.devil_fit(ctx.h, C.byref(prob), C.byref(outs)) == 1
+    with pytest.raises(ValueError):
+        dv.cluster_blocks([1, 1, 2, 1], 4)
+    X = np.column_stack([np.ones(8), np.ones(8)])
+    with pytest.raises(dv.DevilError) as ei:
+        dv.fit_devil(np.ones((3, 8), np.int32), X)
+    assert ei.value.code == 4
