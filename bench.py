@@ -33,9 +33,9 @@ def log(msg):
 
 
 def cpu_baseline(w, Yd, fit, n_threads):
-    """The FP64 oracle on every ceil(G/512)-th gene, timed on the host."""
+    """The FP64 oracle on every ceil(G/2048)-th gene, timed on the host."""
     import oracle
-    stride = math.ceil(w.G / 512)
+    stride = math.ceil(w.G / 2048)
     genes = np.arange(0, w.G, stride)
     Ys = Yd[genes].cpu().numpy().astype(np.float64)
     od = {"MOM": oracle.OD_MOM, "MLE": oracle.OD_MLE}[w.od]

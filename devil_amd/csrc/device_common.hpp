@@ -214,6 +214,82 @@ __device__ inline double trigamma(double x) {
          f * (5.0 / 66 - f * (691.0 / 2730 - f * (7.0 / 6)))))));
 }
 
+// psi(a + y) - psi(a) for a > 0, y >= 0, without the cancellation of the
+// direct difference: a finite sum for small integer y, otherwise a shift of a
+// to >= 10 with exact terms y/(a(a+y)) and the term-by-term difference of
+// the asymptotic series,
+//   log1p(y/a) + y/(2a(a+y)) + sum_k c_k a^-2k (1 - (1+y/a)^-2k).
+__device__ inline double dpsi_diff(double a, double y) {
+  if (y == 0.0) return 0.0;
+  if (y <= 16.0 && y == floor(y)) {
+    double s = 0.0;
+    for (int j = 0; j < (int)y; ++j) s += 1.0 / (a + j);
+    return s;
+  }
+  double s = 0.0;
+  while (a < 10.0) {
+    s += y / (a * (a + y));
+    a += 1.0;
+  }
+  const double L = log1p(y / a);
+  const double ia2 = 1.0 / (a * a);
+  const double q = exp(-2.0 * L);
+  const double c[7] = {1.0 / 12, -1.0 / 120, 1.0 / 252, -1.0 / 240, 1.0 / 132,
+                       -691.0 / 32760, 1.0 / 12};
+  double corr = c[0] * ia2 * (-expm1(-2.0 * L));
+  double ak = ia2, qk = q;
+#pragma unroll
+  for (int k = 1; k < 7; ++k) {
+    ak *= ia2;
+    qk *= q;
+    corr += c[k] * ak * (1.0 - qk);
+  }
+  return s + L + y / (2.0 * a * (a + y)) + corr;
+}
+
+// psi1(a + y) - psi1(a), same construction with
+// psi1(x) ~ 1/x + 1/(2x^2) + sum_k B_2k x^-(2k+1).
+__device__ inline double dpsi1_diff(double a, double y) {
+  if (y == 0.0) return 0.0;
+  if (y <= 16.0 && y == floor(y)) {
+    double s = 0.0;
+    for (int j = 0; j < (int)y; ++j) s -= 1.0 / ((a + j) * (a + j));
+    return s;
+  }
+  double s = 0.0;
+  while (a < 10.0) {
+    s -= y * (2.0 * a + y) / (a * a * (a + y) * (a + y));
+    a += 1.0;
+  }
+  const double L = log1p(y / a);
+  const double ia = 1.0 / a;
+  const double q = exp(-2.0 * L);
+  const double B[7] = {1.0 / 6, -1.0 / 30, 1.0 / 42, -1.0 / 30, 5.0 / 66, -691.0 / 2730,
+                       7.0 / 6};
+  double d = -y / (a * (a + y)) - 0.5 * y * (2.0 * a + y) / (a * a * (a + y) * (a + y));
+  double am = ia * ia * ia, qm = q * exp(-L);
+  d -= B[0] * am * (-expm1(-3.0 * L));
+#pragma unroll
+  for (int k = 1; k < 7; ++k) {
+    am *= ia * ia;
+    qm *= q;
+    d -= B[k] * am * (1.0 - qm);
+  }
+  return s + d;
+}
+
+// f(x) = log(1 + x) - x/(1 + x) for x >= 0; below 1e-2 its alternating series
+// sum_{k>=2} (-1)^k (k-1)/k x^k (Horner, 11 terms).
+__device__ inline double nb_f(double x) {
+  if (x < 1e-2) {
+    double s = 11.0 / 12;
+#pragma unroll
+    for (int k = 11; k >= 2; --k) s = (double)(k - 1) / k - x * s;
+    return x * x * s;
+  }
+  return log1p(x) - x / (1.0 + x);
+}
+
 // Stirling remainder of lgamma for x >= 10.
 __device__ inline double lgamma_corr(double x) {
   const double f = 1.0 / (x * x);

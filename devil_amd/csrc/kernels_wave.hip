@@ -350,7 +350,12 @@ __global__ __launch_bounds__(64) void k_bread_meat(
 // ---------------------------------------------------------------------------
 // MLE overdispersion with Cox-Reid adjustment.
 // Score and its derivative in log(theta): src/overdispersion.cpp:95-178 and
-// :186-242 (sums over cells; mathematically the frequency-table sums).
+// :186-242, evaluated without their cancellations (the same quantities):
+//   sum psi(y + 1/theta) - n psi(1/theta)   ->  sum_i dpsi_diff(1/theta, y_i)
+//   sum psi1(y + 1/theta) - n psi1(1/theta) ->  sum_i dpsi1_diff(1/theta, y_i)
+//   log(1 + mu theta) - mu/(mu + 1/theta)    ->  nb_f(mu theta)
+// The max_y * 1e6 < 1/theta shortcut, the min() clamp and the Cox-Reid terms
+// are as in the reference.  Zero counts contribute 0 to the difference sums.
 // Driver: R/overdispersion.R:25-46 special cases and start value, then the
 // safeguarded Newton iteration on score(log theta) = 0:
 //   bracket [log 1e-8, log 1e16]; x' = x - s/d if d < 0, else x +- 1 towards
@@ -363,7 +368,7 @@ struct MlePass {
 template <int P>
 __device__ MlePass mle_pass(const void* row, int dtype, int64_t n, const double* X,
                             const double* b, const double* off, double x, double sum_y,
-                            double sum_prod_y, double max_y, double n0, bool want_d) {
+                            double sum_prod_y, double max_y, bool want_d) {
   constexpr int NT = P * (P + 1) / 2;
   const int lane = threadIdx.x;
   const double theta = exp(x);
@@ -388,25 +393,14 @@ __device__ MlePass mle_pass(const void* row, int dtype, int64_t n, const double*
     if (mu == 0.0) mu = 1e-6;
     const double y = load_count(row, dtype, i);
     const double mt = mu * theta;
-    if (mt < 1e-10) {
-      sc += mt * mt * (1 / (1 + mt) - 0.5);
-    } else if (mt < 1e-4) {
-      const double inv = 1 / (1 + mt);
-      const double ub = mt * mt * inv;
-      const double lb = mt * mt * (inv - 0.5);
-      const double sug = (log(1 + mt) - mu / (mu + tn1));
-      sc += fmax(fmin(sug, ub), lb);
-    } else {
-      sc += log(1 + mt) - mu / (mu + tn1);
-    }
-    sc += y / (mu + tn1);
-    if (y != 0.0) dg += digamma(y + tn1);
+    sc += nb_f(mt) + y / (mu + tn1);
+    if (y != 0.0) dg += dpsi_diff(tn1, y);
     const double w = 1.0 / (1.0 / mu + theta);
     const double dw = -1.0 * w * w;
     if (want_d) {
-      if (y != 0.0) tg += trigamma(y + tn1);
-      l1 += log(1 + mu * theta) + (y - mu) / (mu + tn1);
-      l2 += (mu * mu * theta + y) / (1 + mu * theta) / (1 + mu * theta);
+      if (y != 0.0) tg += dpsi1_diff(tn1, y);
+      l1 += log1p(mt) + (y - mu) / (mu + tn1);
+      l2 += (mu * mu * theta + y) / (1 + mt) / (1 + mt);
     }
     const double d2w = -2 * dw * w;
 #pragma unroll
@@ -422,8 +416,6 @@ __device__ MlePass mle_pass(const void* row, int dtype, int64_t n, const double*
   }
   sc = wave_sum(sc);
   dg = wave_sum(dg);
-  const double dpsi = digamma(tn1);
-  dg += n0 * dpsi;  // the zero counts all contribute digamma(0 + 1/theta)
   double Bm[P][P], Dm[P][P], D2m[P][P], Bi[P][P];
 #pragma unroll
   for (int a = 0; a < P; ++a)
@@ -443,20 +435,18 @@ __device__ MlePass mle_pass(const void* row, int dtype, int64_t n, const double*
 #pragma unroll
     for (int bb = 0; bb < P; ++bb) tr_bdb += Bi[a][bb] * Dm[bb][a];
   const double cr_s = -0.5 * tr_bdb * 0.99;
-  const double nd = (double)n;
   const double corr = tn1 > 1e5 ? sum_prod_y / (2 * tn1) : 0.0;
   double dterm;
   if (max_y * 1e6 < tn1) {
     dterm = sum_y - corr;
   } else {
-    dterm = (dg - nd * dpsi) * tn1;
-    dterm = fmin(dterm, sum_y - corr);
+    dterm = fmin(dg * tn1, sum_y - corr);
   }
   MlePass r;
   r.score = sc * tn1 - dterm + cr_s * theta;
   r.dscore = 0.0;
   if (want_d) {
-    tg = wave_sum(tg) + n0 * trigamma(tn1);
+    tg = wave_sum(tg);
     l1 = wave_sum(l1);
     l2 = wave_sum(l2);
 #pragma unroll
@@ -491,10 +481,9 @@ __device__ MlePass mle_pass(const void* row, int dtype, int64_t n, const double*
     const double cr_t = (0.5 * ddetb * ddetb - 0.5 * d2detb) * 0.99;
     const double cr_t2 = -0.5 * ddetb * 0.99;
     const double tn2 = 1.0 / (theta * theta);
-    const double dgt = dg - nd * dpsi;
-    const double tgt = tn2 * tg - tn2 * nd * trigamma(tn1);
-    const double ll = -2 * tn1 * (l1 - dgt) + (l2 + tgt);
-    r.dscore = ll + cr_t * (theta * theta) + (l1 - dgt) * tn1 + cr_t2 * theta;
+    const double tgt = tn2 * tg;
+    const double ll = -2 * tn1 * (l1 - dg) + (l2 + tgt);
+    r.dscore = ll + cr_t * (theta * theta) + (l1 - dg) * tn1 + cr_t2 * theta;
   }
   return r;
 }
@@ -541,8 +530,7 @@ __global__ __launch_bounds__(64) void k_mle(Counts c, int64_t G, int64_t n,
     v = wave_sum(v) / (double)(n - 1);
     double start = (v - ybar) / (ybar * ybar);
     if (isnan(start) || start <= 0) start = 0.5;
-    const MlePass fl = mle_pass<P>(row, c.dtype, n, X, b, off, log(1e-8), s1, spy, my, nz,
-                                   false);
+    const MlePass fl = mle_pass<P>(row, c.dtype, n, X, b, off, log(1e-8), s1, spy, my, false);
     it = 1;
     if (fl.score < 0) {
       st = DEVIL_GENE_MLE_ZERO;
@@ -551,7 +539,7 @@ __global__ __launch_bounds__(64) void k_mle(Counts c, int64_t G, int64_t n,
       double x = log(start);
       x = fmin(fmax(x, lo), hi);
       for (int k = 0; k < max_iter; ++k) {
-        const MlePass r = mle_pass<P>(row, c.dtype, n, X, b, off, x, s1, spy, my, nz, true);
+        const MlePass r = mle_pass<P>(row, c.dtype, n, X, b, off, x, s1, spy, my, true);
         ++it;
         const double s = r.score, d = r.dscore;
         if (!isfinite(s) || !isfinite(d)) break;

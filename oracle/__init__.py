@@ -76,6 +76,9 @@ def lib():
             "orc_nb_loglik": (d, [_dp, _dp, i64, d, _dp, C.c_int, C.c_int, _dp, _dp, i64]),
             "orc_nb_score": (d, [_dp, _dp, i64, d, _dp, C.c_int, C.c_int, _dp, _dp, i64]),
             "orc_nb_dscore": (d, [_dp, _dp, i64, d, _dp, C.c_int, C.c_int, _dp, _dp, i64]),
+            "orc_dpsi_diff": (d, [d, d]), "orc_dpsi1_diff": (d, [d, d]),
+            "orc_nb_score_acc": (d, [_dp, _dp, i64, d, _dp, C.c_int, C.c_int]),
+            "orc_nb_dscore_acc": (d, [_dp, _dp, i64, d, _dp, C.c_int, C.c_int]),
             "orc_fit_dispersion": (d, [_dp, _dp, _dp, _dp, i64, C.c_int, d, C.c_int,
                                        C.c_int, C.POINTER(C.c_int)]),
             "orc_estimate_dispersion": (None, [_dp, i64, i64, i64, _dp, _dp]),
@@ -212,6 +215,28 @@ def nb_score(y, mu, log_theta, X, cr=True, table=None):
 
 def nb_dscore(y, mu, log_theta, X, cr=True, table=None):
     return _nbfun("orc_nb_dscore", y, mu, log_theta, X, cr, table)
+
+
+def dpsi_diff(a, y):
+    return lib().orc_dpsi_diff(float(a), float(y))
+
+
+def dpsi1_diff(a, y):
+    return lib().orc_dpsi1_diff(float(a), float(y))
+
+
+def nb_score_acc(y, mu, log_theta, X, cr=True):
+    y, mu = _f64(y), _f64(mu)
+    X = np.asfortranarray(X, dtype=np.float64)
+    return lib().orc_nb_score_acc(_d(y), _d(mu), y.size, float(log_theta),
+                                  X.ctypes.data_as(_dp), X.shape[1], int(bool(cr)))
+
+
+def nb_dscore_acc(y, mu, log_theta, X, cr=True):
+    y, mu = _f64(y), _f64(mu)
+    X = np.asfortranarray(X, dtype=np.float64)
+    return lib().orc_nb_dscore_acc(_d(y), _d(mu), y.size, float(log_theta),
+                                   X.ctypes.data_as(_dp), X.shape[1], int(bool(cr)))
 
 
 def fit_dispersion(beta, X, y, off, tol=1e-3, max_iter=200, cr=True):

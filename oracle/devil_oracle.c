@@ -35,6 +35,10 @@
  *   NB score / dscore       central finite differences of orc_nb_loglik
  *                           (score = d ll / d log theta, dscore = d score /
  *                           d log theta), table branch == per-cell branch
+ *   dpsi / dpsi1 and the    mpmath differences psi(a+y)-psi(a) at 50 digits;
+ *   *_acc score variants    the accurate score equals the reference one where
+ *                           that is well conditioned and equals a 50-digit
+ *                           mpmath evaluation where the reference's cancels
  *   estimate_dispersion     R/test-overdispersion.R:1-45 formula, 0.01
  *                           truncation, all-zero gene
  *   init_beta_qr            normal equations at 1e-10 (test-beta.R:41-77),
@@ -95,6 +99,69 @@ double orc_trigamma(double x) {
              t * f * (1.0 / 6 - f * (1.0 / 30 - f * (1.0 / 42 - f * (1.0 / 30 -
              f * (5.0 / 66 - f * (691.0 / 2730 - f * (7.0 / 6)))))));
   return r + s;
+}
+
+/* psi(a + y) - psi(a) without cancellation (a > 0, y >= 0):
+ *   small integer y: sum_{j<y} 1/(a+j)  (psi(x+1) = psi(x) + 1/x)
+ *   otherwise shift a to >= 10 with the exact terms y/(a(a+y)), then take the
+ *   difference of the asymptotic series term by term:
+ *   log1p(y/a) + y/(2a(a+y)) + sum_k c_k a^-2k (1 - (1+y/a)^-2k). */
+double orc_dpsi_diff(double a, double y) {
+  if (y == 0.0) return 0.0;
+  if (y <= 16.0 && y == floor(y)) {
+    double s = 0.0;
+    for (int j = 0; j < (int)y; ++j) s += 1.0 / (a + j);
+    return s;
+  }
+  double s = 0.0;
+  while (a < 10.0) {
+    s += y / (a * (a + y));
+    a += 1.0;
+  }
+  const double L = log1p(y / a);
+  const double ia2 = 1.0 / (a * a);
+  const double q = exp(-2.0 * L); /* (1+t)^-2 */
+  static const double c[7] = {1.0 / 12, -1.0 / 120, 1.0 / 252, -1.0 / 240, 1.0 / 132,
+                              -691.0 / 32760, 1.0 / 12};
+  double corr = c[0] * ia2 * (-expm1(-2.0 * L));
+  double ak = ia2, qk = q;
+  for (int k = 1; k < 7; ++k) {
+    ak *= ia2;
+    qk *= q;
+    corr += c[k] * ak * (1.0 - qk);
+  }
+  return s + L + y / (2.0 * a * (a + y)) + corr;
+}
+
+/* psi1(a + y) - psi1(a) without cancellation, same construction with
+ * psi1(x) ~ 1/x + 1/(2x^2) + sum_k B_2k x^-(2k+1). */
+double orc_dpsi1_diff(double a, double y) {
+  if (y == 0.0) return 0.0;
+  if (y <= 16.0 && y == floor(y)) {
+    double s = 0.0;
+    for (int j = 0; j < (int)y; ++j) s -= 1.0 / ((a + j) * (a + j));
+    return s;
+  }
+  double s = 0.0;
+  while (a < 10.0) {
+    s -= y * (2.0 * a + y) / (a * a * (a + y) * (a + y));
+    a += 1.0;
+  }
+  const double L = log1p(y / a);
+  const double ia = 1.0 / a;
+  const double q = exp(-2.0 * L);
+  static const double B[7] = {1.0 / 6, -1.0 / 30, 1.0 / 42, -1.0 / 30, 5.0 / 66,
+                              -691.0 / 2730, 7.0 / 6};
+  double d = -y / (a * (a + y)) - 0.5 * y * (2.0 * a + y) / (a * a * (a + y) * (a + y));
+  /* a^-(2k+1) - (a+y)^-(2k+1) = a^-(2k+1) (1 - (1+t)^-(2k+1)) */
+  double am = ia * ia * ia, qm = q * exp(-L);
+  d -= B[0] * am * (-expm1(-3.0 * L));
+  for (int k = 1; k < 7; ++k) {
+    am *= ia * ia;
+    qm *= q;
+    d -= B[k] * am * (1.0 - qm);
+  }
+  return s + d;
 }
 
 /* Stirling remainder lgamma(x) - [(x-1/2) ln x - x + ln(2 pi)/2], x >= 10 */
@@ -663,6 +730,116 @@ double orc_nb_dscore(const double *y, const double *mu, int64_t n,
          cr_term2 * theta;
 }
 
+/* f(x) = log(1 + x) - x / (1 + x), the per-cell term of the score's
+ * likelihood part (src/overdispersion.cpp:159-173 approximates it by Taylor
+ * branches and bounds); series sum_{k>=2} (-1)^k (k-1)/k x^k below 1e-2. */
+static double nb_f(double x) {
+  if (x < 1e-2) {
+    double s = 11.0 / 12;
+    for (int k = 11; k >= 2; --k) s = (double)(k - 1) / k - x * s;
+    return x * x * s;
+  }
+  return log1p(x) - x / (1.0 + x);
+}
+
+/* The score of src/overdispersion.cpp:95-178 evaluated without its two
+ * cancellations: the digamma term (sum psi(y + 1/theta) - n psi(1/theta)) /
+ * theta as sum_i dpsi(1/theta, y_i) / theta, and the per-cell
+ * log(1 + mu theta) - mu / (mu + 1/theta) as nb_f(mu theta).  Where 1/theta
+ * >> y the reference's sums are dominated by rounding; these are the same
+ * quantities to working precision.  The shortcut, the min() clamp and the
+ * Cox-Reid part are as in orc_nb_score. */
+double orc_nb_score_acc(const double *y, const double *mu, int64_t n, double log_theta,
+                        const double *X, int p, int cr) {
+  double theta = exp(log_theta);
+  double tn1 = 1.0 / theta;
+  double cr_term = 0.0;
+  if (cr) {
+    double *w = (double *)malloc(sizeof(double) * (size_t)n * 2);
+    double *dw = w + n;
+    for (int64_t i = 0; i < n; ++i) {
+      w[i] = 1.0 / (1.0 / mu[i] + theta);
+      dw[i] = -1.0 * w[i] * w[i];
+    }
+    double b[PMAX * PMAX], db[PMAX * PMAX], binv[PMAX * PMAX];
+    gram(X, w, n, p, b);
+    gram(X, dw, n, p, db);
+    free(w);
+    for (int a = 0; a < p; ++a) b[a + a * p] += 1e-6;
+    inv_sympd(b, p, binv);
+    cr_term = -0.5 * trace_prod(binv, db, p) * cr_correction_factor;
+  }
+  double dd = 0.0, max_y = 0.0, sum_y = 0.0, sum_prod_y = 0.0;
+  for (int64_t i = 0; i < n; ++i) {
+    dd += orc_dpsi_diff(tn1, y[i]);
+    sum_y += y[i];
+    sum_prod_y += (y[i] - 1) * y[i];
+    max_y = fmax(max_y, y[i]);
+  }
+  double corr = tn1 > 1e5 ? sum_prod_y / (2 * tn1) : 0.0;
+  double digamma_term;
+  if (max_y * 1e6 < tn1) {
+    digamma_term = sum_y - corr;
+  } else {
+    digamma_term = fmin(dd * tn1, sum_y - corr);
+  }
+  double ll_part = 0.0;
+  for (int64_t i = 0; i < n; ++i)
+    ll_part += nb_f(mu[i] * theta) + y[i] / (mu[i] + tn1);
+  ll_part *= tn1;
+  return ll_part - digamma_term + cr_term * theta;
+}
+
+/* src/overdispersion.cpp:186-242 with the digamma and trigamma sums taken as
+ * sum_i dpsi(1/theta, y_i) and sum_i dpsi1(1/theta, y_i). */
+double orc_nb_dscore_acc(const double *y, const double *mu, int64_t n, double log_theta,
+                         const double *X, int p, int cr) {
+  double theta = exp(log_theta);
+  double cr_term = 0.0, cr_term2 = 0.0;
+  if (cr) {
+    double *w = (double *)malloc(sizeof(double) * (size_t)n * 3);
+    double *dw = w + n, *d2w = w + 2 * n;
+    for (int64_t i = 0; i < n; ++i) {
+      w[i] = 1 / (1 / mu[i] + theta);
+      dw[i] = -1 * w[i] * w[i];
+      d2w[i] = -2 * dw[i] * w[i];
+    }
+    double b[PMAX * PMAX], db[PMAX * PMAX], d2b[PMAX * PMAX], binv[PMAX * PMAX];
+    double didb[PMAX * PMAX], sq[PMAX * PMAX];
+    gram(X, w, n, p, b);
+    gram(X, dw, n, p, db);
+    gram(X, d2w, n, p, d2b);
+    free(w);
+    for (int a = 0; a < p; ++a) b[a + a * p] += 1e-6;
+    inv_sympd(b, p, binv);
+    matmul(binv, db, p, didb);
+    double ddetb = 0.0;
+    for (int a = 0; a < p; ++a) ddetb += didb[a + a * p];
+    matmul(didb, didb, p, sq);
+    double tr_sq = 0.0;
+    for (int a = 0; a < p; ++a) tr_sq += sq[a + a * p];
+    double d2detb = (ddetb * ddetb - tr_sq + trace_prod(binv, d2b, p));
+    cr_term = (0.5 * ddetb * ddetb - 0.5 * d2detb) * cr_correction_factor;
+    cr_term2 = -0.5 * ddetb * cr_correction_factor;
+  }
+  double tn1 = 1.0 / theta;
+  double tn2 = 1.0 / (theta * theta);
+  double digamma_term = 0.0, s1 = 0.0;
+  for (int64_t i = 0; i < n; ++i) {
+    digamma_term += orc_dpsi_diff(tn1, y[i]);
+    s1 += orc_dpsi1_diff(tn1, y[i]);
+  }
+  double trigamma_term = tn2 * s1;
+  double l1 = 0.0, l2 = 0.0;
+  for (int64_t i = 0; i < n; ++i) {
+    l1 += log1p(mu[i] * theta) + (y[i] - mu[i]) / (mu[i] + tn1);
+    l2 += (mu[i] * mu[i] * theta + y[i]) / (1 + mu[i] * theta) / (1 + mu[i] * theta);
+  }
+  double ll_part = -2 * tn1 * (l1 - digamma_term) + (l2 + trigamma_term);
+  return ll_part + cr_term * (theta * theta) + (l1 - digamma_term) * tn1 +
+         cr_term2 * theta;
+}
+
 /* R/overdispersion.R:25-115.  Special cases and start value follow the
  * reference; the optimiser is the safeguarded Newton iteration on
  * score(log theta) = 0 that the HIP library runs:
@@ -675,17 +852,10 @@ double orc_fit_dispersion(const double *beta, const double *X, const double *y,
                           int max_iter, int cr, int *iters) {
   (void)tol;
   *iters = 0;
-  double *keys = (double *)malloc(sizeof(double) * (size_t)(n + 1) * 2);
-  double *freqs = keys + n + 1;
-  int64_t nu = orc_make_table(y, n, (int64_t)((double)n / 2.0), keys, freqs);
-  if (nu < 0) nu = 0;
   int allzero = 1;
   for (int64_t i = 0; i < n; ++i)
     if (y[i] != 0) { allzero = 0; break; }
-  if (allzero) {
-    free(keys);
-    return 0.0;
-  }
+  if (allzero) return 0.0;
   double *mu = (double *)malloc(sizeof(double) * (size_t)n);
   double ybar = 0.0;
   for (int64_t i = 0; i < n; ++i) {
@@ -701,12 +871,10 @@ double orc_fit_dispersion(const double *beta, const double *X, const double *y,
   v /= (double)(n - 1);
   double start = (v - ybar) / (ybar * ybar);
   if (isnan(start) || start <= 0) start = 0.5;
-  const double *uq = nu > 0 ? keys : NULL, *fq = nu > 0 ? freqs : NULL;
-  double far_left = orc_nb_score(y, mu, n, log(1e-8), X, p, cr, uq, fq, nu);
+  double far_left = orc_nb_score_acc(y, mu, n, log(1e-8), X, p, cr);
   *iters = 1;
   if (far_left < 0) {
     free(mu);
-    free(keys);
     return 0.0;
   }
   double lo = log(1e-8), hi = log(1e16);
@@ -714,8 +882,8 @@ double orc_fit_dispersion(const double *beta, const double *X, const double *y,
   if (x < lo) x = lo;
   if (x > hi) x = hi;
   for (int it = 0; it < max_iter; ++it) {
-    double s = orc_nb_score(y, mu, n, x, X, p, cr, uq, fq, nu);
-    double d = orc_nb_dscore(y, mu, n, x, X, p, cr, uq, fq, nu);
+    double s = orc_nb_score_acc(y, mu, n, x, X, p, cr);
+    double d = orc_nb_dscore_acc(y, mu, n, x, X, p, cr);
     (*iters)++;
     if (!isfinite(s) || !isfinite(d)) break;
     if (s == 0.0) break;
@@ -728,7 +896,6 @@ double orc_fit_dispersion(const double *beta, const double *X, const double *y,
     if (fabs(step) < 1e-9) break;
   }
   free(mu);
-  free(keys);
   return exp(x);
 }
 

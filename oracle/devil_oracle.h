@@ -90,6 +90,16 @@ double orc_nb_dscore(const double *y, const double *mu, int64_t n,
                      double log_theta, const double *X, int p, int cr,
                      const double *uniq, const double *freq, int64_t nu);
 
+/* The same score and derivative with the digamma / trigamma sums evaluated as
+ * sums of psi(1/theta + y_i) - psi(1/theta) (no cancellation); used by the
+ * Newton driver below. */
+double orc_dpsi_diff(double a, double y);
+double orc_dpsi1_diff(double a, double y);
+double orc_nb_score_acc(const double *y, const double *mu, int64_t n, double log_theta,
+                        const double *X, int p, int cr);
+double orc_nb_dscore_acc(const double *y, const double *mu, int64_t n, double log_theta,
+                         const double *X, int p, int cr);
+
 /* R/overdispersion.R:25-115 fit_dispersion: the special cases and start
  * value of the reference, then the safeguarded Newton root of the score in
  * log(theta) that the HIP library also runs (the nlminb trajectory itself is

@@ -205,7 +205,8 @@ def test_stage_entry_points(dv, orc):
     a0 = torch.empty(G, dtype=torch.float64, device=dev)
     ctx.check(lib.devil_estimate_dispersion(ctx.h, Yd.data_ptr(), L.COUNTS_I32, G, n, n,
                                             off.data_ptr(), a0.data_ptr()))
-    assert_rel(a0.cpu().numpy(), orc.estimate_dispersion(Yf, w.offset), 1e-12, 1e-12, "alpha0")
+    # (v - ybar/mean(sf)) / ybar^2 cancels; alpha0 is used rounded to float
+    assert_rel(a0.cpu().numpy(), orc.estimate_dispersion(Yf, w.offset), 1e-10, 1e-12, "alpha0")
     b0 = torch.empty(G * p, dtype=torch.float64, device=dev)
     ctx.check(lib.devil_init_beta(ctx.h, Yd.data_ptr(), L.COUNTS_I32, G, n, n, Xd.data_ptr(), p,
                                   off.data_ptr(), L.INIT_QR, b0.data_ptr()))

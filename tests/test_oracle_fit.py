@@ -270,6 +270,71 @@ def test_nb_score_is_loglik_derivative(orc):
             assert abs(orc.nb_loglik(y, mu, lt, X, cr, table) - f(lt)) <= 1e-10 * abs(f(lt))
 
 
+def test_dpsi_differences_mpmath(orc):
+    import mpmath as mp
+    mp.mp.dps = 50
+    for a in [1e-8, 0.3, 9.9, 10.0, 1e3, 1e7, 1e12]:
+        for y in [1, 3, 16, 17, 250, 2.5, 4e4]:
+            r = mp.digamma(mp.mpf(a) + y) - mp.digamma(mp.mpf(a))
+            r1 = mp.polygamma(1, mp.mpf(a) + y) - mp.polygamma(1, mp.mpf(a))
+            assert abs(orc.dpsi_diff(a, y) - float(r)) <= 2e-14 * abs(float(r))
+            assert abs(orc.dpsi1_diff(a, y) - float(r1)) <= 2e-14 * abs(float(r1))
+    assert orc.dpsi_diff(5.0, 0.0) == 0.0 and orc.dpsi1_diff(5.0, 0.0) == 0.0
+
+
+def nb_score_mpmath(y, mu, log_theta, X):
+    """The Cox-Reid adjusted score of src/overdispersion.cpp:95-178 at 50 digits
+    (no Taylor branches needed at this precision; includes the 1e-6 ridge)."""
+    import mpmath as mp
+    mp.mp.dps = 50
+    th = mp.e ** mp.mpf(log_theta)
+    a = 1 / th
+    n, p = X.shape
+    w = [1 / (1 / mp.mpf(m) + th) for m in mu]
+    b = mp.matrix(p, p)
+    db = mp.matrix(p, p)
+    for i in range(n):
+        for r in range(p):
+            for c in range(p):
+                b[r, c] += X[i, r] * X[i, c] * w[i]
+                db[r, c] += -X[i, r] * X[i, c] * w[i] ** 2
+    for r in range(p):
+        b[r, r] += mp.mpf("1e-6")
+    cr = -mp.mpf("0.5") * mp.mpf("0.99") * sum((b ** -1 * db)[k, k] for k in range(p))
+    dg = sum(mp.digamma(a + yi) - mp.digamma(a) for yi in y) * a
+    sy = sum(y)
+    corr = sum((yi - 1) * yi for yi in y) / (2 * a) if a > 1e5 else 0
+    dg = sy - corr if max(y) * 1e6 < a else min(dg, sy - corr)
+    ll = sum(mp.log(1 + m * th) - m / (m + a) + yi / (m + a) for yi, m in zip(y, mu)) * a
+    return float(ll - dg + cr * th)
+
+
+def test_nb_score_acc(orc):
+    rng = np.random.default_rng(21)
+    n, p = 60, 2
+    X = np.column_stack([np.ones(n), rng.normal(size=n)])
+    # well-conditioned: the accurate form equals the reference form
+    mu = np.exp(X @ np.array([1.0, 0.3]))
+    y = nb_sample(rng, mu, 0.5)
+    for lt in [-2.0, 0.0, 1.0]:
+        s0 = orc.nb_score(y, mu, lt, X, True)
+        s1 = orc.nb_score_acc(y, mu, lt, X, True)
+        assert abs(s0 - s1) <= 1e-9 * max(1, abs(s0))
+        d0 = orc.nb_dscore(y, mu, lt, X, True)
+        d1 = orc.nb_dscore_acc(y, mu, lt, X, True)
+        assert abs(d0 - d1) <= 1e-9 * max(1, abs(d0))
+    # near-Poisson gene, tiny theta: the reference form cancels; the accurate
+    # one matches 50-digit arithmetic
+    mu2 = np.exp(X @ np.array([-3.0, 0.2]))
+    y2 = rng.poisson(mu2).astype(float)
+    y2[0] = 2.0
+    for lt in [-14.0, -13.0, -12.0]:
+        ref = nb_score_mpmath(y2, mu2, lt, X)
+        acc = orc.nb_score_acc(y2, mu2, lt, X, True)
+        # rounding floor of the remaining sum_y - sum_y cancellation: ~1e-14
+        assert abs(acc - ref) <= 1e-8 * abs(ref) + 1e-13, (lt, acc, ref)
+
+
 def test_make_table(orc):
     y = np.array([3, 0, 0, 7, 3, 0, 1.0])
     k, f = orc.make_table(y, 4)
