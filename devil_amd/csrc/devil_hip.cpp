@@ -293,6 +293,142 @@ struct StageTimer {
   }
 };
 
+// Host construction of the MFMA operand layouts (dv::TileDesign).
+struct TileHost {
+  std::vector<double> Xp, offp, Kp;
+  dv::TileDesign td{};
+};
+
+// Columns of K: the distinct x_a x_b (a <= b) and x_a of the design, exact
+// duplicates merged (an intercept makes x_0 x_a == x_a, a 0/1 column squares
+// to itself) and all-zero columns dropped.
+void build_tile_design(const double* X, const double* off, int64_t n, int p, TileHost& th) {
+  std::vector<std::vector<double>> cols;
+  std::vector<uint64_t> hashes;
+  auto add_col = [&](std::vector<double>&& v) -> int {
+    bool zero = true;
+    for (double x : v)
+      if (x != 0.0) {
+        zero = false;
+        break;
+      }
+    if (zero) return -1;
+    uint64_t hsh = 1469598103934665603ULL;
+    for (double x : v) {
+      uint64_t u;
+      std::memcpy(&u, &x, 8);
+      hsh = (hsh ^ u) * 1099511628211ULL;
+    }
+    for (size_t c = 0; c < cols.size(); ++c)
+      if (hashes[c] == hsh && std::memcmp(cols[c].data(), v.data(), sizeof(double) * n) == 0)
+        return (int)c;
+    cols.push_back(std::move(v));
+    hashes.push_back(hsh);
+    return (int)cols.size() - 1;
+  };
+  dv::TileDesign& td = th.td;
+  for (int a = 0; a < p; ++a) td.kidx_lin[a] = add_col(std::vector<double>(X + a * n, X + (a + 1) * n));
+  int t = 0;
+  for (int a = 0; a < p; ++a)
+    for (int b = a; b < p; ++b) {
+      std::vector<double> v(n);
+      for (int64_t i = 0; i < n; ++i) v[i] = X[i + a * n] * X[i + b * n];
+      td.kidx_pair[t++] = add_col(std::move(v));
+    }
+  for (int a = 0; a < p; ++a) {
+    long double s = 0.0L;
+    for (int64_t i = 0; i < n; ++i) s += X[i + a * n];
+    td.sx[a] = (double)s;
+  }
+  const int ncol = std::max<int>(1, (int)cols.size());
+  td.nt = (ncol + 15) / 16;
+  td.ncp = 16 * td.nt;
+  td.n = n;
+  td.nblk = (n + 15) / 16;
+  td.chunk_blocks = std::max<int64_t>(128, (td.nblk + 63) / 64);
+  td.nchunk = (int)((td.nblk + td.chunk_blocks - 1) / td.chunk_blocks);
+  const int NJ = (p + 3) / 4;
+  th.Xp.assign((size_t)td.nblk * NJ * 64, 0.0);
+  th.offp.assign((size_t)td.nblk * 16, 0.0);
+  th.Kp.assign((size_t)td.nblk * td.nt * 256, 0.0);
+  for (int64_t i = 0; i < n; ++i) th.offp[i] = off[i];
+  for (int64_t b = 0; b < td.nblk; ++b) {
+    // eta-product A operand: row m of the block is cell 4 (m % 4) + m / 4
+    for (int j = 0; j < NJ; ++j)
+      for (int l = 0; l < 64; ++l) {
+        const int m = l % 16;
+        const int64_t cell = 16 * b + 4 * (m % 4) + m / 4;
+        const int f = 4 * j + l / 16;
+        if (cell < n && f < p) th.Xp[(b * NJ + j) * 64 + l] = X[cell + f * n];
+      }
+    // contraction B operand: lane l, step r -> cell 4 (l / 16) + r, column l % 16
+    for (int tt = 0; tt < td.nt; ++tt)
+      for (int l = 0; l < 64; ++l)
+        for (int r = 0; r < 4; ++r) {
+          const int64_t cell = 16 * b + 4 * (l / 16) + r;
+          const int col = 16 * tt + l % 16;
+          if (cell < n && col < (int)cols.size())
+            th.Kp[((b * td.nt + tt) * 64 + l) * 4 + r] = cols[col][cell];
+        }
+  }
+}
+
+struct DeviceTile {
+  DevBuf Xp, offp, Kp;
+};
+
+int upload_tile(devil_ctx* ctx, TileHost& th, DeviceTile& dt) {
+  int rc;
+  if ((rc = upload(ctx, dt.Xp, th.Xp))) return rc;
+  if ((rc = upload(ctx, dt.offp, th.offp))) return rc;
+  if ((rc = upload(ctx, dt.Kp, th.Kp))) return rc;
+  th.td.Xp = dt.Xp.as<double>();
+  th.td.offp = dt.offp.as<double>();
+  th.td.Kp = dt.Kp.as<double>();
+  return DEVIL_OK;
+}
+
+// IRLS over genes [0, G) of the counts view: rounds of accumulate + update
+// over the still-active genes until none is left (src/beta.cpp:37).
+// beta, k, iters, status are gene-major device arrays for these G genes.
+int run_irls_tiled(devil_ctx* ctx, const dv::TileDesign& td, dv::Counts c, int64_t G, int p,
+                   double* beta, const double* kvec, int max_iter, double eps,
+                   int32_t* iters, int32_t* status) {
+  hipStream_t s = ctx->stream;
+  DV_HIP(dv::launch_fill_i32(s, iters, G, 0));
+  DV_HIP(dv::launch_fill_i32(s, status, G, 0));
+  if (max_iter <= 0 || G == 0) {
+    if (max_iter <= 0) DV_HIP(dv::launch_fill_i32(s, status, G, DEVIL_GENE_MAXITER));
+    return DEVIL_OK;
+  }
+  DevBuf act0, act1, cnt, part;
+  DV_ALLOC(act0, sizeof(int32_t) * G);
+  DV_ALLOC(act1, sizeof(int32_t) * G);
+  DV_ALLOC(cnt, sizeof(int32_t) * 2);
+  const int64_t ntile = (G + 15) / 16;
+  DV_ALLOC(part, sizeof(double) * (size_t)ntile * td.nchunk * 16 * td.ncp);
+  DV_HIP(dv::launch_iota(s, act0.as<int32_t>(), (int)G));
+  int32_t* cur = act0.as<int32_t>();
+  int32_t* nxt = act1.as<int32_t>();
+  int32_t* ncur = cnt.as<int32_t>();
+  int32_t* nnxt = cnt.as<int32_t>() + 1;
+  DV_HIP(dv::launch_fill_i32(s, ncur, 1, (int32_t)G));
+  int nA = (int)G;
+  for (int it = 0; it < max_iter && nA > 0; ++it) {
+    DV_HIP(dv::launch_irls_accum(s, td, c, p, cur, ncur, nA, beta, kvec, part.as<double>()));
+    DV_HIP(dv::launch_fill_i32(s, nnxt, 1, 0));
+    DV_HIP(dv::launch_irls_update(s, td, p, cur, ncur, nA, part.as<double>(), beta, kvec,
+                                  max_iter, eps, iters, status, nxt, nnxt));
+    int32_t h = 0;
+    DV_HIP(hipMemcpyAsync(&h, nnxt, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    DV_HIP(hipStreamSynchronize(s));
+    nA = h;
+    std::swap(cur, nxt);
+    std::swap(ncur, nnxt);
+  }
+  return DEVIL_OK;
+}
+
 int set_device(devil_ctx* ctx) {
   if (!ctx) return DEVIL_E_ARG;
   DV_HIP(hipSetDevice(ctx->device));
@@ -394,6 +530,10 @@ int devil_fit(devil_ctx* ctx, const devil_problem* pr, devil_outputs* out) {
   DeviceDesign dd;
   if ((rc = upload_design(ctx, d, dd))) return rc;
   if (C > 0 && (rc = upload(ctx, dd.ends, ends_h))) return rc;
+  TileHost th;
+  build_tile_design(d.X.data(), d.off.data(), n, p, th);
+  DeviceTile dt;
+  if ((rc = upload_tile(ctx, th, dt))) return rc;
 
   // per-gene device state (gene-major)
   DevBuf alpha0, kvec, beta, theta, iters, thit, status, bread, sandwich;
@@ -463,9 +603,9 @@ int devil_fit(devil_ctx* ctx, const devil_problem* pr, devil_outputs* out) {
     DV_HIP(dv::launch_k_from_alpha(s, a0, Gb, kvec.as<double>() + g0));
     tm.end(ev);
     ev = tm.begin(DEVIL_STAGE_IRLS);
-    DV_HIP(dv::launch_irls_fp64(s, c, Gb, n, p, dd.X.as<double>(), dd.off.as<double>(),
-                                kvec.as<double>() + g0, b0, pr->max_iter, eps,
-                                iters.as<int32_t>() + g0, status.as<int32_t>() + g0));
+    if ((rc = run_irls_tiled(ctx, th.td, c, Gb, p, b0, kvec.as<double>() + g0, pr->max_iter,
+                             eps, iters.as<int32_t>() + g0, status.as<int32_t>() + g0)))
+      return rc;
     tm.end(ev);
     double* th = theta.as<double>() + g0;
     ev = tm.begin(DEVIL_STAGE_DISPERSION);
@@ -592,8 +732,22 @@ int devil_beta_fit(devil_ctx* ctx, const void* counts, int32_t dtype, int64_t G,
   DV_HIP(dv::launch_transpose_gp(s, beta, G, p, 1, bg.as<double>()));
   DV_HIP(dv::launch_k_from_alpha(s, alpha0, G, kv.as<double>()));
   dv::Counts c{counts, dtype, ld};
-  DV_HIP(dv::launch_irls_fp64(s, c, G, n, p, X, offset, kv.as<double>(), bg.as<double>(),
-                              max_iter, (double)(float)tolerance, iters, status));
+  std::vector<double> Xh((size_t)n * p), offh(n);
+  DV_HIP(hipMemcpy(Xh.data(), X, sizeof(double) * n * p, hipMemcpyDeviceToHost));
+  DV_HIP(hipMemcpy(offh.data(), offset, sizeof(double) * n, hipMemcpyDeviceToHost));
+  TileHost th;
+  build_tile_design(Xh.data(), offh.data(), n, p, th);
+  DeviceTile dt;
+  if ((rc = upload_tile(ctx, th, dt))) return rc;
+  DevBuf st_local;
+  int32_t* st = status;
+  if (!st) {
+    DV_ALLOC(st_local, sizeof(int32_t) * G);
+    st = st_local.as<int32_t>();
+  }
+  if ((rc = run_irls_tiled(ctx, th.td, c, G, p, bg.as<double>(), kv.as<double>(), max_iter,
+                           (double)(float)tolerance, iters, st)))
+    return rc;
   DV_HIP(dv::launch_transpose_gp(s, bg.as<double>(), G, p, 0, beta));
   DV_HIP(hipStreamSynchronize(s));
   return DEVIL_OK;

@@ -30,6 +30,34 @@ struct InitParams {
   double sum_sf;            // univariate mode
 };
 
+// Design laid out for the MFMA IRLS kernels (kernels_tile.hip).
+#define DEVIL_NCP_MAX 80  // 16 * ceil((10*11/2 + 10) / 16)
+struct TileDesign {
+  int64_t n;             // cells
+  int64_t nblk;          // 16-cell blocks (n padded to a multiple of 16)
+  int64_t chunk_blocks;  // blocks per chunk
+  int nchunk;            // chunks; partial sums are kept per chunk
+  int nt;                // 16-column tiles of K
+  int ncp;               // 16 * nt
+  const double* Xp;      // [blk][j][lane] = X[16 blk + pi(lane%16)][4 j + lane/16] (0-padded),
+                         // pi(m) = 4 (m % 4) + m / 4
+  const double* offp;    // off padded with zeros to 16 nblk
+  const double* Kp;      // [blk][t][lane][r] = K[16 blk + 4 (lane/16) + r][16 t + lane%16]
+  int kidx_pair[55];     // K column of x_a x_b (packed upper triangle), -1 = zero column
+  int kidx_lin[10];      // K column of x_a, -1 = zero column
+  double sx[10];         // sum_i x_ia
+};
+
+hipError_t launch_irls_accum(hipStream_t s, const TileDesign& td, Counts c, int p,
+                             const int32_t* active, const int32_t* nactive, int nA_bound,
+                             const double* beta, const double* kvec, double* part);
+hipError_t launch_irls_update(hipStream_t s, const TileDesign& td, int p,
+                              const int32_t* active, const int32_t* nactive, int nA_bound,
+                              const double* part, double* beta, const double* kvec,
+                              int max_iter, double eps, int32_t* iters, int32_t* status,
+                              int32_t* next, int32_t* nnext);
+hipError_t launch_iota(hipStream_t s, int32_t* p, int n);
+
 hipError_t launch_init(hipStream_t s, Counts c, int64_t G, int64_t n, int p,
                        const InitParams& ip, double* alpha0, double* beta0);
 

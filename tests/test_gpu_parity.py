@@ -116,18 +116,21 @@ def test_fit_count_dtypes(dv, orc, dtype):
     check_fit(fit, ref)
 
 
-@pytest.mark.parametrize("p", [1, 3, 10])
-def test_fit_design_widths(dv, orc, p):
+@pytest.mark.parametrize("p,n", [(1, 3000), (3, 2999), (10, 3001)])
+def test_fit_design_widths(dv, orc, p, n):
+    """p from 1 to 10; n not a multiple of 4 or 16 exercises the unaligned and
+    partial-block count loads."""
     from devil_amd import synth
     rng = np.random.default_rng(100 + p)
-    G, n = 60, 3000
+    G = 60
     X = np.column_stack([np.ones(n)] + [rng.normal(size=n) for _ in range(p - 1)])
     w = synth.Workload("wide", G, n, p, X, np.exp(rng.normal(0, 0.3, n)),
                        np.column_stack([rng.normal(0, 1, G)] +
                                        [rng.normal(0, 0.1, G) for _ in range(p - 1)]),
                        np.exp(rng.normal(0, 1, G)), seed=99 + p)
     Y = synth.counts_host(w)
-    fit, ref = run_both(dv, orc, w, Y, od="MOM", clusters=np.repeat([1, 2, 3], n // 3))
+    clusters = np.repeat([1, 2, 3], [n // 3, n // 3, n - 2 * (n // 3)])
+    fit, ref = run_both(dv, orc, w, Y, od="MOM", clusters=clusters)
     check_fit(fit, ref)
 
 
