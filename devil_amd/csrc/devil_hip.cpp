@@ -296,13 +296,15 @@ struct StageTimer {
 // Host construction of the MFMA operand layouts (dv::TileDesign).
 struct TileHost {
   std::vector<double> Xp, offp, Kp;
+  std::vector<float> Xf, offf, Kf;  // mixed-precision operands (empty for FP64 only)
   dv::TileDesign td{};
 };
 
 // Columns of K: the distinct x_a x_b (a <= b) and x_a of the design, exact
 // duplicates merged (an intercept makes x_0 x_a == x_a, a 0/1 column squares
 // to itself) and all-zero columns dropped.
-void build_tile_design(const double* X, const double* off, int64_t n, int p, TileHost& th) {
+void build_tile_design(const double* X, const double* off, int64_t n, int p, TileHost& th,
+                       bool mixed = false) {
   std::vector<std::vector<double>> cols;
   std::vector<uint64_t> hashes;
   auto add_col = [&](std::vector<double>&& v) -> int {
@@ -371,10 +373,23 @@ void build_tile_design(const double* X, const double* off, int64_t n, int p, Til
             th.Kp[((b * td.nt + tt) * 64 + l) * 4 + r] = cols[col][cell];
         }
   }
+  if (!mixed) return;
+  // FP32 copies; the eta product rows keep the natural cell order (the f32
+  // MFMA returns rows 4 (l / 16) + r in lane l)
+  th.Xf.assign((size_t)td.nblk * NJ * 64, 0.0f);
+  th.offf.assign(th.offp.begin(), th.offp.end());
+  th.Kf.assign(th.Kp.begin(), th.Kp.end());
+  for (int64_t b = 0; b < td.nblk; ++b)
+    for (int j = 0; j < NJ; ++j)
+      for (int l = 0; l < 64; ++l) {
+        const int64_t cell = 16 * b + l % 16;
+        const int f = 4 * j + l / 16;
+        if (cell < n && f < p) th.Xf[(b * NJ + j) * 64 + l] = (float)X[cell + f * n];
+      }
 }
 
 struct DeviceTile {
-  DevBuf Xp, offp, Kp;
+  DevBuf Xp, offp, Kp, Xf, offf, Kf;
 };
 
 int upload_tile(devil_ctx* ctx, TileHost& th, DeviceTile& dt) {
@@ -385,15 +400,31 @@ int upload_tile(devil_ctx* ctx, TileHost& th, DeviceTile& dt) {
   th.td.Xp = dt.Xp.as<double>();
   th.td.offp = dt.offp.as<double>();
   th.td.Kp = dt.Kp.as<double>();
+  th.td.Xf = th.td.offf = th.td.Kf = nullptr;
+  if (!th.Xf.empty()) {
+    if ((rc = upload(ctx, dt.Xf, th.Xf))) return rc;
+    if ((rc = upload(ctx, dt.offf, th.offf))) return rc;
+    if ((rc = upload(ctx, dt.Kf, th.Kf))) return rc;
+    th.td.Xf = dt.Xf.as<float>();
+    th.td.offf = dt.offf.as<float>();
+    th.td.Kf = dt.Kf.as<float>();
+  }
   return DEVIL_OK;
 }
 
 // IRLS over genes [0, G) of the counts view: rounds of accumulate + update
 // over the still-active genes until none is left (src/beta.cpp:37).
 // beta, k, iters, status are gene-major device arrays for these G genes.
+//
+// Mixed precision: each round accumulates in FP32 (FP64 partials); a gene
+// whose FP32-based step lands within kMixedBand * eps of the stopping
+// threshold (or is NaN) is not updated but recomputed in FP64 in the same
+// round, so stopping decisions and iteration counts follow FP64 arithmetic.
+constexpr double kMixedBand = 1e-3;
+
 int run_irls_tiled(devil_ctx* ctx, const dv::TileDesign& td, dv::Counts c, int64_t G, int p,
                    double* beta, const double* kvec, int max_iter, double eps,
-                   int32_t* iters, int32_t* status) {
+                   int32_t* iters, int32_t* status, bool mixed = false) {
   hipStream_t s = ctx->stream;
   DV_HIP(dv::launch_fill_i32(s, iters, G, 0));
   DV_HIP(dv::launch_fill_i32(s, status, G, 0));
@@ -401,10 +432,12 @@ int run_irls_tiled(devil_ctx* ctx, const dv::TileDesign& td, dv::Counts c, int64
     if (max_iter <= 0) DV_HIP(dv::launch_fill_i32(s, status, G, DEVIL_GENE_MAXITER));
     return DEVIL_OK;
   }
-  DevBuf act0, act1, cnt, part;
+  DevBuf act0, act1, cnt, part, rchk;
   DV_ALLOC(act0, sizeof(int32_t) * G);
   DV_ALLOC(act1, sizeof(int32_t) * G);
-  DV_ALLOC(cnt, sizeof(int32_t) * 2);
+  DV_ALLOC(cnt, sizeof(int32_t) * 3);
+  if (mixed) DV_ALLOC(rchk, sizeof(int32_t) * G);
+  int32_t* nrchk = cnt.as<int32_t>() + 2;
   const int64_t ntile = (G + 15) / 16;
   DV_ALLOC(part, sizeof(double) * (size_t)ntile * td.nchunk * 16 * td.ncp);
   DV_HIP(dv::launch_iota(s, act0.as<int32_t>(), (int)G));
@@ -415,10 +448,90 @@ int run_irls_tiled(devil_ctx* ctx, const dv::TileDesign& td, dv::Counts c, int64
   DV_HIP(dv::launch_fill_i32(s, ncur, 1, (int32_t)G));
   int nA = (int)G;
   for (int it = 0; it < max_iter && nA > 0; ++it) {
-    DV_HIP(dv::launch_irls_accum(s, td, c, p, cur, ncur, nA, beta, kvec, part.as<double>()));
+    int32_t h[2] = {0, 0};
+    if (!mixed) {
+      DV_HIP(dv::launch_irls_accum(s, td, c, p, cur, ncur, nA, beta, kvec, part.as<double>()));
+      DV_HIP(dv::launch_fill_i32(s, nnxt, 1, 0));
+      DV_HIP(dv::launch_irls_update(s, td, p, cur, ncur, nA, part.as<double>(), beta, kvec,
+                                    max_iter, eps, iters, status, nxt, nnxt));
+    } else {
+      DV_HIP(dv::launch_irls_accum_mixed(s, td, c, p, cur, ncur, nA, beta, kvec,
+                                         part.as<double>()));
+      DV_HIP(dv::launch_fill_i32(s, nnxt, 1, 0));
+      DV_HIP(dv::launch_fill_i32(s, nrchk, 1, 0));
+      DV_HIP(dv::launch_irls_update(s, td, p, cur, ncur, nA, part.as<double>(), beta, kvec,
+                                    max_iter, eps, iters, status, nxt, nnxt, kMixedBand,
+                                    rchk.as<int32_t>(), nrchk));
+      DV_HIP(hipMemcpyAsync(&h[1], nrchk, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+      DV_HIP(hipStreamSynchronize(s));
+      if (h[1] > 0) {
+        DV_HIP(dv::launch_irls_accum(s, td, c, p, rchk.as<int32_t>(), nrchk, h[1], beta, kvec,
+                                     part.as<double>()));
+        DV_HIP(dv::launch_irls_update(s, td, p, rchk.as<int32_t>(), nrchk, h[1],
+                                      part.as<double>(), beta, kvec, max_iter, eps, iters,
+                                      status, nxt, nnxt));
+      }
+    }
+    DV_HIP(hipMemcpyAsync(&h[0], nnxt, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    DV_HIP(hipStreamSynchronize(s));
+    nA = h[0];
+    std::swap(cur, nxt);
+    std::swap(ncur, nnxt);
+  }
+  return DEVIL_OK;
+}
+
+// MLE overdispersion over genes [0, G): setup pass, then rounds of
+// accumulate + update over the genes still iterating (the first round is the
+// far-left check of R/overdispersion.R:40-46).  theta, iters are outputs;
+// status is or-ed.
+int run_mle_tiled(devil_ctx* ctx, const dv::TileDesign& td, dv::Counts c, int64_t G, int64_t n,
+                  int p, const double* beta, int max_iter, double* theta, int32_t* iters,
+                  int32_t* status) {
+  hipStream_t s = ctx->stream;
+  if (G == 0) return DEVIL_OK;
+  DevBuf dbl, ints, act0, act1, cnt, part, tab;
+  DV_ALLOC(dbl, sizeof(double) * G * 7);
+  DV_ALLOC(ints, sizeof(int32_t) * G * 2);
+  DV_ALLOC(tab, sizeof(int32_t) * G * dv::kMleTabCap * 2);
+  DV_ALLOC(act0, sizeof(int32_t) * G);
+  DV_ALLOC(act1, sizeof(int32_t) * G);
+  DV_ALLOC(cnt, sizeof(int32_t) * 2);
+  const int W = 5 + 3 * td.ncp;
+  const int64_t ntile = (G + 15) / 16;
+  DV_ALLOC(part, sizeof(double) * (size_t)ntile * td.nchunk * 16 * W);
+  dv::MleState ms;
+  double* d = dbl.as<double>();
+  ms.x = d;
+  ms.lo = d + G;
+  ms.hi = d + 2 * G;
+  ms.start = d + 3 * G;
+  ms.sum_y = d + 4 * G;
+  ms.sum_py = d + 5 * G;
+  ms.max_y = d + 6 * G;
+  ms.mode = ints.as<int32_t>();
+  ms.tab_n = ints.as<int32_t>() + G;
+  ms.tab_u = tab.as<int32_t>();
+  ms.tab_f = tab.as<int32_t>() + G * dv::kMleTabCap;
+  ms.iters = iters;
+  ms.theta = theta;
+  ms.status = status;
+  DV_HIP(dv::launch_mle_setup(s, c, G, n, p, ms));
+  int32_t* cur = act0.as<int32_t>();
+  int32_t* nxt = act1.as<int32_t>();
+  int32_t* ncur = cnt.as<int32_t>();
+  int32_t* nnxt = cnt.as<int32_t>() + 1;
+  DV_HIP(dv::launch_fill_i32(s, ncur, 1, 0));
+  DV_HIP(dv::launch_mle_collect(s, ms.mode, (int)G, cur, ncur));
+  int32_t nA = 0;
+  DV_HIP(hipMemcpyAsync(&nA, ncur, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  DV_HIP(hipStreamSynchronize(s));
+  for (int round = 0; round <= max_iter && nA > 0; ++round) {
+    DV_HIP(dv::launch_mle_accum(s, td, c, p, cur, ncur, nA, beta, ms.x, ms.tab_n,
+                                part.as<double>()));
     DV_HIP(dv::launch_fill_i32(s, nnxt, 1, 0));
-    DV_HIP(dv::launch_irls_update(s, td, p, cur, ncur, nA, part.as<double>(), beta, kvec,
-                                  max_iter, eps, iters, status, nxt, nnxt));
+    DV_HIP(dv::launch_mle_update(s, td, p, cur, ncur, nA, part.as<double>(), ms, n, max_iter,
+                                 nxt, nnxt));
     int32_t h = 0;
     DV_HIP(hipMemcpyAsync(&h, nnxt, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     DV_HIP(hipStreamSynchronize(s));
@@ -504,8 +617,6 @@ int devil_fit(devil_ctx* ctx, const devil_problem* pr, devil_outputs* out) {
   if (pr->max_iter < 0) return fail(ctx, DEVIL_E_ARG, "max_iter < 0");
   if (pr->precision != DEVIL_PREC_FP64 && pr->precision != DEVIL_PREC_MIXED)
     return fail(ctx, DEVIL_E_ARG, "unknown precision");
-  if (pr->precision == DEVIL_PREC_MIXED)
-    return fail(ctx, DEVIL_E_UNSUPPORTED, "mixed precision path not built yet");
   if (pr->n_clusters < 0 || (pr->n_clusters > 0 && !pr->cluster_ends))
     return fail(ctx, DEVIL_E_ARG, "n_clusters > 0 needs cluster_ends");
   if (G > 0 && (!out->beta || !out->theta || !out->beta_iters || !out->theta_iters ||
@@ -530,10 +641,11 @@ int devil_fit(devil_ctx* ctx, const devil_problem* pr, devil_outputs* out) {
   DeviceDesign dd;
   if ((rc = upload_design(ctx, d, dd))) return rc;
   if (C > 0 && (rc = upload(ctx, dd.ends, ends_h))) return rc;
-  TileHost th;
-  build_tile_design(d.X.data(), d.off.data(), n, p, th);
+  TileHost tl;
+  const bool mixed = pr->precision == DEVIL_PREC_MIXED;
+  build_tile_design(d.X.data(), d.off.data(), n, p, tl, mixed);
   DeviceTile dt;
-  if ((rc = upload_tile(ctx, th, dt))) return rc;
+  if ((rc = upload_tile(ctx, tl, dt))) return rc;
 
   // per-gene device state (gene-major)
   DevBuf alpha0, kvec, beta, theta, iters, thit, status, bread, sandwich;
@@ -603,8 +715,9 @@ int devil_fit(devil_ctx* ctx, const devil_problem* pr, devil_outputs* out) {
     DV_HIP(dv::launch_k_from_alpha(s, a0, Gb, kvec.as<double>() + g0));
     tm.end(ev);
     ev = tm.begin(DEVIL_STAGE_IRLS);
-    if ((rc = run_irls_tiled(ctx, th.td, c, Gb, p, b0, kvec.as<double>() + g0, pr->max_iter,
-                             eps, iters.as<int32_t>() + g0, status.as<int32_t>() + g0)))
+    if ((rc = run_irls_tiled(ctx, tl.td, c, Gb, p, b0, kvec.as<double>() + g0, pr->max_iter,
+                             eps, iters.as<int32_t>() + g0, status.as<int32_t>() + g0,
+                             mixed)))
       return rc;
     tm.end(ev);
     double* th = theta.as<double>() + g0;
@@ -614,9 +727,9 @@ int devil_fit(devil_ctx* ctx, const devil_problem* pr, devil_outputs* out) {
                             status.as<int32_t>() + g0));
       DV_HIP(dv::launch_fill_i32(s, thit.as<int32_t>() + g0, Gb, 0));
     } else if (pr->od_mode == DEVIL_OD_MLE) {
-      DV_HIP(dv::launch_mle(s, c, Gb, n, p, dd.X.as<double>(), b0, dd.off.as<double>(),
-                            pr->max_iter, th, thit.as<int32_t>() + g0,
-                            status.as<int32_t>() + g0));
+      if ((rc = run_mle_tiled(ctx, tl.td, c, Gb, n, p, b0, pr->max_iter, th,
+                              thit.as<int32_t>() + g0, status.as<int32_t>() + g0)))
+        return rc;
     } else {
       DV_HIP(hipMemsetAsync(th, 0, sizeof(double) * Gb, s));
       DV_HIP(dv::launch_fill_i32(s, thit.as<int32_t>() + g0, Gb, 0));
@@ -722,8 +835,8 @@ int devil_beta_fit(devil_ctx* ctx, const void* counts, int32_t dtype, int64_t G,
   if (!X || !offset || (G > 0 && (!beta || !alpha0 || !iters)))
     return fail(ctx, DEVIL_E_ARG, "NULL pointer");
   if (max_iter < 0) return fail(ctx, DEVIL_E_ARG, "max_iter < 0");
-  if (precision != DEVIL_PREC_FP64)
-    return fail(ctx, DEVIL_E_UNSUPPORTED, "mixed precision path not built yet");
+  if (precision != DEVIL_PREC_FP64 && precision != DEVIL_PREC_MIXED)
+    return fail(ctx, DEVIL_E_ARG, "unknown precision");
   if (G == 0) return DEVIL_OK;
   hipStream_t s = ctx->stream;
   DevBuf bg, kv;
@@ -736,7 +849,7 @@ int devil_beta_fit(devil_ctx* ctx, const void* counts, int32_t dtype, int64_t G,
   DV_HIP(hipMemcpy(Xh.data(), X, sizeof(double) * n * p, hipMemcpyDeviceToHost));
   DV_HIP(hipMemcpy(offh.data(), offset, sizeof(double) * n, hipMemcpyDeviceToHost));
   TileHost th;
-  build_tile_design(Xh.data(), offh.data(), n, p, th);
+  build_tile_design(Xh.data(), offh.data(), n, p, th, precision == DEVIL_PREC_MIXED);
   DeviceTile dt;
   if ((rc = upload_tile(ctx, th, dt))) return rc;
   DevBuf st_local;
@@ -746,7 +859,7 @@ int devil_beta_fit(devil_ctx* ctx, const void* counts, int32_t dtype, int64_t G,
     st = st_local.as<int32_t>();
   }
   if ((rc = run_irls_tiled(ctx, th.td, c, G, p, bg.as<double>(), kv.as<double>(), max_iter,
-                           (double)(float)tolerance, iters, st)))
+                           (double)(float)tolerance, iters, st, precision == DEVIL_PREC_MIXED)))
     return rc;
   DV_HIP(dv::launch_transpose_gp(s, bg.as<double>(), G, p, 0, beta));
   DV_HIP(hipStreamSynchronize(s));
@@ -789,12 +902,22 @@ int devil_fit_dispersion_mle(devil_ctx* ctx, const void* counts, int32_t dtype, 
     return fail(ctx, DEVIL_E_ARG, "NULL pointer");
   if (G == 0) return DEVIL_OK;
   hipStream_t s = ctx->stream;
-  DevBuf bg;
+  DevBuf bg, st;
   DV_ALLOC(bg, sizeof(double) * G * p);
+  DV_ALLOC(st, sizeof(int32_t) * G);
   DV_HIP(dv::launch_transpose_gp(s, beta, G, p, 1, bg.as<double>()));
+  DV_HIP(dv::launch_fill_i32(s, st.as<int32_t>(), G, 0));
   dv::Counts c{counts, dtype, ld};
-  DV_HIP(dv::launch_mle(s, c, G, n, p, X, bg.as<double>(), offset, max_iter, theta, iters,
-                        nullptr));
+  std::vector<double> Xh((size_t)n * p), offh(n);
+  DV_HIP(hipMemcpy(Xh.data(), X, sizeof(double) * n * p, hipMemcpyDeviceToHost));
+  DV_HIP(hipMemcpy(offh.data(), offset, sizeof(double) * n, hipMemcpyDeviceToHost));
+  TileHost th;
+  build_tile_design(Xh.data(), offh.data(), n, p, th);
+  DeviceTile dt;
+  if ((rc = upload_tile(ctx, th, dt))) return rc;
+  if ((rc = run_mle_tiled(ctx, th.td, c, G, n, p, bg.as<double>(), max_iter, theta, iters,
+                          st.as<int32_t>())))
+    return rc;
   DV_HIP(hipStreamSynchronize(s));
   return DEVIL_OK;
 }

@@ -43,6 +43,11 @@ struct TileDesign {
                          // pi(m) = 4 (m % 4) + m / 4
   const double* offp;    // off padded with zeros to 16 nblk
   const double* Kp;      // [blk][t][lane][r] = K[16 blk + 4 (lane/16) + r][16 t + lane%16]
+  // the same operands in FP32 for the mixed-precision IRLS (null unless built);
+  // rows of the eta product follow the f32 MFMA output layout (identity order)
+  const float* Xf;       // [blk][j][lane] = X[16 blk + lane%16][4 j + lane/16]
+  const float* offf;     // off padded with zeros to 16 nblk
+  const float* Kf;       // [blk][t][lane][r] = K[16 blk + 4 (lane/16) + r][16 t + lane%16]
   int kidx_pair[55];     // K column of x_a x_b (packed upper triangle), -1 = zero column
   int kidx_lin[10];      // K column of x_a, -1 = zero column
   double sx[10];         // sum_i x_ia
@@ -51,12 +56,52 @@ struct TileDesign {
 hipError_t launch_irls_accum(hipStream_t s, const TileDesign& td, Counts c, int p,
                              const int32_t* active, const int32_t* nactive, int nA_bound,
                              const double* beta, const double* kvec, double* part);
+// Mixed precision: FP32 transcendentals and v_mfma_f32_16x16x4f32 products,
+// FP32 accumulation flushed to FP64 every 256 cells, FP64 chunk partials (the
+// layout k_irls_update reads).
+hipError_t launch_irls_accum_mixed(hipStream_t s, const TileDesign& td, Counts c, int p,
+                                   const int32_t* active, const int32_t* nactive, int nA_bound,
+                                   const double* beta, const double* kvec, double* part);
+
 hipError_t launch_irls_update(hipStream_t s, const TileDesign& td, int p,
                               const int32_t* active, const int32_t* nactive, int nA_bound,
                               const double* part, double* beta, const double* kvec,
                               int max_iter, double eps, int32_t* iters, int32_t* status,
-                              int32_t* next, int32_t* nnext);
+                              int32_t* next, int32_t* nnext, double band = 0.0,
+                              int32_t* recheck = nullptr, int32_t* nrecheck = nullptr);
 hipError_t launch_iota(hipStream_t s, int32_t* p, int n);
+
+// Per-gene state of the tiled MLE (kernels_mle.hip); gene-major device arrays.
+struct MleState {
+  double* x;       // current log(theta)
+  double* lo;      // Newton bracket
+  double* hi;
+  double* start;   // (var(y) - ybar) / ybar^2, or 0.5
+  double* sum_y;
+  double* sum_py;  // sum (y - 1) y
+  double* max_y;
+  int32_t* mode;   // 0 far-left pass pending, 1 Newton, 2 done
+  int32_t* iters;  // score passes
+  double* theta;   // result
+  int32_t* status; // DEVIL_GENE_* bits (or-ed)
+  int32_t* tab_n;  // distinct positive counts in the gene's table, -1 = none
+  int32_t* tab_u;  // [g * kMleTabCap + k] value
+  int32_t* tab_f;  // [g * kMleTabCap + k] frequency
+};
+constexpr int kMleTabCap = 256;
+
+hipError_t launch_mle_setup(hipStream_t s, Counts c, int64_t G, int64_t n, int p,
+                            const MleState& ms);
+hipError_t launch_mle_collect(hipStream_t s, const int32_t* mode, int G, int32_t* act,
+                              int32_t* nact);
+hipError_t launch_mle_accum(hipStream_t s, const TileDesign& td, Counts c, int p,
+                            const int32_t* active, const int32_t* nactive, int nA_bound,
+                            const double* beta, const double* xs, const int32_t* tab_n,
+                            double* part);
+hipError_t launch_mle_update(hipStream_t s, const TileDesign& td, int p, const int32_t* active,
+                             const int32_t* nactive, int nA_bound, const double* part,
+                             const MleState& ms, int64_t n, int max_iter, int32_t* next,
+                             int32_t* nnext);
 
 hipError_t launch_init(hipStream_t s, Counts c, int64_t G, int64_t n, int p,
                        const InitParams& ip, double* alpha0, double* beta0);

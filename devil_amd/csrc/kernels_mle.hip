@@ -1,0 +1,471 @@
+// gfx950 MLE overdispersion with Cox-Reid adjustment, tiled like the IRLS.
+//
+// R/overdispersion.R:25-115 runs nlminb on the score of
+// src/overdispersion.cpp:95-178 and its derivative (:186-242).  Here every
+// gene runs the safeguarded Newton iteration on score(log theta) = 0 (the
+// same driver as the FP64 oracle), all genes in lock-step rounds:
+//
+//   k_mle_setup   one wavefront per gene: sum y, sum (y-1) y, max y, the
+//                 all-zero rule and the start value (var(y) - ybar) / ybar^2.
+//   k_mle_accum   one wavefront per (16-gene tile, cell chunk) evaluates, at
+//                 each gene's current log theta, the per-cell sums of the
+//                 score and its derivative and the three Cox-Reid Gram
+//                 matrices X' diag(w) X, X' diag(-w^2) X, X' diag(2 w^3) X
+//                 with w = mu / (1 + mu theta) (v_mfma_f64_16x16x4f64 over the
+//                 Khatri-Rao columns, as in the IRLS).
+//   k_mle_update  one wavefront per active gene: forms score and derivative,
+//                 applies the far-left rule (score(log 1e-8) < 0 -> 0) on the
+//                 first round and one Newton / bracketing step after that.
+//
+// Per cell everything is written with r = 1 / (1 + mu theta):
+//   y / (mu + 1/theta) = y theta r,  (y - mu) / (mu + 1/theta) = (y - mu) theta r,
+//   (mu^2 theta + y) / (1 + mu theta)^2 = (mu mu theta + y) r^2,  w = mu r,
+// and the digamma / trigamma sums and log(1 + x) - x / (1 + x) are taken in
+// their cancellation-free forms (dpsi_diff, dpsi1_diff, nb_f).
+#include <hip/hip_runtime.h>
+
+#include "../../include/devil_hip.h"
+#include "device_common.hpp"
+#include "kernels.hpp"
+#include "tile_common.hpp"
+
+namespace dv {
+
+enum { kMleFarLeft = 0, kMleNewton = 1, kMleDone = 2 };
+
+// columns of a gene's partial row: 5 scalars then the three Gram tiles
+constexpr int kMleScal = 5;  // sc, l1, l2, dg, tg
+
+// Per-gene table of the distinct positive counts (value, frequency), the
+// make_table_if_small idea of src/overdispersion.cpp:32-46: the digamma and
+// trigamma sums then cost O(#distinct) per Newton round instead of O(n).
+// Counts of 4096 or more, fractional counts, or more than kMleTabCap distinct
+// values leave the gene on the per-cell path (tab_n = -1).
+constexpr int kMleHist = 4096;
+
+template <int P>
+__global__ __launch_bounds__(64) void k_mle_setup(Counts c, int64_t G, int64_t n,
+                                                  MleState ms) {
+  __shared__ int32_t hist[kMleHist];
+  __shared__ int32_t big;
+  const int64_t g = blockIdx.x;
+  if (g >= G) return;
+  const int lane = threadIdx.x;
+  for (int k = lane; k < kMleHist; k += 64) hist[k] = 0;
+  if (lane == 0) big = 0;
+  __syncthreads();
+  const void* row = count_row(c.ptr, c.dtype, g, c.ld);
+  double s1 = 0.0, spy = 0.0, my = 0.0, nz = 0.0;
+  for (int64_t i = lane; i < n; i += kWave) {
+    const double y = load_count(row, c.dtype, i);
+    s1 += y;
+    spy += (y - 1) * y;
+    my = fmax(my, y);
+    nz += (y == 0.0) ? 1.0 : 0.0;
+    if (y > 0.0) {
+      if (y < (double)kMleHist && y == floor(y)) atomicAdd(&hist[(int)y], 1);
+      else big = 1;
+    }
+  }
+  s1 = wave_sum(s1);
+  spy = wave_sum(spy);
+  my = wave_max(my);
+  nz = wave_sum(nz);
+  __syncthreads();
+  // compact the non-zero bins (ascending value) into the gene's table
+  {
+    constexpr int per = kMleHist / 64;
+    int cnt = 0;
+    for (int k = lane * per; k < (lane + 1) * per; ++k) cnt += hist[k] > 0 ? 1 : 0;
+    int pre = cnt;  // inclusive prefix sum over lanes
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int v = __shfl_up(pre, o, 64);
+      if (lane >= o) pre += v;
+    }
+    const int total = __shfl(pre, 63, 64);
+    const bool ok = big == 0 && total <= kMleTabCap;
+    if (ok) {
+      int pos = pre - cnt;
+      int32_t* tu = ms.tab_u + g * kMleTabCap;
+      int32_t* tf = ms.tab_f + g * kMleTabCap;
+      for (int k = lane * per; k < (lane + 1) * per; ++k)
+        if (hist[k] > 0) {
+          tu[pos] = k;
+          tf[pos] = hist[k];
+          ++pos;
+        }
+    }
+    if (lane == 0) ms.tab_n[g] = ok ? total : -1;
+  }
+  if (nz == (double)n) {  // all(y == 0) -> 0 (R/overdispersion.R:28-30)
+    if (lane == 0) {
+      ms.mode[g] = kMleDone;
+      ms.theta[g] = 0.0;
+      ms.iters[g] = 0;
+      ms.status[g] |= DEVIL_GENE_MLE_ZERO;
+    }
+    return;
+  }
+  const double ybar = s1 / (double)n;
+  double v = 0.0;
+  for (int64_t i = lane; i < n; i += kWave) {
+    const double d = load_count(row, c.dtype, i) - ybar;
+    v += d * d;
+  }
+  v = wave_sum(v) / (double)(n - 1);
+  double start = (v - ybar) / (ybar * ybar);
+  if (isnan(start) || start <= 0) start = 0.5;
+  if (lane == 0) {
+    ms.mode[g] = kMleFarLeft;
+    ms.x[g] = log(1e-8);
+    ms.start[g] = start;
+    ms.sum_y[g] = s1;
+    ms.sum_py[g] = spy;
+    ms.max_y[g] = my;
+    ms.iters[g] = 0;
+  }
+}
+
+__global__ void k_mle_collect(const int32_t* __restrict__ mode, int G, int32_t* __restrict__ act,
+                              int32_t* __restrict__ nact) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < G && mode[g] != kMleDone) act[atomicAdd(nact, 1)] = g;
+}
+
+template <int P, typename CT>
+__device__ __forceinline__ void mle_tile_chunk(const TileDesign& td, const CT* __restrict__ Y,
+                                               int64_t ld, const int32_t* __restrict__ active,
+                                               int nA, const double* __restrict__ beta,
+                                               const double* __restrict__ xs,
+                                               const int32_t* __restrict__ tab_n, int tile,
+                                               int chunk, double* __restrict__ part) {
+  constexpr int NJ = (P + 3) / 4;
+  constexpr int NTMAX = ((P * (P + 1)) / 2 + P + 15) / 16;
+  const int lane = threadIdx.x & 63;
+  const int s = lane & 15, h = lane >> 4;
+  const int slot = tile * 16 + s;
+  const bool live = slot < nA;
+  const int g = live ? active[slot] : active[tile * 16];
+  const int NT = td.nt;
+  double bpos[NJ];
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) {
+    const int f = 4 * j + h;
+    bpos[j] = (live && f < P) ? beta[(int64_t)g * P + f] : 0.0;
+  }
+  const double theta = exp(xs[g]);
+  const double tn1 = 1.0 / theta;
+  const bool cell_psi = tab_n[g] < 0;
+  const CT* yrow = Y + (int64_t)g * ld;
+  const bool vec = (ld & 3) == 0 && ((reinterpret_cast<uintptr_t>(Y) & 15) == 0);
+  d4 a0[NTMAX], a1[NTMAX], a2[NTMAX];
+#pragma unroll
+  for (int t = 0; t < NTMAX; ++t) {
+    a0[t] = d4{0.0, 0.0, 0.0, 0.0};
+    a1[t] = a0[t];
+    a2[t] = a0[t];
+  }
+  double sc = 0.0, l1 = 0.0, l2 = 0.0, dg = 0.0, tg = 0.0;
+  const int64_t b0 = (int64_t)chunk * td.chunk_blocks;
+  const int64_t b1 = min((int64_t)(chunk + 1) * td.chunk_blocks, td.nblk);
+  BlockOps<NJ, NTMAX> cur, nxt;
+  load_block<NJ, NTMAX, CT>(td, yrow, vec, b0, lane, h, NT, cur);
+  for (int64_t b = b0; b < b1; ++b) {
+    if (b + 1 < b1) load_block<NJ, NTMAX, CT>(td, yrow, vec, b + 1, lane, h, NT, nxt);
+    d4 eta = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) eta = mfma16x16x4(cur.xa[j], bpos[j], eta);
+    const int64_t c0 = b * 16 + 4 * h;
+    double w0[4], w1[4], w2[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      double mu = exp_f64(eta[r] + cur.off[r]);
+      if (mu == 0.0) mu = 1e-6;
+      const double y = cur.y[r];
+      const double mt = mu * theta;
+      const double rr = rcp_f64(1.0 + mt);
+      const double L = log1p_f64(mt);
+      const double f = (mt < 1e-2) ? nb_f(mt) : L - mt * rr;
+      const double w = mu * rr;
+      w0[r] = w;
+      w1[r] = -1.0 * w * w;
+      w2[r] = -2.0 * w1[r] * w;
+      if (c0 + r < td.n) {
+        sc += f + y * theta * rr;
+        l1 += L + (y - mu) * theta * rr;
+        l2 += (mu * mt + y) * rr * rr;
+        if (cell_psi && y != 0.0) {
+          dg += dpsi_diff(tn1, y);
+          tg += dpsi1_diff(tn1, y);
+        }
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < NTMAX; ++t) {
+      if (t < NT) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          a0[t] = mfma16x16x4(w0[r], cur.kb[t][r], a0[t]);
+          a1[t] = mfma16x16x4(w1[r], cur.kb[t][r], a1[t]);
+          a2[t] = mfma16x16x4(w2[r], cur.kb[t][r], a2[t]);
+        }
+      }
+    }
+    cur = nxt;
+  }
+  // the four lanes of a gene slot (h = 0..3) hold disjoint cells
+#pragma unroll
+  for (int o = 16; o <= 32; o <<= 1) {
+    sc += __shfl_xor(sc, o, 64);
+    l1 += __shfl_xor(l1, o, 64);
+    l2 += __shfl_xor(l2, o, 64);
+    dg += __shfl_xor(dg, o, 64);
+    tg += __shfl_xor(tg, o, 64);
+  }
+  const int W = kMleScal + 3 * td.ncp;
+  double* out = part + ((int64_t)tile * td.nchunk + chunk) * 16 * W;
+  if (h == 0) {
+    double* o = out + s * W;
+    o[0] = sc;
+    o[1] = l1;
+    o[2] = l2;
+    o[3] = dg;
+    o[4] = tg;
+  }
+#pragma unroll
+  for (int t = 0; t < NTMAX; ++t) {
+    if (t < NT) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        double* o = out + (h + 4 * r) * W + kMleScal + 16 * t + s;
+        o[0] = a0[t][r];
+        o[td.ncp] = a1[t][r];
+        o[2 * td.ncp] = a2[t][r];
+      }
+    }
+  }
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void k_mle_accum(TileDesign td, Counts c,
+                                                   const int32_t* __restrict__ active,
+                                                   const int32_t* __restrict__ nactive,
+                                                   const double* __restrict__ beta,
+                                                   const double* __restrict__ xs,
+                                                   const int32_t* __restrict__ tab_n,
+                                                   double* __restrict__ part) {
+  const int nA = *nactive;
+  const int ntile = (nA + 15) / 16;
+  const int wave = threadIdx.x >> 6;
+  const int chunk = blockIdx.x % td.nchunk;
+  const int tile = (blockIdx.x / td.nchunk) * 4 + wave;
+  if (tile >= ntile) return;
+  if (c.dtype == 0)
+    mle_tile_chunk<P>(td, static_cast<const int32_t*>(c.ptr), c.ld, active, nA, beta, xs,
+                      tab_n, tile, chunk, part);
+  else if (c.dtype == 1)
+    mle_tile_chunk<P>(td, static_cast<const float*>(c.ptr), c.ld, active, nA, beta, xs, tab_n,
+                      tile, chunk, part);
+  else
+    mle_tile_chunk<P>(td, static_cast<const double*>(c.ptr), c.ld, active, nA, beta, xs,
+                      tab_n, tile, chunk, part);
+}
+
+template <int P>
+__global__ __launch_bounds__(64) void k_mle_update(TileDesign td, const int32_t* __restrict__ active,
+                                                   const int32_t* __restrict__ nactive,
+                                                   const double* __restrict__ part, MleState ms,
+                                                   int64_t n, int max_iter,
+                                                   int32_t* __restrict__ next,
+                                                   int32_t* __restrict__ nnext) {
+  __shared__ double acc[kMleScal + 3 * DEVIL_NCP_MAX];
+  const int nA = *nactive;
+  const int i = blockIdx.x;
+  if (i >= nA) return;
+  const int lane = threadIdx.x;
+  const int g = active[i];
+  const int tile = i >> 4, s = i & 15;
+  const int W = kMleScal + 3 * td.ncp;
+  const double* src = part + (int64_t)tile * td.nchunk * 16 * W + (int64_t)s * W;
+  for (int col = lane; col < W; col += 64) {
+    double a = 0.0;
+    for (int ch = 0; ch < td.nchunk; ++ch) a += src[(int64_t)ch * 16 * W + col];
+    acc[col] = a;
+  }
+  __syncthreads();
+  const double x = ms.x[g];
+  const double theta = exp(x);
+  const double tn1 = 1.0 / theta;
+  // digamma / trigamma difference sums over the distinct-count table
+  double tdg = 0.0, ttg = 0.0;
+  const int tn = ms.tab_n[g];
+  for (int k = lane; k < tn; k += 64) {
+    const double u = (double)ms.tab_u[(int64_t)g * kMleTabCap + k];
+    const double f = (double)ms.tab_f[(int64_t)g * kMleTabCap + k];
+    tdg += f * dpsi_diff(tn1, u);
+    ttg += f * dpsi1_diff(tn1, u);
+  }
+  tdg = wave_sum(tdg);
+  ttg = wave_sum(ttg);
+  if (lane != 0) return;
+  const double sc = acc[0], l1 = acc[1], l2 = acc[2];
+  const double dg = acc[3] + tdg, tg = acc[4] + ttg;
+  const double* B0 = acc + kMleScal;
+  const double* B1 = B0 + td.ncp;
+  const double* B2 = B1 + td.ncp;
+  double Bm[P][P], Dm[P][P], D2m[P][P], Bi[P][P];
+#pragma unroll
+  for (int a = 0; a < P; ++a)
+#pragma unroll
+    for (int bb = a; bb < P; ++bb) {
+      const int pi = td.kidx_pair[tri<P>(a, bb)];
+      const double v0 = (pi >= 0 ? B0[pi] : 0.0) + (a == bb ? 1e-6 : 0.0);
+      const double v1 = pi >= 0 ? B1[pi] : 0.0;
+      const double v2 = pi >= 0 ? B2[pi] : 0.0;
+      Bm[a][bb] = v0;
+      Bm[bb][a] = v0;
+      Dm[a][bb] = v1;
+      Dm[bb][a] = v1;
+      D2m[a][bb] = v2;
+      D2m[bb][a] = v2;
+    }
+  chol_inverse<P>(Bm, Bi);
+  double tr_bdb = 0.0;
+#pragma unroll
+  for (int a = 0; a < P; ++a)
+#pragma unroll
+    for (int bb = 0; bb < P; ++bb) tr_bdb += Bi[a][bb] * Dm[bb][a];
+  const double cr_s = -0.5 * tr_bdb * 0.99;
+  const double sum_y = ms.sum_y[g], spy = ms.sum_py[g], max_y = ms.max_y[g];
+  const double corr = tn1 > 1e5 ? spy / (2 * tn1) : 0.0;
+  const double dterm = (max_y * 1e6 < tn1) ? sum_y - corr : fmin(dg * tn1, sum_y - corr);
+  const double score = sc * tn1 - dterm + cr_s * theta;
+  int mode = ms.mode[g];
+  int it = ms.iters[g] + 1;
+  ms.iters[g] = it;
+  if (mode == kMleFarLeft) {
+    if (score < 0) {  // R/overdispersion.R:40-46
+      ms.mode[g] = kMleDone;
+      ms.theta[g] = 0.0;
+      ms.status[g] |= DEVIL_GENE_MLE_ZERO;
+      return;
+    }
+    const double lo = log(1e-8), hi = log(1e16);
+    ms.lo[g] = lo;
+    ms.hi[g] = hi;
+    ms.x[g] = fmin(fmax(log(ms.start[g]), lo), hi);
+    ms.mode[g] = kMleNewton;
+    if (max_iter > 0) {
+      next[atomicAdd(nnext, 1)] = g;
+    } else {
+      ms.mode[g] = kMleDone;
+      ms.theta[g] = exp(ms.x[g]);
+    }
+    return;
+  }
+  // derivative of the score (src/overdispersion.cpp:186-242)
+  double DiDb[P][P];
+#pragma unroll
+  for (int a = 0; a < P; ++a)
+#pragma unroll
+    for (int bb = 0; bb < P; ++bb) {
+      double t = 0.0;
+#pragma unroll
+      for (int k = 0; k < P; ++k) t += Bi[a][k] * Dm[k][bb];
+      DiDb[a][bb] = t;
+    }
+  double ddetb = 0.0, tr_sq = 0.0, tr_d2 = 0.0;
+#pragma unroll
+  for (int a = 0; a < P; ++a) {
+    ddetb += DiDb[a][a];
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+      tr_sq += DiDb[a][k] * DiDb[k][a];
+      tr_d2 += Bi[a][k] * D2m[k][a];
+    }
+  }
+  const double d2detb = ddetb * ddetb - tr_sq + tr_d2;
+  const double cr_t = (0.5 * ddetb * ddetb - 0.5 * d2detb) * 0.99;
+  const double cr_t2 = -0.5 * ddetb * 0.99;
+  const double tn2 = 1.0 / (theta * theta);
+  const double ll = -2 * tn1 * (l1 - dg) + (l2 + tn2 * tg);
+  const double d = ll + cr_t * (theta * theta) + (l1 - dg) * tn1 + cr_t2 * theta;
+  // one safeguarded Newton step (same driver as the oracle)
+  bool done = false;
+  double xn = x;
+  if (!isfinite(score) || !isfinite(d) || score == 0.0) {
+    done = true;
+  } else {
+    double lo = ms.lo[g], hi = ms.hi[g];
+    if (score > 0) lo = fmax(lo, x);
+    else hi = fmin(hi, x);
+    xn = (d < 0) ? x - score / d : (score > 0 ? x + 1.0 : x - 1.0);
+    if (!(xn > lo && xn < hi)) xn = 0.5 * (lo + hi);
+    ms.lo[g] = lo;
+    ms.hi[g] = hi;
+    ms.x[g] = xn;
+    if (fabs(xn - x) < 1e-9) done = true;
+  }
+  if (it - 1 >= max_iter) done = true;  // Newton passes = iters - 1 (far-left pass)
+  if (done) {
+    ms.mode[g] = kMleDone;
+    ms.theta[g] = exp(xn);
+  } else {
+    next[atomicAdd(nnext, 1)] = g;
+  }
+}
+
+#define DV_MLE_P(p, KERNEL, GRID, BLOCK, STREAM, ...)                                \
+  switch (p) {                                                                      \
+    case 1: hipLaunchKernelGGL(KERNEL<1>, GRID, BLOCK, 0, STREAM, __VA_ARGS__); break; \
+    case 2: hipLaunchKernelGGL(KERNEL<2>, GRID, BLOCK, 0, STREAM, __VA_ARGS__); break; \
+    case 3: hipLaunchKernelGGL(KERNEL<3>, GRID, BLOCK, 0, STREAM, __VA_ARGS__); break; \
+    case 4: hipLaunchKernelGGL(KERNEL<4>, GRID, BLOCK, 0, STREAM, __VA_ARGS__); break; \
+    case 5: hipLaunchKernelGGL(KERNEL<5>, GRID, BLOCK, 0, STREAM, __VA_ARGS__); break; \
+    case 6: hipLaunchKernelGGL(KERNEL<6>, GRID, BLOCK, 0, STREAM, __VA_ARGS__); break; \
+    case 7: hipLaunchKernelGGL(KERNEL<7>, GRID, BLOCK, 0, STREAM, __VA_ARGS__); break; \
+    case 8: hipLaunchKernelGGL(KERNEL<8>, GRID, BLOCK, 0, STREAM, __VA_ARGS__); break; \
+    case 9: hipLaunchKernelGGL(KERNEL<9>, GRID, BLOCK, 0, STREAM, __VA_ARGS__); break; \
+    case 10: hipLaunchKernelGGL(KERNEL<10>, GRID, BLOCK, 0, STREAM, __VA_ARGS__); break; \
+    default: return hipErrorInvalidValue;                                           \
+  }
+
+hipError_t launch_mle_setup(hipStream_t s, Counts c, int64_t G, int64_t n, int p,
+                            const MleState& ms) {
+  if (G <= 0) return hipSuccess;
+  DV_MLE_P(p, k_mle_setup, dim3((unsigned)G), dim3(64), s, c, G, n, ms);
+  return hipGetLastError();
+}
+
+hipError_t launch_mle_collect(hipStream_t s, const int32_t* mode, int G, int32_t* act,
+                              int32_t* nact) {
+  if (G <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_mle_collect, dim3((unsigned)((G + 255) / 256)), dim3(256), 0, s, mode,
+                     G, act, nact);
+  return hipGetLastError();
+}
+
+hipError_t launch_mle_accum(hipStream_t s, const TileDesign& td, Counts c, int p,
+                            const int32_t* active, const int32_t* nactive, int nA_bound,
+                            const double* beta, const double* xs, const int32_t* tab_n,
+                            double* part) {
+  if (nA_bound <= 0) return hipSuccess;
+  const int ntile = (nA_bound + 15) / 16;
+  const int groups = (ntile + 3) / 4;
+  const dim3 grid((unsigned)((int64_t)groups * td.nchunk));
+  DV_MLE_P(p, k_mle_accum, grid, dim3(256), s, td, c, active, nactive, beta, xs, tab_n, part);
+  return hipGetLastError();
+}
+
+hipError_t launch_mle_update(hipStream_t s, const TileDesign& td, int p, const int32_t* active,
+                             const int32_t* nactive, int nA_bound, const double* part,
+                             const MleState& ms, int64_t n, int max_iter, int32_t* next,
+                             int32_t* nnext) {
+  if (nA_bound <= 0) return hipSuccess;
+  DV_MLE_P(p, k_mle_update, dim3((unsigned)nA_bound), dim3(64), s, td, active, nactive, part,
+           ms, n, max_iter, next, nnext);
+  return hipGetLastError();
+}
+
+}  // namespace dv

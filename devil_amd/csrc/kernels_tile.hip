@@ -33,117 +33,9 @@
 #include "../../include/devil_hip.h"
 #include "device_common.hpp"
 #include "kernels.hpp"
+#include "tile_common.hpp"
 
 namespace dv {
-
-typedef double d4 __attribute__((ext_vector_type(4)));
-typedef int i4v __attribute__((ext_vector_type(4)));
-typedef float f4v __attribute__((ext_vector_type(4)));
-
-__device__ __forceinline__ d4 mfma16x16x4(double a, double b, d4 c) {
-  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-}
-
-// exp(x) = 2^n e^r with n = rint(x / ln 2) and r = x - n ln 2 (Cody-Waite,
-// two constants), e^r by its degree-12 Taylor polynomial (|r| <= 0.347,
-// truncation < 2.5e-16 relative), scaled with v_ldexp_f64 which saturates to
-// inf / 0 outside the double range.
-__device__ __forceinline__ double exp_f64(double x) {
-  const double n = rint(x * 1.4426950408889634);
-  double r = fma(-n, 6.93147180369123816490e-01, x);
-  r = fma(-n, 1.90821492927058770002e-10, r);
-  double p = 1.0 / 479001600.0;
-  p = fma(p, r, 1.0 / 39916800.0);
-  p = fma(p, r, 1.0 / 3628800.0);
-  p = fma(p, r, 1.0 / 362880.0);
-  p = fma(p, r, 1.0 / 40320.0);
-  p = fma(p, r, 1.0 / 5040.0);
-  p = fma(p, r, 1.0 / 720.0);
-  p = fma(p, r, 1.0 / 120.0);
-  p = fma(p, r, 1.0 / 24.0);
-  p = fma(p, r, 1.0 / 6.0);
-  p = fma(p, r, 0.5);
-  p = fma(p, r, 1.0);
-  p = fma(p, r, 1.0);
-  return ldexp(p, (int)n);
-}
-
-// 1/d from v_rcp_f64 refined by two Newton steps (about 1 ulp).
-__device__ __forceinline__ double rcp_f64(double d) {
-  double r = __builtin_amdgcn_rcp(d);
-  double e = fma(-d, r, 1.0);
-  r = fma(r, e, r);
-  e = fma(-d, r, 1.0);
-  return fma(r, e, r);
-}
-
-// Four consecutive counts of a row as doubles.
-template <typename CT>
-__device__ __forceinline__ void load4(const CT* p, bool vec, double (&y)[4]);
-
-template <>
-__device__ __forceinline__ void load4<int32_t>(const int32_t* p, bool vec, double (&y)[4]) {
-  if (vec) {
-    const i4v v = *reinterpret_cast<const i4v*>(p);
-#pragma unroll
-    for (int r = 0; r < 4; ++r) y[r] = (double)v[r];
-  } else {
-#pragma unroll
-    for (int r = 0; r < 4; ++r) y[r] = (double)p[r];
-  }
-}
-
-template <>
-__device__ __forceinline__ void load4<float>(const float* p, bool vec, double (&y)[4]) {
-  if (vec) {
-    const f4v v = *reinterpret_cast<const f4v*>(p);
-#pragma unroll
-    for (int r = 0; r < 4; ++r) y[r] = (double)v[r];
-  } else {
-#pragma unroll
-    for (int r = 0; r < 4; ++r) y[r] = (double)p[r];
-  }
-}
-
-template <>
-__device__ __forceinline__ void load4<double>(const double* p, bool vec, double (&y)[4]) {
-  if (vec) {
-    const d4 v = *reinterpret_cast<const d4*>(p);
-#pragma unroll
-    for (int r = 0; r < 4; ++r) y[r] = v[r];
-  } else {
-#pragma unroll
-    for (int r = 0; r < 4; ++r) y[r] = p[r];
-  }
-}
-
-// Operands of one 16-cell block for one lane.
-template <int NJ, int NTMAX>
-struct BlockOps {
-  double y[4];
-  double xa[NJ];
-  d4 off;
-  d4 kb[NTMAX];
-};
-
-template <int NJ, int NTMAX, typename CT>
-__device__ __forceinline__ void load_block(const TileDesign& td, const CT* yrow, bool vec,
-                                           int64_t b, int lane, int h, int NT,
-                                           BlockOps<NJ, NTMAX>& o) {
-  const int64_t c = b * 16 + 4 * h;
-  if (c + 4 <= td.n) {
-    load4<CT>(yrow + c, vec, o.y);
-  } else {
-#pragma unroll
-    for (int r = 0; r < 4; ++r) o.y[r] = (c + r < td.n) ? (double)yrow[c + r] : 0.0;
-  }
-#pragma unroll
-  for (int j = 0; j < NJ; ++j) o.xa[j] = td.Xp[(b * NJ + j) * 64 + lane];
-  o.off = *reinterpret_cast<const d4*>(td.offp + c);
-#pragma unroll
-  for (int t = 0; t < NTMAX; ++t)
-    if (t < NT) o.kb[t] = *reinterpret_cast<const d4*>(td.Kp + ((b * NT + t) * 64 + lane) * 4);
-}
 
 // lane l of a wave: gene slot s = l % 16, cell group h = l / 16; its cells in
 // block b are 16 b + 4 h + r.
@@ -188,7 +80,7 @@ __device__ __forceinline__ void irls_tile_chunk(const TileDesign& td, const CT* 
     for (int r = 0; r < 4; ++r) {
       const double w = exp_f64(eta[r] - cur.off[r]);
       const double d = fma(k, w, 1.0);
-      q[r] = ((k + cur.y[r]) * w) * rcp_f64(d);
+      q[r] = ((k + cur.y[r]) * w) * rcp1_f64(d);
     }
 #pragma unroll
     for (int t = 0; t < NTMAX; ++t) {
@@ -208,6 +100,159 @@ __device__ __forceinline__ void irls_tile_chunk(const TileDesign& td, const CT* 
       for (int r = 0; r < 4; ++r) out[(h + 4 * r) * td.ncp + 16 * t + s] = acc[t][r];
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// Mixed precision accumulate.  Same tile / chunk / partial-sum structure; per
+// element in FP32: mu = 2^((x'beta + off) log2 e) (v_exp_f32),
+// q = (k + y) / (k + mu) (v_rcp_f32) -- the reference's m w written with mu,
+// which keeps the right limits where FP32 mu under- or overflows -- and
+// q' K on v_mfma_f32_16x16x4f32 with FP32 accumulators flushed into FP64
+// every 16 blocks (256 cells).
+// v_mfma_f32_16x16x4f32 (checked on gfx950): lane l supplies A[l%16][l/16]
+// and B[l/16][l%16]; register r of the result holds D[4 (l/16) + r][l%16].
+// ---------------------------------------------------------------------------
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ f4 mfma32(float a, float b, f4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+template <int NJ, int NTMAX>
+struct BlockOpsF {
+  float y[4];
+  float xa[NJ];
+  f4 off;
+  f4 kb[NTMAX];
+};
+
+template <typename CT>
+__device__ __forceinline__ void load4f(const CT* p, bool vec, float (&y)[4]) {
+  double t[4];
+  load4<CT>(p, vec, t);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) y[r] = (float)t[r];
+}
+
+template <int NJ, int NTMAX, typename CT>
+__device__ __forceinline__ void load_block_f(const TileDesign& td, const CT* yrow, bool vec,
+                                             int64_t b, int lane, int h, int NT,
+                                             BlockOpsF<NJ, NTMAX>& o) {
+  const int64_t c = b * 16 + 4 * h;
+  if (c + 4 <= td.n) {
+    load4f<CT>(yrow + c, vec, o.y);
+  } else {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) o.y[r] = (c + r < td.n) ? (float)yrow[c + r] : 0.0f;
+  }
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) o.xa[j] = td.Xf[(b * NJ + j) * 64 + lane];
+  o.off = *reinterpret_cast<const f4*>(td.offf + c);
+#pragma unroll
+  for (int t = 0; t < NTMAX; ++t)
+    if (t < NT) o.kb[t] = *reinterpret_cast<const f4*>(td.Kf + ((b * NT + t) * 64 + lane) * 4);
+}
+
+template <int P, typename CT>
+__device__ __forceinline__ void irls_tile_chunk_mixed(const TileDesign& td,
+                                                      const CT* __restrict__ Y, int64_t ld,
+                                                      const int32_t* __restrict__ active, int nA,
+                                                      const double* __restrict__ beta,
+                                                      const double* __restrict__ kvec, int tile,
+                                                      int chunk, double* __restrict__ part) {
+  constexpr int NJ = (P + 3) / 4;
+  constexpr int NTMAX = ((P * (P + 1)) / 2 + P + 15) / 16;
+  constexpr int kSpan = 16;  // blocks per FP32 accumulation span
+  const int lane = threadIdx.x & 63;
+  const int s = lane & 15, h = lane >> 4;
+  const int slot = tile * 16 + s;
+  const bool live = slot < nA;
+  const int g = live ? active[slot] : active[tile * 16];
+  const int NT = td.nt;
+  float bpos[NJ];
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) {
+    const int f = 4 * j + h;
+    bpos[j] = (live && f < P) ? (float)beta[(int64_t)g * P + f] : 0.0f;
+  }
+  const float k = live ? (float)kvec[g] : 0.0f;  // k is a float in the reference
+  const CT* yrow = Y + (int64_t)g * ld;
+  const bool vec = (ld & 3) == 0 && ((reinterpret_cast<uintptr_t>(Y) & 15) == 0);
+  f4 acc[NTMAX];
+  double acc64[NTMAX][4];
+#pragma unroll
+  for (int t = 0; t < NTMAX; ++t) {
+    acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int r = 0; r < 4; ++r) acc64[t][r] = 0.0;
+  }
+  const int64_t b0 = (int64_t)chunk * td.chunk_blocks;
+  const int64_t b1 = min((int64_t)(chunk + 1) * td.chunk_blocks, td.nblk);
+  BlockOpsF<NJ, NTMAX> cur, nxt;
+  load_block_f<NJ, NTMAX, CT>(td, yrow, vec, b0, lane, h, NT, cur);
+  int span = 0;
+  for (int64_t b = b0; b < b1; ++b) {
+    if (b + 1 < b1) load_block_f<NJ, NTMAX, CT>(td, yrow, vec, b + 1, lane, h, NT, nxt);
+    f4 eta = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) eta = mfma32(cur.xa[j], bpos[j], eta);
+    float q[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const float mu = __builtin_amdgcn_exp2f((eta[r] + cur.off[r]) * 1.44269504088896341f);
+      q[r] = (k + cur.y[r]) * __builtin_amdgcn_rcpf(k + mu);
+    }
+#pragma unroll
+    for (int t = 0; t < NTMAX; ++t) {
+      if (t < NT) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[t] = mfma32(q[r], cur.kb[t][r], acc[t]);
+      }
+    }
+    if (++span == kSpan || b + 1 == b1) {
+      span = 0;
+#pragma unroll
+      for (int t = 0; t < NTMAX; ++t) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc64[t][r] += (double)acc[t][r];
+        acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+      }
+    }
+    cur = nxt;
+  }
+  // register r holds gene slot 4 h + r, column 16 t + s
+  double* out = part + ((int64_t)tile * td.nchunk + chunk) * 16 * td.ncp;
+#pragma unroll
+  for (int t = 0; t < NTMAX; ++t) {
+    if (t < NT) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) out[(4 * h + r) * td.ncp + 16 * t + s] = acc64[t][r];
+    }
+  }
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void k_irls_accum_mixed(TileDesign td, Counts c,
+                                                          const int32_t* __restrict__ active,
+                                                          const int32_t* __restrict__ nactive,
+                                                          const double* __restrict__ beta,
+                                                          const double* __restrict__ kvec,
+                                                          double* __restrict__ part) {
+  const int nA = *nactive;
+  const int ntile = (nA + 15) / 16;
+  const int wave = threadIdx.x >> 6;
+  const int chunk = blockIdx.x % td.nchunk;
+  const int tile = (blockIdx.x / td.nchunk) * 4 + wave;
+  if (tile >= ntile) return;
+  if (c.dtype == 0)
+    irls_tile_chunk_mixed<P>(td, static_cast<const int32_t*>(c.ptr), c.ld, active, nA, beta,
+                             kvec, tile, chunk, part);
+  else if (c.dtype == 1)
+    irls_tile_chunk_mixed<P>(td, static_cast<const float*>(c.ptr), c.ld, active, nA, beta,
+                             kvec, tile, chunk, part);
+  else
+    irls_tile_chunk_mixed<P>(td, static_cast<const double*>(c.ptr), c.ld, active, nA, beta,
+                             kvec, tile, chunk, part);
 }
 
 template <int P>
@@ -248,7 +293,9 @@ __global__ __launch_bounds__(64) void k_irls_update(TileDesign td,
                                                     int32_t* __restrict__ iters,
                                                     int32_t* __restrict__ status,
                                                     int32_t* __restrict__ next,
-                                                    int32_t* __restrict__ nnext) {
+                                                    int32_t* __restrict__ nnext, double band,
+                                                    int32_t* __restrict__ recheck,
+                                                    int32_t* __restrict__ nrecheck) {
   __shared__ double acc[DEVIL_NCP_MAX];
   const int nA = *nactive;
   const int i = blockIdx.x;
@@ -286,11 +333,19 @@ __global__ __launch_bounds__(64) void k_irls_update(TileDesign td,
   bool has_nan = false;
 #pragma unroll
   for (int j = 0; j < P; ++j) {
-    beta[(int64_t)g * P + j] += v[j];
     const double d = fabs(v[j]);
     if (isnan(d)) has_nan = true;
     else mx = fmax(mx, d);
   }
+  // partials from the mixed-precision pass: a step whose stopping decision
+  // could differ from FP64 arithmetic (max |delta| within band * eps of eps,
+  // or NaN) is left undone and the gene is redone in FP64 this iteration
+  if (band > 0.0 && (has_nan || fabs(mx - eps) <= band * eps)) {
+    recheck[atomicAdd(nrecheck, 1)] = g;
+    return;
+  }
+#pragma unroll
+  for (int j = 0; j < P; ++j) beta[(int64_t)g * P + j] += v[j];
   bool conv = !has_nan && mx < eps;
   const int it = iters[g] + 1;
   iters[g] = it;
@@ -334,15 +389,27 @@ hipError_t launch_irls_accum(hipStream_t s, const TileDesign& td, Counts c, int 
   return hipGetLastError();
 }
 
+hipError_t launch_irls_accum_mixed(hipStream_t s, const TileDesign& td, Counts c, int p,
+                                   const int32_t* active, const int32_t* nactive, int nA_bound,
+                                   const double* beta, const double* kvec, double* part) {
+  if (nA_bound <= 0) return hipSuccess;
+  const int ntile = (nA_bound + 15) / 16;
+  const int groups = (ntile + 3) / 4;
+  const dim3 grid((unsigned)((int64_t)groups * td.nchunk));
+  DV_TILE_P(p, k_irls_accum_mixed, grid, dim3(256), s, td, c, active, nactive, beta, kvec, part);
+  return hipGetLastError();
+}
+
 hipError_t launch_irls_update(hipStream_t s, const TileDesign& td, int p,
                               const int32_t* active, const int32_t* nactive, int nA_bound,
                               const double* part, double* beta, const double* kvec,
                               int max_iter, double eps, int32_t* iters, int32_t* status,
-                              int32_t* next, int32_t* nnext) {
+                              int32_t* next, int32_t* nnext, double band, int32_t* recheck,
+                              int32_t* nrecheck) {
   if (nA_bound <= 0) return hipSuccess;
   const dim3 grid((unsigned)nA_bound);
   DV_TILE_P(p, k_irls_update, grid, dim3(64), s, td, active, nactive, part, beta, kvec,
-            max_iter, eps, iters, status, next, nnext);
+            max_iter, eps, iters, status, next, nnext, band, recheck, nrecheck);
   return hipGetLastError();
 }
 
