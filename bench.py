@@ -3,12 +3,17 @@
 One step = the whole fitting path over a synthetic count matrix already
 resident in HBM: init pass (alpha0, beta0), IRLS, overdispersion, bread (+
 clustered meat when the workload has patients) and the Wald test with BH.
-Default workload: G=20000 genes x n=100000 cells int32 counts, p=4 design,
-MLE overdispersion (the plain FP64 path).
+Default workload: G=20000 genes x n=500000 cells int32 counts, p=8 design
+(intercept, 3-level condition, five covariates), MOM overdispersion, fitted
+with the mixed-precision IRLS; the same counts are then fitted with the FP64
+path, which gives the speed-up and the accuracy check (beta, log10 p, BH
+calls) of the mixed path against FP64.
 
-Prints one JSON line with the throughput, a roofline entry for the IRLS
-kernel (algorithmic bytes 4*n*sum_g iters_g over its HIP-event duration) and
-a CPU baseline: the FP64 oracle timed on a gene subsample on the host cores.
+Prints one JSON line with the throughput, a roofline entry for the dominant
+kernel (the IRLS accumulate pass: algorithmic bytes = count element size x n
+x active genes per launch, over its average launch duration from HIP events
+on the library's stream) and a CPU baseline: the FP64 oracle timed on a gene
+subsample on the host cores.
 """
 from __future__ import annotations
 
@@ -32,10 +37,12 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(w, Yd, fit, n_threads):
-    """The FP64 oracle on every ceil(G/2048)-th gene, timed on the host."""
+def cpu_baseline(w, Yd, fit, n_threads, max_genes=None):
+    """The FP64 oracle on an evenly strided gene subsample (about 10-20 s of
+    host work), timed on the host; also checks the GPU fit on those genes."""
     import oracle
-    stride = math.ceil(w.G / 2048)
+    target = max_genes or int(np.clip(round(4096 * 1.6e6 / (w.n * w.p * w.p)), 32, 2048))
+    stride = math.ceil(w.G / target)
     genes = np.arange(0, w.G, stride)
     Ys = Yd[genes].cpu().numpy().astype(np.float64)
     od = {"MOM": oracle.OD_MOM, "MLE": oracle.OD_MLE}[w.od]
@@ -50,8 +57,11 @@ def cpu_baseline(w, Yd, fit, n_threads):
     rel = np.abs(beta - ref["beta"]) / np.maximum(np.abs(ref["beta"]), 1e-6)
     th = fit["overdispersion"].cpu().numpy()[genes]
     rel_th = np.abs(th - ref["theta"]) / np.maximum(np.abs(ref["theta"]), 1e-8)
+    worst = int(np.nanargmax(rel_th))
     check = {"genes": int(len(genes)), "beta_max_rel_err": float(np.nanmax(rel)),
              "theta_max_rel_err": float(np.nanmax(rel_th)),
+             "theta_worst_gene": {"gene": int(genes[worst]), "gpu": float(th[worst]),
+                                  "oracle": float(ref["theta"][worst])},
              "iter_mismatch": int(np.sum(it != ref["beta_iters"]))}
     return {"value": len(genes) * w.n / t, "unit": "gene*cell fits/s", "cores": n_threads,
             "kind": "port",
@@ -59,15 +69,44 @@ def cpu_baseline(w, Yd, fit, n_threads):
                       f"all {w.n} cells of the same workload, {t:.1f} s"}, check
 
 
+def compare(dv, mix, f64, de_mix, de_64):
+    """Mixed against FP64 on the same counts: the accuracy side of the metric."""
+    def rel(a, b, floor=1e-6):
+        a = np.asarray(a, np.float64)
+        b = np.asarray(b, np.float64)
+        e = np.abs(a - b) / np.maximum(np.abs(b), floor)
+        e[np.isnan(a) & np.isnan(b)] = 0.0
+        return float(np.max(e))
+    cpu = lambda t: t.cpu().numpy() if hasattr(t, "cpu") else np.asarray(t)
+    lp_m = cpu(de_mix["log_pval"]) / math.log(10)
+    lp_f = cpu(de_64["log_pval"]) / math.log(10)
+    calls_m = cpu(de_mix["adj_pval"]) < 0.05
+    calls_f = cpu(de_64["adj_pval"]) < 0.05
+    it_m = cpu(mix["iterations"]["beta_iters"])
+    it_f = cpu(f64["iterations"]["beta_iters"])
+    return {"beta_max_rel_err": rel(cpu(mix["beta"]), cpu(f64["beta"]), 1e-3),
+            "beta_max_rel_err_floor_1e-6": rel(cpu(mix["beta"]), cpu(f64["beta"]), 1e-6),
+            "beta_max_abs_err": float(np.nanmax(np.abs(cpu(mix["beta"]) - cpu(f64["beta"])))),
+            "log10p_max_rel_err": rel(lp_m, lp_f),
+            "theta_max_rel_err": rel(cpu(mix["overdispersion"]), cpu(f64["overdispersion"]),
+                                     1e-8),
+            "bh_calls_fp64": int(calls_f.sum()),
+            "bh_call_mismatch": int(np.sum(calls_m != calls_f)),
+            "iter_mismatch": int(np.sum(it_m != it_f))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="C2")
+    ap.add_argument("--config", default="C3")
     ap.add_argument("--genes", type=int, default=None)
     ap.add_argument("--cells", type=int, default=None)
-    ap.add_argument("--precision", default="fp64")
+    ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--fp64-steps", type=int, default=1,
+                    help="timed FP64 steps on the same counts (0: no comparison)")
+    ap.add_argument("--cpu-genes", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -94,40 +133,51 @@ def main():
     ctx = devil_amd.context(0)
     ctx.set_timing(True)
 
-    def step():
+    def step(prec):
         fit = devil_amd.fit_devil(Yd, w.X, clusters=w.clusters, overdispersion=w.od,
-                                  size_factors=w.sf, precision=args.precision)
+                                  size_factors=w.sf, precision=prec)
         de = devil_amd.test_de(fit, contrast)
         return fit, de
 
+    def timed(prec, steps):
+        stage_sum = {k: 0.0 for k in devil_amd._lib.STAGES}
+        kern = {k: [0, 0.0, 0.0] for k in devil_amd._lib.KERNELS}
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(steps):
+            fit, de = step(prec)
+            for k, v in ctx.stage_times().items():
+                stage_sum[k] += v
+            for k, v in ctx.kernel_stats().items():
+                for j in range(3):
+                    kern[k][j] += v[j]
+        torch.cuda.synchronize()
+        T = time.perf_counter() - t0
+        log(f"[bench] {prec}: {steps} steps, {1e3 * T / steps:.1f} ms/step")
+        return T, {k: v / steps for k, v in stage_sum.items()}, kern, fit, de
+
     for i in range(args.warmup):
         t1 = time.perf_counter()
-        fit, de = step()
+        step(args.precision)
         torch.cuda.synchronize()
         log(f"[bench] warmup {i}: {time.perf_counter() - t1:.3f} s")
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    stage_sum = {k: 0.0 for k in devil_amd._lib.STAGES}
-    for i in range(args.steps):
-        fit, de = step()
-        for k, v in ctx.stage_times().items():
-            stage_sum[k] += v
-    torch.cuda.synchronize()
-    T = time.perf_counter() - t0
+    T, stages, kern, fit, de = timed(args.precision, args.steps)
     ms_step = 1e3 * T / args.steps
-    log(f"[bench] {args.steps} steps: {ms_step:.1f} ms/step")
-    stages = {k: v / args.steps for k, v in stage_sum.items()}
     iters = fit["iterations"]["beta_iters"].cpu().numpy().astype(np.int64)
     th_it = fit["iterations"]["theta_iters"].cpu().numpy().astype(np.int64)
     sum_it = int(iters.sum())
-    irls_bytes = 4.0 * w.n * sum_it
-    irls_gbs = irls_bytes / (stages["irls"] * 1e-3) / 1e9
+    kname = "irls_mixed" if args.precision == "mixed" else "irls_fp64"
+    nl, kms, kbytes = kern[kname]
+    avg_ms = kms / max(nl, 1)
+    achieved = kbytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
     # whole-fit algorithmic bytes: init (2 reads: mean, variance) + IRLS +
     # dispersion passes + bread/meat, 4 bytes per count per pass
     disp_passes = int(th_it.sum()) if w.od == "MLE" else w.G
     fit_bytes = 4.0 * w.n * (2 * w.G + sum_it + disp_passes + w.G)
     fit_ms = sum(stages[k] for k in ("init", "irls", "dispersion", "bread_meat"))
     value = w.G * w.n * args.steps / T
+    kdesc = {"irls_mixed": f"k_irls_accum_mixed<{w.p}> (FP32 MFMA, FP64 partials)",
+             "irls_fp64": f"k_irls_accum<{w.p}> (FP64 MFMA)"}[kname]
     out = {
         "metric": METRIC,
         "value": value,
@@ -139,26 +189,39 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32" if args.precision == "mixed" else "f64",
         "data": "synthetic",
         "config": {"workload": f"{w.name}: G={w.G} genes x n={w.n} cells int32 NB counts "
                                f"(zero fraction {zero_frac:.2f}), p={w.p}, {w.od} "
-                               "overdispersion, bread, Wald test + BH",
+                               "overdispersion, bread"
+                               + (", clustered sandwich" if w.clusters is not None else "")
+                               + ", Wald test + BH",
                    "G": w.G, "n": w.n, "p": w.p, "precision": args.precision,
                    "overdispersion": w.od},
-        "roofline": {"bound": "hbm", "kernel": f"k_irls<{w.p}> (IRLS, FP64)",
-                     "achieved": irls_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": irls_gbs / HBM_PEAK_GBS, "traffic": None,
-                     "algorithmic_bytes": irls_bytes, "avg_launch_ms": stages["irls"]},
+        "roofline": {"bound": "hbm", "kernel": kdesc,
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "launches_per_step": nl / args.steps,
+                     "avg_launch_ms": avg_ms,
+                     "algorithmic_bytes_per_launch": kbytes / max(nl, 1)},
         "stages_ms": stages,
         "fit_alg_GBps": fit_bytes / (fit_ms * 1e-3) / 1e9,
         "sum_beta_iters": sum_it,
         "sum_theta_passes": int(th_it.sum()),
         "max_beta_iters": int(iters.max()),
     }
+    if args.precision == "mixed" and args.fp64_steps > 0:
+        T64, st64, kern64, f64, de64 = timed("fp64", args.fp64_steps)
+        ms64 = 1e3 * T64 / args.fp64_steps
+        n64, kms64, kb64 = kern64["irls_fp64"]
+        out["fp64"] = {"ms_per_step": ms64, "value": w.G * w.n / (ms64 * 1e-3),
+                       "speedup_mixed_vs_fp64": ms64 / ms_step,
+                       "stages_ms": st64,
+                       "irls_kernel_GBps": kb64 / (kms64 * 1e-3) / 1e9 if kms64 > 0 else 0.0}
+        out["mixed_vs_fp64"] = compare(devil_amd, fit, f64, de, de64)
     if not args.no_cpu_baseline:
         n_threads = max(1, min(16, os.cpu_count() or 1))
-        cb, check = cpu_baseline(w, Yd, fit, n_threads)
+        cb, check = cpu_baseline(w, Yd, fit, n_threads, args.cpu_genes)
         out["cpu_baseline"] = cb
         out["cpu_check"] = check
     print(json.dumps(out), flush=True)

@@ -83,6 +83,9 @@ def lib():
                                     _vp]),
         "devil_set_timing": (C.c_int, [_vp, C.c_int]),
         "devil_stage_times": (C.c_int, [_vp, C.POINTER(C.c_double)]),
+        "devil_set_tuning": (C.c_int, [_vp, C.c_int32, C.c_int64]),
+        "devil_kernel_stats": (C.c_int, [_vp, C.c_int32, C.POINTER(C.c_int64),
+                                         C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)
@@ -95,9 +98,13 @@ def lib():
 EXPORTED = ["devil_create", "devil_destroy", "devil_strerror", "devil_last_error",
             "devil_version", "devil_fit", "devil_estimate_dispersion", "devil_init_beta",
             "devil_beta_fit", "devil_mom_dispersion", "devil_fit_dispersion_mle",
-            "devil_bread_meat", "devil_test_de", "devil_set_timing", "devil_stage_times"]
+            "devil_bread_meat", "devil_test_de", "devil_set_timing", "devil_stage_times",
+            "devil_kernel_stats", "devil_set_tuning"]
 
 STAGES = ["init", "irls", "dispersion", "bread_meat", "test_de"]
+KERNELS = ["irls_fp64", "irls_mixed"]
+TUNE_MIXED_KERNEL = 0
+MIXED_KERNELS = {"f32": 0, "bf16": 1}
 
 
 class Context:
@@ -126,6 +133,21 @@ class Context:
         arr = (C.c_double * len(STAGES))()
         self.check(lib().devil_stage_times(self.h, arr))
         return dict(zip(STAGES, list(arr)))
+
+    def set_mixed_kernel(self, name: str):
+        """Kernel of the mixed-precision IRLS: "f32" or "bf16"."""
+        self.check(lib().devil_set_tuning(self.h, TUNE_MIXED_KERNEL, MIXED_KERNELS[name]))
+
+    def kernel_stats(self) -> dict:
+        """{kernel: (launches, device ms, algorithmic bytes)} of the IRLS
+        accumulate kernels in the last devil_fit (timing enabled)."""
+        out = {}
+        for k, name in enumerate(KERNELS):
+            nl, ms, by = C.c_int64(), C.c_double(), C.c_double()
+            self.check(lib().devil_kernel_stats(self.h, k, C.byref(nl), C.byref(ms),
+                                                C.byref(by)))
+            out[name] = (int(nl.value), float(ms.value), float(by.value))
+        return out
 
     def close(self):
         if getattr(self, "h", None):

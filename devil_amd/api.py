@@ -68,7 +68,8 @@ def fit_devil(x, design_matrix, clusters=None, overdispersion: Any = "MOM",
 
     ``x`` is genes x cells.  ``overdispersion`` is "MOM", "MLE"/"old" or
     False.  ``batch_size`` is accepted for signature compatibility; the
-    library sizes its own gene batches.  ``precision`` is "fp64".
+    library sizes its own gene batches.  ``precision`` is "fp64" or "mixed"
+    (FP32 MFMA contractions with FP64 solves and stopping decisions).
     """
     if not CUDA:
         raise NotImplementedError("devil_amd only implements the GPU fit (CUDA=True)")

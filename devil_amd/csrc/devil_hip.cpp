@@ -21,7 +21,19 @@ struct devil_ctx {
   hipStream_t stream = nullptr;
   std::string err;
   int timing = 0;
+  int tune[DEVIL_N_TUNE] = {DEVIL_MIXED_BF16};
   double stage_ms[DEVIL_N_STAGES] = {0, 0, 0, 0, 0};
+  // per-launch records of the IRLS accumulate kernels (timing on)
+  struct Launch {
+    hipEvent_t a, b;
+    int kind;
+    double bytes;
+  };
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<Launch> launches;
+  int64_t k_n[DEVIL_N_KERNELS] = {0, 0};
+  double k_ms[DEVIL_N_KERNELS] = {0, 0};
+  double k_bytes[DEVIL_N_KERNELS] = {0, 0};
 };
 
 namespace {
@@ -294,19 +306,33 @@ struct StageTimer {
 };
 
 // Host construction of the MFMA operand layouts (dv::TileDesign).
-struct TileHost {
-  std::vector<double> Xp, offp, Kp;
-  std::vector<float> Xf, offf, Kf;  // mixed-precision operands (empty for FP64 only)
-  dv::TileDesign td{};
+enum MixedKernel { kMixedNone = 0, kMixedF32 = 1, kMixedBf16 = 2 };
+
+int mixed_kernel(const devil_ctx* ctx) {
+  return ctx->tune[DEVIL_TUNE_MIXED_KERNEL] == DEVIL_MIXED_F32 ? kMixedF32 : kMixedBf16;
+}
+
+struct ColSet {
+  std::vector<std::vector<double>> cols;
+  int kidx_pair[55];
+  int kidx_lin[10];
 };
 
-// Columns of K: the distinct x_a x_b (a <= b) and x_a of the design, exact
-// duplicates merged (an intercept makes x_0 x_a == x_a, a 0/1 column squares
-// to itself) and all-zero columns dropped.
-void build_tile_design(const double* X, const double* off, int64_t n, int p, TileHost& th,
-                       bool mixed = false) {
-  std::vector<std::vector<double>> cols;
+// Columns of K: x_a x_b (a <= b) and x_a of the design, exact duplicates
+// merged (an intercept makes x_0 x_a == x_a, a 0/1 column squares to itself)
+// and all-zero columns dropped.  lin16: the p design columns sit at 0..p-1
+// as they are and the pair columns start at column 16.
+void build_columns(const double* X, int64_t n, int p, bool lin16, ColSet& cs) {
   std::vector<uint64_t> hashes;
+  auto hash_of = [&](const std::vector<double>& v) {
+    uint64_t hsh = 1469598103934665603ULL;
+    for (double x : v) {
+      uint64_t u;
+      std::memcpy(&u, &x, 8);
+      hsh = (hsh ^ u) * 1099511628211ULL;
+    }
+    return hsh;
+  };
   auto add_col = [&](std::vector<double>&& v) -> int {
     bool zero = true;
     for (double x : v)
@@ -315,40 +341,102 @@ void build_tile_design(const double* X, const double* off, int64_t n, int p, Til
         break;
       }
     if (zero) return -1;
-    uint64_t hsh = 1469598103934665603ULL;
-    for (double x : v) {
-      uint64_t u;
-      std::memcpy(&u, &x, 8);
-      hsh = (hsh ^ u) * 1099511628211ULL;
-    }
-    for (size_t c = 0; c < cols.size(); ++c)
-      if (hashes[c] == hsh && std::memcmp(cols[c].data(), v.data(), sizeof(double) * n) == 0)
+    const uint64_t hsh = hash_of(v);
+    for (size_t c = 0; c < cs.cols.size(); ++c)
+      if (hashes[c] == hsh && std::memcmp(cs.cols[c].data(), v.data(), sizeof(double) * n) == 0)
         return (int)c;
-    cols.push_back(std::move(v));
+    cs.cols.push_back(std::move(v));
     hashes.push_back(hsh);
-    return (int)cols.size() - 1;
+    return (int)cs.cols.size() - 1;
   };
-  dv::TileDesign& td = th.td;
-  for (int a = 0; a < p; ++a) td.kidx_lin[a] = add_col(std::vector<double>(X + a * n, X + (a + 1) * n));
+  if (lin16) {
+    for (int a = 0; a < p; ++a) {
+      std::vector<double> v(X + a * n, X + (a + 1) * n);
+      hashes.push_back(hash_of(v));
+      cs.cols.push_back(std::move(v));
+      cs.kidx_lin[a] = a;
+    }
+    while (cs.cols.size() < 16) {
+      cs.cols.emplace_back(n, 0.0);
+      hashes.push_back(~0ULL);
+    }
+  } else {
+    for (int a = 0; a < p; ++a)
+      cs.kidx_lin[a] = add_col(std::vector<double>(X + a * n, X + (a + 1) * n));
+  }
   int t = 0;
   for (int a = 0; a < p; ++a)
     for (int b = a; b < p; ++b) {
       std::vector<double> v(n);
       for (int64_t i = 0; i < n; ++i) v[i] = X[i + a * n] * X[i + b * n];
-      td.kidx_pair[t++] = add_col(std::move(v));
+      cs.kidx_pair[t++] = add_col(std::move(v));
     }
-  for (int a = 0; a < p; ++a) {
-    long double s = 0.0L;
-    for (int64_t i = 0; i < n; ++i) s += X[i + a * n];
-    td.sx[a] = (double)s;
+}
+
+// Shape of a design with cells_per_block-cell blocks over the given columns.
+void shape_design(const ColSet& cs, int64_t n, int p, int cells_per_block, dv::TileDesign& td) {
+  std::memset(&td, 0, sizeof(td));
+  for (int a = 0; a < p; ++a) td.kidx_lin[a] = cs.kidx_lin[a];
+  for (int q = 0; q < p * (p + 1) / 2; ++q) td.kidx_pair[q] = cs.kidx_pair[q];
+  for (size_t c = 0; c < (size_t)DEVIL_NCP_MAX; ++c) {
+    long double sum = 0.0L;
+    if (c < cs.cols.size())
+      for (int64_t i = 0; i < n; ++i) sum += cs.cols[c][i];
+    td.ksum[c] = (double)sum;
   }
-  const int ncol = std::max<int>(1, (int)cols.size());
+  const int ncol = std::max<int>(1, (int)cs.cols.size());
   td.nt = (ncol + 15) / 16;
   td.ncp = 16 * td.nt;
   td.n = n;
-  td.nblk = (n + 15) / 16;
-  td.chunk_blocks = std::max<int64_t>(128, (td.nblk + 63) / 64);
+  td.nblk = (n + cells_per_block - 1) / cells_per_block;
+  const int64_t min_blocks = 2048 / cells_per_block;
+  td.chunk_blocks = std::max<int64_t>(min_blocks, (td.nblk + 63) / 64);
   td.nchunk = (int)((td.nblk + td.chunk_blocks - 1) / td.chunk_blocks);
+  if (td.nchunk >= 8) {
+    // a multiple of 8 chunks: with blocks dealt round-robin to the 8 XCDs and
+    // chunk = blockIdx % nchunk, each chunk's operands then live in one
+    // XCD's L2 (trailing chunks may come out empty)
+    td.nchunk = 8 * ((td.nchunk + 7) / 8);
+    td.chunk_blocks = (td.nblk + td.nchunk - 1) / td.nchunk;
+  }
+}
+
+uint16_t to_bf16(float x) {  // round to nearest even
+  uint32_t u;
+  std::memcpy(&u, &x, 4);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+float from_bf16(uint16_t b) {
+  const uint32_t u = (uint32_t)b << 16;
+  float x;
+  std::memcpy(&x, &u, 4);
+  return x;
+}
+
+// cell of MFMA k index (h, i) within a 32-cell step of the bf16 design
+int step_cell(int h, int i) { return i < 4 ? 4 * h + i : 16 + 4 * h + (i - 4); }
+
+struct TileHost {
+  dv::TileDesign td{};   // FP64 design (16-cell blocks, packed columns)
+  dv::TileDesign tdm{};  // mixed design (copy of td for FP32, 32-cell steps for bf16)
+  int mixed = kMixedNone;
+  std::vector<double> Xp, offp, Kp;
+  std::vector<float> Xf, offf, Kf;  // FP32 kernel operands
+  std::vector<uint16_t> xab, kh;    // bf16 kernel operands
+  std::vector<float> offl, kg;
+};
+
+void build_tile_design(const double* X, const double* off, int64_t n, int p, TileHost& th,
+                       int mixed = kMixedNone) {
+  th.mixed = mixed;
+  ColSet cs;
+  build_columns(X, n, p, false, cs);
+  dv::TileDesign& td = th.td;
+  shape_design(cs, n, p, 16, td);
+  const auto& cols = cs.cols;
   const int NJ = (p + 3) / 4;
   th.Xp.assign((size_t)td.nblk * NJ * 64, 0.0);
   th.offp.assign((size_t)td.nblk * 16, 0.0);
@@ -373,23 +461,68 @@ void build_tile_design(const double* X, const double* off, int64_t n, int p, Til
             th.Kp[((b * td.nt + tt) * 64 + l) * 4 + r] = cols[col][cell];
         }
   }
-  if (!mixed) return;
-  // FP32 copies; the eta product rows keep the natural cell order (the f32
-  // MFMA returns rows 4 (l / 16) + r in lane l)
-  th.Xf.assign((size_t)td.nblk * NJ * 64, 0.0f);
-  th.offf.assign(th.offp.begin(), th.offp.end());
-  th.Kf.assign(th.Kp.begin(), th.Kp.end());
-  for (int64_t b = 0; b < td.nblk; ++b)
-    for (int j = 0; j < NJ; ++j)
-      for (int l = 0; l < 64; ++l) {
-        const int64_t cell = 16 * b + l % 16;
-        const int f = 4 * j + l / 16;
-        if (cell < n && f < p) th.Xf[(b * NJ + j) * 64 + l] = (float)X[cell + f * n];
-      }
+  if (mixed == kMixedF32) {
+    // FP32 copies; the eta product rows keep the natural cell order (the f32
+    // MFMA returns rows 4 (l / 16) + r in lane l)
+    th.tdm = td;
+    th.Xf.assign((size_t)td.nblk * NJ * 64, 0.0f);
+    th.offf.assign(th.offp.begin(), th.offp.end());
+    th.Kf.assign(th.Kp.begin(), th.Kp.end());
+    for (int64_t b = 0; b < td.nblk; ++b)
+      for (int j = 0; j < NJ; ++j)
+        for (int l = 0; l < 64; ++l) {
+          const int64_t cell = 16 * b + l % 16;
+          const int f = 4 * j + l / 16;
+          if (cell < n && f < p) th.Xf[(b * NJ + j) * 64 + l] = (float)X[cell + f * n];
+        }
+  } else if (mixed == kMixedBf16) {
+    ColSet cm;
+    build_columns(X, n, p, true, cm);
+    dv::TileDesign& tm = th.tdm;
+    shape_design(cm, n, p, 32, tm);
+    const int64_t nst = tm.nblk;
+    const int NE = (7 * p + 31) / 32;
+    const int nth = tm.nt - 1;
+    const double L2E = 1.44269504088896340736;
+    th.xab.assign((size_t)nst * 2 * NE * 64 * 8, 0);
+    th.offl.assign((size_t)nst * 32, 0.0f);
+    th.kg.assign((size_t)nst * 64 * 8, 0.0f);
+    th.kh.assign((size_t)nst * std::max(nth, 0) * 64 * 8, 0);
+    for (int64_t i = 0; i < n; ++i) th.offl[i] = (float)(off[i] * L2E);
+    for (int64_t st = 0; st < nst; ++st) {
+      for (int u = 0; u < 2; ++u)
+        for (int j = 0; j < NE; ++j)
+          for (int l = 0; l < 64; ++l)
+            for (int i = 0; i < 8; ++i) {
+              const int64_t cell = st * 32 + 16 * u + l % 16;
+              const int e = 32 * j + 8 * (l / 16) + i;
+              const int f = e / 7, term = e % 7;
+              if (cell >= n || f >= p) continue;
+              const double x = X[cell + f * n];
+              const uint16_t x1 = to_bf16((float)x);
+              const uint16_t x2 = to_bf16((float)(x - (double)from_bf16(x1)));
+              th.xab[((((size_t)st * 2 + u) * NE + j) * 64 + l) * 8 + i] = term < 4 ? x1 : x2;
+            }
+      for (int l = 0; l < 64; ++l)
+        for (int i = 0; i < 8; ++i) {
+          const int64_t cell = st * 32 + step_cell(l / 16, i);
+          const int col = l % 16;
+          if (cell < n && col < p) th.kg[((size_t)st * 64 + l) * 8 + i] = (float)X[cell + col * n];
+        }
+      for (int t = 0; t < nth; ++t)
+        for (int l = 0; l < 64; ++l)
+          for (int i = 0; i < 8; ++i) {
+            const int64_t cell = st * 32 + step_cell(l / 16, i);
+            const int col = 16 * (t + 1) + l % 16;
+            if (cell < n && col < (int)cm.cols.size())
+              th.kh[(((size_t)st * nth + t) * 64 + l) * 8 + i] = to_bf16((float)cm.cols[col][cell]);
+          }
+    }
+  }
 }
 
 struct DeviceTile {
-  DevBuf Xp, offp, Kp, Xf, offf, Kf;
+  DevBuf Xp, offp, Kp, Xf, offf, Kf, xab, offl, kg, kh;
 };
 
 int upload_tile(devil_ctx* ctx, TileHost& th, DeviceTile& dt) {
@@ -400,14 +533,22 @@ int upload_tile(devil_ctx* ctx, TileHost& th, DeviceTile& dt) {
   th.td.Xp = dt.Xp.as<double>();
   th.td.offp = dt.offp.as<double>();
   th.td.Kp = dt.Kp.as<double>();
-  th.td.Xf = th.td.offf = th.td.Kf = nullptr;
-  if (!th.Xf.empty()) {
+  if (th.mixed == kMixedF32) {
     if ((rc = upload(ctx, dt.Xf, th.Xf))) return rc;
     if ((rc = upload(ctx, dt.offf, th.offf))) return rc;
     if ((rc = upload(ctx, dt.Kf, th.Kf))) return rc;
-    th.td.Xf = dt.Xf.as<float>();
-    th.td.offf = dt.offf.as<float>();
-    th.td.Kf = dt.Kf.as<float>();
+    th.tdm.Xf = dt.Xf.as<float>();
+    th.tdm.offf = dt.offf.as<float>();
+    th.tdm.Kf = dt.Kf.as<float>();
+  } else if (th.mixed == kMixedBf16) {
+    if ((rc = upload(ctx, dt.xab, th.xab))) return rc;
+    if ((rc = upload(ctx, dt.offl, th.offl))) return rc;
+    if ((rc = upload(ctx, dt.kg, th.kg))) return rc;
+    if (!th.kh.empty() && (rc = upload(ctx, dt.kh, th.kh))) return rc;
+    th.tdm.xab = dt.xab.as<uint16_t>();
+    th.tdm.offl = dt.offl.as<float>();
+    th.tdm.kg = dt.kg.as<float>();
+    th.tdm.kh = th.kh.empty() ? nullptr : dt.kh.as<uint16_t>();
   }
   return DEVIL_OK;
 }
@@ -416,16 +557,74 @@ int upload_tile(devil_ctx* ctx, TileHost& th, DeviceTile& dt) {
 // over the still-active genes until none is left (src/beta.cpp:37).
 // beta, k, iters, status are gene-major device arrays for these G genes.
 //
+// With timing on, each accumulate launch is bracketed by a pair of events;
+// the pairs are read once the caller has synchronised (collect_launches).
+hipEvent_t pool_event(devil_ctx* ctx) {
+  if (!ctx->ev_pool.empty()) {
+    hipEvent_t e = ctx->ev_pool.back();
+    ctx->ev_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+struct LaunchTimer {
+  devil_ctx* ctx;
+  hipEvent_t a = nullptr;
+  int kind;
+  double bytes;
+  LaunchTimer(devil_ctx* c, int k, double b) : ctx(c), kind(k), bytes(b) {
+    if (!ctx->timing) return;
+    a = pool_event(ctx);
+    if (a && hipEventRecord(a, ctx->stream) != hipSuccess) {
+      ctx->ev_pool.push_back(a);
+      a = nullptr;
+    }
+  }
+  ~LaunchTimer() {
+    if (!a) return;
+    hipEvent_t b = pool_event(ctx);
+    if (!b || hipEventRecord(b, ctx->stream) != hipSuccess) {
+      ctx->ev_pool.push_back(a);
+      if (b) ctx->ev_pool.push_back(b);
+      return;
+    }
+    ctx->launches.push_back({a, b, kind, bytes});
+  }
+};
+
+// after a stream synchronisation: fold the recorded launches into the stats
+void collect_launches(devil_ctx* ctx) {
+  for (auto& l : ctx->launches) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, l.a, l.b) == hipSuccess) {
+      ctx->k_n[l.kind] += 1;
+      ctx->k_ms[l.kind] += ms;
+      ctx->k_bytes[l.kind] += l.bytes;
+    }
+    ctx->ev_pool.push_back(l.a);
+    ctx->ev_pool.push_back(l.b);
+  }
+  ctx->launches.clear();
+}
+
+//
 // Mixed precision: each round accumulates in FP32 (FP64 partials); a gene
 // whose FP32-based step lands within kMixedBand * eps of the stopping
 // threshold (or is NaN) is not updated but recomputed in FP64 in the same
 // round, so stopping decisions and iteration counts follow FP64 arithmetic.
 constexpr double kMixedBand = 1e-3;
 
-int run_irls_tiled(devil_ctx* ctx, const dv::TileDesign& td, dv::Counts c, int64_t G, int p,
+int run_irls_tiled(devil_ctx* ctx, const TileHost& tl, dv::Counts c, int64_t G, int p,
                    double* beta, const double* kvec, int max_iter, double eps,
-                   int32_t* iters, int32_t* status, bool mixed = false) {
+                   int32_t* iters, int32_t* status) {
+  const dv::TileDesign& td = tl.td;
+  const dv::TileDesign& tdm = tl.tdm;
+  const bool mixed = tl.mixed != kMixedNone;
   hipStream_t s = ctx->stream;
+  const size_t es = count_elem_size(c.dtype);
   DV_HIP(dv::launch_fill_i32(s, iters, G, 0));
   DV_HIP(dv::launch_fill_i32(s, status, G, 0));
   if (max_iter <= 0 || G == 0) {
@@ -439,7 +638,9 @@ int run_irls_tiled(devil_ctx* ctx, const dv::TileDesign& td, dv::Counts c, int64
   if (mixed) DV_ALLOC(rchk, sizeof(int32_t) * G);
   int32_t* nrchk = cnt.as<int32_t>() + 2;
   const int64_t ntile = (G + 15) / 16;
-  DV_ALLOC(part, sizeof(double) * (size_t)ntile * td.nchunk * 16 * td.ncp);
+  size_t part_per_tile = (size_t)td.nchunk * 16 * td.ncp;
+  if (mixed) part_per_tile = std::max(part_per_tile, (size_t)tdm.nchunk * 16 * tdm.ncp);
+  DV_ALLOC(part, sizeof(double) * (size_t)ntile * part_per_tile);
   DV_HIP(dv::launch_iota(s, act0.as<int32_t>(), (int)G));
   int32_t* cur = act0.as<int32_t>();
   int32_t* nxt = act1.as<int32_t>();
@@ -449,22 +650,34 @@ int run_irls_tiled(devil_ctx* ctx, const dv::TileDesign& td, dv::Counts c, int64
   int nA = (int)G;
   for (int it = 0; it < max_iter && nA > 0; ++it) {
     int32_t h[2] = {0, 0};
+    const double row_bytes = (double)es * (double)td.n;
     if (!mixed) {
-      DV_HIP(dv::launch_irls_accum(s, td, c, p, cur, ncur, nA, beta, kvec, part.as<double>()));
+      {
+        LaunchTimer lt(ctx, DEVIL_KERNEL_IRLS_FP64, row_bytes * nA);
+        DV_HIP(dv::launch_irls_accum(s, td, c, p, cur, ncur, nA, beta, kvec, part.as<double>()));
+      }
       DV_HIP(dv::launch_fill_i32(s, nnxt, 1, 0));
       DV_HIP(dv::launch_irls_update(s, td, p, cur, ncur, nA, part.as<double>(), beta, kvec,
                                     max_iter, eps, iters, status, nxt, nnxt));
     } else {
-      DV_HIP(dv::launch_irls_accum_mixed(s, td, c, p, cur, ncur, nA, beta, kvec,
-                                         part.as<double>()));
+      {
+        LaunchTimer lt(ctx, DEVIL_KERNEL_IRLS_MIXED, row_bytes * nA);
+        if (tl.mixed == kMixedBf16)
+          DV_HIP(dv::launch_irls_accum_bf16(s, tdm, c, p, cur, ncur, nA, beta, kvec,
+                                            part.as<double>()));
+        else
+          DV_HIP(dv::launch_irls_accum_mixed(s, tdm, c, p, cur, ncur, nA, beta, kvec,
+                                             part.as<double>()));
+      }
       DV_HIP(dv::launch_fill_i32(s, nnxt, 1, 0));
       DV_HIP(dv::launch_fill_i32(s, nrchk, 1, 0));
-      DV_HIP(dv::launch_irls_update(s, td, p, cur, ncur, nA, part.as<double>(), beta, kvec,
+      DV_HIP(dv::launch_irls_update(s, tdm, p, cur, ncur, nA, part.as<double>(), beta, kvec,
                                     max_iter, eps, iters, status, nxt, nnxt, kMixedBand,
                                     rchk.as<int32_t>(), nrchk));
       DV_HIP(hipMemcpyAsync(&h[1], nrchk, sizeof(int32_t), hipMemcpyDeviceToHost, s));
       DV_HIP(hipStreamSynchronize(s));
       if (h[1] > 0) {
+        LaunchTimer lt(ctx, DEVIL_KERNEL_IRLS_FP64, row_bytes * h[1]);
         DV_HIP(dv::launch_irls_accum(s, td, c, p, rchk.as<int32_t>(), nrchk, h[1], beta, kvec,
                                      part.as<double>()));
         DV_HIP(dv::launch_irls_update(s, td, p, rchk.as<int32_t>(), nrchk, h[1],
@@ -474,6 +687,7 @@ int run_irls_tiled(devil_ctx* ctx, const dv::TileDesign& td, dv::Counts c, int64
     }
     DV_HIP(hipMemcpyAsync(&h[0], nnxt, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     DV_HIP(hipStreamSynchronize(s));
+    collect_launches(ctx);
     nA = h[0];
     std::swap(cur, nxt);
     std::swap(ncur, nnxt);
@@ -595,6 +809,11 @@ void devil_destroy(devil_ctx* ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipStreamDestroy(ctx->stream);
   }
+  for (auto& l : ctx->launches) {
+    (void)hipEventDestroy(l.a);
+    (void)hipEventDestroy(l.b);
+  }
+  for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
   delete ctx;
 }
 
@@ -642,8 +861,8 @@ int devil_fit(devil_ctx* ctx, const devil_problem* pr, devil_outputs* out) {
   if ((rc = upload_design(ctx, d, dd))) return rc;
   if (C > 0 && (rc = upload(ctx, dd.ends, ends_h))) return rc;
   TileHost tl;
-  const bool mixed = pr->precision == DEVIL_PREC_MIXED;
-  build_tile_design(d.X.data(), d.off.data(), n, p, tl, mixed);
+  build_tile_design(d.X.data(), d.off.data(), n, p, tl,
+                    pr->precision == DEVIL_PREC_MIXED ? mixed_kernel(ctx) : kMixedNone);
   DeviceTile dt;
   if ((rc = upload_tile(ctx, tl, dt))) return rc;
 
@@ -693,6 +912,10 @@ int devil_fit(devil_ctx* ctx, const devil_problem* pr, devil_outputs* out) {
   const double eps = (double)(float)pr->tolerance;  // src/beta.cpp:25 float eps
   const dv::InitParams ip = init_params(d, dd, pr->disp_init ? 0 : 1, init_mode);
   for (int k = 0; k < DEVIL_STAGE_TEST_DE; ++k) ctx->stage_ms[k] = 0.0;
+  for (int k = 0; k < DEVIL_N_KERNELS; ++k) {
+    ctx->k_n[k] = 0;
+    ctx->k_ms[k] = ctx->k_bytes[k] = 0.0;
+  }
   StageTimer tm(ctx);
   hipEvent_t ev;
   for (int64_t g0 = 0; g0 < G; g0 += batch) {
@@ -715,9 +938,8 @@ int devil_fit(devil_ctx* ctx, const devil_problem* pr, devil_outputs* out) {
     DV_HIP(dv::launch_k_from_alpha(s, a0, Gb, kvec.as<double>() + g0));
     tm.end(ev);
     ev = tm.begin(DEVIL_STAGE_IRLS);
-    if ((rc = run_irls_tiled(ctx, tl.td, c, Gb, p, b0, kvec.as<double>() + g0, pr->max_iter,
-                             eps, iters.as<int32_t>() + g0, status.as<int32_t>() + g0,
-                             mixed)))
+    if ((rc = run_irls_tiled(ctx, tl, c, Gb, p, b0, kvec.as<double>() + g0, pr->max_iter,
+                             eps, iters.as<int32_t>() + g0, status.as<int32_t>() + g0)))
       return rc;
     tm.end(ev);
     double* th = theta.as<double>() + g0;
@@ -773,6 +995,23 @@ int devil_set_timing(devil_ctx* ctx, int enable) {
 int devil_stage_times(const devil_ctx* ctx, double* ms) {
   if (!ctx || !ms) return DEVIL_E_ARG;
   for (int k = 0; k < DEVIL_N_STAGES; ++k) ms[k] = ctx->stage_ms[k];
+  return DEVIL_OK;
+}
+
+int devil_set_tuning(devil_ctx* ctx, int32_t key, int64_t value) {
+  if (!ctx || key < 0 || key >= DEVIL_N_TUNE) return DEVIL_E_ARG;
+  if (key == DEVIL_TUNE_MIXED_KERNEL && value != DEVIL_MIXED_F32 && value != DEVIL_MIXED_BF16)
+    return fail(ctx, DEVIL_E_ARG, "unknown mixed kernel");
+  ctx->tune[key] = (int)value;
+  return DEVIL_OK;
+}
+
+int devil_kernel_stats(const devil_ctx* ctx, int32_t kernel, int64_t* launches, double* ms,
+                       double* bytes) {
+  if (!ctx || kernel < 0 || kernel >= DEVIL_N_KERNELS) return DEVIL_E_ARG;
+  if (launches) *launches = ctx->k_n[kernel];
+  if (ms) *ms = ctx->k_ms[kernel];
+  if (bytes) *bytes = ctx->k_bytes[kernel];
   return DEVIL_OK;
 }
 
@@ -849,7 +1088,8 @@ int devil_beta_fit(devil_ctx* ctx, const void* counts, int32_t dtype, int64_t G,
   DV_HIP(hipMemcpy(Xh.data(), X, sizeof(double) * n * p, hipMemcpyDeviceToHost));
   DV_HIP(hipMemcpy(offh.data(), offset, sizeof(double) * n, hipMemcpyDeviceToHost));
   TileHost th;
-  build_tile_design(Xh.data(), offh.data(), n, p, th, precision == DEVIL_PREC_MIXED);
+  build_tile_design(Xh.data(), offh.data(), n, p, th,
+                    precision == DEVIL_PREC_MIXED ? mixed_kernel(ctx) : kMixedNone);
   DeviceTile dt;
   if ((rc = upload_tile(ctx, th, dt))) return rc;
   DevBuf st_local;
@@ -858,8 +1098,8 @@ int devil_beta_fit(devil_ctx* ctx, const void* counts, int32_t dtype, int64_t G,
     DV_ALLOC(st_local, sizeof(int32_t) * G);
     st = st_local.as<int32_t>();
   }
-  if ((rc = run_irls_tiled(ctx, th.td, c, G, p, bg.as<double>(), kv.as<double>(), max_iter,
-                           (double)(float)tolerance, iters, st, precision == DEVIL_PREC_MIXED)))
+  if ((rc = run_irls_tiled(ctx, th, c, G, p, bg.as<double>(), kv.as<double>(), max_iter,
+                           (double)(float)tolerance, iters, st)))
     return rc;
   DV_HIP(dv::launch_transpose_gp(s, bg.as<double>(), G, p, 0, beta));
   DV_HIP(hipStreamSynchronize(s));

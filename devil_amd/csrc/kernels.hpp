@@ -48,9 +48,18 @@ struct TileDesign {
   const float* Xf;       // [blk][j][lane] = X[16 blk + lane%16][4 j + lane/16]
   const float* offf;     // off padded with zeros to 16 nblk
   const float* Kf;       // [blk][t][lane][r] = K[16 blk + 4 (lane/16) + r][16 t + lane%16]
+  // bf16 mixed design: blocks are 32-cell steps, K column tile 0 holds the p
+  // design columns (FP32 products), tiles 1.. the pair columns (bf16 products);
+  // c(h, i) = i < 4 ? 4 h + i : 16 + 4 h + i - 4 is the cell of k index (h, i)
+  const uint16_t* xab;   // [st][u][j][lane][8] bf16 eta entries 32 j + 8 (lane/16) + i of
+                         // cell 32 st + 16 u + lane%16: per feature f, entries 7 f + t
+                         // hold (x1, x1, x1, x1, x2, x2, x2), x = x1 + x2
+  const float* offl;     // off * log2(e), padded with zeros to 32 nblk
+  const float* kg;       // [st][lane][8] = X[32 st + c(lane/16, i)][lane%16]
+  const uint16_t* kh;    // [st][t][lane][8] bf16 = K[32 st + c(lane/16, i)][16 (t+1) + lane%16]
   int kidx_pair[55];     // K column of x_a x_b (packed upper triangle), -1 = zero column
   int kidx_lin[10];      // K column of x_a, -1 = zero column
-  double sx[10];         // sum_i x_ia
+  double ksum[DEVIL_NCP_MAX];  // sum over cells of each K column
 };
 
 hipError_t launch_irls_accum(hipStream_t s, const TileDesign& td, Counts c, int p,
@@ -62,6 +71,11 @@ hipError_t launch_irls_accum(hipStream_t s, const TileDesign& td, Counts c, int 
 hipError_t launch_irls_accum_mixed(hipStream_t s, const TileDesign& td, Counts c, int p,
                                    const int32_t* active, const int32_t* nactive, int nA_bound,
                                    const double* beta, const double* kvec, double* part);
+
+// bf16 variant on the 32-cell-step design.
+hipError_t launch_irls_accum_bf16(hipStream_t s, const TileDesign& td, Counts c, int p,
+                                  const int32_t* active, const int32_t* nactive, int nA_bound,
+                                  const double* beta, const double* kvec, double* part);
 
 hipError_t launch_irls_update(hipStream_t s, const TileDesign& td, int p,
                               const int32_t* active, const int32_t* nactive, int nA_bound,

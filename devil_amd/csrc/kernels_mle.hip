@@ -170,7 +170,7 @@ __device__ __forceinline__ void mle_tile_chunk(const TileDesign& td, const CT* _
   const int64_t b0 = (int64_t)chunk * td.chunk_blocks;
   const int64_t b1 = min((int64_t)(chunk + 1) * td.chunk_blocks, td.nblk);
   BlockOps<NJ, NTMAX> cur, nxt;
-  load_block<NJ, NTMAX, CT>(td, yrow, vec, b0, lane, h, NT, cur);
+  if (b0 < b1) load_block<NJ, NTMAX, CT>(td, yrow, vec, b0, lane, h, NT, cur);  // trailing chunks may be empty
   for (int64_t b = b0; b < b1; ++b) {
     if (b + 1 < b1) load_block<NJ, NTMAX, CT>(td, yrow, vec, b + 1, lane, h, NT, nxt);
     d4 eta = d4{0.0, 0.0, 0.0, 0.0};

@@ -6,14 +6,15 @@
 //   k_irls_accum  one wavefront per (16-gene tile, cell chunk).  For every
 //                 16-cell block:
 //                   eta' = -X beta      v_mfma_f64_16x16x4f64, D[cell][gene]
-//                   w = exp(eta' - off), q = (k + y) w / (1 + k w)      (VALU)
-//                   acc += q' K         v_mfma_f64_16x16x4f64 per 16 columns
+//                   w = exp(eta' - off), r = m w - 1 = (y w - 1) / (1 + k w)
+//                   acc += r' K         v_mfma_f64_16x16x4f64 per 16 columns
 //                 where K holds the distinct columns x_a x_b (a <= b) and x_a
 //                 of the design (the Khatri-Rao product, deduplicated on the
 //                 host).  The tile's 16 x NCP partial sums are written per
 //                 chunk.
 //   k_irls_update one wavefront per active gene: sums its chunk partials in
-//                 chunk order, forms A = k X'QX and g = k (X'q - X'1), solves
+//                 chunk order, forms A = k (X'diag(r)X + X'X) and g = k X'r
+//                 (the column sums of K are precomputed in FP64), solves
 //                 with LU (partial pivoting), updates beta and applies the
 //                 reference stopping rules; unconverged genes are appended to
 //                 the next active list.
@@ -69,18 +70,20 @@ __device__ __forceinline__ void irls_tile_chunk(const TileDesign& td, const CT* 
   const int64_t b0 = (int64_t)chunk * td.chunk_blocks;
   const int64_t b1 = min((int64_t)(chunk + 1) * td.chunk_blocks, td.nblk);
   BlockOps<NJ, NTMAX> cur, nxt;
-  load_block<NJ, NTMAX, CT>(td, yrow, vec, b0, lane, h, NT, cur);
+  if (b0 < b1) load_block<NJ, NTMAX, CT>(td, yrow, vec, b0, lane, h, NT, cur);  // trailing chunks may be empty
   for (int64_t b = b0; b < b1; ++b) {
     if (b + 1 < b1) load_block<NJ, NTMAX, CT>(td, yrow, vec, b + 1, lane, h, NT, nxt);
     d4 eta = d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int j = 0; j < NJ; ++j) eta = mfma16x16x4(cur.xa[j], bneg[j], eta);
+    // m w - 1 = (y w - 1) / (1 + k w), summed without the n-term
+    // cancellation of sum (m w) - sum 1
     double q[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const double w = exp_f64(eta[r] - cur.off[r]);
       const double d = fma(k, w, 1.0);
-      q[r] = ((k + cur.y[r]) * w) * rcp1_f64(d);
+      q[r] = fma(cur.y[r], w, -1.0) * rcp1_f64(d);
     }
 #pragma unroll
     for (int t = 0; t < NTMAX; ++t) {
@@ -105,10 +108,10 @@ __device__ __forceinline__ void irls_tile_chunk(const TileDesign& td, const CT* 
 // ---------------------------------------------------------------------------
 // Mixed precision accumulate.  Same tile / chunk / partial-sum structure; per
 // element in FP32: mu = 2^((x'beta + off) log2 e) (v_exp_f32),
-// q = (k + y) / (k + mu) (v_rcp_f32) -- the reference's m w written with mu,
-// which keeps the right limits where FP32 mu under- or overflows -- and
-// q' K on v_mfma_f32_16x16x4f32 with FP32 accumulators flushed into FP64
-// every 16 blocks (256 cells).
+// r = (y - mu) / (k + mu) (v_rcp_f32) -- the reference's m w - 1 written with
+// mu; mu = 0 gives y / k as in FP64, an overflowed mu gives NaN, which sends
+// the gene to the FP64 recheck -- and r' K on v_mfma_f32_16x16x4f32 with FP32
+// accumulators flushed into FP64 every 4 blocks (64 cells).
 // v_mfma_f32_16x16x4f32 (checked on gfx950): lane l supplies A[l%16][l/16]
 // and B[l/16][l%16]; register r of the result holds D[4 (l/16) + r][l%16].
 // ---------------------------------------------------------------------------
@@ -162,18 +165,23 @@ __device__ __forceinline__ void irls_tile_chunk_mixed(const TileDesign& td,
                                                       int chunk, double* __restrict__ part) {
   constexpr int NJ = (P + 3) / 4;
   constexpr int NTMAX = ((P * (P + 1)) / 2 + P + 15) / 16;
-  constexpr int kSpan = 16;  // blocks per FP32 accumulation span
+  constexpr int kSpan = 4;  // blocks per FP32 accumulation span (64 cells)
   const int lane = threadIdx.x & 63;
   const int s = lane & 15, h = lane >> 4;
   const int slot = tile * 16 + s;
   const bool live = slot < nA;
   const int g = live ? active[slot] : active[tile * 16];
   const int NT = td.nt;
-  float bpos[NJ];
+  // beta = bhi + blo in two floats: rounding beta to one float would shift
+  // every gene's evaluation point by 2^-24 |beta| per component, and through
+  // the iteration's coupling that reaches the small coefficients
+  float bhi[NJ], blo[NJ];
 #pragma unroll
   for (int j = 0; j < NJ; ++j) {
     const int f = 4 * j + h;
-    bpos[j] = (live && f < P) ? (float)beta[(int64_t)g * P + f] : 0.0f;
+    const double bd = (live && f < P) ? beta[(int64_t)g * P + f] : 0.0;
+    bhi[j] = (float)bd;
+    blo[j] = (float)(bd - (double)bhi[j]);
   }
   const float k = live ? (float)kvec[g] : 0.0f;  // k is a float in the reference
   const CT* yrow = Y + (int64_t)g * ld;
@@ -189,18 +197,23 @@ __device__ __forceinline__ void irls_tile_chunk_mixed(const TileDesign& td,
   const int64_t b0 = (int64_t)chunk * td.chunk_blocks;
   const int64_t b1 = min((int64_t)(chunk + 1) * td.chunk_blocks, td.nblk);
   BlockOpsF<NJ, NTMAX> cur, nxt;
-  load_block_f<NJ, NTMAX, CT>(td, yrow, vec, b0, lane, h, NT, cur);
+  if (b0 < b1) load_block_f<NJ, NTMAX, CT>(td, yrow, vec, b0, lane, h, NT, cur);  // trailing chunks may be empty
   int span = 0;
   for (int64_t b = b0; b < b1; ++b) {
     if (b + 1 < b1) load_block_f<NJ, NTMAX, CT>(td, yrow, vec, b + 1, lane, h, NT, nxt);
     f4 eta = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int j = 0; j < NJ; ++j) eta = mfma32(cur.xa[j], bpos[j], eta);
+    for (int j = 0; j < NJ; ++j) {
+      eta = mfma32(cur.xa[j], blo[j], eta);
+      eta = mfma32(cur.xa[j], bhi[j], eta);
+    }
+    // m w - 1 = (y - mu) / (k + mu): no cancellation against the design sums,
+    // and relative FP32 error even where mu << k
     float q[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const float mu = __builtin_amdgcn_exp2f((eta[r] + cur.off[r]) * 1.44269504088896341f);
-      q[r] = (k + cur.y[r]) * __builtin_amdgcn_rcpf(k + mu);
+      q[r] = (cur.y[r] - mu) * __builtin_amdgcn_rcpf(k + mu);
     }
 #pragma unroll
     for (int t = 0; t < NTMAX; ++t) {
@@ -253,6 +266,192 @@ __global__ __launch_bounds__(256) void k_irls_accum_mixed(TileDesign td, Counts 
   else
     irls_tile_chunk_mixed<P>(td, static_cast<const double*>(c.ptr), c.ld, active, nA, beta,
                              kvec, tile, chunk, part);
+}
+
+// ---------------------------------------------------------------------------
+// Mixed precision accumulate, bf16 variant (gfx950 v_mfma_f32_16x16x32_bf16).
+// A step is 32 cells (two 16-cell halves); lane l = (gene slot s = l % 16,
+// group h = l / 16) owns cells 4h + r and 16 + 4h + r, r < 4.
+//   eta: per half, sum over split entries of A (cells x 32 entries) B (entries
+//        x genes): x = x1 + x2 and log2(e) beta = b1 + b2 + b3 + b4 in bf16,
+//        entries (x1 b1, x1 b2, x1 b3, x1 b4, x2 b1, x2 b2, x2 b3) per feature,
+//        FP32 accumulation; register r holds cell 4h + r of the half.
+//   r = (y - mu) / (k + mu), mu = 2^(eta + off log2 e)             (FP32 VALU)
+//   gradient tile (the p design columns): r' X on v_mfma_f32_16x16x4f32, one
+//        product per cell (k index h, step j -> cell c(h, j))
+//   pair tiles (x_a x_b): bf16(r)' K on v_mfma_f32_16x16x32_bf16, k index
+//        (h, i) -> cell c(h, i) = i < 4 ? 4h + i : 16 + 4h + i - 4
+// FP32 accumulators are flushed to FP64 every 8 steps (256 cells).
+// ---------------------------------------------------------------------------
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ f4 mfma_bf(bf8 a, bf8 b, f4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+template <int NE, int NTH>
+struct StepOpsB {
+  float y[8];
+  f4 off[2];
+  bf8 xa[2][NE];
+  f4 kg[2];
+  bf8 kh[NTH];
+};
+
+// host-side operand tables of the bf16 design (per 32-cell step st):
+//   xab [st][u][j][lane] bf8 : entries 32 j + 8 (lane/16) + i of cell 32 st + 16 u + lane % 16
+//   offl[32 nstep]       f32 : off * log2(e)
+//   kg  [st][lane][8]    f32 : x_{lane%16}[c(lane/16, j)]
+//   kh  [st][t][lane]    bf8 : Kpair_{16 t + lane%16}[c(lane/16, i)]
+template <int NE, int NTH, typename CT>
+__device__ __forceinline__ void load_step_b(const TileDesign& td, const CT* yrow, bool vec,
+                                            int64_t st, int lane, int h, int nth,
+                                            StepOpsB<NE, NTH>& o) {
+  const int64_t c0 = st * 32 + 4 * h;
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int64_t c = c0 + 16 * u;
+    double t[4];
+    if (c + 4 <= td.n) {
+      load4<CT>(yrow + c, vec, t);
+    } else {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) t[r] = (c + r < td.n) ? (double)yrow[c + r] : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) o.y[4 * u + r] = (float)t[r];
+    o.off[u] = *reinterpret_cast<const f4*>(td.offl + c);
+#pragma unroll
+    for (int j = 0; j < NE; ++j)
+      o.xa[u][j] = reinterpret_cast<const bf8*>(td.xab)[((st * 2 + u) * NE + j) * 64 + lane];
+  }
+  const f4* kg = reinterpret_cast<const f4*>(td.kg) + (st * 64 + lane) * 2;
+  o.kg[0] = kg[0];
+  o.kg[1] = kg[1];
+#pragma unroll
+  for (int t = 0; t < NTH; ++t)
+    if (t < nth) o.kh[t] = reinterpret_cast<const bf8*>(td.kh)[(st * nth + t) * 64 + lane];
+}
+
+template <int P, typename CT>
+__device__ __forceinline__ void irls_tile_chunk_bf(const TileDesign& td, const CT* __restrict__ Y,
+                                                   int64_t ld, const int32_t* __restrict__ active,
+                                                   int nA, const double* __restrict__ beta,
+                                                   const double* __restrict__ kvec, int tile,
+                                                   int chunk, double* __restrict__ part) {
+  constexpr int NE = (7 * P + 31) / 32;
+  constexpr int NTH = ((P * (P + 1)) / 2 + 15) / 16;  // pair tiles at most
+  constexpr int kSpan = 8;                            // steps per FP32 span
+  const int lane = threadIdx.x & 63;
+  const int s = lane & 15, h = lane >> 4;
+  const int slot = tile * 16 + s;
+  const bool live = slot < nA;
+  const int g = live ? active[slot] : active[tile * 16];
+  const int nth = td.nt - 1;
+  // B operand of eta: entries 32 j + 8 h + i of gene slot s
+  bf8 bb[NE];
+#pragma unroll
+  for (int j = 0; j < NE; ++j) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int e = 32 * j + 8 * h + i;
+      const int f = e / 7, term = e % 7;
+      float v = 0.0f;
+      if (live && f < P) {
+        double rem = beta[(int64_t)g * P + f] * 1.44269504088896340736;
+        const int want = term < 4 ? term : term - 4;
+        float piece = 0.0f;
+        for (int q = 0; q <= want; ++q) {
+          piece = (float)(__bf16)(float)rem;
+          rem -= (double)piece;
+        }
+        v = piece;
+      }
+      bb[j][i] = (__bf16)v;
+    }
+  }
+  const float k = live ? (float)kvec[g] : 0.0f;
+  const CT* yrow = Y + (int64_t)g * ld;
+  const bool vec = (ld & 3) == 0 && ((reinterpret_cast<uintptr_t>(Y) & 15) == 0);
+  f4 acc[NTH + 1];
+  double acc64[NTH + 1][4];
+#pragma unroll
+  for (int t = 0; t <= NTH; ++t) {
+    acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int r = 0; r < 4; ++r) acc64[t][r] = 0.0;
+  }
+  const int64_t s0 = (int64_t)chunk * td.chunk_blocks;
+  const int64_t s1 = min((int64_t)(chunk + 1) * td.chunk_blocks, td.nblk);
+  StepOpsB<NE, NTH> cur, nxt;
+  if (s0 < s1) load_step_b<NE, NTH, CT>(td, yrow, vec, s0, lane, h, nth, cur);  // trailing chunks may be empty
+  int span = 0;
+  for (int64_t st = s0; st < s1; ++st) {
+    if (st + 1 < s1) load_step_b<NE, NTH, CT>(td, yrow, vec, st + 1, lane, h, nth, nxt);
+    float rr[8];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      f4 eta = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int j = 0; j < NE; ++j) eta = mfma_bf(cur.xa[u][j], bb[j], eta);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const float mu = __builtin_amdgcn_exp2f(eta[r] + cur.off[u][r]);
+        rr[4 * u + r] = (cur.y[4 * u + r] - mu) * __builtin_amdgcn_rcpf(k + mu);
+      }
+    }
+    // gradient tile: FP32 products, cell c(h, j) for step j
+#pragma unroll
+    for (int j = 0; j < 8; ++j) acc[0] = mfma32(rr[j], cur.kg[j >> 2][j & 3], acc[0]);
+    bf8 rb;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) rb[i] = (__bf16)rr[i];
+#pragma unroll
+    for (int t = 0; t < NTH; ++t)
+      if (t < nth) acc[t + 1] = mfma_bf(rb, cur.kh[t], acc[t + 1]);
+    if (++span == kSpan || st + 1 == s1) {
+      span = 0;
+#pragma unroll
+      for (int t = 0; t <= NTH; ++t) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc64[t][r] += (double)acc[t][r];
+        acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+      }
+    }
+    cur = nxt;
+  }
+  double* out = part + ((int64_t)tile * td.nchunk + chunk) * 16 * td.ncp;
+#pragma unroll
+  for (int t = 0; t <= NTH; ++t) {
+    if (t <= nth) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) out[(4 * h + r) * td.ncp + 16 * t + s] = acc64[t][r];
+    }
+  }
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void k_irls_accum_bf16(TileDesign td, Counts c,
+                                                         const int32_t* __restrict__ active,
+                                                         const int32_t* __restrict__ nactive,
+                                                         const double* __restrict__ beta,
+                                                         const double* __restrict__ kvec,
+                                                         double* __restrict__ part) {
+  const int nA = *nactive;
+  const int ntile = (nA + 15) / 16;
+  const int wave = threadIdx.x >> 6;
+  const int chunk = blockIdx.x % td.nchunk;
+  const int tile = (blockIdx.x / td.nchunk) * 4 + wave;
+  if (tile >= ntile) return;
+  if (c.dtype == 0)
+    irls_tile_chunk_bf<P>(td, static_cast<const int32_t*>(c.ptr), c.ld, active, nA, beta, kvec,
+                          tile, chunk, part);
+  else if (c.dtype == 1)
+    irls_tile_chunk_bf<P>(td, static_cast<const float*>(c.ptr), c.ld, active, nA, beta, kvec,
+                          tile, chunk, part);
+  else
+    irls_tile_chunk_bf<P>(td, static_cast<const double*>(c.ptr), c.ld, active, nA, beta, kvec,
+                          tile, chunk, part);
 }
 
 template <int P>
@@ -315,11 +514,11 @@ __global__ __launch_bounds__(64) void k_irls_update(TileDesign td,
 #pragma unroll
   for (int a = 0; a < P; ++a) {
     const int gi = td.kidx_lin[a];
-    v[a] = k * ((gi >= 0 ? acc[gi] : 0.0) - td.sx[a]);
+    v[a] = k * (gi >= 0 ? acc[gi] : 0.0);
 #pragma unroll
     for (int bb = a; bb < P; ++bb) {
       const int pi = td.kidx_pair[tri<P>(a, bb)];
-      const double e = k * (pi >= 0 ? acc[pi] : 0.0);
+      const double e = k * (pi >= 0 ? acc[pi] + td.ksum[pi] : 0.0);
       M[a][bb] = e;
       M[bb][a] = e;
     }
@@ -397,6 +596,17 @@ hipError_t launch_irls_accum_mixed(hipStream_t s, const TileDesign& td, Counts c
   const int groups = (ntile + 3) / 4;
   const dim3 grid((unsigned)((int64_t)groups * td.nchunk));
   DV_TILE_P(p, k_irls_accum_mixed, grid, dim3(256), s, td, c, active, nactive, beta, kvec, part);
+  return hipGetLastError();
+}
+
+hipError_t launch_irls_accum_bf16(hipStream_t s, const TileDesign& td, Counts c, int p,
+                                  const int32_t* active, const int32_t* nactive, int nA_bound,
+                                  const double* beta, const double* kvec, double* part) {
+  if (nA_bound <= 0) return hipSuccess;
+  const int ntile = (nA_bound + 15) / 16;
+  const int groups = (ntile + 3) / 4;
+  const dim3 grid((unsigned)((int64_t)groups * td.nchunk));
+  DV_TILE_P(p, k_irls_accum_bf16, grid, dim3(256), s, td, c, active, nactive, beta, kvec, part);
   return hipGetLastError();
 }
 

@@ -182,6 +182,23 @@ int devil_set_timing(devil_ctx *ctx, int enable);
  * last devil_test_de (stage 4); ms must hold DEVIL_N_STAGES doubles. */
 int devil_stage_times(const devil_ctx *ctx, double *ms);
 
+/* Kernel choices of the mixed-precision path (set per context; the library
+ * defaults are what the autotuner found best on gfx950). */
+enum { DEVIL_TUNE_MIXED_KERNEL = 0, DEVIL_N_TUNE = 1 };
+enum { DEVIL_MIXED_F32 = 0,   /* eta and all contractions on v_mfma_f32_16x16x4f32 */
+       DEVIL_MIXED_BF16 = 1   /* eta (split operands) and pair columns on
+                                 v_mfma_f32_16x16x32_bf16, design columns FP32 */ };
+int devil_set_tuning(devil_ctx *ctx, int32_t key, int64_t value);
+
+/* Per-launch timing of the IRLS accumulate kernels (the streaming pass over
+ * the counts), kept while timing is enabled and reset by each devil_fit. */
+enum { DEVIL_KERNEL_IRLS_FP64 = 0, DEVIL_KERNEL_IRLS_MIXED = 1, DEVIL_N_KERNELS = 2 };
+/* launches: number of launches; ms: summed device milliseconds (HIP events
+ * on the context's stream around each launch); bytes: summed algorithmic
+ * bytes (count element size x n x active genes of each launch). */
+int devil_kernel_stats(const devil_ctx *ctx, int32_t kernel, int64_t *launches, double *ms,
+                       double *bytes);
+
 #ifdef __cplusplus
 }
 #endif
