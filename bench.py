@@ -107,6 +107,8 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=1,
                     help="timed FP64 steps on the same counts (0: no comparison)")
     ap.add_argument("--cpu-genes", type=int, default=None)
+    ap.add_argument("--mixed-kernel", default=None, choices=["f32", "bf16"],
+                    help="mixed IRLS kernel (default: the library's choice)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -132,6 +134,8 @@ def main():
     contrast[1] = 1.0
     ctx = devil_amd.context(0)
     ctx.set_timing(True)
+    if args.mixed_kernel:
+        ctx.set_mixed_kernel(args.mixed_kernel)
 
     def step(prec):
         fit = devil_amd.fit_devil(Yd, w.X, clusters=w.clusters, overdispersion=w.od,
@@ -197,6 +201,7 @@ def main():
                                + (", clustered sandwich" if w.clusters is not None else "")
                                + ", Wald test + BH",
                    "G": w.G, "n": w.n, "p": w.p, "precision": args.precision,
+                   "mixed_kernel": args.mixed_kernel or "library default",
                    "overdispersion": w.od},
         "roofline": {"bound": "hbm", "kernel": kdesc,
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

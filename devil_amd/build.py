@@ -20,7 +20,8 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libdevil_hip.so")
 ROOT = os.path.dirname(HERE)
 
-SOURCES = ["kernels_wave.hip", "kernels_tile.hip", "kernels_mle.hip", "devil_hip.cpp"]
+SOURCES = ["kernels_wave.hip", "kernels_tile.hip", "kernels_mle.hip", "kernels_post.hip",
+           "kernels_design.hip", "devil_hip.cpp"]
 
 COMMON = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall",

@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/devil_hip.h"
@@ -21,7 +22,7 @@ struct devil_ctx {
   hipStream_t stream = nullptr;
   std::string err;
   int timing = 0;
-  int tune[DEVIL_N_TUNE] = {DEVIL_MIXED_BF16};
+  int tune[DEVIL_N_TUNE] = {DEVIL_MIXED_F32};
   double stage_ms[DEVIL_N_STAGES] = {0, 0, 0, 0, 0};
   // per-launch records of the IRLS accumulate kernels (timing on)
   struct Launch {
@@ -83,6 +84,40 @@ struct DevBuf {
 
 size_t count_elem_size(int dtype) { return dtype == DEVIL_COUNTS_F64 ? 8 : 4; }
 
+// Host loops over cells split across threads ([lo, hi) ranges).
+template <typename F>
+void parallel_for(int64_t n, F&& fn) {
+  const int64_t hw = std::max<int64_t>(1, std::thread::hardware_concurrency());
+  const int64_t T = std::min<int64_t>({hw, 16, std::max<int64_t>(1, n / 65536)});
+  if (T <= 1) {
+    fn((int64_t)0, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  th.reserve(T);
+  for (int64_t t = 0; t < T; ++t) th.emplace_back([&, t] { fn(n * t / T, n * (t + 1) / T); });
+  for (auto& x : th) x.join();
+}
+
+// Upper Cholesky factor of the p x p SPD matrix A (column-major): A = R'R.
+bool cholesky_upper(const double* A, int p, double* R) {
+  for (int j = 0; j < p; ++j)
+    for (int i = 0; i < p; ++i) R[i + j * p] = 0.0;
+  for (int j = 0; j < p; ++j) {
+    double d = A[j + j * p];
+    for (int k = 0; k < j; ++k) d -= R[k + j * p] * R[k + j * p];
+    if (!(d > 0.0)) return false;
+    const double r = std::sqrt(d);
+    R[j + j * p] = r;
+    for (int i = j + 1; i < p; ++i) {
+      double s = A[j + i * p];
+      for (int k = 0; k < j; ++k) s -= R[k + j * p] * R[k + i * p];
+      R[j + i * p] = s / r;
+    }
+  }
+  return true;
+}
+
 // Householder QR of the n x p column-major X: thin Q (n x p) and upper R.
 // Returns false when a diagonal of R is below 1e-7 of the largest, the rank
 // tolerance R's qr() uses (LINPACK dqrdc2 tol = 1e-7).
@@ -133,45 +168,149 @@ bool householder_qr(const double* X, int64_t n, int p, std::vector<double>& Q,
   return ok;
 }
 
+
+template <typename T>
+int upload(devil_ctx* ctx, DevBuf& b, const std::vector<T>& v) {
+  DV_ALLOC(b, sizeof(T) * v.size());
+  if (!v.empty())
+    DV_HIP(hipMemcpyAsync(b.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice,
+                          ctx->stream));
+  return DEVIL_OK;
+}
+
+int upload_raw(devil_ctx* ctx, DevBuf& b, const void* src, size_t bytes, hipMemcpyKind kind) {
+  DV_ALLOC(b, bytes);
+  if (bytes) DV_HIP(hipMemcpyAsync(b.p, src, bytes, kind, ctx->stream));
+  return DEVIL_OK;
+}
+
 // Host copies of the small per-cell inputs plus what the init pass needs.
 struct Design {
   int64_t n = 0;
   int p = 0;
-  std::vector<double> X, off, eo, logsf, Q, R, xmean, denom;
+  std::vector<double> X;  // host copy of X, made only when a host step needs it
+  std::vector<double> off, eo, logsf, R, xmean, denom;
   double moi = 0.0, sum_sf = 0.0;
   int intercept_first = 0;
 };
 
-// Device copies of the design, offsets and cluster ends.
+// Device copies of the design, offsets and init operands.
 struct DeviceDesign {
-  DevBuf X, off, eo, Q, R, logsf, xmean, denom, ends;
+  DevBuf X, off, eo, Q, R, logsf, xmean, denom;
+  const double* x = nullptr;  // device X (the caller's own when it is device memory)
 };
 
+void combine_gram(const std::vector<double>& part, int p, std::vector<double>& G) {
+  G.assign((size_t)p * p, 0.0);
+  int q = 0;
+  for (int a = 0; a < p; ++a)
+    for (int b = a; b < p; ++b, ++q) {
+      long double su = 0.0L;
+      for (int x = 0; x < dv::kStatBlocksHost; ++x) su += part[x * 55 + q];
+      G[a + b * p] = G[b + a * p] = (double)su;
+    }
+}
+
+// Thin QR of the device X (n x p) by Cholesky QR applied twice on the GPU:
+// X'X = R1'R1, Q1 = X R1^-1, Q1'Q1 = R2'R2, Q = Q1 R2^-1, R = R2 R1.  ok is
+// false when a Gram matrix is not positive definite or a diagonal of R is
+// below 1e-7 of the largest (the rank tolerance of R's qr(), LINPACK dqrdc2).
+int qr_device(devil_ctx* ctx, const double* Xd, int64_t n, int p, DevBuf& Q,
+              std::vector<double>& R, bool& ok) {
+  hipStream_t s = ctx->stream;
+  ok = false;
+  DV_ALLOC(Q, sizeof(double) * n * p);
+  DV_HIP(hipMemcpyAsync(Q.p, Xd, sizeof(double) * n * p, hipMemcpyDeviceToDevice, s));
+  DevBuf part, Rd;
+  DV_ALLOC(part, sizeof(double) * dv::kStatBlocksHost * 55);
+  DV_ALLOC(Rd, sizeof(double) * p * p);
+  std::vector<double> hp((size_t)dv::kStatBlocksHost * 55), G, R1((size_t)p * p),
+      R2((size_t)p * p);
+  auto gram = [&]() -> int {
+    DV_HIP(dv::launch_gram(s, Q.as<double>(), n, p, part.as<double>()));
+    DV_HIP(hipMemcpyAsync(hp.data(), part.p, sizeof(double) * hp.size(), hipMemcpyDeviceToHost,
+                          s));
+    DV_HIP(hipStreamSynchronize(s));
+    combine_gram(hp, p, G);
+    return DEVIL_OK;
+  };
+  int rc;
+  if ((rc = gram())) return rc;
+  if (!cholesky_upper(G.data(), p, R1.data())) return DEVIL_OK;
+  DV_HIP(hipMemcpyAsync(Rd.p, R1.data(), sizeof(double) * p * p, hipMemcpyHostToDevice, s));
+  DV_HIP(dv::launch_right_solve(s, Q.as<double>(), n, p, Rd.as<double>()));
+  if ((rc = gram())) return rc;
+  if (!cholesky_upper(G.data(), p, R2.data())) return DEVIL_OK;
+  DV_HIP(hipMemcpyAsync(Rd.p, R2.data(), sizeof(double) * p * p, hipMemcpyHostToDevice, s));
+  DV_HIP(dv::launch_right_solve(s, Q.as<double>(), n, p, Rd.as<double>()));
+  DV_HIP(hipStreamSynchronize(s));
+  R.assign((size_t)p * p, 0.0);
+  for (int j = 0; j < p; ++j)
+    for (int i = 0; i <= j; ++i) {
+      double su = 0.0;
+      for (int k = i; k <= j; ++k) su += R2[i + k * p] * R1[k + j * p];
+      R[i + j * p] = su;
+    }
+  double dmax = 0.0;
+  for (int a = 0; a < p; ++a) dmax = std::max(dmax, std::fabs(R[a + a * p]));
+  ok = true;
+  for (int a = 0; a < p; ++a)
+    if (!(std::fabs(R[a + a * p]) > 1e-7 * dmax)) ok = false;
+  return DEVIL_OK;
+}
+
+int host_copy_of_x(devil_ctx* ctx, const double* X, int mem, Design& d) {
+  if (!d.X.empty()) return DEVIL_OK;
+  d.X.resize((size_t)d.n * d.p);
+  if (mem == DEVIL_MEM_DEVICE)
+    DV_HIP(hipMemcpy(d.X.data(), X, sizeof(double) * d.n * d.p, hipMemcpyDeviceToHost));
+  else
+    std::memcpy(d.X.data(), X, sizeof(double) * d.n * d.p);
+  return DEVIL_OK;
+}
+
+// Offsets, exp(offset) and the init operands on host and device; X stays on
+// the device (the caller's pointer when it is device memory).
 int prepare_design(devil_ctx* ctx, const double* X, const double* off, int64_t n, int p,
-                   int mem, int init_mode, Design& d) {
+                   int mem, int init_mode, Design& d, DeviceDesign& dd) {
   d.n = n;
   d.p = p;
-  d.X.resize((size_t)n * p);
   d.off.resize(n);
-  if (mem == DEVIL_MEM_DEVICE) {
-    DV_HIP(hipMemcpy(d.X.data(), X, sizeof(double) * n * p, hipMemcpyDeviceToHost));
+  if (mem == DEVIL_MEM_DEVICE)
     DV_HIP(hipMemcpy(d.off.data(), off, sizeof(double) * n, hipMemcpyDeviceToHost));
-  } else {
-    std::memcpy(d.X.data(), X, sizeof(double) * n * p);
+  else
     std::memcpy(d.off.data(), off, sizeof(double) * n);
-  }
   d.eo.resize(n);
+  parallel_for(n, [&](int64_t lo, int64_t hi) {
+    for (int64_t i = lo; i < hi; ++i) d.eo[i] = std::exp(d.off[i]);  // exp_offset, R/main.R:390
+  });
   double se = 0.0;
-  for (int64_t i = 0; i < n; ++i) {
-    d.eo[i] = std::exp(d.off[i]);  // exp_offset, R/main.R:390
-    se += d.eo[i];
-  }
+  for (int64_t i = 0; i < n; ++i) se += d.eo[i];
   d.moi = 1.0 / (se / (double)n);  // R/overdispersion.R:130
+  int rc;
+  if (mem == DEVIL_MEM_DEVICE) {
+    dd.x = X;
+  } else {
+    if ((rc = upload_raw(ctx, dd.X, X, sizeof(double) * n * p, hipMemcpyHostToDevice))) return rc;
+    dd.x = dd.X.as<double>();
+  }
+  if ((rc = upload(ctx, dd.off, d.off))) return rc;
+  if ((rc = upload(ctx, dd.eo, d.eo))) return rc;
   if (init_mode == DEVIL_INIT_QR) {
-    if (!householder_qr(d.X.data(), n, p, d.Q, d.R))
-      return fail(ctx, DEVIL_E_RANK, "design matrix is rank deficient (QR init, R/beta.R:14)");
+    bool ok = false;
+    if ((rc = qr_device(ctx, dd.x, n, p, dd.Q, d.R, ok))) return rc;
+    if (!ok) {  // ill-conditioned for Cholesky QR: Householder on the host
+      if ((rc = host_copy_of_x(ctx, X, mem, d))) return rc;
+      std::vector<double> Qh;
+      if (!householder_qr(d.X.data(), n, p, Qh, d.R))
+        return fail(ctx, DEVIL_E_RANK, "design matrix is rank deficient (QR init, R/beta.R:14)");
+      if ((rc = upload(ctx, dd.Q, Qh))) return rc;
+      DV_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    if ((rc = upload(ctx, dd.R, d.R))) return rc;
   } else if (init_mode == DEVIL_INIT_UNIVARIATE) {
     // src/beta_init_univariate.cpp:48-84
+    if ((rc = host_copy_of_x(ctx, X, mem, d))) return rc;
     double mx = 0.0;
     for (int64_t i = 0; i < n; ++i) mx = std::max(mx, std::fabs(d.X[i] - 1.0));
     d.intercept_first = mx < 1e-10;
@@ -184,9 +323,9 @@ int prepare_design(devil_ctx* ctx, const double* X, const double* off, int64_t n
     d.xmean.assign(p, 0.0);
     d.denom.assign(p, 0.0);
     for (int j = 0; j < p; ++j) {
-      double s = 0.0;
-      for (int64_t i = 0; i < n; ++i) s += d.X[i + j * n];
-      d.xmean[j] = s / (double)n;
+      double su = 0.0;
+      for (int64_t i = 0; i < n; ++i) su += d.X[i + j * n];
+      d.xmean[j] = su / (double)n;
       double q = 0.0;
       for (int64_t i = 0; i < n; ++i) {
         const double c = d.X[i + j * n] - d.xmean[j];
@@ -194,52 +333,11 @@ int prepare_design(devil_ctx* ctx, const double* X, const double* off, int64_t n
       }
       d.denom[j] = q <= 0.0 ? 1e-12 : q;
     }
-  }
-  return DEVIL_OK;
-}
-
-template <typename T>
-int upload(devil_ctx* ctx, DevBuf& b, const std::vector<T>& v) {
-  DV_ALLOC(b, sizeof(T) * v.size());
-  if (!v.empty())
-    DV_HIP(hipMemcpyAsync(b.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice,
-                          ctx->stream));
-  return DEVIL_OK;
-}
-
-int upload_design(devil_ctx* ctx, const Design& d, DeviceDesign& dd) {
-  int rc;
-  if ((rc = upload(ctx, dd.X, d.X))) return rc;
-  if ((rc = upload(ctx, dd.off, d.off))) return rc;
-  if ((rc = upload(ctx, dd.eo, d.eo))) return rc;
-  if (!d.Q.empty()) {
-    if ((rc = upload(ctx, dd.Q, d.Q))) return rc;
-    if ((rc = upload(ctx, dd.R, d.R))) return rc;
-  }
-  if (!d.logsf.empty()) {
     if ((rc = upload(ctx, dd.logsf, d.logsf))) return rc;
     if ((rc = upload(ctx, dd.xmean, d.xmean))) return rc;
     if ((rc = upload(ctx, dd.denom, d.denom))) return rc;
   }
   return DEVIL_OK;
-}
-
-dv::InitParams init_params(const Design& d, const DeviceDesign& dd, int want_alpha,
-                           int init_mode) {
-  dv::InitParams ip{};
-  ip.want_alpha = want_alpha;
-  ip.moi = d.moi;
-  ip.init_mode = init_mode;
-  ip.eo = dd.eo.as<double>();
-  ip.Q = dd.Q.as<double>();
-  ip.R = dd.R.as<double>();
-  ip.X = dd.X.as<double>();
-  ip.logsf = dd.logsf.as<double>();
-  ip.xmean = dd.xmean.as<double>();
-  ip.denom = dd.denom.as<double>();
-  ip.intercept_first = d.intercept_first;
-  ip.sum_sf = d.sum_sf;
-  return ip;
 }
 
 int check_clusters(devil_ctx* ctx, const int32_t* ends, int C, int64_t n) {
@@ -312,79 +410,104 @@ int mixed_kernel(const devil_ctx* ctx) {
   return ctx->tune[DEVIL_TUNE_MIXED_KERNEL] == DEVIL_MIXED_F32 ? kMixedF32 : kMixedBf16;
 }
 
-struct ColSet {
-  std::vector<std::vector<double>> cols;
-  int kidx_pair[55];
-  int kidx_lin[10];
-};
-
 // Columns of K: x_a x_b (a <= b) and x_a of the design, exact duplicates
 // merged (an intercept makes x_0 x_a == x_a, a 0/1 column squares to itself)
 // and all-zero columns dropped.  lin16: the p design columns sit at 0..p-1
 // as they are and the pair columns start at column 16.
-void build_columns(const double* X, int64_t n, int p, bool lin16, ColSet& cs) {
-  std::vector<uint64_t> hashes;
-  auto hash_of = [&](const std::vector<double>& v) {
-    uint64_t hsh = 1469598103934665603ULL;
-    for (double x : v) {
-      uint64_t u;
-      std::memcpy(&u, &x, 8);
-      hsh = (hsh ^ u) * 1099511628211ULL;
+struct ColPlan {
+  std::vector<dv::ColSpec> spec;
+  std::vector<double> ksum;  // sum over cells of each column
+  int kidx_pair[55];
+  int kidx_lin[10];
+};
+
+// Statistics of every candidate column computed on the device (Xd: device
+// n x p X); equal hashes are confirmed by a cell-by-cell comparison.
+int plan_columns(devil_ctx* ctx, const double* Xd, int64_t n, int p, bool lin16, ColPlan& cp) {
+  hipStream_t s = ctx->stream;
+  std::vector<dv::ColSpec> cand;
+  for (int a = 0; a < p; ++a) cand.push_back({a, -1});
+  for (int a = 0; a < p; ++a)
+    for (int b = a; b < p; ++b) cand.push_back({a, b});
+  const int C = (int)cand.size();
+  const int B = dv::kStatBlocksHost;
+  DevBuf dc, dh, dnz, dsum, dmm;
+  int rc;
+  if ((rc = upload(ctx, dc, cand))) return rc;
+  DV_ALLOC(dh, sizeof(uint64_t) * C * B);
+  DV_ALLOC(dnz, sizeof(uint64_t) * C * B);
+  DV_ALLOC(dsum, sizeof(double) * C * B);
+  DV_ALLOC(dmm, sizeof(uint64_t) * B);
+  DV_HIP(dv::launch_col_stats(s, Xd, n, dc.as<dv::ColSpec>(), C, dh.as<uint64_t>(),
+                              dnz.as<uint64_t>(), dsum.as<double>()));
+  std::vector<uint64_t> ph((size_t)C * B), pnz((size_t)C * B);
+  std::vector<double> psum((size_t)C * B);
+  DV_HIP(hipMemcpyAsync(ph.data(), dh.p, sizeof(uint64_t) * C * B, hipMemcpyDeviceToHost, s));
+  DV_HIP(hipMemcpyAsync(pnz.data(), dnz.p, sizeof(uint64_t) * C * B, hipMemcpyDeviceToHost, s));
+  DV_HIP(hipMemcpyAsync(psum.data(), dsum.p, sizeof(double) * C * B, hipMemcpyDeviceToHost, s));
+  DV_HIP(hipStreamSynchronize(s));
+  std::vector<uint64_t> hash(C, 0), nonzero(C, 0);
+  std::vector<double> sum(C, 0.0);
+  for (int c = 0; c < C; ++c) {
+    long double su = 0.0L;
+    for (int x = 0; x < B; ++x) {
+      hash[c] += ph[(size_t)c * B + x];
+      nonzero[c] += pnz[(size_t)c * B + x];
+      su += psum[(size_t)c * B + x];
     }
-    return hsh;
+    sum[c] = (double)su;
+  }
+  std::vector<int> members;  // candidate behind each column (-1: padding)
+  std::vector<uint64_t> mm(B);
+  int err = DEVIL_OK;
+  auto equal = [&](int c1, int c2) -> bool {
+    if (dv::launch_col_mismatch(s, Xd, n, cand[c1], cand[c2], dmm.as<uint64_t>()) != hipSuccess ||
+        hipMemcpyAsync(mm.data(), dmm.p, sizeof(uint64_t) * B, hipMemcpyDeviceToHost, s) !=
+            hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+      err = fail(ctx, DEVIL_E_HIP, "column comparison failed");
+      return false;
+    }
+    uint64_t tot = 0;
+    for (uint64_t v : mm) tot += v;
+    return tot == 0;
   };
-  auto add_col = [&](std::vector<double>&& v) -> int {
-    bool zero = true;
-    for (double x : v)
-      if (x != 0.0) {
-        zero = false;
-        break;
-      }
-    if (zero) return -1;
-    const uint64_t hsh = hash_of(v);
-    for (size_t c = 0; c < cs.cols.size(); ++c)
-      if (hashes[c] == hsh && std::memcmp(cs.cols[c].data(), v.data(), sizeof(double) * n) == 0)
-        return (int)c;
-    cs.cols.push_back(std::move(v));
-    hashes.push_back(hsh);
-    return (int)cs.cols.size() - 1;
+  auto add = [&](int c) -> int {
+    if (!nonzero[c]) return -1;
+    for (size_t k = 0; k < cp.spec.size(); ++k)
+      if (members[k] >= 0 && hash[members[k]] == hash[c] && equal(members[k], c)) return (int)k;
+    cp.spec.push_back(cand[c]);
+    cp.ksum.push_back(sum[c]);
+    members.push_back(c);
+    return (int)cp.spec.size() - 1;
   };
   if (lin16) {
     for (int a = 0; a < p; ++a) {
-      std::vector<double> v(X + a * n, X + (a + 1) * n);
-      hashes.push_back(hash_of(v));
-      cs.cols.push_back(std::move(v));
-      cs.kidx_lin[a] = a;
+      cp.spec.push_back(cand[a]);
+      cp.ksum.push_back(sum[a]);
+      members.push_back(a);
+      cp.kidx_lin[a] = a;
     }
-    while (cs.cols.size() < 16) {
-      cs.cols.emplace_back(n, 0.0);
-      hashes.push_back(~0ULL);
+    while (cp.spec.size() < 16) {
+      cp.spec.push_back({-1, -1});
+      cp.ksum.push_back(0.0);
+      members.push_back(-1);
     }
   } else {
-    for (int a = 0; a < p; ++a)
-      cs.kidx_lin[a] = add_col(std::vector<double>(X + a * n, X + (a + 1) * n));
+    for (int a = 0; a < p; ++a) cp.kidx_lin[a] = add(a);
   }
-  int t = 0;
-  for (int a = 0; a < p; ++a)
-    for (int b = a; b < p; ++b) {
-      std::vector<double> v(n);
-      for (int64_t i = 0; i < n; ++i) v[i] = X[i + a * n] * X[i + b * n];
-      cs.kidx_pair[t++] = add_col(std::move(v));
-    }
+  for (int q = 0; q < p * (p + 1) / 2; ++q) cp.kidx_pair[q] = add(p + q);
+  return err;
 }
 
 // Shape of a design with cells_per_block-cell blocks over the given columns.
-void shape_design(const ColSet& cs, int64_t n, int p, int cells_per_block, dv::TileDesign& td) {
+void shape_design(const ColPlan& cp, int64_t n, int p, int cells_per_block, dv::TileDesign& td) {
   std::memset(&td, 0, sizeof(td));
-  for (int a = 0; a < p; ++a) td.kidx_lin[a] = cs.kidx_lin[a];
-  for (int q = 0; q < p * (p + 1) / 2; ++q) td.kidx_pair[q] = cs.kidx_pair[q];
-  for (size_t c = 0; c < (size_t)DEVIL_NCP_MAX; ++c) {
-    long double sum = 0.0L;
-    if (c < cs.cols.size())
-      for (int64_t i = 0; i < n; ++i) sum += cs.cols[c][i];
-    td.ksum[c] = (double)sum;
-  }
-  const int ncol = std::max<int>(1, (int)cs.cols.size());
+  for (int a = 0; a < p; ++a) td.kidx_lin[a] = cp.kidx_lin[a];
+  for (int q = 0; q < p * (p + 1) / 2; ++q) td.kidx_pair[q] = cp.kidx_pair[q];
+  for (size_t c = 0; c < cp.ksum.size() && c < (size_t)DEVIL_NCP_MAX; ++c)
+    td.ksum[c] = cp.ksum[c];
+  const int ncol = std::max<int>(1, (int)cp.spec.size());
   td.nt = (ncol + 15) / 16;
   td.ncp = 16 * td.nt;
   td.n = n;
@@ -401,156 +524,173 @@ void shape_design(const ColSet& cs, int64_t n, int p, int cells_per_block, dv::T
   }
 }
 
-uint16_t to_bf16(float x) {  // round to nearest even
-  uint32_t u;
-  std::memcpy(&u, &x, 4);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
-}
-
-float from_bf16(uint16_t b) {
-  const uint32_t u = (uint32_t)b << 16;
-  float x;
-  std::memcpy(&x, &u, 4);
-  return x;
-}
-
-// cell of MFMA k index (h, i) within a 32-cell step of the bf16 design
-int step_cell(int h, int i) { return i < 4 ? 4 * h + i : 16 + 4 * h + (i - 4); }
-
+// Operand tables of a fit's design on the device (built by kernels from the
+// device copies of X, offset and exp(offset)).
 struct TileHost {
   dv::TileDesign td{};   // FP64 design (16-cell blocks, packed columns)
-  dv::TileDesign tdm{};  // mixed design (copy of td for FP32, 32-cell steps for bf16)
+  dv::TileDesign tdm{};  // mixed design (td for FP32, 32-cell steps for bf16)
   int mixed = kMixedNone;
-  std::vector<double> Xp, offp, Kp;
-  std::vector<float> Xf, offf, Kf;  // FP32 kernel operands
-  std::vector<uint16_t> xab, kh;    // bf16 kernel operands
-  std::vector<float> offl, kg;
+  DevBuf spec, specm, Xp, offp, eop, Kp, Xf, offf, Kf, xab, offl, kg, kh;
 };
 
-void build_tile_design(const double* X, const double* off, int64_t n, int p, TileHost& th,
-                       int mixed = kMixedNone) {
+// Xd, offd, eod: device n x p X, offset and exp(offset).
+int make_tile(devil_ctx* ctx, const double* Xd, const double* offd, const double* eod,
+              int64_t n, int p, int mixed, TileHost& th) {
+  hipStream_t s = ctx->stream;
   th.mixed = mixed;
-  ColSet cs;
-  build_columns(X, n, p, false, cs);
+  ColPlan cp;
+  int rc;
+  if ((rc = plan_columns(ctx, Xd, n, p, false, cp))) return rc;
   dv::TileDesign& td = th.td;
-  shape_design(cs, n, p, 16, td);
-  const auto& cols = cs.cols;
+  shape_design(cp, n, p, 16, td);
+  if ((rc = upload(ctx, th.spec, cp.spec))) return rc;
   const int NJ = (p + 3) / 4;
-  th.Xp.assign((size_t)td.nblk * NJ * 64, 0.0);
-  th.offp.assign((size_t)td.nblk * 16, 0.0);
-  th.Kp.assign((size_t)td.nblk * td.nt * 256, 0.0);
-  for (int64_t i = 0; i < n; ++i) th.offp[i] = off[i];
-  for (int64_t b = 0; b < td.nblk; ++b) {
-    // eta-product A operand: row m of the block is cell 4 (m % 4) + m / 4
-    for (int j = 0; j < NJ; ++j)
-      for (int l = 0; l < 64; ++l) {
-        const int m = l % 16;
-        const int64_t cell = 16 * b + 4 * (m % 4) + m / 4;
-        const int f = 4 * j + l / 16;
-        if (cell < n && f < p) th.Xp[(b * NJ + j) * 64 + l] = X[cell + f * n];
-      }
-    // contraction B operand: lane l, step r -> cell 4 (l / 16) + r, column l % 16
-    for (int tt = 0; tt < td.nt; ++tt)
-      for (int l = 0; l < 64; ++l)
-        for (int r = 0; r < 4; ++r) {
-          const int64_t cell = 16 * b + 4 * (l / 16) + r;
-          const int col = 16 * tt + l % 16;
-          if (cell < n && col < (int)cols.size())
-            th.Kp[((b * td.nt + tt) * 64 + l) * 4 + r] = cols[col][cell];
-        }
+  DV_ALLOC(th.Xp, sizeof(double) * td.nblk * NJ * 64);
+  DV_ALLOC(th.offp, sizeof(double) * td.nblk * 16);
+  DV_ALLOC(th.eop, sizeof(double) * td.nblk * 16);
+  DV_ALLOC(th.Kp, sizeof(double) * td.nblk * td.nt * 256);
+  const bool f32 = mixed == kMixedF32;
+  if (f32) {
+    DV_ALLOC(th.Xf, sizeof(float) * td.nblk * NJ * 64);
+    DV_ALLOC(th.offf, sizeof(float) * td.nblk * 16);
+    DV_ALLOC(th.Kf, sizeof(float) * td.nblk * td.nt * 256);
   }
-  if (mixed == kMixedF32) {
-    // FP32 copies; the eta product rows keep the natural cell order (the f32
-    // MFMA returns rows 4 (l / 16) + r in lane l)
+  DV_HIP(dv::launch_build_tables(s, Xd, offd, eod, n, p, th.spec.as<dv::ColSpec>(),
+                                 (int)cp.spec.size(), td.nt, td.nblk, th.Xp.as<double>(),
+                                 th.offp.as<double>(), th.eop.as<double>(), th.Kp.as<double>(),
+                                 f32 ? 1 : 0, th.Xf.as<float>(), th.offf.as<float>(),
+                                 th.Kf.as<float>()));
+  td.Xp = th.Xp.as<double>();
+  td.offp = th.offp.as<double>();
+  td.eop = th.eop.as<double>();
+  td.Kp = th.Kp.as<double>();
+  if (f32) {
     th.tdm = td;
-    th.Xf.assign((size_t)td.nblk * NJ * 64, 0.0f);
-    th.offf.assign(th.offp.begin(), th.offp.end());
-    th.Kf.assign(th.Kp.begin(), th.Kp.end());
-    for (int64_t b = 0; b < td.nblk; ++b)
-      for (int j = 0; j < NJ; ++j)
-        for (int l = 0; l < 64; ++l) {
-          const int64_t cell = 16 * b + l % 16;
-          const int f = 4 * j + l / 16;
-          if (cell < n && f < p) th.Xf[(b * NJ + j) * 64 + l] = (float)X[cell + f * n];
-        }
+    th.tdm.Xf = th.Xf.as<float>();
+    th.tdm.offf = th.offf.as<float>();
+    th.tdm.Kf = th.Kf.as<float>();
   } else if (mixed == kMixedBf16) {
-    ColSet cm;
-    build_columns(X, n, p, true, cm);
+    ColPlan cm;
+    if ((rc = plan_columns(ctx, Xd, n, p, true, cm))) return rc;
     dv::TileDesign& tm = th.tdm;
     shape_design(cm, n, p, 32, tm);
+    if ((rc = upload(ctx, th.specm, cm.spec))) return rc;
     const int64_t nst = tm.nblk;
     const int NE = (7 * p + 31) / 32;
     const int nth = tm.nt - 1;
-    const double L2E = 1.44269504088896340736;
-    th.xab.assign((size_t)nst * 2 * NE * 64 * 8, 0);
-    th.offl.assign((size_t)nst * 32, 0.0f);
-    th.kg.assign((size_t)nst * 64 * 8, 0.0f);
-    th.kh.assign((size_t)nst * std::max(nth, 0) * 64 * 8, 0);
-    for (int64_t i = 0; i < n; ++i) th.offl[i] = (float)(off[i] * L2E);
-    for (int64_t st = 0; st < nst; ++st) {
-      for (int u = 0; u < 2; ++u)
-        for (int j = 0; j < NE; ++j)
-          for (int l = 0; l < 64; ++l)
-            for (int i = 0; i < 8; ++i) {
-              const int64_t cell = st * 32 + 16 * u + l % 16;
-              const int e = 32 * j + 8 * (l / 16) + i;
-              const int f = e / 7, term = e % 7;
-              if (cell >= n || f >= p) continue;
-              const double x = X[cell + f * n];
-              const uint16_t x1 = to_bf16((float)x);
-              const uint16_t x2 = to_bf16((float)(x - (double)from_bf16(x1)));
-              th.xab[((((size_t)st * 2 + u) * NE + j) * 64 + l) * 8 + i] = term < 4 ? x1 : x2;
-            }
-      for (int l = 0; l < 64; ++l)
-        for (int i = 0; i < 8; ++i) {
-          const int64_t cell = st * 32 + step_cell(l / 16, i);
-          const int col = l % 16;
-          if (cell < n && col < p) th.kg[((size_t)st * 64 + l) * 8 + i] = (float)X[cell + col * n];
-        }
-      for (int t = 0; t < nth; ++t)
-        for (int l = 0; l < 64; ++l)
-          for (int i = 0; i < 8; ++i) {
-            const int64_t cell = st * 32 + step_cell(l / 16, i);
-            const int col = 16 * (t + 1) + l % 16;
-            if (cell < n && col < (int)cm.cols.size())
-              th.kh[(((size_t)st * nth + t) * 64 + l) * 8 + i] = to_bf16((float)cm.cols[col][cell]);
-          }
-    }
-  }
-}
-
-struct DeviceTile {
-  DevBuf Xp, offp, Kp, Xf, offf, Kf, xab, offl, kg, kh;
-};
-
-int upload_tile(devil_ctx* ctx, TileHost& th, DeviceTile& dt) {
-  int rc;
-  if ((rc = upload(ctx, dt.Xp, th.Xp))) return rc;
-  if ((rc = upload(ctx, dt.offp, th.offp))) return rc;
-  if ((rc = upload(ctx, dt.Kp, th.Kp))) return rc;
-  th.td.Xp = dt.Xp.as<double>();
-  th.td.offp = dt.offp.as<double>();
-  th.td.Kp = dt.Kp.as<double>();
-  if (th.mixed == kMixedF32) {
-    if ((rc = upload(ctx, dt.Xf, th.Xf))) return rc;
-    if ((rc = upload(ctx, dt.offf, th.offf))) return rc;
-    if ((rc = upload(ctx, dt.Kf, th.Kf))) return rc;
-    th.tdm.Xf = dt.Xf.as<float>();
-    th.tdm.offf = dt.offf.as<float>();
-    th.tdm.Kf = dt.Kf.as<float>();
-  } else if (th.mixed == kMixedBf16) {
-    if ((rc = upload(ctx, dt.xab, th.xab))) return rc;
-    if ((rc = upload(ctx, dt.offl, th.offl))) return rc;
-    if ((rc = upload(ctx, dt.kg, th.kg))) return rc;
-    if (!th.kh.empty() && (rc = upload(ctx, dt.kh, th.kh))) return rc;
-    th.tdm.xab = dt.xab.as<uint16_t>();
-    th.tdm.offl = dt.offl.as<float>();
-    th.tdm.kg = dt.kg.as<float>();
-    th.tdm.kh = th.kh.empty() ? nullptr : dt.kh.as<uint16_t>();
+    DV_ALLOC(th.xab, sizeof(uint16_t) * nst * 2 * NE * 64 * 8);
+    DV_ALLOC(th.offl, sizeof(float) * nst * 32);
+    DV_ALLOC(th.kg, sizeof(float) * nst * 64 * 8);
+    if (nth > 0) DV_ALLOC(th.kh, sizeof(uint16_t) * nst * nth * 64 * 8);
+    DV_HIP(dv::launch_build_tables_bf16(s, Xd, offd, n, p, th.specm.as<dv::ColSpec>(),
+                                        (int)cm.spec.size(), nth, nst, th.xab.as<uint16_t>(),
+                                        th.offl.as<float>(), th.kg.as<float>(),
+                                        th.kh.as<uint16_t>()));
+    tm.xab = th.xab.as<uint16_t>();
+    tm.offl = th.offl.as<float>();
+    tm.kg = th.kg.as<float>();
+    tm.kh = nth > 0 ? th.kh.as<uint16_t>() : nullptr;
+    tm.eop = td.eop;
   }
   return DEVIL_OK;
+}
+
+// Operands of the init tile pass (mode kInitNone: alpha0 only), built on the
+// device from the uploaded design (dd: X, eo, Q for QR, logsf and xmean for
+// the univariate start).
+struct InitHost {
+  DevBuf dZ, dE, dL, dR, dDen, dShift;
+  dv::InitTile it{};
+};
+
+int build_init_tile(devil_ctx* ctx, const Design& d, const DeviceDesign& dd,
+                    const dv::TileDesign& td, int want_alpha, int mode, InitHost& ih) {
+  const int64_t n = d.n;
+  const int p = d.p;
+  dv::InitTile& it = ih.it;
+  it.want_alpha = want_alpha;
+  it.moi = d.moi;
+  it.mode = mode;
+  it.intercept_first = d.intercept_first;
+  it.sum_sf = d.sum_sf;
+  const size_t padded = (size_t)td.nblk * 16;
+  const bool qr = mode == dv::kInitQR, uni = mode == dv::kInitUnivariate;
+  DV_ALLOC(ih.dE, sizeof(double) * padded);
+  if (qr || uni) DV_ALLOC(ih.dZ, sizeof(double) * padded * 16);
+  if (uni) DV_ALLOC(ih.dL, sizeof(double) * padded);
+  DV_HIP(dv::launch_build_init(ctx->stream, qr ? dd.Q.as<double>() : dd.x,
+                               uni ? dd.xmean.as<double>() : nullptr, dd.eo.as<double>(),
+                               uni ? dd.logsf.as<double>() : nullptr, n, p, td.nblk,
+                               (qr || uni) ? ih.dZ.as<double>() : nullptr, ih.dE.as<double>(),
+                               uni ? ih.dL.as<double>() : nullptr));
+  it.einv = ih.dE.as<double>();
+  it.Zp = (qr || uni) ? ih.dZ.as<double>() : nullptr;
+  it.logsf = uni ? ih.dL.as<double>() : nullptr;
+  it.R = qr ? dd.R.as<double>() : nullptr;
+  it.denom = uni ? dd.denom.as<double>() : nullptr;
+  return DEVIL_OK;
+}
+
+// FP64 tile design from a device-resident design matrix and offset.
+int tile_from_device(devil_ctx* ctx, const double* X, const double* offset, int64_t n, int p,
+                     TileHost& th, int mixed = kMixedNone) {
+  std::vector<double> eo(n);
+  DV_HIP(hipMemcpy(eo.data(), offset, sizeof(double) * n, hipMemcpyDeviceToHost));
+  parallel_for(n, [&](int64_t lo, int64_t hi) {
+    for (int64_t i = lo; i < hi; ++i) eo[i] = std::exp(eo[i]);
+  });
+  DevBuf eod;
+  int rc;
+  if ((rc = upload(ctx, eod, eo))) return rc;
+  if ((rc = make_tile(ctx, X, offset, eod.as<double>(), n, p, mixed, th))) return rc;
+  DV_HIP(hipStreamSynchronize(ctx->stream));  // eod is freed on return
+  return DEVIL_OK;
+}
+
+// Chunk table of the post-IRLS passes: cell ranges of at most about
+// max(2048, n / 64) cells, never straddling a patient block; the count is
+// rounded up to a multiple of 8 with empty chunks (one XCD per chunk).
+struct ChunkHost {
+  std::vector<int64_t> c0, c1;
+  std::vector<int32_t> cl;
+  DevBuf d0, d1, dcl;
+  dv::ChunkTable ct{};
+};
+
+int build_chunks(devil_ctx* ctx, int64_t n, const int32_t* ends, int C, ChunkHost& ch) {
+  const int64_t target = std::max<int64_t>(2048, (n + 63) / 64);
+  const int nc = C > 0 ? C : 1;
+  int64_t start = 0;
+  for (int k = 0; k < nc; ++k) {
+    const int64_t end = C > 0 ? (int64_t)ends[k] : n;
+    const int64_t len = end - start;
+    const int64_t pieces = (len + target - 1) / target;
+    for (int64_t q = 0; q < pieces; ++q) {
+      ch.c0.push_back(start + len * q / pieces);
+      ch.c1.push_back(start + len * (q + 1) / pieces);
+      ch.cl.push_back(k);
+    }
+    start = end;
+  }
+  if (ch.c0.empty() || ch.c0.size() >= 8)
+    while (ch.c0.empty() || ch.c0.size() % 8) {
+      ch.c0.push_back(n);
+      ch.c1.push_back(n);
+      ch.cl.push_back(nc - 1);
+    }
+  int rc;
+  if ((rc = upload(ctx, ch.d0, ch.c0))) return rc;
+  if ((rc = upload(ctx, ch.d1, ch.c1))) return rc;
+  if ((rc = upload(ctx, ch.dcl, ch.cl))) return rc;
+  ch.ct.c0 = ch.d0.as<int64_t>();
+  ch.ct.c1 = ch.d1.as<int64_t>();
+  ch.ct.cl = ch.dcl.as<int32_t>();
+  ch.ct.nchunk = (int)ch.c0.size();
+  return DEVIL_OK;
+}
+
+size_t post_part_doubles(const dv::TileDesign& td, const ChunkHost& ch, int64_t G) {
+  return (size_t)((G + 15) / 16) * ch.ct.nchunk * 16 * (td.ncp + 16);
 }
 
 // IRLS over genes [0, G) of the counts view: rounds of accumulate + update
@@ -856,15 +996,12 @@ int devil_fit(devil_ctx* ctx, const devil_problem* pr, devil_outputs* out) {
   }
   const int init_mode = pr->beta_init ? dv::kInitNone : pr->init_mode;
   Design d;
-  if ((rc = prepare_design(ctx, pr->X, pr->offset, n, p, mem, init_mode, d))) return rc;
   DeviceDesign dd;
-  if ((rc = upload_design(ctx, d, dd))) return rc;
-  if (C > 0 && (rc = upload(ctx, dd.ends, ends_h))) return rc;
+  if ((rc = prepare_design(ctx, pr->X, pr->offset, n, p, mem, init_mode, d, dd))) return rc;
   TileHost tl;
-  build_tile_design(d.X.data(), d.off.data(), n, p, tl,
-                    pr->precision == DEVIL_PREC_MIXED ? mixed_kernel(ctx) : kMixedNone);
-  DeviceTile dt;
-  if ((rc = upload_tile(ctx, tl, dt))) return rc;
+  if ((rc = make_tile(ctx, dd.x, dd.off.as<double>(), dd.eo.as<double>(), n, p,
+                      pr->precision == DEVIL_PREC_MIXED ? mixed_kernel(ctx) : kMixedNone, tl)))
+    return rc;
 
   // per-gene device state (gene-major)
   DevBuf alpha0, kvec, beta, theta, iters, thit, status, bread, sandwich;
@@ -909,8 +1046,15 @@ int devil_fit(devil_ctx* ctx, const devil_problem* pr, devil_outputs* out) {
     batch = std::max<int64_t>(1, std::min<int64_t>(G, (int64_t)(budget / row_bytes)));
     DV_ALLOC(cbuf, row_bytes * batch);
   }
+  ChunkHost chunks;
+  if ((rc = build_chunks(ctx, n, C > 0 ? ends_h.data() : nullptr, C, chunks))) return rc;
+  DevBuf post;
+  DV_ALLOC(post, sizeof(double) * post_part_doubles(tl.td, chunks, batch));
   const double eps = (double)(float)pr->tolerance;  // src/beta.cpp:25 float eps
-  const dv::InitParams ip = init_params(d, dd, pr->disp_init ? 0 : 1, init_mode);
+  InitHost ih;
+  if ((rc = build_init_tile(ctx, d, dd, tl.td, pr->disp_init ? 0 : 1, init_mode, ih)))
+    return rc;
+  const bool run_init = ih.it.want_alpha || init_mode != dv::kInitNone;
   for (int k = 0; k < DEVIL_STAGE_TEST_DE; ++k) ctx->stage_ms[k] = 0.0;
   for (int k = 0; k < DEVIL_N_KERNELS; ++k) {
     ctx->k_n[k] = 0;
@@ -934,7 +1078,9 @@ int devil_fit(devil_ctx* ctx, const devil_problem* pr, devil_outputs* out) {
     double* a0 = alpha0.as<double>() + g0;
     double* b0 = beta.as<double>() + g0 * p;
     ev = tm.begin(DEVIL_STAGE_INIT);
-    DV_HIP(dv::launch_init(s, c, Gb, n, p, ip, a0, b0));
+    if (run_init)
+      DV_HIP(dv::launch_init_tiled(s, tl.td, ih.it, chunks.ct, c, Gb, n, p, post.as<double>(),
+                                   a0, b0));
     DV_HIP(dv::launch_k_from_alpha(s, a0, Gb, kvec.as<double>() + g0));
     tm.end(ev);
     ev = tm.begin(DEVIL_STAGE_IRLS);
@@ -945,8 +1091,8 @@ int devil_fit(devil_ctx* ctx, const devil_problem* pr, devil_outputs* out) {
     double* th = theta.as<double>() + g0;
     ev = tm.begin(DEVIL_STAGE_DISPERSION);
     if (pr->od_mode == DEVIL_OD_MOM) {
-      DV_HIP(dv::launch_mom(s, c, Gb, n, p, dd.X.as<double>(), b0, dd.eo.as<double>(), th,
-                            status.as<int32_t>() + g0));
+      DV_HIP(dv::launch_mom_tiled(s, tl.td, chunks.ct, c, Gb, n, p, b0, post.as<double>(), th,
+                                  status.as<int32_t>() + g0));
       DV_HIP(dv::launch_fill_i32(s, thit.as<int32_t>() + g0, Gb, 0));
     } else if (pr->od_mode == DEVIL_OD_MLE) {
       if ((rc = run_mle_tiled(ctx, tl.td, c, Gb, n, p, b0, pr->max_iter, th,
@@ -958,11 +1104,10 @@ int devil_fit(devil_ctx* ctx, const devil_problem* pr, devil_outputs* out) {
     }
     tm.end(ev);
     ev = tm.begin(DEVIL_STAGE_BREAD_MEAT);
-    DV_HIP(dv::launch_bread_meat(s, c, Gb, n, p, dd.X.as<double>(), b0, th,
-                                 dd.eo.as<double>(), C > 0 ? dd.ends.as<int32_t>() : nullptr,
-                                 C, bread.as<double>() + g0 * p * p,
-                                 want_sw ? sandwich.as<double>() + g0 * p * p : nullptr,
-                                 status.as<int32_t>() + g0));
+    DV_HIP(dv::launch_bread_tiled(s, tl.td, chunks.ct, c, Gb, n, p, b0, th, C,
+                                  post.as<double>(), bread.as<double>() + g0 * p * p,
+                                  want_sw ? sandwich.as<double>() + g0 * p * p : nullptr,
+                                  status.as<int32_t>() + g0));
     tm.end(ev);
   }
 
@@ -1029,12 +1174,27 @@ int devil_estimate_dispersion(devil_ctx* ctx, const void* counts, int32_t dtype,
   DV_HIP(hipMemcpy(off.data(), offset, sizeof(double) * n, hipMemcpyDeviceToHost));
   double se = 0.0;
   for (int64_t i = 0; i < n; ++i) se += std::exp(off[i]);
-  dv::InitParams ip{};
-  ip.want_alpha = 1;
-  ip.moi = 1.0 / (se / (double)n);
-  ip.init_mode = dv::kInitNone;
+  if (G == 0) return DEVIL_OK;
+  Design d;
+  d.n = n;
+  d.p = 1;
+  d.moi = 1.0 / (se / (double)n);
+  d.eo.resize(n);
+  for (int64_t i = 0; i < n; ++i) d.eo[i] = std::exp(off[i]);
+  dv::TileDesign td{};
+  td.n = n;
+  td.nblk = (n + 15) / 16;
+  DeviceDesign dd;
+  if ((rc = upload(ctx, dd.eo, d.eo))) return rc;
+  InitHost ih;
+  if ((rc = build_init_tile(ctx, d, dd, td, 1, dv::kInitNone, ih))) return rc;
+  ChunkHost chunks;
+  if ((rc = build_chunks(ctx, n, nullptr, 0, chunks))) return rc;
+  DevBuf part;
+  DV_ALLOC(part, sizeof(double) * (size_t)((G + 15) / 16) * chunks.ct.nchunk * 16 * 20);
   dv::Counts c{counts, dtype, ld};
-  DV_HIP(dv::launch_init(ctx->stream, c, G, n, 1, ip, alpha0, nullptr));
+  DV_HIP(dv::launch_init_tiled(ctx->stream, td, ih.it, chunks.ct, c, G, n, 1,
+                               part.as<double>(), alpha0, nullptr));
   DV_HIP(hipStreamSynchronize(ctx->stream));
   return DEVIL_OK;
 }
@@ -1050,14 +1210,22 @@ int devil_init_beta(devil_ctx* ctx, const void* counts, int32_t dtype, int64_t G
   if (init_mode < DEVIL_INIT_QR || init_mode > DEVIL_INIT_UNIVARIATE)
     return fail(ctx, DEVIL_E_ARG, "unknown init_mode");
   Design d;
-  if ((rc = prepare_design(ctx, X, offset, n, p, DEVIL_MEM_DEVICE, init_mode, d))) return rc;
   DeviceDesign dd;
-  if ((rc = upload_design(ctx, d, dd))) return rc;
-  DevBuf tmp;
+  if ((rc = prepare_design(ctx, X, offset, n, p, DEVIL_MEM_DEVICE, init_mode, d, dd))) return rc;
+  if (G == 0) return DEVIL_OK;
+  dv::TileDesign td{};
+  td.n = n;
+  td.nblk = (n + 15) / 16;
+  InitHost ih;
+  if ((rc = build_init_tile(ctx, d, dd, td, 0, init_mode, ih))) return rc;
+  ChunkHost chunks;
+  if ((rc = build_chunks(ctx, n, nullptr, 0, chunks))) return rc;
+  DevBuf tmp, part;
   DV_ALLOC(tmp, sizeof(double) * G * p);
+  DV_ALLOC(part, sizeof(double) * (size_t)((G + 15) / 16) * chunks.ct.nchunk * 16 * 20);
   dv::Counts c{counts, dtype, ld};
-  const dv::InitParams ip = init_params(d, dd, 0, init_mode);
-  DV_HIP(dv::launch_init(ctx->stream, c, G, n, p, ip, nullptr, tmp.as<double>()));
+  DV_HIP(dv::launch_init_tiled(ctx->stream, td, ih.it, chunks.ct, c, G, n, p, part.as<double>(),
+                               nullptr, tmp.as<double>()));
   DV_HIP(dv::launch_transpose_gp(ctx->stream, tmp.as<double>(), G, p, 0, beta0));
   DV_HIP(hipStreamSynchronize(ctx->stream));
   return DEVIL_OK;
@@ -1084,14 +1252,10 @@ int devil_beta_fit(devil_ctx* ctx, const void* counts, int32_t dtype, int64_t G,
   DV_HIP(dv::launch_transpose_gp(s, beta, G, p, 1, bg.as<double>()));
   DV_HIP(dv::launch_k_from_alpha(s, alpha0, G, kv.as<double>()));
   dv::Counts c{counts, dtype, ld};
-  std::vector<double> Xh((size_t)n * p), offh(n);
-  DV_HIP(hipMemcpy(Xh.data(), X, sizeof(double) * n * p, hipMemcpyDeviceToHost));
-  DV_HIP(hipMemcpy(offh.data(), offset, sizeof(double) * n, hipMemcpyDeviceToHost));
   TileHost th;
-  build_tile_design(Xh.data(), offh.data(), n, p, th,
-                    precision == DEVIL_PREC_MIXED ? mixed_kernel(ctx) : kMixedNone);
-  DeviceTile dt;
-  if ((rc = upload_tile(ctx, th, dt))) return rc;
+  if ((rc = tile_from_device(ctx, X, offset, n, p, th,
+                             precision == DEVIL_PREC_MIXED ? mixed_kernel(ctx) : kMixedNone)))
+    return rc;
   DevBuf st_local;
   int32_t* st = status;
   if (!st) {
@@ -1117,15 +1281,17 @@ int devil_mom_dispersion(devil_ctx* ctx, const void* counts, int32_t dtype, int6
     return fail(ctx, DEVIL_E_ARG, "NULL pointer");
   if (G == 0) return DEVIL_OK;
   hipStream_t s = ctx->stream;
-  std::vector<double> off(n);
-  DV_HIP(hipMemcpy(off.data(), offset, sizeof(double) * n, hipMemcpyDeviceToHost));
-  for (auto& v : off) v = std::exp(v);
-  DevBuf eo, bg;
-  if ((rc = upload(ctx, eo, off))) return rc;
+  TileHost th;
+  if ((rc = tile_from_device(ctx, X, offset, n, p, th))) return rc;
+  ChunkHost chunks;
+  if ((rc = build_chunks(ctx, n, nullptr, 0, chunks))) return rc;
+  DevBuf bg, post;
   DV_ALLOC(bg, sizeof(double) * G * p);
+  DV_ALLOC(post, sizeof(double) * post_part_doubles(th.td, chunks, G));
   DV_HIP(dv::launch_transpose_gp(s, beta, G, p, 1, bg.as<double>()));
   dv::Counts c{counts, dtype, ld};
-  DV_HIP(dv::launch_mom(s, c, G, n, p, X, bg.as<double>(), eo.as<double>(), theta, nullptr));
+  DV_HIP(dv::launch_mom_tiled(s, th.td, chunks.ct, c, G, n, p, bg.as<double>(),
+                              post.as<double>(), theta, nullptr));
   DV_HIP(hipStreamSynchronize(s));
   return DEVIL_OK;
 }
@@ -1148,13 +1314,8 @@ int devil_fit_dispersion_mle(devil_ctx* ctx, const void* counts, int32_t dtype, 
   DV_HIP(dv::launch_transpose_gp(s, beta, G, p, 1, bg.as<double>()));
   DV_HIP(dv::launch_fill_i32(s, st.as<int32_t>(), G, 0));
   dv::Counts c{counts, dtype, ld};
-  std::vector<double> Xh((size_t)n * p), offh(n);
-  DV_HIP(hipMemcpy(Xh.data(), X, sizeof(double) * n * p, hipMemcpyDeviceToHost));
-  DV_HIP(hipMemcpy(offh.data(), offset, sizeof(double) * n, hipMemcpyDeviceToHost));
   TileHost th;
-  build_tile_design(Xh.data(), offh.data(), n, p, th);
-  DeviceTile dt;
-  if ((rc = upload_tile(ctx, th, dt))) return rc;
+  if ((rc = tile_from_device(ctx, X, offset, n, p, th))) return rc;
   if ((rc = run_mle_tiled(ctx, th.td, c, G, n, p, bg.as<double>(), max_iter, theta, iters,
                           st.as<int32_t>())))
     return rc;
@@ -1182,17 +1343,22 @@ int devil_bread_meat(devil_ctx* ctx, const void* counts, int32_t dtype, int64_t 
                      hipMemcpyDeviceToHost));
     if ((rc = check_clusters(ctx, e.data(), n_clusters, n))) return rc;
   }
-  std::vector<double> off(n);
-  DV_HIP(hipMemcpy(off.data(), offset, sizeof(double) * n, hipMemcpyDeviceToHost));
-  for (auto& v : off) v = std::exp(v);
-  DevBuf eo, bg;
-  if ((rc = upload(ctx, eo, off))) return rc;
+  std::vector<int32_t> ends_h(n_clusters);
+  if (n_clusters > 0)
+    DV_HIP(hipMemcpy(ends_h.data(), cluster_ends, sizeof(int32_t) * n_clusters,
+                     hipMemcpyDeviceToHost));
+  TileHost th;
+  if ((rc = tile_from_device(ctx, X, offset, n, p, th))) return rc;
+  ChunkHost chunks;
+  if ((rc = build_chunks(ctx, n, ends_h.data(), n_clusters, chunks))) return rc;
+  DevBuf bg, post;
   DV_ALLOC(bg, sizeof(double) * G * p);
+  DV_ALLOC(post, sizeof(double) * post_part_doubles(th.td, chunks, G));
   DV_HIP(dv::launch_transpose_gp(s, beta, G, p, 1, bg.as<double>()));
   dv::Counts c{counts, dtype, ld};
-  DV_HIP(dv::launch_bread_meat(s, c, G, n, p, X, bg.as<double>(), theta, eo.as<double>(),
-                               cluster_ends, n_clusters, bread,
-                               n_clusters > 0 ? sandwich : nullptr, status));
+  DV_HIP(dv::launch_bread_tiled(s, th.td, chunks.ct, c, G, n, p, bg.as<double>(), theta,
+                                n_clusters, post.as<double>(), bread,
+                                n_clusters > 0 ? sandwich : nullptr, status));
   DV_HIP(hipStreamSynchronize(s));
   return DEVIL_OK;
 }

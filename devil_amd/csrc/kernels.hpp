@@ -57,10 +57,73 @@ struct TileDesign {
   const float* offl;     // off * log2(e), padded with zeros to 32 nblk
   const float* kg;       // [st][lane][8] = X[32 st + c(lane/16, i)][lane%16]
   const uint16_t* kh;    // [st][t][lane][8] bf16 = K[32 st + c(lane/16, i)][16 (t+1) + lane%16]
+  const double* eop;     // exp(off) padded with zeros to 16 nblk (FP64 design)
   int kidx_pair[55];     // K column of x_a x_b (packed upper triangle), -1 = zero column
   int kidx_lin[10];      // K column of x_a, -1 = zero column
   double ksum[DEVIL_NCP_MAX];  // sum over cells of each K column
 };
+
+// Column of K: a < 0 zero, b < 0 the design column x_a, else x_a x_b.
+struct ColSpec {
+  int a, b;
+};
+hipError_t launch_build_tables(hipStream_t s, const double* X, const double* off,
+                               const double* eo, int64_t n, int p, const ColSpec* spec, int ncol,
+                               int nt, int64_t nblk, double* Xp, double* offp, double* eop,
+                               double* Kp, int f32, float* Xf, float* offf, float* Kf);
+// Design statistics on the device: partials per kStatBlocks cell ranges.
+constexpr int kStatBlocksHost = 64;
+hipError_t launch_col_stats(hipStream_t s, const double* X, int64_t n, const ColSpec* cand,
+                            int C, uint64_t* ph, uint64_t* pnz, double* ps);
+hipError_t launch_col_mismatch(hipStream_t s, const double* X, int64_t n, ColSpec c1, ColSpec c2,
+                               uint64_t* out);
+hipError_t launch_gram(hipStream_t s, const double* M, int64_t n, int p, double* out);  // [64][55]
+hipError_t launch_right_solve(hipStream_t s, double* M, int64_t n, int p, const double* R);
+hipError_t launch_build_init(hipStream_t s, const double* M, const double* shift,
+                             const double* eo, const double* logsf, int64_t n, int p,
+                             int64_t nblk, double* Zp, double* einv, double* logsfp);
+hipError_t launch_build_tables_bf16(hipStream_t s, const double* X, const double* off, int64_t n,
+                                    int p, const ColSpec* spec, int ncol, int nth, int64_t nst,
+                                    uint16_t* xab, float* offl, float* kg, uint16_t* kh);
+
+// Cell ranges of the post-IRLS tile passes, aligned to the patient blocks of a
+// clustered fit (chunks of one cluster are consecutive; trailing chunks may be
+// empty, c0 == c1).
+struct ChunkTable {
+  const int64_t* c0;
+  const int64_t* c1;
+  const int32_t* cl;
+  int nchunk;
+};
+
+// Init pass operands (kernels_post.hip): Zp in the K-operand lane layout of
+// one 16-column tile, [blk][lane][r] = Z[16 blk + 4 (lane/16) + r][lane%16].
+struct InitTile {
+  int want_alpha;
+  double moi;            // 1 / mean(exp(offset))
+  int mode;              // kInit*
+  const double* Zp;      // projection columns (QR: thin Q; univariate: X - mean)
+  const double* einv;    // exp(-offset), padded to 16 nblk
+  const double* logsf;   // offset, padded (univariate)
+  const double* R;       // p x p column-major upper R (QR)
+  const double* denom;   // p (univariate)
+  int intercept_first;   // univariate
+  double sum_sf;         // univariate
+};
+hipError_t launch_init_tiled(hipStream_t s, const TileDesign& td, const InitTile& it,
+                             const ChunkTable& ct, Counts c, int64_t G, int64_t n, int p,
+                             double* part, double* alpha0, double* beta0);
+
+// MOM overdispersion and bread (+ clustered meat, sandwich) as FP64 MFMA tile
+// passes (kernels_post.hip); part holds ceil(G/16) * nchunk * 16 * (ncp + 16)
+// doubles.
+hipError_t launch_mom_tiled(hipStream_t s, const TileDesign& td, const ChunkTable& ct, Counts c,
+                            int64_t G, int64_t n, int p, const double* beta, double* part,
+                            double* theta, int32_t* status);
+hipError_t launch_bread_tiled(hipStream_t s, const TileDesign& td, const ChunkTable& ct,
+                              Counts c, int64_t G, int64_t n, int p, const double* beta,
+                              const double* theta, int C, double* part, double* bread,
+                              double* sandwich, int32_t* status);
 
 hipError_t launch_irls_accum(hipStream_t s, const TileDesign& td, Counts c, int p,
                              const int32_t* active, const int32_t* nactive, int nA_bound,

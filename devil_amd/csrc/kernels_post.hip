@@ -1,0 +1,543 @@
+// gfx950 FP64 tile passes after the IRLS: method-of-moments overdispersion
+// (src/mom_overdispersion.cpp:24-53) and bread + clustered meat
+// (src/variance.cpp:22-47, :186-228; R/main.R:481-492).
+//
+// Same wavefront tiling as the IRLS accumulate (kernels_tile.hip): a wave
+// owns 16 genes and one chunk of cells; eta = X beta on
+// v_mfma_f64_16x16x4f64 with the permuted design rows, so lane l holds the
+// four consecutive cells 4 (l / 16) + r of gene slot l % 16.  Chunks come
+// from a chunk table (cell ranges [c0, c1) and their cluster), aligned to
+// the patient blocks when the fit is clustered; cells of a 16-cell block
+// outside the chunk's range are masked.
+//
+//   k_mom_accum    per cell mu = s_i exp(x'beta); sums (y - mu)^2 - mu and
+//                  mu^2 per (gene, chunk)
+//   k_mom_finish   theta = max(0, n/(n-p) num/den), 0 when den <= 0
+//   k_bread_accum  w = (y theta + 1) mu / (1 + theta mu)^2 against every K
+//                  column and r = (y - mu) / (1 + mu theta) against the design
+//                  columns (K tile 0), per (gene, chunk)
+//   k_bread_finish H = sum of the w partials, bread = H^-1 (LU), S_c = sum of
+//                  the r partials of cluster c's chunks, meat = adj/n sum_c
+//                  S_c S_c', sandwich = bread meat bread n
+#include <hip/hip_runtime.h>
+
+#include "../../include/devil_hip.h"
+#include "device_common.hpp"
+#include "kernels.hpp"
+#include "tile_common.hpp"
+
+namespace dv {
+
+template <int P, typename CT>
+__device__ __forceinline__ void mom_tile_chunk(const TileDesign& td, ChunkTable ct,
+                                               const CT* __restrict__ Y, int64_t ld, int64_t G,
+                                               const double* __restrict__ beta, int tile,
+                                               int chunk, double* __restrict__ part) {
+  constexpr int NJ = (P + 3) / 4;
+  const int lane = threadIdx.x & 63;
+  const int s = lane & 15, h = lane >> 4;
+  const int64_t g0 = (int64_t)tile * 16 + s;
+  const bool live = g0 < G;
+  const int64_t g = live ? g0 : (int64_t)tile * 16;
+  double bpos[NJ];
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) {
+    const int f = 4 * j + h;
+    bpos[j] = (live && f < P) ? beta[g * P + f] : 0.0;
+  }
+  const CT* yrow = Y + g * ld;
+  const bool vec = (ld & 3) == 0 && ((reinterpret_cast<uintptr_t>(Y) & 15) == 0);
+  const int64_t c0 = ct.c0[chunk], c1 = ct.c1[chunk];
+  const int64_t b0 = c0 / 16, b1 = (c1 + 15) / 16;
+  double num = 0.0, den = 0.0;
+  BlockOps<NJ, 1> cur, nxt;
+  if (b0 < b1 && c0 < c1) load_block<NJ, 1, CT>(td, yrow, vec, b0, lane, h, 0, cur);
+  for (int64_t b = b0; b < b1 && c0 < c1; ++b) {
+    if (b + 1 < b1) load_block<NJ, 1, CT>(td, yrow, vec, b + 1, lane, h, 0, nxt);
+    d4 eta = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) eta = mfma16x16x4(cur.xa[j], bpos[j], eta);
+    const int64_t cb = b * 16 + 4 * h;
+    const d4 eo = *reinterpret_cast<const d4*>(td.eop + cb);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      if (cb + r >= c0 && cb + r < c1) {
+        const double mu = eo[r] * exp_f64(eta[r]);
+        const double d = cur.y[r] - mu;
+        num += d * d - mu;
+        den += mu * mu;
+      }
+    }
+    cur = nxt;
+  }
+#pragma unroll
+  for (int o = 16; o <= 32; o <<= 1) {
+    num += __shfl_xor(num, o, 64);
+    den += __shfl_xor(den, o, 64);
+  }
+  if (h == 0) {
+    double* out = part + (((int64_t)tile * ct.nchunk + chunk) * 16 + s) * 2;
+    out[0] = num;
+    out[1] = den;
+  }
+}
+
+template <int P, typename CT>
+__device__ __forceinline__ void bread_tile_chunk(const TileDesign& td, ChunkTable ct,
+                                                 const CT* __restrict__ Y, int64_t ld,
+                                                 int64_t G, const double* __restrict__ beta,
+                                                 const double* __restrict__ theta, int tile,
+                                                 int chunk, double* __restrict__ part) {
+  constexpr int NJ = (P + 3) / 4;
+  constexpr int NTMAX = ((P * (P + 1)) / 2 + P + 15) / 16;
+  const int lane = threadIdx.x & 63;
+  const int s = lane & 15, h = lane >> 4;
+  const int64_t g0 = (int64_t)tile * 16 + s;
+  const bool live = g0 < G;
+  const int64_t g = live ? g0 : (int64_t)tile * 16;
+  const int NT = td.nt;
+  double bpos[NJ];
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) {
+    const int f = 4 * j + h;
+    bpos[j] = (live && f < P) ? beta[g * P + f] : 0.0;
+  }
+  const double th = live ? theta[g] : 0.0;
+  const CT* yrow = Y + g * ld;
+  const bool vec = (ld & 3) == 0 && ((reinterpret_cast<uintptr_t>(Y) & 15) == 0);
+  d4 aw[NTMAX], ar;
+#pragma unroll
+  for (int t = 0; t < NTMAX; ++t) aw[t] = d4{0.0, 0.0, 0.0, 0.0};
+  ar = d4{0.0, 0.0, 0.0, 0.0};
+  const int64_t c0 = ct.c0[chunk], c1 = ct.c1[chunk];
+  const int64_t b0 = c0 / 16, b1 = c0 < c1 ? (c1 + 15) / 16 : b0;
+  BlockOps<NJ, NTMAX> cur, nxt;
+  if (b0 < b1) load_block<NJ, NTMAX, CT>(td, yrow, vec, b0, lane, h, NT, cur);
+  for (int64_t b = b0; b < b1; ++b) {
+    if (b + 1 < b1) load_block<NJ, NTMAX, CT>(td, yrow, vec, b + 1, lane, h, NT, nxt);
+    d4 eta = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) eta = mfma16x16x4(cur.xa[j], bpos[j], eta);
+    const int64_t cb = b * 16 + 4 * h;
+    const d4 eo = *reinterpret_cast<const d4*>(td.eop + cb);
+    double w[4], rr[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const double mu = eo[r] * exp_f64(eta[r]);
+      const double y = cur.y[r];
+      const double d = fma(th, mu, 1.0);
+      const double id = rcp_f64(d);
+      const bool in = cb + r >= c0 && cb + r < c1;
+      w[r] = in ? fma(y, th, 1.0) * mu * id * id : 0.0;
+      rr[r] = in ? (y - mu) * id : 0.0;
+    }
+#pragma unroll
+    for (int t = 0; t < NTMAX; ++t) {
+      if (t < NT) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) aw[t] = mfma16x16x4(w[r], cur.kb[t][r], aw[t]);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) ar = mfma16x16x4(rr[r], cur.kb[0][r], ar);
+    cur = nxt;
+  }
+  // register r holds gene slot h + 4 r, column 16 t + s
+  const int W = td.ncp + 16;
+  double* out = part + ((int64_t)tile * ct.nchunk + chunk) * 16 * W;
+#pragma unroll
+  for (int t = 0; t < NTMAX; ++t) {
+    if (t < NT) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) out[(h + 4 * r) * W + 16 * t + s] = aw[t][r];
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) out[(h + 4 * r) * W + td.ncp + s] = ar[r];
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void k_mom_accum(TileDesign td, ChunkTable ct, Counts c,
+                                                   int64_t G, const double* __restrict__ beta,
+                                                   double* __restrict__ part) {
+  const int wave = threadIdx.x >> 6;
+  const int chunk = blockIdx.x % ct.nchunk;
+  const int tile = (blockIdx.x / ct.nchunk) * 4 + wave;
+  if ((int64_t)tile * 16 >= G) return;
+  if (c.dtype == 0)
+    mom_tile_chunk<P>(td, ct, static_cast<const int32_t*>(c.ptr), c.ld, G, beta, tile, chunk,
+                      part);
+  else if (c.dtype == 1)
+    mom_tile_chunk<P>(td, ct, static_cast<const float*>(c.ptr), c.ld, G, beta, tile, chunk,
+                      part);
+  else
+    mom_tile_chunk<P>(td, ct, static_cast<const double*>(c.ptr), c.ld, G, beta, tile, chunk,
+                      part);
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void k_mom_finish(ChunkTable ct, int64_t G, int64_t n,
+                                                    const double* __restrict__ part,
+                                                    double* __restrict__ theta,
+                                                    int32_t* __restrict__ status) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= G) return;
+  const int64_t tile = g / 16, s = g % 16;
+  double num = 0.0, den = 0.0;
+  for (int ch = 0; ch < ct.nchunk; ++ch) {
+    const double* src = part + ((tile * ct.nchunk + ch) * 16 + s) * 2;
+    num += src[0];
+    den += src[1];
+  }
+  const double corr = (double)n / (double)(n - P);
+  double th = 0.0;
+  if (den > 0.0) {
+    th = corr * num / den;
+    if (th < 0.0) th = 0.0;
+  }
+  theta[g] = th;
+  if (status && !(den > 0.0)) status[g] |= DEVIL_GENE_MOM_DEN0;
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void k_bread_accum(TileDesign td, ChunkTable ct, Counts c,
+                                                     int64_t G, const double* __restrict__ beta,
+                                                     const double* __restrict__ theta,
+                                                     double* __restrict__ part) {
+  const int wave = threadIdx.x >> 6;
+  const int chunk = blockIdx.x % ct.nchunk;
+  const int tile = (blockIdx.x / ct.nchunk) * 4 + wave;
+  if ((int64_t)tile * 16 >= G) return;
+  if (c.dtype == 0)
+    bread_tile_chunk<P>(td, ct, static_cast<const int32_t*>(c.ptr), c.ld, G, beta, theta, tile,
+                        chunk, part);
+  else if (c.dtype == 1)
+    bread_tile_chunk<P>(td, ct, static_cast<const float*>(c.ptr), c.ld, G, beta, theta, tile,
+                        chunk, part);
+  else
+    bread_tile_chunk<P>(td, ct, static_cast<const double*>(c.ptr), c.ld, G, beta, theta, tile,
+                        chunk, part);
+}
+
+// One wavefront per gene.  C = number of clusters (0: no sandwich).
+template <int P>
+__global__ __launch_bounds__(64) void k_bread_finish(TileDesign td, ChunkTable ct, int64_t G,
+                                                     int64_t n, int C,
+                                                     const double* __restrict__ part,
+                                                     double* __restrict__ bread,
+                                                     double* __restrict__ sandwich,
+                                                     int32_t* __restrict__ status) {
+  constexpr int NPAIR = P * (P + 1) / 2;
+  __shared__ double accw[DEVIL_NCP_MAX];
+  __shared__ double sc[16];
+  __shared__ double meat[NPAIR];
+  const int64_t g = blockIdx.x;
+  if (g >= G) return;
+  const int lane = threadIdx.x;
+  const int W = td.ncp + 16;
+  const double* src = part + (g / 16) * ct.nchunk * 16 * W + (g % 16) * W;
+  for (int col = lane; col < td.ncp; col += 64) {
+    double a = 0.0;
+    for (int ch = 0; ch < ct.nchunk; ++ch) a += src[(int64_t)ch * 16 * W + col];
+    accw[col] = a;
+  }
+  // lane t < NPAIR owns meat entry t = tri(a, b)
+  int ma = 0, mb = 0;
+  if (lane < NPAIR) {
+    int t = lane, a = 0;
+    while (t >= P - a) {
+      t -= P - a;
+      ++a;
+    }
+    ma = a;
+    mb = a + t;
+    meat[lane] = 0.0;
+  }
+  if (C > 0 && sandwich) {
+    int cl = ct.cl[0];
+    double S = 0.0;
+    for (int ch = 0; ch <= ct.nchunk; ++ch) {
+      const bool flush = ch == ct.nchunk || ct.cl[ch] != cl;
+      if (flush) {
+        if (lane < 16) sc[lane] = S;
+        __syncthreads();
+        if (lane < NPAIR) {
+          const int ia = td.kidx_lin[ma], ib = td.kidx_lin[mb];
+          meat[lane] += (ia >= 0 ? sc[ia] : 0.0) * (ib >= 0 ? sc[ib] : 0.0);
+        }
+        __syncthreads();
+        S = 0.0;
+        if (ch == ct.nchunk) break;
+        cl = ct.cl[ch];
+      }
+      if (lane < 16) S += src[(int64_t)ch * 16 * W + td.ncp + lane];
+    }
+  }
+  __syncthreads();
+  if (lane != 0) return;
+  double Hm[P][P], Bi[P][P];
+#pragma unroll
+  for (int a = 0; a < P; ++a)
+#pragma unroll
+    for (int bb = a; bb < P; ++bb) {
+      const int pi = td.kidx_pair[tri<P>(a, bb)];
+      const double v = pi >= 0 ? accw[pi] : 0.0;
+      Hm[a][bb] = v;
+      Hm[bb][a] = v;
+    }
+  const int sing = lu_inverse<P>(Hm, Bi);
+  double* out = bread + g * P * P;
+#pragma unroll
+  for (int a = 0; a < P; ++a)
+#pragma unroll
+    for (int bb = 0; bb < P; ++bb) out[a + bb * P] = Bi[a][bb];
+  if (status && sing) status[g] |= DEVIL_GENE_SINGULAR;
+  if (C > 0 && sandwich) {
+    const double adj = (C > 1) ? (double)C / (C - 1.0) : 1.0;
+    const double scl = adj / (double)n;
+    double Me[P][P], T1[P][P];
+#pragma unroll
+    for (int a = 0; a < P; ++a)
+#pragma unroll
+      for (int bb = a; bb < P; ++bb) {
+        const double v = scl * meat[tri<P>(a, bb)];
+        Me[a][bb] = v;
+        Me[bb][a] = v;
+      }
+#pragma unroll
+    for (int a = 0; a < P; ++a)
+#pragma unroll
+      for (int bb = 0; bb < P; ++bb) {
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < P; ++k) s += Bi[a][k] * Me[k][bb];
+        T1[a][bb] = s;
+      }
+    double* so = sandwich + g * P * P;
+#pragma unroll
+    for (int a = 0; a < P; ++a)
+#pragma unroll
+      for (int bb = 0; bb < P; ++bb) {
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < P; ++k) s += T1[a][k] * Bi[k][bb];
+        so[a + bb * P] = s * (double)n;
+      }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Init pass (R/overdispersion.R:129-135, R/beta.R:13-20, R/main.R:404-406,
+// src/beta_init_univariate.cpp:13-95) on the same tiling: per (gene, chunk)
+// the count sums sum y, sum y^2 (exact 64-bit integers for int32 counts) and
+// z' Z on v_mfma_f64_16x16x4f64, where Z holds the p columns of the
+// projection (thin Q of X for the least-squares start, X - mean(X) for the
+// univariate start) and z = log1p(y / s) or log(y + 0.5) - log s.
+// ---------------------------------------------------------------------------
+template <int P, typename CT>
+__device__ __forceinline__ void init_tile_chunk(const TileDesign& td, InitTile it, ChunkTable ct,
+                                                const CT* __restrict__ Y, int64_t ld, int64_t G,
+                                                int tile, int chunk, double* __restrict__ part) {
+  const int lane = threadIdx.x & 63;
+  const int s = lane & 15, h = lane >> 4;
+  const int64_t g0 = (int64_t)tile * 16 + s;
+  const int64_t g = g0 < G ? g0 : (int64_t)tile * 16;
+  const CT* yrow = Y + g * ld;
+  const bool vec = (ld & 3) == 0 && ((reinterpret_cast<uintptr_t>(Y) & 15) == 0);
+  const int64_t c0 = ct.c0[chunk], c1 = ct.c1[chunk];
+  const int64_t b0 = c0 / 16, b1 = c0 < c1 ? (c1 + 15) / 16 : b0;
+  d4 az = d4{0.0, 0.0, 0.0, 0.0};
+  int64_t s1i = 0, s2i = 0;
+  double s1d = 0.0, s2d = 0.0;
+  for (int64_t b = b0; b < b1; ++b) {
+    const int64_t cb = b * 16 + 4 * h;
+    double y[4];
+    if (cb + 4 <= td.n) {
+      load4<CT>(yrow + cb, vec, y);
+    } else {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) y[r] = (cb + r < td.n) ? (double)yrow[cb + r] : 0.0;
+    }
+    double z[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const bool in = cb + r >= c0 && cb + r < c1;
+      if (!in) y[r] = 0.0;
+      if (it.mode == kInitQR) {
+        z[r] = y[r] != 0.0 ? log1p_f64(y[r] * it.einv[cb + r]) : 0.0;
+      } else if (it.mode == kInitUnivariate) {
+        z[r] = in ? log(y[r] + 0.5) - it.logsf[cb + r] : 0.0;
+      } else {
+        z[r] = 0.0;
+      }
+      if (sizeof(CT) == 4 && CT(0.5) == CT(0)) {  // integer counts
+        const int64_t yi = (int64_t)y[r];
+        s1i += yi;
+        s2i += yi * yi;
+      } else {
+        s1d += y[r];
+        s2d += y[r] * y[r];
+      }
+    }
+    if (it.mode == kInitQR || it.mode == kInitUnivariate) {
+      const d4 zb = *reinterpret_cast<const d4*>(it.Zp + (b * 64 + lane) * 4);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) az = mfma16x16x4(z[r], zb[r], az);
+    }
+  }
+#pragma unroll
+  for (int o = 16; o <= 32; o <<= 1) {
+    s1i += __shfl_xor(s1i, o, 64);
+    s2i += __shfl_xor(s2i, o, 64);
+    s1d += __shfl_xor(s1d, o, 64);
+    s2d += __shfl_xor(s2d, o, 64);
+  }
+  // register r holds gene slot h + 4 r, column s
+  double* out = part + ((int64_t)tile * ct.nchunk + chunk) * 16 * 20;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) out[(h + 4 * r) * 20 + s] = az[r];
+  if (h == 0) {
+    double* o = out + s * 20 + 16;
+    o[0] = s1d;
+    o[1] = s2d;
+    reinterpret_cast<int64_t*>(o)[2] = s1i;
+    reinterpret_cast<int64_t*>(o)[3] = s2i;
+  }
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void k_init_accum(TileDesign td, InitTile it, ChunkTable ct,
+                                                    Counts c, int64_t G,
+                                                    double* __restrict__ part) {
+  const int wave = threadIdx.x >> 6;
+  const int chunk = blockIdx.x % ct.nchunk;
+  const int tile = (blockIdx.x / ct.nchunk) * 4 + wave;
+  if ((int64_t)tile * 16 >= G) return;
+  if (c.dtype == 0)
+    init_tile_chunk<P>(td, it, ct, static_cast<const int32_t*>(c.ptr), c.ld, G, tile, chunk,
+                       part);
+  else if (c.dtype == 1)
+    init_tile_chunk<P>(td, it, ct, static_cast<const float*>(c.ptr), c.ld, G, tile, chunk, part);
+  else
+    init_tile_chunk<P>(td, it, ct, static_cast<const double*>(c.ptr), c.ld, G, tile, chunk,
+                       part);
+}
+
+// n * s2 - s1^2 exactly (128-bit), then one rounding
+__device__ __forceinline__ double exact_ssd(int64_t s1, int64_t s2, int64_t n) {
+  const __int128 v = (__int128)n * (__int128)s2 - (__int128)s1 * (__int128)s1;
+  return (double)v;
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void k_init_finish(InitTile it, ChunkTable ct, int64_t G,
+                                                     int64_t n, int integer_counts,
+                                                     const double* __restrict__ part,
+                                                     double* __restrict__ alpha0,
+                                                     double* __restrict__ beta0) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= G) return;
+  const int64_t tile = g / 16, s = g % 16;
+  double w[P], s1d = 0.0, s2d = 0.0;
+  int64_t s1i = 0, s2i = 0;
+#pragma unroll
+  for (int j = 0; j < P; ++j) w[j] = 0.0;
+  for (int ch = 0; ch < ct.nchunk; ++ch) {
+    const double* src = part + ((tile * ct.nchunk + ch) * 16 + s) * 20;
+#pragma unroll
+    for (int j = 0; j < P; ++j) w[j] += src[j];
+    s1d += src[16];
+    s2d += src[17];
+    s1i += reinterpret_cast<const int64_t*>(src)[18];
+    s2i += reinterpret_cast<const int64_t*>(src)[19];
+  }
+  double s1, ssd;  // sum y and sum (y - mean)^2
+  if (integer_counts) {
+    s1 = (double)s1i;
+    ssd = exact_ssd(s1i, s2i, n) / (double)n;
+  } else {
+    s1 = s1d;
+    ssd = s2d - s1d * s1d / (double)n;
+  }
+  const double mean = s1 / (double)n;
+  if (it.want_alpha) {
+    const double v = ssd / (double)(n - 1);
+    const double d = (v - it.moi * mean) / (mean * mean);
+    alpha0[g] = (isnan(d) || d < 0.0) ? 0.01 : d;
+  }
+  if (it.mode == kInitNone) return;
+  double b[P];
+  if (it.mode == kInitQR) {
+#pragma unroll
+    for (int a = P - 1; a >= 0; --a) {
+      double t = w[a];
+#pragma unroll
+      for (int k = a + 1; k < P; ++k) t -= it.R[a + k * P] * b[k];
+      b[a] = t / it.R[a + a * P];
+    }
+  } else if (it.mode == kInitRough) {
+    b[0] = log1p(mean);
+#pragma unroll
+    for (int j = 1; j < P; ++j) b[j] = 0.0;
+  } else {
+#pragma unroll
+    for (int j = 0; j < P; ++j) b[j] = w[j] / it.denom[j];
+    if (it.intercept_first) b[0] = log((s1 + 1e-8) / (it.sum_sf + 1e-8));
+  }
+#pragma unroll
+  for (int j = 0; j < P; ++j) beta0[g * P + j] = b[j];
+}
+
+#define DV_POST_P(p, KERNEL, GRID, BLOCK, STREAM, ...)                               \
+  switch (p) {                                                                      \
+    case 1: hipLaunchKernelGGL(KERNEL<1>, GRID, BLOCK, 0, STREAM, __VA_ARGS__); break; \
+    case 2: hipLaunchKernelGGL(KERNEL<2>, GRID, BLOCK, 0, STREAM, __VA_ARGS__); break; \
+    case 3: hipLaunchKernelGGL(KERNEL<3>, GRID, BLOCK, 0, STREAM, __VA_ARGS__); break; \
+    case 4: hipLaunchKernelGGL(KERNEL<4>, GRID, BLOCK, 0, STREAM, __VA_ARGS__); break; \
+    case 5: hipLaunchKernelGGL(KERNEL<5>, GRID, BLOCK, 0, STREAM, __VA_ARGS__); break; \
+    case 6: hipLaunchKernelGGL(KERNEL<6>, GRID, BLOCK, 0, STREAM, __VA_ARGS__); break; \
+    case 7: hipLaunchKernelGGL(KERNEL<7>, GRID, BLOCK, 0, STREAM, __VA_ARGS__); break; \
+    case 8: hipLaunchKernelGGL(KERNEL<8>, GRID, BLOCK, 0, STREAM, __VA_ARGS__); break; \
+    case 9: hipLaunchKernelGGL(KERNEL<9>, GRID, BLOCK, 0, STREAM, __VA_ARGS__); break; \
+    case 10: hipLaunchKernelGGL(KERNEL<10>, GRID, BLOCK, 0, STREAM, __VA_ARGS__); break; \
+    default: return hipErrorInvalidValue;                                           \
+  }
+
+hipError_t launch_mom_tiled(hipStream_t s, const TileDesign& td, const ChunkTable& ct, Counts c,
+                            int64_t G, int64_t n, int p, const double* beta, double* part,
+                            double* theta, int32_t* status) {
+  if (G <= 0) return hipSuccess;
+  const int64_t groups = ((G + 15) / 16 + 3) / 4;
+  DV_POST_P(p, k_mom_accum, dim3((unsigned)(groups * ct.nchunk)), dim3(256), s, td, ct, c, G,
+            beta, part);
+  DV_POST_P(p, k_mom_finish, dim3((unsigned)((G + 255) / 256)), dim3(256), s, ct, G, n, part,
+            theta, status);
+  return hipGetLastError();
+}
+
+hipError_t launch_bread_tiled(hipStream_t s, const TileDesign& td, const ChunkTable& ct,
+                              Counts c, int64_t G, int64_t n, int p, const double* beta,
+                              const double* theta, int C, double* part, double* bread,
+                              double* sandwich, int32_t* status) {
+  if (G <= 0) return hipSuccess;
+  const int64_t groups = ((G + 15) / 16 + 3) / 4;
+  DV_POST_P(p, k_bread_accum, dim3((unsigned)(groups * ct.nchunk)), dim3(256), s, td, ct, c, G,
+            beta, theta, part);
+  DV_POST_P(p, k_bread_finish, dim3((unsigned)G), dim3(64), s, td, ct, G, n, C, part, bread,
+            sandwich, status);
+  return hipGetLastError();
+}
+
+hipError_t launch_init_tiled(hipStream_t s, const TileDesign& td, const InitTile& it,
+                             const ChunkTable& ct, Counts c, int64_t G, int64_t n, int p,
+                             double* part, double* alpha0, double* beta0) {
+  if (G <= 0) return hipSuccess;
+  const int64_t groups = ((G + 15) / 16 + 3) / 4;
+  DV_POST_P(p, k_init_accum, dim3((unsigned)(groups * ct.nchunk)), dim3(256), s, td, it, ct, c,
+            G, part);
+  DV_POST_P(p, k_init_finish, dim3((unsigned)((G + 255) / 256)), dim3(256), s, it, ct, G, n,
+            c.dtype == 0 ? 1 : 0, part, alpha0, beta0);
+  return hipGetLastError();
+}
+
+}  // namespace dv

@@ -199,6 +199,7 @@ __device__ __forceinline__ void irls_tile_chunk_mixed(const TileDesign& td,
   BlockOpsF<NJ, NTMAX> cur, nxt;
   if (b0 < b1) load_block_f<NJ, NTMAX, CT>(td, yrow, vec, b0, lane, h, NT, cur);  // trailing chunks may be empty
   int span = 0;
+  float tmin = 0.0f;  // smallest log2(mu) seen by this lane
   for (int64_t b = b0; b < b1; ++b) {
     if (b + 1 < b1) load_block_f<NJ, NTMAX, CT>(td, yrow, vec, b + 1, lane, h, NT, nxt);
     f4 eta = f4{0.f, 0.f, 0.f, 0.f};
@@ -212,7 +213,9 @@ __device__ __forceinline__ void irls_tile_chunk_mixed(const TileDesign& td,
     float q[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const float mu = __builtin_amdgcn_exp2f((eta[r] + cur.off[r]) * 1.44269504088896341f);
+      const float t2 = (eta[r] + cur.off[r]) * 1.44269504088896341f;
+      tmin = fminf(tmin, t2);
+      const float mu = __builtin_amdgcn_exp2f(t2);
       q[r] = (cur.y[r] - mu) * __builtin_amdgcn_rcpf(k + mu);
     }
 #pragma unroll
@@ -234,12 +237,21 @@ __device__ __forceinline__ void irls_tile_chunk_mixed(const TileDesign& td,
     cur = nxt;
   }
   // register r holds gene slot 4 h + r, column 16 t + s
+  // exp(-eta - off) overflows in FP64 where log2(mu) < -1024: the reference
+  // then takes a NaN step, so such a gene's partials are made NaN and the
+  // gene is redone in FP64.  Register r of this lane holds gene slot 4 h + r.
+  tmin = fminf(tmin, __shfl_xor(tmin, 16, 64));
+  tmin = fminf(tmin, __shfl_xor(tmin, 32, 64));
+  bool poison[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) poison[r] = __shfl(tmin, 4 * h + r, 64) < -1024.0f;
   double* out = part + ((int64_t)tile * td.nchunk + chunk) * 16 * td.ncp;
 #pragma unroll
   for (int t = 0; t < NTMAX; ++t) {
     if (t < NT) {
 #pragma unroll
-      for (int r = 0; r < 4; ++r) out[(4 * h + r) * td.ncp + 16 * t + s] = acc64[t][r];
+      for (int r = 0; r < 4; ++r)
+        out[(4 * h + r) * td.ncp + 16 * t + s] = poison[r] ? __builtin_nan("") : acc64[t][r];
     }
   }
 }
@@ -386,6 +398,7 @@ __device__ __forceinline__ void irls_tile_chunk_bf(const TileDesign& td, const C
   StepOpsB<NE, NTH> cur, nxt;
   if (s0 < s1) load_step_b<NE, NTH, CT>(td, yrow, vec, s0, lane, h, nth, cur);  // trailing chunks may be empty
   int span = 0;
+  float tmin = 0.0f;  // smallest log2(mu) seen by this lane
   for (int64_t st = s0; st < s1; ++st) {
     if (st + 1 < s1) load_step_b<NE, NTH, CT>(td, yrow, vec, st + 1, lane, h, nth, nxt);
     float rr[8];
@@ -396,7 +409,9 @@ __device__ __forceinline__ void irls_tile_chunk_bf(const TileDesign& td, const C
       for (int j = 0; j < NE; ++j) eta = mfma_bf(cur.xa[u][j], bb[j], eta);
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const float mu = __builtin_amdgcn_exp2f(eta[r] + cur.off[u][r]);
+        const float t2 = eta[r] + cur.off[u][r];
+        tmin = fminf(tmin, t2);
+        const float mu = __builtin_amdgcn_exp2f(t2);
         rr[4 * u + r] = (cur.y[4 * u + r] - mu) * __builtin_amdgcn_rcpf(k + mu);
       }
     }
@@ -420,12 +435,21 @@ __device__ __forceinline__ void irls_tile_chunk_bf(const TileDesign& td, const C
     }
     cur = nxt;
   }
+  // exp(-eta - off) overflows in FP64 where log2(mu) < -1024: the reference
+  // then takes a NaN step, so such a gene's partials are made NaN and the
+  // gene is redone in FP64.  Register r of this lane holds gene slot 4 h + r.
+  tmin = fminf(tmin, __shfl_xor(tmin, 16, 64));
+  tmin = fminf(tmin, __shfl_xor(tmin, 32, 64));
+  bool poison[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) poison[r] = __shfl(tmin, 4 * h + r, 64) < -1024.0f;
   double* out = part + ((int64_t)tile * td.nchunk + chunk) * 16 * td.ncp;
 #pragma unroll
   for (int t = 0; t <= NTH; ++t) {
     if (t <= nth) {
 #pragma unroll
-      for (int r = 0; r < 4; ++r) out[(4 * h + r) * td.ncp + 16 * t + s] = acc64[t][r];
+      for (int r = 0; r < 4; ++r)
+        out[(4 * h + r) * td.ncp + 16 * t + s] = poison[r] ? __builtin_nan("") : acc64[t][r];
     }
   }
 }

@@ -46,7 +46,12 @@ def relmax(a, b, floor=FLOOR):
     return float(e.max()) if e.size else 0.0
 
 
-@pytest.fixture(scope="module", params=["bf16", "f32"])
+# per-variant stated tolerance on beta and log10 p (f32 is the default kernel;
+# the bf16 kernel rounds the pair-column products to bf16 and is looser)
+KERNEL_TOL = {"f32": TOL, "bf16": 5e-4}
+
+
+@pytest.fixture(scope="module", params=["f32", "bf16"])
 def dv(request):
     """The package with the context's mixed kernel set to each variant."""
     if not torch.cuda.is_available():
@@ -56,8 +61,9 @@ def dv(request):
     ctx = devil_amd.context(0)
     say(f"[mixed kernel] {request.param}")
     ctx.set_mixed_kernel(request.param)
+    devil_amd.mixed_tol = KERNEL_TOL[request.param]
     yield devil_amd
-    ctx.set_mixed_kernel("bf16")
+    ctx.set_mixed_kernel("f32")
 
 
 def fit_and_oracle(dv, orc, w, Y, *, od="MOM", clusters=None, max_iter=200):
@@ -81,7 +87,8 @@ def check_mixed(dv, orc, w, fit, ref, contrast):
     lr = rr["log_pval"] / np.log(10.0)
     ep = relmax(lp, lr)
     say(f"  beta {eb:.2e}  theta {et:.2e}  log10 p {ep:.2e}")
-    assert eb <= TOL and ep <= TOL and et <= TOL
+    tol = dv.mixed_tol
+    assert eb <= tol and ep <= tol and et <= TOL
     calls = res["adj_pval"] < 0.05
     calls_ref = rr["adj_pval"] < 0.05
     np.testing.assert_array_equal(calls, calls_ref)
@@ -145,7 +152,7 @@ def test_mixed_edge_cases(dv, orc):
     Y[9] *= 1000                   # large counts: mu far from 1 in FP32
     fit, ref = fit_and_oracle(dv, orc, w, Y, max_iter=5)
     np.testing.assert_array_equal(fit["iterations"]["beta_iters"], ref["beta_iters"])
-    assert relmax(fit["beta"], ref["beta"], BETA_FLOOR) <= TOL
+    assert relmax(fit["beta"], ref["beta"], BETA_FLOOR) <= dv.mixed_tol
 
 
 def test_mixed_matches_fp64_gpu_at_scale(dv):
@@ -161,7 +168,7 @@ def test_mixed_matches_fp64_gpu_at_scale(dv):
     same = float(np.mean(it64 == itm))
     eb = relmax(mix["beta"].cpu().numpy(), f64["beta"].cpu().numpy(), BETA_FLOOR)
     say(f"  iterations equal {same:.4f}  beta {eb:.2e}")
-    assert same == 1.0 and eb <= TOL
+    assert same == 1.0 and eb <= dv.mixed_tol
 
 
 def test_mixed_stage_entry_point(dv, orc):
@@ -179,14 +186,15 @@ def test_mixed_stage_entry_point(dv, orc):
     Yf = Y.astype(np.float64)
     b0 = orc.init_beta_qr(Yf, w.X, w.offset)
     a0 = orc.estimate_dispersion(Yf, w.offset)
+    # device buffers held by name for the duration of the call
+    Yd, Xd, offd, a0d = t(Y, torch.int32), t(w.X.T.reshape(-1)), t(w.offset), t(a0)
     beta = t(b0.T.reshape(-1))
     it = torch.empty(G, dtype=torch.int32, device="cuda")
-    ctx.check(lib.devil_beta_fit(ctx.h, t(Y, torch.int32).data_ptr(), L.COUNTS_I32, G, n, n,
-                                 t(w.X.T.reshape(-1)).data_ptr(), p, beta.data_ptr(),
-                                 t(w.offset).data_ptr(), t(a0).data_ptr(), 200, 1e-3,
+    ctx.check(lib.devil_beta_fit(ctx.h, Yd.data_ptr(), L.COUNTS_I32, G, n, n, Xd.data_ptr(), p,
+                                 beta.data_ptr(), offd.data_ptr(), a0d.data_ptr(), 200, 1e-3,
                                  L.PREC_MIXED, it.data_ptr(), None))
     bn = beta.cpu().numpy().reshape(p, G).T
     for g in range(G):
         rb, ri = orc.beta_fit(Yf[g], w.X, b0[g], w.offset, a0[g], 200, 1e-3)
         assert ri == int(it[g])
-        assert relmax(bn[g], rb, BETA_FLOOR) <= TOL
+        assert relmax(bn[g], rb, BETA_FLOOR) <= dv.mixed_tol
