@@ -84,7 +84,21 @@ def compare(dv, mix, f64, de_mix, de_64):
     calls_f = cpu(de_64["adj_pval"]) < 0.05
     it_m = cpu(mix["iterations"]["beta_iters"])
     it_f = cpu(f64["iterations"]["beta_iters"])
+    def worst(a, b, floor):
+        a = np.asarray(a, np.float64)
+        b = np.asarray(b, np.float64)
+        e = np.abs(a - b) / np.maximum(np.abs(b), floor)
+        e[np.isnan(a) & np.isnan(b)] = 0.0
+        i = int(np.nanargmax(e))
+        return {"index": i, "mixed": float(a.flat[i]), "fp64": float(b.flat[i])}
+    sig = np.abs(lp_f) >= 1.0  # p <= 0.1
+    th_m, th_f = cpu(mix["overdispersion"]), cpu(f64["overdispersion"])
     return {"beta_max_rel_err": rel(cpu(mix["beta"]), cpu(f64["beta"]), 1e-3),
+            "log10p_max_rel_err_floor_1": rel(lp_m, lp_f, 1.0),
+            "log10p_max_rel_err_p_le_0.1": rel(lp_m[sig], lp_f[sig]) if sig.any() else 0.0,
+            "log10p_worst": worst(lp_m, lp_f, 1e-6),
+            "theta_max_rel_err_floor_1e-3": rel(th_m, th_f, 1e-3),
+            "theta_worst": worst(th_m, th_f, 1e-8),
             "beta_max_rel_err_floor_1e-6": rel(cpu(mix["beta"]), cpu(f64["beta"]), 1e-6),
             "beta_max_abs_err": float(np.nanmax(np.abs(cpu(mix["beta"]) - cpu(f64["beta"])))),
             "log10p_max_rel_err": rel(lp_m, lp_f),

@@ -530,7 +530,7 @@ struct TileHost {
   dv::TileDesign td{};   // FP64 design (16-cell blocks, packed columns)
   dv::TileDesign tdm{};  // mixed design (td for FP32, 32-cell steps for bf16)
   int mixed = kMixedNone;
-  DevBuf spec, specm, Xp, offp, eop, Kp, Xf, offf, Kf, xab, offl, kg, kh;
+  DevBuf spec, specm, Xp, offp, eop, Kp, Xf, offf, Kf, xab, offl, kg, kh, kh2;
 };
 
 // Xd, offd, eod: device n x p X, offset and exp(offset).
@@ -581,15 +581,19 @@ int make_tile(devil_ctx* ctx, const double* Xd, const double* offd, const double
     DV_ALLOC(th.xab, sizeof(uint16_t) * nst * 2 * NE * 64 * 8);
     DV_ALLOC(th.offl, sizeof(float) * nst * 32);
     DV_ALLOC(th.kg, sizeof(float) * nst * 64 * 8);
-    if (nth > 0) DV_ALLOC(th.kh, sizeof(uint16_t) * nst * nth * 64 * 8);
+    if (nth > 0) {
+      DV_ALLOC(th.kh, sizeof(uint16_t) * nst * nth * 64 * 8);
+      DV_ALLOC(th.kh2, sizeof(uint16_t) * nst * nth * 64 * 8);
+    }
     DV_HIP(dv::launch_build_tables_bf16(s, Xd, offd, n, p, th.specm.as<dv::ColSpec>(),
                                         (int)cm.spec.size(), nth, nst, th.xab.as<uint16_t>(),
                                         th.offl.as<float>(), th.kg.as<float>(),
-                                        th.kh.as<uint16_t>()));
+                                        th.kh.as<uint16_t>(), th.kh2.as<uint16_t>()));
     tm.xab = th.xab.as<uint16_t>();
     tm.offl = th.offl.as<float>();
     tm.kg = th.kg.as<float>();
     tm.kh = nth > 0 ? th.kh.as<uint16_t>() : nullptr;
+    tm.kh2 = nth > 0 ? th.kh2.as<uint16_t>() : nullptr;
     tm.eop = td.eop;
   }
   return DEVIL_OK;

@@ -258,8 +258,9 @@ __global__ __launch_bounds__(256) void k_mle_accum(TileDesign td, Counts c,
   const int nA = *nactive;
   const int ntile = (nA + 15) / 16;
   const int wave = threadIdx.x >> 6;
-  const int chunk = blockIdx.x % td.nchunk;
-  const int tile = (blockIdx.x / td.nchunk) * 4 + wave;
+  int chunk, group;
+  tile_coords(td.nchunk, chunk, group);
+  const int tile = group * 4 + wave;
   if (tile >= ntile) return;
   if (c.dtype == 0)
     mle_tile_chunk<P>(td, static_cast<const int32_t*>(c.ptr), c.ld, active, nA, beta, xs,

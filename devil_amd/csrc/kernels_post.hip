@@ -161,8 +161,9 @@ __global__ __launch_bounds__(256) void k_mom_accum(TileDesign td, ChunkTable ct,
                                                    int64_t G, const double* __restrict__ beta,
                                                    double* __restrict__ part) {
   const int wave = threadIdx.x >> 6;
-  const int chunk = blockIdx.x % ct.nchunk;
-  const int tile = (blockIdx.x / ct.nchunk) * 4 + wave;
+  int chunk, group;
+  tile_coords(ct.nchunk, chunk, group);
+  const int tile = group * 4 + wave;
   if ((int64_t)tile * 16 >= G) return;
   if (c.dtype == 0)
     mom_tile_chunk<P>(td, ct, static_cast<const int32_t*>(c.ptr), c.ld, G, beta, tile, chunk,
@@ -205,8 +206,9 @@ __global__ __launch_bounds__(256) void k_bread_accum(TileDesign td, ChunkTable c
                                                      const double* __restrict__ theta,
                                                      double* __restrict__ part) {
   const int wave = threadIdx.x >> 6;
-  const int chunk = blockIdx.x % ct.nchunk;
-  const int tile = (blockIdx.x / ct.nchunk) * 4 + wave;
+  int chunk, group;
+  tile_coords(ct.nchunk, chunk, group);
+  const int tile = group * 4 + wave;
   if ((int64_t)tile * 16 >= G) return;
   if (c.dtype == 0)
     bread_tile_chunk<P>(td, ct, static_cast<const int32_t*>(c.ptr), c.ld, G, beta, theta, tile,
@@ -410,8 +412,9 @@ __global__ __launch_bounds__(256) void k_init_accum(TileDesign td, InitTile it, 
                                                     Counts c, int64_t G,
                                                     double* __restrict__ part) {
   const int wave = threadIdx.x >> 6;
-  const int chunk = blockIdx.x % ct.nchunk;
-  const int tile = (blockIdx.x / ct.nchunk) * 4 + wave;
+  int chunk, group;
+  tile_coords(ct.nchunk, chunk, group);
+  const int tile = group * 4 + wave;
   if ((int64_t)tile * 16 >= G) return;
   if (c.dtype == 0)
     init_tile_chunk<P>(td, it, ct, static_cast<const int32_t*>(c.ptr), c.ld, G, tile, chunk,

@@ -129,13 +129,39 @@ struct BlockOpsF {
   f4 kb[NTMAX];
 };
 
+// Four consecutive counts as floats (counts are exact in FP32 up to 2^24).
 template <typename CT>
-__device__ __forceinline__ void load4f(const CT* p, bool vec, float (&y)[4]) {
+__device__ __forceinline__ void load4f(const CT* p, bool vec, float (&y)[4]);
+template <>
+__device__ __forceinline__ void load4f<int32_t>(const int32_t* p, bool vec, float (&y)[4]) {
+  if (vec) {
+    const i4v v = *reinterpret_cast<const i4v*>(p);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) y[r] = (float)v[r];
+  } else {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) y[r] = (float)p[r];
+  }
+}
+template <>
+__device__ __forceinline__ void load4f<float>(const float* p, bool vec, float (&y)[4]) {
+  if (vec) {
+    const f4v v = *reinterpret_cast<const f4v*>(p);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) y[r] = v[r];
+  } else {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) y[r] = p[r];
+  }
+}
+template <>
+__device__ __forceinline__ void load4f<double>(const double* p, bool vec, float (&y)[4]) {
   double t[4];
-  load4<CT>(p, vec, t);
+  load4<double>(p, vec, t);
 #pragma unroll
   for (int r = 0; r < 4; ++r) y[r] = (float)t[r];
 }
+
 
 template <int NJ, int NTMAX, typename CT>
 __device__ __forceinline__ void load_block_f(const TileDesign& td, const CT* yrow, bool vec,
@@ -266,8 +292,9 @@ __global__ __launch_bounds__(256) void k_irls_accum_mixed(TileDesign td, Counts 
   const int nA = *nactive;
   const int ntile = (nA + 15) / 16;
   const int wave = threadIdx.x >> 6;
-  const int chunk = blockIdx.x % td.nchunk;
-  const int tile = (blockIdx.x / td.nchunk) * 4 + wave;
+  int chunk, group;
+  tile_coords(td.nchunk, chunk, group);
+  const int tile = group * 4 + wave;
   if (tile >= ntile) return;
   if (c.dtype == 0)
     irls_tile_chunk_mixed<P>(td, static_cast<const int32_t*>(c.ptr), c.ld, active, nA, beta,
@@ -291,9 +318,11 @@ __global__ __launch_bounds__(256) void k_irls_accum_mixed(TileDesign td, Counts 
 //   r = (y - mu) / (k + mu), mu = 2^(eta + off log2 e)             (FP32 VALU)
 //   gradient tile (the p design columns): r' X on v_mfma_f32_16x16x4f32, one
 //        product per cell (k index h, step j -> cell c(h, j))
-//   pair tiles (x_a x_b): bf16(r)' K on v_mfma_f32_16x16x32_bf16, k index
-//        (h, i) -> cell c(h, i) = i < 4 ? 4h + i : 16 + 4h + i - 4
-// FP32 accumulators are flushed to FP64 every 8 steps (256 cells).
+//   pair tiles (x_a x_b): three bf16 products r1 K1 + r1 K2 + r2 K1 with
+//        r = r1 + r2 and K = K1 + K2 split into bf16 (about 16 significant
+//        bits per product), k index (h, i) -> cell
+//        c(h, i) = i < 4 ? 4h + i : 16 + 4h + i - 4
+// FP32 accumulators are flushed to FP64 every 4 steps (128 cells).
 // ---------------------------------------------------------------------------
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 
@@ -301,48 +330,23 @@ __device__ __forceinline__ f4 mfma_bf(bf8 a, bf8 b, f4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
-template <int NE, int NTH>
-struct StepOpsB {
-  float y[8];
-  f4 off[2];
-  bf8 xa[2][NE];
-  f4 kg[2];
-  bf8 kh[NTH];
-};
-
-// host-side operand tables of the bf16 design (per 32-cell step st):
-//   xab [st][u][j][lane] bf8 : entries 32 j + 8 (lane/16) + i of cell 32 st + 16 u + lane % 16
-//   offl[32 nstep]       f32 : off * log2(e)
-//   kg  [st][lane][8]    f32 : x_{lane%16}[c(lane/16, j)]
-//   kh  [st][t][lane]    bf8 : Kpair_{16 t + lane%16}[c(lane/16, i)]
-template <int NE, int NTH, typename CT>
-__device__ __forceinline__ void load_step_b(const TileDesign& td, const CT* yrow, bool vec,
-                                            int64_t st, int lane, int h, int nth,
-                                            StepOpsB<NE, NTH>& o) {
-  const int64_t c0 = st * 32 + 4 * h;
+// Counts of one 32-cell step for lane (s, h): cells 4h + r and 16 + 4h + r.
+template <typename CT>
+__device__ __forceinline__ void load_step_y(const TileDesign& td, const CT* yrow, bool vec,
+                                            int64_t st, int h, float (&y)[8]) {
 #pragma unroll
   for (int u = 0; u < 2; ++u) {
-    const int64_t c = c0 + 16 * u;
-    double t[4];
+    const int64_t c = st * 32 + 16 * u + 4 * h;
+    float t[4];
     if (c + 4 <= td.n) {
-      load4<CT>(yrow + c, vec, t);
+      load4f<CT>(yrow + c, vec, t);
     } else {
 #pragma unroll
-      for (int r = 0; r < 4; ++r) t[r] = (c + r < td.n) ? (double)yrow[c + r] : 0.0;
+      for (int r = 0; r < 4; ++r) t[r] = (c + r < td.n) ? (float)yrow[c + r] : 0.0f;
     }
 #pragma unroll
-    for (int r = 0; r < 4; ++r) o.y[4 * u + r] = (float)t[r];
-    o.off[u] = *reinterpret_cast<const f4*>(td.offl + c);
-#pragma unroll
-    for (int j = 0; j < NE; ++j)
-      o.xa[u][j] = reinterpret_cast<const bf8*>(td.xab)[((st * 2 + u) * NE + j) * 64 + lane];
+    for (int r = 0; r < 4; ++r) y[4 * u + r] = t[r];
   }
-  const f4* kg = reinterpret_cast<const f4*>(td.kg) + (st * 64 + lane) * 2;
-  o.kg[0] = kg[0];
-  o.kg[1] = kg[1];
-#pragma unroll
-  for (int t = 0; t < NTH; ++t)
-    if (t < nth) o.kh[t] = reinterpret_cast<const bf8*>(td.kh)[(st * nth + t) * 64 + lane];
 }
 
 template <int P, typename CT>
@@ -353,7 +357,7 @@ __device__ __forceinline__ void irls_tile_chunk_bf(const TileDesign& td, const C
                                                    int chunk, double* __restrict__ part) {
   constexpr int NE = (7 * P + 31) / 32;
   constexpr int NTH = ((P * (P + 1)) / 2 + 15) / 16;  // pair tiles at most
-  constexpr int kSpan = 8;                            // steps per FP32 span
+  constexpr int kSpan = 4;                            // steps per FP32 span (gradient tile)
   const int lane = threadIdx.x & 63;
   const int s = lane & 15, h = lane >> 4;
   const int slot = tile * 16 + s;
@@ -385,56 +389,91 @@ __device__ __forceinline__ void irls_tile_chunk_bf(const TileDesign& td, const C
   const float k = live ? (float)kvec[g] : 0.0f;
   const CT* yrow = Y + (int64_t)g * ld;
   const bool vec = (ld & 3) == 0 && ((reinterpret_cast<uintptr_t>(Y) & 15) == 0);
-  f4 acc[NTH + 1];
-  double acc64[NTH + 1][4];
+  // gradient tile: FP32 span sums flushed to FP64; pair tiles: FP32 over the
+  // chunk (relative error ~1e-6 of a Hessian entry).  Each accumulator's
+  // MFMAs are issued back to back (dependent accumulation runs without
+  // stalls; interleaving accumulators measured slower).
+  f4 ag0 = f4{0.f, 0.f, 0.f, 0.f};
+  double ag64[4] = {0.0, 0.0, 0.0, 0.0};
+  f4 ah[NTH];
 #pragma unroll
-  for (int t = 0; t <= NTH; ++t) {
-    acc[t] = f4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int r = 0; r < 4; ++r) acc64[t][r] = 0.0;
-  }
-  const int64_t s0 = (int64_t)chunk * td.chunk_blocks;
-  const int64_t s1 = min((int64_t)(chunk + 1) * td.chunk_blocks, td.nblk);
-  StepOpsB<NE, NTH> cur, nxt;
-  if (s0 < s1) load_step_b<NE, NTH, CT>(td, yrow, vec, s0, lane, h, nth, cur);  // trailing chunks may be empty
-  int span = 0;
+  for (int t = 0; t < NTH; ++t) ah[t] = f4{0.f, 0.f, 0.f, 0.f};
   float tmin = 0.0f;  // smallest log2(mu) seen by this lane
-  for (int64_t st = s0; st < s1; ++st) {
-    if (st + 1 < s1) load_step_b<NE, NTH, CT>(td, yrow, vec, st + 1, lane, h, nth, nxt);
+  const bf8* xab = reinterpret_cast<const bf8*>(td.xab);
+  const bf8* kh = reinterpret_cast<const bf8*>(td.kh);
+  const bf8* kh2 = reinterpret_cast<const bf8*>(td.kh2);
+  const f4* kgp = reinterpret_cast<const f4*>(td.kg);
+  auto step = [&](int64_t st, const float (&y)[8]) {
+    const f4 off0 = *reinterpret_cast<const f4*>(td.offl + st * 32 + 4 * h);
+    const f4 off1 = *reinterpret_cast<const f4*>(td.offl + st * 32 + 16 + 4 * h);
+    f4 eta0 = f4{0.f, 0.f, 0.f, 0.f}, eta1 = eta0;
+#pragma unroll
+    for (int j = 0; j < NE; ++j) eta0 = mfma_bf(xab[(st * 2 * NE + j) * 64 + lane], bb[j], eta0);
+#pragma unroll
+    for (int j = 0; j < NE; ++j)
+      eta1 = mfma_bf(xab[(st * 2 * NE + NE + j) * 64 + lane], bb[j], eta1);
+    const f4 kg0 = kgp[(st * 2) * 64 + lane], kg1 = kgp[(st * 2 + 1) * 64 + lane];
     float rr[8];
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
-      f4 eta = f4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int j = 0; j < NE; ++j) eta = mfma_bf(cur.xa[u][j], bb[j], eta);
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const float t2 = eta[r] + cur.off[u][r];
-        tmin = fminf(tmin, t2);
-        const float mu = __builtin_amdgcn_exp2f(t2);
-        rr[4 * u + r] = (cur.y[4 * u + r] - mu) * __builtin_amdgcn_rcpf(k + mu);
-      }
+    for (int r = 0; r < 8; ++r) {
+      const float t2 = (r < 4 ? eta0[r] : eta1[r - 4]) + (r < 4 ? off0[r] : off1[r - 4]);
+      tmin = fminf(tmin, t2);
+      const float mu = __builtin_amdgcn_exp2f(t2);
+      rr[r] = (y[r] - mu) * __builtin_amdgcn_rcpf(k + mu);
     }
     // gradient tile: FP32 products, cell c(h, j) for step j
 #pragma unroll
-    for (int j = 0; j < 8; ++j) acc[0] = mfma32(rr[j], cur.kg[j >> 2][j & 3], acc[0]);
-    bf8 rb;
+    for (int j = 0; j < 4; ++j) ag0 = mfma32(rr[j], kg0[j], ag0);
 #pragma unroll
-    for (int i = 0; i < 8; ++i) rb[i] = (__bf16)rr[i];
+    for (int j = 0; j < 4; ++j) ag0 = mfma32(rr[4 + j], kg1[j], ag0);
+    bf8 r1, r2;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      r1[i] = (__bf16)rr[i];
+      r2[i] = (__bf16)(rr[i] - (float)r1[i]);
+    }
+    bf8 k1[NTH], k2[NTH];
 #pragma unroll
     for (int t = 0; t < NTH; ++t)
-      if (t < nth) acc[t + 1] = mfma_bf(rb, cur.kh[t], acc[t + 1]);
-    if (++span == kSpan || st + 1 == s1) {
-      span = 0;
-#pragma unroll
-      for (int t = 0; t <= NTH; ++t) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) acc64[t][r] += (double)acc[t][r];
-        acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+      if (t < nth) {
+        k1[t] = kh[(st * nth + t) * 64 + lane];
+        k2[t] = kh2[(st * nth + t) * 64 + lane];
       }
+#pragma unroll
+    for (int t = 0; t < NTH; ++t)
+      if (t < nth) {
+        ah[t] = mfma_bf(r2, k1[t], ah[t]);
+        ah[t] = mfma_bf(r1, k2[t], ah[t]);
+        ah[t] = mfma_bf(r1, k1[t], ah[t]);
+      }
+  };
+  auto flush = [&]() {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) ag64[r] += (double)ag0[r];
+    ag0 = f4{0.f, 0.f, 0.f, 0.f};
+  };
+  const int64_t s0 = (int64_t)chunk * td.chunk_blocks;
+  const int64_t s1 = min((int64_t)(chunk + 1) * td.chunk_blocks, td.nblk);
+  // counts of the next step are loaded a step ahead, into two buffers in turn
+  float yA[8], yB[8];
+  if (s0 < s1) load_step_y<CT>(td, yrow, vec, s0, h, yA);  // trailing chunks may be empty
+  int span = 0;
+  for (int64_t st = s0; st < s1; st += 2) {
+    if (st + 1 < s1) load_step_y<CT>(td, yrow, vec, st + 1, h, yB);
+    step(st, yA);
+    if (++span == kSpan) {
+      span = 0;
+      flush();
     }
-    cur = nxt;
+    if (st + 1 >= s1) break;
+    if (st + 2 < s1) load_step_y<CT>(td, yrow, vec, st + 2, h, yA);
+    step(st + 1, yB);
+    if (++span == kSpan) {
+      span = 0;
+      flush();
+    }
   }
+  flush();
   // exp(-eta - off) overflows in FP64 where log2(mu) < -1024: the reference
   // then takes a NaN step, so such a gene's partials are made NaN and the
   // gene is redone in FP64.  Register r of this lane holds gene slot 4 h + r.
@@ -445,12 +484,12 @@ __device__ __forceinline__ void irls_tile_chunk_bf(const TileDesign& td, const C
   for (int r = 0; r < 4; ++r) poison[r] = __shfl(tmin, 4 * h + r, 64) < -1024.0f;
   double* out = part + ((int64_t)tile * td.nchunk + chunk) * 16 * td.ncp;
 #pragma unroll
-  for (int t = 0; t <= NTH; ++t) {
-    if (t <= nth) {
+  for (int r = 0; r < 4; ++r) {
+    const double nanv = __builtin_nan("");
+    out[(4 * h + r) * td.ncp + s] = poison[r] ? nanv : ag64[r];
 #pragma unroll
-      for (int r = 0; r < 4; ++r)
-        out[(4 * h + r) * td.ncp + 16 * t + s] = poison[r] ? __builtin_nan("") : acc64[t][r];
-    }
+    for (int t = 0; t < NTH; ++t)
+      if (t < nth) out[(4 * h + r) * td.ncp + 16 * (t + 1) + s] = poison[r] ? nanv : (double)ah[t][r];
   }
 }
 
@@ -464,8 +503,9 @@ __global__ __launch_bounds__(256) void k_irls_accum_bf16(TileDesign td, Counts c
   const int nA = *nactive;
   const int ntile = (nA + 15) / 16;
   const int wave = threadIdx.x >> 6;
-  const int chunk = blockIdx.x % td.nchunk;
-  const int tile = (blockIdx.x / td.nchunk) * 4 + wave;
+  int chunk, group;
+  tile_coords(td.nchunk, chunk, group);
+  const int tile = group * 4 + wave;
   if (tile >= ntile) return;
   if (c.dtype == 0)
     irls_tile_chunk_bf<P>(td, static_cast<const int32_t*>(c.ptr), c.ld, active, nA, beta, kvec,
@@ -488,8 +528,9 @@ __global__ __launch_bounds__(256) void k_irls_accum(TileDesign td, Counts c,
   const int nA = *nactive;
   const int ntile = (nA + 15) / 16;
   const int wave = threadIdx.x >> 6;
-  const int chunk = blockIdx.x % td.nchunk;
-  const int tile = (blockIdx.x / td.nchunk) * 4 + wave;
+  int chunk, group;
+  tile_coords(td.nchunk, chunk, group);
+  const int tile = group * 4 + wave;
   if (tile >= ntile) return;
   if (c.dtype == 0)
     irls_tile_chunk<P>(td, static_cast<const int32_t*>(c.ptr), c.ld, active, nA, beta, kvec,

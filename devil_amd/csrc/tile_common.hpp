@@ -9,6 +9,24 @@
 
 namespace dv {
 
+// (chunk, gene group) of this workgroup in a grid of groups x nchunk.  Blocks
+// are dealt to the 8 XCDs round-robin by blockIdx; with nchunk a multiple of 8
+// the XCD b % 8 takes the chunks congruent to it and walks them chunk-major,
+// so the workgroups resident on one XCD share one chunk's operand tables in
+// that XCD's L2.
+__device__ __forceinline__ void tile_coords(int nchunk, int& chunk, int& group) {
+  const int b = blockIdx.x;
+  const int groups = gridDim.x / nchunk;
+  if ((nchunk & 7) == 0) {
+    const int x = b & 7, i = b >> 3;
+    chunk = x + 8 * (i / groups);
+    group = i % groups;
+  } else {
+    chunk = b % nchunk;
+    group = b / nchunk;
+  }
+}
+
 // Four consecutive counts of a row as doubles.
 template <typename CT>
 __device__ __forceinline__ void load4(const CT* p, bool vec, double (&y)[4]);
