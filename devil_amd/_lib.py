@@ -68,7 +68,7 @@ def lib():
         "devil_strerror": (C.c_char_p, [C.c_int]),
         "devil_last_error": (C.c_char_p, [_vp]),
         "devil_version": (C.c_char_p, []),
-        "devil_fit": (C.c_int, [_vp, C.POINTER(Problem), C.POINTER(Outputs)]),
+        "devil_fit": (C.c_int, [_vp, C.POINTER(Problem), C.POINTER(Outputs), C.c_int32]),
         "devil_estimate_dispersion": (C.c_int, [_vp, _vp, i32, i64, i64, i64, _vp, _vp]),
         "devil_init_beta": (C.c_int, [_vp, _vp, i32, i64, i64, i64, _vp, i32, _vp, i32, _vp]),
         "devil_beta_fit": (C.c_int, [_vp, _vp, i32, i64, i64, i64, _vp, i32, _vp, _vp, _vp,
@@ -84,6 +84,7 @@ def lib():
         "devil_set_timing": (C.c_int, [_vp, C.c_int]),
         "devil_stage_times": (C.c_int, [_vp, C.POINTER(C.c_double)]),
         "devil_set_tuning": (C.c_int, [_vp, C.c_int32, C.c_int64]),
+        "devil_fit_info": (C.c_int, [_vp, C.c_int32, C.POINTER(C.c_int64)]),
         "devil_kernel_stats": (C.c_int, [_vp, C.c_int32, C.POINTER(C.c_int64),
                                          C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     }
@@ -99,12 +100,12 @@ EXPORTED = ["devil_create", "devil_destroy", "devil_strerror", "devil_last_error
             "devil_version", "devil_fit", "devil_estimate_dispersion", "devil_init_beta",
             "devil_beta_fit", "devil_mom_dispersion", "devil_fit_dispersion_mle",
             "devil_bread_meat", "devil_test_de", "devil_set_timing", "devil_stage_times",
-            "devil_kernel_stats", "devil_set_tuning"]
+            "devil_kernel_stats", "devil_set_tuning", "devil_fit_info"]
 
 STAGES = ["init", "irls", "dispersion", "bread_meat", "test_de"]
 KERNELS = ["irls_fp64", "irls_mixed"]
-TUNE_MIXED_KERNEL = 0
-MIXED_KERNELS = {"f32": 0, "bf16": 1}
+TUNE_MIXED_KERNEL, TUNE_CHUNK_CELLS, TUNE_CONCURRENCY = 0, 1, 2
+MIXED_KERNELS = {"f32": 0, "bf16": 1, "auto": 2}
 
 
 class Context:
@@ -135,8 +136,23 @@ class Context:
         return dict(zip(STAGES, list(arr)))
 
     def set_mixed_kernel(self, name: str):
-        """Kernel of the mixed-precision IRLS: "f32" or "bf16"."""
+        """Kernel of the mixed-precision IRLS: "f32", "bf16" or "auto"."""
         self.check(lib().devil_set_tuning(self.h, TUNE_MIXED_KERNEL, MIXED_KERNELS[name]))
+
+    def set_tuning(self, key: int, value: int):
+        self.check(lib().devil_set_tuning(self.h, int(key), int(value)))
+
+    def fit_info(self) -> dict:
+        """Genes refit in FP64 by the last mixed fit, and its kernel and chunk
+        size (the autotuner's choice when tuning was left open)."""
+        out = {}
+        for k, name in enumerate(("replayed", "mixed_kernel", "chunk_cells")):
+            v = C.c_int64()
+            self.check(lib().devil_fit_info(self.h, k, C.byref(v)))
+            out[name] = int(v.value)
+        inv = {v: k for k, v in MIXED_KERNELS.items()}
+        out["mixed_kernel"] = inv.get(out["mixed_kernel"], "fp64")
+        return out
 
     def kernel_stats(self) -> dict:
         """{kernel: (launches, device ms, algorithmic bytes)} of the IRLS

@@ -59,6 +59,117 @@ def compute_offset_vector(off, n: int, size_factors) -> np.ndarray:
     return v
 
 
+class _Job:
+    """One problem's C-ABI structs, the buffers they point to and the
+    translation of its outputs to fit_devil's return value."""
+
+    def __init__(self, x, design_matrix, clusters=None, overdispersion: Any = "MOM",
+                 init_overdispersion=None, init_beta_rough: bool = False, offset: float = 0,
+                 size_factors=None, max_iter: int = 200, tolerance: float = 1e-3,
+                 precision: str = "fp64", init_mode: str | None = None):
+        X = np.asfortranarray(np.asarray(design_matrix, dtype=np.float64))
+        if X.ndim != 2:
+            raise ValueError("design_matrix must be a matrix")
+        n, p = X.shape
+        G = int(x.shape[0])
+        if int(x.shape[1]) != n:
+            raise ValueError("ncol(x) must equal nrow(design_matrix)")
+        if overdispersion is False:
+            od = L.OD_NONE
+        elif overdispersion in _OD_MODES:
+            od = _OD_MODES[overdispersion]
+        else:
+            raise ValueError(f"Unknown overdispersion mode: {overdispersion}")
+        if isinstance(size_factors, str):
+            raise NotImplementedError("size-factor estimation is outside the fitted path; "
+                                      "pass numeric size_factors")
+        sf = np.ones(n) if size_factors is None else np.asarray(size_factors, dtype=np.float64)
+        offset_vector = compute_offset_vector(offset, n, sf)
+        ends = None if clusters is None else cluster_blocks(clusters, n)
+        if init_mode is None:
+            init = L.INIT_ROUGH if init_beta_rough else L.INIT_QR
+        else:
+            init = {"qr": L.INIT_QR, "rough": L.INIT_ROUGH,
+                    "univariate": L.INIT_UNIVARIATE}[init_mode]
+        prec = {"fp64": L.PREC_FP64, "mixed": L.PREC_MIXED}[precision]
+        disp = None
+        if init_overdispersion is not None:
+            disp = np.broadcast_to(np.asarray(init_overdispersion, dtype=np.float64),
+                                   (G,)).copy()
+        self.G, self.n, self.p, self.od = G, n, p, od
+        self.meta = dict(size_factors=sf, offset_vector=offset_vector,
+                         design_matrix=design_matrix, input_matrix=x,
+                         input_parameters={"max_iter": max_iter, "tolerance": tolerance})
+        self.on_device = _is_torch(x) and x.is_cuda
+        common = dict(G=G, n=n, p=p, n_clusters=0 if ends is None else len(ends),
+                      beta_init=None, init_mode=init, od_mode=od, max_iter=int(max_iter),
+                      tolerance=float(tolerance), precision=prec)
+        if self.on_device:
+            import torch
+            dev = x.device
+            if not x.is_contiguous() and x.stride(1) != 1:
+                x = x.contiguous()
+            t = lambda a, dt=torch.float64: torch.as_tensor(a, dtype=dt, device=dev)
+            self.hold = dict(x=x, Xd=t(X.T.copy().reshape(-1)), offd=t(offset_vector),
+                             endsd=None if ends is None else t(ends, torch.int32),
+                             dispd=None if disp is None else t(disp))
+            e = lambda *shape, dt=torch.float64: torch.empty(*shape, dtype=dt, device=dev)
+            self.out = dict(beta=e(G * p), theta=e(G), bit=e(G, dt=torch.int32),
+                            tit=e(G, dt=torch.int32), bread=e(G * p * p),
+                            sw=e(G * p * p) if ends is not None else None,
+                            st=e(G, dt=torch.int32))
+            ptr = lambda a: None if a is None else a.data_ptr()
+            h = self.hold
+            self.prob = L.Problem(mem=L.MEM_DEVICE, counts=x.data_ptr(),
+                                  counts_dtype=_counts_dtype(x.dtype), counts_ld=x.stride(0),
+                                  X=ptr(h["Xd"]), offset=ptr(h["offd"]),
+                                  cluster_ends=ptr(h["endsd"]), disp_init=ptr(h["dispd"]),
+                                  **common)
+        else:
+            Y = np.asarray(x)
+            if not Y.flags.c_contiguous:
+                Y = np.ascontiguousarray(Y)
+            if Y.dtype not in (np.int32, np.float32, np.float64):
+                Y = Y.astype(np.float64)
+            self.hold = dict(Y=Y, X=X, off=offset_vector, ends=ends, disp=disp)
+            self.out = dict(beta=np.zeros((G, p), order="F"), theta=np.zeros(G),
+                            bit=np.zeros(G, np.int32), tit=np.zeros(G, np.int32),
+                            bread=np.zeros((G, p, p)),
+                            sw=np.zeros((G, p, p)) if ends is not None else None,
+                            st=np.zeros(G, np.int32))
+            ptr = lambda a: None if a is None else a.ctypes.data
+            self.prob = L.Problem(mem=L.MEM_HOST, counts=ptr(Y),
+                                  counts_dtype=_counts_dtype(Y.dtype), counts_ld=n, X=ptr(X),
+                                  offset=ptr(offset_vector), cluster_ends=ptr(ends),
+                                  disp_init=ptr(disp), **common)
+        o = self.out
+        self.outs = L.Outputs(beta=ptr(o["beta"]), theta=ptr(o["theta"]),
+                              beta_iters=ptr(o["bit"]), theta_iters=ptr(o["tit"]),
+                              bread=ptr(o["bread"]), sandwich=ptr(o["sw"]), status=ptr(o["st"]))
+
+    def result(self):
+        G, p, o = self.G, self.p, self.out
+        if self.on_device:
+            beta = o["beta"].view(p, G).T
+            bread = o["bread"].view(G, p, p).transpose(1, 2)
+            sw = None if o["sw"] is None else o["sw"].view(G, p, p).transpose(1, 2)
+        else:
+            # per-gene blocks are column-major: [g, a, b] = M[a + b p]
+            beta = o["beta"]
+            bread = o["bread"].transpose(0, 2, 1)
+            sw = None if o["sw"] is None else o["sw"].transpose(0, 2, 1)
+        res = {
+            "beta": beta,
+            "beta_sandwiches_null": bread,
+            "beta_sandwiches": sw,
+            "overdispersion": o["theta"],
+            "iterations": {"beta_iters": o["bit"], "theta_iters": o["tit"]},
+            "status": o["st"],
+        }
+        res.update(self.meta)
+        return res
+
+
 def fit_devil(x, design_matrix, clusters=None, overdispersion: Any = "MOM",
               init_overdispersion=None, init_beta_rough: bool = False, offset: float = 0,
               size_factors=None, verbose: bool = False, max_iter: int = 200,
@@ -66,121 +177,39 @@ def fit_devil(x, design_matrix, clusters=None, overdispersion: Any = "MOM",
               precision: str = "fp64", device: int = 0, init_mode: str | None = None):
     """Fit the per-gene NB GLM (R/main.R:164-273 with cpu_fit semantics).
 
-    ``x`` is genes x cells.  ``overdispersion`` is "MOM", "MLE"/"old" or
-    False.  ``batch_size`` is accepted for signature compatibility; the
-    library sizes its own gene batches.  ``precision`` is "fp64" or "mixed"
-    (FP32 MFMA contractions with FP64 solves and stopping decisions).
+    ``x`` is genes x cells (numpy on the host or a torch CUDA tensor).
+    ``overdispersion`` is "MOM", "MLE"/"old" or False.  ``batch_size`` is
+    accepted for signature compatibility; the library sizes its own gene
+    batches.  ``precision`` is "fp64" or "mixed" (FP32 element arithmetic and
+    FP32/bf16 MFMA contractions with FP64 solves and stopping decisions).
     """
     if not CUDA:
         raise NotImplementedError("devil_amd only implements the GPU fit (CUDA=True)")
     ctx = L.context(device)
-    X = np.asfortranarray(np.asarray(design_matrix, dtype=np.float64))
-    if X.ndim != 2:
-        raise ValueError("design_matrix must be a matrix")
-    n, p = X.shape
-    G = int(x.shape[0])
-    if int(x.shape[1]) != n:
-        raise ValueError("ncol(x) must equal nrow(design_matrix)")
-    if overdispersion is False:
-        od = L.OD_NONE
-    elif overdispersion in _OD_MODES:
-        od = _OD_MODES[overdispersion]
-    else:
-        raise ValueError(f"Unknown overdispersion mode: {overdispersion}")
-    if isinstance(size_factors, str):
-        raise NotImplementedError("size-factor estimation is outside the fitted path; "
-                                  "pass numeric size_factors")
-    sf = np.ones(n) if size_factors is None else np.asarray(size_factors, dtype=np.float64)
-    offset_vector = compute_offset_vector(offset, n, sf)
-    ends = None if clusters is None else cluster_blocks(clusters, n)
-    if init_mode is None:
-        init = L.INIT_ROUGH if init_beta_rough else L.INIT_QR
-    else:
-        init = {"qr": L.INIT_QR, "rough": L.INIT_ROUGH, "univariate": L.INIT_UNIVARIATE}[init_mode]
-    prec = {"fp64": L.PREC_FP64, "mixed": L.PREC_MIXED}[precision]
-    disp = None
-    if init_overdispersion is not None:
-        disp = np.broadcast_to(np.asarray(init_overdispersion, dtype=np.float64),
-                               (G,)).copy()
+    job = _Job(x, design_matrix, clusters=clusters, overdispersion=overdispersion,
+               init_overdispersion=init_overdispersion, init_beta_rough=init_beta_rough,
+               offset=offset, size_factors=size_factors, max_iter=max_iter,
+               tolerance=tolerance, precision=precision, init_mode=init_mode)
+    ctx.check(L.lib().devil_fit(ctx.h, C.byref(job.prob), C.byref(job.outs), 1))
+    return job.result()
 
-    on_device = _is_torch(x) and x.is_cuda
-    if on_device:
-        import torch
-        dev = x.device
-        if not x.is_contiguous() and x.stride(1) != 1:
-            x = x.contiguous()
-        ld = x.stride(0)
-        t = lambda a, dt=torch.float64: torch.as_tensor(a, dtype=dt, device=dev)
-        Xd = t(X.T.copy().reshape(-1))  # column-major n x p
-        offd = t(offset_vector)
-        endsd = None if ends is None else t(ends, torch.int32)
-        dispd = None if disp is None else t(disp)
-        beta = torch.empty(G * p, dtype=torch.float64, device=dev)
-        theta = torch.empty(G, dtype=torch.float64, device=dev)
-        bit = torch.empty(G, dtype=torch.int32, device=dev)
-        tit = torch.empty(G, dtype=torch.int32, device=dev)
-        bread = torch.empty(G * p * p, dtype=torch.float64, device=dev)
-        sw = torch.empty(G * p * p, dtype=torch.float64, device=dev) if ends is not None else None
-        st = torch.empty(G, dtype=torch.int32, device=dev)
-        ptr = lambda a: None if a is None else a.data_ptr()
-        prob = L.Problem(G=G, n=n, p=p, mem=L.MEM_DEVICE, counts=x.data_ptr(),
-                         counts_dtype=_counts_dtype(x.dtype), counts_ld=ld, X=ptr(Xd),
-                         offset=ptr(offd), cluster_ends=ptr(endsd),
-                         n_clusters=0 if ends is None else len(ends), disp_init=ptr(dispd),
-                         beta_init=None, init_mode=init, od_mode=od, max_iter=int(max_iter),
-                         tolerance=float(tolerance), precision=prec)
-        outs = L.Outputs(beta=ptr(beta), theta=ptr(theta), beta_iters=ptr(bit),
-                         theta_iters=ptr(tit), bread=ptr(bread), sandwich=ptr(sw),
-                         status=ptr(st))
-        ctx.check(L.lib().devil_fit(ctx.h, C.byref(prob), C.byref(outs)))
-        res = dict(beta=beta.view(p, G).T, theta=theta, beta_iters=bit, theta_iters=tit,
-                   bread=bread.view(G, p, p).transpose(1, 2),
-                   sandwich=None if sw is None else sw.view(G, p, p).transpose(1, 2),
-                   status=st)
-    else:
-        Y = np.asarray(x)
-        if not Y.flags.c_contiguous:
-            Y = np.ascontiguousarray(Y)
-        if Y.dtype not in (np.int32, np.float32, np.float64):
-            Y = Y.astype(np.float64)
-        beta = np.zeros((G, p), order="F")
-        theta = np.zeros(G)
-        bit = np.zeros(G, np.int32)
-        tit = np.zeros(G, np.int32)
-        bread = np.zeros((G, p, p))
-        sw = np.zeros((G, p, p)) if ends is not None else None
-        st = np.zeros(G, np.int32)
-        ptr = lambda a: None if a is None else a.ctypes.data
-        prob = L.Problem(G=G, n=n, p=p, mem=L.MEM_HOST, counts=ptr(Y),
-                         counts_dtype=_counts_dtype(Y.dtype), counts_ld=n, X=ptr(X),
-                         offset=ptr(offset_vector), cluster_ends=ptr(ends),
-                         n_clusters=0 if ends is None else len(ends), disp_init=ptr(disp),
-                         beta_init=None, init_mode=init, od_mode=od, max_iter=int(max_iter),
-                         tolerance=float(tolerance), precision=prec)
-        outs = L.Outputs(beta=ptr(beta), theta=ptr(theta), beta_iters=ptr(bit),
-                         theta_iters=ptr(tit), bread=ptr(bread), sandwich=ptr(sw),
-                         status=ptr(st))
-        ctx.check(L.lib().devil_fit(ctx.h, C.byref(prob), C.byref(outs)))
-        # per-gene blocks are column-major: [g, a, b] = M[a + b p]
-        res = dict(beta=beta, theta=theta, beta_iters=bit, theta_iters=tit,
-                   bread=bread.transpose(0, 2, 1), sandwich=None if sw is None else
-                   sw.transpose(0, 2, 1), status=st)
-    if od == L.OD_MLE:
-        # cpu_fit reports theta_iters = NA for the MLE mode (R/main.R:432)
-        res["theta_passes"] = res["theta_iters"]
-    return {
-        "beta": res["beta"],
-        "beta_sandwiches_null": res["bread"],
-        "beta_sandwiches": res["sandwich"],
-        "overdispersion": res["theta"],
-        "iterations": {"beta_iters": res["beta_iters"], "theta_iters": res["theta_iters"]},
-        "size_factors": sf,
-        "offset_vector": offset_vector,
-        "design_matrix": design_matrix,
-        "input_matrix": x,
-        "input_parameters": {"max_iter": max_iter, "tolerance": tolerance},
-        "status": res["status"],
-    }
+
+def fit_devil_many(problems, device: int = 0, **common):
+    """Fit several independent problems in one library call (devil_fit with
+    n_problems > 1: the problems run concurrently on the context's streams).
+
+    ``problems`` is a list of dicts of fit_devil arguments (at least ``x``
+    and ``design_matrix``); ``common`` supplies defaults for all of them.
+    Returns the list of fit_devil results in the same order.
+    """
+    ctx = L.context(device)
+    jobs = [_Job(**{**common, **pr}) for pr in problems]
+    if not jobs:
+        return []
+    probs = (L.Problem * len(jobs))(*[j.prob for j in jobs])
+    outs = (L.Outputs * len(jobs))(*[j.outs for j in jobs])
+    ctx.check(L.lib().devil_fit(ctx.h, probs, outs, len(jobs)))
+    return [j.result() for j in jobs]
 
 
 def test_de(devil_fit, contrast, clusters=None, pval_adjust_method: str = "BH",

@@ -8,6 +8,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <map>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -22,7 +24,16 @@ struct devil_ctx {
   hipStream_t stream = nullptr;
   std::string err;
   int timing = 0;
-  int tune[DEVIL_N_TUNE] = {DEVIL_MIXED_F32};
+  int tune[DEVIL_N_TUNE] = {DEVIL_MIXED_AUTO, 0, 4};  // kernel, chunk cells (0 auto), streams
+  std::map<uint64_t, int> tuned;      // autotuner choices by problem shape
+  std::vector<devil_ctx*> subs;       // per-stream contexts of multi-problem fits
+  int64_t replayed = 0;                // genes refit in FP64 by the mixed path (last fit)
+  int64_t replay_sum = 0;
+  int last_mk = 0;                     // mixed kernel and chunk cells of the last fit
+  int64_t last_cc = 0;
+  int64_t k_sum_n[DEVIL_N_KERNELS] = {0, 0};
+  double k_sum_ms[DEVIL_N_KERNELS] = {0, 0};
+  double k_sum_bytes[DEVIL_N_KERNELS] = {0, 0};
   double stage_ms[DEVIL_N_STAGES] = {0, 0, 0, 0, 0};
   // per-launch records of the IRLS accumulate kernels (timing on)
   struct Launch {
@@ -57,6 +68,15 @@ struct DevBuf {
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p) { o.p = nullptr; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) {
+      if (p) (void)hipFree(p);
+      p = o.p;
+      o.p = nullptr;
+    }
+    return *this;
+  }
   ~DevBuf() {
     if (p) (void)hipFree(p);
   }
@@ -407,7 +427,7 @@ struct StageTimer {
 enum MixedKernel { kMixedNone = 0, kMixedF32 = 1, kMixedBf16 = 2 };
 
 int mixed_kernel(const devil_ctx* ctx) {
-  return ctx->tune[DEVIL_TUNE_MIXED_KERNEL] == DEVIL_MIXED_F32 ? kMixedF32 : kMixedBf16;
+  return ctx->tune[DEVIL_TUNE_MIXED_KERNEL] == DEVIL_MIXED_BF16 ? kMixedBf16 : kMixedF32;
 }
 
 // Columns of K: x_a x_b (a <= b) and x_a of the design, exact duplicates
@@ -501,7 +521,8 @@ int plan_columns(devil_ctx* ctx, const double* Xd, int64_t n, int p, bool lin16,
 }
 
 // Shape of a design with cells_per_block-cell blocks over the given columns.
-void shape_design(const ColPlan& cp, int64_t n, int p, int cells_per_block, dv::TileDesign& td) {
+void shape_design(const ColPlan& cp, int64_t n, int p, int cells_per_block, dv::TileDesign& td,
+                  int64_t chunk_cells = 2048) {
   std::memset(&td, 0, sizeof(td));
   for (int a = 0; a < p; ++a) td.kidx_lin[a] = cp.kidx_lin[a];
   for (int q = 0; q < p * (p + 1) / 2; ++q) td.kidx_pair[q] = cp.kidx_pair[q];
@@ -512,7 +533,7 @@ void shape_design(const ColPlan& cp, int64_t n, int p, int cells_per_block, dv::
   td.ncp = 16 * td.nt;
   td.n = n;
   td.nblk = (n + cells_per_block - 1) / cells_per_block;
-  const int64_t min_blocks = 2048 / cells_per_block;
+  const int64_t min_blocks = std::max<int64_t>(1, chunk_cells / cells_per_block);
   td.chunk_blocks = std::max<int64_t>(min_blocks, (td.nblk + 63) / 64);
   td.nchunk = (int)((td.nblk + td.chunk_blocks - 1) / td.chunk_blocks);
   if (td.nchunk >= 8) {
@@ -535,14 +556,14 @@ struct TileHost {
 
 // Xd, offd, eod: device n x p X, offset and exp(offset).
 int make_tile(devil_ctx* ctx, const double* Xd, const double* offd, const double* eod,
-              int64_t n, int p, int mixed, TileHost& th) {
+              int64_t n, int p, int mixed, TileHost& th, int64_t chunk_cells = 2048) {
   hipStream_t s = ctx->stream;
   th.mixed = mixed;
   ColPlan cp;
   int rc;
   if ((rc = plan_columns(ctx, Xd, n, p, false, cp))) return rc;
   dv::TileDesign& td = th.td;
-  shape_design(cp, n, p, 16, td);
+  shape_design(cp, n, p, 16, td, chunk_cells);
   if ((rc = upload(ctx, th.spec, cp.spec))) return rc;
   const int NJ = (p + 3) / 4;
   DV_ALLOC(th.Xp, sizeof(double) * td.nblk * NJ * 64);
@@ -573,7 +594,7 @@ int make_tile(devil_ctx* ctx, const double* Xd, const double* offd, const double
     ColPlan cm;
     if ((rc = plan_columns(ctx, Xd, n, p, true, cm))) return rc;
     dv::TileDesign& tm = th.tdm;
-    shape_design(cm, n, p, 32, tm);
+    shape_design(cm, n, p, 32, tm, chunk_cells);
     if ((rc = upload(ctx, th.specm, cm.spec))) return rc;
     const int64_t nst = tm.nblk;
     const int NE = (7 * p + 31) / 32;
@@ -755,11 +776,54 @@ void collect_launches(devil_ctx* ctx) {
 }
 
 //
-// Mixed precision: each round accumulates in FP32 (FP64 partials); a gene
-// whose FP32-based step lands within kMixedBand * eps of the stopping
-// threshold (or is NaN) is not updated but recomputed in FP64 in the same
-// round, so stopping decisions and iteration counts follow FP64 arithmetic.
+// Mixed precision: each round accumulates in FP32/bf16 (FP64 partials).  A
+// gene whose step lands within kMixedBand * eps of the stopping threshold (or
+// is NaN) leaves the mixed rounds; once they end, those genes are refit from
+// their starting beta with the FP64 rounds.  Stopping decisions closer to the
+// threshold than the mixed trajectory's accuracy (~1e-8 in the step) are
+// therefore always FP64 decisions on an FP64 trajectory.
 constexpr double kMixedBand = 1e-3;
+constexpr int64_t kMixedMinCells = 4096;
+
+// Rounds of accumulate + update over the active list act (count ncnt on the
+// device, nA on the host) until none is left.  With band > 0 the mixed kernel
+// runs and genes near the threshold are appended to replay / nreplay.
+int irls_rounds(devil_ctx* ctx, const TileHost& tl, bool mixed, dv::Counts c, int p,
+                double* beta, const double* kvec, int max_iter, double eps, int32_t* iters,
+                int32_t* status, int32_t* act, int32_t* act_spare, int32_t* ncnt,
+                int32_t* ncnt_spare, int nA, double* part, int32_t* replay, int32_t* nreplay) {
+  const dv::TileDesign& td = mixed ? tl.tdm : tl.td;
+  hipStream_t s = ctx->stream;
+  const double row_bytes = (double)count_elem_size(c.dtype) * (double)td.n;
+  int32_t* cur = act;
+  int32_t* nxt = act_spare;
+  int32_t* ncur = ncnt;
+  int32_t* nnxt = ncnt_spare;
+  for (int it = 0; it < max_iter && nA > 0; ++it) {
+    {
+      LaunchTimer lt(ctx, mixed ? DEVIL_KERNEL_IRLS_MIXED : DEVIL_KERNEL_IRLS_FP64,
+                     row_bytes * nA);
+      if (!mixed)
+        DV_HIP(dv::launch_irls_accum(s, td, c, p, cur, ncur, nA, beta, kvec, part));
+      else if (tl.mixed == kMixedBf16)
+        DV_HIP(dv::launch_irls_accum_bf16(s, td, c, p, cur, ncur, nA, beta, kvec, part));
+      else
+        DV_HIP(dv::launch_irls_accum_mixed(s, td, c, p, cur, ncur, nA, beta, kvec, part));
+    }
+    DV_HIP(dv::launch_fill_i32(s, nnxt, 1, 0));
+    DV_HIP(dv::launch_irls_update(s, td, p, cur, ncur, nA, part, beta, kvec, max_iter, eps,
+                                  iters, status, nxt, nnxt, mixed ? kMixedBand : 0.0, replay,
+                                  nreplay));
+    int32_t h = 0;
+    DV_HIP(hipMemcpyAsync(&h, nnxt, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    DV_HIP(hipStreamSynchronize(s));
+    collect_launches(ctx);
+    nA = h;
+    std::swap(cur, nxt);
+    std::swap(ncur, nnxt);
+  }
+  return DEVIL_OK;
+}
 
 int run_irls_tiled(devil_ctx* ctx, const TileHost& tl, dv::Counts c, int64_t G, int p,
                    double* beta, const double* kvec, int max_iter, double eps,
@@ -768,75 +832,48 @@ int run_irls_tiled(devil_ctx* ctx, const TileHost& tl, dv::Counts c, int64_t G, 
   const dv::TileDesign& tdm = tl.tdm;
   const bool mixed = tl.mixed != kMixedNone;
   hipStream_t s = ctx->stream;
-  const size_t es = count_elem_size(c.dtype);
   DV_HIP(dv::launch_fill_i32(s, iters, G, 0));
   DV_HIP(dv::launch_fill_i32(s, status, G, 0));
   if (max_iter <= 0 || G == 0) {
     if (max_iter <= 0) DV_HIP(dv::launch_fill_i32(s, status, G, DEVIL_GENE_MAXITER));
     return DEVIL_OK;
   }
-  DevBuf act0, act1, cnt, part, rchk;
+  DevBuf act0, act1, cnt, part, rep, beta0;
   DV_ALLOC(act0, sizeof(int32_t) * G);
   DV_ALLOC(act1, sizeof(int32_t) * G);
-  DV_ALLOC(cnt, sizeof(int32_t) * 3);
-  if (mixed) DV_ALLOC(rchk, sizeof(int32_t) * G);
-  int32_t* nrchk = cnt.as<int32_t>() + 2;
+  DV_ALLOC(cnt, sizeof(int32_t) * 4);
   const int64_t ntile = (G + 15) / 16;
   size_t part_per_tile = (size_t)td.nchunk * 16 * td.ncp;
   if (mixed) part_per_tile = std::max(part_per_tile, (size_t)tdm.nchunk * 16 * tdm.ncp);
   DV_ALLOC(part, sizeof(double) * (size_t)ntile * part_per_tile);
+  int32_t* c0 = cnt.as<int32_t>();
   DV_HIP(dv::launch_iota(s, act0.as<int32_t>(), (int)G));
-  int32_t* cur = act0.as<int32_t>();
-  int32_t* nxt = act1.as<int32_t>();
-  int32_t* ncur = cnt.as<int32_t>();
-  int32_t* nnxt = cnt.as<int32_t>() + 1;
-  DV_HIP(dv::launch_fill_i32(s, ncur, 1, (int32_t)G));
-  int nA = (int)G;
-  for (int it = 0; it < max_iter && nA > 0; ++it) {
-    int32_t h[2] = {0, 0};
-    const double row_bytes = (double)es * (double)td.n;
-    if (!mixed) {
-      {
-        LaunchTimer lt(ctx, DEVIL_KERNEL_IRLS_FP64, row_bytes * nA);
-        DV_HIP(dv::launch_irls_accum(s, td, c, p, cur, ncur, nA, beta, kvec, part.as<double>()));
-      }
-      DV_HIP(dv::launch_fill_i32(s, nnxt, 1, 0));
-      DV_HIP(dv::launch_irls_update(s, td, p, cur, ncur, nA, part.as<double>(), beta, kvec,
-                                    max_iter, eps, iters, status, nxt, nnxt));
-    } else {
-      {
-        LaunchTimer lt(ctx, DEVIL_KERNEL_IRLS_MIXED, row_bytes * nA);
-        if (tl.mixed == kMixedBf16)
-          DV_HIP(dv::launch_irls_accum_bf16(s, tdm, c, p, cur, ncur, nA, beta, kvec,
-                                            part.as<double>()));
-        else
-          DV_HIP(dv::launch_irls_accum_mixed(s, tdm, c, p, cur, ncur, nA, beta, kvec,
-                                             part.as<double>()));
-      }
-      DV_HIP(dv::launch_fill_i32(s, nnxt, 1, 0));
-      DV_HIP(dv::launch_fill_i32(s, nrchk, 1, 0));
-      DV_HIP(dv::launch_irls_update(s, tdm, p, cur, ncur, nA, part.as<double>(), beta, kvec,
-                                    max_iter, eps, iters, status, nxt, nnxt, kMixedBand,
-                                    rchk.as<int32_t>(), nrchk));
-      DV_HIP(hipMemcpyAsync(&h[1], nrchk, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-      DV_HIP(hipStreamSynchronize(s));
-      if (h[1] > 0) {
-        LaunchTimer lt(ctx, DEVIL_KERNEL_IRLS_FP64, row_bytes * h[1]);
-        DV_HIP(dv::launch_irls_accum(s, td, c, p, rchk.as<int32_t>(), nrchk, h[1], beta, kvec,
-                                     part.as<double>()));
-        DV_HIP(dv::launch_irls_update(s, td, p, rchk.as<int32_t>(), nrchk, h[1],
-                                      part.as<double>(), beta, kvec, max_iter, eps, iters,
-                                      status, nxt, nnxt));
-      }
-    }
-    DV_HIP(hipMemcpyAsync(&h[0], nnxt, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    DV_HIP(hipStreamSynchronize(s));
-    collect_launches(ctx);
-    nA = h[0];
-    std::swap(cur, nxt);
-    std::swap(ncur, nnxt);
-  }
-  return DEVIL_OK;
+  DV_HIP(dv::launch_fill_i32(s, c0, 1, (int32_t)G));
+  int rc;
+  if (!mixed)
+    return irls_rounds(ctx, tl, false, c, p, beta, kvec, max_iter, eps, iters, status,
+                       act0.as<int32_t>(), act1.as<int32_t>(), c0, c0 + 1, (int)G,
+                       part.as<double>(), nullptr, nullptr);
+  DV_ALLOC(rep, sizeof(int32_t) * G);
+  DV_ALLOC(beta0, sizeof(double) * G * p);
+  DV_HIP(hipMemcpyAsync(beta0.p, beta, sizeof(double) * G * p, hipMemcpyDeviceToDevice, s));
+  int32_t* nrep = c0 + 2;
+  DV_HIP(dv::launch_fill_i32(s, nrep, 1, 0));
+  if ((rc = irls_rounds(ctx, tl, true, c, p, beta, kvec, max_iter, eps, iters, status,
+                        act0.as<int32_t>(), act1.as<int32_t>(), c0, c0 + 1, (int)G,
+                        part.as<double>(), rep.as<int32_t>(), nrep)))
+    return rc;
+  int32_t nr = 0;
+  DV_HIP(hipMemcpyAsync(&nr, nrep, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  DV_HIP(hipStreamSynchronize(s));
+  ctx->replayed += nr;
+  if (nr == 0) return DEVIL_OK;
+  // FP64 refit of the replay genes from their starting beta
+  DV_HIP(dv::launch_restart_genes(s, rep.as<int32_t>(), nr, p, beta0.as<double>(), beta, iters,
+                                  status));
+  return irls_rounds(ctx, tl, false, c, p, beta, kvec, max_iter, eps, iters, status,
+                     rep.as<int32_t>(), act1.as<int32_t>(), nrep, c0 + 3, nr,
+                     part.as<double>(), nullptr, nullptr);
 }
 
 // MLE overdispersion over genes [0, G): setup pass, then rounds of
@@ -900,6 +937,68 @@ int run_mle_tiled(devil_ctx* ctx, const dv::TileDesign& td, dv::Counts c, int64_
   return DEVIL_OK;
 }
 
+// Autotuner of the mixed-precision IRLS: times one accumulate pass over the
+// first min(G, 1024) genes for each candidate kernel and chunk size (those the
+// context leaves open) and returns the fastest.
+uint64_t tune_key(int64_t n, int p, int dtype) {
+  int lg = 0;
+  while ((int64_t(1) << lg) < n) ++lg;
+  return ((uint64_t)lg << 16) | ((uint64_t)p << 8) | (uint64_t)dtype;
+}
+
+int autotune_mixed(devil_ctx* ctx, const DeviceDesign& dd, int64_t n, int p, dv::Counts c,
+                   int64_t G, const double* beta, const double* kvec, int& mk, int64_t& cc) {
+  hipStream_t s = ctx->stream;
+  const int64_t Gs = std::min<int64_t>(G, 1024);
+  std::vector<int> kinds;
+  if (ctx->tune[DEVIL_TUNE_MIXED_KERNEL] == DEVIL_MIXED_AUTO) kinds = {kMixedF32, kMixedBf16};
+  else kinds = {mixed_kernel(ctx)};
+  std::vector<int64_t> cells;
+  if (ctx->tune[DEVIL_TUNE_CHUNK_CELLS] == 0) cells = {1024, 2048, 4096};
+  else cells = {(int64_t)ctx->tune[DEVIL_TUNE_CHUNK_CELLS]};
+  DevBuf act, cnt;
+  DV_ALLOC(act, sizeof(int32_t) * Gs);
+  DV_ALLOC(cnt, sizeof(int32_t));
+  DV_HIP(dv::launch_iota(s, act.as<int32_t>(), (int)Gs));
+  DV_HIP(dv::launch_fill_i32(s, cnt.as<int32_t>(), 1, (int32_t)Gs));
+  hipEvent_t e0 = pool_event(ctx), e1 = pool_event(ctx);
+  if (!e0 || !e1) return fail(ctx, DEVIL_E_HIP, "hipEventCreate");
+  float best = 1e30f;
+  int rc = DEVIL_OK;
+  for (int kind : kinds)
+    for (int64_t cl : cells) {
+      TileHost th;
+      if ((rc = make_tile(ctx, dd.x, dd.off.as<double>(), dd.eo.as<double>(), n, p, kind, th, cl)))
+        break;
+      const dv::TileDesign& tm = th.tdm;
+      DevBuf part;
+      DV_ALLOC(part, sizeof(double) * (size_t)((Gs + 15) / 16) * tm.nchunk * 16 * tm.ncp);
+      auto launch = [&]() {
+        return kind == kMixedBf16
+                   ? dv::launch_irls_accum_bf16(s, tm, c, p, act.as<int32_t>(), cnt.as<int32_t>(),
+                                                (int)Gs, beta, kvec, part.as<double>())
+                   : dv::launch_irls_accum_mixed(s, tm, c, p, act.as<int32_t>(), cnt.as<int32_t>(),
+                                                 (int)Gs, beta, kvec, part.as<double>());
+      };
+      DV_HIP(launch());
+      DV_HIP(hipEventRecord(e0, s));
+      DV_HIP(launch());
+      DV_HIP(launch());
+      DV_HIP(hipEventRecord(e1, s));
+      DV_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      DV_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best) {
+        best = ms;
+        mk = kind;
+        cc = cl;
+      }
+    }
+  ctx->ev_pool.push_back(e0);
+  ctx->ev_pool.push_back(e1);
+  return rc;
+}
+
 int set_device(devil_ctx* ctx) {
   if (!ctx) return DEVIL_E_ARG;
   DV_HIP(hipSetDevice(ctx->device));
@@ -958,10 +1057,16 @@ void devil_destroy(devil_ctx* ctx) {
     (void)hipEventDestroy(l.b);
   }
   for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
+  for (devil_ctx* sub : ctx->subs) devil_destroy(sub);
   delete ctx;
 }
 
-int devil_fit(devil_ctx* ctx, const devil_problem* pr, devil_outputs* out) {
+}  // extern "C"
+
+namespace {
+
+// One problem on ctx's stream (the body of devil_fit for n_problems == 1).
+int fit_one(devil_ctx* ctx, const devil_problem* pr, devil_outputs* out) {
   int rc;
   if ((rc = set_device(ctx))) return rc;
   if (!pr || !out) return fail(ctx, DEVIL_E_ARG, "NULL problem or outputs");
@@ -1002,9 +1107,25 @@ int devil_fit(devil_ctx* ctx, const devil_problem* pr, devil_outputs* out) {
   Design d;
   DeviceDesign dd;
   if ((rc = prepare_design(ctx, pr->X, pr->offset, n, p, mem, init_mode, d, dd))) return rc;
+  // mixed kernel and chunk size: fixed by the context, remembered from an
+  // earlier fit of this shape, or tuned after the first batch's init pass
+  // mixed precision pays where the counts stream is long: problems with fewer
+  // than kMixedMinCells cells run the FP64 kernels (exact; their cost is small)
+  const bool mixed = pr->precision == DEVIL_PREC_MIXED && n >= kMixedMinCells;
+  int mk = mixed ? mixed_kernel(ctx) : kMixedNone;
+  int64_t cc = ctx->tune[DEVIL_TUNE_CHUNK_CELLS] > 0 ? ctx->tune[DEVIL_TUNE_CHUNK_CELLS] : 2048;
+  const uint64_t key = tune_key(n, p, pr->counts_dtype);
+  bool autotune = mixed && (ctx->tune[DEVIL_TUNE_MIXED_KERNEL] == DEVIL_MIXED_AUTO ||
+                            ctx->tune[DEVIL_TUNE_CHUNK_CELLS] == 0);
+  if (autotune && ctx->tuned.count(key)) {
+    const int v = ctx->tuned[key];
+    if (ctx->tune[DEVIL_TUNE_MIXED_KERNEL] == DEVIL_MIXED_AUTO) mk = v & 0xff;
+    if (ctx->tune[DEVIL_TUNE_CHUNK_CELLS] == 0) cc = (int64_t)(v >> 8);
+    autotune = false;
+  }
   TileHost tl;
   if ((rc = make_tile(ctx, dd.x, dd.off.as<double>(), dd.eo.as<double>(), n, p,
-                      pr->precision == DEVIL_PREC_MIXED ? mixed_kernel(ctx) : kMixedNone, tl)))
+                      autotune ? kMixedNone : mk, tl, cc)))
     return rc;
 
   // per-gene device state (gene-major)
@@ -1064,6 +1185,7 @@ int devil_fit(devil_ctx* ctx, const devil_problem* pr, devil_outputs* out) {
     ctx->k_n[k] = 0;
     ctx->k_ms[k] = ctx->k_bytes[k] = 0.0;
   }
+  ctx->replayed = 0;
   StageTimer tm(ctx);
   hipEvent_t ev;
   for (int64_t g0 = 0; g0 < G; g0 += batch) {
@@ -1087,6 +1209,17 @@ int devil_fit(devil_ctx* ctx, const devil_problem* pr, devil_outputs* out) {
                                    a0, b0));
     DV_HIP(dv::launch_k_from_alpha(s, a0, Gb, kvec.as<double>() + g0));
     tm.end(ev);
+    if (autotune) {
+      if ((rc = autotune_mixed(ctx, dd, n, p, c, Gb, b0, kvec.as<double>() + g0, mk, cc)))
+        return rc;
+      ctx->tuned[key] = mk | (int)(cc << 8);
+      autotune = false;
+      tl = TileHost();
+      if ((rc = make_tile(ctx, dd.x, dd.off.as<double>(), dd.eo.as<double>(), n, p, mk, tl, cc)))
+        return rc;
+    }
+    ctx->last_mk = mixed ? mk : kMixedNone;
+    ctx->last_cc = cc;
     ev = tm.begin(DEVIL_STAGE_IRLS);
     if ((rc = run_irls_tiled(ctx, tl, c, Gb, p, b0, kvec.as<double>() + g0, pr->max_iter,
                              eps, iters.as<int32_t>() + g0, status.as<int32_t>() + g0)))
@@ -1135,6 +1268,98 @@ int devil_fit(devil_ctx* ctx, const devil_problem* pr, devil_outputs* out) {
   return DEVIL_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
+// Several problems: each runs as a whole fit on its own stream (one host
+// thread per stream, up to DEVIL_TUNE_CONCURRENCY at a time), so the kernels
+// of different problems overlap on the GPU; larger problems start first.
+int devil_fit(devil_ctx* ctx, const devil_problem* probs, devil_outputs* outs,
+              int32_t n_problems) {
+  int rc;
+  if ((rc = set_device(ctx))) return rc;
+  if (n_problems < 0 || (n_problems > 0 && (!probs || !outs)))
+    return fail(ctx, DEVIL_E_ARG, "NULL problems or outputs");
+  if (n_problems == 0) return DEVIL_OK;
+  for (int k = 0; k < DEVIL_STAGE_TEST_DE; ++k) ctx->stage_ms[k] = 0.0;
+  for (int k = 0; k < DEVIL_N_KERNELS; ++k) {
+    ctx->k_n[k] = 0;
+    ctx->k_ms[k] = ctx->k_bytes[k] = 0.0;
+  }
+  ctx->replayed = 0;
+  if (n_problems == 1) return fit_one(ctx, probs, outs);
+  const int S = std::max(1, std::min<int>(n_problems, ctx->tune[DEVIL_TUNE_CONCURRENCY]));
+  while ((int)ctx->subs.size() < S) {
+    devil_ctx* sub = new (std::nothrow) devil_ctx();
+    if (!sub) return fail(ctx, DEVIL_E_NOMEM, "sub-context");
+    sub->device = ctx->device;
+    if (hipStreamCreateWithFlags(&sub->stream, hipStreamNonBlocking) != hipSuccess) {
+      delete sub;
+      return fail(ctx, DEVIL_E_HIP, "hipStreamCreate");
+    }
+    ctx->subs.push_back(sub);
+  }
+  std::vector<int> order(n_problems);
+  for (int j = 0; j < n_problems; ++j) order[j] = j;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return probs[a].G * probs[a].n > probs[b].G * probs[b].n;
+  });
+  std::atomic<int> next{0};
+  std::vector<int> rcs(n_problems, DEVIL_OK);
+  std::vector<std::thread> th;
+  for (int t = 0; t < S; ++t) {
+    devil_ctx* sub = ctx->subs[t];
+    std::memcpy(sub->tune, ctx->tune, sizeof(ctx->tune));
+    sub->tuned = ctx->tuned;
+    sub->timing = ctx->timing;
+    for (int k = 0; k < DEVIL_N_STAGES; ++k) sub->stage_ms[k] = 0.0;
+    th.emplace_back([&, sub] {
+      for (int q; (q = next.fetch_add(1)) < n_problems;) {
+        const int j = order[q];
+        for (int k = 0; k < DEVIL_N_KERNELS; ++k) {
+          sub->k_n[k] = 0;
+          sub->k_ms[k] = sub->k_bytes[k] = 0.0;
+        }
+        double st[DEVIL_N_STAGES] = {0, 0, 0, 0, 0};
+        for (int k = 0; k < DEVIL_N_STAGES; ++k) st[k] = sub->stage_ms[k];
+        rcs[j] = fit_one(sub, &probs[j], &outs[j]);
+        sub->replay_sum += sub->replayed;
+        // fit_one resets the sub-context's stage and kernel figures: keep sums
+        for (int k = 0; k < DEVIL_N_STAGES; ++k) sub->stage_ms[k] += st[k];
+        for (int k = 0; k < DEVIL_N_KERNELS; ++k) {
+          sub->k_sum_n[k] += sub->k_n[k];
+          sub->k_sum_ms[k] += sub->k_ms[k];
+          sub->k_sum_bytes[k] += sub->k_bytes[k];
+        }
+        if (rcs[j] != DEVIL_OK) return;
+      }
+    });
+  }
+  for (auto& x : th) x.join();
+  for (int t = 0; t < S; ++t) {
+    devil_ctx* sub = ctx->subs[t];
+    for (int k = 0; k < DEVIL_STAGE_TEST_DE; ++k) ctx->stage_ms[k] += sub->stage_ms[k];
+    for (int k = 0; k < DEVIL_N_KERNELS; ++k) {
+      ctx->k_n[k] += sub->k_sum_n[k];
+      ctx->k_ms[k] += sub->k_sum_ms[k];
+      ctx->k_bytes[k] += sub->k_sum_bytes[k];
+      sub->k_sum_n[k] = 0;
+      sub->k_sum_ms[k] = sub->k_sum_bytes[k] = 0.0;
+    }
+    for (auto& kv : sub->tuned) ctx->tuned[kv.first] = kv.second;
+    ctx->replayed += sub->replay_sum;
+    sub->replay_sum = 0;
+  }
+  for (int j = 0; j < n_problems; ++j)
+    if (rcs[j] != DEVIL_OK) {
+      for (devil_ctx* sub : ctx->subs)
+        if (!sub->err.empty()) ctx->err = "problem " + std::to_string(j) + ": " + sub->err;
+      return rcs[j];
+    }
+  return DEVIL_OK;
+}
+
 int devil_set_timing(devil_ctx* ctx, int enable) {
   if (!ctx) return DEVIL_E_ARG;
   ctx->timing = enable ? 1 : 0;
@@ -1149,10 +1374,28 @@ int devil_stage_times(const devil_ctx* ctx, double* ms) {
 
 int devil_set_tuning(devil_ctx* ctx, int32_t key, int64_t value) {
   if (!ctx || key < 0 || key >= DEVIL_N_TUNE) return DEVIL_E_ARG;
-  if (key == DEVIL_TUNE_MIXED_KERNEL && value != DEVIL_MIXED_F32 && value != DEVIL_MIXED_BF16)
+  if (key == DEVIL_TUNE_MIXED_KERNEL && value != DEVIL_MIXED_F32 && value != DEVIL_MIXED_BF16 &&
+      value != DEVIL_MIXED_AUTO)
     return fail(ctx, DEVIL_E_ARG, "unknown mixed kernel");
+  if (key == DEVIL_TUNE_CHUNK_CELLS && value != 0 && (value < 256 || value > (1 << 20)))
+    return fail(ctx, DEVIL_E_ARG, "chunk cells must be 0 (autotune) or in [256, 2^20]");
+  if (key == DEVIL_TUNE_CONCURRENCY && (value < 1 || value > 16))
+    return fail(ctx, DEVIL_E_ARG, "concurrency must be in [1, 16]");
   ctx->tune[key] = (int)value;
   return DEVIL_OK;
+}
+
+int devil_fit_info(const devil_ctx* ctx, int32_t key, int64_t* value) {
+  if (!ctx || !value) return DEVIL_E_ARG;
+  switch (key) {
+    case DEVIL_INFO_REPLAYED: *value = ctx->replayed; return DEVIL_OK;
+    case DEVIL_INFO_MIXED_KERNEL:
+      *value = ctx->last_mk == kMixedBf16 ? DEVIL_MIXED_BF16
+               : ctx->last_mk == kMixedF32 ? DEVIL_MIXED_F32 : -1;
+      return DEVIL_OK;
+    case DEVIL_INFO_CHUNK_CELLS: *value = ctx->last_cc; return DEVIL_OK;
+    default: return DEVIL_E_ARG;
+  }
 }
 
 int devil_kernel_stats(const devil_ctx* ctx, int32_t kernel, int64_t* launches, double* ms,

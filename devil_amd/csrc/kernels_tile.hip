@@ -603,7 +603,8 @@ __global__ __launch_bounds__(64) void k_irls_update(TileDesign td,
   }
   // partials from the mixed-precision pass: a step whose stopping decision
   // could differ from FP64 arithmetic (max |delta| within band * eps of eps,
-  // or NaN) is left undone and the gene is redone in FP64 this iteration
+  // or NaN) is left undone; the gene leaves the mixed rounds and is refit in
+  // FP64 from its starting beta afterwards
   if (band > 0.0 && (has_nan || fabs(mx - eps) <= band * eps)) {
     recheck[atomicAdd(nrecheck, 1)] = g;
     return;
@@ -620,6 +621,27 @@ __global__ __launch_bounds__(64) void k_irls_update(TileDesign td,
   if (!conv && it >= max_iter) st |= DEVIL_GENE_MAXITER;
   status[g] = st;
   if (!conv && it < max_iter) next[atomicAdd(nnext, 1)] = g;
+}
+
+// genes list[0..n): beta <- beta0, iteration count and status cleared
+__global__ void k_restart_genes(const int32_t* __restrict__ list, int n, int p,
+                                const double* __restrict__ beta0, double* __restrict__ beta,
+                                int32_t* __restrict__ iters, int32_t* __restrict__ status) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t g = list[i];
+  for (int j = 0; j < p; ++j) beta[g * p + j] = beta0[g * p + j];
+  iters[g] = 0;
+  status[g] = 0;
+}
+
+hipError_t launch_restart_genes(hipStream_t s, const int32_t* list, int n, int p,
+                                const double* beta0, double* beta, int32_t* iters,
+                                int32_t* status) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_restart_genes, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, list, n,
+                     p, beta0, beta, iters, status);
+  return hipGetLastError();
 }
 
 __global__ void k_iota(int32_t* p, int n) {

@@ -14,6 +14,10 @@ Configurations (genes x cells, design width p):
                             level effects N(0,.5^2), covariates N(0,.1^2); MOM
   C4  15000 x ~1.2M,   p=6 [1, Bern, z1..z4]; 40 patients of U{2000..60000}
                             cells, patient effect N(0,.3^2) per gene; MOM, clustered
+  C5  64 independent problems over 15000 shared genes: problem j has
+      n_j = round(exp(U(log 500, log 2e5))) cells (sum ~2M) and design
+      [1, Bern, z1..z(p_j - 2)] with p_j = 2 + (j mod 9); beta per (problem,
+      gene) as in C4; MOM (make_problems)
 """
 from __future__ import annotations
 
@@ -99,6 +103,33 @@ def make_workload(config: str, G: int | None = None, n: int | None = None,
     sf = np.exp(r.normal(0, 0.3, n))
     size = np.exp(r.normal(0, 1, G))
     return Workload(config, G, n, X.shape[1], X, sf, B, size, od, ends, psd, seed)
+
+
+def make_problems(config: str = "C5", G: int | None = None, n_problems: int = 64,
+                  n_min: int = 500, n_max: int = 200000, seed: int | None = None) -> list:
+    """The independent problems of the many-problems configuration: one
+    Workload per problem (shared genes, own cells, design width and
+    coefficients).  ``G``, ``n_problems``, ``n_min`` and ``n_max`` shrink it
+    for tests."""
+    if config != "C5":
+        raise ValueError("make_problems builds the C5 configuration")
+    seed = 20260005 if seed is None else seed
+    r = _rng(seed)
+    G = G or 15000
+    ns = np.round(np.exp(r.uniform(np.log(n_min), np.log(n_max), n_problems))).astype(int)
+    out = []
+    for j in range(n_problems):
+        p = 2 + (j % 9)
+        n = int(ns[j])
+        X = np.column_stack([np.ones(n), r.uniform(size=n) < 0.5] +
+                            [r.normal(size=n) for _ in range(p - 2)]).astype(np.float64)
+        B = np.column_stack([r.normal(0, 1, G), r.normal(0, 0.5, G)] +
+                            [r.normal(0, 0.1, G) for _ in range(p - 2)])
+        sf = np.exp(r.normal(0, 0.3, n))
+        size = np.exp(r.normal(0, 1, G))
+        out.append(Workload(f"C5.{j}", G, n, p, X, sf, B, size, "MOM", None, 0.0,
+                            seed + 1000 * (j + 1)))
+    return out
 
 
 def counts_host(w: Workload, genes: np.ndarray | None = None, dtype=np.int32) -> np.ndarray:

@@ -112,7 +112,12 @@ typedef struct {                /* caller-allocated, same memory space as the pr
   int32_t *status;              /* G, or NULL: DEVIL_GENE_* bits */
 } devil_outputs;
 
-int devil_fit(devil_ctx *ctx, const devil_problem *prob, devil_outputs *out);
+/* Fits n_problems independent problems (probs[j] -> outs[j]); problems may
+ * differ in G, n, p, design, clusters and mode.  With more than one problem
+ * they run concurrently on DEVIL_TUNE_CONCURRENCY streams.  Returns the first
+ * failing problem's code (devil_last_error names the problem). */
+int devil_fit(devil_ctx *ctx, const devil_problem *probs, devil_outputs *outs,
+              int32_t n_problems);
 
 /* ------------------------------------------------------------------------ */
 /* Stage entry points (device pointers; counts as in devil_problem).         */
@@ -184,10 +189,16 @@ int devil_stage_times(const devil_ctx *ctx, double *ms);
 
 /* Kernel choices of the mixed-precision path (set per context; the library
  * defaults are what the autotuner found best on gfx950). */
-enum { DEVIL_TUNE_MIXED_KERNEL = 0, DEVIL_N_TUNE = 1 };
+enum { DEVIL_TUNE_MIXED_KERNEL = 0,  /* DEVIL_MIXED_* (default AUTO) */
+       DEVIL_TUNE_CHUNK_CELLS = 1,   /* cells per chunk of the tile passes, 0 = autotune */
+       DEVIL_TUNE_CONCURRENCY = 2,   /* streams for multi-problem fits (default 4) */
+       DEVIL_N_TUNE = 3 };
 enum { DEVIL_MIXED_F32 = 0,   /* eta and all contractions on v_mfma_f32_16x16x4f32 */
-       DEVIL_MIXED_BF16 = 1   /* eta (split operands) and pair columns on
-                                 v_mfma_f32_16x16x32_bf16, design columns FP32 */ };
+       DEVIL_MIXED_BF16 = 1,  /* eta (split operands) and pair columns on
+                                 v_mfma_f32_16x16x32_bf16, design columns FP32 */
+       DEVIL_MIXED_AUTO = 2   /* autotuner: times the candidates on the problem's
+                                 first genes once per (p, n, count type) and keeps
+                                 the fastest for the context */ };
 int devil_set_tuning(devil_ctx *ctx, int32_t key, int64_t value);
 
 /* Per-launch timing of the IRLS accumulate kernels (the streaming pass over
@@ -198,6 +209,12 @@ enum { DEVIL_KERNEL_IRLS_FP64 = 0, DEVIL_KERNEL_IRLS_MIXED = 1, DEVIL_N_KERNELS 
  * bytes (count element size x n x active genes of each launch). */
 int devil_kernel_stats(const devil_ctx *ctx, int32_t kernel, int64_t *launches, double *ms,
                        double *bytes);
+
+/* Facts about the last devil_fit: genes the mixed path refit in FP64 (their
+ * stopping step came within 1e-3 eps of the threshold, or was NaN), and the
+ * mixed kernel (DEVIL_MIXED_*, -1 for FP64) and chunk size it ran with. */
+enum { DEVIL_INFO_REPLAYED = 0, DEVIL_INFO_MIXED_KERNEL = 1, DEVIL_INFO_CHUNK_CELLS = 2 };
+int devil_fit_info(const devil_ctx *ctx, int32_t key, int64_t *value);
 
 #ifdef __cplusplus
 }

@@ -1,13 +1,17 @@
 """GPU parity of the mixed-precision IRLS (FP32 MFMA contractions, FP64
 partial sums, FP64 solve and stopping decisions) against the FP64 oracle.
 
+Problems with fewer than 4096 cells run the FP64 kernels (the test sizes
+here are above that).
+
 Stated tolerance: beta within 1e-4 relative with a floor of 1e-3 on
 |reference| (beta is a natural-log effect: the floor is a 0.1% fold change,
 far below any coefficient's standard error), log10 p within 1e-4 relative
-(floor 1e-6), theta within 1e-4 relative (floor 1e-8), identical BH-adjusted
-p < 0.05 calls, and identical per-gene IRLS iteration counts (steps within
-1e-3 * eps of the threshold are redone in FP64, so the stopping decisions
-follow FP64 arithmetic).
+(floor 1 in log10 units: absolute 1e-4 for p > 0.1), theta within 1e-4
+relative (floor 1e-8), identical BH-adjusted p < 0.05 calls, and identical
+per-gene IRLS iteration counts (a gene whose step comes within 1e-3 * eps of
+the threshold is refit from its start in FP64, so decisions that close are
+FP64 decisions).
 
 Why the beta floor: FP32 element arithmetic leaves ~1e-7 relative noise per
 cell in the working residuals, which averages to an absolute beta error of
@@ -63,7 +67,7 @@ def dv(request):
     ctx.set_mixed_kernel(request.param)
     devil_amd.mixed_tol = KERNEL_TOL[request.param]
     yield devil_amd
-    ctx.set_mixed_kernel("f32")
+    ctx.set_mixed_kernel("auto")  # library default
 
 
 def fit_and_oracle(dv, orc, w, Y, *, od="MOM", clusters=None, max_iter=200):
@@ -85,7 +89,7 @@ def check_mixed(dv, orc, w, fit, ref, contrast):
     rr = orc.de_test(ref["beta"], ref["bread"], ref["sandwich"], w.n, contrast)
     lp = res["log_pval"] / np.log(10.0)
     lr = rr["log_pval"] / np.log(10.0)
-    ep = relmax(lp, lr)
+    ep = relmax(lp, lr, 1.0)
     say(f"  beta {eb:.2e}  theta {et:.2e}  log10 p {ep:.2e}")
     tol = dv.mixed_tol
     assert eb <= tol and ep <= tol and et <= TOL
@@ -127,7 +131,7 @@ def test_mixed_clustered_c4(dv, orc):
     check_mixed(dv, orc, w, fit, ref, [0, 1, 0, 0, 0, 0])
 
 
-@pytest.mark.parametrize("p,n", [(1, 3001), (5, 2999), (10, 4003)])
+@pytest.mark.parametrize("p,n", [(1, 4099), (5, 5001), (10, 6007)])
 def test_mixed_design_widths(dv, orc, p, n):
     from devil_amd import synth
     rng = np.random.default_rng(300 + p)
@@ -144,7 +148,7 @@ def test_mixed_design_widths(dv, orc, p, n):
 
 def test_mixed_edge_cases(dv, orc):
     from devil_amd import synth
-    w = synth.make_workload("C1", G=50, n=2000, seed=13)
+    w = synth.make_workload("C1", G=50, n=4500, seed=13)
     Y = synth.counts_host(w)
     Y[3] = 0
     Y[7] = 0

@@ -267,7 +267,8 @@ def test_abi_errors_on_gpu(dv):
     assert rc == 1
     prob = L.Problem(G=2, n=10, p=11, mem=0)
     outs = L.Outputs()
-    assert lib.devil_fit(ctx.h, C.byref(prob), C.byref(outs)) == 1
+    assert lib.devil_fit(ctx.h, C.byref(prob), C.byref(outs), 1) == 1
+    assert lib.devil_fit(ctx.h, None, None, 2) == 1
     with pytest.raises(ValueError):
         dv.cluster_blocks([1, 1, 2, 1], 4)
     X = np.column_stack([np.ones(8), np.ones(8)])
