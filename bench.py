@@ -37,6 +37,21 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def measured_traffic(kdesc, alg_bytes_per_launch):
+    """HBM bytes per launch of the kernel from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE measurement (traffic per algorithmic byte), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic_irls_C3.json")
+    try:
+        with open(path) as f:
+            meas = json.load(f)["kernels"]
+    except (OSError, ValueError, KeyError):
+        return None
+    name = kdesc.split(" ")[0]
+    if name not in meas:
+        return None
+    return meas[name]["traffic_per_algorithmic_byte"] * alg_bytes_per_launch
+
+
 def cpu_baseline(w, Yd, fit, n_threads, max_genes=None):
     """The FP64 oracle on an evenly strided gene subsample (about 10-20 s of
     host work), timed on the host; also checks the GPU fit on those genes."""
@@ -274,8 +289,12 @@ def main():
     fit_bytes = 4.0 * w.n * (2 * w.G + sum_it + disp_passes + w.G)
     fit_ms = sum(stages[k] for k in ("init", "irls", "dispersion", "bread_meat"))
     value = w.G * w.n * args.steps / T
-    kdesc = {"irls_mixed": f"k_irls_accum_mixed<{w.p}> (FP32 MFMA, FP64 partials)",
-             "irls_fp64": f"k_irls_accum<{w.p}> (FP64 MFMA)"}[kname]
+    if kname == "irls_mixed" and info.get("mixed_kernel") == "bf16":
+        kdesc = f"k_irls_accum_bf16<{w.p}> (bf16/FP32 MFMA, FP64 partials)"
+    elif kname == "irls_mixed":
+        kdesc = f"k_irls_accum_mixed<{w.p}> (FP32 MFMA, FP64 partials)"
+    else:
+        kdesc = f"k_irls_accum<{w.p}> (FP64 MFMA)"
     out = {
         "metric": METRIC,
         "value": value,
@@ -299,7 +318,8 @@ def main():
                    "overdispersion": w.od},
         "roofline": {"bound": "hbm", "kernel": kdesc,
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(kdesc, kbytes / max(nl, 1)),
                      "launches_per_step": nl / args.steps,
                      "avg_launch_ms": avg_ms,
                      "algorithmic_bytes_per_launch": kbytes / max(nl, 1)},
