@@ -212,7 +212,7 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=1,
                     help="timed FP64 steps on the same counts (0: no comparison)")
     ap.add_argument("--cpu-genes", type=int, default=None)
-    ap.add_argument("--mixed-kernel", default=None, choices=["f32", "bf16"],
+    ap.add_argument("--mixed-kernel", default=None, choices=["f32", "bf16", "bf16_ws"],
                     help="mixed IRLS kernel (default: the library's choice)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -291,6 +291,8 @@ def main():
     value = w.G * w.n * args.steps / T
     if kname == "irls_mixed" and info.get("mixed_kernel") == "bf16":
         kdesc = f"k_irls_accum_bf16<{w.p}> (bf16/FP32 MFMA, FP64 partials)"
+    elif kname == "irls_mixed" and info.get("mixed_kernel") == "bf16_ws":
+        kdesc = f"k_irls_accum_ws<{w.p}> (bf16/FP32 MFMA, producer/consumer waves)"
     elif kname == "irls_mixed":
         kdesc = f"k_irls_accum_mixed<{w.p}> (FP32 MFMA, FP64 partials)"
     else:

@@ -424,10 +424,27 @@ struct StageTimer {
 };
 
 // Host construction of the MFMA operand layouts (dv::TileDesign).
-enum MixedKernel { kMixedNone = 0, kMixedF32 = 1, kMixedBf16 = 2 };
+enum MixedKernel { kMixedNone = 0, kMixedF32 = 1, kMixedBf16 = 2, kMixedBf16Ws = 3 };
 
 int mixed_kernel(const devil_ctx* ctx) {
-  return ctx->tune[DEVIL_TUNE_MIXED_KERNEL] == DEVIL_MIXED_BF16 ? kMixedBf16 : kMixedF32;
+  switch (ctx->tune[DEVIL_TUNE_MIXED_KERNEL]) {
+    case DEVIL_MIXED_BF16: return kMixedBf16;
+    case DEVIL_MIXED_BF16_WS: return kMixedBf16Ws;
+    default: return kMixedF32;
+  }
+}
+
+// the bf16 kernels share one operand layout (32-cell steps)
+bool bf16_layout(int mixed) { return mixed == kMixedBf16 || mixed == kMixedBf16Ws; }
+
+hipError_t launch_mixed_accum(int kind, hipStream_t s, const dv::TileDesign& td, dv::Counts c,
+                              int p, const int32_t* cur, const int32_t* ncur, int nA,
+                              const double* beta, const double* kvec, double* part) {
+  if (kind == kMixedBf16)
+    return dv::launch_irls_accum_bf16(s, td, c, p, cur, ncur, nA, beta, kvec, part);
+  if (kind == kMixedBf16Ws)
+    return dv::launch_irls_accum_ws(s, td, c, p, cur, ncur, nA, beta, kvec, part);
+  return dv::launch_irls_accum_mixed(s, td, c, p, cur, ncur, nA, beta, kvec, part);
 }
 
 // Columns of K: x_a x_b (a <= b) and x_a of the design, exact duplicates
@@ -590,7 +607,7 @@ int make_tile(devil_ctx* ctx, const double* Xd, const double* offd, const double
     th.tdm.Xf = th.Xf.as<float>();
     th.tdm.offf = th.offf.as<float>();
     th.tdm.Kf = th.Kf.as<float>();
-  } else if (mixed == kMixedBf16) {
+  } else if (bf16_layout(mixed)) {
     ColPlan cm;
     if ((rc = plan_columns(ctx, Xd, n, p, true, cm))) return rc;
     dv::TileDesign& tm = th.tdm;
@@ -602,9 +619,15 @@ int make_tile(devil_ctx* ctx, const double* Xd, const double* offd, const double
     DV_ALLOC(th.xab, sizeof(uint16_t) * nst * 2 * NE * 64 * 8);
     DV_ALLOC(th.offl, sizeof(float) * nst * 32);
     DV_ALLOC(th.kg, sizeof(float) * nst * 64 * 8);
-    if (nth > 0) {
-      DV_ALLOC(th.kh, sizeof(uint16_t) * nst * nth * 64 * 8);
-      DV_ALLOC(th.kh2, sizeof(uint16_t) * nst * nth * 64 * 8);
+    // at least one pair tile: the kernels' operand loads are unconditional
+    // (a design without pair columns reads zeros it never uses)
+    const int64_t nth_alloc = std::max(nth, 1);
+    DV_ALLOC(th.kh, sizeof(uint16_t) * nst * nth_alloc * 64 * 8);
+    DV_ALLOC(th.kh2, sizeof(uint16_t) * nst * nth_alloc * 64 * 8);
+    if (nth == 0) {
+      const int64_t words = nst * 64 * 8 / 2;  // uint16 pairs
+      DV_HIP(dv::launch_fill_i32(s, reinterpret_cast<int32_t*>(th.kh.p), words, 0));
+      DV_HIP(dv::launch_fill_i32(s, reinterpret_cast<int32_t*>(th.kh2.p), words, 0));
     }
     DV_HIP(dv::launch_build_tables_bf16(s, Xd, offd, n, p, th.specm.as<dv::ColSpec>(),
                                         (int)cm.spec.size(), nth, nst, th.xab.as<uint16_t>(),
@@ -613,8 +636,8 @@ int make_tile(devil_ctx* ctx, const double* Xd, const double* offd, const double
     tm.xab = th.xab.as<uint16_t>();
     tm.offl = th.offl.as<float>();
     tm.kg = th.kg.as<float>();
-    tm.kh = nth > 0 ? th.kh.as<uint16_t>() : nullptr;
-    tm.kh2 = nth > 0 ? th.kh2.as<uint16_t>() : nullptr;
+    tm.kh = th.kh.as<uint16_t>();
+    tm.kh2 = th.kh2.as<uint16_t>();
     tm.eop = td.eop;
   }
   return DEVIL_OK;
@@ -805,10 +828,8 @@ int irls_rounds(devil_ctx* ctx, const TileHost& tl, bool mixed, dv::Counts c, in
                      row_bytes * nA);
       if (!mixed)
         DV_HIP(dv::launch_irls_accum(s, td, c, p, cur, ncur, nA, beta, kvec, part));
-      else if (tl.mixed == kMixedBf16)
-        DV_HIP(dv::launch_irls_accum_bf16(s, td, c, p, cur, ncur, nA, beta, kvec, part));
       else
-        DV_HIP(dv::launch_irls_accum_mixed(s, td, c, p, cur, ncur, nA, beta, kvec, part));
+        DV_HIP(launch_mixed_accum(tl.mixed, s, td, c, p, cur, ncur, nA, beta, kvec, part));
     }
     DV_HIP(dv::launch_fill_i32(s, nnxt, 1, 0));
     DV_HIP(dv::launch_irls_update(s, td, p, cur, ncur, nA, part, beta, kvec, max_iter, eps,
@@ -951,7 +972,8 @@ int autotune_mixed(devil_ctx* ctx, const DeviceDesign& dd, int64_t n, int p, dv:
   hipStream_t s = ctx->stream;
   const int64_t Gs = std::min<int64_t>(G, 1024);
   std::vector<int> kinds;
-  if (ctx->tune[DEVIL_TUNE_MIXED_KERNEL] == DEVIL_MIXED_AUTO) kinds = {kMixedF32, kMixedBf16};
+  if (ctx->tune[DEVIL_TUNE_MIXED_KERNEL] == DEVIL_MIXED_AUTO)
+    kinds = {kMixedF32, kMixedBf16, kMixedBf16Ws};
   else kinds = {mixed_kernel(ctx)};
   std::vector<int64_t> cells;
   if (ctx->tune[DEVIL_TUNE_CHUNK_CELLS] == 0) cells = {1024, 2048, 4096};
@@ -974,11 +996,8 @@ int autotune_mixed(devil_ctx* ctx, const DeviceDesign& dd, int64_t n, int p, dv:
       DevBuf part;
       DV_ALLOC(part, sizeof(double) * (size_t)((Gs + 15) / 16) * tm.nchunk * 16 * tm.ncp);
       auto launch = [&]() {
-        return kind == kMixedBf16
-                   ? dv::launch_irls_accum_bf16(s, tm, c, p, act.as<int32_t>(), cnt.as<int32_t>(),
-                                                (int)Gs, beta, kvec, part.as<double>())
-                   : dv::launch_irls_accum_mixed(s, tm, c, p, act.as<int32_t>(), cnt.as<int32_t>(),
-                                                 (int)Gs, beta, kvec, part.as<double>());
+        return launch_mixed_accum(kind, s, tm, c, p, act.as<int32_t>(), cnt.as<int32_t>(),
+                                  (int)Gs, beta, kvec, part.as<double>());
       };
       DV_HIP(launch());
       DV_HIP(hipEventRecord(e0, s));
@@ -1375,7 +1394,7 @@ int devil_stage_times(const devil_ctx* ctx, double* ms) {
 int devil_set_tuning(devil_ctx* ctx, int32_t key, int64_t value) {
   if (!ctx || key < 0 || key >= DEVIL_N_TUNE) return DEVIL_E_ARG;
   if (key == DEVIL_TUNE_MIXED_KERNEL && value != DEVIL_MIXED_F32 && value != DEVIL_MIXED_BF16 &&
-      value != DEVIL_MIXED_AUTO)
+      value != DEVIL_MIXED_AUTO && value != DEVIL_MIXED_BF16_WS)
     return fail(ctx, DEVIL_E_ARG, "unknown mixed kernel");
   if (key == DEVIL_TUNE_CHUNK_CELLS && value != 0 && (value < 256 || value > (1 << 20)))
     return fail(ctx, DEVIL_E_ARG, "chunk cells must be 0 (autotune) or in [256, 2^20]");
@@ -1390,8 +1409,10 @@ int devil_fit_info(const devil_ctx* ctx, int32_t key, int64_t* value) {
   switch (key) {
     case DEVIL_INFO_REPLAYED: *value = ctx->replayed; return DEVIL_OK;
     case DEVIL_INFO_MIXED_KERNEL:
-      *value = ctx->last_mk == kMixedBf16 ? DEVIL_MIXED_BF16
-               : ctx->last_mk == kMixedF32 ? DEVIL_MIXED_F32 : -1;
+      *value = ctx->last_mk == kMixedBf16     ? DEVIL_MIXED_BF16
+               : ctx->last_mk == kMixedBf16Ws ? DEVIL_MIXED_BF16_WS
+               : ctx->last_mk == kMixedF32    ? DEVIL_MIXED_F32
+                                              : -1;
       return DEVIL_OK;
     case DEVIL_INFO_CHUNK_CELLS: *value = ctx->last_cc; return DEVIL_OK;
     default: return DEVIL_E_ARG;

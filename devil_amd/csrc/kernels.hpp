@@ -142,6 +142,11 @@ hipError_t launch_irls_accum_bf16(hipStream_t s, const TileDesign& td, Counts c,
                                   const int32_t* active, const int32_t* nactive, int nA_bound,
                                   const double* beta, const double* kvec, double* part);
 
+// Wave-specialised bf16 variant (producer / consumer wavefronts, same design).
+hipError_t launch_irls_accum_ws(hipStream_t s, const TileDesign& td, Counts c, int p,
+                                const int32_t* active, const int32_t* nactive, int nA_bound,
+                                const double* beta, const double* kvec, double* part);
+
 hipError_t launch_restart_genes(hipStream_t s, const int32_t* list, int n, int p,
                                 const double* beta0, double* beta, int32_t* iters,
                                 int32_t* status);

@@ -158,6 +158,7 @@ __device__ __forceinline__ void mle_tile_chunk(const TileDesign& td, const CT* _
   const double tn1 = 1.0 / theta;
   const bool cell_psi = tab_n[g] < 0;
   const CT* yrow = Y + (int64_t)g * ld;
+  const CT* yh = yrow + 4 * h;
   const bool vec = (ld & 3) == 0 && ((reinterpret_cast<uintptr_t>(Y) & 15) == 0);
   d4 a0[NTMAX], a1[NTMAX], a2[NTMAX];
 #pragma unroll
@@ -169,10 +170,8 @@ __device__ __forceinline__ void mle_tile_chunk(const TileDesign& td, const CT* _
   double sc = 0.0, l1 = 0.0, l2 = 0.0, dg = 0.0, tg = 0.0;
   const int64_t b0 = (int64_t)chunk * td.chunk_blocks;
   const int64_t b1 = min((int64_t)(chunk + 1) * td.chunk_blocks, td.nblk);
-  BlockOps<NJ, NTMAX> cur, nxt;
-  if (b0 < b1) load_block<NJ, NTMAX, CT>(td, yrow, vec, b0, lane, h, NT, cur);  // trailing chunks may be empty
-  for (int64_t b = b0; b < b1; ++b) {
-    if (b + 1 < b1) load_block<NJ, NTMAX, CT>(td, yrow, vec, b + 1, lane, h, NT, nxt);
+  typedef TileBlock<NJ, NTMAX, CT> Blk;
+  auto body = [&](const Blk& cur, int64_t b) {
     d4 eta = d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int j = 0; j < NJ; ++j) eta = mfma16x16x4(cur.xa[j], bpos[j], eta);
@@ -182,7 +181,7 @@ __device__ __forceinline__ void mle_tile_chunk(const TileDesign& td, const CT* _
     for (int r = 0; r < 4; ++r) {
       double mu = exp_f64(eta[r] + cur.off[r]);
       if (mu == 0.0) mu = 1e-6;
-      const double y = cur.y[r];
+      const double y = (double)cur.y[r];
       const double mt = mu * theta;
       const double rr = rcp_f64(1.0 + mt);
       const double L = log1p_f64(mt);
@@ -212,8 +211,11 @@ __device__ __forceinline__ void mle_tile_chunk(const TileDesign& td, const CT* _
         }
       }
     }
-    cur = nxt;
-  }
+  };
+  auto load = [&](int64_t b, Blk& o) {
+    load_tile_block<NJ, NTMAX, CT>(td, yrow, yh, vec, b, lane, h, NT, td.offp, o);
+  };
+  pipeline<Blk>(b0, b1, load, body);  // trailing chunks may be empty
   // the four lanes of a gene slot (h = 0..3) hold disjoint cells
 #pragma unroll
   for (int o = 16; o <= 32; o <<= 1) {

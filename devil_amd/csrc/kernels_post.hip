@@ -46,30 +46,31 @@ __device__ __forceinline__ void mom_tile_chunk(const TileDesign& td, ChunkTable 
     bpos[j] = (live && f < P) ? beta[g * P + f] : 0.0;
   }
   const CT* yrow = Y + g * ld;
+  const CT* yh = yrow + 4 * h;
   const bool vec = (ld & 3) == 0 && ((reinterpret_cast<uintptr_t>(Y) & 15) == 0);
   const int64_t c0 = ct.c0[chunk], c1 = ct.c1[chunk];
   const int64_t b0 = c0 / 16, b1 = (c1 + 15) / 16;
   double num = 0.0, den = 0.0;
-  BlockOps<NJ, 1> cur, nxt;
-  if (b0 < b1 && c0 < c1) load_block<NJ, 1, CT>(td, yrow, vec, b0, lane, h, 0, cur);
-  for (int64_t b = b0; b < b1 && c0 < c1; ++b) {
-    if (b + 1 < b1) load_block<NJ, 1, CT>(td, yrow, vec, b + 1, lane, h, 0, nxt);
+  typedef TileBlock<NJ, 0, CT> Blk;  // off holds exp(offset)
+  auto body = [&](const Blk& cur, int64_t b) {
     d4 eta = d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int j = 0; j < NJ; ++j) eta = mfma16x16x4(cur.xa[j], bpos[j], eta);
     const int64_t cb = b * 16 + 4 * h;
-    const d4 eo = *reinterpret_cast<const d4*>(td.eop + cb);
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       if (cb + r >= c0 && cb + r < c1) {
-        const double mu = eo[r] * exp_f64(eta[r]);
-        const double d = cur.y[r] - mu;
+        const double mu = cur.off[r] * exp_f64(eta[r]);
+        const double d = (double)cur.y[r] - mu;
         num += d * d - mu;
         den += mu * mu;
       }
     }
-    cur = nxt;
-  }
+  };
+  auto load = [&](int64_t b, Blk& o) {
+    load_tile_block<NJ, 0, CT>(td, yrow, yh, vec, b, lane, h, 0, td.eop, o);
+  };
+  if (c0 < c1) pipeline<Blk>(b0, b1, load, body);
 #pragma unroll
   for (int o = 16; o <= 32; o <<= 1) {
     num += __shfl_xor(num, o, 64);
@@ -104,6 +105,7 @@ __device__ __forceinline__ void bread_tile_chunk(const TileDesign& td, ChunkTabl
   }
   const double th = live ? theta[g] : 0.0;
   const CT* yrow = Y + g * ld;
+  const CT* yh = yrow + 4 * h;
   const bool vec = (ld & 3) == 0 && ((reinterpret_cast<uintptr_t>(Y) & 15) == 0);
   d4 aw[NTMAX], ar;
 #pragma unroll
@@ -111,20 +113,17 @@ __device__ __forceinline__ void bread_tile_chunk(const TileDesign& td, ChunkTabl
   ar = d4{0.0, 0.0, 0.0, 0.0};
   const int64_t c0 = ct.c0[chunk], c1 = ct.c1[chunk];
   const int64_t b0 = c0 / 16, b1 = c0 < c1 ? (c1 + 15) / 16 : b0;
-  BlockOps<NJ, NTMAX> cur, nxt;
-  if (b0 < b1) load_block<NJ, NTMAX, CT>(td, yrow, vec, b0, lane, h, NT, cur);
-  for (int64_t b = b0; b < b1; ++b) {
-    if (b + 1 < b1) load_block<NJ, NTMAX, CT>(td, yrow, vec, b + 1, lane, h, NT, nxt);
+  typedef TileBlock<NJ, NTMAX, CT> Blk;  // off holds exp(offset)
+  auto body = [&](const Blk& cur, int64_t b) {
     d4 eta = d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int j = 0; j < NJ; ++j) eta = mfma16x16x4(cur.xa[j], bpos[j], eta);
     const int64_t cb = b * 16 + 4 * h;
-    const d4 eo = *reinterpret_cast<const d4*>(td.eop + cb);
     double w[4], rr[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const double mu = eo[r] * exp_f64(eta[r]);
-      const double y = cur.y[r];
+      const double mu = cur.off[r] * exp_f64(eta[r]);
+      const double y = (double)cur.y[r];
       const double d = fma(th, mu, 1.0);
       const double id = rcp_f64(d);
       const bool in = cb + r >= c0 && cb + r < c1;
@@ -140,8 +139,11 @@ __device__ __forceinline__ void bread_tile_chunk(const TileDesign& td, ChunkTabl
     }
 #pragma unroll
     for (int r = 0; r < 4; ++r) ar = mfma16x16x4(rr[r], cur.kb[0][r], ar);
-    cur = nxt;
-  }
+  };
+  auto load = [&](int64_t b, Blk& o) {
+    load_tile_block<NJ, NTMAX, CT>(td, yrow, yh, vec, b, lane, h, NT, td.eop, o);
+  };
+  pipeline<Blk>(b0, b1, load, body);
   // register r holds gene slot h + 4 r, column 16 t + s
   const int W = td.ncp + 16;
   double* out = part + ((int64_t)tile * ct.nchunk + chunk) * 16 * W;
@@ -336,7 +338,9 @@ __global__ __launch_bounds__(64) void k_bread_finish(TileDesign td, ChunkTable c
 // projection (thin Q of X for the least-squares start, X - mean(X) for the
 // univariate start) and z = log1p(y / s) or log(y + 0.5) - log s.
 // ---------------------------------------------------------------------------
-template <int P, typename CT>
+// ZM: 0 no projection (alpha only, rough start), 1 least squares (QR),
+// 2 univariate.
+template <typename CT, int ZM>
 __device__ __forceinline__ void init_tile_chunk(const TileDesign& td, InitTile it, ChunkTable ct,
                                                 const CT* __restrict__ Y, int64_t ld, int64_t G,
                                                 int tile, int chunk, double* __restrict__ part) {
@@ -345,48 +349,56 @@ __device__ __forceinline__ void init_tile_chunk(const TileDesign& td, InitTile i
   const int64_t g0 = (int64_t)tile * 16 + s;
   const int64_t g = g0 < G ? g0 : (int64_t)tile * 16;
   const CT* yrow = Y + g * ld;
+  const CT* yh = yrow + 4 * h;
   const bool vec = (ld & 3) == 0 && ((reinterpret_cast<uintptr_t>(Y) & 15) == 0);
   const int64_t c0 = ct.c0[chunk], c1 = ct.c1[chunk];
   const int64_t b0 = c0 / 16, b1 = c0 < c1 ? (c1 + 15) / 16 : b0;
+  constexpr bool kInt = sizeof(CT) == 4 && CT(0.5) == CT(0);  // integer counts
   d4 az = d4{0.0, 0.0, 0.0, 0.0};
   int64_t s1i = 0, s2i = 0;
   double s1d = 0.0, s2d = 0.0;
-  for (int64_t b = b0; b < b1; ++b) {
+  struct Blk {
+    CT y[4];
+    d4 zb;   // projection columns of the block (B operand)
+    d4 aux;  // exp(-offset) (QR) or offset (univariate) of the lane's cells
+  };
+  auto load = [&](int64_t b, Blk& o) {
     const int64_t cb = b * 16 + 4 * h;
-    double y[4];
-    if (cb + 4 <= td.n) {
-      load4<CT>(yrow + cb, vec, y);
-    } else {
-#pragma unroll
-      for (int r = 0; r < 4; ++r) y[r] = (cb + r < td.n) ? (double)yrow[cb + r] : 0.0;
+    if constexpr (ZM != 0) {
+      o.zb = *reinterpret_cast<const d4*>(it.Zp + (b * 64 + lane) * 4);
+      o.aux = *reinterpret_cast<const d4*>((ZM == 1 ? it.einv : it.logsf) + cb);
     }
+    load_block_y<CT>(td, yrow, yh, vec, b, h, o.y);
+  };
+  auto body = [&](const Blk& o, int64_t b) {
+    const int64_t cb = b * 16 + 4 * h;
     double z[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const bool in = cb + r >= c0 && cb + r < c1;
-      if (!in) y[r] = 0.0;
-      if (it.mode == kInitQR) {
-        z[r] = y[r] != 0.0 ? log1p_f64(y[r] * it.einv[cb + r]) : 0.0;
-      } else if (it.mode == kInitUnivariate) {
-        z[r] = in ? log(y[r] + 0.5) - it.logsf[cb + r] : 0.0;
+      const double y = in ? (double)o.y[r] : 0.0;
+      if constexpr (ZM == 1) {
+        z[r] = y != 0.0 ? log1p_f64(y * o.aux[r]) : 0.0;
+      } else if constexpr (ZM == 2) {
+        z[r] = in ? log(y + 0.5) - o.aux[r] : 0.0;
       } else {
         z[r] = 0.0;
       }
-      if (sizeof(CT) == 4 && CT(0.5) == CT(0)) {  // integer counts
-        const int64_t yi = (int64_t)y[r];
+      if constexpr (kInt) {
+        const int64_t yi = in ? (int64_t)o.y[r] : 0;
         s1i += yi;
         s2i += yi * yi;
       } else {
-        s1d += y[r];
-        s2d += y[r] * y[r];
+        s1d += y;
+        s2d += y * y;
       }
     }
-    if (it.mode == kInitQR || it.mode == kInitUnivariate) {
-      const d4 zb = *reinterpret_cast<const d4*>(it.Zp + (b * 64 + lane) * 4);
+    if constexpr (ZM != 0) {
 #pragma unroll
-      for (int r = 0; r < 4; ++r) az = mfma16x16x4(z[r], zb[r], az);
+      for (int r = 0; r < 4; ++r) az = mfma16x16x4(z[r], o.zb[r], az);
     }
-  }
+  };
+  pipeline<Blk>(b0, b1, load, body);
 #pragma unroll
   for (int o = 16; o <= 32; o <<= 1) {
     s1i += __shfl_xor(s1i, o, 64);
@@ -407,7 +419,7 @@ __device__ __forceinline__ void init_tile_chunk(const TileDesign& td, InitTile i
   }
 }
 
-template <int P>
+template <int ZM>
 __global__ __launch_bounds__(256) void k_init_accum(TileDesign td, InitTile it, ChunkTable ct,
                                                     Counts c, int64_t G,
                                                     double* __restrict__ part) {
@@ -417,13 +429,14 @@ __global__ __launch_bounds__(256) void k_init_accum(TileDesign td, InitTile it, 
   const int tile = group * 4 + wave;
   if ((int64_t)tile * 16 >= G) return;
   if (c.dtype == 0)
-    init_tile_chunk<P>(td, it, ct, static_cast<const int32_t*>(c.ptr), c.ld, G, tile, chunk,
-                       part);
+    init_tile_chunk<int32_t, ZM>(td, it, ct, static_cast<const int32_t*>(c.ptr), c.ld, G, tile,
+                                 chunk, part);
   else if (c.dtype == 1)
-    init_tile_chunk<P>(td, it, ct, static_cast<const float*>(c.ptr), c.ld, G, tile, chunk, part);
+    init_tile_chunk<float, ZM>(td, it, ct, static_cast<const float*>(c.ptr), c.ld, G, tile,
+                               chunk, part);
   else
-    init_tile_chunk<P>(td, it, ct, static_cast<const double*>(c.ptr), c.ld, G, tile, chunk,
-                       part);
+    init_tile_chunk<double, ZM>(td, it, ct, static_cast<const double*>(c.ptr), c.ld, G, tile,
+                                chunk, part);
 }
 
 // n * s2 - s1^2 exactly (128-bit), then one rounding
@@ -536,8 +549,13 @@ hipError_t launch_init_tiled(hipStream_t s, const TileDesign& td, const InitTile
                              double* part, double* alpha0, double* beta0) {
   if (G <= 0) return hipSuccess;
   const int64_t groups = ((G + 15) / 16 + 3) / 4;
-  DV_POST_P(p, k_init_accum, dim3((unsigned)(groups * ct.nchunk)), dim3(256), s, td, it, ct, c,
-            G, part);
+  const dim3 grid((unsigned)(groups * ct.nchunk));
+  if (it.mode == kInitQR)
+    hipLaunchKernelGGL(k_init_accum<1>, grid, dim3(256), 0, s, td, it, ct, c, G, part);
+  else if (it.mode == kInitUnivariate)
+    hipLaunchKernelGGL(k_init_accum<2>, grid, dim3(256), 0, s, td, it, ct, c, G, part);
+  else
+    hipLaunchKernelGGL(k_init_accum<0>, grid, dim3(256), 0, s, td, it, ct, c, G, part);
   DV_POST_P(p, k_init_finish, dim3((unsigned)((G + 255) / 256)), dim3(256), s, it, ct, G, n,
             c.dtype == 0 ? 1 : 0, part, alpha0, beta0);
   return hipGetLastError();

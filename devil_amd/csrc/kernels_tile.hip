@@ -63,16 +63,15 @@ __device__ __forceinline__ void irls_tile_chunk(const TileDesign& td, const CT* 
   }
   const double k = live ? kvec[g] : 0.0;
   const CT* yrow = Y + (int64_t)g * ld;
+  const CT* yh = yrow + 4 * h;
   const bool vec = (ld & 3) == 0 && ((reinterpret_cast<uintptr_t>(Y) & 15) == 0);
   d4 acc[NTMAX];
 #pragma unroll
   for (int t = 0; t < NTMAX; ++t) acc[t] = d4{0.0, 0.0, 0.0, 0.0};
   const int64_t b0 = (int64_t)chunk * td.chunk_blocks;
   const int64_t b1 = min((int64_t)(chunk + 1) * td.chunk_blocks, td.nblk);
-  BlockOps<NJ, NTMAX> cur, nxt;
-  if (b0 < b1) load_block<NJ, NTMAX, CT>(td, yrow, vec, b0, lane, h, NT, cur);  // trailing chunks may be empty
-  for (int64_t b = b0; b < b1; ++b) {
-    if (b + 1 < b1) load_block<NJ, NTMAX, CT>(td, yrow, vec, b + 1, lane, h, NT, nxt);
+  typedef TileBlock<NJ, NTMAX, CT> Blk;
+  auto body = [&](const Blk& cur, int64_t) {
     d4 eta = d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int j = 0; j < NJ; ++j) eta = mfma16x16x4(cur.xa[j], bneg[j], eta);
@@ -83,7 +82,7 @@ __device__ __forceinline__ void irls_tile_chunk(const TileDesign& td, const CT* 
     for (int r = 0; r < 4; ++r) {
       const double w = exp_f64(eta[r] - cur.off[r]);
       const double d = fma(k, w, 1.0);
-      q[r] = fma(cur.y[r], w, -1.0) * rcp1_f64(d);
+      q[r] = fma((double)cur.y[r], w, -1.0) * rcp1_f64(d);
     }
 #pragma unroll
     for (int t = 0; t < NTMAX; ++t) {
@@ -92,8 +91,11 @@ __device__ __forceinline__ void irls_tile_chunk(const TileDesign& td, const CT* 
         for (int r = 0; r < 4; ++r) acc[t] = mfma16x16x4(q[r], cur.kb[t][r], acc[t]);
       }
     }
-    cur = nxt;
-  }
+  };
+  auto load = [&](int64_t b, Blk& o) {
+    load_tile_block<NJ, NTMAX, CT>(td, yrow, yh, vec, b, lane, h, NT, td.offp, o);
+  };
+  pipeline<Blk>(b0, b1, load, body);  // trailing chunks may be empty
   // partial sums: register r holds gene slot h + 4 r, column 16 t + s
   double* out = part + ((int64_t)tile * td.nchunk + chunk) * 16 * td.ncp;
 #pragma unroll
@@ -121,65 +123,29 @@ __device__ __forceinline__ f4 mfma32(float a, float b, f4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
-template <int NJ, int NTMAX>
+template <int NJ, int NTMAX, typename CT>
 struct BlockOpsF {
-  float y[4];
+  CT y[4];
   float xa[NJ];
   f4 off;
   f4 kb[NTMAX];
 };
 
-// Four consecutive counts as floats (counts are exact in FP32 up to 2^24).
-template <typename CT>
-__device__ __forceinline__ void load4f(const CT* p, bool vec, float (&y)[4]);
-template <>
-__device__ __forceinline__ void load4f<int32_t>(const int32_t* p, bool vec, float (&y)[4]) {
-  if (vec) {
-    const i4v v = *reinterpret_cast<const i4v*>(p);
-#pragma unroll
-    for (int r = 0; r < 4; ++r) y[r] = (float)v[r];
-  } else {
-#pragma unroll
-    for (int r = 0; r < 4; ++r) y[r] = (float)p[r];
-  }
-}
-template <>
-__device__ __forceinline__ void load4f<float>(const float* p, bool vec, float (&y)[4]) {
-  if (vec) {
-    const f4v v = *reinterpret_cast<const f4v*>(p);
-#pragma unroll
-    for (int r = 0; r < 4; ++r) y[r] = v[r];
-  } else {
-#pragma unroll
-    for (int r = 0; r < 4; ++r) y[r] = p[r];
-  }
-}
-template <>
-__device__ __forceinline__ void load4f<double>(const double* p, bool vec, float (&y)[4]) {
-  double t[4];
-  load4<double>(p, vec, t);
-#pragma unroll
-  for (int r = 0; r < 4; ++r) y[r] = (float)t[r];
-}
-
-
 template <int NJ, int NTMAX, typename CT>
-__device__ __forceinline__ void load_block_f(const TileDesign& td, const CT* yrow, bool vec,
-                                             int64_t b, int lane, int h, int NT,
-                                             BlockOpsF<NJ, NTMAX>& o) {
-  const int64_t c = b * 16 + 4 * h;
-  if (c + 4 <= td.n) {
-    load4f<CT>(yrow + c, vec, o.y);
-  } else {
+__device__ __forceinline__ void load_block_f(const TileDesign& td, const CT* yrow, const CT* yh,
+                                             bool vec, int64_t b, int lane, int h, int NT,
+                                             BlockOpsF<NJ, NTMAX, CT>& o) {
+  const float* xs = td.Xf + b * (NJ * 64);
 #pragma unroll
-    for (int r = 0; r < 4; ++r) o.y[r] = (c + r < td.n) ? (float)yrow[c + r] : 0.0f;
+  for (int j = 0; j < NJ; ++j) o.xa[j] = xs[j * 64 + lane];
+  o.off = reinterpret_cast<const f4*>(td.offf + b * 16)[h];
+  const f4* ks = reinterpret_cast<const f4*>(td.Kf) + b * NT * 64;
+#pragma unroll
+  for (int t = 0; t < NTMAX; ++t) {
+    const int tt = t < NT ? t : NT - 1;
+    o.kb[t] = ks[tt * 64 + lane];
   }
-#pragma unroll
-  for (int j = 0; j < NJ; ++j) o.xa[j] = td.Xf[(b * NJ + j) * 64 + lane];
-  o.off = *reinterpret_cast<const f4*>(td.offf + c);
-#pragma unroll
-  for (int t = 0; t < NTMAX; ++t)
-    if (t < NT) o.kb[t] = *reinterpret_cast<const f4*>(td.Kf + ((b * NT + t) * 64 + lane) * 4);
+  load_block_y<CT>(td, yrow, yh, vec, b, h, o.y);
 }
 
 template <int P, typename CT>
@@ -211,6 +177,7 @@ __device__ __forceinline__ void irls_tile_chunk_mixed(const TileDesign& td,
   }
   const float k = live ? (float)kvec[g] : 0.0f;  // k is a float in the reference
   const CT* yrow = Y + (int64_t)g * ld;
+  const CT* yh = yrow + 4 * h;
   const bool vec = (ld & 3) == 0 && ((reinterpret_cast<uintptr_t>(Y) & 15) == 0);
   f4 acc[NTMAX];
   double acc64[NTMAX][4];
@@ -222,12 +189,10 @@ __device__ __forceinline__ void irls_tile_chunk_mixed(const TileDesign& td,
   }
   const int64_t b0 = (int64_t)chunk * td.chunk_blocks;
   const int64_t b1 = min((int64_t)(chunk + 1) * td.chunk_blocks, td.nblk);
-  BlockOpsF<NJ, NTMAX> cur, nxt;
-  if (b0 < b1) load_block_f<NJ, NTMAX, CT>(td, yrow, vec, b0, lane, h, NT, cur);  // trailing chunks may be empty
+  typedef BlockOpsF<NJ, NTMAX, CT> Blk;
   int span = 0;
   float tmin = 0.0f;  // smallest log2(mu) seen by this lane
-  for (int64_t b = b0; b < b1; ++b) {
-    if (b + 1 < b1) load_block_f<NJ, NTMAX, CT>(td, yrow, vec, b + 1, lane, h, NT, nxt);
+  auto body = [&](const Blk& cur, int64_t b) {
     f4 eta = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int j = 0; j < NJ; ++j) {
@@ -242,7 +207,7 @@ __device__ __forceinline__ void irls_tile_chunk_mixed(const TileDesign& td,
       const float t2 = (eta[r] + cur.off[r]) * 1.44269504088896341f;
       tmin = fminf(tmin, t2);
       const float mu = __builtin_amdgcn_exp2f(t2);
-      q[r] = (cur.y[r] - mu) * __builtin_amdgcn_rcpf(k + mu);
+      q[r] = ((float)cur.y[r] - mu) * __builtin_amdgcn_rcpf(k + mu);
     }
 #pragma unroll
     for (int t = 0; t < NTMAX; ++t) {
@@ -260,8 +225,11 @@ __device__ __forceinline__ void irls_tile_chunk_mixed(const TileDesign& td,
         acc[t] = f4{0.f, 0.f, 0.f, 0.f};
       }
     }
-    cur = nxt;
-  }
+  };
+  auto load = [&](int64_t b, Blk& o) {
+    load_block_f<NJ, NTMAX, CT>(td, yrow, yh, vec, b, lane, h, NT, o);
+  };
+  pipeline<Blk>(b0, b1, load, body);  // trailing chunks may be empty
   // register r holds gene slot 4 h + r, column 16 t + s
   // exp(-eta - off) overflows in FP64 where log2(mu) < -1024: the reference
   // then takes a NaN step, so such a gene's partials are made NaN and the
@@ -330,42 +298,89 @@ __device__ __forceinline__ f4 mfma_bf(bf8 a, bf8 b, f4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
-// Counts of one 32-cell step for lane (s, h): cells 4h + r and 16 + 4h + r.
-template <typename CT>
-__device__ __forceinline__ void load_step_y(const TileDesign& td, const CT* yrow, bool vec,
-                                            int64_t st, int h, float (&y)[8]) {
+// Operands of one 32-cell step for lane (s, h): counts (as stored) of cells
+// 4h + r and 16 + 4h + r, the eta A operands of both halves, the offsets
+// (times log2 e), the gradient-tile and pair-tile B operands.
+template <int NE, int NTH, typename CT>
+struct BfStep {
+  bf8 xa[2 * NE];
+  f4 off0, off1;
+  f4 kg0, kg1;
+  bf8 k1[NTH], k2[NTH];
+  CT y[8];
+};
+
+// Counts of one step for lane (s, h).  yh = the gene's row + 4 h.  Steps
+// wholly inside the row (all but the last) take the unchecked loads; the
+// branch is uniform.  Table addresses are a uniform per-step base plus the
+// lane's offset, so they need no 64-bit vector arithmetic.
+template <int NE, int NTH, typename CT>
+__device__ __forceinline__ void load_bf_y(const TileDesign& td, const CT* yrow, const CT* yh,
+                                          bool vec, int64_t st, int h, CT (&y)[8]) {
+  const int64_t cs = st * 32;
+  if (cs + 32 <= td.n) {
+    const CT* p = yh + cs;
+    if (vec) {
+      typedef typename Vec4Of<CT>::T V4;
+      const V4 a = *reinterpret_cast<const V4*>(p);
+      const V4 b = *reinterpret_cast<const V4*>(p + 16);
 #pragma unroll
-  for (int u = 0; u < 2; ++u) {
-    const int64_t c = st * 32 + 16 * u + 4 * h;
-    float t[4];
-    if (c + 4 <= td.n) {
-      load4f<CT>(yrow + c, vec, t);
+      for (int r = 0; r < 4; ++r) {
+        y[r] = a[r];
+        y[4 + r] = b[r];
+      }
     } else {
 #pragma unroll
-      for (int r = 0; r < 4; ++r) t[r] = (c + r < td.n) ? (float)yrow[c + r] : 0.0f;
+      for (int r = 0; r < 4; ++r) {
+        y[r] = p[r];
+        y[4 + r] = p[16 + r];
+      }
     }
+  } else {
+    CT t0[4], t1[4];
+    load4_raw<CT>(yrow, cs + 4 * h, td.n, vec, t0);
+    load4_raw<CT>(yrow, cs + 16 + 4 * h, td.n, vec, t1);
 #pragma unroll
-    for (int r = 0; r < 4; ++r) y[4 * u + r] = t[r];
+    for (int r = 0; r < 4; ++r) {
+      y[r] = t0[r];
+      y[4 + r] = t1[r];
+    }
   }
 }
 
-template <int P, typename CT>
-__device__ __forceinline__ void irls_tile_chunk_bf(const TileDesign& td, const CT* __restrict__ Y,
-                                                   int64_t ld, const int32_t* __restrict__ active,
-                                                   int nA, const double* __restrict__ beta,
-                                                   const double* __restrict__ kvec, int tile,
-                                                   int chunk, double* __restrict__ part) {
-  constexpr int NE = (7 * P + 31) / 32;
-  constexpr int NTH = ((P * (P + 1)) / 2 + 15) / 16;  // pair tiles at most
-  constexpr int kSpan = 4;                            // steps per FP32 span (gradient tile)
-  const int lane = threadIdx.x & 63;
-  const int s = lane & 15, h = lane >> 4;
-  const int slot = tile * 16 + s;
-  const bool live = slot < nA;
-  const int g = live ? active[slot] : active[tile * 16];
-  const int nth = td.nt - 1;
-  // B operand of eta: entries 32 j + 8 h + i of gene slot s
-  bf8 bb[NE];
+template <int NE, typename CT, typename S>
+__device__ __forceinline__ void load_bf_eta(const TileDesign& td, int64_t st, int lane, int h,
+                                            S& o) {
+  const bf8* xs = reinterpret_cast<const bf8*>(td.xab) + st * (2 * NE * 64);
+#pragma unroll
+  for (int j = 0; j < 2 * NE; ++j) o.xa[j] = xs[j * 64 + lane];
+  const f4* os = reinterpret_cast<const f4*>(td.offl + st * 32);
+  o.off0 = os[h];
+  o.off1 = os[4 + h];
+}
+
+template <int NTH, typename S>
+__device__ __forceinline__ void load_bf_k(const TileDesign& td, int64_t st, int lane, int nth,
+                                          S& o) {
+  const f4* ks = reinterpret_cast<const f4*>(td.kg) + st * 128;
+  o.kg0 = ks[lane];
+  o.kg1 = ks[64 + lane];
+  const bf8* k1s = reinterpret_cast<const bf8*>(td.kh) + st * nth * 64;
+  const bf8* k2s = reinterpret_cast<const bf8*>(td.kh2) + st * nth * 64;
+#pragma unroll
+  for (int t = 0; t < NTH; ++t) {
+    // unused tiles load a used one (or tile 0 of a zero table): no branch
+    const int tt = t < nth ? t : (nth > 0 ? nth - 1 : 0);
+    o.k1[t] = k1s[tt * 64 + lane];
+    o.k2[t] = k2s[tt * 64 + lane];
+  }
+}
+
+// B operand of eta for gene slot s: entries 32 j + 8 h + i, entry e = 7 f + term
+// holding the bf16 piece of log2(e) beta_f that term selects (see above).
+template <int P, int NE>
+__device__ __forceinline__ void bf_beta_operand(const double* __restrict__ beta, int g, bool live,
+                                                int h, bf8 (&bb)[NE]) {
 #pragma unroll
   for (int j = 0; j < NE; ++j) {
 #pragma unroll
@@ -386,6 +401,43 @@ __device__ __forceinline__ void irls_tile_chunk_bf(const TileDesign& td, const C
       bb[j][i] = (__bf16)v;
     }
   }
+}
+
+// eta of the step's 8 cells and their residuals r = (y - mu) / (k + mu).
+template <int NE, typename CT, typename S>
+__device__ __forceinline__ void bf_residuals(const S& o, const CT (&y)[8], const bf8 (&bb)[NE],
+                                             float k, float& tmin, float (&rr)[8]) {
+  f4 eta0 = f4{0.f, 0.f, 0.f, 0.f}, eta1 = eta0;
+#pragma unroll
+  for (int j = 0; j < NE; ++j) eta0 = mfma_bf(o.xa[j], bb[j], eta0);
+#pragma unroll
+  for (int j = 0; j < NE; ++j) eta1 = mfma_bf(o.xa[NE + j], bb[j], eta1);
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const float t2 = (r < 4 ? eta0[r] : eta1[r - 4]) + (r < 4 ? o.off0[r] : o.off1[r - 4]);
+    tmin = fminf(tmin, t2);
+    const float mu = __builtin_amdgcn_exp2f(t2);
+    rr[r] = ((float)y[r] - mu) * __builtin_amdgcn_rcpf(k + mu);
+  }
+}
+
+template <int P, typename CT>
+__device__ __forceinline__ void irls_tile_chunk_bf(const TileDesign& td, const CT* __restrict__ Y,
+                                                   int64_t ld, const int32_t* __restrict__ active,
+                                                   int nA, const double* __restrict__ beta,
+                                                   const double* __restrict__ kvec, int tile,
+                                                   int chunk, double* __restrict__ part) {
+  constexpr int NE = (7 * P + 31) / 32;
+  constexpr int NTH = ((P * (P + 1)) / 2 + 15) / 16;  // pair tiles at most
+  constexpr int kSpan = 4;                            // steps per FP32 span (gradient tile)
+  const int lane = threadIdx.x & 63;
+  const int s = lane & 15, h = lane >> 4;
+  const int slot = tile * 16 + s;
+  const bool live = slot < nA;
+  const int g = live ? active[slot] : active[tile * 16];
+  const int nth = td.nt - 1;
+  bf8 bb[NE];
+  bf_beta_operand<P, NE>(beta, g, live, h, bb);
   const float k = live ? (float)kvec[g] : 0.0f;
   const CT* yrow = Y + (int64_t)g * ld;
   const bool vec = (ld & 3) == 0 && ((reinterpret_cast<uintptr_t>(Y) & 15) == 0);
@@ -399,81 +451,47 @@ __device__ __forceinline__ void irls_tile_chunk_bf(const TileDesign& td, const C
 #pragma unroll
   for (int t = 0; t < NTH; ++t) ah[t] = f4{0.f, 0.f, 0.f, 0.f};
   float tmin = 0.0f;  // smallest log2(mu) seen by this lane
-  const bf8* xab = reinterpret_cast<const bf8*>(td.xab);
-  const bf8* kh = reinterpret_cast<const bf8*>(td.kh);
-  const bf8* kh2 = reinterpret_cast<const bf8*>(td.kh2);
-  const f4* kgp = reinterpret_cast<const f4*>(td.kg);
-  auto step = [&](int64_t st, const float (&y)[8]) {
-    const f4 off0 = *reinterpret_cast<const f4*>(td.offl + st * 32 + 4 * h);
-    const f4 off1 = *reinterpret_cast<const f4*>(td.offl + st * 32 + 16 + 4 * h);
-    f4 eta0 = f4{0.f, 0.f, 0.f, 0.f}, eta1 = eta0;
-#pragma unroll
-    for (int j = 0; j < NE; ++j) eta0 = mfma_bf(xab[(st * 2 * NE + j) * 64 + lane], bb[j], eta0);
-#pragma unroll
-    for (int j = 0; j < NE; ++j)
-      eta1 = mfma_bf(xab[(st * 2 * NE + NE + j) * 64 + lane], bb[j], eta1);
-    const f4 kg0 = kgp[(st * 2) * 64 + lane], kg1 = kgp[(st * 2 + 1) * 64 + lane];
+  typedef BfStep<NE, NTH, CT> Step;
+  const CT* yh = yrow + 4 * h;
+  auto load = [&](int64_t st, Step& o) {
+    load_bf_eta<NE, CT>(td, st, lane, h, o);
+    load_bf_k<NTH>(td, st, lane, nth, o);
+    load_bf_y<NE, NTH, CT>(td, yrow, yh, vec, st, h, o.y);
+  };
+  int span = 0;
+  auto step = [&](const Step& o, int64_t) {
     float rr[8];
-#pragma unroll
-    for (int r = 0; r < 8; ++r) {
-      const float t2 = (r < 4 ? eta0[r] : eta1[r - 4]) + (r < 4 ? off0[r] : off1[r - 4]);
-      tmin = fminf(tmin, t2);
-      const float mu = __builtin_amdgcn_exp2f(t2);
-      rr[r] = (y[r] - mu) * __builtin_amdgcn_rcpf(k + mu);
-    }
+    bf_residuals<NE, CT>(o, o.y, bb, k, tmin, rr);
     // gradient tile: FP32 products, cell c(h, j) for step j
 #pragma unroll
-    for (int j = 0; j < 4; ++j) ag0 = mfma32(rr[j], kg0[j], ag0);
+    for (int j = 0; j < 4; ++j) ag0 = mfma32(rr[j], o.kg0[j], ag0);
 #pragma unroll
-    for (int j = 0; j < 4; ++j) ag0 = mfma32(rr[4 + j], kg1[j], ag0);
+    for (int j = 0; j < 4; ++j) ag0 = mfma32(rr[4 + j], o.kg1[j], ag0);
     bf8 r1, r2;
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
       r1[i] = (__bf16)rr[i];
       r2[i] = (__bf16)(rr[i] - (float)r1[i]);
     }
-    bf8 k1[NTH], k2[NTH];
 #pragma unroll
     for (int t = 0; t < NTH; ++t)
       if (t < nth) {
-        k1[t] = kh[(st * nth + t) * 64 + lane];
-        k2[t] = kh2[(st * nth + t) * 64 + lane];
+        ah[t] = mfma_bf(r2, o.k1[t], ah[t]);
+        ah[t] = mfma_bf(r1, o.k2[t], ah[t]);
+        ah[t] = mfma_bf(r1, o.k1[t], ah[t]);
       }
+    if (++span == kSpan) {
+      span = 0;
 #pragma unroll
-    for (int t = 0; t < NTH; ++t)
-      if (t < nth) {
-        ah[t] = mfma_bf(r2, k1[t], ah[t]);
-        ah[t] = mfma_bf(r1, k2[t], ah[t]);
-        ah[t] = mfma_bf(r1, k1[t], ah[t]);
-      }
-  };
-  auto flush = [&]() {
-#pragma unroll
-    for (int r = 0; r < 4; ++r) ag64[r] += (double)ag0[r];
-    ag0 = f4{0.f, 0.f, 0.f, 0.f};
+      for (int r = 0; r < 4; ++r) ag64[r] += (double)ag0[r];
+      ag0 = f4{0.f, 0.f, 0.f, 0.f};
+    }
   };
   const int64_t s0 = (int64_t)chunk * td.chunk_blocks;
   const int64_t s1 = min((int64_t)(chunk + 1) * td.chunk_blocks, td.nblk);
-  // counts of the next step are loaded a step ahead, into two buffers in turn
-  float yA[8], yB[8];
-  if (s0 < s1) load_step_y<CT>(td, yrow, vec, s0, h, yA);  // trailing chunks may be empty
-  int span = 0;
-  for (int64_t st = s0; st < s1; st += 2) {
-    if (st + 1 < s1) load_step_y<CT>(td, yrow, vec, st + 1, h, yB);
-    step(st, yA);
-    if (++span == kSpan) {
-      span = 0;
-      flush();
-    }
-    if (st + 1 >= s1) break;
-    if (st + 2 < s1) load_step_y<CT>(td, yrow, vec, st + 2, h, yA);
-    step(st + 1, yB);
-    if (++span == kSpan) {
-      span = 0;
-      flush();
-    }
-  }
-  flush();
+  pipeline<Step>(s0, s1, load, step);  // trailing chunks may be empty
+#pragma unroll
+  for (int r = 0; r < 4; ++r) ag64[r] += (double)ag0[r];
   // exp(-eta - off) overflows in FP64 where log2(mu) < -1024: the reference
   // then takes a NaN step, so such a gene's partials are made NaN and the
   // gene is redone in FP64.  Register r of this lane holds gene slot 4 h + r.
@@ -516,6 +534,201 @@ __global__ __launch_bounds__(256) void k_irls_accum_bf16(TileDesign td, Counts c
   else
     irls_tile_chunk_bf<P>(td, static_cast<const double*>(c.ptr), c.ld, active, nA, beta, kvec,
                           tile, chunk, part);
+}
+
+// ---------------------------------------------------------------------------
+// Wave-specialised bf16 variant.  The SIMD overlaps MFMA and VALU work only
+// across wavefronts, so each workgroup pairs, on every SIMD, a producer wave
+// (counts, eta, exp / reciprocal, bf16 splits: VALU work plus the small eta
+// MFMAs) with a consumer wave (the gradient and pair-column contractions:
+// MFMA work) for the same 16-gene tile.  The producer of step i hands r, r1,
+// r2 to its consumer through a two-slot LDS ring; one workgroup barrier per
+// step (the consumer works on step i - 1 while the producer makes step i).
+// Waves 0-3 produce for tiles 0-3, waves 4-7 consume (wave w runs on SIMD
+// w % 4).  Results are identical to k_irls_accum_bf16 (same arithmetic).
+// ---------------------------------------------------------------------------
+struct WsSlot {
+  f4 r0[4][64];   // r of cells c(h, 0..3), per tile and lane
+  f4 r1f[4][64];  // r of cells c(h, 4..7)
+  bf8 b1[4][64];  // bf16 r1
+  bf8 b2[4][64];  // bf16 r2 = r - r1
+};
+
+template <int NE, typename CT>
+struct BfEtaOps {  // producer operands of one step
+  CT y[8];
+  bf8 xa[2 * NE];
+  f4 off0, off1;
+};
+
+template <int NTH>
+struct BfKOps {  // consumer operands of one step
+  f4 kg0, kg1;
+  bf8 k1[NTH], k2[NTH];
+};
+
+template <int P, typename CT>
+__device__ __forceinline__ void irls_ws_producer(const TileDesign& td, const CT* __restrict__ Y,
+                                                 int64_t ld, const int32_t* __restrict__ active,
+                                                 int nA, const double* __restrict__ beta,
+                                                 const double* __restrict__ kvec, int tile,
+                                                 bool tile_live, int w, int64_t s0, int64_t s1,
+                                                 WsSlot* ring, float* tmin_out) {
+  constexpr int NE = (7 * P + 31) / 32;
+  constexpr int NTH = ((P * (P + 1)) / 2 + 15) / 16;
+  const int lane = threadIdx.x & 63;
+  const int s = lane & 15, h = lane >> 4;
+  const int slot = tile * 16 + s;
+  const bool live = tile_live && slot < nA;
+  const int g = tile_live ? (live ? active[slot] : active[tile * 16]) : 0;
+  bf8 bb[NE];
+  bf_beta_operand<P, NE>(beta, g, live, h, bb);
+  const float k = live ? (float)kvec[g] : 0.0f;
+  const CT* yrow = Y + (int64_t)g * ld;
+  const bool vec = (ld & 3) == 0 && ((reinterpret_cast<uintptr_t>(Y) & 15) == 0);
+  float tmin = 0.0f;
+  typedef BfEtaOps<NE, CT> Ops;
+  const CT* yh = yrow + 4 * h;
+  auto load = [&](int64_t st, Ops& o) {
+    load_bf_eta<NE, CT>(td, st, lane, h, o);
+    load_bf_y<NE, NTH, CT>(td, yrow, yh, vec, st, h, o.y);
+  };
+  // operands are loaded a step ahead; the residuals of step st go to ring
+  // slot (st - s0) & 1, read by the consumer after the barrier
+  auto body = [&](int64_t st, const Ops& cur, Ops& nxt) {
+    if (tile_live && st < s1) {
+      load(min(st + 1, s1 - 1), nxt);
+      float rr[8];
+      bf_residuals<NE, CT>(cur, cur.y, bb, k, tmin, rr);
+      bf8 r1, r2;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        r1[i] = (__bf16)rr[i];
+        r2[i] = (__bf16)(rr[i] - (float)r1[i]);
+      }
+      WsSlot& o = ring[(st - s0) & 1];
+      o.r0[w][lane] = f4{rr[0], rr[1], rr[2], rr[3]};
+      o.r1f[w][lane] = f4{rr[4], rr[5], rr[6], rr[7]};
+      o.b1[w][lane] = r1;
+      o.b2[w][lane] = r2;
+    }
+    __syncthreads();
+  };
+  Ops A, B;
+  if (tile_live && s0 < s1) load(s0, A);
+  for (int64_t st = s0; st <= s1; st += 2) {
+    body(st, A, B);
+    if (st + 1 > s1) break;
+    body(st + 1, B, A);
+  }
+  tmin = fminf(tmin, __shfl_xor(tmin, 16, 64));
+  tmin = fminf(tmin, __shfl_xor(tmin, 32, 64));
+  if (tile_live && h == 0) tmin_out[w * 16 + s] = tmin;
+  __syncthreads();
+}
+
+template <int P>
+__device__ __forceinline__ void irls_ws_consumer(const TileDesign& td, int tile, bool tile_live,
+                                                 int w, int chunk, int64_t s0, int64_t s1,
+                                                 WsSlot* ring, const float* tmin_in,
+                                                 double* __restrict__ part) {
+  constexpr int NTH = ((P * (P + 1)) / 2 + 15) / 16;
+  constexpr int kSpan = 4;
+  const int lane = threadIdx.x & 63;
+  const int s = lane & 15, h = lane >> 4;
+  const int nth = td.nt - 1;
+  f4 ag = f4{0.f, 0.f, 0.f, 0.f};
+  double ag64[4] = {0.0, 0.0, 0.0, 0.0};
+  f4 ah[NTH];
+#pragma unroll
+  for (int t = 0; t < NTH; ++t) ah[t] = f4{0.f, 0.f, 0.f, 0.f};
+  int span = 0;
+  typedef BfKOps<NTH> Ops;
+  // iteration st loads the operands of step st and contracts step st - 1
+  // (its residuals were written to the ring before the last barrier)
+  auto body = [&](int64_t st, const Ops& cur, Ops& nxt) {
+    if (tile_live && s0 < s1) {  // trailing chunks may be empty
+      load_bf_k<NTH>(td, min(st, s1 - 1), lane, nth, nxt);
+      if (st > s0) {
+        const WsSlot& o = ring[(st - 1 - s0) & 1];
+        const f4 ra = o.r0[w][lane], rb = o.r1f[w][lane];
+        const bf8 r1 = o.b1[w][lane], r2 = o.b2[w][lane];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) ag = mfma32(ra[j], cur.kg0[j], ag);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) ag = mfma32(rb[j], cur.kg1[j], ag);
+#pragma unroll
+        for (int t = 0; t < NTH; ++t)
+          if (t < nth) {
+            ah[t] = mfma_bf(r2, cur.k1[t], ah[t]);
+            ah[t] = mfma_bf(r1, cur.k2[t], ah[t]);
+            ah[t] = mfma_bf(r1, cur.k1[t], ah[t]);
+          }
+        if (++span == kSpan) {
+          span = 0;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) ag64[r] += (double)ag[r];
+          ag = f4{0.f, 0.f, 0.f, 0.f};
+        }
+      }
+    }
+    __syncthreads();
+  };
+  Ops A, B;
+  for (int64_t st = s0; st <= s1; st += 2) {
+    body(st, B, A);
+    if (st + 1 > s1) break;
+    body(st + 1, A, B);
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) ag64[r] += (double)ag[r];
+  __syncthreads();  // the producers' tmin is in LDS
+  if (!tile_live) return;
+  double* out = part + ((int64_t)tile * td.nchunk + chunk) * 16 * td.ncp;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const bool poison = tmin_in[w * 16 + 4 * h + r] < -1024.0f;
+    const double nanv = __builtin_nan("");
+    out[(4 * h + r) * td.ncp + s] = poison ? nanv : ag64[r];
+#pragma unroll
+    for (int t = 0; t < NTH; ++t)
+      if (t < nth) out[(4 * h + r) * td.ncp + 16 * (t + 1) + s] = poison ? nanv : (double)ah[t][r];
+  }
+}
+
+template <int P>
+__global__ __launch_bounds__(512) void k_irls_accum_ws(TileDesign td, Counts c,
+                                                       const int32_t* __restrict__ active,
+                                                       const int32_t* __restrict__ nactive,
+                                                       const double* __restrict__ beta,
+                                                       const double* __restrict__ kvec,
+                                                       double* __restrict__ part) {
+  __shared__ WsSlot ring[2];
+  __shared__ float tmin_sh[64];
+  const int nA = *nactive;
+  const int ntile = (nA + 15) / 16;
+  const int wave = threadIdx.x >> 6;
+  const int w = wave & 3;
+  int chunk, group;
+  tile_coords(td.nchunk, chunk, group);
+  if (group * 4 >= ntile) return;  // the whole workgroup
+  const int tile = group * 4 + w;
+  const bool tile_live = tile < ntile;
+  const int64_t s0 = (int64_t)chunk * td.chunk_blocks;
+  const int64_t s1 = max(s0, min((int64_t)(chunk + 1) * td.chunk_blocks, td.nblk));
+  if (wave < 4) {
+    if (c.dtype == 0)
+      irls_ws_producer<P>(td, static_cast<const int32_t*>(c.ptr), c.ld, active, nA, beta, kvec,
+                          tile, tile_live, w, s0, s1, ring, tmin_sh);
+    else if (c.dtype == 1)
+      irls_ws_producer<P>(td, static_cast<const float*>(c.ptr), c.ld, active, nA, beta, kvec,
+                          tile, tile_live, w, s0, s1, ring, tmin_sh);
+    else
+      irls_ws_producer<P>(td, static_cast<const double*>(c.ptr), c.ld, active, nA, beta, kvec,
+                          tile, tile_live, w, s0, s1, ring, tmin_sh);
+  } else {
+    irls_ws_consumer<P>(td, tile, tile_live, w, chunk, s0, s1, ring, tmin_sh, part);
+  }
 }
 
 template <int P>
@@ -694,6 +907,17 @@ hipError_t launch_irls_accum_bf16(hipStream_t s, const TileDesign& td, Counts c,
   const int groups = (ntile + 3) / 4;
   const dim3 grid((unsigned)((int64_t)groups * td.nchunk));
   DV_TILE_P(p, k_irls_accum_bf16, grid, dim3(256), s, td, c, active, nactive, beta, kvec, part);
+  return hipGetLastError();
+}
+
+hipError_t launch_irls_accum_ws(hipStream_t s, const TileDesign& td, Counts c, int p,
+                                const int32_t* active, const int32_t* nactive, int nA_bound,
+                                const double* beta, const double* kvec, double* part) {
+  if (nA_bound <= 0) return hipSuccess;
+  const int ntile = (nA_bound + 15) / 16;
+  const int groups = (ntile + 3) / 4;
+  const dim3 grid((unsigned)((int64_t)groups * td.nchunk));
+  DV_TILE_P(p, k_irls_accum_ws, grid, dim3(512), s, td, c, active, nactive, beta, kvec, part);
   return hipGetLastError();
 }
 

@@ -196,9 +196,13 @@ enum { DEVIL_TUNE_MIXED_KERNEL = 0,  /* DEVIL_MIXED_* (default AUTO) */
 enum { DEVIL_MIXED_F32 = 0,   /* eta and all contractions on v_mfma_f32_16x16x4f32 */
        DEVIL_MIXED_BF16 = 1,  /* eta (split operands) and pair columns on
                                  v_mfma_f32_16x16x32_bf16, design columns FP32 */
-       DEVIL_MIXED_AUTO = 2   /* autotuner: times the candidates on the problem's
+       DEVIL_MIXED_AUTO = 2,  /* autotuner: times the candidates on the problem's
                                  first genes once per (p, n, count type) and keeps
-                                 the fastest for the context */ };
+                                 the fastest for the context */
+       DEVIL_MIXED_BF16_WS = 3 /* DEVIL_MIXED_BF16 arithmetic on wave-specialised
+                                  workgroups: producer wavefronts (eta, exp, residual)
+                                  hand residuals through LDS to consumer wavefronts
+                                  (the contractions) */ };
 int devil_set_tuning(devil_ctx *ctx, int32_t key, int64_t value);
 
 /* Per-launch timing of the IRLS accumulate kernels (the streaming pass over
