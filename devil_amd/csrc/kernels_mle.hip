@@ -133,7 +133,7 @@ __global__ void k_mle_collect(const int32_t* __restrict__ mode, int G, int32_t* 
   if (g < G && mode[g] != kMleDone) act[atomicAdd(nact, 1)] = g;
 }
 
-template <int P, typename CT>
+template <int P, typename CT, bool VEC>
 __device__ __forceinline__ void mle_tile_chunk(const TileDesign& td, const CT* __restrict__ Y,
                                                int64_t ld, const int32_t* __restrict__ active,
                                                int nA, const double* __restrict__ beta,
@@ -159,7 +159,6 @@ __device__ __forceinline__ void mle_tile_chunk(const TileDesign& td, const CT* _
   const bool cell_psi = tab_n[g] < 0;
   const CT* yrow = Y + (int64_t)g * ld;
   const CT* yh = yrow + 4 * h;
-  const bool vec = (ld & 3) == 0 && ((reinterpret_cast<uintptr_t>(Y) & 15) == 0);
   d4 a0[NTMAX], a1[NTMAX], a2[NTMAX];
 #pragma unroll
   for (int t = 0; t < NTMAX; ++t) {
@@ -213,9 +212,12 @@ __device__ __forceinline__ void mle_tile_chunk(const TileDesign& td, const CT* _
     }
   };
   auto load = [&](int64_t b, Blk& o) {
-    load_tile_block<NJ, NTMAX, CT>(td, yrow, yh, vec, b, lane, h, NT, td.offp, o);
+    load_tile_block<NJ, NTMAX, CT, VEC>(td, yh, b, lane, h, NT, td.offp, o);
   };
-  pipeline<Blk>(b0, b1, load, body);  // trailing chunks may be empty
+  auto load_tail = [&](int64_t b, Blk& o) {
+    load_tile_block_tail<NJ, NTMAX, CT>(td, yrow, b, lane, h, NT, td.offp, o);
+  };
+  pipeline<Blk>(b0, min(b1, td.n / 16), b1, load, load_tail, body);  // trailing chunks may be empty
   // the four lanes of a gene slot (h = 0..3) hold disjoint cells
 #pragma unroll
   for (int o = 16; o <= 32; o <<= 1) {
@@ -264,15 +266,11 @@ __global__ __launch_bounds__(256) void k_mle_accum(TileDesign td, Counts c,
   tile_coords(td.nchunk, chunk, group);
   const int tile = group * 4 + wave;
   if (tile >= ntile) return;
-  if (c.dtype == 0)
-    mle_tile_chunk<P>(td, static_cast<const int32_t*>(c.ptr), c.ld, active, nA, beta, xs,
-                      tab_n, tile, chunk, part);
-  else if (c.dtype == 1)
-    mle_tile_chunk<P>(td, static_cast<const float*>(c.ptr), c.ld, active, nA, beta, xs, tab_n,
-                      tile, chunk, part);
-  else
-    mle_tile_chunk<P>(td, static_cast<const double*>(c.ptr), c.ld, active, nA, beta, xs,
-                      tab_n, tile, chunk, part);
+  with_counts(c, [&](auto Y, auto vec) {
+    typedef std::remove_cv_t<std::remove_pointer_t<decltype(Y)>> CT;
+    mle_tile_chunk<P, CT, decltype(vec)::value>(td, Y, c.ld, active, nA, beta, xs, tab_n, tile,
+                                                chunk, part);
+  });
 }
 
 template <int P>

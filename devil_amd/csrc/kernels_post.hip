@@ -28,7 +28,7 @@
 
 namespace dv {
 
-template <int P, typename CT>
+template <int P, typename CT, bool VEC>
 __device__ __forceinline__ void mom_tile_chunk(const TileDesign& td, ChunkTable ct,
                                                const CT* __restrict__ Y, int64_t ld, int64_t G,
                                                const double* __restrict__ beta, int tile,
@@ -47,7 +47,6 @@ __device__ __forceinline__ void mom_tile_chunk(const TileDesign& td, ChunkTable 
   }
   const CT* yrow = Y + g * ld;
   const CT* yh = yrow + 4 * h;
-  const bool vec = (ld & 3) == 0 && ((reinterpret_cast<uintptr_t>(Y) & 15) == 0);
   const int64_t c0 = ct.c0[chunk], c1 = ct.c1[chunk];
   const int64_t b0 = c0 / 16, b1 = (c1 + 15) / 16;
   double num = 0.0, den = 0.0;
@@ -68,9 +67,12 @@ __device__ __forceinline__ void mom_tile_chunk(const TileDesign& td, ChunkTable 
     }
   };
   auto load = [&](int64_t b, Blk& o) {
-    load_tile_block<NJ, 0, CT>(td, yrow, yh, vec, b, lane, h, 0, td.eop, o);
+    load_tile_block<NJ, 0, CT, VEC>(td, yh, b, lane, h, 0, td.eop, o);
   };
-  if (c0 < c1) pipeline<Blk>(b0, b1, load, body);
+  auto load_tail = [&](int64_t b, Blk& o) {
+    load_tile_block_tail<NJ, 0, CT>(td, yrow, b, lane, h, 0, td.eop, o);
+  };
+  if (c0 < c1) pipeline<Blk>(b0, min(b1, td.n / 16), b1, load, load_tail, body);
 #pragma unroll
   for (int o = 16; o <= 32; o <<= 1) {
     num += __shfl_xor(num, o, 64);
@@ -83,7 +85,7 @@ __device__ __forceinline__ void mom_tile_chunk(const TileDesign& td, ChunkTable 
   }
 }
 
-template <int P, typename CT>
+template <int P, typename CT, bool VEC>
 __device__ __forceinline__ void bread_tile_chunk(const TileDesign& td, ChunkTable ct,
                                                  const CT* __restrict__ Y, int64_t ld,
                                                  int64_t G, const double* __restrict__ beta,
@@ -106,7 +108,6 @@ __device__ __forceinline__ void bread_tile_chunk(const TileDesign& td, ChunkTabl
   const double th = live ? theta[g] : 0.0;
   const CT* yrow = Y + g * ld;
   const CT* yh = yrow + 4 * h;
-  const bool vec = (ld & 3) == 0 && ((reinterpret_cast<uintptr_t>(Y) & 15) == 0);
   d4 aw[NTMAX], ar;
 #pragma unroll
   for (int t = 0; t < NTMAX; ++t) aw[t] = d4{0.0, 0.0, 0.0, 0.0};
@@ -141,9 +142,12 @@ __device__ __forceinline__ void bread_tile_chunk(const TileDesign& td, ChunkTabl
     for (int r = 0; r < 4; ++r) ar = mfma16x16x4(rr[r], cur.kb[0][r], ar);
   };
   auto load = [&](int64_t b, Blk& o) {
-    load_tile_block<NJ, NTMAX, CT>(td, yrow, yh, vec, b, lane, h, NT, td.eop, o);
+    load_tile_block<NJ, NTMAX, CT, VEC>(td, yh, b, lane, h, NT, td.eop, o);
   };
-  pipeline<Blk>(b0, b1, load, body);
+  auto load_tail = [&](int64_t b, Blk& o) {
+    load_tile_block_tail<NJ, NTMAX, CT>(td, yrow, b, lane, h, NT, td.eop, o);
+  };
+  pipeline<Blk>(b0, min(b1, td.n / 16), b1, load, load_tail, body);
   // register r holds gene slot h + 4 r, column 16 t + s
   const int W = td.ncp + 16;
   double* out = part + ((int64_t)tile * ct.nchunk + chunk) * 16 * W;
@@ -167,15 +171,10 @@ __global__ __launch_bounds__(256) void k_mom_accum(TileDesign td, ChunkTable ct,
   tile_coords(ct.nchunk, chunk, group);
   const int tile = group * 4 + wave;
   if ((int64_t)tile * 16 >= G) return;
-  if (c.dtype == 0)
-    mom_tile_chunk<P>(td, ct, static_cast<const int32_t*>(c.ptr), c.ld, G, beta, tile, chunk,
-                      part);
-  else if (c.dtype == 1)
-    mom_tile_chunk<P>(td, ct, static_cast<const float*>(c.ptr), c.ld, G, beta, tile, chunk,
-                      part);
-  else
-    mom_tile_chunk<P>(td, ct, static_cast<const double*>(c.ptr), c.ld, G, beta, tile, chunk,
-                      part);
+  with_counts(c, [&](auto Y, auto vec) {
+    typedef std::remove_cv_t<std::remove_pointer_t<decltype(Y)>> CT;
+    mom_tile_chunk<P, CT, decltype(vec)::value>(td, ct, Y, c.ld, G, beta, tile, chunk, part);
+  });
 }
 
 template <int P>
@@ -212,15 +211,11 @@ __global__ __launch_bounds__(256) void k_bread_accum(TileDesign td, ChunkTable c
   tile_coords(ct.nchunk, chunk, group);
   const int tile = group * 4 + wave;
   if ((int64_t)tile * 16 >= G) return;
-  if (c.dtype == 0)
-    bread_tile_chunk<P>(td, ct, static_cast<const int32_t*>(c.ptr), c.ld, G, beta, theta, tile,
-                        chunk, part);
-  else if (c.dtype == 1)
-    bread_tile_chunk<P>(td, ct, static_cast<const float*>(c.ptr), c.ld, G, beta, theta, tile,
-                        chunk, part);
-  else
-    bread_tile_chunk<P>(td, ct, static_cast<const double*>(c.ptr), c.ld, G, beta, theta, tile,
-                        chunk, part);
+  with_counts(c, [&](auto Y, auto vec) {
+    typedef std::remove_cv_t<std::remove_pointer_t<decltype(Y)>> CT;
+    bread_tile_chunk<P, CT, decltype(vec)::value>(td, ct, Y, c.ld, G, beta, theta, tile, chunk,
+                                                  part);
+  });
 }
 
 // One wavefront per gene.  C = number of clusters (0: no sandwich).
@@ -340,7 +335,7 @@ __global__ __launch_bounds__(64) void k_bread_finish(TileDesign td, ChunkTable c
 // ---------------------------------------------------------------------------
 // ZM: 0 no projection (alpha only, rough start), 1 least squares (QR),
 // 2 univariate.
-template <typename CT, int ZM>
+template <typename CT, int ZM, bool VEC>
 __device__ __forceinline__ void init_tile_chunk(const TileDesign& td, InitTile it, ChunkTable ct,
                                                 const CT* __restrict__ Y, int64_t ld, int64_t G,
                                                 int tile, int chunk, double* __restrict__ part) {
@@ -350,7 +345,6 @@ __device__ __forceinline__ void init_tile_chunk(const TileDesign& td, InitTile i
   const int64_t g = g0 < G ? g0 : (int64_t)tile * 16;
   const CT* yrow = Y + g * ld;
   const CT* yh = yrow + 4 * h;
-  const bool vec = (ld & 3) == 0 && ((reinterpret_cast<uintptr_t>(Y) & 15) == 0);
   const int64_t c0 = ct.c0[chunk], c1 = ct.c1[chunk];
   const int64_t b0 = c0 / 16, b1 = c0 < c1 ? (c1 + 15) / 16 : b0;
   constexpr bool kInt = sizeof(CT) == 4 && CT(0.5) == CT(0);  // integer counts
@@ -362,13 +356,19 @@ __device__ __forceinline__ void init_tile_chunk(const TileDesign& td, InitTile i
     d4 zb;   // projection columns of the block (B operand)
     d4 aux;  // exp(-offset) (QR) or offset (univariate) of the lane's cells
   };
-  auto load = [&](int64_t b, Blk& o) {
-    const int64_t cb = b * 16 + 4 * h;
+  auto load_tables = [&](int64_t b, Blk& o) {
     if constexpr (ZM != 0) {
-      o.zb = *reinterpret_cast<const d4*>(it.Zp + (b * 64 + lane) * 4);
-      o.aux = *reinterpret_cast<const d4*>((ZM == 1 ? it.einv : it.logsf) + cb);
+      o.zb = reinterpret_cast<const d4*>(it.Zp + b * 256)[lane];
+      o.aux = reinterpret_cast<const d4*>((ZM == 1 ? it.einv : it.logsf) + b * 16)[h];
     }
-    load_block_y<CT>(td, yrow, yh, vec, b, h, o.y);
+  };
+  auto load = [&](int64_t b, Blk& o) {
+    load_tables(b, o);
+    load4_fast<CT, VEC>(yh + b * 16, o.y);
+  };
+  auto load_tail = [&](int64_t b, Blk& o) {
+    load_tables(b, o);
+    load4_tail<CT>(yrow, b * 16 + 4 * h, td.n, o.y);
   };
   auto body = [&](const Blk& o, int64_t b) {
     const int64_t cb = b * 16 + 4 * h;
@@ -398,7 +398,7 @@ __device__ __forceinline__ void init_tile_chunk(const TileDesign& td, InitTile i
       for (int r = 0; r < 4; ++r) az = mfma16x16x4(z[r], o.zb[r], az);
     }
   };
-  pipeline<Blk>(b0, b1, load, body);
+  pipeline<Blk>(b0, min(b1, td.n / 16), b1, load, load_tail, body);
 #pragma unroll
   for (int o = 16; o <= 32; o <<= 1) {
     s1i += __shfl_xor(s1i, o, 64);
@@ -428,15 +428,10 @@ __global__ __launch_bounds__(256) void k_init_accum(TileDesign td, InitTile it, 
   tile_coords(ct.nchunk, chunk, group);
   const int tile = group * 4 + wave;
   if ((int64_t)tile * 16 >= G) return;
-  if (c.dtype == 0)
-    init_tile_chunk<int32_t, ZM>(td, it, ct, static_cast<const int32_t*>(c.ptr), c.ld, G, tile,
-                                 chunk, part);
-  else if (c.dtype == 1)
-    init_tile_chunk<float, ZM>(td, it, ct, static_cast<const float*>(c.ptr), c.ld, G, tile,
-                               chunk, part);
-  else
-    init_tile_chunk<double, ZM>(td, it, ct, static_cast<const double*>(c.ptr), c.ld, G, tile,
-                                chunk, part);
+  with_counts(c, [&](auto Y, auto vec) {
+    typedef std::remove_cv_t<std::remove_pointer_t<decltype(Y)>> CT;
+    init_tile_chunk<CT, ZM, decltype(vec)::value>(td, it, ct, Y, c.ld, G, tile, chunk, part);
+  });
 }
 
 // n * s2 - s1^2 exactly (128-bit), then one rounding

@@ -40,7 +40,7 @@ namespace dv {
 
 // lane l of a wave: gene slot s = l % 16, cell group h = l / 16; its cells in
 // block b are 16 b + 4 h + r.
-template <int P, typename CT>
+template <int P, typename CT, bool VEC>
 __device__ __forceinline__ void irls_tile_chunk(const TileDesign& td, const CT* __restrict__ Y,
                                                 int64_t ld, const int32_t* __restrict__ active,
                                                 int nA, const double* __restrict__ beta,
@@ -64,7 +64,6 @@ __device__ __forceinline__ void irls_tile_chunk(const TileDesign& td, const CT* 
   const double k = live ? kvec[g] : 0.0;
   const CT* yrow = Y + (int64_t)g * ld;
   const CT* yh = yrow + 4 * h;
-  const bool vec = (ld & 3) == 0 && ((reinterpret_cast<uintptr_t>(Y) & 15) == 0);
   d4 acc[NTMAX];
 #pragma unroll
   for (int t = 0; t < NTMAX; ++t) acc[t] = d4{0.0, 0.0, 0.0, 0.0};
@@ -93,9 +92,12 @@ __device__ __forceinline__ void irls_tile_chunk(const TileDesign& td, const CT* 
     }
   };
   auto load = [&](int64_t b, Blk& o) {
-    load_tile_block<NJ, NTMAX, CT>(td, yrow, yh, vec, b, lane, h, NT, td.offp, o);
+    load_tile_block<NJ, NTMAX, CT, VEC>(td, yh, b, lane, h, NT, td.offp, o);
   };
-  pipeline<Blk>(b0, b1, load, body);  // trailing chunks may be empty
+  auto load_tail = [&](int64_t b, Blk& o) {
+    load_tile_block_tail<NJ, NTMAX, CT>(td, yrow, b, lane, h, NT, td.offp, o);
+  };
+  pipeline<Blk>(b0, min(b1, td.n / 16), b1, load, load_tail, body);  // trailing chunks may be empty
   // partial sums: register r holds gene slot h + 4 r, column 16 t + s
   double* out = part + ((int64_t)tile * td.nchunk + chunk) * 16 * td.ncp;
 #pragma unroll
@@ -132,9 +134,8 @@ struct BlockOpsF {
 };
 
 template <int NJ, int NTMAX, typename CT>
-__device__ __forceinline__ void load_block_f(const TileDesign& td, const CT* yrow, const CT* yh,
-                                             bool vec, int64_t b, int lane, int h, int NT,
-                                             BlockOpsF<NJ, NTMAX, CT>& o) {
+__device__ __forceinline__ void load_block_f(const TileDesign& td, int64_t b, int lane, int h,
+                                             int NT, BlockOpsF<NJ, NTMAX, CT>& o) {
   const float* xs = td.Xf + b * (NJ * 64);
 #pragma unroll
   for (int j = 0; j < NJ; ++j) o.xa[j] = xs[j * 64 + lane];
@@ -145,10 +146,9 @@ __device__ __forceinline__ void load_block_f(const TileDesign& td, const CT* yro
     const int tt = t < NT ? t : NT - 1;
     o.kb[t] = ks[tt * 64 + lane];
   }
-  load_block_y<CT>(td, yrow, yh, vec, b, h, o.y);
 }
 
-template <int P, typename CT>
+template <int P, typename CT, bool VEC>
 __device__ __forceinline__ void irls_tile_chunk_mixed(const TileDesign& td,
                                                       const CT* __restrict__ Y, int64_t ld,
                                                       const int32_t* __restrict__ active, int nA,
@@ -178,7 +178,6 @@ __device__ __forceinline__ void irls_tile_chunk_mixed(const TileDesign& td,
   const float k = live ? (float)kvec[g] : 0.0f;  // k is a float in the reference
   const CT* yrow = Y + (int64_t)g * ld;
   const CT* yh = yrow + 4 * h;
-  const bool vec = (ld & 3) == 0 && ((reinterpret_cast<uintptr_t>(Y) & 15) == 0);
   f4 acc[NTMAX];
   double acc64[NTMAX][4];
 #pragma unroll
@@ -227,9 +226,14 @@ __device__ __forceinline__ void irls_tile_chunk_mixed(const TileDesign& td,
     }
   };
   auto load = [&](int64_t b, Blk& o) {
-    load_block_f<NJ, NTMAX, CT>(td, yrow, yh, vec, b, lane, h, NT, o);
+    load_block_f<NJ, NTMAX, CT>(td, b, lane, h, NT, o);  // tables first
+    load4_fast<CT, VEC>(yh + b * 16, o.y);
   };
-  pipeline<Blk>(b0, b1, load, body);  // trailing chunks may be empty
+  auto load_tail = [&](int64_t b, Blk& o) {
+    load_block_f<NJ, NTMAX, CT>(td, b, lane, h, NT, o);
+    load4_tail<CT>(yrow, b * 16 + 4 * h, td.n, o.y);
+  };
+  pipeline<Blk>(b0, min(b1, td.n / 16), b1, load, load_tail, body);  // trailing chunks may be empty
   // register r holds gene slot 4 h + r, column 16 t + s
   // exp(-eta - off) overflows in FP64 where log2(mu) < -1024: the reference
   // then takes a NaN step, so such a gene's partials are made NaN and the
@@ -264,15 +268,11 @@ __global__ __launch_bounds__(256) void k_irls_accum_mixed(TileDesign td, Counts 
   tile_coords(td.nchunk, chunk, group);
   const int tile = group * 4 + wave;
   if (tile >= ntile) return;
-  if (c.dtype == 0)
-    irls_tile_chunk_mixed<P>(td, static_cast<const int32_t*>(c.ptr), c.ld, active, nA, beta,
-                             kvec, tile, chunk, part);
-  else if (c.dtype == 1)
-    irls_tile_chunk_mixed<P>(td, static_cast<const float*>(c.ptr), c.ld, active, nA, beta,
-                             kvec, tile, chunk, part);
-  else
-    irls_tile_chunk_mixed<P>(td, static_cast<const double*>(c.ptr), c.ld, active, nA, beta,
-                             kvec, tile, chunk, part);
+  with_counts(c, [&](auto Y, auto vec) {
+    typedef std::remove_cv_t<std::remove_pointer_t<decltype(Y)>> CT;
+    irls_tile_chunk_mixed<P, CT, decltype(vec)::value>(td, Y, c.ld, active, nA, beta, kvec, tile,
+                                                       chunk, part);
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -301,6 +301,19 @@ __device__ __forceinline__ f4 mfma_bf(bf8 a, bf8 b, f4 c) {
 // Operands of one 32-cell step for lane (s, h): counts (as stored) of cells
 // 4h + r and 16 + 4h + r, the eta A operands of both halves, the offsets
 // (times log2 e), the gradient-tile and pair-tile B operands.
+template <int NE, int NTH>
+struct BfTables {
+  bf8 xa[2 * NE];
+  f4 off0, off1;
+  f4 kg0, kg1;
+  bf8 k1[NTH], k2[NTH];
+};
+
+template <typename CT>
+struct BfCounts {
+  CT y[8];
+};
+
 template <int NE, int NTH, typename CT>
 struct BfStep {
   bf8 xa[2 * NE];
@@ -310,41 +323,32 @@ struct BfStep {
   CT y[8];
 };
 
-// Counts of one step for lane (s, h).  yh = the gene's row + 4 h.  Steps
-// wholly inside the row (all but the last) take the unchecked loads; the
-// branch is uniform.  Table addresses are a uniform per-step base plus the
-// lane's offset, so they need no 64-bit vector arithmetic.
-template <int NE, int NTH, typename CT>
-__device__ __forceinline__ void load_bf_y(const TileDesign& td, const CT* yrow, const CT* yh,
-                                          bool vec, int64_t st, int h, CT (&y)[8]) {
-  const int64_t cs = st * 32;
-  if (cs + 32 <= td.n) {
-    const CT* p = yh + cs;
-    if (vec) {
-      typedef typename Vec4Of<CT>::T V4;
-      const V4 a = *reinterpret_cast<const V4*>(p);
-      const V4 b = *reinterpret_cast<const V4*>(p + 16);
+// Counts of one step for lane (s, h): cells 4 h + r and 16 + 4 h + r.
+// yh = the gene's row + 4 h.  Table addresses are a uniform per-step base
+// plus the lane's offset, so they need no 64-bit vector arithmetic.
+template <typename CT, bool VEC>
+__device__ __forceinline__ void load_bf_y(const CT* yh, int64_t st, CT (&y)[8]) {
+  CT t0[4], t1[4];
+  load4_fast<CT, VEC>(yh + st * 32, t0);
+  load4_fast<CT, VEC>(yh + st * 32 + 16, t1);
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        y[r] = a[r];
-        y[4 + r] = b[r];
-      }
-    } else {
+  for (int r = 0; r < 4; ++r) {
+    y[r] = t0[r];
+    y[4 + r] = t1[r];
+  }
+}
+
+// The row's partial last step.
+template <typename CT>
+__device__ __forceinline__ void load_bf_y_tail(const TileDesign& td, const CT* yrow, int64_t st,
+                                               int h, CT (&y)[8]) {
+  CT t0[4], t1[4];
+  load4_tail<CT>(yrow, st * 32 + 4 * h, td.n, t0);
+  load4_tail<CT>(yrow, st * 32 + 16 + 4 * h, td.n, t1);
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        y[r] = p[r];
-        y[4 + r] = p[16 + r];
-      }
-    }
-  } else {
-    CT t0[4], t1[4];
-    load4_raw<CT>(yrow, cs + 4 * h, td.n, vec, t0);
-    load4_raw<CT>(yrow, cs + 16 + 4 * h, td.n, vec, t1);
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      y[r] = t0[r];
-      y[4 + r] = t1[r];
-    }
+  for (int r = 0; r < 4; ++r) {
+    y[r] = t0[r];
+    y[4 + r] = t1[r];
   }
 }
 
@@ -421,7 +425,7 @@ __device__ __forceinline__ void bf_residuals(const S& o, const CT (&y)[8], const
   }
 }
 
-template <int P, typename CT>
+template <int P, typename CT, bool VEC>
 __device__ __forceinline__ void irls_tile_chunk_bf(const TileDesign& td, const CT* __restrict__ Y,
                                                    int64_t ld, const int32_t* __restrict__ active,
                                                    int nA, const double* __restrict__ beta,
@@ -440,7 +444,6 @@ __device__ __forceinline__ void irls_tile_chunk_bf(const TileDesign& td, const C
   bf_beta_operand<P, NE>(beta, g, live, h, bb);
   const float k = live ? (float)kvec[g] : 0.0f;
   const CT* yrow = Y + (int64_t)g * ld;
-  const bool vec = (ld & 3) == 0 && ((reinterpret_cast<uintptr_t>(Y) & 15) == 0);
   // gradient tile: FP32 span sums flushed to FP64; pair tiles: FP32 over the
   // chunk (relative error ~1e-6 of a Hessian entry).  Each accumulator's
   // MFMAs are issued back to back (dependent accumulation runs without
@@ -456,7 +459,12 @@ __device__ __forceinline__ void irls_tile_chunk_bf(const TileDesign& td, const C
   auto load = [&](int64_t st, Step& o) {
     load_bf_eta<NE, CT>(td, st, lane, h, o);
     load_bf_k<NTH>(td, st, lane, nth, o);
-    load_bf_y<NE, NTH, CT>(td, yrow, yh, vec, st, h, o.y);
+    load_bf_y<CT, VEC>(yh, st, o.y);
+  };
+  auto load_tail = [&](int64_t st, Step& o) {
+    load_bf_eta<NE, CT>(td, st, lane, h, o);
+    load_bf_k<NTH>(td, st, lane, nth, o);
+    load_bf_y_tail<CT>(td, yrow, st, h, o.y);
   };
   int span = 0;
   auto step = [&](const Step& o, int64_t) {
@@ -489,7 +497,7 @@ __device__ __forceinline__ void irls_tile_chunk_bf(const TileDesign& td, const C
   };
   const int64_t s0 = (int64_t)chunk * td.chunk_blocks;
   const int64_t s1 = min((int64_t)(chunk + 1) * td.chunk_blocks, td.nblk);
-  pipeline<Step>(s0, s1, load, step);  // trailing chunks may be empty
+  pipeline<Step>(s0, min(s1, td.n / 32), s1, load, load_tail, step);  // trailing chunks may be empty
 #pragma unroll
   for (int r = 0; r < 4; ++r) ag64[r] += (double)ag0[r];
   // exp(-eta - off) overflows in FP64 where log2(mu) < -1024: the reference
@@ -525,210 +533,311 @@ __global__ __launch_bounds__(256) void k_irls_accum_bf16(TileDesign td, Counts c
   tile_coords(td.nchunk, chunk, group);
   const int tile = group * 4 + wave;
   if (tile >= ntile) return;
-  if (c.dtype == 0)
-    irls_tile_chunk_bf<P>(td, static_cast<const int32_t*>(c.ptr), c.ld, active, nA, beta, kvec,
-                          tile, chunk, part);
-  else if (c.dtype == 1)
-    irls_tile_chunk_bf<P>(td, static_cast<const float*>(c.ptr), c.ld, active, nA, beta, kvec,
-                          tile, chunk, part);
-  else
-    irls_tile_chunk_bf<P>(td, static_cast<const double*>(c.ptr), c.ld, active, nA, beta, kvec,
-                          tile, chunk, part);
+  with_counts(c, [&](auto Y, auto vec) {
+    typedef std::remove_cv_t<std::remove_pointer_t<decltype(Y)>> CT;
+    irls_tile_chunk_bf<P, CT, decltype(vec)::value>(td, Y, c.ld, active, nA, beta, kvec, tile,
+                                                    chunk, part);
+  });
 }
 
 // ---------------------------------------------------------------------------
-// Wave-specialised bf16 variant.  The SIMD overlaps MFMA and VALU work only
-// across wavefronts, so each workgroup pairs, on every SIMD, a producer wave
-// (counts, eta, exp / reciprocal, bf16 splits: VALU work plus the small eta
-// MFMAs) with a consumer wave (the gradient and pair-column contractions:
-// MFMA work) for the same 16-gene tile.  The producer of step i hands r, r1,
-// r2 to its consumer through a two-slot LDS ring; one workgroup barrier per
-// step (the consumer works on step i - 1 while the producer makes step i).
-// Waves 0-3 produce for tiles 0-3, waves 4-7 consume (wave w runs on SIMD
-// w % 4).  Results are identical to k_irls_accum_bf16 (same arithmetic).
+// Wave-specialised bf16 variant: the bf16 kernel's arithmetic with every
+// operand staged in LDS by global_load_lds_dwordx4 (no registers held by
+// loads in flight).  Loads of one wave complete in issue order, so a wave
+// that loads both the tables (mostly L2 hits) and the counts (HBM) cannot
+// keep counts in flight for longer than its table prefetch distance; here
+// the four compute waves (16-gene tiles) load only their counts, three
+// steps ahead into a private 4-slot ring, and read the tables of a step from
+// a shared 3-slot ring that wave 4 fills two steps ahead.  One workgroup
+// barrier per step: after barrier k the compute waves read table slot k % 3
+// while the loader overwrites the slot read in step k - 1.
 // ---------------------------------------------------------------------------
-struct WsSlot {
-  f4 r0[4][64];   // r of cells c(h, 0..3), per tile and lane
-  f4 r1f[4][64];  // r of cells c(h, 4..7)
-  bf8 b1[4][64];  // bf16 r1
-  bf8 b2[4][64];  // bf16 r2 = r - r1
+typedef __attribute__((address_space(3))) void* lds_ptr_t;
+
+template <int NE, int NTH>
+struct WsLayout {
+  static constexpr int kX = 0;                          // 2 NE x 1 KB: eta A operands
+  static constexpr int kG = 2 * NE * 1024;              // 2 x 1 KB: gradient tile
+  static constexpr int kH1 = kG + 2 * 1024;             // NTH x 1 KB: pair tiles (hi)
+  static constexpr int kH2 = kH1 + NTH * 1024;          // NTH x 1 KB: pair tiles (lo)
+  static constexpr int kOff = kH2 + NTH * 1024;         // 128 B: 32 offsets (x log2 e)
+  static constexpr int kSlot = kOff + 128;
+  static constexpr int kSlots = 3;
+  static constexpr int kLoads = 2 * NE + 2 + 2 * NTH + 1;  // loads per step
+  // per compute wave: 4 count slots of 2 x 1 KB (cells 4 h + r, 16 + 4 h + r)
+  static constexpr int kY = kSlots * kSlot;
+  static constexpr int kYSlot = 2048;
+  static constexpr int kYWave = 4 * kYSlot;
+  static constexpr int kBytes = kY + 4 * kYWave;
 };
 
-template <int NE, typename CT>
-struct BfEtaOps {  // producer operands of one step
-  CT y[8];
-  bf8 xa[2 * NE];
-  f4 off0, off1;
-};
+// s_waitcnt vmcnt(N) leaving expcnt and lgkmcnt unconstrained (gfx9 encoding).
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+}
 
-template <int NTH>
-struct BfKOps {  // consumer operands of one step
-  f4 kg0, kg1;
-  bf8 k1[NTH], k2[NTH];
-};
+// Workgroup barrier for waves with LDS-DMA loads in flight.  __syncthreads'
+// release fence would wait for all of them (they write LDS); instead: this
+// wave's LDS reads are complete (lgkmcnt(0)), the barrier, and an acquire
+// fence that keeps later LDS reads after it.  Completion of the DMA loads a
+// step needs is made explicit with wait_vmcnt by the wave that issued them.
+__device__ __forceinline__ void ws_barrier() {
+  __builtin_amdgcn_s_waitcnt(0xC07F);  // vmcnt 63, expcnt 7, lgkmcnt 0
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+__device__ __forceinline__ void lds_dma16(const void* g, unsigned char* l) {
+  __builtin_amdgcn_global_load_lds(g, (lds_ptr_t)l, 16, 0, 0);
+}
+
+template <int NE, int NTH>
+__device__ __forceinline__ void ws_loader(const TileDesign& td, int64_t s0, int64_t s1,
+                                          unsigned char* ring) {
+  typedef WsLayout<NE, NTH> Lay;
+  const int lane = threadIdx.x & 63;
+  const int nth = td.nt - 1;
+  const int64_t nsteps = s1 - s0;
+  auto issue = [&](int64_t st, int slot) {
+    unsigned char* b = ring + slot * Lay::kSlot;
+    const bf8* xs = reinterpret_cast<const bf8*>(td.xab) + st * (2 * NE * 64);
+#pragma unroll
+    for (int j = 0; j < 2 * NE; ++j) lds_dma16(xs + j * 64 + lane, b + Lay::kX + j * 1024);
+    const f4* ks = reinterpret_cast<const f4*>(td.kg) + st * 128;
+    lds_dma16(ks + lane, b + Lay::kG);
+    lds_dma16(ks + 64 + lane, b + Lay::kG + 1024);
+    const bf8* k1s = reinterpret_cast<const bf8*>(td.kh) + st * nth * 64;
+    const bf8* k2s = reinterpret_cast<const bf8*>(td.kh2) + st * nth * 64;
+#pragma unroll
+    for (int t = 0; t < NTH; ++t) {
+      // a fixed number of loads per step (the waits below count them):
+      // unused tiles load a used one again, to the same place
+      const int tt = t < nth ? t : (nth > 0 ? nth - 1 : 0);
+      lds_dma16(k1s + tt * 64 + lane, b + Lay::kH1 + tt * 1024);
+      lds_dma16(k2s + tt * 64 + lane, b + Lay::kH2 + tt * 1024);
+    }
+    if (lane < 8) lds_dma16(reinterpret_cast<const f4*>(td.offl + st * 32) + lane, b + Lay::kOff);
+  };
+  // steps s0 and s0 + 1 in flight; step s0 complete before barrier 0
+  if (nsteps > 0) issue(s0, 0);
+  if (nsteps > 1) {
+    issue(s0 + 1, 1);
+    wait_vmcnt<Lay::kLoads>();
+  } else {
+    wait_vmcnt<0>();
+  }
+  ws_barrier();
+  for (int64_t k = 0; k < nsteps; ++k) {
+    // step s0 + k + 2 into the slot read in step k - 1; step s0 + k + 1
+    // complete before the barrier that ends step k
+    if (k + 2 < nsteps) {
+      issue(s0 + k + 2, (int)((k + 2) % Lay::kSlots));
+      wait_vmcnt<Lay::kLoads>();
+    } else {
+      wait_vmcnt<0>();
+    }
+    ws_barrier();
+  }
+}
 
 template <int P, typename CT>
-__device__ __forceinline__ void irls_ws_producer(const TileDesign& td, const CT* __restrict__ Y,
-                                                 int64_t ld, const int32_t* __restrict__ active,
-                                                 int nA, const double* __restrict__ beta,
-                                                 const double* __restrict__ kvec, int tile,
-                                                 bool tile_live, int w, int64_t s0, int64_t s1,
-                                                 WsSlot* ring, float* tmin_out) {
+__device__ __forceinline__ void ws_compute(const TileDesign& td, const CT* __restrict__ Y,
+                                           int64_t ld, const int32_t* __restrict__ active,
+                                           int nA, const double* __restrict__ beta,
+                                           const double* __restrict__ kvec, int tile,
+                                           bool tile_live, int chunk, int64_t s0, int64_t s1,
+                                           unsigned char* ring, double* __restrict__ part) {
   constexpr int NE = (7 * P + 31) / 32;
   constexpr int NTH = ((P * (P + 1)) / 2 + 15) / 16;
+  constexpr int kSpan = 4;
+  typedef WsLayout<NE, NTH> Lay;
   const int lane = threadIdx.x & 63;
   const int s = lane & 15, h = lane >> 4;
   const int slot = tile * 16 + s;
   const bool live = tile_live && slot < nA;
   const int g = tile_live ? (live ? active[slot] : active[tile * 16]) : 0;
+  const int nth = td.nt - 1;
   bf8 bb[NE];
   bf_beta_operand<P, NE>(beta, g, live, h, bb);
   const float k = live ? (float)kvec[g] : 0.0f;
   const CT* yrow = Y + (int64_t)g * ld;
-  const bool vec = (ld & 3) == 0 && ((reinterpret_cast<uintptr_t>(Y) & 15) == 0);
-  float tmin = 0.0f;
-  typedef BfEtaOps<NE, CT> Ops;
   const CT* yh = yrow + 4 * h;
-  auto load = [&](int64_t st, Ops& o) {
-    load_bf_eta<NE, CT>(td, st, lane, h, o);
-    load_bf_y<NE, NTH, CT>(td, yrow, yh, vec, st, h, o.y);
-  };
-  // operands are loaded a step ahead; the residuals of step st go to ring
-  // slot (st - s0) & 1, read by the consumer after the barrier
-  auto body = [&](int64_t st, const Ops& cur, Ops& nxt) {
-    if (tile_live && st < s1) {
-      load(min(st + 1, s1 - 1), nxt);
-      float rr[8];
-      bf_residuals<NE, CT>(cur, cur.y, bb, k, tmin, rr);
-      bf8 r1, r2;
-#pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        r1[i] = (__bf16)rr[i];
-        r2[i] = (__bf16)(rr[i] - (float)r1[i]);
-      }
-      WsSlot& o = ring[(st - s0) & 1];
-      o.r0[w][lane] = f4{rr[0], rr[1], rr[2], rr[3]};
-      o.r1f[w][lane] = f4{rr[4], rr[5], rr[6], rr[7]};
-      o.b1[w][lane] = r1;
-      o.b2[w][lane] = r2;
-    }
-    __syncthreads();
-  };
-  Ops A, B;
-  if (tile_live && s0 < s1) load(s0, A);
-  for (int64_t st = s0; st <= s1; st += 2) {
-    body(st, A, B);
-    if (st + 1 > s1) break;
-    body(st + 1, B, A);
-  }
-  tmin = fminf(tmin, __shfl_xor(tmin, 16, 64));
-  tmin = fminf(tmin, __shfl_xor(tmin, 32, 64));
-  if (tile_live && h == 0) tmin_out[w * 16 + s] = tmin;
-  __syncthreads();
-}
-
-template <int P>
-__device__ __forceinline__ void irls_ws_consumer(const TileDesign& td, int tile, bool tile_live,
-                                                 int w, int chunk, int64_t s0, int64_t s1,
-                                                 WsSlot* ring, const float* tmin_in,
-                                                 double* __restrict__ part) {
-  constexpr int NTH = ((P * (P + 1)) / 2 + 15) / 16;
-  constexpr int kSpan = 4;
-  const int lane = threadIdx.x & 63;
-  const int s = lane & 15, h = lane >> 4;
-  const int nth = td.nt - 1;
-  f4 ag = f4{0.f, 0.f, 0.f, 0.f};
+  const int64_t nsteps = s1 - s0;
+  f4 ag0 = f4{0.f, 0.f, 0.f, 0.f};
   double ag64[4] = {0.0, 0.0, 0.0, 0.0};
   f4 ah[NTH];
 #pragma unroll
   for (int t = 0; t < NTH; ++t) ah[t] = f4{0.f, 0.f, 0.f, 0.f};
+  float tmin = 0.0f;
   int span = 0;
-  typedef BfKOps<NTH> Ops;
-  // iteration st loads the operands of step st and contracts step st - 1
-  // (its residuals were written to the ring before the last barrier)
-  auto body = [&](int64_t st, const Ops& cur, Ops& nxt) {
-    if (tile_live && s0 < s1) {  // trailing chunks may be empty
-      load_bf_k<NTH>(td, min(st, s1 - 1), lane, nth, nxt);
-      if (st > s0) {
-        const WsSlot& o = ring[(st - 1 - s0) & 1];
-        const f4 ra = o.r0[w][lane], rb = o.r1f[w][lane];
-        const bf8 r1 = o.b1[w][lane], r2 = o.b2[w][lane];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) ag = mfma32(ra[j], cur.kg0[j], ag);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) ag = mfma32(rb[j], cur.kg1[j], ag);
-#pragma unroll
-        for (int t = 0; t < NTH; ++t)
-          if (t < nth) {
-            ah[t] = mfma_bf(r2, cur.k1[t], ah[t]);
-            ah[t] = mfma_bf(r1, cur.k2[t], ah[t]);
-            ah[t] = mfma_bf(r1, cur.k1[t], ah[t]);
-          }
-        if (++span == kSpan) {
-          span = 0;
-#pragma unroll
-          for (int r = 0; r < 4; ++r) ag64[r] += (double)ag[r];
-          ag = f4{0.f, 0.f, 0.f, 0.f};
-        }
-      }
-    }
-    __syncthreads();
+  typedef BfCounts<CT> Cnt;
+  unsigned char* yring = ring + Lay::kY + (threadIdx.x >> 6) * Lay::kYWave;
+  // counts of step st into count slot q (lane l's 16 bytes at l * 16)
+  auto issuey = [&](int64_t st, int q) {
+    lds_dma16(yh + st * 32, yring + q * Lay::kYSlot);
+    lds_dma16(yh + st * 32 + 16, yring + q * Lay::kYSlot + 1024);
   };
-  Ops A, B;
-  for (int64_t st = s0; st <= s1; st += 2) {
-    body(st, B, A);
-    if (st + 1 > s1) break;
-    body(st + 1, A, B);
-  }
+  auto ready = [&](int q, Cnt& o) {
+    typedef typename Vec4Of<CT>::T V4;
+    const V4 a = reinterpret_cast<const V4*>(yring + q * Lay::kYSlot)[lane];
+    const V4 b = reinterpret_cast<const V4*>(yring + q * Lay::kYSlot + 1024)[lane];
 #pragma unroll
-  for (int r = 0; r < 4; ++r) ag64[r] += (double)ag[r];
-  __syncthreads();  // the producers' tmin is in LDS
+    for (int r = 0; r < 4; ++r) {
+      o.y[r] = a[r];
+      o.y[4 + r] = b[r];
+    }
+  };
+  // step s0 + k: tables from ring slot k % 3, counts from the register ring
+  auto step = [&](int64_t kk, const Cnt& yc) {
+    const unsigned char* b = ring + (int)(kk % Lay::kSlots) * Lay::kSlot;
+    BfTables<NE, NTH> o;
+    const bf8* lx = reinterpret_cast<const bf8*>(b + Lay::kX);
+#pragma unroll
+    for (int j = 0; j < 2 * NE; ++j) o.xa[j] = lx[j * 64 + lane];
+    const f4* lo = reinterpret_cast<const f4*>(b + Lay::kOff);
+    o.off0 = lo[h];
+    o.off1 = lo[4 + h];
+    const f4* lg = reinterpret_cast<const f4*>(b + Lay::kG);
+    o.kg0 = lg[lane];
+    o.kg1 = lg[64 + lane];
+    const bf8* l1 = reinterpret_cast<const bf8*>(b + Lay::kH1);
+    const bf8* l2 = reinterpret_cast<const bf8*>(b + Lay::kH2);
+#pragma unroll
+    for (int t = 0; t < NTH; ++t)
+      if (t < nth) {
+        o.k1[t] = l1[t * 64 + lane];
+        o.k2[t] = l2[t * 64 + lane];
+      }
+    float rr[8];
+    bf_residuals<NE, CT>(o, yc.y, bb, k, tmin, rr);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) ag0 = mfma32(rr[j], o.kg0[j], ag0);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) ag0 = mfma32(rr[4 + j], o.kg1[j], ag0);
+    bf8 r1, r2;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      r1[i] = (__bf16)rr[i];
+      r2[i] = (__bf16)(rr[i] - (float)r1[i]);
+    }
+#pragma unroll
+    for (int t = 0; t < NTH; ++t)
+      if (t < nth) {
+        ah[t] = mfma_bf(r2, o.k1[t], ah[t]);
+        ah[t] = mfma_bf(r1, o.k2[t], ah[t]);
+        ah[t] = mfma_bf(r1, o.k1[t], ah[t]);
+      }
+    if (++span == kSpan) {
+      span = 0;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) ag64[r] += (double)ag0[r];
+      ag0 = f4{0.f, 0.f, 0.f, 0.f};
+    }
+  };
+  // Steps [s0, s1f) lie inside the rows.  Their counts are in flight three
+  // steps ahead: s0 .. s0 + 2 before barrier 0, then in step k the load of
+  // step s0 + k + 3 (clamped to s1f - 1) into count slot (k + 3) % 4, whose
+  // step k - 1 was read.  This wave's vector-memory instructions are these
+  // loads alone (two per step), so vmcnt(6) after issuing step k + 3 leaves
+  // steps k + 1 .. k + 3 in flight and step k in LDS.  A row's partial last
+  // step comes after them and loads its counts when it runs.
+  const int64_t s1f = max(s0, min(s1, td.n / 32));
+  const int64_t nf = s1f - s0;
+  if (tile_live && nf > 0) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) issuey(min(s0 + d, s1f - 1), d);
+  }
+  ws_barrier();
+  int64_t kk = 0;
+  for (; kk + 4 <= nf; kk += 4) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (tile_live) {
+        issuey(min(s0 + kk + u + 3, s1f - 1), (u + 3) & 3);
+        wait_vmcnt<6>();
+        Cnt yc;
+        ready(u, yc);
+        step(kk + u, yc);
+      }
+      ws_barrier();
+    }
+  }
+  // at most three full steps left, in count slots 0 .. 2; every load of
+  // this wave completes here (none may land in LDS after the wave ends)
+  wait_vmcnt<0>();
+#pragma unroll
+  for (int u = 0; u < 3; ++u) {
+    if (kk + u < nf) {
+      if (tile_live) {
+        Cnt yc;
+        ready(u, yc);
+        step(kk + u, yc);
+      }
+      ws_barrier();
+    }
+  }
+  for (kk = nf; kk < nsteps; ++kk) {
+    if (tile_live) {
+      Cnt t;
+      load_bf_y_tail<CT>(td, yrow, s0 + kk, h, t.y);
+      step(kk, t);
+    }
+    ws_barrier();
+  }
   if (!tile_live) return;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) ag64[r] += (double)ag0[r];
+  tmin = fminf(tmin, __shfl_xor(tmin, 16, 64));
+  tmin = fminf(tmin, __shfl_xor(tmin, 32, 64));
+  bool poison[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) poison[r] = __shfl(tmin, 4 * h + r, 64) < -1024.0f;
   double* out = part + ((int64_t)tile * td.nchunk + chunk) * 16 * td.ncp;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    const bool poison = tmin_in[w * 16 + 4 * h + r] < -1024.0f;
     const double nanv = __builtin_nan("");
-    out[(4 * h + r) * td.ncp + s] = poison ? nanv : ag64[r];
+    out[(4 * h + r) * td.ncp + s] = poison[r] ? nanv : ag64[r];
 #pragma unroll
     for (int t = 0; t < NTH; ++t)
-      if (t < nth) out[(4 * h + r) * td.ncp + 16 * (t + 1) + s] = poison ? nanv : (double)ah[t][r];
+      if (t < nth) out[(4 * h + r) * td.ncp + 16 * (t + 1) + s] = poison[r] ? nanv : (double)ah[t][r];
   }
 }
 
+// Counts must be 4-byte elements in 16-byte aligned rows (the launcher falls
+// back to k_irls_accum_bf16 otherwise).
 template <int P>
-__global__ __launch_bounds__(512) void k_irls_accum_ws(TileDesign td, Counts c,
+__global__ __launch_bounds__(320) void k_irls_accum_ws(TileDesign td, Counts c,
                                                        const int32_t* __restrict__ active,
                                                        const int32_t* __restrict__ nactive,
                                                        const double* __restrict__ beta,
                                                        const double* __restrict__ kvec,
                                                        double* __restrict__ part) {
-  __shared__ WsSlot ring[2];
-  __shared__ float tmin_sh[64];
+  constexpr int NE = (7 * P + 31) / 32;
+  constexpr int NTH = ((P * (P + 1)) / 2 + 15) / 16;
+  typedef WsLayout<NE, NTH> Lay;
+  __shared__ __attribute__((aligned(16))) unsigned char ring[Lay::kBytes];
   const int nA = *nactive;
   const int ntile = (nA + 15) / 16;
   const int wave = threadIdx.x >> 6;
-  const int w = wave & 3;
   int chunk, group;
   tile_coords(td.nchunk, chunk, group);
   if (group * 4 >= ntile) return;  // the whole workgroup
-  const int tile = group * 4 + w;
-  const bool tile_live = tile < ntile;
   const int64_t s0 = (int64_t)chunk * td.chunk_blocks;
   const int64_t s1 = max(s0, min((int64_t)(chunk + 1) * td.chunk_blocks, td.nblk));
-  if (wave < 4) {
-    if (c.dtype == 0)
-      irls_ws_producer<P>(td, static_cast<const int32_t*>(c.ptr), c.ld, active, nA, beta, kvec,
-                          tile, tile_live, w, s0, s1, ring, tmin_sh);
-    else if (c.dtype == 1)
-      irls_ws_producer<P>(td, static_cast<const float*>(c.ptr), c.ld, active, nA, beta, kvec,
-                          tile, tile_live, w, s0, s1, ring, tmin_sh);
-    else
-      irls_ws_producer<P>(td, static_cast<const double*>(c.ptr), c.ld, active, nA, beta, kvec,
-                          tile, tile_live, w, s0, s1, ring, tmin_sh);
-  } else {
-    irls_ws_consumer<P>(td, tile, tile_live, w, chunk, s0, s1, ring, tmin_sh, part);
+  if (wave == 4) {
+    ws_loader<NE, NTH>(td, s0, s1, ring);
+    return;
   }
+  const int tile = group * 4 + wave;
+  const bool tile_live = tile < ntile;
+  if (c.dtype == 0)
+    ws_compute<P>(td, static_cast<const int32_t*>(c.ptr), c.ld, active, nA, beta, kvec, tile,
+                  tile_live, chunk, s0, s1, ring, part);
+  else
+    ws_compute<P>(td, static_cast<const float*>(c.ptr), c.ld, active, nA, beta, kvec, tile,
+                  tile_live, chunk, s0, s1, ring, part);
 }
 
 template <int P>
@@ -745,15 +854,11 @@ __global__ __launch_bounds__(256) void k_irls_accum(TileDesign td, Counts c,
   tile_coords(td.nchunk, chunk, group);
   const int tile = group * 4 + wave;
   if (tile >= ntile) return;
-  if (c.dtype == 0)
-    irls_tile_chunk<P>(td, static_cast<const int32_t*>(c.ptr), c.ld, active, nA, beta, kvec,
-                       tile, chunk, part);
-  else if (c.dtype == 1)
-    irls_tile_chunk<P>(td, static_cast<const float*>(c.ptr), c.ld, active, nA, beta, kvec,
-                       tile, chunk, part);
-  else
-    irls_tile_chunk<P>(td, static_cast<const double*>(c.ptr), c.ld, active, nA, beta, kvec,
-                       tile, chunk, part);
+  with_counts(c, [&](auto Y, auto vec) {
+    typedef std::remove_cv_t<std::remove_pointer_t<decltype(Y)>> CT;
+    irls_tile_chunk<P, CT, decltype(vec)::value>(td, Y, c.ld, active, nA, beta, kvec, tile,
+                                                 chunk, part);
+  });
 }
 
 // One wavefront per active gene: lanes sum the chunk partials of columns
@@ -916,8 +1021,10 @@ hipError_t launch_irls_accum_ws(hipStream_t s, const TileDesign& td, Counts c, i
   if (nA_bound <= 0) return hipSuccess;
   const int ntile = (nA_bound + 15) / 16;
   const int groups = (ntile + 3) / 4;
+  const bool vec4 = c.dtype != 2 && (c.ld & 3) == 0 && (reinterpret_cast<uintptr_t>(c.ptr) & 15) == 0;
+  if (!vec4) return launch_irls_accum_bf16(s, td, c, p, active, nactive, nA_bound, beta, kvec, part);
   const dim3 grid((unsigned)((int64_t)groups * td.nchunk));
-  DV_TILE_P(p, k_irls_accum_ws, grid, dim3(512), s, td, c, active, nactive, beta, kvec, part);
+  DV_TILE_P(p, k_irls_accum_ws, grid, dim3(320), s, td, c, active, nactive, beta, kvec, part);
   return hipGetLastError();
 }
 

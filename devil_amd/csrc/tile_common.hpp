@@ -4,6 +4,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "device_common.hpp"
 #include "kernels.hpp"
 
@@ -27,12 +29,9 @@ __device__ __forceinline__ void tile_coords(int nchunk, int& chunk, int& group) 
   }
 }
 
-// Four consecutive counts of a row in their storage type, for kernels that
-// load a block ahead of its use: converting right after the load would make
-// the wave wait for the load there, so the conversion is left to the use.
-// Cells at or past n read the row's cell n - 1 (a finite count of the same
-// gene); the operand tables of padded cells are zero, so such a cell adds
-// exactly zero to every product.
+// Counts are loaded in their storage type and converted where they are used:
+// converting right after the load would make the wave wait for the load
+// there, and a prefetch would hide nothing.
 template <typename CT>
 struct Vec4Of;
 template <>
@@ -42,75 +41,75 @@ struct Vec4Of<float> { typedef f4v T; };
 template <>
 struct Vec4Of<double> { typedef d4 T; };
 
-template <typename CT>
-__device__ __forceinline__ void load4_raw(const CT* row, int64_t c, int64_t n, bool vec,
-                                          CT (&v)[4]) {
-  if (c + 4 <= n) {
-    if (vec) {
-      const typename Vec4Of<CT>::T t = *reinterpret_cast<const typename Vec4Of<CT>::T*>(row + c);
+// Four consecutive counts at p, inside the row.  VEC: rows are 16-byte
+// aligned with a multiple of 4 elements, one vector load.
+template <typename CT, bool VEC>
+__device__ __forceinline__ void load4_fast(const CT* p, CT (&v)[4]) {
+  if constexpr (VEC) {
+    const typename Vec4Of<CT>::T t = *reinterpret_cast<const typename Vec4Of<CT>::T*>(p);
 #pragma unroll
-      for (int r = 0; r < 4; ++r) v[r] = t[r];
-    } else {
-#pragma unroll
-      for (int r = 0; r < 4; ++r) v[r] = row[c + r];
-    }
+    for (int r = 0; r < 4; ++r) v[r] = t[r];
   } else {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) v[r] = row[min(c + r, n - 1)];
+    for (int r = 0; r < 4; ++r) v[r] = p[r];
   }
 }
 
-// body(ops, i) for the steps i of [i0, i1), the operands of step i + 1
-// loading while step i computes, into two buffers in turn.  Loads return in
-// issue order, so the prefetch is kept free of what would make the wait for
-// the current operands also wait for it: it is unconditional (the last step
-// loads itself again), the buffers are never copied, and a loader issues its
-// count loads (from HBM) after its operand-table loads (mostly from L2).
-template <typename Ops, typename Load, typename Body>
-__device__ __forceinline__ void pipeline(int64_t i0, int64_t i1, Load&& load, Body&& body) {
-  if (i0 >= i1) return;
-  Ops A, B;
-  load(i0, A);
-  for (int64_t i = i0; i < i1; i += 2) {
-    load(min(i + 1, i1 - 1), B);
-    body(A, i);
-    if (i + 1 >= i1) break;
-    load(min(i + 2, i1 - 1), A);
-    body(B, i + 1);
-  }
-}
-
-// As pipeline, with the counts of step i + 2 in flight while step i
-// computes: the operand tables (read from L2) are loaded a step ahead and the
-// counts (read from HBM) two steps ahead, issued after the tables, so that
-// the wait for a step's tables never waits for counts issued later.  Steps
-// run in groups of 4 (four count buffers, two table buffers, fixed indices);
-// the last group's leftover steps load and run one at a time.
-template <typename T, typename Y, typename LoadT, typename LoadY, typename Body>
-__device__ __forceinline__ void pipeline_ty(int64_t i0, int64_t i1, LoadT&& loadt, LoadY&& loady,
-                                            Body&& body) {
-  if (i0 >= i1) return;
-  T ta[2];
-  Y ya[4];
-  int64_t i = i0;
-  if (i1 - i0 >= 4) {
-    loadt(i0, ta[0]);
-    loady(i0, ya[0]);
-    loady(i0 + 1, ya[1]);
-    const int64_t last = i1 - 1;
-    for (; i + 4 <= i1; i += 4) {
+// Counts of cells c .. c + 3 of a row of n cells, for the row's last block:
+// cells at or past n read cell n - 1 (a finite count of the same gene); the
+// operand tables of padded cells are zero, so such a cell adds exactly zero
+// to every product, and the passes that use counts directly mask them.
+template <typename CT>
+__device__ __forceinline__ void load4_tail(const CT* row, int64_t c, int64_t n, CT (&v)[4]) {
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        loadt(min(i + u + 1, last), ta[(u + 1) & 1]);
-        loady(min(i + u + 2, last), ya[(u + 2) & 3]);
-        body(ta[u & 1], ya[u], i + u);
-      }
+  for (int r = 0; r < 4; ++r) v[r] = row[min(c + r, n - 1)];
+}
+
+// (count type, VEC) dispatch of a tile pass on its counts.
+template <typename F>
+__device__ __forceinline__ void with_counts(const Counts& c, F&& f) {
+  const bool vec = (c.ld & 3) == 0 && (reinterpret_cast<uintptr_t>(c.ptr) & 15) == 0;
+  if (c.dtype == 0) {
+    const int32_t* y = static_cast<const int32_t*>(c.ptr);
+    if (vec) f(y, std::true_type{});
+    else f(y, std::false_type{});
+  } else if (c.dtype == 1) {
+    const float* y = static_cast<const float*>(c.ptr);
+    if (vec) f(y, std::true_type{});
+    else f(y, std::false_type{});
+  } else {
+    const double* y = static_cast<const double*>(c.ptr);
+    if (vec) f(y, std::true_type{});
+    else f(y, std::false_type{});
+  }
+}
+
+// body(ops, i) for the steps i of [i0, i1).  The steps of [i0, i1f) lie
+// inside every row: their operands are loaded a step ahead, into two buffers
+// in turn.  Loads return in issue order, so nothing may make the wait for
+// the current operands also wait for the prefetch: the prefetch is
+// unconditional (the last step loads itself again) and branch-free, the
+// buffers are never copied, and a loader issues its count loads (from HBM)
+// after its operand-table loads (mostly from L2).  Steps from i1f on (a
+// row's partial last step) load with load_tail when they run.
+template <typename Ops, typename Load, typename LoadTail, typename Body>
+__device__ __forceinline__ void pipeline(int64_t i0, int64_t i1f, int64_t i1, Load&& load,
+                                         LoadTail&& load_tail, Body&& body) {
+  if (i0 < i1f) {
+    Ops A, B;
+    load(i0, A);
+    for (int64_t i = i0; i < i1f; i += 2) {
+      load(min(i + 1, i1f - 1), B);
+      body(A, i);
+      if (i + 1 >= i1f) break;
+      load(min(i + 2, i1f - 1), A);
+      body(B, i + 1);
     }
   }
-  for (; i < i1; ++i) {
-    loadt(i, ta[0]);
-    loady(i, ya[0]);
-    body(ta[0], ya[0], i);
+  for (int64_t i = max(i0, i1f); i < i1; ++i) {
+    Ops T;
+    load_tail(i, T);
+    body(T, i);
   }
 }
 
@@ -125,34 +124,12 @@ struct TileBlock {
   d4 kb[NTMAX > 0 ? NTMAX : 1];
 };
 
-// Counts of 16-cell block b for lane (s, h); yh = the gene's row + 4 h.
-// Blocks wholly inside the row (all but the last) take the unchecked loads;
-// the branch is uniform.
-template <typename CT>
-__device__ __forceinline__ void load_block_y(const TileDesign& td, const CT* yrow, const CT* yh,
-                                             bool vec, int64_t b, int h, CT (&y)[4]) {
-  if (b * 16 + 16 <= td.n) {
-    const CT* p = yh + b * 16;
-    if (vec) {
-      const typename Vec4Of<CT>::T t = *reinterpret_cast<const typename Vec4Of<CT>::T*>(p);
-#pragma unroll
-      for (int r = 0; r < 4; ++r) y[r] = t[r];
-    } else {
-#pragma unroll
-      for (int r = 0; r < 4; ++r) y[r] = p[r];
-    }
-  } else {
-    load4_raw<CT>(yrow, b * 16 + 4 * h, td.n, vec, y);
-  }
-}
-
 // Table addresses are a uniform per-block base plus the lane's offset (no
 // 64-bit vector arithmetic).
 template <int NJ, int NTMAX, typename CT>
-__device__ __forceinline__ void load_tile_block(const TileDesign& td, const CT* yrow,
-                                                const CT* yh, bool vec, int64_t b, int lane,
-                                                int h, int NT, const double* __restrict__ offs,
-                                                TileBlock<NJ, NTMAX, CT>& o) {
+__device__ __forceinline__ void load_tile_tables(const TileDesign& td, int64_t b, int lane, int h,
+                                                 int NT, const double* __restrict__ offs,
+                                                 TileBlock<NJ, NTMAX, CT>& o) {
   const double* xs = td.Xp + b * (NJ * 64);
 #pragma unroll
   for (int j = 0; j < NJ; ++j) o.xa[j] = xs[j * 64 + lane];
@@ -165,7 +142,26 @@ __device__ __forceinline__ void load_tile_block(const TileDesign& td, const CT* 
       o.kb[t] = ks[tt * 64 + lane];
     }
   }
-  load_block_y<CT>(td, yrow, yh, vec, b, h, o.y);
+}
+
+// Block b inside the row; yh = the gene's row + 4 h.
+template <int NJ, int NTMAX, typename CT, bool VEC>
+__device__ __forceinline__ void load_tile_block(const TileDesign& td, const CT* yh, int64_t b,
+                                                int lane, int h, int NT,
+                                                const double* __restrict__ offs,
+                                                TileBlock<NJ, NTMAX, CT>& o) {
+  load_tile_tables<NJ, NTMAX, CT>(td, b, lane, h, NT, offs, o);
+  load4_fast<CT, VEC>(yh + b * 16, o.y);
+}
+
+// The row's partial last block.
+template <int NJ, int NTMAX, typename CT>
+__device__ __forceinline__ void load_tile_block_tail(const TileDesign& td, const CT* yrow,
+                                                     int64_t b, int lane, int h, int NT,
+                                                     const double* __restrict__ offs,
+                                                     TileBlock<NJ, NTMAX, CT>& o) {
+  load_tile_tables<NJ, NTMAX, CT>(td, b, lane, h, NT, offs, o);
+  load4_tail<CT>(yrow, b * 16 + 4 * h, td.n, o.y);
 }
 
 }  // namespace dv
