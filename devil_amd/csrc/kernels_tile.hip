@@ -554,6 +554,17 @@ __global__ __launch_bounds__(256) void k_irls_accum_bf16(TileDesign td, Counts c
 // ---------------------------------------------------------------------------
 typedef __attribute__((address_space(3))) void* lds_ptr_t;
 
+// Compute waves (16-gene tiles) per workgroup of the wave-specialised kernel
+// (one more wave loads the tables): as many as the LDS holds, at most 15
+// (a workgroup of 1024 threads).  Table ring: 3 slots of (2 NE + 2 + 2 NTH)
+// KB + 128 B; each compute wave: 4 count slots of 2 KB.
+__host__ __device__ constexpr int ws_waves(int p) {
+  const int ne = (7 * p + 31) / 32, nth = (p * (p + 1) / 2 + 15) / 16;
+  const int tables = 3 * ((2 * ne + 2 + 2 * nth) * 1024 + 128);
+  const int fit = (160 * 1024 - tables) / (4 * 2048);
+  return fit < 15 ? fit : 15;
+}
+
 template <int NE, int NTH>
 struct WsLayout {
   static constexpr int kX = 0;                          // 2 NE x 1 KB: eta A operands
@@ -568,7 +579,6 @@ struct WsLayout {
   static constexpr int kY = kSlots * kSlot;
   static constexpr int kYSlot = 2048;
   static constexpr int kYWave = 4 * kYSlot;
-  static constexpr int kBytes = kY + 4 * kYWave;
 };
 
 // s_waitcnt vmcnt(N) leaving expcnt and lgkmcnt unconstrained (gfx9 encoding).
@@ -808,7 +818,8 @@ __device__ __forceinline__ void ws_compute(const TileDesign& td, const CT* __res
 // Counts must be 4-byte elements in 16-byte aligned rows (the launcher falls
 // back to k_irls_accum_bf16 otherwise).
 template <int P>
-__global__ __launch_bounds__(320) void k_irls_accum_ws(TileDesign td, Counts c,
+__global__ __launch_bounds__(64 * (ws_waves(P) + 1)) void k_irls_accum_ws(TileDesign td,
+                                                                          Counts c,
                                                        const int32_t* __restrict__ active,
                                                        const int32_t* __restrict__ nactive,
                                                        const double* __restrict__ beta,
@@ -817,20 +828,21 @@ __global__ __launch_bounds__(320) void k_irls_accum_ws(TileDesign td, Counts c,
   constexpr int NE = (7 * P + 31) / 32;
   constexpr int NTH = ((P * (P + 1)) / 2 + 15) / 16;
   typedef WsLayout<NE, NTH> Lay;
-  __shared__ __attribute__((aligned(16))) unsigned char ring[Lay::kBytes];
+  constexpr int kWsWaves = ws_waves(P);
+  __shared__ __attribute__((aligned(16))) unsigned char ring[Lay::kY + kWsWaves * Lay::kYWave];
   const int nA = *nactive;
   const int ntile = (nA + 15) / 16;
   const int wave = threadIdx.x >> 6;
   int chunk, group;
   tile_coords(td.nchunk, chunk, group);
-  if (group * 4 >= ntile) return;  // the whole workgroup
+  if (group * kWsWaves >= ntile) return;  // the whole workgroup
   const int64_t s0 = (int64_t)chunk * td.chunk_blocks;
   const int64_t s1 = max(s0, min((int64_t)(chunk + 1) * td.chunk_blocks, td.nblk));
-  if (wave == 4) {
+  if (wave == kWsWaves) {
     ws_loader<NE, NTH>(td, s0, s1, ring);
     return;
   }
-  const int tile = group * 4 + wave;
+  const int tile = group * kWsWaves + wave;
   const bool tile_live = tile < ntile;
   if (c.dtype == 0)
     ws_compute<P>(td, static_cast<const int32_t*>(c.ptr), c.ld, active, nA, beta, kvec, tile,
@@ -1020,11 +1032,13 @@ hipError_t launch_irls_accum_ws(hipStream_t s, const TileDesign& td, Counts c, i
                                 const double* beta, const double* kvec, double* part) {
   if (nA_bound <= 0) return hipSuccess;
   const int ntile = (nA_bound + 15) / 16;
-  const int groups = (ntile + 3) / 4;
   const bool vec4 = c.dtype != 2 && (c.ld & 3) == 0 && (reinterpret_cast<uintptr_t>(c.ptr) & 15) == 0;
   if (!vec4) return launch_irls_accum_bf16(s, td, c, p, active, nactive, nA_bound, beta, kvec, part);
-  const dim3 grid((unsigned)((int64_t)groups * td.nchunk));
-  DV_TILE_P(p, k_irls_accum_ws, grid, dim3(320), s, td, c, active, nactive, beta, kvec, part);
+  const int nw = ws_waves(p);
+  const int wgroups = (ntile + nw - 1) / nw;
+  const dim3 grid((unsigned)((int64_t)wgroups * td.nchunk));
+  DV_TILE_P(p, k_irls_accum_ws, grid, dim3(64 * (nw + 1)), s, td, c, active, nactive, beta, kvec,
+            part);
   return hipGetLastError();
 }
 
