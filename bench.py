@@ -215,6 +215,8 @@ def main():
     ap.add_argument("--mixed-kernel", default=None, choices=["f32", "bf16", "bf16_ws"],
                     help="mixed IRLS kernel (default: the library's choice)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--chunk-cells", type=int, default=0,
+                    help="cells per chunk of the tile passes (0: autotuned)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -244,6 +246,9 @@ def main():
     ctx.set_timing(True)
     if args.mixed_kernel:
         ctx.set_mixed_kernel(args.mixed_kernel)
+    if args.chunk_cells:
+        from devil_amd import _lib
+        ctx.set_tuning(_lib.TUNE_CHUNK_CELLS, args.chunk_cells)
 
     def step(prec):
         fit = devil_amd.fit_devil(Yd, w.X, clusters=w.clusters, overdispersion=w.od,
@@ -292,7 +297,7 @@ def main():
     if kname == "irls_mixed" and info.get("mixed_kernel") == "bf16":
         kdesc = f"k_irls_accum_bf16<{w.p}> (bf16/FP32 MFMA, FP64 partials)"
     elif kname == "irls_mixed" and info.get("mixed_kernel") == "bf16_ws":
-        kdesc = f"k_irls_accum_ws<{w.p}> (bf16/FP32 MFMA, producer/consumer waves)"
+        kdesc = f"k_irls_accum_ws<{w.p}> (bf16/FP32 MFMA, LDS-staged operands)"
     elif kname == "irls_mixed":
         kdesc = f"k_irls_accum_mixed<{w.p}> (FP32 MFMA, FP64 partials)"
     else:

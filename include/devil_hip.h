@@ -200,9 +200,11 @@ enum { DEVIL_MIXED_F32 = 0,   /* eta and all contractions on v_mfma_f32_16x16x4f
                                  first genes once per (p, n, count type) and keeps
                                  the fastest for the context */
        DEVIL_MIXED_BF16_WS = 3 /* DEVIL_MIXED_BF16 arithmetic on wave-specialised
-                                  workgroups: producer wavefronts (eta, exp, residual)
-                                  hand residuals through LDS to consumer wavefronts
-                                  (the contractions) */ };
+                                  workgroups: a loader wavefront stages the design
+                                  tables in LDS, the compute wavefronts stage their
+                                  counts in LDS three steps ahead (direct-to-LDS
+                                  loads); 4-byte counts in 16-byte aligned rows,
+                                  else DEVIL_MIXED_BF16 runs */ };
 int devil_set_tuning(devil_ctx *ctx, int32_t key, int64_t value);
 
 /* Per-launch timing of the IRLS accumulate kernels (the streaming pass over
