@@ -970,7 +970,8 @@ uint64_t tune_key(int64_t n, int p, int dtype) {
 int autotune_mixed(devil_ctx* ctx, const DeviceDesign& dd, int64_t n, int p, dv::Counts c,
                    int64_t G, const double* beta, const double* kvec, int& mk, int64_t& cc) {
   hipStream_t s = ctx->stream;
-  const int64_t Gs = std::min<int64_t>(G, 1024);
+  // enough genes that every candidate fills the GPU as the full fit does
+  const int64_t Gs = std::min<int64_t>(G, 4096);
   std::vector<int> kinds;
   if (ctx->tune[DEVIL_TUNE_MIXED_KERNEL] == DEVIL_MIXED_AUTO)
     kinds = {kMixedF32, kMixedBf16, kMixedBf16Ws};
