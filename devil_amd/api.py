@@ -67,10 +67,10 @@ class _Job:
                  init_overdispersion=None, init_beta_rough: bool = False, offset: float = 0,
                  size_factors=None, max_iter: int = 200, tolerance: float = 1e-3,
                  precision: str = "fp64", init_mode: str | None = None):
-        X = np.asfortranarray(np.asarray(design_matrix, dtype=np.float64))
-        if X.ndim != 2:
+        Xa = np.asarray(design_matrix, dtype=np.float64)
+        if Xa.ndim != 2:
             raise ValueError("design_matrix must be a matrix")
-        n, p = X.shape
+        n, p = Xa.shape
         G = int(x.shape[0])
         if int(x.shape[1]) != n:
             raise ValueError("ncol(x) must equal nrow(design_matrix)")
@@ -110,7 +110,13 @@ class _Job:
             if not x.is_contiguous() and x.stride(1) != 1:
                 x = x.contiguous()
             t = lambda a, dt=torch.float64: torch.as_tensor(a, dtype=dt, device=dev)
-            self.hold = dict(x=x, Xd=t(X.T.copy().reshape(-1)), offd=t(offset_vector),
+            # X column-major on the device: a C-ordered X is uploaded as is and
+            # transposed there (no host copies of the n x p matrix)
+            if Xa.flags.f_contiguous:
+                Xd = t(Xa.ravel(order="F"))
+            else:
+                Xd = t(np.ascontiguousarray(Xa)).t().contiguous().reshape(-1)
+            self.hold = dict(x=x, Xd=Xd, offd=t(offset_vector),
                              endsd=None if ends is None else t(ends, torch.int32),
                              dispd=None if disp is None else t(disp))
             e = lambda *shape, dt=torch.float64: torch.empty(*shape, dtype=dt, device=dev)
@@ -126,6 +132,7 @@ class _Job:
                                   cluster_ends=ptr(h["endsd"]), disp_init=ptr(h["dispd"]),
                                   **common)
         else:
+            X = np.asfortranarray(Xa)
             Y = np.asarray(x)
             if not Y.flags.c_contiguous:
                 Y = np.ascontiguousarray(Y)
