@@ -967,6 +967,40 @@ uint64_t tune_key(int64_t n, int p, int dtype) {
   return ((uint64_t)lg << 16) | ((uint64_t)p << 8) | (uint64_t)dtype;
 }
 
+// 4-byte counts in 16-byte aligned rows (what the LDS-staged kernels load).
+bool counts_vec4(const dv::Counts& c) {
+  return c.dtype != DEVIL_COUNTS_F64 && (c.ld & 3) == 0 &&
+         (reinterpret_cast<uintptr_t>(c.ptr) & 15) == 0;
+}
+
+// Mixed-precision bread of genes [0, G) on the bf16 operand tables (no
+// clusters).  ok = false when some gene's FP32 element range was exceeded:
+// the caller then runs the FP64 bread for the problem.
+int bread_mixed(devil_ctx* ctx, const TileHost& tl, dv::Counts c, int64_t G, int p,
+                const double* beta, const double* theta, double* bread, int32_t* status,
+                bool& ok) {
+  hipStream_t s = ctx->stream;
+  const dv::TileDesign& tm = tl.tdm;
+  DevBuf genes, cnt, sing, part;
+  DV_ALLOC(genes, sizeof(int32_t) * G);
+  DV_ALLOC(sing, sizeof(int32_t) * G);
+  DV_ALLOC(cnt, sizeof(int32_t) * 2);
+  DV_ALLOC(part, sizeof(double) * (size_t)((G + 15) / 16) * tm.nchunk * 16 * tm.ncp);
+  DV_HIP(dv::launch_iota(s, genes.as<int32_t>(), (int)G));
+  DV_HIP(dv::launch_fill_i32(s, cnt.as<int32_t>(), 1, (int32_t)G));
+  DV_HIP(dv::launch_fill_i32(s, cnt.as<int32_t>() + 1, 1, 0));
+  DV_HIP(dv::launch_bread_mixed(s, tm, c, p, genes.as<int32_t>(), cnt.as<int32_t>(), G, beta,
+                                theta, part.as<double>(), bread, sing.as<int32_t>(),
+                                cnt.as<int32_t>() + 1));
+  int32_t nbad = 0;
+  DV_HIP(hipMemcpyAsync(&nbad, cnt.as<int32_t>() + 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  DV_HIP(hipStreamSynchronize(s));
+  ok = nbad == 0;
+  if (ok)
+    DV_HIP(dv::launch_merge_flags(s, status, sing.as<int32_t>(), G, DEVIL_GENE_SINGULAR));
+  return DEVIL_OK;
+}
+
 int autotune_mixed(devil_ctx* ctx, const DeviceDesign& dd, int64_t n, int p, dv::Counts c,
                    int64_t G, const double* beta, const double* kvec, int& mk, int64_t& cc) {
   hipStream_t s = ctx->stream;
@@ -1261,10 +1295,19 @@ int fit_one(devil_ctx* ctx, const devil_problem* pr, devil_outputs* out) {
     }
     tm.end(ev);
     ev = tm.begin(DEVIL_STAGE_BREAD_MEAT);
-    DV_HIP(dv::launch_bread_tiled(s, tl.td, chunks.ct, c, Gb, n, p, b0, th, C,
-                                  post.as<double>(), bread.as<double>() + g0 * p * p,
-                                  want_sw ? sandwich.as<double>() + g0 * p * p : nullptr,
-                                  status.as<int32_t>() + g0));
+    // mixed precision: the bread on the bf16 tables (FP32 element math, FP64
+    // sums and inverse) where there is no meat to build
+    bool bread_done = false;
+    if (mixed && bf16_layout(tl.mixed) && C == 0 && counts_vec4(c)) {
+      if ((rc = bread_mixed(ctx, tl, c, Gb, p, b0, th, bread.as<double>() + g0 * p * p,
+                            status.as<int32_t>() + g0, bread_done)))
+        return rc;
+    }
+    if (!bread_done)
+      DV_HIP(dv::launch_bread_tiled(s, tl.td, chunks.ct, c, Gb, n, p, b0, th, C,
+                                    post.as<double>(), bread.as<double>() + g0 * p * p,
+                                    want_sw ? sandwich.as<double>() + g0 * p * p : nullptr,
+                                    status.as<int32_t>() + g0));
     tm.end(ev);
   }
 

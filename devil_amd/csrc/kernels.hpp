@@ -147,6 +147,20 @@ hipError_t launch_irls_accum_ws(hipStream_t s, const TileDesign& td, Counts c, i
                                 const int32_t* active, const int32_t* nactive, int nA_bound,
                                 const double* beta, const double* kvec, double* part);
 
+// Mixed-precision bread (no clusters) on the bf16 operand tables td: the
+// wave-specialised kernel with bread weights, then H^-1 per gene.  Genes are
+// genes[0 .. *ngenes) (G of them, in order); 4-byte counts in 16-byte aligned
+// rows only (hipErrorInvalidValue otherwise).  *fallback (zeroed by the
+// caller) counts genes whose FP32 element range was exceeded: the caller then
+// runs the FP64 bread.  singular[g] = 1 where H was singular.
+hipError_t launch_bread_mixed(hipStream_t s, const TileDesign& td, Counts c, int p,
+                              const int32_t* genes, const int32_t* ngenes, int64_t G,
+                              const double* beta, const double* theta, double* part,
+                              double* bread, int32_t* singular, int32_t* fallback);
+// status[g] |= bit where flag[g] != 0.
+hipError_t launch_merge_flags(hipStream_t s, int32_t* status, const int32_t* flag, int64_t G,
+                              int32_t bit);
+
 hipError_t launch_restart_genes(hipStream_t s, const int32_t* list, int n, int p,
                                 const double* beta0, double* beta, int32_t* iters,
                                 int32_t* status);

@@ -425,6 +425,28 @@ __device__ __forceinline__ void bf_residuals(const S& o, const CT (&y)[8], const
   }
 }
 
+// Bread weights of the step's 8 cells, w = (y th + 1) mu / (1 + th mu)^2
+// (src/variance.cpp:22-47); tmax tracks the largest log2(mu) (FP32 exp2
+// overflows past 128 where FP64 does not).
+template <int NE, typename CT, typename S>
+__device__ __forceinline__ void bf_bread_weights(const S& o, const CT (&y)[8],
+                                                 const bf8 (&bb)[NE], float th, float& tmax,
+                                                 float (&ww)[8]) {
+  f4 eta0 = f4{0.f, 0.f, 0.f, 0.f}, eta1 = eta0;
+#pragma unroll
+  for (int j = 0; j < NE; ++j) eta0 = mfma_bf(o.xa[j], bb[j], eta0);
+#pragma unroll
+  for (int j = 0; j < NE; ++j) eta1 = mfma_bf(o.xa[NE + j], bb[j], eta1);
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const float t2 = (r < 4 ? eta0[r] : eta1[r - 4]) + (r < 4 ? o.off0[r] : o.off1[r - 4]);
+    tmax = fmaxf(tmax, t2);
+    const float mu = __builtin_amdgcn_exp2f(t2);
+    const float rd = __builtin_amdgcn_rcpf(fmaf(th, mu, 1.0f));
+    ww[r] = fmaf((float)y[r], th, 1.0f) * (mu * rd) * rd;
+  }
+}
+
 template <int P, typename CT, bool VEC>
 __device__ __forceinline__ void irls_tile_chunk_bf(const TileDesign& td, const CT* __restrict__ Y,
                                                    int64_t ld, const int32_t* __restrict__ active,
@@ -652,7 +674,10 @@ __device__ __forceinline__ void ws_loader(const TileDesign& td, int64_t s0, int6
   }
 }
 
-template <int P, typename CT>
+// MODE 0: IRLS accumulate (gpar = k, residuals r); MODE 1: bread (gpar =
+// theta, weights w): the same contractions of a per-cell scalar with the
+// design columns and the pair columns.
+template <int P, int MODE, typename CT>
 __device__ __forceinline__ void ws_compute(const TileDesign& td, const CT* __restrict__ Y,
                                            int64_t ld, const int32_t* __restrict__ active,
                                            int nA, const double* __restrict__ beta,
@@ -721,7 +746,10 @@ __device__ __forceinline__ void ws_compute(const TileDesign& td, const CT* __res
         o.k2[t] = l2[t * 64 + lane];
       }
     float rr[8];
-    bf_residuals<NE, CT>(o, yc.y, bb, k, tmin, rr);
+    if constexpr (MODE == 0)
+      bf_residuals<NE, CT>(o, yc.y, bb, k, tmin, rr);
+    else
+      bf_bread_weights<NE, CT>(o, yc.y, bb, k, tmin, rr);
 #pragma unroll
     for (int j = 0; j < 4; ++j) ag0 = mfma32(rr[j], o.kg0[j], ag0);
 #pragma unroll
@@ -799,11 +827,21 @@ __device__ __forceinline__ void ws_compute(const TileDesign& td, const CT* __res
   if (!tile_live) return;
 #pragma unroll
   for (int r = 0; r < 4; ++r) ag64[r] += (double)ag0[r];
-  tmin = fminf(tmin, __shfl_xor(tmin, 16, 64));
-  tmin = fminf(tmin, __shfl_xor(tmin, 32, 64));
+  // IRLS: tmin is the smallest log2(mu), poison below -1024 (FP64's
+  // exp(-eta - off) overflows there); bread: tmin holds the largest, poison
+  // from 126 on (FP32 mu overflows)
   bool poison[4];
+  if constexpr (MODE == 0) {
+    tmin = fminf(tmin, __shfl_xor(tmin, 16, 64));
+    tmin = fminf(tmin, __shfl_xor(tmin, 32, 64));
 #pragma unroll
-  for (int r = 0; r < 4; ++r) poison[r] = __shfl(tmin, 4 * h + r, 64) < -1024.0f;
+    for (int r = 0; r < 4; ++r) poison[r] = __shfl(tmin, 4 * h + r, 64) < -1024.0f;
+  } else {
+    tmin = fmaxf(tmin, __shfl_xor(tmin, 16, 64));
+    tmin = fmaxf(tmin, __shfl_xor(tmin, 32, 64));
+#pragma unroll
+    for (int r = 0; r < 4; ++r) poison[r] = !(__shfl(tmin, 4 * h + r, 64) < 126.0f);
+  }
   double* out = part + ((int64_t)tile * td.nchunk + chunk) * 16 * td.ncp;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
@@ -815,16 +853,13 @@ __device__ __forceinline__ void ws_compute(const TileDesign& td, const CT* __res
   }
 }
 
-// Counts must be 4-byte elements in 16-byte aligned rows (the launcher falls
-// back to k_irls_accum_bf16 otherwise).
-template <int P>
-__global__ __launch_bounds__(64 * (ws_waves(P) + 1)) void k_irls_accum_ws(TileDesign td,
-                                                                          Counts c,
-                                                       const int32_t* __restrict__ active,
-                                                       const int32_t* __restrict__ nactive,
-                                                       const double* __restrict__ beta,
-                                                       const double* __restrict__ kvec,
-                                                       double* __restrict__ part) {
+template <int P, int MODE>
+__device__ __forceinline__ void ws_body(const TileDesign& td, const Counts& c,
+                                        const int32_t* __restrict__ active,
+                                        const int32_t* __restrict__ nactive,
+                                        const double* __restrict__ beta,
+                                        const double* __restrict__ gpar,
+                                        double* __restrict__ part) {
   constexpr int NE = (7 * P + 31) / 32;
   constexpr int NTH = ((P * (P + 1)) / 2 + 15) / 16;
   typedef WsLayout<NE, NTH> Lay;
@@ -845,11 +880,88 @@ __global__ __launch_bounds__(64 * (ws_waves(P) + 1)) void k_irls_accum_ws(TileDe
   const int tile = group * kWsWaves + wave;
   const bool tile_live = tile < ntile;
   if (c.dtype == 0)
-    ws_compute<P>(td, static_cast<const int32_t*>(c.ptr), c.ld, active, nA, beta, kvec, tile,
-                  tile_live, chunk, s0, s1, ring, part);
+    ws_compute<P, MODE>(td, static_cast<const int32_t*>(c.ptr), c.ld, active, nA, beta, gpar,
+                        tile, tile_live, chunk, s0, s1, ring, part);
   else
-    ws_compute<P>(td, static_cast<const float*>(c.ptr), c.ld, active, nA, beta, kvec, tile,
-                  tile_live, chunk, s0, s1, ring, part);
+    ws_compute<P, MODE>(td, static_cast<const float*>(c.ptr), c.ld, active, nA, beta, gpar,
+                        tile, tile_live, chunk, s0, s1, ring, part);
+}
+
+// Counts must be 4-byte elements in 16-byte aligned rows (the launchers fall
+// back to the register kernels otherwise).
+template <int P>
+__global__ __launch_bounds__(64 * (ws_waves(P) + 1)) void k_irls_accum_ws(
+    TileDesign td, Counts c, const int32_t* __restrict__ active,
+    const int32_t* __restrict__ nactive, const double* __restrict__ beta,
+    const double* __restrict__ kvec, double* __restrict__ part) {
+  ws_body<P, 0>(td, c, active, nactive, beta, kvec, part);
+}
+
+// Mixed-precision bread partials: per (16-gene tile, chunk) the sums of
+// w x_a (design columns) and w K (pair columns) with FP32 element math,
+// bf16x3 pair products and FP64 partial sums (genes in active order).
+template <int P>
+__global__ __launch_bounds__(64 * (ws_waves(P) + 1)) void k_bread_accum_ws(
+    TileDesign td, Counts c, const int32_t* __restrict__ genes,
+    const int32_t* __restrict__ ngenes, const double* __restrict__ beta,
+    const double* __restrict__ theta, double* __restrict__ part) {
+  ws_body<P, 1>(td, c, genes, ngenes, beta, theta, part);
+}
+
+// bread = H^-1 with H[a][b] = the sum over chunks of the pair column of
+// (a, b) (src/variance.cpp:22-47); one 64-thread block per gene.  A NaN
+// partial (FP32 range exceeded) counts in *fallback: the caller then runs
+// the FP64 bread for the problem (singular[] is merged into the status only
+// when it does not).
+template <int P>
+__global__ __launch_bounds__(64) void k_bread_finish_tile(TileDesign td, int64_t G,
+                                                          const double* __restrict__ part,
+                                                          double* __restrict__ bread,
+                                                          int32_t* __restrict__ singular,
+                                                          int32_t* __restrict__ fallback) {
+  __shared__ double acc[DEVIL_NCP_MAX];
+  const int64_t g = blockIdx.x;
+  if (g >= G) return;
+  const int lane = threadIdx.x;
+  const int64_t tile = g >> 4, s = g & 15;
+  const double* src = part + tile * td.nchunk * 16 * td.ncp + s * td.ncp;
+  for (int col = lane; col < td.ncp; col += 64) {
+    double a = 0.0;
+    for (int ch = 0; ch < td.nchunk; ++ch) a += src[(int64_t)ch * 16 * td.ncp + col];
+    acc[col] = a;
+  }
+  __syncthreads();
+  if (lane != 0) return;
+  double Hm[P][P], Bi[P][P];
+  bool bad = false;
+#pragma unroll
+  for (int a = 0; a < P; ++a)
+#pragma unroll
+    for (int bb = a; bb < P; ++bb) {
+      const int pi = td.kidx_pair[tri<P>(a, bb)];
+      const double v = pi >= 0 ? acc[pi] : 0.0;
+      bad |= isnan(v);
+      Hm[a][bb] = v;
+      Hm[bb][a] = v;
+    }
+  if (bad) {
+    atomicAdd(fallback, 1);
+    return;
+  }
+  const int sing = lu_inverse<P>(Hm, Bi);
+  double* out = bread + g * P * P;
+#pragma unroll
+  for (int a = 0; a < P; ++a)
+#pragma unroll
+    for (int bb = 0; bb < P; ++bb) out[a + bb * P] = Bi[a][bb];
+  singular[g] = sing ? 1 : 0;
+}
+
+// status[g] |= bit where flag[g] is set
+__global__ void k_merge_flags(int32_t* __restrict__ status, const int32_t* __restrict__ flag,
+                              int64_t G, int32_t bit) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < G && flag[g]) status[g] |= bit;
 }
 
 template <int P>
@@ -1039,6 +1151,32 @@ hipError_t launch_irls_accum_ws(hipStream_t s, const TileDesign& td, Counts c, i
   const dim3 grid((unsigned)((int64_t)wgroups * td.nchunk));
   DV_TILE_P(p, k_irls_accum_ws, grid, dim3(64 * (nw + 1)), s, td, c, active, nactive, beta, kvec,
             part);
+  return hipGetLastError();
+}
+
+hipError_t launch_merge_flags(hipStream_t s, int32_t* status, const int32_t* flag, int64_t G,
+                              int32_t bit) {
+  if (G <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_merge_flags, dim3((unsigned)((G + 255) / 256)), dim3(256), 0, s, status,
+                     flag, G, bit);
+  return hipGetLastError();
+}
+
+hipError_t launch_bread_mixed(hipStream_t s, const TileDesign& td, Counts c, int p,
+                              const int32_t* genes, const int32_t* ngenes, int64_t G,
+                              const double* beta, const double* theta, double* part,
+                              double* bread, int32_t* singular, int32_t* fallback) {
+  if (G <= 0) return hipSuccess;
+  const bool vec4 = c.dtype != 2 && (c.ld & 3) == 0 && (reinterpret_cast<uintptr_t>(c.ptr) & 15) == 0;
+  if (!vec4) return hipErrorInvalidValue;
+  const int ntile = (int)((G + 15) / 16);
+  const int nw = ws_waves(p);
+  const int wgroups = (ntile + nw - 1) / nw;
+  const dim3 grid((unsigned)((int64_t)wgroups * td.nchunk));
+  DV_TILE_P(p, k_bread_accum_ws, grid, dim3(64 * (nw + 1)), s, td, c, genes, ngenes, beta, theta,
+            part);
+  DV_TILE_P(p, k_bread_finish_tile, dim3((unsigned)G), dim3(64), s, td, G, part, bread, singular,
+            fallback);
   return hipGetLastError();
 }
 

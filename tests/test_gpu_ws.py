@@ -6,7 +6,8 @@ that holds only a row's partial last step (n = 5128 with 1024-cell chunks),
 rows without a partial step (n = 6144), a partial step inside a longer chunk
 (n = 5000), 16-byte-misaligned rows (n = 5001: the launcher falls back to the
 bf16 kernel), workgroups with idle tiles (G not a multiple of the tiles per
-workgroup), float counts, and p = 4 and 8.
+workgroup), float counts, and p = 4 and 8.  The same kernel computes the
+mixed path's bread where there are no clusters (test_mixed_bread_matches_fp64).
 """
 import sys
 
@@ -70,3 +71,25 @@ def test_ws_equals_bf16(dv, config, G, n, dtype):
     assert np.array_equal(a["iterations"]["beta_iters"].cpu().numpy(),
                           b["iterations"]["beta_iters"].cpu().numpy())
     assert ia["replayed"] == ib["replayed"]
+
+
+def test_mixed_bread_matches_fp64(dv):
+    """Without clusters the mixed path computes the bread on the bf16 tables
+    (FP32 element math, bf16x3 pair products, FP64 sums and inverse): within
+    1e-5 of the FP64 bread relative to each gene's largest entry."""
+    from devil_amd import synth
+    w = synth.make_workload("C3", G=487, n=6144, seed=777)
+    Y = synth.counts_device(w, device="cuda")
+    ctx = dv.context(0)
+    ctx.set_tuning(dv._lib.TUNE_CHUNK_CELLS, 0)
+    ref = dv.fit_devil(Y, w.X, size_factors=w.sf, precision="fp64")
+    ctx.set_mixed_kernel("bf16_ws")
+    fit = dv.fit_devil(Y, w.X, size_factors=w.sf, precision="mixed")
+    x = fit["beta_sandwiches_null"].cpu().numpy()
+    y = ref["beta_sandwiches_null"].cpu().numpy()
+    scale = np.abs(y).reshape(len(y), -1).max(axis=1)
+    e = np.abs(x - y).reshape(len(y), -1).max(axis=1) / scale
+    say(f"  bread rel err max {e.max():.2e} median {np.median(e):.2e}")
+    assert e.max() < 1e-5
+    st = fit["status"].cpu().numpy() & 4  # DEVIL_GENE_SINGULAR
+    assert np.array_equal(st, ref["status"].cpu().numpy() & 4)
