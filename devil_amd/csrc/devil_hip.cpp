@@ -426,6 +426,17 @@ struct StageTimer {
 // Host construction of the MFMA operand layouts (dv::TileDesign).
 enum MixedKernel { kMixedNone = 0, kMixedF32 = 1, kMixedBf16 = 2, kMixedBf16Ws = 3 };
 
+// 4-byte counts in 16-byte aligned rows (what the LDS-staged kernels load).
+bool counts_vec4(const dv::Counts& c) {
+  return c.dtype != DEVIL_COUNTS_F64 && (c.ld & 3) == 0 &&
+         (reinterpret_cast<uintptr_t>(c.ptr) & 15) == 0;
+}
+
+// 16-gene tiles per workgroup of a round's accumulate kernel.
+int accum_tiles_per_wg(int kind, int p, const dv::Counts& c) {
+  return (kind == kMixedBf16Ws && counts_vec4(c)) ? dv::ws_tiles_per_workgroup(p) : 4;
+}
+
 int mixed_kernel(const devil_ctx* ctx) {
   switch (ctx->tune[DEVIL_TUNE_MIXED_KERNEL]) {
     case DEVIL_MIXED_BF16: return kMixedBf16;
@@ -537,6 +548,24 @@ int plan_columns(devil_ctx* ctx, const double* Xd, int64_t n, int p, bool lin16,
   return err;
 }
 
+// Chunks of about chunk_blocks blocks.  Chunk boundaries fall on multiples of
+// kChunkAlign blocks, where the mixed kernels' FP32 spans restart, so every
+// chunking sums the same FP32 spans and differs only in the order of FP64
+// sums.  A multiple of 8 chunks: with blocks dealt round-robin to the 8 XCDs
+// and chunk = blockIdx % nchunk, each chunk's operands then live in one XCD's
+// L2 (trailing chunks may come out empty).
+constexpr int64_t kChunkAlign = 4;
+
+void set_chunks(dv::TileDesign& td, int64_t chunk_blocks) {
+  auto align = [](int64_t b) { return kChunkAlign * ((b + kChunkAlign - 1) / kChunkAlign); };
+  td.chunk_blocks = align(std::max<int64_t>(1, chunk_blocks));
+  td.nchunk = (int)((td.nblk + td.chunk_blocks - 1) / td.chunk_blocks);
+  if (td.nchunk >= 8) {
+    td.nchunk = 8 * ((td.nchunk + 7) / 8);
+    td.chunk_blocks = align((td.nblk + td.nchunk - 1) / td.nchunk);
+  }
+}
+
 // Shape of a design with cells_per_block-cell blocks over the given columns.
 void shape_design(const ColPlan& cp, int64_t n, int p, int cells_per_block, dv::TileDesign& td,
                   int64_t chunk_cells = 2048) {
@@ -551,16 +580,9 @@ void shape_design(const ColPlan& cp, int64_t n, int p, int cells_per_block, dv::
   td.n = n;
   td.nblk = (n + cells_per_block - 1) / cells_per_block;
   const int64_t min_blocks = std::max<int64_t>(1, chunk_cells / cells_per_block);
-  td.chunk_blocks = std::max<int64_t>(min_blocks, (td.nblk + 63) / 64);
-  td.nchunk = (int)((td.nblk + td.chunk_blocks - 1) / td.chunk_blocks);
-  if (td.nchunk >= 8) {
-    // a multiple of 8 chunks: with blocks dealt round-robin to the 8 XCDs and
-    // chunk = blockIdx % nchunk, each chunk's operands then live in one
-    // XCD's L2 (trailing chunks may come out empty)
-    td.nchunk = 8 * ((td.nchunk + 7) / 8);
-    td.chunk_blocks = (td.nblk + td.nchunk - 1) / td.nchunk;
-  }
+  set_chunks(td, std::max<int64_t>(min_blocks, (td.nblk + 63) / 64));
 }
+
 
 // Operand tables of a fit's design on the device (built by kernels from the
 // device copies of X, offset and exp(offset)).
@@ -808,21 +830,48 @@ void collect_launches(devil_ctx* ctx) {
 constexpr double kMixedBand = 1e-3;
 constexpr int64_t kMixedMinCells = 4096;
 
+// Chunking of one round over nA active genes.  A round whose workgroups
+// (tiles_per_wg 16-gene tiles by nchunk chunks) cannot fill the GPU at the
+// design's chunking gets shorter chunks, so the few genes left in the last
+// rounds (and the FP64 refits) stream their rows with many waves rather than
+// one wave per 1/64 of the row.  Bounded by the partial buffer (part_cap
+// doubles) and by kMaxRoundChunks partials per gene for the update to sum.
+constexpr int64_t kRoundWorkgroups = 1024;
+constexpr int64_t kMaxRoundChunks = 512;
+constexpr int64_t kMinRoundBlocks = 8;
+
+dv::TileDesign round_shape(const dv::TileDesign& td, int nA, int tiles_per_wg,
+                           size_t part_cap) {
+  const int64_t ntile = (nA + 15) / 16;
+  const int64_t groups = (ntile + tiles_per_wg - 1) / tiles_per_wg;
+  if (ntile == 0 || groups * td.nchunk >= kRoundWorkgroups) return td;
+  const int64_t cap = (int64_t)(part_cap / ((size_t)ntile * 16 * td.ncp));
+  const int64_t want = std::min({(kRoundWorkgroups + groups - 1) / groups, kMaxRoundChunks, cap});
+  if (want <= td.nchunk) return td;
+  dv::TileDesign tr = td;
+  set_chunks(tr, std::max(kMinRoundBlocks, (td.nblk + want - 1) / want));
+  return (tr.nchunk > td.nchunk && tr.nchunk <= cap) ? tr : td;
+}
+
 // Rounds of accumulate + update over the active list act (count ncnt on the
 // device, nA on the host) until none is left.  With band > 0 the mixed kernel
-// runs and genes near the threshold are appended to replay / nreplay.
+// runs and genes near the threshold are appended to replay / nreplay.  part
+// holds part_cap doubles.
 int irls_rounds(devil_ctx* ctx, const TileHost& tl, bool mixed, dv::Counts c, int p,
                 double* beta, const double* kvec, int max_iter, double eps, int32_t* iters,
                 int32_t* status, int32_t* act, int32_t* act_spare, int32_t* ncnt,
-                int32_t* ncnt_spare, int nA, double* part, int32_t* replay, int32_t* nreplay) {
-  const dv::TileDesign& td = mixed ? tl.tdm : tl.td;
+                int32_t* ncnt_spare, int nA, double* part, size_t part_cap, int32_t* replay,
+                int32_t* nreplay) {
+  const dv::TileDesign& tfull = mixed ? tl.tdm : tl.td;
+  const int tiles_per_wg = accum_tiles_per_wg(mixed ? tl.mixed : kMixedNone, p, c);
   hipStream_t s = ctx->stream;
-  const double row_bytes = (double)count_elem_size(c.dtype) * (double)td.n;
+  const double row_bytes = (double)count_elem_size(c.dtype) * (double)tfull.n;
   int32_t* cur = act;
   int32_t* nxt = act_spare;
   int32_t* ncur = ncnt;
   int32_t* nnxt = ncnt_spare;
   for (int it = 0; it < max_iter && nA > 0; ++it) {
+    const dv::TileDesign td = round_shape(tfull, nA, tiles_per_wg, part_cap);
     {
       LaunchTimer lt(ctx, mixed ? DEVIL_KERNEL_IRLS_MIXED : DEVIL_KERNEL_IRLS_FP64,
                      row_bytes * nA);
@@ -866,7 +915,8 @@ int run_irls_tiled(devil_ctx* ctx, const TileHost& tl, dv::Counts c, int64_t G, 
   const int64_t ntile = (G + 15) / 16;
   size_t part_per_tile = (size_t)td.nchunk * 16 * td.ncp;
   if (mixed) part_per_tile = std::max(part_per_tile, (size_t)tdm.nchunk * 16 * tdm.ncp);
-  DV_ALLOC(part, sizeof(double) * (size_t)ntile * part_per_tile);
+  const size_t part_cap = (size_t)ntile * part_per_tile;
+  DV_ALLOC(part, sizeof(double) * part_cap);
   int32_t* c0 = cnt.as<int32_t>();
   DV_HIP(dv::launch_iota(s, act0.as<int32_t>(), (int)G));
   DV_HIP(dv::launch_fill_i32(s, c0, 1, (int32_t)G));
@@ -874,7 +924,7 @@ int run_irls_tiled(devil_ctx* ctx, const TileHost& tl, dv::Counts c, int64_t G, 
   if (!mixed)
     return irls_rounds(ctx, tl, false, c, p, beta, kvec, max_iter, eps, iters, status,
                        act0.as<int32_t>(), act1.as<int32_t>(), c0, c0 + 1, (int)G,
-                       part.as<double>(), nullptr, nullptr);
+                       part.as<double>(), part_cap, nullptr, nullptr);
   DV_ALLOC(rep, sizeof(int32_t) * G);
   DV_ALLOC(beta0, sizeof(double) * G * p);
   DV_HIP(hipMemcpyAsync(beta0.p, beta, sizeof(double) * G * p, hipMemcpyDeviceToDevice, s));
@@ -882,7 +932,7 @@ int run_irls_tiled(devil_ctx* ctx, const TileHost& tl, dv::Counts c, int64_t G, 
   DV_HIP(dv::launch_fill_i32(s, nrep, 1, 0));
   if ((rc = irls_rounds(ctx, tl, true, c, p, beta, kvec, max_iter, eps, iters, status,
                         act0.as<int32_t>(), act1.as<int32_t>(), c0, c0 + 1, (int)G,
-                        part.as<double>(), rep.as<int32_t>(), nrep)))
+                        part.as<double>(), part_cap, rep.as<int32_t>(), nrep)))
     return rc;
   int32_t nr = 0;
   DV_HIP(hipMemcpyAsync(&nr, nrep, sizeof(int32_t), hipMemcpyDeviceToHost, s));
@@ -894,7 +944,7 @@ int run_irls_tiled(devil_ctx* ctx, const TileHost& tl, dv::Counts c, int64_t G, 
                                   status));
   return irls_rounds(ctx, tl, false, c, p, beta, kvec, max_iter, eps, iters, status,
                      rep.as<int32_t>(), act1.as<int32_t>(), nrep, c0 + 3, nr,
-                     part.as<double>(), nullptr, nullptr);
+                     part.as<double>(), part_cap, nullptr, nullptr);
 }
 
 // MLE overdispersion over genes [0, G): setup pass, then rounds of
@@ -967,11 +1017,6 @@ uint64_t tune_key(int64_t n, int p, int dtype) {
   return ((uint64_t)lg << 16) | ((uint64_t)p << 8) | (uint64_t)dtype;
 }
 
-// 4-byte counts in 16-byte aligned rows (what the LDS-staged kernels load).
-bool counts_vec4(const dv::Counts& c) {
-  return c.dtype != DEVIL_COUNTS_F64 && (c.ld & 3) == 0 &&
-         (reinterpret_cast<uintptr_t>(c.ptr) & 15) == 0;
-}
 
 // Mixed-precision bread of genes [0, G) on the bf16 operand tables (no
 // clusters).  ok = false when some gene's FP32 element range was exceeded:

@@ -985,11 +985,15 @@ __global__ __launch_bounds__(256) void k_irls_accum(TileDesign td, Counts c,
   });
 }
 
-// One wavefront per active gene: lanes sum the chunk partials of columns
-// lane and lane + 64 (coalesced, in chunk order) into LDS; every lane then
-// runs the same p x p solve.
+// One workgroup per active gene.  Its threads sum the chunk partials in
+// kUpdSlices interleaved slices per column (two accumulators each, so a
+// round of up to 512 chunks keeps many loads in flight), the slices are
+// added in LDS, and one lane runs the p x p solve.
+constexpr int kUpdThreads = 256;
+constexpr int kUpdSlices = 16;
+
 template <int P>
-__global__ __launch_bounds__(64) void k_irls_update(TileDesign td,
+__global__ __launch_bounds__(kUpdThreads) void k_irls_update(TileDesign td,
                                                     const int32_t* __restrict__ active,
                                                     const int32_t* __restrict__ nactive,
                                                     const double* __restrict__ part,
@@ -1003,19 +1007,37 @@ __global__ __launch_bounds__(64) void k_irls_update(TileDesign td,
                                                     int32_t* __restrict__ recheck,
                                                     int32_t* __restrict__ nrecheck) {
   __shared__ double acc[DEVIL_NCP_MAX];
+  __shared__ double slice[kUpdSlices][DEVIL_NCP_MAX];
   const int nA = *nactive;
   const int i = blockIdx.x;
   if (i >= nA) return;
   const int lane = threadIdx.x;
   const int g = active[i];
   const int tile = i >> 4, s = i & 15;
-  const double* src = part + (int64_t)tile * td.nchunk * 16 * td.ncp + (int64_t)s * td.ncp;
-  for (int col = lane; col < td.ncp; col += 64) {
+  const int ncp = td.ncp;
+  const int64_t stride = (int64_t)16 * ncp;
+  const double* src = part + (int64_t)tile * td.nchunk * stride + (int64_t)s * ncp;
+  for (int idx = lane; idx < kUpdSlices * ncp; idx += kUpdThreads) {
+    const int col = idx % ncp, q = idx / ncp;
+    const double* sc = src + col;
+    double a0 = 0.0, a1 = 0.0;
+    int ch = q;
+    for (; ch + kUpdSlices < td.nchunk; ch += 2 * kUpdSlices) {
+      a0 += sc[(int64_t)ch * stride];
+      a1 += sc[(int64_t)(ch + kUpdSlices) * stride];
+    }
+    if (ch < td.nchunk) a0 += sc[(int64_t)ch * stride];
+    slice[q][col] = a0 + a1;
+  }
+  __syncthreads();
+  for (int col = lane; col < ncp; col += kUpdThreads) {
     double a = 0.0;
-    for (int ch = 0; ch < td.nchunk; ++ch) a += src[(int64_t)ch * 16 * td.ncp + col];
+#pragma unroll
+    for (int q = 0; q < kUpdSlices; ++q) a += slice[q][col];
     acc[col] = a;
   }
   __syncthreads();
+  if (lane >= 64) return;
   const double k = kvec[g];
   double M[P][P], v[P];
 #pragma unroll
@@ -1154,6 +1176,8 @@ hipError_t launch_irls_accum_ws(hipStream_t s, const TileDesign& td, Counts c, i
   return hipGetLastError();
 }
 
+int ws_tiles_per_workgroup(int p) { return ws_waves(p); }
+
 hipError_t launch_merge_flags(hipStream_t s, int32_t* status, const int32_t* flag, int64_t G,
                               int32_t bit) {
   if (G <= 0) return hipSuccess;
@@ -1188,7 +1212,7 @@ hipError_t launch_irls_update(hipStream_t s, const TileDesign& td, int p,
                               int32_t* nrecheck) {
   if (nA_bound <= 0) return hipSuccess;
   const dim3 grid((unsigned)nA_bound);
-  DV_TILE_P(p, k_irls_update, grid, dim3(64), s, td, active, nactive, part, beta, kvec,
+  DV_TILE_P(p, k_irls_update, grid, dim3(kUpdThreads), s, td, active, nactive, part, beta, kvec,
             max_iter, eps, iters, status, next, nnext, band, recheck, nrecheck);
   return hipGetLastError();
 }
