@@ -250,9 +250,14 @@ def main():
         from devil_amd import _lib
         ctx.set_tuning(_lib.TUNE_CHUNK_CELLS, args.chunk_cells)
 
+    # every input resident in HBM before timing: counts, the design matrix
+    # (column-major, as the library reads it) and the size factors
+    Xdev = torch.as_tensor(np.ascontiguousarray(w.X.T), device=dev).t()
+    sfdev = torch.as_tensor(np.asarray(w.sf, dtype=np.float64), device=dev)
+
     def step(prec):
-        fit = devil_amd.fit_devil(Yd, w.X, clusters=w.clusters, overdispersion=w.od,
-                                  size_factors=w.sf, precision=prec)
+        fit = devil_amd.fit_devil(Yd, Xdev, clusters=w.clusters, overdispersion=w.od,
+                                  size_factors=sfdev, precision=prec)
         de = devil_amd.test_de(fit, contrast)
         return fit, de
 

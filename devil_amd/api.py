@@ -4,7 +4,8 @@
 behaviour of the R functions (R/main.R:164-273, R/test.R:85-134), and run the
 fit through libdevil_hip.so on an MI355X.  Counts may be a numpy array (host;
 the library streams them to HBM in gene batches) or a torch tensor already on
-the GPU (nothing is copied).  There is no CPU fitting path.
+the GPU (nothing is copied); with GPU counts the design matrix and size factors
+may be GPU tensors too.  There is no CPU fitting path.
 """
 from __future__ import annotations
 
@@ -67,10 +68,20 @@ class _Job:
                  init_overdispersion=None, init_beta_rough: bool = False, offset: float = 0,
                  size_factors=None, max_iter: int = 200, tolerance: float = 1e-3,
                  precision: str = "fp64", init_mode: str | None = None):
-        Xa = np.asarray(design_matrix, dtype=np.float64)
-        if Xa.ndim != 2:
-            raise ValueError("design_matrix must be a matrix")
-        n, p = Xa.shape
+        self.on_device = _is_torch(x) and x.is_cuda
+        x_dev = _is_torch(design_matrix) and design_matrix.is_cuda and self.on_device
+        sf_dev = _is_torch(size_factors) and size_factors.is_cuda and self.on_device
+        if x_dev:
+            Xa = None
+            if design_matrix.dim() != 2:
+                raise ValueError("design_matrix must be a matrix")
+            n, p = (int(v) for v in design_matrix.shape)
+        else:
+            Xa = np.asarray(design_matrix.cpu() if _is_torch(design_matrix) else design_matrix,
+                            dtype=np.float64)
+            if Xa.ndim != 2:
+                raise ValueError("design_matrix must be a matrix")
+            n, p = Xa.shape
         G = int(x.shape[0])
         if int(x.shape[1]) != n:
             raise ValueError("ncol(x) must equal nrow(design_matrix)")
@@ -83,8 +94,17 @@ class _Job:
         if isinstance(size_factors, str):
             raise NotImplementedError("size-factor estimation is outside the fitted path; "
                                       "pass numeric size_factors")
-        sf = np.ones(n) if size_factors is None else np.asarray(size_factors, dtype=np.float64)
-        offset_vector = compute_offset_vector(offset, n, sf)
+        if sf_dev:
+            import torch
+            sf = size_factors.to(torch.float64).reshape(-1)
+            if sf.numel() != n:
+                raise ValueError("size_factors must have one entry per cell")
+            offset_vector = float(offset) + torch.log(sf)
+        else:
+            sf = np.ones(n) if size_factors is None else np.asarray(
+                size_factors.cpu() if _is_torch(size_factors) else size_factors,
+                dtype=np.float64)
+            offset_vector = compute_offset_vector(offset, n, sf)
         ends = None if clusters is None else cluster_blocks(clusters, n)
         if init_mode is None:
             init = L.INIT_ROUGH if init_beta_rough else L.INIT_QR
@@ -100,7 +120,6 @@ class _Job:
         self.meta = dict(size_factors=sf, offset_vector=offset_vector,
                          design_matrix=design_matrix, input_matrix=x,
                          input_parameters={"max_iter": max_iter, "tolerance": tolerance})
-        self.on_device = _is_torch(x) and x.is_cuda
         common = dict(G=G, n=n, p=p, n_clusters=0 if ends is None else len(ends),
                       beta_init=None, init_mode=init, od_mode=od, max_iter=int(max_iter),
                       tolerance=float(tolerance), precision=prec)
@@ -111,12 +130,16 @@ class _Job:
                 x = x.contiguous()
             t = lambda a, dt=torch.float64: torch.as_tensor(a, dtype=dt, device=dev)
             # X column-major on the device: a C-ordered X is uploaded as is and
-            # transposed there (no host copies of the n x p matrix)
-            if Xa.flags.f_contiguous:
+            # transposed there (no host copies of the n x p matrix); a GPU X is
+            # used in place when it is column-major already
+            if x_dev:
+                Xd = design_matrix.to(device=dev, dtype=torch.float64).t().contiguous().reshape(-1)
+            elif Xa.flags.f_contiguous:
                 Xd = t(Xa.ravel(order="F"))
             else:
                 Xd = t(np.ascontiguousarray(Xa)).t().contiguous().reshape(-1)
-            self.hold = dict(x=x, Xd=Xd, offd=t(offset_vector),
+            offd = (offset_vector.to(dev).contiguous() if sf_dev else t(offset_vector))
+            self.hold = dict(x=x, Xd=Xd, offd=offd,
                              endsd=None if ends is None else t(ends, torch.int32),
                              dispd=None if disp is None else t(disp))
             e = lambda *shape, dt=torch.float64: torch.empty(*shape, dtype=dt, device=dev)
@@ -232,11 +255,16 @@ def test_de(devil_fit, contrast, clusters=None, pval_adjust_method: str = "BH",
         raise NotImplementedError("only the BH adjustment is implemented")
     ctx = L.context(device)
     dev = torch.device("cuda", device)
-    f64 = lambda a: torch.as_tensor(np.asarray(a.cpu() if _is_torch(a) else a,
-                                               dtype=np.float64), device=dev)
+
+    def f64(a):
+        if _is_torch(a):
+            return a.to(device=dev, dtype=torch.float64)
+        return torch.as_tensor(np.asarray(a, dtype=np.float64), device=dev)
+
     beta = f64(devil_fit["beta"])
     G, p = beta.shape
-    n = int(np.asarray(devil_fit["design_matrix"]).shape[0])
+    n = int(devil_fit["design_matrix"].shape[0]) if hasattr(devil_fit["design_matrix"], "shape") \
+        else int(np.asarray(devil_fit["design_matrix"]).shape[0])
     bcol = beta.T.contiguous()  # column-major G x p
     bread = f64(devil_fit["beta_sandwiches_null"]).transpose(1, 2).contiguous()
     sw = devil_fit.get("beta_sandwiches")

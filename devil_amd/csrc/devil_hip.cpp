@@ -19,9 +19,44 @@
 #include "../../include/devil_hip.h"
 #include "kernels.hpp"
 
+// Device blocks of a context kept for reuse after the scope that used them
+// ends, so repeated fits do not pay hipMalloc / hipFree (which synchronises
+// the device).  Every access to a context's blocks is ordered on that
+// context's stream, so work queued by the next call cannot touch a reused
+// block before the earlier call's work on it has finished.
+struct BlockCache {
+  static constexpr size_t kMaxCached = size_t(6) << 30;
+  std::multimap<size_t, void*> blocks;  // size -> block
+  size_t cached = 0;
+  // the smallest cached block of at least bytes (and not far larger)
+  void* take(size_t bytes, size_t& got) {
+    auto it = blocks.lower_bound(bytes);
+    if (it == blocks.end() || it->first > 2 * bytes + (size_t(1) << 20)) return nullptr;
+    void* p = it->second;
+    got = it->first;
+    cached -= got;
+    blocks.erase(it);
+    return p;
+  }
+  void give(void* p, size_t bytes) {
+    if (cached + bytes > kMaxCached) {
+      (void)hipFree(p);
+      return;
+    }
+    blocks.emplace(bytes, p);
+    cached += bytes;
+  }
+  void clear() {
+    for (auto& b : blocks) (void)hipFree(b.second);
+    blocks.clear();
+    cached = 0;
+  }
+};
+
 struct devil_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  BlockCache cache;
   std::string err;
   int timing = 0;
   int tune[DEVIL_N_TUNE] = {DEVIL_MIXED_AUTO, 0, 4};  // kernel, chunk cells (0 auto), streams
@@ -46,6 +81,9 @@ struct devil_ctx {
   int64_t k_n[DEVIL_N_KERNELS] = {0, 0};
   double k_ms[DEVIL_N_KERNELS] = {0, 0};
   double k_bytes[DEVIL_N_KERNELS] = {0, 0};
+  // IRLS round counts read back without a stream synchronisation
+  int32_t* round_counts = nullptr;  // pinned host slots
+  hipEvent_t round_ev[2] = {nullptr, nullptr};
 };
 
 namespace {
@@ -62,31 +100,46 @@ int fail(devil_ctx* ctx, int code, const std::string& msg) {
       return fail(ctx, DEVIL_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
-// Device allocation owned by the caller's scope.
+// Device allocation owned by the caller's scope; its block returns to the
+// context's cache when the scope ends.
 struct DevBuf {
   void* p = nullptr;
+  size_t size = 0;
+  BlockCache* cache = nullptr;
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : p(o.p) { o.p = nullptr; }
+  DevBuf(DevBuf&& o) noexcept : p(o.p), size(o.size), cache(o.cache) { o.p = nullptr; }
   DevBuf& operator=(DevBuf&& o) noexcept {
     if (this != &o) {
-      if (p) (void)hipFree(p);
+      release();
       p = o.p;
+      size = o.size;
+      cache = o.cache;
       o.p = nullptr;
     }
     return *this;
   }
-  ~DevBuf() {
-    if (p) (void)hipFree(p);
+  ~DevBuf() { release(); }
+  void release() {
+    if (!p) return;
+    if (cache) cache->give(p, size);
+    else (void)hipFree(p);
+    p = nullptr;
   }
-  hipError_t alloc(size_t bytes) {
-    if (p) {
-      (void)hipFree(p);
-      p = nullptr;
+  hipError_t alloc(BlockCache* c, size_t bytes) {
+    release();
+    bytes = 256 * ((std::max<size_t>(bytes, 8) + 255) / 256);
+    cache = c;
+    size = bytes;
+    if (c && (p = c->take(bytes, size))) return hipSuccess;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e == hipErrorOutOfMemory && c && c->cached) {  // cached blocks back to the device
+      c->clear();
+      e = hipMalloc(&p, bytes);
     }
-    if (bytes == 0) bytes = 8;
-    return hipMalloc(&p, bytes);
+    if (e != hipSuccess) p = nullptr;
+    return e;
   }
   template <typename T>
   T* as() const {
@@ -96,7 +149,7 @@ struct DevBuf {
 
 #define DV_ALLOC(buf, bytes)                                                             \
   do {                                                                                   \
-    hipError_t e_ = (buf).alloc(bytes);                                                  \
+    hipError_t e_ = (buf).alloc(&ctx->cache, bytes);                                     \
     if (e_ != hipSuccess)                                                                \
       return fail(ctx, DEVIL_E_NOMEM, std::string("hipMalloc ") + std::to_string(bytes) + \
                                           " bytes: " + hipGetErrorString(e_));           \
@@ -809,7 +862,7 @@ struct LaunchTimer {
 void collect_launches(devil_ctx* ctx) {
   for (auto& l : ctx->launches) {
     float ms = 0.f;
-    if (hipEventElapsedTime(&ms, l.a, l.b) == hipSuccess) {
+    if (l.kind >= 0 && hipEventElapsedTime(&ms, l.a, l.b) == hipSuccess) {
       ctx->k_n[l.kind] += 1;
       ctx->k_ms[l.kind] += ms;
       ctx->k_bytes[l.kind] += l.bytes;
@@ -853,10 +906,30 @@ dv::TileDesign round_shape(const dv::TileDesign& td, int nA, int tiles_per_wg,
   return (tr.nchunk > td.nchunk && tr.nchunk <= cap) ? tr : td;
 }
 
+// Pinned host slots and events through which irls_rounds reads the active
+// counts (made on a context's first fit).
+int round_sync_setup(devil_ctx* ctx) {
+  if (!ctx->round_counts) {
+    void* h = nullptr;
+    DV_HIP(hipHostMalloc(&h, 64, hipHostMallocDefault));
+    ctx->round_counts = static_cast<int32_t*>(h);
+  }
+  for (hipEvent_t& e : ctx->round_ev)
+    if (!e) DV_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  return DEVIL_OK;
+}
+
 // Rounds of accumulate + update over the active list act (count ncnt on the
 // device, nA on the host) until none is left.  With band > 0 the mixed kernel
 // runs and genes near the threshold are appended to replay / nreplay.  part
 // holds part_cap doubles.
+//
+// The host never waits for the round it has just queued: round k's grid is
+// sized for the count of round k - 1's list (lists only shrink), which round
+// k - 2's update wrote and the host reads once that update has finished, so
+// the GPU always has the next round queued.  A round queued for a list that
+// has meanwhile emptied exits at once (the kernels read the count on the
+// device).  The launch records get the exact count of their round.
 int irls_rounds(devil_ctx* ctx, const TileHost& tl, bool mixed, dv::Counts c, int p,
                 double* beta, const double* kvec, int max_iter, double eps, int32_t* iters,
                 int32_t* status, int32_t* act, int32_t* act_spare, int32_t* ncnt,
@@ -865,33 +938,55 @@ int irls_rounds(devil_ctx* ctx, const TileHost& tl, bool mixed, dv::Counts c, in
   const dv::TileDesign& tfull = mixed ? tl.tdm : tl.td;
   const int tiles_per_wg = accum_tiles_per_wg(mixed ? tl.mixed : kMixedNone, p, c);
   hipStream_t s = ctx->stream;
+  int rc;
+  if ((rc = round_sync_setup(ctx))) return rc;
   const double row_bytes = (double)count_elem_size(c.dtype) * (double)tfull.n;
   int32_t* cur = act;
   int32_t* nxt = act_spare;
   int32_t* ncur = ncnt;
   int32_t* nnxt = ncnt_spare;
-  for (int it = 0; it < max_iter && nA > 0; ++it) {
-    const dv::TileDesign td = round_shape(tfull, nA, tiles_per_wg, part_cap);
-    {
-      LaunchTimer lt(ctx, mixed ? DEVIL_KERNEL_IRLS_MIXED : DEVIL_KERNEL_IRLS_FP64,
-                     row_bytes * nA);
-      if (!mixed)
-        DV_HIP(dv::launch_irls_accum(s, td, c, p, cur, ncur, nA, beta, kvec, part));
-      else
-        DV_HIP(launch_mixed_accum(tl.mixed, s, td, c, p, cur, ncur, nA, beta, kvec, part));
+  volatile int32_t* slot = ctx->round_counts;
+  std::vector<int> count{nA};       // count[k]: genes of round k's list
+  std::vector<long> rec;            // round k's launch record (-1: none)
+  int k = 0;
+  for (; k < max_iter; ++k) {
+    if (k >= 2) {
+      DV_HIP(hipEventSynchronize(ctx->round_ev[k & 1]));
+      count.push_back((int)slot[k & 1]);  // round k - 2's update: round k - 1's list
     }
+    const int bound = k == 0 ? nA : count[k - 1];
+    if (bound <= 0) break;
+    const dv::TileDesign td = round_shape(tfull, bound, tiles_per_wg, part_cap);
+    const size_t nrec = ctx->launches.size();
+    {
+      LaunchTimer lt(ctx, mixed ? DEVIL_KERNEL_IRLS_MIXED : DEVIL_KERNEL_IRLS_FP64, 0.0);
+      if (!mixed)
+        DV_HIP(dv::launch_irls_accum(s, td, c, p, cur, ncur, bound, beta, kvec, part));
+      else
+        DV_HIP(launch_mixed_accum(tl.mixed, s, td, c, p, cur, ncur, bound, beta, kvec, part));
+    }
+    rec.push_back(ctx->launches.size() > nrec ? (long)nrec : -1);
     DV_HIP(dv::launch_fill_i32(s, nnxt, 1, 0));
-    DV_HIP(dv::launch_irls_update(s, td, p, cur, ncur, nA, part, beta, kvec, max_iter, eps,
+    DV_HIP(dv::launch_irls_update(s, td, p, cur, ncur, bound, part, beta, kvec, max_iter, eps,
                                   iters, status, nxt, nnxt, mixed ? kMixedBand : 0.0, replay,
                                   nreplay));
-    int32_t h = 0;
-    DV_HIP(hipMemcpyAsync(&h, nnxt, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    DV_HIP(hipStreamSynchronize(s));
-    collect_launches(ctx);
-    nA = h;
+    DV_HIP(hipMemcpyAsync(ctx->round_counts + (k & 1), nnxt, sizeof(int32_t),
+                          hipMemcpyDeviceToHost, s));
+    DV_HIP(hipEventRecord(ctx->round_ev[k & 1], s));
     std::swap(cur, nxt);
     std::swap(ncur, nnxt);
   }
+  DV_HIP(hipStreamSynchronize(s));
+  // counts of the lists the loop did not read (rounds k - 2 and k - 1)
+  for (int j = std::max(0, k - 2); j < k; ++j)
+    if ((int)count.size() == j + 1) count.push_back((int)slot[j & 1]);
+  // exact bytes per launch; launches over an emptied list are dropped
+  for (size_t j = 0; j < rec.size(); ++j) {
+    if (rec[j] < 0) continue;
+    ctx->launches[rec[j]].bytes = row_bytes * (double)count[j];
+    if (count[j] == 0) ctx->launches[rec[j]].kind = -1;
+  }
+  collect_launches(ctx);
   return DEVIL_OK;
 }
 
@@ -1157,6 +1252,10 @@ void devil_destroy(devil_ctx* ctx) {
   }
   for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
   for (devil_ctx* sub : ctx->subs) devil_destroy(sub);
+  for (hipEvent_t e : ctx->round_ev)
+    if (e) (void)hipEventDestroy(e);
+  if (ctx->round_counts) (void)hipHostFree(ctx->round_counts);
+  ctx->cache.clear();
   delete ctx;
 }
 

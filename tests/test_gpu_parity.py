@@ -161,10 +161,26 @@ def test_fit_device_resident_and_deterministic(dv, orc):
     Yd = torch.as_tensor(Y, device="cuda")
     d1 = dv.fit_devil(Yd, w.X, size_factors=w.sf)
     d2 = dv.fit_devil(Yd, w.X, size_factors=w.sf)
+    # the design matrix on the GPU (C- and column-ordered) is read as is
+    Xc = torch.as_tensor(np.ascontiguousarray(w.X), device="cuda")
+    Xf = torch.as_tensor(np.ascontiguousarray(w.X.T), device="cuda").t()
+    d3 = dv.fit_devil(Yd, Xc, size_factors=w.sf)
+    d4 = dv.fit_devil(Yd, Xf, size_factors=w.sf)
     for key in ("beta", "overdispersion", "beta_sandwiches_null"):
         a = d1[key].cpu().numpy()
         np.testing.assert_array_equal(a, d2[key].cpu().numpy())
         np.testing.assert_array_equal(a, host[key])
+        np.testing.assert_array_equal(a, d3[key].cpu().numpy())
+        np.testing.assert_array_equal(a, d4[key].cpu().numpy())
+    # size factors on the GPU: the offset log(sf) is taken there (the two
+    # logarithms may differ in the last bit)
+    d5 = dv.fit_devil(Yd, Xf, size_factors=torch.as_tensor(w.sf, device="cuda"))
+    np.testing.assert_array_equal(d5["iterations"]["beta_iters"].cpu().numpy(),
+                                  host["iterations"]["beta_iters"])
+    assert_rel(d5["beta"].cpu().numpy(), host["beta"], 1e-10, 1e-3, "beta (GPU sf)")
+    r1 = dv.test_de(d1, contrast=[0, 1])
+    r5 = dv.test_de(host, contrast=[0, 1])
+    np.testing.assert_array_equal(r1["pval"], r5["pval"])
 
 
 def test_test_de_parity(dv, orc):
