@@ -442,4 +442,58 @@ __device__ __forceinline__ double log1p_f64(double x) {
   return isfinite(u) ? r + c : u;
 }
 
+// Table-driven logarithm (P. T. P. Tang, "Table-driven implementation of the
+// logarithm function in IEEE floating-point arithmetic", ACM TOMS 16(4),
+// 1990): u = 2^e m, m in [1, 2), F = 1 + j/128 the nearest breakpoint,
+// log u = e log 2 + log F + log1p(r) with r = (m - F) / F, |r| <= 2^-8, and
+// log1p(r) by its Taylor series to r^7 (truncation below 2^-67).  Entry j of
+// the table holds 1/F and log F; a workgroup fills it once with log_tab_fill.
+struct LogTabEntry {
+  double inv;  // 1 / (1 + j/128), correctly rounded
+  double ln;   // log(1 + j/128) (log1p_f64)
+};
+constexpr int kLogTab = 129;
+
+// threads [0, nthreads) of the workgroup fill tab; the caller synchronises
+__device__ __forceinline__ void log_tab_fill(LogTabEntry* tab, int tid, int nthreads) {
+  for (int j = tid; j < kLogTab; j += nthreads) {
+    const double f = (double)j * (1.0 / 128);
+    tab[j].inv = 1.0 / (1.0 + f);
+    tab[j].ln = log1p_f64(f);
+  }
+}
+
+// log(u) + c / u for u > 0 finite, where c is a correction much smaller than
+// u (the rounding error of u when u = 1 + x); other u as log: -inf at 0,
+// NaN below, u itself for inf / NaN.
+__device__ __forceinline__ double log_tab(double u, double c, const LogTabEntry* tab) {
+  int e = __builtin_amdgcn_frexp_exp(u) - 1;
+  const double m = 2.0 * __builtin_amdgcn_frexp_mant(u);  // [1, 2)
+  const double jd = __builtin_rint(fma(m, 128.0, -128.0));
+  // jd is in [0, 128] for finite u > 0; the clamp keeps other u in the table
+  const LogTabEntry t = tab[min(max((int)jd, 0), kLogTab - 1)];
+  const double f = m - fma(jd, 1.0 / 128, 1.0);  // exact
+  const double r = f * t.inv;
+  double q = 1.0 / 7;
+  q = fma(q, r, -1.0 / 6);
+  q = fma(q, r, 1.0 / 5);
+  q = fma(q, r, -1.0 / 4);
+  q = fma(q, r, 1.0 / 3);
+  q = fma(q, r, -0.5);
+  const double lp = fma(r * r, q, r);
+  const double ed = (double)e;
+  const double hi = fma(ed, 6.93147180369123816490e-01, t.ln);
+  const double lo = fma(ed, 1.90821492927058770002e-10, __builtin_amdgcn_ldexp(c * t.inv, -e));
+  const double res = hi + (lp + lo);
+  if (u > 0.0 && u < __builtin_inf()) return res;
+  return u == 0.0 ? -__builtin_inf() : (u > 0.0 || u != u ? u : __builtin_nan(""));
+}
+
+// log(1 + x) through log_tab: u = 1 + x rounded, c = x - (u - 1) its rounding
+// error (exact for 0 <= x < 2^53).
+__device__ __forceinline__ double log1p_tab(double x, const LogTabEntry* tab) {
+  const double u = 1.0 + x;
+  return log_tab(u, x - (u - 1.0), tab);
+}
+
 }  // namespace dv

@@ -338,7 +338,8 @@ __global__ __launch_bounds__(64) void k_bread_finish(TileDesign td, ChunkTable c
 template <typename CT, int ZM, bool VEC>
 __device__ __forceinline__ void init_tile_chunk(const TileDesign& td, InitTile it, ChunkTable ct,
                                                 const CT* __restrict__ Y, int64_t ld, int64_t G,
-                                                int tile, int chunk, double* __restrict__ part) {
+                                                int tile, int chunk, const LogTabEntry* ltab,
+                                                double* __restrict__ part) {
   const int lane = threadIdx.x & 63;
   const int s = lane & 15, h = lane >> 4;
   const int64_t g0 = (int64_t)tile * 16 + s;
@@ -378,9 +379,9 @@ __device__ __forceinline__ void init_tile_chunk(const TileDesign& td, InitTile i
       const bool in = cb + r >= c0 && cb + r < c1;
       const double y = in ? (double)o.y[r] : 0.0;
       if constexpr (ZM == 1) {
-        z[r] = y != 0.0 ? log1p_f64(y * o.aux[r]) : 0.0;
+        z[r] = y != 0.0 ? log1p_tab(y * o.aux[r], ltab) : 0.0;
       } else if constexpr (ZM == 2) {
-        z[r] = in ? log(y + 0.5) - o.aux[r] : 0.0;
+        z[r] = in ? log_tab(y + 0.5, 0.0, ltab) - o.aux[r] : 0.0;
       } else {
         z[r] = 0.0;
       }
@@ -423,6 +424,11 @@ template <int ZM>
 __global__ __launch_bounds__(256) void k_init_accum(TileDesign td, InitTile it, ChunkTable ct,
                                                     Counts c, int64_t G,
                                                     double* __restrict__ part) {
+  __shared__ LogTabEntry ltab[kLogTab];
+  if constexpr (ZM != 0) {
+    log_tab_fill(ltab, threadIdx.x, blockDim.x);
+    __syncthreads();
+  }
   const int wave = threadIdx.x >> 6;
   int chunk, group;
   tile_coords(ct.nchunk, chunk, group);
@@ -430,7 +436,8 @@ __global__ __launch_bounds__(256) void k_init_accum(TileDesign td, InitTile it, 
   if ((int64_t)tile * 16 >= G) return;
   with_counts(c, [&](auto Y, auto vec) {
     typedef std::remove_cv_t<std::remove_pointer_t<decltype(Y)>> CT;
-    init_tile_chunk<CT, ZM, decltype(vec)::value>(td, it, ct, Y, c.ld, G, tile, chunk, part);
+    init_tile_chunk<CT, ZM, decltype(vec)::value>(td, it, ct, Y, c.ld, G, tile, chunk, ltab,
+                                                  part);
   });
 }
 

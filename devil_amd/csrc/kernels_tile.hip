@@ -298,6 +298,29 @@ __device__ __forceinline__ f4 mfma_bf(bf8 a, bf8 b, f4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
+typedef float f2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+
+// x = hi + lo in bf16: hi = x rounded to bf16, lo = (x - hi) rounded.  Pairs
+// are converted together (one v_cvt_pk_bf16_f32 each) and hi is read back
+// as FP32 from the packed word (shift / mask), not converted twice.
+__device__ __forceinline__ void bf_split8(const float (&x)[8], bf8& hi, bf8& lo) {
+  u4 h, l;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const unsigned int u = __builtin_bit_cast(unsigned int,
+                                              __builtin_convertvector(f2{x[2 * i], x[2 * i + 1]}, bf2));
+    const float h0 = __builtin_bit_cast(float, u << 16);
+    const float h1 = __builtin_bit_cast(float, u & 0xffff0000u);
+    h[i] = u;
+    l[i] = __builtin_bit_cast(unsigned int,
+                              __builtin_convertvector(f2{x[2 * i] - h0, x[2 * i + 1] - h1}, bf2));
+  }
+  hi = __builtin_bit_cast(bf8, h);
+  lo = __builtin_bit_cast(bf8, l);
+}
+
 // Operands of one 32-cell step for lane (s, h): counts (as stored) of cells
 // 4h + r and 16 + 4h + r, the eta A operands of both halves, the offsets
 // (times log2 e), the gradient-tile and pair-tile B operands.
@@ -498,11 +521,7 @@ __device__ __forceinline__ void irls_tile_chunk_bf(const TileDesign& td, const C
 #pragma unroll
     for (int j = 0; j < 4; ++j) ag0 = mfma32(rr[4 + j], o.kg1[j], ag0);
     bf8 r1, r2;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      r1[i] = (__bf16)rr[i];
-      r2[i] = (__bf16)(rr[i] - (float)r1[i]);
-    }
+    bf_split8(rr, r1, r2);
 #pragma unroll
     for (int t = 0; t < NTH; ++t)
       if (t < nth) {
@@ -755,11 +774,7 @@ __device__ __forceinline__ void ws_compute(const TileDesign& td, const CT* __res
 #pragma unroll
     for (int j = 0; j < 4; ++j) ag0 = mfma32(rr[4 + j], o.kg1[j], ag0);
     bf8 r1, r2;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      r1[i] = (__bf16)rr[i];
-      r2[i] = (__bf16)(rr[i] - (float)r1[i]);
-    }
+    bf_split8(rr, r1, r2);
 #pragma unroll
     for (int t = 0; t < NTH; ++t)
       if (t < nth) {

@@ -204,6 +204,42 @@ def test_test_de_parity(dv, orc):
     np.testing.assert_array_equal(np.isnan(r0["adj_pval"]), np.isnan(f0["adj_pval"]))
 
 
+def test_init_logarithm_accuracy(dv):
+    """The init pass's logarithm (table-driven: log u = e log 2 + log F +
+    log1p(r), |r| <= 2^-8) against numpy on isolated values.  An intercept-only
+    design on 16 cells gives Q = 1/4 and R = 4 exactly, so beta0 of a gene is
+    exactly (1/16) sum_i log1p(y_i / s_i); each gene has one nonzero count, so
+    16 beta0 is a single value.  Counts 1 .. 2^31 - 1 and size factors 1e-3 ..
+    1e3 cover every table slot region and binade in between: within 5 ulp
+    (numpy's and the device's exp(-offset) may differ in the last bit)."""
+    from devil_amd import _lib as L
+    ctx = dv.context(0)
+    lib = dv.lib()
+    n = 16
+    ys = [1, 2, 3, 5, 7, 10, 31, 100, 255, 256, 257, 1000, 4095, 12345, 65535, 99991,
+          2 ** 20 + 1, 2 ** 24 + 3, 2 ** 30 - 1, 2 ** 31 - 1]
+    sf = np.array([1.0, 1e-3, 0.37, 0.5, 0.9999, 1.0001, 1.9, 2.0, 2.1, 3.3, 7.0, 15.5, 100.0,
+                   999.0, 1e3, 0.0123])
+    G = len(ys) * n
+    Y = np.zeros((G, n), np.int32)
+    for a, y in enumerate(ys):
+        for c in range(n):
+            Y[a * n + c, c] = y
+    off = np.log(sf)
+    say("[case] init logarithm, 320 isolated values")
+    t = lambda a, dt=torch.float64: torch.as_tensor(np.ascontiguousarray(a), dtype=dt,
+                                                    device="cuda")
+    Yd, Xd, offd = t(Y, torch.int32), t(np.ones(n)), t(off)
+    b0 = torch.empty(G, dtype=torch.float64, device="cuda")
+    ctx.check(lib.devil_init_beta(ctx.h, Yd.data_ptr(), L.COUNTS_I32, G, n, n, Xd.data_ptr(), 1,
+                                  offd.data_ptr(), L.INIT_QR, b0.data_ptr()))
+    z = 16.0 * b0.cpu().numpy()
+    ref = np.log1p(Y.max(axis=1) * np.exp(-off[np.arange(G) % n]))
+    e = np.abs(z - ref) / ref
+    say(f"  max rel err {e.max():.2e}")
+    assert e.max() <= 5 * 2.0 ** -53
+
+
 def test_stage_entry_points(dv, orc):
     """Each C-ABI stage against the oracle function it restates."""
     import ctypes as C
