@@ -693,7 +693,7 @@ int make_tile(devil_ctx* ctx, const double* Xd, const double* offd, const double
     const int nth = tm.nt - 1;
     DV_ALLOC(th.xab, sizeof(uint16_t) * nst * 2 * NE * 64 * 8);
     DV_ALLOC(th.offl, sizeof(float) * nst * 32);
-    DV_ALLOC(th.kg, sizeof(float) * nst * 64 * 8);
+    DV_ALLOC(th.kg, sizeof(uint16_t) * nst * 3 * 64 * 8);
     // at least one pair tile: the kernels' operand loads are unconditional
     // (a design without pair columns reads zeros it never uses)
     const int64_t nth_alloc = std::max(nth, 1);
@@ -706,11 +706,11 @@ int make_tile(devil_ctx* ctx, const double* Xd, const double* offd, const double
     }
     DV_HIP(dv::launch_build_tables_bf16(s, Xd, offd, n, p, th.specm.as<dv::ColSpec>(),
                                         (int)cm.spec.size(), nth, nst, th.xab.as<uint16_t>(),
-                                        th.offl.as<float>(), th.kg.as<float>(),
+                                        th.offl.as<float>(), th.kg.as<uint16_t>(),
                                         th.kh.as<uint16_t>(), th.kh2.as<uint16_t>()));
     tm.xab = th.xab.as<uint16_t>();
     tm.offl = th.offl.as<float>();
-    tm.kg = th.kg.as<float>();
+    tm.kg = th.kg.as<uint16_t>();
     tm.kh = th.kh.as<uint16_t>();
     tm.kh2 = th.kh2.as<uint16_t>();
     tm.eop = td.eop;

@@ -55,7 +55,8 @@ struct TileDesign {
                          // cell 32 st + 16 u + lane%16: per feature f, entries 7 f + t
                          // hold (x1, x1, x1, x1, x2, x2, x2), x = x1 + x2
   const float* offl;     // off * log2(e), padded with zeros to 32 nblk
-  const float* kg;       // [st][i/4][lane][i%4] = X[32 st + c(lane/16, i)][lane%16]
+  const uint16_t* kg;    // [st][q][lane][8] bf16 piece q of X[32 st + c(lane/16, i)][lane%16]:
+                         // x = x1 + x2 + x3, each piece rounded from what the earlier leave
   const uint16_t* kh;    // [st][t][lane][8] bf16 K1 of K[32 st + c(lane/16, i)][16 (t+1) + lane%16]
   const uint16_t* kh2;   // the same layout, bf16 K2 = K - K1
   const double* eop;     // exp(off) padded with zeros to 16 nblk (FP64 design)
@@ -85,7 +86,7 @@ hipError_t launch_build_init(hipStream_t s, const double* M, const double* shift
                              int64_t nblk, double* Zp, double* einv, double* logsfp);
 hipError_t launch_build_tables_bf16(hipStream_t s, const double* X, const double* off, int64_t n,
                                     int p, const ColSpec* spec, int ncol, int nth, int64_t nst,
-                                    uint16_t* xab, float* offl, float* kg, uint16_t* kh,
+                                    uint16_t* xab, float* offl, uint16_t* kg, uint16_t* kh,
                                     uint16_t* kh2);
 
 // Cell ranges of the post-IRLS tile passes, aligned to the patient blocks of a

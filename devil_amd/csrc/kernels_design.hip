@@ -67,7 +67,7 @@ __global__ void k_build_tables(const double* __restrict__ X, const double* __res
 __global__ void k_build_tables_bf16(const double* __restrict__ X, const double* __restrict__ off,
                                     int64_t n, int p, const ColSpec* __restrict__ spec,
                                     int ncol, int nth, int64_t nst, uint16_t* __restrict__ xab,
-                                    float* __restrict__ offl, float* __restrict__ kg,
+                                    float* __restrict__ offl, uint16_t* __restrict__ kg,
                                     uint16_t* __restrict__ kh, uint16_t* __restrict__ kh2) {
   const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= nst * 64) return;
@@ -97,8 +97,12 @@ __global__ void k_build_tables_bf16(const double* __restrict__ X, const double* 
   for (int i = 0; i < 8; ++i) {
     const int64_t cell = st * 32 + (i < 4 ? 4 * (l / 16) + i : 16 + 4 * (l / 16) + (i - 4));
     const int col = l % 16;
-    kg[((st * 2 + (i >> 2)) * 64 + l) * 4 + (i & 3)] =
-        (cell < n && col < p) ? (float)X[cell + col * n] : 0.0f;
+    double x = (cell < n && col < p) ? X[cell + col * n] : 0.0;
+    for (int q = 0; q < 3; ++q) {  // x = x1 + x2 + x3 (each rounded from the rest)
+      const uint16_t xq = bf16_rne((float)x);
+      kg[((st * 3 + q) * 64 + l) * 8 + i] = xq;
+      x -= (double)__uint_as_float((uint32_t)xq << 16);
+    }
     for (int t = 0; t < nth; ++t) {
       const int kc = 16 * (t + 1) + l % 16;
       const ColSpec cs = kc < ncol ? spec[kc] : ColSpec{-1, -1};
@@ -279,7 +283,7 @@ hipError_t launch_build_tables(hipStream_t s, const double* X, const double* off
 
 hipError_t launch_build_tables_bf16(hipStream_t s, const double* X, const double* off, int64_t n,
                                     int p, const ColSpec* spec, int ncol, int nth, int64_t nst,
-                                    uint16_t* xab, float* offl, float* kg, uint16_t* kh,
+                                    uint16_t* xab, float* offl, uint16_t* kg, uint16_t* kh,
                                     uint16_t* kh2) {
   const int64_t threads = nst * 64;
   hipLaunchKernelGGL(k_build_tables_bf16, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0,

@@ -284,8 +284,9 @@ __global__ __launch_bounds__(256) void k_irls_accum_mixed(TileDesign td, Counts 
 //        entries (x1 b1, x1 b2, x1 b3, x1 b4, x2 b1, x2 b2, x2 b3) per feature,
 //        FP32 accumulation; register r holds cell 4h + r of the half.
 //   r = (y - mu) / (k + mu), mu = 2^(eta + off log2 e)             (FP32 VALU)
-//   gradient tile (the p design columns): r' X on v_mfma_f32_16x16x4f32, one
-//        product per cell (k index h, step j -> cell c(h, j))
+//   gradient tile (the p design columns): r' X from r = r1 + r2 + r3 and
+//        X = X1 + X2 + X3 in bf16, the six products of total order <= 2
+//        (below 2^-26 relative error per product: FP32-product accuracy)
 //   pair tiles (x_a x_b): three bf16 products r1 K1 + r1 K2 + r2 K1 with
 //        r = r1 + r2 and K = K1 + K2 split into bf16 (about 16 significant
 //        bits per product), k index (h, i) -> cell
@@ -302,23 +303,44 @@ typedef float f2 __attribute__((ext_vector_type(2)));
 typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
 typedef unsigned int u4 __attribute__((ext_vector_type(4)));
 
-// x = hi + lo in bf16: hi = x rounded to bf16, lo = (x - hi) rounded.  Pairs
-// are converted together (one v_cvt_pk_bf16_f32 each) and hi is read back
-// as FP32 from the packed word (shift / mask), not converted twice.
-__device__ __forceinline__ void bf_split8(const float (&x)[8], bf8& hi, bf8& lo) {
-  u4 h, l;
+// x = x1 + x2 + x3 in bf16, each piece the rounding of what the earlier ones
+// leave (the differences are exact in FP32), so |x - x1 - x2 - x3| <= 2^-27
+// |x|.  Pairs are converted together (one v_cvt_pk_bf16_f32 each) and read
+// back as FP32 from the packed word (shift / mask).
+__device__ __forceinline__ float bf_lo_f(unsigned int u) { return __builtin_bit_cast(float, u << 16); }
+__device__ __forceinline__ float bf_hi_f(unsigned int u) {
+  return __builtin_bit_cast(float, u & 0xffff0000u);
+}
+__device__ __forceinline__ unsigned int bf_pack(float a, float b) {
+  return __builtin_bit_cast(unsigned int, __builtin_convertvector(f2{a, b}, bf2));
+}
+
+__device__ __forceinline__ void bf_split8_2(const float (&x)[8], bf8& x1, bf8& x2) {
+  u4 a, b;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    const unsigned int u = __builtin_bit_cast(unsigned int,
-                                              __builtin_convertvector(f2{x[2 * i], x[2 * i + 1]}, bf2));
-    const float h0 = __builtin_bit_cast(float, u << 16);
-    const float h1 = __builtin_bit_cast(float, u & 0xffff0000u);
-    h[i] = u;
-    l[i] = __builtin_bit_cast(unsigned int,
-                              __builtin_convertvector(f2{x[2 * i] - h0, x[2 * i + 1] - h1}, bf2));
+    const unsigned int u = bf_pack(x[2 * i], x[2 * i + 1]);
+    a[i] = u;
+    b[i] = bf_pack(x[2 * i] - bf_lo_f(u), x[2 * i + 1] - bf_hi_f(u));
   }
-  hi = __builtin_bit_cast(bf8, h);
-  lo = __builtin_bit_cast(bf8, l);
+  x1 = __builtin_bit_cast(bf8, a);
+  x2 = __builtin_bit_cast(bf8, b);
+}
+
+__device__ __forceinline__ void bf_split8_3(const float (&x)[8], bf8& x1, bf8& x2, bf8& x3) {
+  u4 a, b, c;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const unsigned int u = bf_pack(x[2 * i], x[2 * i + 1]);
+    const float d0 = x[2 * i] - bf_lo_f(u), d1 = x[2 * i + 1] - bf_hi_f(u);
+    const unsigned int v = bf_pack(d0, d1);
+    a[i] = u;
+    b[i] = v;
+    c[i] = bf_pack(d0 - bf_lo_f(v), d1 - bf_hi_f(v));
+  }
+  x1 = __builtin_bit_cast(bf8, a);
+  x2 = __builtin_bit_cast(bf8, b);
+  x3 = __builtin_bit_cast(bf8, c);
 }
 
 // Operands of one 32-cell step for lane (s, h): counts (as stored) of cells
@@ -328,7 +350,7 @@ template <int NE, int NTH>
 struct BfTables {
   bf8 xa[2 * NE];
   f4 off0, off1;
-  f4 kg0, kg1;
+  bf8 kg[3];
   bf8 k1[NTH], k2[NTH];
 };
 
@@ -337,11 +359,45 @@ struct BfCounts {
   CT y[8];
 };
 
+// One step's contractions of the cell scalars rr (residuals or bread
+// weights) of cells c(h, i).  Gradient tile (the design columns), GRAD3:
+// r and x each in three bf16 pieces, the six products of total order <= 2
+// (smallest first); what is dropped and the split residues are below 2^-26
+// of |r x|, under the rounding of an FP32 product.  Without GRAD3 (the
+// bread, whose design columns hold pair products too) and in the pair
+// tiles: r1 K1 + r2 K1 + r1 K2 (about 16 significant bits per product).
+template <int NTH, bool GRAD3, typename S>
+__device__ __forceinline__ void bf_accumulate(const float (&rr)[8], const S& o, int nth, f4& ag,
+                                              f4 (&ah)[NTH]) {
+  bf8 r1, r2, r3;
+  if constexpr (GRAD3) {
+    bf_split8_3(rr, r1, r2, r3);
+    ag = mfma_bf(r3, o.kg[0], ag);
+    ag = mfma_bf(r2, o.kg[1], ag);
+    ag = mfma_bf(r1, o.kg[2], ag);
+    ag = mfma_bf(r2, o.kg[0], ag);
+    ag = mfma_bf(r1, o.kg[1], ag);
+    ag = mfma_bf(r1, o.kg[0], ag);
+  } else {
+    bf_split8_2(rr, r1, r2);
+    ag = mfma_bf(r2, o.kg[0], ag);
+    ag = mfma_bf(r1, o.kg[1], ag);
+    ag = mfma_bf(r1, o.kg[0], ag);
+  }
+#pragma unroll
+  for (int t = 0; t < NTH; ++t)
+    if (t < nth) {
+      ah[t] = mfma_bf(r2, o.k1[t], ah[t]);
+      ah[t] = mfma_bf(r1, o.k2[t], ah[t]);
+      ah[t] = mfma_bf(r1, o.k1[t], ah[t]);
+    }
+}
+
 template <int NE, int NTH, typename CT>
 struct BfStep {
   bf8 xa[2 * NE];
   f4 off0, off1;
-  f4 kg0, kg1;
+  bf8 kg[3];
   bf8 k1[NTH], k2[NTH];
   CT y[8];
 };
@@ -389,9 +445,9 @@ __device__ __forceinline__ void load_bf_eta(const TileDesign& td, int64_t st, in
 template <int NTH, typename S>
 __device__ __forceinline__ void load_bf_k(const TileDesign& td, int64_t st, int lane, int nth,
                                           S& o) {
-  const f4* ks = reinterpret_cast<const f4*>(td.kg) + st * 128;
-  o.kg0 = ks[lane];
-  o.kg1 = ks[64 + lane];
+  const bf8* gs = reinterpret_cast<const bf8*>(td.kg) + st * (3 * 64);
+#pragma unroll
+  for (int q = 0; q < 3; ++q) o.kg[q] = gs[q * 64 + lane];
   const bf8* k1s = reinterpret_cast<const bf8*>(td.kh) + st * nth * 64;
   const bf8* k2s = reinterpret_cast<const bf8*>(td.kh2) + st * nth * 64;
 #pragma unroll
@@ -515,20 +571,7 @@ __device__ __forceinline__ void irls_tile_chunk_bf(const TileDesign& td, const C
   auto step = [&](const Step& o, int64_t) {
     float rr[8];
     bf_residuals<NE, CT>(o, o.y, bb, k, tmin, rr);
-    // gradient tile: FP32 products, cell c(h, j) for step j
-#pragma unroll
-    for (int j = 0; j < 4; ++j) ag0 = mfma32(rr[j], o.kg0[j], ag0);
-#pragma unroll
-    for (int j = 0; j < 4; ++j) ag0 = mfma32(rr[4 + j], o.kg1[j], ag0);
-    bf8 r1, r2;
-    bf_split8(rr, r1, r2);
-#pragma unroll
-    for (int t = 0; t < NTH; ++t)
-      if (t < nth) {
-        ah[t] = mfma_bf(r2, o.k1[t], ah[t]);
-        ah[t] = mfma_bf(r1, o.k2[t], ah[t]);
-        ah[t] = mfma_bf(r1, o.k1[t], ah[t]);
-      }
+    bf_accumulate<NTH, true>(rr, o, nth, ag0, ah);
     if (++span == kSpan) {
       span = 0;
 #pragma unroll
@@ -597,11 +640,11 @@ typedef __attribute__((address_space(3))) void* lds_ptr_t;
 
 // Compute waves (16-gene tiles) per workgroup of the wave-specialised kernel
 // (one more wave loads the tables): as many as the LDS holds, at most 15
-// (a workgroup of 1024 threads).  Table ring: 3 slots of (2 NE + 2 + 2 NTH)
+// (a workgroup of 1024 threads).  Table ring: 3 slots of (2 NE + 3 + 2 NTH)
 // KB + 128 B; each compute wave: 4 count slots of 2 KB.
 __host__ __device__ constexpr int ws_waves(int p) {
   const int ne = (7 * p + 31) / 32, nth = (p * (p + 1) / 2 + 15) / 16;
-  const int tables = 3 * ((2 * ne + 2 + 2 * nth) * 1024 + 128);
+  const int tables = 3 * ((2 * ne + 3 + 2 * nth) * 1024 + 128);
   const int fit = (160 * 1024 - tables) / (4 * 2048);
   return fit < 15 ? fit : 15;
 }
@@ -609,13 +652,13 @@ __host__ __device__ constexpr int ws_waves(int p) {
 template <int NE, int NTH>
 struct WsLayout {
   static constexpr int kX = 0;                          // 2 NE x 1 KB: eta A operands
-  static constexpr int kG = 2 * NE * 1024;              // 2 x 1 KB: gradient tile
-  static constexpr int kH1 = kG + 2 * 1024;             // NTH x 1 KB: pair tiles (hi)
+  static constexpr int kG = 2 * NE * 1024;              // 3 x 1 KB: gradient tile (3 pieces)
+  static constexpr int kH1 = kG + 3 * 1024;             // NTH x 1 KB: pair tiles (hi)
   static constexpr int kH2 = kH1 + NTH * 1024;          // NTH x 1 KB: pair tiles (lo)
   static constexpr int kOff = kH2 + NTH * 1024;         // 128 B: 32 offsets (x log2 e)
   static constexpr int kSlot = kOff + 128;
   static constexpr int kSlots = 3;
-  static constexpr int kLoads = 2 * NE + 2 + 2 * NTH + 1;  // loads per step
+  static constexpr int kLoads = 2 * NE + 3 + 2 * NTH + 1;  // loads per step
   // per compute wave: 4 count slots of 2 x 1 KB (cells 4 h + r, 16 + 4 h + r)
   static constexpr int kY = kSlots * kSlot;
   static constexpr int kYSlot = 2048;
@@ -656,9 +699,9 @@ __device__ __forceinline__ void ws_loader(const TileDesign& td, int64_t s0, int6
     const bf8* xs = reinterpret_cast<const bf8*>(td.xab) + st * (2 * NE * 64);
 #pragma unroll
     for (int j = 0; j < 2 * NE; ++j) lds_dma16(xs + j * 64 + lane, b + Lay::kX + j * 1024);
-    const f4* ks = reinterpret_cast<const f4*>(td.kg) + st * 128;
-    lds_dma16(ks + lane, b + Lay::kG);
-    lds_dma16(ks + 64 + lane, b + Lay::kG + 1024);
+    const bf8* gs = reinterpret_cast<const bf8*>(td.kg) + st * (3 * 64);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) lds_dma16(gs + q * 64 + lane, b + Lay::kG + q * 1024);
     const bf8* k1s = reinterpret_cast<const bf8*>(td.kh) + st * nth * 64;
     const bf8* k2s = reinterpret_cast<const bf8*>(td.kh2) + st * nth * 64;
 #pragma unroll
@@ -753,9 +796,9 @@ __device__ __forceinline__ void ws_compute(const TileDesign& td, const CT* __res
     const f4* lo = reinterpret_cast<const f4*>(b + Lay::kOff);
     o.off0 = lo[h];
     o.off1 = lo[4 + h];
-    const f4* lg = reinterpret_cast<const f4*>(b + Lay::kG);
-    o.kg0 = lg[lane];
-    o.kg1 = lg[64 + lane];
+    const bf8* lg = reinterpret_cast<const bf8*>(b + Lay::kG);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) o.kg[q] = lg[q * 64 + lane];
     const bf8* l1 = reinterpret_cast<const bf8*>(b + Lay::kH1);
     const bf8* l2 = reinterpret_cast<const bf8*>(b + Lay::kH2);
 #pragma unroll
@@ -769,19 +812,7 @@ __device__ __forceinline__ void ws_compute(const TileDesign& td, const CT* __res
       bf_residuals<NE, CT>(o, yc.y, bb, k, tmin, rr);
     else
       bf_bread_weights<NE, CT>(o, yc.y, bb, k, tmin, rr);
-#pragma unroll
-    for (int j = 0; j < 4; ++j) ag0 = mfma32(rr[j], o.kg0[j], ag0);
-#pragma unroll
-    for (int j = 0; j < 4; ++j) ag0 = mfma32(rr[4 + j], o.kg1[j], ag0);
-    bf8 r1, r2;
-    bf_split8(rr, r1, r2);
-#pragma unroll
-    for (int t = 0; t < NTH; ++t)
-      if (t < nth) {
-        ah[t] = mfma_bf(r2, o.k1[t], ah[t]);
-        ah[t] = mfma_bf(r1, o.k2[t], ah[t]);
-        ah[t] = mfma_bf(r1, o.k1[t], ah[t]);
-      }
+    bf_accumulate<NTH, MODE == 0>(rr, o, nth, ag0, ah);
     if (++span == kSpan) {
       span = 0;
 #pragma unroll
