@@ -442,6 +442,59 @@ __device__ __forceinline__ double log1p_f64(double x) {
   return isfinite(u) ? r + c : u;
 }
 
+// Table-driven exponential (P. T. P. Tang, "Table-driven implementation of
+// the exponential function in IEEE floating-point arithmetic", ACM TOMS
+// 15(2), 1989): x = (64 m + j) ln2/64 + r with |r| <= ln2/128, exp x =
+// 2^m 2^(j/64) e^r, e^r - 1 by its Taylor series to r^6 (truncation below
+// 3e-20); 2^(j/64) from the table, scaled with v_ldexp_f64 (inf / 0 outside
+// the double range).  The index j = n mod 64 stays in [0, 64) for any x.
+// Entries are 2^(j/64) correctly rounded (computed to 60 digits).
+static __device__ constexpr double kExp2Tab[64] = {
+    0x1.0000000000000p+0, 0x1.02c9a3e778061p+0, 0x1.059b0d3158574p+0,
+    0x1.0874518759bc8p+0, 0x1.0b5586cf9890fp+0, 0x1.0e3ec32d3d1a2p+0,
+    0x1.11301d0125b51p+0, 0x1.1429aaea92de0p+0, 0x1.172b83c7d517bp+0,
+    0x1.1a35beb6fcb75p+0, 0x1.1d4873168b9aap+0, 0x1.2063b88628cd6p+0,
+    0x1.2387a6e756238p+0, 0x1.26b4565e27cddp+0, 0x1.29e9df51fdee1p+0,
+    0x1.2d285a6e4030bp+0, 0x1.306fe0a31b715p+0, 0x1.33c08b26416ffp+0,
+    0x1.371a7373aa9cbp+0, 0x1.3a7db34e59ff7p+0, 0x1.3dea64c123422p+0,
+    0x1.4160a21f72e2ap+0, 0x1.44e086061892dp+0, 0x1.486a2b5c13cd0p+0,
+    0x1.4bfdad5362a27p+0, 0x1.4f9b2769d2ca7p+0, 0x1.5342b569d4f82p+0,
+    0x1.56f4736b527dap+0, 0x1.5ab07dd485429p+0, 0x1.5e76f15ad2148p+0,
+    0x1.6247eb03a5585p+0, 0x1.6623882552225p+0, 0x1.6a09e667f3bcdp+0,
+    0x1.6dfb23c651a2fp+0, 0x1.71f75e8ec5f74p+0, 0x1.75feb564267c9p+0,
+    0x1.7a11473eb0187p+0, 0x1.7e2f336cf4e62p+0, 0x1.82589994cce13p+0,
+    0x1.868d99b4492edp+0, 0x1.8ace5422aa0dbp+0, 0x1.8f1ae99157736p+0,
+    0x1.93737b0cdc5e5p+0, 0x1.97d829fde4e50p+0, 0x1.9c49182a3f090p+0,
+    0x1.a0c667b5de565p+0, 0x1.a5503b23e255dp+0, 0x1.a9e6b5579fdbfp+0,
+    0x1.ae89f995ad3adp+0, 0x1.b33a2b84f15fbp+0, 0x1.b7f76f2fb5e47p+0,
+    0x1.bcc1e904bc1d2p+0, 0x1.c199bdd85529cp+0, 0x1.c67f12e57d14bp+0,
+    0x1.cb720dcef9069p+0, 0x1.d072d4a07897cp+0, 0x1.d5818dcfba487p+0,
+    0x1.da9e603db3285p+0, 0x1.dfc97337b9b5fp+0, 0x1.e502ee78b3ff6p+0,
+    0x1.ea4afa2a490dap+0, 0x1.efa1bee615a27p+0, 0x1.f50765b6e4540p+0,
+    0x1.fa7c1819e90d8p+0,
+};
+
+// threads [0, nthreads) of the workgroup copy the table to tab (LDS); the
+// caller synchronises
+__device__ __forceinline__ void exp_tab_fill(double* tab, int tid, int nthreads) {
+  for (int j = tid; j < 64; j += nthreads) tab[j] = kExp2Tab[j];
+}
+
+__device__ __forceinline__ double exp_tab(double x, const double* tab) {
+  const double nd = rint(x * 92.332482616893656877);  // 64 / ln 2
+  double r = fma(-nd, 6.93147180369123816490e-01 / 64, x);
+  r = fma(-nd, 1.90821492927058770002e-10 / 64, r);
+  const int n = (int)nd;
+  double q = 1.0 / 720;
+  q = fma(q, r, 1.0 / 120);
+  q = fma(q, r, 1.0 / 24);
+  q = fma(q, r, 1.0 / 6);
+  q = fma(q, r, 0.5);
+  const double p = fma(r * r, q, r);  // e^r - 1
+  const double t = tab[n & 63];
+  return __builtin_amdgcn_ldexp(fma(t, p, t), n >> 6);
+}
+
 // Table-driven logarithm (P. T. P. Tang, "Table-driven implementation of the
 // logarithm function in IEEE floating-point arithmetic", ACM TOMS 16(4),
 // 1990): u = 2^e m, m in [1, 2), F = 1 + j/128 the nearest breakpoint,

@@ -32,7 +32,8 @@ template <int P, typename CT, bool VEC>
 __device__ __forceinline__ void mom_tile_chunk(const TileDesign& td, ChunkTable ct,
                                                const CT* __restrict__ Y, int64_t ld, int64_t G,
                                                const double* __restrict__ beta, int tile,
-                                               int chunk, double* __restrict__ part) {
+                                               int chunk, double* __restrict__ part,
+                                               const double* __restrict__ e2) {
   constexpr int NJ = (P + 3) / 4;
   const int lane = threadIdx.x & 63;
   const int s = lane & 15, h = lane >> 4;
@@ -59,7 +60,7 @@ __device__ __forceinline__ void mom_tile_chunk(const TileDesign& td, ChunkTable 
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       if (cb + r >= c0 && cb + r < c1) {
-        const double mu = cur.off[r] * exp_f64(eta[r]);
+        const double mu = cur.off[r] * exp_tab(eta[r], e2);
         const double d = (double)cur.y[r] - mu;
         num += d * d - mu;
         den += mu * mu;
@@ -90,7 +91,8 @@ __device__ __forceinline__ void bread_tile_chunk(const TileDesign& td, ChunkTabl
                                                  const CT* __restrict__ Y, int64_t ld,
                                                  int64_t G, const double* __restrict__ beta,
                                                  const double* __restrict__ theta, int tile,
-                                                 int chunk, double* __restrict__ part) {
+                                                 int chunk, double* __restrict__ part,
+                                                 const double* __restrict__ e2) {
   constexpr int NJ = (P + 3) / 4;
   constexpr int NTMAX = ((P * (P + 1)) / 2 + P + 15) / 16;
   const int lane = threadIdx.x & 63;
@@ -123,7 +125,7 @@ __device__ __forceinline__ void bread_tile_chunk(const TileDesign& td, ChunkTabl
     double w[4], rr[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const double mu = cur.off[r] * exp_f64(eta[r]);
+      const double mu = cur.off[r] * exp_tab(eta[r], e2);
       const double y = (double)cur.y[r];
       const double d = fma(th, mu, 1.0);
       const double id = rcp_f64(d);
@@ -166,6 +168,9 @@ template <int P>
 __global__ __launch_bounds__(256) void k_mom_accum(TileDesign td, ChunkTable ct, Counts c,
                                                    int64_t G, const double* __restrict__ beta,
                                                    double* __restrict__ part) {
+  __shared__ double e2[64];
+  exp_tab_fill(e2, threadIdx.x, blockDim.x);
+  __syncthreads();
   const int wave = threadIdx.x >> 6;
   int chunk, group;
   tile_coords(ct.nchunk, chunk, group);
@@ -173,7 +178,8 @@ __global__ __launch_bounds__(256) void k_mom_accum(TileDesign td, ChunkTable ct,
   if ((int64_t)tile * 16 >= G) return;
   with_counts(c, [&](auto Y, auto vec) {
     typedef std::remove_cv_t<std::remove_pointer_t<decltype(Y)>> CT;
-    mom_tile_chunk<P, CT, decltype(vec)::value>(td, ct, Y, c.ld, G, beta, tile, chunk, part);
+    mom_tile_chunk<P, CT, decltype(vec)::value>(td, ct, Y, c.ld, G, beta, tile, chunk, part,
+                                                e2);
   });
 }
 
@@ -206,6 +212,9 @@ __global__ __launch_bounds__(256) void k_bread_accum(TileDesign td, ChunkTable c
                                                      int64_t G, const double* __restrict__ beta,
                                                      const double* __restrict__ theta,
                                                      double* __restrict__ part) {
+  __shared__ double e2[64];
+  exp_tab_fill(e2, threadIdx.x, blockDim.x);
+  __syncthreads();
   const int wave = threadIdx.x >> 6;
   int chunk, group;
   tile_coords(ct.nchunk, chunk, group);
@@ -214,7 +223,7 @@ __global__ __launch_bounds__(256) void k_bread_accum(TileDesign td, ChunkTable c
   with_counts(c, [&](auto Y, auto vec) {
     typedef std::remove_cv_t<std::remove_pointer_t<decltype(Y)>> CT;
     bread_tile_chunk<P, CT, decltype(vec)::value>(td, ct, Y, c.ld, G, beta, theta, tile, chunk,
-                                                  part);
+                                                  part, e2);
   });
 }
 

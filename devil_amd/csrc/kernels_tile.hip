@@ -45,7 +45,8 @@ __device__ __forceinline__ void irls_tile_chunk(const TileDesign& td, const CT* 
                                                 int64_t ld, const int32_t* __restrict__ active,
                                                 int nA, const double* __restrict__ beta,
                                                 const double* __restrict__ kvec, int tile,
-                                                int chunk, double* __restrict__ part) {
+                                                int chunk, double* __restrict__ part,
+                                                const double* __restrict__ e2) {
   constexpr int NJ = (P + 3) / 4;
   constexpr int NTMAX = ((P * (P + 1)) / 2 + P + 15) / 16;
   const int lane = threadIdx.x & 63;
@@ -79,7 +80,7 @@ __device__ __forceinline__ void irls_tile_chunk(const TileDesign& td, const CT* 
     double q[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const double w = exp_f64(eta[r] - cur.off[r]);
+      const double w = exp_tab(eta[r] - cur.off[r], e2);
       const double d = fma(k, w, 1.0);
       q[r] = fma((double)cur.y[r], w, -1.0) * rcp1_f64(d);
     }
@@ -1017,6 +1018,9 @@ __global__ __launch_bounds__(256) void k_irls_accum(TileDesign td, Counts c,
                                                     const double* __restrict__ beta,
                                                     const double* __restrict__ kvec,
                                                     double* __restrict__ part) {
+  __shared__ double e2[64];
+  exp_tab_fill(e2, threadIdx.x, blockDim.x);
+  __syncthreads();
   const int nA = *nactive;
   const int ntile = (nA + 15) / 16;
   const int wave = threadIdx.x >> 6;
@@ -1027,7 +1031,7 @@ __global__ __launch_bounds__(256) void k_irls_accum(TileDesign td, Counts c,
   with_counts(c, [&](auto Y, auto vec) {
     typedef std::remove_cv_t<std::remove_pointer_t<decltype(Y)>> CT;
     irls_tile_chunk<P, CT, decltype(vec)::value>(td, Y, c.ld, active, nA, beta, kvec, tile,
-                                                 chunk, part);
+                                                 chunk, part, e2);
   });
 }
 
