@@ -212,7 +212,7 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=1,
                     help="timed FP64 steps on the same counts (0: no comparison)")
     ap.add_argument("--cpu-genes", type=int, default=None)
-    ap.add_argument("--mixed-kernel", default=None, choices=["f32", "bf16", "bf16_ws"],
+    ap.add_argument("--mixed-kernel", default=None, choices=["f32", "bf16", "bf16_ws", "bf16_ws2"],
                     help="mixed IRLS kernel (default: the library's choice)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--chunk-cells", type=int, default=0,
@@ -300,9 +300,11 @@ def main():
     fit_ms = sum(stages[k] for k in ("init", "irls", "dispersion", "bread_meat"))
     value = w.G * w.n * args.steps / T
     if kname == "irls_mixed" and info.get("mixed_kernel") == "bf16":
-        kdesc = f"k_irls_accum_bf16<{w.p}> (bf16/FP32 MFMA, FP64 partials)"
+        kdesc = f"k_irls_accum_bf16<{w.p}> (bf16 MFMA, FP64 partials)"
     elif kname == "irls_mixed" and info.get("mixed_kernel") == "bf16_ws":
-        kdesc = f"k_irls_accum_ws<{w.p}> (bf16/FP32 MFMA, LDS-staged operands)"
+        kdesc = f"k_irls_accum_ws<{w.p}> (bf16 MFMA, LDS-staged operands)"
+    elif kname == "irls_mixed" and info.get("mixed_kernel") == "bf16_ws2":
+        kdesc = f"k_irls_accum_ws2<{w.p}> (bf16 MFMA, LDS-staged operands, two tiles per wave)"
     elif kname == "irls_mixed":
         kdesc = f"k_irls_accum_mixed<{w.p}> (FP32 MFMA, FP64 partials)"
     else:

@@ -477,7 +477,8 @@ struct StageTimer {
 };
 
 // Host construction of the MFMA operand layouts (dv::TileDesign).
-enum MixedKernel { kMixedNone = 0, kMixedF32 = 1, kMixedBf16 = 2, kMixedBf16Ws = 3 };
+enum MixedKernel { kMixedNone = 0, kMixedF32 = 1, kMixedBf16 = 2, kMixedBf16Ws = 3,
+                   kMixedBf16Ws2 = 4 };
 
 // 4-byte counts in 16-byte aligned rows (what the LDS-staged kernels load).
 bool counts_vec4(const dv::Counts& c) {
@@ -487,19 +488,25 @@ bool counts_vec4(const dv::Counts& c) {
 
 // 16-gene tiles per workgroup of a round's accumulate kernel.
 int accum_tiles_per_wg(int kind, int p, const dv::Counts& c) {
-  return (kind == kMixedBf16Ws && counts_vec4(c)) ? dv::ws_tiles_per_workgroup(p) : 4;
+  if (!counts_vec4(c)) return 4;
+  if (kind == kMixedBf16Ws) return dv::ws_tiles_per_workgroup(p, 1);
+  if (kind == kMixedBf16Ws2) return dv::ws_tiles_per_workgroup(p, 2);
+  return 4;
 }
 
 int mixed_kernel(const devil_ctx* ctx) {
   switch (ctx->tune[DEVIL_TUNE_MIXED_KERNEL]) {
     case DEVIL_MIXED_BF16: return kMixedBf16;
     case DEVIL_MIXED_BF16_WS: return kMixedBf16Ws;
+    case DEVIL_MIXED_BF16_WS2: return kMixedBf16Ws2;
     default: return kMixedF32;
   }
 }
 
 // the bf16 kernels share one operand layout (32-cell steps)
-bool bf16_layout(int mixed) { return mixed == kMixedBf16 || mixed == kMixedBf16Ws; }
+bool bf16_layout(int mixed) {
+  return mixed == kMixedBf16 || mixed == kMixedBf16Ws || mixed == kMixedBf16Ws2;
+}
 
 hipError_t launch_mixed_accum(int kind, hipStream_t s, const dv::TileDesign& td, dv::Counts c,
                               int p, const int32_t* cur, const int32_t* ncur, int nA,
@@ -507,7 +514,9 @@ hipError_t launch_mixed_accum(int kind, hipStream_t s, const dv::TileDesign& td,
   if (kind == kMixedBf16)
     return dv::launch_irls_accum_bf16(s, td, c, p, cur, ncur, nA, beta, kvec, part);
   if (kind == kMixedBf16Ws)
-    return dv::launch_irls_accum_ws(s, td, c, p, cur, ncur, nA, beta, kvec, part);
+    return dv::launch_irls_accum_ws(s, td, c, p, cur, ncur, nA, beta, kvec, part, 1);
+  if (kind == kMixedBf16Ws2)
+    return dv::launch_irls_accum_ws(s, td, c, p, cur, ncur, nA, beta, kvec, part, 2);
   return dv::launch_irls_accum_mixed(s, td, c, p, cur, ncur, nA, beta, kvec, part);
 }
 
@@ -1148,7 +1157,7 @@ int autotune_mixed(devil_ctx* ctx, const DeviceDesign& dd, int64_t n, int p, dv:
   const int64_t Gs = std::min<int64_t>(G, 4096);
   std::vector<int> kinds;
   if (ctx->tune[DEVIL_TUNE_MIXED_KERNEL] == DEVIL_MIXED_AUTO)
-    kinds = {kMixedF32, kMixedBf16, kMixedBf16Ws};
+    kinds = {kMixedF32, kMixedBf16, kMixedBf16Ws};  // two-tile waves measured slower on C3
   else kinds = {mixed_kernel(ctx)};
   std::vector<int64_t> cells;
   if (ctx->tune[DEVIL_TUNE_CHUNK_CELLS] == 0) cells = {1024, 2048, 4096};
@@ -1582,7 +1591,7 @@ int devil_stage_times(const devil_ctx* ctx, double* ms) {
 int devil_set_tuning(devil_ctx* ctx, int32_t key, int64_t value) {
   if (!ctx || key < 0 || key >= DEVIL_N_TUNE) return DEVIL_E_ARG;
   if (key == DEVIL_TUNE_MIXED_KERNEL && value != DEVIL_MIXED_F32 && value != DEVIL_MIXED_BF16 &&
-      value != DEVIL_MIXED_AUTO && value != DEVIL_MIXED_BF16_WS)
+      value != DEVIL_MIXED_AUTO && value != DEVIL_MIXED_BF16_WS && value != DEVIL_MIXED_BF16_WS2)
     return fail(ctx, DEVIL_E_ARG, "unknown mixed kernel");
   if (key == DEVIL_TUNE_CHUNK_CELLS && value != 0 && (value < 256 || value > (1 << 20)))
     return fail(ctx, DEVIL_E_ARG, "chunk cells must be 0 (autotune) or in [256, 2^20]");
@@ -1599,6 +1608,7 @@ int devil_fit_info(const devil_ctx* ctx, int32_t key, int64_t* value) {
     case DEVIL_INFO_MIXED_KERNEL:
       *value = ctx->last_mk == kMixedBf16     ? DEVIL_MIXED_BF16
                : ctx->last_mk == kMixedBf16Ws ? DEVIL_MIXED_BF16_WS
+               : ctx->last_mk == kMixedBf16Ws2 ? DEVIL_MIXED_BF16_WS2
                : ctx->last_mk == kMixedF32    ? DEVIL_MIXED_F32
                                               : -1;
       return DEVIL_OK;

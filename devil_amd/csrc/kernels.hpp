@@ -144,12 +144,14 @@ hipError_t launch_irls_accum_bf16(hipStream_t s, const TileDesign& td, Counts c,
                                   const double* beta, const double* kvec, double* part);
 
 // Wave-specialised bf16 variant (producer / consumer wavefronts, same design).
-// 16-gene tiles per workgroup of the LDS-staged kernels at p columns.
-int ws_tiles_per_workgroup(int p);
+// 16-gene tiles per workgroup of the LDS-staged kernels at p columns with
+// tpw tiles per compute wave.
+int ws_tiles_per_workgroup(int p, int tpw);
 
 hipError_t launch_irls_accum_ws(hipStream_t s, const TileDesign& td, Counts c, int p,
                                 const int32_t* active, const int32_t* nactive, int nA_bound,
-                                const double* beta, const double* kvec, double* part);
+                                const double* beta, const double* kvec, double* part,
+                                int tpw = 1);
 
 // Mixed-precision bread (no clusters) on the bf16 operand tables td: the
 // wave-specialised kernel with bread weights, then H^-1 per gene.  Genes are

@@ -643,11 +643,11 @@ typedef __attribute__((address_space(3))) void* lds_ptr_t;
 // (one more wave loads the tables): as many as the LDS holds, at most 15
 // (a workgroup of 1024 threads).  Table ring: 3 slots of (2 NE + 3 + 2 NTH)
 // KB + 128 B; each compute wave: 4 count slots of 2 KB.
-__host__ __device__ constexpr int ws_waves(int p) {
+__host__ __device__ constexpr int ws_waves(int p, int tpw = 1) {
   const int ne = (7 * p + 31) / 32, nth = (p * (p + 1) / 2 + 15) / 16;
   const int tables = 3 * ((2 * ne + 3 + 2 * nth) * 1024 + 128);
-  const int fit = (160 * 1024 - tables) / (4 * 2048);
-  return fit < 15 ? fit : 15;
+  const int fit = (160 * 1024 - tables) / (tpw * 4 * 2048);
+  return fit < 15 / tpw ? fit : 15 / tpw;
 }
 
 template <int NE, int NTH>
@@ -740,12 +740,12 @@ __device__ __forceinline__ void ws_loader(const TileDesign& td, int64_t s0, int6
 // MODE 0: IRLS accumulate (gpar = k, residuals r); MODE 1: bread (gpar =
 // theta, weights w): the same contractions of a per-cell scalar with the
 // design columns and the pair columns.
-template <int P, int MODE, typename CT>
+template <int P, int MODE, int TPW, typename CT>
 __device__ __forceinline__ void ws_compute(const TileDesign& td, const CT* __restrict__ Y,
                                            int64_t ld, const int32_t* __restrict__ active,
                                            int nA, const double* __restrict__ beta,
-                                           const double* __restrict__ kvec, int tile,
-                                           bool tile_live, int chunk, int64_t s0, int64_t s1,
+                                           const double* __restrict__ kvec, int tile0,
+                                           int ntile, int chunk, int64_t s0, int64_t s1,
                                            unsigned char* ring, double* __restrict__ part) {
   constexpr int NE = (7 * P + 31) / 32;
   constexpr int NTH = ((P * (P + 1)) / 2 + 15) / 16;
@@ -753,42 +753,73 @@ __device__ __forceinline__ void ws_compute(const TileDesign& td, const CT* __res
   typedef WsLayout<NE, NTH> Lay;
   const int lane = threadIdx.x & 63;
   const int s = lane & 15, h = lane >> 4;
-  const int slot = tile * 16 + s;
-  const bool live = tile_live && slot < nA;
-  const int g = tile_live ? (live ? active[slot] : active[tile * 16]) : 0;
   const int nth = td.nt - 1;
-  bf8 bb[NE];
-  bf_beta_operand<P, NE>(beta, g, live, h, bb);
-  const float k = live ? (float)kvec[g] : 0.0f;
-  const CT* yrow = Y + (int64_t)g * ld;
-  const CT* yh = yrow + 4 * h;
-  const int64_t nsteps = s1 - s0;
-  f4 ag0 = f4{0.f, 0.f, 0.f, 0.f};
-  double ag64[4] = {0.0, 0.0, 0.0, 0.0};
-  f4 ah[NTH];
+  // the wave's tiles tile0 .. tile0 + TPW - 1; a wave whose first tile is
+  // live runs all of them (tiles past the last one repeat the first one's
+  // rows, so every tile issues the same loads, and write nothing)
+  const bool wave_live = tile0 < ntile;
+  bool tile_live[TPW];
+  int g[TPW];
+  bf8 bb[TPW][NE];
+  float k[TPW];
+  const CT* yrow[TPW];
+  const CT* yh[TPW];
 #pragma unroll
-  for (int t = 0; t < NTH; ++t) ah[t] = f4{0.f, 0.f, 0.f, 0.f};
-  float tmin = 0.0f;
+  for (int u = 0; u < TPW; ++u) {
+    const int tile = tile0 + u;
+    tile_live[u] = tile < ntile;
+    const int slot = (tile_live[u] ? tile : tile0) * 16 + s;
+    const bool live = wave_live && slot < nA;
+    g[u] = wave_live ? (live ? active[slot] : active[(tile_live[u] ? tile : tile0) * 16]) : 0;
+    bf_beta_operand<P, NE>(beta, g[u], live, h, bb[u]);
+    k[u] = live ? (float)kvec[g[u]] : 0.0f;
+    yrow[u] = Y + (int64_t)g[u] * ld;
+    yh[u] = yrow[u] + 4 * h;
+  }
+  const int64_t nsteps = s1 - s0;
+  f4 ag0[TPW];
+  double ag64[TPW][4];
+  f4 ah[TPW][NTH];
+  float tmin[TPW];
+#pragma unroll
+  for (int u = 0; u < TPW; ++u) {
+    ag0[u] = f4{0.f, 0.f, 0.f, 0.f};
+    tmin[u] = 0.0f;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) ag64[u][r] = 0.0;
+#pragma unroll
+    for (int t = 0; t < NTH; ++t) ah[u][t] = f4{0.f, 0.f, 0.f, 0.f};
+  }
   int span = 0;
   typedef BfCounts<CT> Cnt;
-  unsigned char* yring = ring + Lay::kY + (threadIdx.x >> 6) * Lay::kYWave;
-  // counts of step st into count slot q (lane l's 16 bytes at l * 16)
+  unsigned char* yring = ring + Lay::kY + (threadIdx.x >> 6) * (TPW * Lay::kYWave);
+  // counts of step st into count slot q of every tile (lane l's 16 bytes at
+  // l * 16)
   auto issuey = [&](int64_t st, int q) {
-    lds_dma16(yh + st * 32, yring + q * Lay::kYSlot);
-    lds_dma16(yh + st * 32 + 16, yring + q * Lay::kYSlot + 1024);
-  };
-  auto ready = [&](int q, Cnt& o) {
-    typedef typename Vec4Of<CT>::T V4;
-    const V4 a = reinterpret_cast<const V4*>(yring + q * Lay::kYSlot)[lane];
-    const V4 b = reinterpret_cast<const V4*>(yring + q * Lay::kYSlot + 1024)[lane];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      o.y[r] = a[r];
-      o.y[4 + r] = b[r];
+    for (int u = 0; u < TPW; ++u) {
+      unsigned char* yr = yring + u * Lay::kYWave + q * Lay::kYSlot;
+      lds_dma16(yh[u] + st * 32, yr);
+      lds_dma16(yh[u] + st * 32 + 16, yr + 1024);
     }
   };
-  // step s0 + k: tables from ring slot k % 3, counts from the register ring
-  auto step = [&](int64_t kk, const Cnt& yc) {
+  auto ready = [&](int q, Cnt (&o)[TPW]) {
+    typedef typename Vec4Of<CT>::T V4;
+#pragma unroll
+    for (int u = 0; u < TPW; ++u) {
+      const unsigned char* yr = yring + u * Lay::kYWave + q * Lay::kYSlot;
+      const V4 a = reinterpret_cast<const V4*>(yr)[lane];
+      const V4 b = reinterpret_cast<const V4*>(yr + 1024)[lane];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        o[u].y[r] = a[r];
+        o[u].y[4 + r] = b[r];
+      }
+    }
+  };
+  // step s0 + k: tables from ring slot k % 3 (read once for the wave's
+  // tiles), counts from the count rings
+  auto step = [&](int64_t kk, const Cnt (&yc)[TPW]) {
     const unsigned char* b = ring + (int)(kk % Lay::kSlots) * Lay::kSlot;
     BfTables<NE, NTH> o;
     const bf8* lx = reinterpret_cast<const bf8*>(b + Lay::kX);
@@ -808,29 +839,35 @@ __device__ __forceinline__ void ws_compute(const TileDesign& td, const CT* __res
         o.k1[t] = l1[t * 64 + lane];
         o.k2[t] = l2[t * 64 + lane];
       }
-    float rr[8];
-    if constexpr (MODE == 0)
-      bf_residuals<NE, CT>(o, yc.y, bb, k, tmin, rr);
-    else
-      bf_bread_weights<NE, CT>(o, yc.y, bb, k, tmin, rr);
-    bf_accumulate<NTH, MODE == 0>(rr, o, nth, ag0, ah);
+#pragma unroll
+    for (int u = 0; u < TPW; ++u) {
+      float rr[8];
+      if constexpr (MODE == 0)
+        bf_residuals<NE, CT>(o, yc[u].y, bb[u], k[u], tmin[u], rr);
+      else
+        bf_bread_weights<NE, CT>(o, yc[u].y, bb[u], k[u], tmin[u], rr);
+      bf_accumulate<NTH, MODE == 0>(rr, o, nth, ag0[u], ah[u]);
+    }
     if (++span == kSpan) {
       span = 0;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) ag64[r] += (double)ag0[r];
-      ag0 = f4{0.f, 0.f, 0.f, 0.f};
+      for (int u = 0; u < TPW; ++u) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) ag64[u][r] += (double)ag0[u][r];
+        ag0[u] = f4{0.f, 0.f, 0.f, 0.f};
+      }
     }
   };
   // Steps [s0, s1f) lie inside the rows.  Their counts are in flight three
   // steps ahead: s0 .. s0 + 2 before barrier 0, then in step k the load of
   // step s0 + k + 3 (clamped to s1f - 1) into count slot (k + 3) % 4, whose
   // step k - 1 was read.  This wave's vector-memory instructions are these
-  // loads alone (two per step), so vmcnt(6) after issuing step k + 3 leaves
-  // steps k + 1 .. k + 3 in flight and step k in LDS.  A row's partial last
-  // step comes after them and loads its counts when it runs.
+  // loads alone (two per tile and step), so vmcnt(6 TPW) after issuing step
+  // k + 3 leaves steps k + 1 .. k + 3 in flight and step k in LDS.  A row's
+  // partial last step comes after them and loads its counts when it runs.
   const int64_t s1f = max(s0, min(s1, td.n / 32));
   const int64_t nf = s1f - s0;
-  if (tile_live && nf > 0) {
+  if (wave_live && nf > 0) {
 #pragma unroll
     for (int d = 0; d < 3; ++d) issuey(min(s0 + d, s1f - 1), d);
   }
@@ -839,10 +876,10 @@ __device__ __forceinline__ void ws_compute(const TileDesign& td, const CT* __res
   for (; kk + 4 <= nf; kk += 4) {
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      if (tile_live) {
+      if (wave_live) {
         issuey(min(s0 + kk + u + 3, s1f - 1), (u + 3) & 3);
-        wait_vmcnt<6>();
-        Cnt yc;
+        wait_vmcnt<6 * TPW>();
+        Cnt yc[TPW];
         ready(u, yc);
         step(kk + u, yc);
       }
@@ -855,8 +892,8 @@ __device__ __forceinline__ void ws_compute(const TileDesign& td, const CT* __res
 #pragma unroll
   for (int u = 0; u < 3; ++u) {
     if (kk + u < nf) {
-      if (tile_live) {
-        Cnt yc;
+      if (wave_live) {
+        Cnt yc[TPW];
         ready(u, yc);
         step(kk + u, yc);
       }
@@ -864,43 +901,50 @@ __device__ __forceinline__ void ws_compute(const TileDesign& td, const CT* __res
     }
   }
   for (kk = nf; kk < nsteps; ++kk) {
-    if (tile_live) {
-      Cnt t;
-      load_bf_y_tail<CT>(td, yrow, s0 + kk, h, t.y);
+    if (wave_live) {
+      Cnt t[TPW];
+#pragma unroll
+      for (int u = 0; u < TPW; ++u) load_bf_y_tail<CT>(td, yrow[u], s0 + kk, h, t[u].y);
       step(kk, t);
     }
     ws_barrier();
   }
-  if (!tile_live) return;
+  if (!wave_live) return;
 #pragma unroll
-  for (int r = 0; r < 4; ++r) ag64[r] += (double)ag0[r];
-  // IRLS: tmin is the smallest log2(mu), poison below -1024 (FP64's
-  // exp(-eta - off) overflows there); bread: tmin holds the largest, poison
-  // from 126 on (FP32 mu overflows)
-  bool poison[4];
-  if constexpr (MODE == 0) {
-    tmin = fminf(tmin, __shfl_xor(tmin, 16, 64));
-    tmin = fminf(tmin, __shfl_xor(tmin, 32, 64));
+  for (int u = 0; u < TPW; ++u) {
+    if (!tile_live[u]) continue;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) poison[r] = __shfl(tmin, 4 * h + r, 64) < -1024.0f;
-  } else {
-    tmin = fmaxf(tmin, __shfl_xor(tmin, 16, 64));
-    tmin = fmaxf(tmin, __shfl_xor(tmin, 32, 64));
+    for (int r = 0; r < 4; ++r) ag64[u][r] += (double)ag0[u][r];
+    // IRLS: tmin is the smallest log2(mu), poison below -1024 (FP64's
+    // exp(-eta - off) overflows there); bread: tmin holds the largest,
+    // poison from 126 on (FP32 mu overflows)
+    float tm = tmin[u];
+    bool poison[4];
+    if constexpr (MODE == 0) {
+      tm = fminf(tm, __shfl_xor(tm, 16, 64));
+      tm = fminf(tm, __shfl_xor(tm, 32, 64));
 #pragma unroll
-    for (int r = 0; r < 4; ++r) poison[r] = !(__shfl(tmin, 4 * h + r, 64) < 126.0f);
-  }
-  double* out = part + ((int64_t)tile * td.nchunk + chunk) * 16 * td.ncp;
+      for (int r = 0; r < 4; ++r) poison[r] = __shfl(tm, 4 * h + r, 64) < -1024.0f;
+    } else {
+      tm = fmaxf(tm, __shfl_xor(tm, 16, 64));
+      tm = fmaxf(tm, __shfl_xor(tm, 32, 64));
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const double nanv = __builtin_nan("");
-    out[(4 * h + r) * td.ncp + s] = poison[r] ? nanv : ag64[r];
+      for (int r = 0; r < 4; ++r) poison[r] = !(__shfl(tm, 4 * h + r, 64) < 126.0f);
+    }
+    double* out = part + ((int64_t)(tile0 + u) * td.nchunk + chunk) * 16 * td.ncp;
 #pragma unroll
-    for (int t = 0; t < NTH; ++t)
-      if (t < nth) out[(4 * h + r) * td.ncp + 16 * (t + 1) + s] = poison[r] ? nanv : (double)ah[t][r];
+    for (int r = 0; r < 4; ++r) {
+      const double nanv = __builtin_nan("");
+      out[(4 * h + r) * td.ncp + s] = poison[r] ? nanv : ag64[u][r];
+#pragma unroll
+      for (int t = 0; t < NTH; ++t)
+        if (t < nth)
+          out[(4 * h + r) * td.ncp + 16 * (t + 1) + s] = poison[r] ? nanv : (double)ah[u][t][r];
+    }
   }
 }
 
-template <int P, int MODE>
+template <int P, int MODE, int TPW>
 __device__ __forceinline__ void ws_body(const TileDesign& td, const Counts& c,
                                         const int32_t* __restrict__ active,
                                         const int32_t* __restrict__ nactive,
@@ -910,28 +954,28 @@ __device__ __forceinline__ void ws_body(const TileDesign& td, const Counts& c,
   constexpr int NE = (7 * P + 31) / 32;
   constexpr int NTH = ((P * (P + 1)) / 2 + 15) / 16;
   typedef WsLayout<NE, NTH> Lay;
-  constexpr int kWsWaves = ws_waves(P);
-  __shared__ __attribute__((aligned(16))) unsigned char ring[Lay::kY + kWsWaves * Lay::kYWave];
+  constexpr int kWsWaves = ws_waves(P, TPW);
+  __shared__ __attribute__((aligned(16)))
+  unsigned char ring[Lay::kY + kWsWaves * TPW * Lay::kYWave];
   const int nA = *nactive;
   const int ntile = (nA + 15) / 16;
   const int wave = threadIdx.x >> 6;
   int chunk, group;
   tile_coords(td.nchunk, chunk, group);
-  if (group * kWsWaves >= ntile) return;  // the whole workgroup
+  if (group * kWsWaves * TPW >= ntile) return;  // the whole workgroup
   const int64_t s0 = (int64_t)chunk * td.chunk_blocks;
   const int64_t s1 = max(s0, min((int64_t)(chunk + 1) * td.chunk_blocks, td.nblk));
   if (wave == kWsWaves) {
     ws_loader<NE, NTH>(td, s0, s1, ring);
     return;
   }
-  const int tile = group * kWsWaves + wave;
-  const bool tile_live = tile < ntile;
+  const int tile0 = (group * kWsWaves + wave) * TPW;
   if (c.dtype == 0)
-    ws_compute<P, MODE>(td, static_cast<const int32_t*>(c.ptr), c.ld, active, nA, beta, gpar,
-                        tile, tile_live, chunk, s0, s1, ring, part);
+    ws_compute<P, MODE, TPW>(td, static_cast<const int32_t*>(c.ptr), c.ld, active, nA, beta, gpar,
+                             tile0, ntile, chunk, s0, s1, ring, part);
   else
-    ws_compute<P, MODE>(td, static_cast<const float*>(c.ptr), c.ld, active, nA, beta, gpar,
-                        tile, tile_live, chunk, s0, s1, ring, part);
+    ws_compute<P, MODE, TPW>(td, static_cast<const float*>(c.ptr), c.ld, active, nA, beta, gpar,
+                             tile0, ntile, chunk, s0, s1, ring, part);
 }
 
 // Counts must be 4-byte elements in 16-byte aligned rows (the launchers fall
@@ -941,7 +985,18 @@ __global__ __launch_bounds__(64 * (ws_waves(P) + 1)) void k_irls_accum_ws(
     TileDesign td, Counts c, const int32_t* __restrict__ active,
     const int32_t* __restrict__ nactive, const double* __restrict__ beta,
     const double* __restrict__ kvec, double* __restrict__ part) {
-  ws_body<P, 0>(td, c, active, nactive, beta, kvec, part);
+  ws_body<P, 0, 1>(td, c, active, nactive, beta, kvec, part);
+}
+
+// The same with two 16-gene tiles per compute wave: half the waves, each
+// reading a step's tables once for 32 genes and holding two independent
+// chains of MFMA and VALU work.
+template <int P>
+__global__ __launch_bounds__(64 * (ws_waves(P, 2) + 1)) void k_irls_accum_ws2(
+    TileDesign td, Counts c, const int32_t* __restrict__ active,
+    const int32_t* __restrict__ nactive, const double* __restrict__ beta,
+    const double* __restrict__ kvec, double* __restrict__ part) {
+  ws_body<P, 0, 2>(td, c, active, nactive, beta, kvec, part);
 }
 
 // Mixed-precision bread partials: per (16-gene tile, chunk) the sums of
@@ -952,7 +1007,7 @@ __global__ __launch_bounds__(64 * (ws_waves(P) + 1)) void k_bread_accum_ws(
     TileDesign td, Counts c, const int32_t* __restrict__ genes,
     const int32_t* __restrict__ ngenes, const double* __restrict__ beta,
     const double* __restrict__ theta, double* __restrict__ part) {
-  ws_body<P, 1>(td, c, genes, ngenes, beta, theta, part);
+  ws_body<P, 1, 1>(td, c, genes, ngenes, beta, theta, part);
 }
 
 // bread = H^-1 with H[a][b] = the sum over chunks of the pair column of
@@ -1213,20 +1268,26 @@ hipError_t launch_irls_accum_bf16(hipStream_t s, const TileDesign& td, Counts c,
 
 hipError_t launch_irls_accum_ws(hipStream_t s, const TileDesign& td, Counts c, int p,
                                 const int32_t* active, const int32_t* nactive, int nA_bound,
-                                const double* beta, const double* kvec, double* part) {
+                                const double* beta, const double* kvec, double* part, int tpw) {
   if (nA_bound <= 0) return hipSuccess;
   const int ntile = (nA_bound + 15) / 16;
   const bool vec4 = c.dtype != 2 && (c.ld & 3) == 0 && (reinterpret_cast<uintptr_t>(c.ptr) & 15) == 0;
   if (!vec4) return launch_irls_accum_bf16(s, td, c, p, active, nactive, nA_bound, beta, kvec, part);
-  const int nw = ws_waves(p);
-  const int wgroups = (ntile + nw - 1) / nw;
+  const int nw = ws_waves(p, tpw);
+  const int per_group = nw * tpw;
+  const int wgroups = (ntile + per_group - 1) / per_group;
   const dim3 grid((unsigned)((int64_t)wgroups * td.nchunk));
-  DV_TILE_P(p, k_irls_accum_ws, grid, dim3(64 * (nw + 1)), s, td, c, active, nactive, beta, kvec,
-            part);
+  if (tpw == 2) {
+    DV_TILE_P(p, k_irls_accum_ws2, grid, dim3(64 * (nw + 1)), s, td, c, active, nactive, beta,
+              kvec, part);
+  } else {
+    DV_TILE_P(p, k_irls_accum_ws, grid, dim3(64 * (nw + 1)), s, td, c, active, nactive, beta,
+              kvec, part);
+  }
   return hipGetLastError();
 }
 
-int ws_tiles_per_workgroup(int p) { return ws_waves(p); }
+int ws_tiles_per_workgroup(int p, int tpw) { return ws_waves(p, tpw) * tpw; }
 
 hipError_t launch_merge_flags(hipStream_t s, int32_t* status, const int32_t* flag, int64_t G,
                               int32_t bit) {

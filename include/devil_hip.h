@@ -194,17 +194,22 @@ enum { DEVIL_TUNE_MIXED_KERNEL = 0,  /* DEVIL_MIXED_* (default AUTO) */
        DEVIL_TUNE_CONCURRENCY = 2,   /* streams for multi-problem fits (default 4) */
        DEVIL_N_TUNE = 3 };
 enum { DEVIL_MIXED_F32 = 0,   /* eta and all contractions on v_mfma_f32_16x16x4f32 */
-       DEVIL_MIXED_BF16 = 1,  /* eta (split operands) and pair columns on
-                                 v_mfma_f32_16x16x32_bf16, design columns FP32 */
+       DEVIL_MIXED_BF16 = 1,  /* eta, design columns (six products of three-piece
+                                 splits) and pair columns (three products) on
+                                 v_mfma_f32_16x16x32_bf16 */
        DEVIL_MIXED_AUTO = 2,  /* autotuner: times the candidates on the problem's
                                  first genes once per (p, n, count type) and keeps
                                  the fastest for the context */
-       DEVIL_MIXED_BF16_WS = 3 /* DEVIL_MIXED_BF16 arithmetic on wave-specialised
+       DEVIL_MIXED_BF16_WS = 3, /* DEVIL_MIXED_BF16 arithmetic on wave-specialised
                                   workgroups: a loader wavefront stages the design
                                   tables in LDS, the compute wavefronts stage their
                                   counts in LDS three steps ahead (direct-to-LDS
                                   loads); 4-byte counts in 16-byte aligned rows,
-                                  else DEVIL_MIXED_BF16 runs */ };
+                                  else DEVIL_MIXED_BF16 runs */
+       DEVIL_MIXED_BF16_WS2 = 4 /* DEVIL_MIXED_BF16_WS with two 16-gene tiles per
+                                   compute wavefront (half the wavefronts; same
+                                   results bit for bit); not among the autotuner's
+                                   candidates */ };
 int devil_set_tuning(devil_ctx *ctx, int32_t key, int64_t value);
 
 /* Per-launch timing of the IRLS accumulate kernels (the streaming pass over

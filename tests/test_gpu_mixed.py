@@ -52,10 +52,10 @@ def relmax(a, b, floor=FLOOR):
 
 # per-variant stated tolerance on beta and log10 p (f32 is the default kernel;
 # the bf16 kernel rounds the pair-column products to bf16 and is looser)
-KERNEL_TOL = {"f32": TOL, "bf16": TOL, "bf16_ws": TOL}
+KERNEL_TOL = {"f32": TOL, "bf16": TOL, "bf16_ws": TOL, "bf16_ws2": TOL}
 
 
-@pytest.fixture(scope="module", params=["f32", "bf16", "bf16_ws"])
+@pytest.fixture(scope="module", params=["f32", "bf16", "bf16_ws", "bf16_ws2"])
 def dv(request):
     """The package with the context's mixed kernel set to each variant."""
     if not torch.cuda.is_available():

@@ -6,7 +6,9 @@ that holds only a row's partial last step (n = 5128 with 1024-cell chunks),
 rows without a partial step (n = 6144), a partial step inside a longer chunk
 (n = 5000), 16-byte-misaligned rows (n = 5001: the launcher falls back to the
 bf16 kernel), workgroups with idle tiles (G not a multiple of the tiles per
-workgroup), float counts, and p = 4 and 8.  The same kernel computes the
+workgroup; 17 genes leave the second tile of a two-tile wave idle), float
+counts, and p = 4 and 8, for the kernel with one and with two 16-gene tiles per
+compute wave.  The same kernel computes the
 mixed path's bread where there are no clusters (test_mixed_bread_matches_fp64).
 """
 import sys
@@ -45,25 +47,27 @@ def fit_with(dv, kernel, Y, w):
     return fit, info
 
 
+@pytest.mark.parametrize("kernel", ["bf16_ws", "bf16_ws2"])
 @pytest.mark.parametrize("config,G,n,dtype", [
     ("C3", 487, 5128, "int32"),
     ("C3", 487, 6144, "int32"),
     ("C2", 300, 5000, "int32"),
     ("C3", 250, 5001, "int32"),
     ("C2", 487, 5128, "float32"),
+    ("C3", 17, 6144, "int32"),
 ])
-def test_ws_equals_bf16(dv, config, G, n, dtype):
+def test_ws_equals_bf16(dv, config, G, n, dtype, kernel):
     from devil_amd import synth
     w = synth.make_workload(config, G=G, n=n, seed=4242 + n)
     Y = synth.counts_device(w, device="cuda")
     if dtype == "float32":
         Y = Y.float()
-    say(f"[case] {config} G={G} n={n} p={w.p} {dtype}")
+    say(f"[case] {kernel} {config} G={G} n={n} p={w.p} {dtype}")
     a, ia = fit_with(dv, "bf16", Y, w)
-    b, ib = fit_with(dv, "bf16_ws", Y, w)
+    b, ib = fit_with(dv, kernel, Y, w)
     say(f"  kernels {ia['mixed_kernel']} / {ib['mixed_kernel']}, replayed {ia['replayed']} / "
         f"{ib['replayed']}")
-    assert ia["mixed_kernel"] == "bf16" and ib["mixed_kernel"] == "bf16_ws"
+    assert ia["mixed_kernel"] == "bf16" and ib["mixed_kernel"] == kernel
     for key in ("beta", "overdispersion"):
         x, y = a[key].cpu().numpy(), b[key].cpu().numpy()
         same = (x == y) | (np.isnan(x) & np.isnan(y))
