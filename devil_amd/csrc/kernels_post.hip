@@ -52,20 +52,26 @@ __device__ __forceinline__ void mom_tile_chunk(const TileDesign& td, ChunkTable 
   const int64_t b0 = c0 / 16, b1 = (c1 + 15) / 16;
   double num = 0.0, den = 0.0;
   typedef TileBlock<NJ, 0, CT> Blk;  // off holds exp(offset)
-  auto body = [&](const Blk& cur, int64_t b) {
+  // blocks inside [c0, c1) skip the per-cell range test
+  auto body_as = [&](const Blk& cur, int64_t b, auto inner) {
+    constexpr bool kInner = decltype(inner)::value;
     d4 eta = d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int j = 0; j < NJ; ++j) eta = mfma16x16x4(cur.xa[j], bpos[j], eta);
     const int64_t cb = b * 16 + 4 * h;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      if (cb + r >= c0 && cb + r < c1) {
+      if (kInner || (cb + r >= c0 && cb + r < c1)) {
         const double mu = cur.off[r] * exp_tab(eta[r], e2);
         const double d = (double)cur.y[r] - mu;
         num += d * d - mu;
         den += mu * mu;
       }
     }
+  };
+  auto body = [&](const Blk& cur, int64_t b) {
+    if (b * 16 >= c0 && b * 16 + 16 <= c1) body_as(cur, b, std::true_type{});
+    else body_as(cur, b, std::false_type{});
   };
   auto load = [&](int64_t b, Blk& o) {
     load_tile_block<NJ, 0, CT, VEC>(td, yh, b, lane, h, 0, td.eop, o);
@@ -380,15 +386,18 @@ __device__ __forceinline__ void init_tile_chunk(const TileDesign& td, InitTile i
     load_tables(b, o);
     load4_tail<CT>(yrow, b * 16 + 4 * h, td.n, o.y);
   };
-  auto body = [&](const Blk& o, int64_t b) {
+  // blocks inside [c0, c1) (all but a chunk's first and last) skip the
+  // per-cell range test; log1p(0) = 0 exactly, so zero counts need no branch
+  auto body_as = [&](const Blk& o, int64_t b, auto inner) {
+    constexpr bool kInner = decltype(inner)::value;
     const int64_t cb = b * 16 + 4 * h;
     double z[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const bool in = cb + r >= c0 && cb + r < c1;
+      const bool in = kInner || (cb + r >= c0 && cb + r < c1);
       const double y = in ? (double)o.y[r] : 0.0;
       if constexpr (ZM == 1) {
-        z[r] = y != 0.0 ? log1p_tab(y * o.aux[r], ltab) : 0.0;
+        z[r] = log1p_tab(y * o.aux[r], ltab);
       } else if constexpr (ZM == 2) {
         z[r] = in ? log_tab(y + 0.5, 0.0, ltab) - o.aux[r] : 0.0;
       } else {
@@ -407,6 +416,10 @@ __device__ __forceinline__ void init_tile_chunk(const TileDesign& td, InitTile i
 #pragma unroll
       for (int r = 0; r < 4; ++r) az = mfma16x16x4(z[r], o.zb[r], az);
     }
+  };
+  auto body = [&](const Blk& o, int64_t b) {
+    if (b * 16 >= c0 && b * 16 + 16 <= c1) body_as(o, b, std::true_type{});
+    else body_as(o, b, std::false_type{});
   };
   pipeline<Blk>(b0, min(b1, td.n / 16), b1, load, load_tail, body);
 #pragma unroll
