@@ -698,7 +698,7 @@ int make_tile(devil_ctx* ctx, const double* Xd, const double* offd, const double
     shape_design(cm, n, p, 32, tm, chunk_cells);
     if ((rc = upload(ctx, th.specm, cm.spec))) return rc;
     const int64_t nst = tm.nblk;
-    const int NE = (7 * p + 31) / 32;
+    const int NE = dv::eta_blocks(p);
     const int nth = tm.nt - 1;
     DV_ALLOC(th.xab, sizeof(uint16_t) * nst * 2 * NE * 64 * 8);
     DV_ALLOC(th.offl, sizeof(float) * nst * 32);

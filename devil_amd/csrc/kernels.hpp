@@ -32,6 +32,15 @@ struct InitParams {
 
 // Design laid out for the MFMA IRLS kernels (kernels_tile.hip).
 #define DEVIL_NCP_MAX 80  // 16 * ceil((10*11/2 + 10) / 16)
+// Entries per design column of the bf16 eta operands: x and log2(e) beta
+// each in three bf16 pieces (each rounded from what the earlier leave), and
+// the six products of total order <= 2: (x1 b1, x1 b2, x1 b3, x2 b1, x2 b2,
+// x3 b1).  What is dropped and the split residues are below 2^-25 |x beta|.
+constexpr int kEtaTerms = 6;
+__host__ __device__ constexpr int eta_blocks(int p) { return (kEtaTerms * p + 31) / 32; }
+__host__ __device__ constexpr int eta_x_piece(int t) { return t < 3 ? 0 : (t < 5 ? 1 : 2); }
+__host__ __device__ constexpr int eta_b_piece(int t) { return t < 3 ? t : (t < 5 ? t - 3 : 0); }
+
 struct TileDesign {
   int64_t n;             // cells
   int64_t nblk;          // 16-cell blocks (n padded to a multiple of 16)
@@ -52,8 +61,8 @@ struct TileDesign {
   // design columns (FP32 products), tiles 1.. the pair columns (bf16 products);
   // c(h, i) = i < 4 ? 4 h + i : 16 + 4 h + i - 4 is the cell of k index (h, i)
   const uint16_t* xab;   // [st][u][j][lane][8] bf16 eta entries 32 j + 8 (lane/16) + i of
-                         // cell 32 st + 16 u + lane%16: per feature f, entries 7 f + t
-                         // hold (x1, x1, x1, x1, x2, x2, x2), x = x1 + x2
+                         // cell 32 st + 16 u + lane%16: per feature f, entries 6 f + t
+                         // hold (x1, x1, x1, x2, x2, x3), x = x1 + x2 + x3 (eta_pieces)
   const float* offl;     // off * log2(e), padded with zeros to 32 nblk
   const uint16_t* kg;    // [st][q][lane][8] bf16 piece q of X[32 st + c(lane/16, i)][lane%16]:
                          // x = x1 + x2 + x3, each piece rounded from what the earlier leave

@@ -73,20 +73,21 @@ __global__ void k_build_tables_bf16(const double* __restrict__ X, const double* 
   if (id >= nst * 64) return;
   const int64_t st = id / 64;
   const int l = (int)(id % 64);
-  const int NE = (7 * p + 31) / 32;
+  const int NE = eta_blocks(p);
   const double L2E = 1.44269504088896340736;
   for (int u = 0; u < 2; ++u)
     for (int j = 0; j < NE; ++j)
       for (int i = 0; i < 8; ++i) {
         const int64_t cell = st * 32 + 16 * u + l % 16;
         const int e = 32 * j + 8 * (l / 16) + i;
-        const int f = e / 7, term = e % 7;
+        const int f = e / kEtaTerms, term = e % kEtaTerms;
         uint16_t v = 0;
         if (cell < n && f < p) {
-          const double x = X[cell + f * n];
-          const uint16_t x1 = bf16_rne((float)x);
-          const uint16_t x2 = bf16_rne((float)(x - (double)__uint_as_float((uint32_t)x1 << 16)));
-          v = term < 4 ? x1 : x2;
+          double rem = X[cell + f * n];
+          for (int q = 0; q <= eta_x_piece(term); ++q) {  // x = x1 + x2 + x3
+            v = bf16_rne((float)rem);
+            rem -= (double)__uint_as_float((uint32_t)v << 16);
+          }
         }
         xab[(((st * 2 + u) * NE + j) * 64 + l) * 8 + i] = v;
       }

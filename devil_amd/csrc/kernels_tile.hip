@@ -281,9 +281,10 @@ __global__ __launch_bounds__(256) void k_irls_accum_mixed(TileDesign td, Counts 
 // A step is 32 cells (two 16-cell halves); lane l = (gene slot s = l % 16,
 // group h = l / 16) owns cells 4h + r and 16 + 4h + r, r < 4.
 //   eta: per half, sum over split entries of A (cells x 32 entries) B (entries
-//        x genes): x = x1 + x2 and log2(e) beta = b1 + b2 + b3 + b4 in bf16,
-//        entries (x1 b1, x1 b2, x1 b3, x1 b4, x2 b1, x2 b2, x2 b3) per feature,
-//        FP32 accumulation; register r holds cell 4h + r of the half.
+//        x genes): x = x1 + x2 + x3 and log2(e) beta = b1 + b2 + b3 in bf16,
+//        entries (x1 b1, x1 b2, x1 b3, x2 b1, x2 b2, x3 b1) per feature (below
+//        2^-25 |x beta| from the product), FP32 accumulation; register r holds
+//        cell 4h + r of the half.
 //   r = (y - mu) / (k + mu), mu = 2^(eta + off log2 e)             (FP32 VALU)
 //   gradient tile (the p design columns): r' X from r = r1 + r2 + r3 and
 //        X = X1 + X2 + X3 in bf16, the six products of total order <= 2
@@ -460,7 +461,7 @@ __device__ __forceinline__ void load_bf_k(const TileDesign& td, int64_t st, int 
   }
 }
 
-// B operand of eta for gene slot s: entries 32 j + 8 h + i, entry e = 7 f + term
+// B operand of eta for gene slot s: entries 32 j + 8 h + i, entry e = 6 f + term
 // holding the bf16 piece of log2(e) beta_f that term selects (see above).
 template <int P, int NE>
 __device__ __forceinline__ void bf_beta_operand(const double* __restrict__ beta, int g, bool live,
@@ -470,11 +471,11 @@ __device__ __forceinline__ void bf_beta_operand(const double* __restrict__ beta,
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
       const int e = 32 * j + 8 * h + i;
-      const int f = e / 7, term = e % 7;
+      const int f = e / kEtaTerms, term = e % kEtaTerms;
       float v = 0.0f;
       if (live && f < P) {
         double rem = beta[(int64_t)g * P + f] * 1.44269504088896340736;
-        const int want = term < 4 ? term : term - 4;
+        const int want = eta_b_piece(term);
         float piece = 0.0f;
         for (int q = 0; q <= want; ++q) {
           piece = (float)(__bf16)(float)rem;
@@ -533,7 +534,7 @@ __device__ __forceinline__ void irls_tile_chunk_bf(const TileDesign& td, const C
                                                    int nA, const double* __restrict__ beta,
                                                    const double* __restrict__ kvec, int tile,
                                                    int chunk, double* __restrict__ part) {
-  constexpr int NE = (7 * P + 31) / 32;
+  constexpr int NE = eta_blocks(P);
   constexpr int NTH = ((P * (P + 1)) / 2 + 15) / 16;  // pair tiles at most
   constexpr int kSpan = 4;                            // steps per FP32 span (gradient tile)
   const int lane = threadIdx.x & 63;
@@ -644,7 +645,7 @@ typedef __attribute__((address_space(3))) void* lds_ptr_t;
 // (a workgroup of 1024 threads).  Table ring: 3 slots of (2 NE + 3 + 2 NTH)
 // KB + 128 B; each compute wave: 4 count slots of 2 KB.
 __host__ __device__ constexpr int ws_waves(int p, int tpw = 1) {
-  const int ne = (7 * p + 31) / 32, nth = (p * (p + 1) / 2 + 15) / 16;
+  const int ne = eta_blocks(p), nth = (p * (p + 1) / 2 + 15) / 16;
   const int tables = 3 * ((2 * ne + 3 + 2 * nth) * 1024 + 128);
   const int fit = (160 * 1024 - tables) / (tpw * 4 * 2048);
   return fit < 15 / tpw ? fit : 15 / tpw;
@@ -747,7 +748,7 @@ __device__ __forceinline__ void ws_compute(const TileDesign& td, const CT* __res
                                            const double* __restrict__ kvec, int tile0,
                                            int ntile, int chunk, int64_t s0, int64_t s1,
                                            unsigned char* ring, double* __restrict__ part) {
-  constexpr int NE = (7 * P + 31) / 32;
+  constexpr int NE = eta_blocks(P);
   constexpr int NTH = ((P * (P + 1)) / 2 + 15) / 16;
   constexpr int kSpan = 4;
   typedef WsLayout<NE, NTH> Lay;
@@ -951,7 +952,7 @@ __device__ __forceinline__ void ws_body(const TileDesign& td, const Counts& c,
                                         const double* __restrict__ beta,
                                         const double* __restrict__ gpar,
                                         double* __restrict__ part) {
-  constexpr int NE = (7 * P + 31) / 32;
+  constexpr int NE = eta_blocks(P);
   constexpr int NTH = ((P * (P + 1)) / 2 + 15) / 16;
   typedef WsLayout<NE, NTH> Lay;
   constexpr int kWsWaves = ws_waves(P, TPW);
