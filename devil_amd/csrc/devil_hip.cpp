@@ -1840,15 +1840,16 @@ int devil_test_de(devil_ctx* ctx, const double* beta, const double* bread,
     return fail(ctx, DEVIL_E_ARG, "NULL pointer or G < 0");
   if (G == 0) return DEVIL_OK;
   hipStream_t s = ctx->stream;
-  DevBuf bg, rank;
+  DevBuf bg, bh;
   DV_ALLOC(bg, sizeof(double) * G * p);
-  DV_ALLOC(rank, sizeof(int32_t) * G);
+  const size_t bh_bytes = dv::bh_scratch_bytes(G);
+  DV_ALLOC(bh, bh_bytes);
   ctx->stage_ms[DEVIL_STAGE_TEST_DE] = 0.0;
   StageTimer tm(ctx);
   hipEvent_t ev = tm.begin(DEVIL_STAGE_TEST_DE);
   DV_HIP(dv::launch_transpose_gp(s, beta, G, p, 1, bg.as<double>()));
   DV_HIP(dv::launch_test_de(s, bg.as<double>(), bread, sandwich, G, p, (double)n - 2.0,
-                            contrast, max_lfc, pval, log_pval, lfc, rank.as<int32_t>(),
+                            contrast, max_lfc, pval, log_pval, lfc, bh.p, bh_bytes,
                             adj_pval));
   tm.end(ev);
   DV_HIP(hipStreamSynchronize(s));

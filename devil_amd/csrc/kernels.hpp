@@ -247,8 +247,14 @@ hipError_t launch_mle(hipStream_t s, Counts c, int64_t G, int64_t n, int p,
 hipError_t launch_test_de(hipStream_t s, const double* beta, const double* bread,
                           const double* sandwich, int64_t G, int p, double df,
                           const double* contrast, double max_lfc, double* pval,
-                          double* logp, double* lfc, int32_t* rank_scratch,
+                          double* logp, double* lfc, void* bh_scratch, size_t bh_bytes,
                           double* adj);
+
+// Benjamini-Hochberg adjustment of pval (device) into adj through a device
+// sort; scratch of bh_scratch_bytes(G) bytes.
+size_t bh_scratch_bytes(int64_t G);
+hipError_t launch_bh(hipStream_t s, const double* pval, int64_t G, double* adj, void* scratch,
+                     size_t scratch_bytes);
 
 // G x p column-major <-> gene-major (to_gene_major = 1: col-major -> gene-major)
 hipError_t launch_transpose_gp(hipStream_t s, const double* src, int64_t G, int p,
