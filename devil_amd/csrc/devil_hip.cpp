@@ -901,6 +901,10 @@ constexpr int64_t kMixedMinCells = 4096;
 constexpr int64_t kRoundWorkgroups = 1024;
 constexpr int64_t kMaxRoundChunks = 512;
 constexpr int64_t kMinRoundBlocks = 8;
+// rounds of at most kSplitTiles tiles with more than kSplitSlices chunks sum
+// their partials in kSplitSlices slices per tile first (k_irls_reduce)
+constexpr int kSplitTiles = 16;
+constexpr int kSplitSlices = 64;
 
 dv::TileDesign round_shape(const dv::TileDesign& td, int nA, int tiles_per_wg,
                            size_t part_cap) {
@@ -955,6 +959,8 @@ int irls_rounds(devil_ctx* ctx, const TileHost& tl, bool mixed, dv::Counts c, in
   int32_t* ncur = ncnt;
   int32_t* nnxt = ncnt_spare;
   volatile int32_t* slot = ctx->round_counts;
+  DevBuf red;
+  DV_ALLOC(red, sizeof(double) * kSplitTiles * kSplitSlices * 16 * tfull.ncp);
   std::vector<int> count{nA};       // count[k]: genes of round k's list
   std::vector<long> rec;            // round k's launch record (-1: none)
   int k = 0;
@@ -976,9 +982,20 @@ int irls_rounds(devil_ctx* ctx, const TileHost& tl, bool mixed, dv::Counts c, in
     }
     rec.push_back(ctx->launches.size() > nrec ? (long)nrec : -1);
     DV_HIP(dv::launch_fill_i32(s, nnxt, 1, 0));
-    DV_HIP(dv::launch_irls_update(s, td, p, cur, ncur, bound, part, beta, kvec, max_iter, eps,
-                                  iters, status, nxt, nnxt, mixed ? kMixedBand : 0.0, replay,
-                                  nreplay));
+    const int ntile_b = (bound + 15) / 16;
+    if (ntile_b <= kSplitTiles && td.nchunk > kSplitSlices) {
+      DV_HIP(dv::launch_irls_reduce(s, td.nchunk, td.ncp, kSplitSlices, ntile_b, ncur, part,
+                                    red.as<double>()));
+      dv::TileDesign tu = td;
+      tu.nchunk = kSplitSlices;
+      DV_HIP(dv::launch_irls_update(s, tu, p, cur, ncur, bound, red.as<double>(), beta, kvec,
+                                    max_iter, eps, iters, status, nxt, nnxt,
+                                    mixed ? kMixedBand : 0.0, replay, nreplay));
+    } else {
+      DV_HIP(dv::launch_irls_update(s, td, p, cur, ncur, bound, part, beta, kvec, max_iter,
+                                    eps, iters, status, nxt, nnxt, mixed ? kMixedBand : 0.0,
+                                    replay, nreplay));
+    }
     DV_HIP(hipMemcpyAsync(ctx->round_counts + (k & 1), nnxt, sizeof(int32_t),
                           hipMemcpyDeviceToHost, s));
     DV_HIP(hipEventRecord(ctx->round_ev[k & 1], s));

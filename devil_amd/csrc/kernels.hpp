@@ -180,6 +180,11 @@ hipError_t launch_restart_genes(hipStream_t s, const int32_t* list, int n, int p
                                 const double* beta0, double* beta, int32_t* iters,
                                 int32_t* status);
 
+// Sums the chunk partials of each active tile into R slices per tile
+// (out[tile][R][16][ncp]); for rounds of few tiles and many chunks.
+hipError_t launch_irls_reduce(hipStream_t s, int nchunk, int ncp, int R, int ntile_bound,
+                              const int32_t* nactive, const double* part, double* out);
+
 hipError_t launch_irls_update(hipStream_t s, const TileDesign& td, int p,
                               const int32_t* active, const int32_t* nactive, int nA_bound,
                               const double* part, double* beta, const double* kvec,

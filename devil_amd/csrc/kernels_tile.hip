@@ -1316,6 +1316,45 @@ hipError_t launch_bread_mixed(hipStream_t s, const TileDesign& td, Counts c, int
   return hipGetLastError();
 }
 
+// Split reduction of a round's chunk partials for rounds of few tiles (many
+// chunks each): workgroup (tile, r) sums chunks [r L, (r + 1) L) of the
+// tile's 16 x ncp partials into out[tile][r][16][ncp], L = ceil(nchunk / R),
+// every thread's loads independent of each other.  The update then sums the
+// R slices (a design with nchunk = R over out).
+constexpr int kRedThreads = 256;
+
+__global__ __launch_bounds__(kRedThreads) void k_irls_reduce(int nchunk, int ncp, int R,
+                                                             const int32_t* __restrict__ nactive,
+                                                             const double* __restrict__ part,
+                                                             double* __restrict__ out) {
+  const int ntile = (*nactive + 15) / 16;
+  const int tile = blockIdx.x / R, r = blockIdx.x % R;
+  if (tile >= ntile) return;
+  const int L = (nchunk + R - 1) / R;
+  const int c0 = r * L, c1 = min(nchunk, c0 + L);
+  const int64_t stride = (int64_t)16 * ncp;
+  const double* src = part + (int64_t)tile * nchunk * stride;
+  double* dst = out + ((int64_t)tile * R + r) * stride;
+  for (int idx = threadIdx.x; idx < 16 * ncp; idx += kRedThreads) {
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    int ch = c0;
+    for (; ch + 4 <= c1; ch += 4) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) a[u] += src[(int64_t)(ch + u) * stride + idx];
+    }
+    for (; ch < c1; ++ch) a[0] += src[(int64_t)ch * stride + idx];
+    dst[idx] = (a[0] + a[1]) + (a[2] + a[3]);
+  }
+}
+
+hipError_t launch_irls_reduce(hipStream_t s, int nchunk, int ncp, int R, int ntile_bound,
+                              const int32_t* nactive, const double* part, double* out) {
+  if (ntile_bound <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_irls_reduce, dim3((unsigned)(ntile_bound * R)), dim3(kRedThreads), 0, s,
+                     nchunk, ncp, R, nactive, part, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_irls_update(hipStream_t s, const TileDesign& td, int p,
                               const int32_t* active, const int32_t* nactive, int nA_bound,
                               const double* part, double* beta, const double* kvec,
