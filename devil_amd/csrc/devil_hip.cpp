@@ -59,7 +59,7 @@ struct devil_ctx {
   BlockCache cache;
   std::string err;
   int timing = 0;
-  int tune[DEVIL_N_TUNE] = {DEVIL_MIXED_AUTO, 0, 4};  // kernel, chunk cells (0 auto), streams
+  int tune[DEVIL_N_TUNE] = {DEVIL_MIXED_AUTO, 0, 8};  // kernel, chunk cells (0 auto), streams
   std::map<uint64_t, int> tuned;      // autotuner choices by problem shape
   std::vector<devil_ctx*> subs;       // per-stream contexts of multi-problem fits
   int64_t replayed = 0;                // genes refit in FP64 by the mixed path (last fit)

@@ -191,7 +191,7 @@ int devil_stage_times(const devil_ctx *ctx, double *ms);
  * defaults are what the autotuner found best on gfx950). */
 enum { DEVIL_TUNE_MIXED_KERNEL = 0,  /* DEVIL_MIXED_* (default AUTO) */
        DEVIL_TUNE_CHUNK_CELLS = 1,   /* cells per chunk of the tile passes, 0 = autotune */
-       DEVIL_TUNE_CONCURRENCY = 2,   /* streams for multi-problem fits (default 4) */
+       DEVIL_TUNE_CONCURRENCY = 2,   /* streams for multi-problem fits (default 8) */
        DEVIL_N_TUNE = 3 };
 enum { DEVIL_MIXED_F32 = 0,   /* eta and all contractions on v_mfma_f32_16x16x4f32 */
        DEVIL_MIXED_BF16 = 1,  /* eta, design columns (six products of three-piece
