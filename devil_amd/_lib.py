@@ -146,7 +146,7 @@ class Context:
         """Genes refit in FP64 by the last mixed fit, and its kernel and chunk
         size (the autotuner's choice when tuning was left open)."""
         out = {}
-        for k, name in enumerate(("replayed", "mixed_kernel", "chunk_cells")):
+        for k, name in enumerate(("replayed", "mixed_kernel", "chunk_cells", "polished")):
             v = C.c_int64()
             self.check(lib().devil_fit_info(self.h, k, C.byref(v)))
             out[name] = int(v.value)

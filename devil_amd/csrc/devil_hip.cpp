@@ -63,7 +63,9 @@ struct devil_ctx {
   std::map<uint64_t, int> tuned;      // autotuner choices by problem shape
   std::vector<devil_ctx*> subs;       // per-stream contexts of multi-problem fits
   int64_t replayed = 0;                // genes refit in FP64 by the mixed path (last fit)
+  int64_t polished = 0;                // genes whose last mixed step was redone in FP64
   int64_t replay_sum = 0;
+  int64_t polish_sum = 0;
   int last_mk = 0;                     // mixed kernel and chunk cells of the last fit
   int64_t last_cc = 0;
   int64_t k_sum_n[DEVIL_N_KERNELS] = {0, 0};
@@ -890,6 +892,9 @@ void collect_launches(devil_ctx* ctx) {
 // threshold than the mixed trajectory's accuracy (~1e-8 in the step) are
 // therefore always FP64 decisions on an FP64 trajectory.
 constexpr double kMixedBand = 1e-3;
+// mixed genes with a coefficient below this magnitude get their last step
+// redone in FP64
+constexpr double kPolishBelow = 5e-3;
 constexpr int64_t kMixedMinCells = 4096;
 
 // Chunking of one round over nA active genes.  A round whose workgroups
@@ -947,7 +952,7 @@ int irls_rounds(devil_ctx* ctx, const TileHost& tl, bool mixed, dv::Counts c, in
                 double* beta, const double* kvec, int max_iter, double eps, int32_t* iters,
                 int32_t* status, int32_t* act, int32_t* act_spare, int32_t* ncnt,
                 int32_t* ncnt_spare, int nA, double* part, size_t part_cap, int32_t* replay,
-                int32_t* nreplay) {
+                int32_t* nreplay, double* bprev = nullptr) {
   const dv::TileDesign& tfull = mixed ? tl.tdm : tl.td;
   const int tiles_per_wg = accum_tiles_per_wg(mixed ? tl.mixed : kMixedNone, p, c);
   hipStream_t s = ctx->stream;
@@ -990,11 +995,11 @@ int irls_rounds(devil_ctx* ctx, const TileHost& tl, bool mixed, dv::Counts c, in
       tu.nchunk = kSplitSlices;
       DV_HIP(dv::launch_irls_update(s, tu, p, cur, ncur, bound, red.as<double>(), beta, kvec,
                                     max_iter, eps, iters, status, nxt, nnxt,
-                                    mixed ? kMixedBand : 0.0, replay, nreplay));
+                                    mixed ? kMixedBand : 0.0, replay, nreplay, bprev));
     } else {
       DV_HIP(dv::launch_irls_update(s, td, p, cur, ncur, bound, part, beta, kvec, max_iter,
                                     eps, iters, status, nxt, nnxt, mixed ? kMixedBand : 0.0,
-                                    replay, nreplay));
+                                    replay, nreplay, bprev));
     }
     DV_HIP(hipMemcpyAsync(ctx->round_counts + (k & 1), nnxt, sizeof(int32_t),
                           hipMemcpyDeviceToHost, s));
@@ -1046,26 +1051,52 @@ int run_irls_tiled(devil_ctx* ctx, const TileHost& tl, dv::Counts c, int64_t G, 
     return irls_rounds(ctx, tl, false, c, p, beta, kvec, max_iter, eps, iters, status,
                        act0.as<int32_t>(), act1.as<int32_t>(), c0, c0 + 1, (int)G,
                        part.as<double>(), part_cap, nullptr, nullptr);
+  DevBuf bprev;
   DV_ALLOC(rep, sizeof(int32_t) * G);
   DV_ALLOC(beta0, sizeof(double) * G * p);
+  DV_ALLOC(bprev, sizeof(double) * G * p);
   DV_HIP(hipMemcpyAsync(beta0.p, beta, sizeof(double) * G * p, hipMemcpyDeviceToDevice, s));
+  DV_HIP(dv::launch_fill_f64(s, bprev.as<double>(), G * p, std::nan("")));
   int32_t* nrep = c0 + 2;
   DV_HIP(dv::launch_fill_i32(s, nrep, 1, 0));
   if ((rc = irls_rounds(ctx, tl, true, c, p, beta, kvec, max_iter, eps, iters, status,
                         act0.as<int32_t>(), act1.as<int32_t>(), c0, c0 + 1, (int)G,
-                        part.as<double>(), part_cap, rep.as<int32_t>(), nrep)))
+                        part.as<double>(), part_cap, rep.as<int32_t>(), nrep,
+                        bprev.as<double>())))
     return rc;
   int32_t nr = 0;
   DV_HIP(hipMemcpyAsync(&nr, nrep, sizeof(int32_t), hipMemcpyDeviceToHost, s));
   DV_HIP(hipStreamSynchronize(s));
   ctx->replayed += nr;
-  if (nr == 0) return DEVIL_OK;
-  // FP64 refit of the replay genes from their starting beta
-  DV_HIP(dv::launch_restart_genes(s, rep.as<int32_t>(), nr, p, beta0.as<double>(), beta, iters,
-                                  status));
-  return irls_rounds(ctx, tl, false, c, p, beta, kvec, max_iter, eps, iters, status,
-                     rep.as<int32_t>(), act1.as<int32_t>(), nrep, c0 + 3, nr,
-                     part.as<double>(), part_cap, nullptr, nullptr);
+  if (nr > 0) {
+    // FP64 refit of the replay genes from their starting beta
+    DV_HIP(dv::launch_restart_genes(s, rep.as<int32_t>(), nr, p, beta0.as<double>(), beta,
+                                    iters, status, bprev.as<double>()));
+    if ((rc = irls_rounds(ctx, tl, false, c, p, beta, kvec, max_iter, eps, iters, status,
+                          rep.as<int32_t>(), act1.as<int32_t>(), nrep, c0 + 3, nr,
+                          part.as<double>(), part_cap, nullptr, nullptr)))
+      return rc;
+  }
+  // The mixed trajectory leaves ~5e-7 absolute on beta; where a coefficient
+  // is small that is a large relative error.  Such genes get their last step
+  // redone in FP64 from the beta it started at (the stopping decisions and
+  // iteration counts are not touched).
+  int32_t* list = act0.as<int32_t>();
+  DV_HIP(dv::launch_fill_i32(s, c0, 1, 0));
+  DV_HIP(dv::launch_polish_select(s, G, p, beta, bprev.as<double>(), status, kPolishBelow, list,
+                                  c0));
+  int32_t np = 0;
+  DV_HIP(hipMemcpyAsync(&np, c0, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  DV_HIP(hipStreamSynchronize(s));
+  ctx->polished += np;
+  if (np == 0) return DEVIL_OK;
+  const dv::TileDesign td1 = round_shape(td, np, 4, part_cap);
+  DV_HIP(dv::launch_irls_accum(s, td1, c, p, list, c0, np, bprev.as<double>(), kvec,
+                               part.as<double>()));
+  DV_HIP(dv::launch_irls_update(s, td1, p, list, c0, np, part.as<double>(), beta, kvec, max_iter,
+                                eps, iters, status, act1.as<int32_t>(), c0 + 1, 0.0, nullptr,
+                                nullptr, bprev.as<double>(), 1));
+  return DEVIL_OK;
 }
 
 // MLE overdispersion over genes [0, G): setup pass, then rounds of
@@ -1410,6 +1441,7 @@ int fit_one(devil_ctx* ctx, const devil_problem* pr, devil_outputs* out) {
     ctx->k_ms[k] = ctx->k_bytes[k] = 0.0;
   }
   ctx->replayed = 0;
+  ctx->polished = 0;
   StageTimer tm(ctx);
   hipEvent_t ev;
   for (int64_t g0 = 0; g0 < G; g0 += batch) {
@@ -1521,6 +1553,7 @@ int devil_fit(devil_ctx* ctx, const devil_problem* probs, devil_outputs* outs,
     ctx->k_ms[k] = ctx->k_bytes[k] = 0.0;
   }
   ctx->replayed = 0;
+  ctx->polished = 0;
   if (n_problems == 1) return fit_one(ctx, probs, outs);
   const int S = std::max(1, std::min<int>(n_problems, ctx->tune[DEVIL_TUNE_CONCURRENCY]));
   while ((int)ctx->subs.size() < S) {
@@ -1558,6 +1591,7 @@ int devil_fit(devil_ctx* ctx, const devil_problem* probs, devil_outputs* outs,
         for (int k = 0; k < DEVIL_N_STAGES; ++k) st[k] = sub->stage_ms[k];
         rcs[j] = fit_one(sub, &probs[j], &outs[j]);
         sub->replay_sum += sub->replayed;
+        sub->polish_sum += sub->polished;
         // fit_one resets the sub-context's stage and kernel figures: keep sums
         for (int k = 0; k < DEVIL_N_STAGES; ++k) sub->stage_ms[k] += st[k];
         for (int k = 0; k < DEVIL_N_KERNELS; ++k) {
@@ -1582,7 +1616,9 @@ int devil_fit(devil_ctx* ctx, const devil_problem* probs, devil_outputs* outs,
     }
     for (auto& kv : sub->tuned) ctx->tuned[kv.first] = kv.second;
     ctx->replayed += sub->replay_sum;
+    ctx->polished += sub->polish_sum;
     sub->replay_sum = 0;
+    sub->polish_sum = 0;
   }
   for (int j = 0; j < n_problems; ++j)
     if (rcs[j] != DEVIL_OK) {
@@ -1630,6 +1666,7 @@ int devil_fit_info(const devil_ctx* ctx, int32_t key, int64_t* value) {
                                               : -1;
       return DEVIL_OK;
     case DEVIL_INFO_CHUNK_CELLS: *value = ctx->last_cc; return DEVIL_OK;
+    case DEVIL_INFO_POLISHED: *value = ctx->polished; return DEVIL_OK;
     default: return DEVIL_E_ARG;
   }
 }

@@ -178,7 +178,7 @@ hipError_t launch_merge_flags(hipStream_t s, int32_t* status, const int32_t* fla
 
 hipError_t launch_restart_genes(hipStream_t s, const int32_t* list, int n, int p,
                                 const double* beta0, double* beta, int32_t* iters,
-                                int32_t* status);
+                                int32_t* status, double* bprev = nullptr);
 
 // Sums the chunk partials of each active tile into R slices per tile
 // (out[tile][R][16][ncp]); for rounds of few tiles and many chunks.
@@ -190,7 +190,13 @@ hipError_t launch_irls_update(hipStream_t s, const TileDesign& td, int p,
                               const double* part, double* beta, const double* kvec,
                               int max_iter, double eps, int32_t* iters, int32_t* status,
                               int32_t* next, int32_t* nnext, double band = 0.0,
-                              int32_t* recheck = nullptr, int32_t* nrecheck = nullptr);
+                              int32_t* recheck = nullptr, int32_t* nrecheck = nullptr,
+                              double* bprev = nullptr, int polish = 0);
+hipError_t launch_fill_f64(hipStream_t s, double* x, int64_t n, double v);
+// genes to polish (see k_polish_select) appended to list / *nlist
+hipError_t launch_polish_select(hipStream_t s, int64_t G, int p, const double* beta,
+                                const double* bprev, const int32_t* status, double below,
+                                int32_t* list, int32_t* nlist);
 hipError_t launch_iota(hipStream_t s, int32_t* p, int n);
 
 // Per-gene state of the tiled MLE (kernels_mle.hip); gene-major device arrays.

@@ -1111,7 +1111,8 @@ __global__ __launch_bounds__(kUpdThreads) void k_irls_update(TileDesign td,
                                                     int32_t* __restrict__ next,
                                                     int32_t* __restrict__ nnext, double band,
                                                     int32_t* __restrict__ recheck,
-                                                    int32_t* __restrict__ nrecheck) {
+                                                    int32_t* __restrict__ nrecheck,
+                                                    double* __restrict__ bprev, int polish) {
   __shared__ double acc[DEVIL_NCP_MAX];
   __shared__ double slice[kUpdSlices][DEVIL_NCP_MAX];
   const int nA = *nactive;
@@ -1171,6 +1172,15 @@ __global__ __launch_bounds__(kUpdThreads) void k_irls_update(TileDesign td,
     if (isnan(d)) has_nan = true;
     else mx = fmax(mx, d);
   }
+  // polish: the partials were taken at the gene's beta before its last
+  // (mixed) step; that step is redone in FP64, decisions unchanged
+  if (polish) {
+    if (!has_nan) {
+#pragma unroll
+      for (int j = 0; j < P; ++j) beta[(int64_t)g * P + j] = bprev[(int64_t)g * P + j] + v[j];
+    }
+    return;
+  }
   // partials from the mixed-precision pass: a step whose stopping decision
   // could differ from FP64 arithmetic (max |delta| within band * eps of eps,
   // or NaN) is left undone; the gene leaves the mixed rounds and is refit in
@@ -1178,6 +1188,10 @@ __global__ __launch_bounds__(kUpdThreads) void k_irls_update(TileDesign td,
   if (band > 0.0 && (has_nan || fabs(mx - eps) <= band * eps)) {
     recheck[atomicAdd(nrecheck, 1)] = g;
     return;
+  }
+  if (bprev) {
+#pragma unroll
+    for (int j = 0; j < P; ++j) bprev[(int64_t)g * P + j] = beta[(int64_t)g * P + j];
   }
 #pragma unroll
   for (int j = 0; j < P; ++j) beta[(int64_t)g * P + j] += v[j];
@@ -1196,21 +1210,23 @@ __global__ __launch_bounds__(kUpdThreads) void k_irls_update(TileDesign td,
 // genes list[0..n): beta <- beta0, iteration count and status cleared
 __global__ void k_restart_genes(const int32_t* __restrict__ list, int n, int p,
                                 const double* __restrict__ beta0, double* __restrict__ beta,
-                                int32_t* __restrict__ iters, int32_t* __restrict__ status) {
+                                int32_t* __restrict__ iters, int32_t* __restrict__ status,
+                                double* __restrict__ bprev) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int64_t g = list[i];
   for (int j = 0; j < p; ++j) beta[g * p + j] = beta0[g * p + j];
+  if (bprev) bprev[g * p] = __builtin_nan("");  // refit in FP64: nothing to polish
   iters[g] = 0;
   status[g] = 0;
 }
 
 hipError_t launch_restart_genes(hipStream_t s, const int32_t* list, int n, int p,
                                 const double* beta0, double* beta, int32_t* iters,
-                                int32_t* status) {
+                                int32_t* status, double* bprev) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_restart_genes, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, list, n,
-                     p, beta0, beta, iters, status);
+                     p, beta0, beta, iters, status, bprev);
   return hipGetLastError();
 }
 
@@ -1360,11 +1376,48 @@ hipError_t launch_irls_update(hipStream_t s, const TileDesign& td, int p,
                               const double* part, double* beta, const double* kvec,
                               int max_iter, double eps, int32_t* iters, int32_t* status,
                               int32_t* next, int32_t* nnext, double band, int32_t* recheck,
-                              int32_t* nrecheck) {
+                              int32_t* nrecheck, double* bprev, int polish) {
   if (nA_bound <= 0) return hipSuccess;
   const dim3 grid((unsigned)nA_bound);
   DV_TILE_P(p, k_irls_update, grid, dim3(kUpdThreads), s, td, active, nactive, part, beta, kvec,
-            max_iter, eps, iters, status, next, nnext, band, recheck, nrecheck);
+            max_iter, eps, iters, status, next, nnext, band, recheck, nrecheck, bprev, polish);
+  return hipGetLastError();
+}
+
+__global__ void k_fill_f64(double* __restrict__ x, int64_t n, double v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = v;
+}
+
+hipError_t launch_fill_f64(hipStream_t s, double* x, int64_t n, double v) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, n, v);
+  return hipGetLastError();
+}
+
+// Genes whose last mixed step is redone in FP64: a saved pre-step beta (not
+// NaN: refit genes and genes never stepped keep NaN), no NaN / max-iteration
+// status, and some coefficient below `below` in magnitude (where the mixed
+// trajectory's ~5e-7 absolute error is a large relative one).
+__global__ void k_polish_select(int64_t G, int p, const double* __restrict__ beta,
+                                const double* __restrict__ bprev,
+                                const int32_t* __restrict__ status, double below,
+                                int32_t* __restrict__ list, int32_t* __restrict__ nlist) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= G) return;
+  if (isnan(bprev[g * p])) return;
+  if (status[g] & (DEVIL_GENE_NAN | DEVIL_GENE_MAXITER)) return;
+  bool small = false;
+  for (int j = 0; j < p; ++j) small = small || fabs(beta[g * p + j]) < below;
+  if (small) list[atomicAdd(nlist, 1)] = (int32_t)g;
+}
+
+hipError_t launch_polish_select(hipStream_t s, int64_t G, int p, const double* beta,
+                                const double* bprev, const int32_t* status, double below,
+                                int32_t* list, int32_t* nlist) {
+  if (G <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_polish_select, dim3((unsigned)((G + 255) / 256)), dim3(256), 0, s, G, p,
+                     beta, bprev, status, below, list, nlist);
   return hipGetLastError();
 }
 

@@ -224,7 +224,8 @@ int devil_kernel_stats(const devil_ctx *ctx, int32_t kernel, int64_t *launches, 
 /* Facts about the last devil_fit: genes the mixed path refit in FP64 (their
  * stopping step came within 1e-3 eps of the threshold, or was NaN), and the
  * mixed kernel (DEVIL_MIXED_*, -1 for FP64) and chunk size it ran with. */
-enum { DEVIL_INFO_REPLAYED = 0, DEVIL_INFO_MIXED_KERNEL = 1, DEVIL_INFO_CHUNK_CELLS = 2 };
+enum { DEVIL_INFO_REPLAYED = 0, DEVIL_INFO_MIXED_KERNEL = 1, DEVIL_INFO_CHUNK_CELLS = 2,
+       DEVIL_INFO_POLISHED = 3 /* genes whose last mixed IRLS step was redone in FP64 */ };
 int devil_fit_info(const devil_ctx *ctx, int32_t key, int64_t *value);
 
 #ifdef __cplusplus
