@@ -11,7 +11,8 @@ far below any coefficient's standard error), log10 p within 1e-4 relative
 relative (floor 1e-8), identical BH-adjusted p < 0.05 calls, and identical
 per-gene IRLS iteration counts (a gene whose step comes within 1e-3 * eps of
 the threshold is refit from its start in FP64, so decisions that close are
-FP64 decisions).
+FP64 decisions; a gene with a coefficient below 5e-3 in magnitude has its last
+step redone in FP64).
 
 Why the beta floor: FP32 element arithmetic leaves ~1e-7 relative noise per
 cell in the working residuals, which averages to an absolute beta error of
