@@ -203,3 +203,25 @@ def test_mixed_stage_entry_point(dv, orc):
         rb, ri = orc.beta_fit(Yf[g], w.X, b0[g], w.offset, a0[g], 200, 1e-3)
         assert ri == int(it[g])
         assert relmax(bn[g], rb, BETA_FLOOR) <= dv.mixed_tol
+
+
+def test_mixed_small_coefficients_c5(dv):
+    """Four problems of the C5 set (n = 6386 .. 39023, p = 3 .. 7) whose mixed fits
+    left 1.0-1.5e-4 relative error (floor 1e-3) on coefficients below 1e-3
+    before genes with small coefficients had their last step redone in FP64:
+    within the tolerance against the FP64 path, iteration counts equal."""
+    from devil_amd import synth
+    probs = synth.make_problems("C5")
+    ctx = dv.context(0)
+    for j in (10, 22, 23, 39):
+        w = probs[j]
+        say(f"[case] C5 problem {j}: G={w.G} n={w.n} p={w.p}")
+        Yd = synth.counts_device(w, "cuda")
+        f64 = dv.fit_devil(Yd, w.X, size_factors=w.sf, precision="fp64")
+        mix = dv.fit_devil(Yd, w.X, size_factors=w.sf, precision="mixed")
+        assert ctx.fit_info()["polished"] > 0
+        np.testing.assert_array_equal(mix["iterations"]["beta_iters"].cpu().numpy(),
+                                      f64["iterations"]["beta_iters"].cpu().numpy())
+        eb = relmax(mix["beta"].cpu().numpy(), f64["beta"].cpu().numpy(), BETA_FLOOR)
+        say(f"  beta {eb:.2e}")
+        assert eb <= TOL
