@@ -51,7 +51,7 @@ extern "C" {
 #define DEVIL_E_RANK 4         /* design matrix rank deficient (QR init) */
 #define DEVIL_E_CLUSTERS 5     /* cluster ends not increasing or last end != n */
 #define DEVIL_E_NODEVICE 6     /* no HIP device / device index out of range */
-#define DEVIL_E_UNSUPPORTED 7  /* valid request this build does not implement */
+#define DEVIL_E_UNSUPPORTED 7  /* valid request this library does not implement */
 
 /* per-gene status bits (devil_outputs.status) */
 #define DEVIL_GENE_MAXITER 1   /* beta loop stopped at max_iter (src/beta.cpp:37) */
@@ -60,7 +60,7 @@ extern "C" {
 #define DEVIL_GENE_MOM_DEN0 8  /* MOM denominator <= 0, theta = 0 */
 #define DEVIL_GENE_MLE_ZERO 16 /* MLE special case returned 0 (all-zero gene or score(1e-8) < 0) */
 
-#define DEVIL_PMAX 10          /* largest design width this build supports */
+#define DEVIL_PMAX 10          /* largest design width this library supports */
 
 enum { DEVIL_MEM_HOST = 0, DEVIL_MEM_DEVICE = 1 };
 enum { DEVIL_COUNTS_I32 = 0, DEVIL_COUNTS_F32 = 1, DEVIL_COUNTS_F64 = 2 };
