@@ -258,6 +258,11 @@ def test_bh_matches_oracle(dv, orc):
         res = dv.test_de(fit, contrast=[0, 1, 0])
         ref = orc.p_adjust_bh(res["pval"])
         np.testing.assert_array_equal(res["adj_pval"], ref)
+    # every p-value NaN (no non-NaN p-values: m = 0)
+    fit = {"beta": np.zeros((4, 2)), "beta_sandwiches_null": np.full((4, 2, 2), np.nan),
+           "beta_sandwiches": None, "design_matrix": np.zeros((50, 2))}
+    res = dv.test_de(fit, contrast=[0, 1])
+    assert np.all(np.isnan(res["pval"])) and np.all(np.isnan(res["adj_pval"]))
 
 
 def test_stage_entry_points(dv, orc):
