@@ -31,6 +31,27 @@ sys.path.insert(0, ROOT)
 
 METRIC = "gene×cell fits/s: mixed-precision vs FP64 GPU path at ≤1e-4 β/p rel err"
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_FLOPS = 2.5e15
+
+
+def hbm_copy_gbps(dev, nbytes=4 << 30, reps=10):
+    """HBM bandwidth of a plain device copy (torch's copy kernel) of nbytes:
+    bytes read + written per copy over its event-timed duration."""
+    import torch
+    a = torch.empty(nbytes // 4, dtype=torch.int32, device=dev)
+    b = torch.empty_like(a)
+    a.fill_(1)
+    b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    e1.synchronize()
+    gbps = 2.0 * nbytes * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del a, b
+    torch.cuda.empty_cache()
+    return gbps
 
 
 def log(msg):
@@ -52,36 +73,93 @@ def measured_traffic(kdesc, alg_bytes_per_launch):
     return meas[name]["traffic_per_algorithmic_byte"] * alg_bytes_per_launch
 
 
-def cpu_baseline(w, Yd, fit, n_threads, max_genes=None):
-    """The FP64 oracle on an evenly strided gene subsample (about 10-20 s of
-    host work), timed on the host; also checks the GPU fit on those genes."""
+def host_cpu():
+    """Host facts recorded with the CPU baseline: logical CPUs the OS reports,
+    the CPUs this process may run on, and the CPU model."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = os.cpu_count() or 1
+    return {"nproc": os.cpu_count() or 1, "affinity": allowed, "model": model}
+
+
+def rel_err(a, b, floor):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    e = np.abs(a - b) / np.maximum(np.abs(b), floor)
+    e[np.isnan(a) & np.isnan(b)] = 0.0
+    return float(np.max(e)) if e.size else 0.0
+
+
+def cpu_baseline(w, Yd, fits, n_threads, max_genes=None):
+    """The FP64 oracle on an evenly strided gene subsample (about 10 s of host
+    work on n_threads threads, plus a 1-thread run on an eighth of it), timed
+    on the host; the oracle's fit of the sample also checks every GPU fit in
+    ``fits`` ({precision: (fit, test_de result)}) gene by gene."""
     import oracle
-    target = max_genes or int(np.clip(round(4096 * 6.4e6 / (w.n * w.p * w.p)), 32, 4096))
+    cost = w.n * w.p * w.p * (4 if w.od == "MLE" else 1)
+    target = max_genes or int(np.clip(round(4096 * 6.4e6 / cost), 32, 4096))
     stride = math.ceil(w.G / target)
     genes = np.arange(0, w.G, stride)
     Ys = Yd[genes].cpu().numpy().astype(np.float64)
     od = {"MOM": oracle.OD_MOM, "MLE": oracle.OD_MLE}[w.od]
     ends = w.cluster_ends
+    contrast = np.zeros(w.p)
+    contrast[1] = 1.0
     log(f"[bench] cpu baseline: {len(genes)} genes x {w.n} cells, {n_threads} threads")
     t0 = time.perf_counter()
     ref = oracle.cpu_fit(Ys, w.X, w.offset, clusters_ends=ends, od_mode=od,
                          n_threads=n_threads)
+    rde = oracle.de_test(ref["beta"], ref["bread"], ref.get("sandwich"), w.n, contrast)
     t = time.perf_counter() - t0
-    beta = fit["beta"].cpu().numpy()[genes]
-    it = fit["iterations"]["beta_iters"].cpu().numpy()[genes]
-    rel = np.abs(beta - ref["beta"]) / np.maximum(np.abs(ref["beta"]), 1e-6)
-    th = fit["overdispersion"].cpu().numpy()[genes]
-    rel_th = np.abs(th - ref["theta"]) / np.maximum(np.abs(ref["theta"]), 1e-8)
-    worst = int(np.nanargmax(rel_th))
-    check = {"genes": int(len(genes)), "beta_max_rel_err": float(np.nanmax(rel)),
-             "theta_max_rel_err": float(np.nanmax(rel_th)),
-             "theta_worst_gene": {"gene": int(genes[worst]), "gpu": float(th[worst]),
-                                  "oracle": float(ref["theta"][worst])},
-             "iter_mismatch": int(np.sum(it != ref["beta_iters"]))}
+    g1 = genes[::8]
+    log(f"[bench] cpu baseline: {len(g1)} genes x {w.n} cells, 1 thread")
+    t1 = time.perf_counter()
+    ref1 = oracle.cpu_fit(Ys[::8], w.X, w.offset, clusters_ends=ends, od_mode=od, n_threads=1)
+    oracle.de_test(ref1["beta"], ref1["bread"], ref1.get("sandwich"), w.n, contrast)
+    t1 = time.perf_counter() - t1
+    check = {"genes": int(len(genes))}
+    for prec, (fit, de) in fits.items():
+        beta = fit["beta"].cpu().numpy()[genes]
+        it = fit["iterations"]["beta_iters"].cpu().numpy()[genes]
+        th = fit["overdispersion"].cpu().numpy()[genes]
+        pv = np.asarray(de["pval"])[genes]
+        lp = np.asarray(de["log_pval"])[genes]
+        bread = fit["beta_sandwiches_null"].cpu().numpy()[genes]
+        rel_th = np.abs(th - ref["theta"]) / np.maximum(np.abs(ref["theta"]), 1e-8)
+        worst = int(np.nanargmax(rel_th))
+        se = np.sqrt(np.abs(np.diagonal(bread, axis1=1, axis2=2)))
+        se_ref = np.sqrt(np.abs(np.diagonal(ref["bread"], axis1=1, axis2=2)))
+        check[prec] = {
+            "beta_max_rel_err_floor_1e-6": rel_err(beta, ref["beta"], 1e-6),
+            "beta_max_rel_err_floor_1e-3": rel_err(beta, ref["beta"], 1e-3),
+            "theta_max_rel_err": float(np.nanmax(rel_th)),
+            "theta_worst_gene": {"gene": int(genes[worst]), "gpu": float(th[worst]),
+                                 "oracle": float(ref["theta"][worst])},
+            "se_max_rel_err": rel_err(se, se_ref, 1e-300),
+            "pval_max_rel_err": rel_err(pv, rde["pval"], 1e-300),
+            "log_pval_max_rel_err": rel_err(lp, rde["log_pval"], 1e-300),
+            "iter_mismatch": int(np.sum(it != ref["beta_iters"]))}
+    hc = host_cpu()
     return {"value": len(genes) * w.n / t, "unit": "gene*cell fits/s", "cores": n_threads,
             "kind": "port",
-            "sample": f"FP64 oracle cpu_fit on every {stride}th gene ({len(genes)} genes) x "
-                      f"all {w.n} cells of the same workload, {t:.1f} s"}, check
+            "sample": f"FP64 oracle cpu_fit + test_de on every {stride}th gene "
+                      f"({len(genes)} genes) x all {w.n} cells of the same workload, "
+                      f"{t:.1f} s on {n_threads} threads (the host's CPU share for one GPU; "
+                      f"nproc {hc['nproc']}, {hc['affinity']} CPUs in the affinity mask, "
+                      f"{hc['model']})",
+            "one_thread": {"value": len(g1) * w.n / t1, "genes": int(len(g1)),
+                           "seconds": t1},
+            "host": hc}, check
 
 
 def compare(dv, mix, f64, de_mix, de_64):
@@ -108,7 +186,11 @@ def compare(dv, mix, f64, de_mix, de_64):
         return {"index": i, "mixed": float(a.flat[i]), "fp64": float(b.flat[i])}
     sig = np.abs(lp_f) >= 1.0  # p <= 0.1
     th_m, th_f = cpu(mix["overdispersion"]), cpu(f64["overdispersion"])
-    return {"beta_max_rel_err": rel(cpu(mix["beta"]), cpu(f64["beta"]), 1e-3),
+    se = lambda f: np.sqrt(np.abs(np.diagonal(cpu(f["beta_sandwiches_null"]), axis1=1, axis2=2)))
+    return {"definition": "relative error |mixed - fp64| / max(|fp64|, floor); the 1e-4 "
+                          "target holds for beta at floor 1e-3 and log10 p at floor 1, not "
+                          "at floor 1e-6 (beta_max_rel_err_floor_1e-6, log10p_max_rel_err)",
+            "se_max_rel_err": rel(se(mix), se(f64), 1e-300),"beta_max_rel_err": rel(cpu(mix["beta"]), cpu(f64["beta"]), 1e-3),
             "log10p_max_rel_err_floor_1": rel(lp_m, lp_f, 1.0),
             "log10p_max_rel_err_p_le_0.1": rel(lp_m[sig], lp_f[sig]) if sig.any() else 0.0,
             "log10p_worst": worst(lp_m, lp_f, 1e-6),
@@ -245,6 +327,8 @@ def main():
     log(f"[bench] counts ready in {time.perf_counter() - t0:.1f} s, zero fraction {zero_frac:.3f}")
     contrast = np.zeros(w.p)
     contrast[1] = 1.0
+    hbm_copy = hbm_copy_gbps(dev)
+    log(f"[bench] device copy: {hbm_copy:.0f} GB/s")
     ctx = devil_amd.context(0)
     ctx.set_timing(True)
     if args.mixed_kernel:
@@ -290,16 +374,18 @@ def main():
     info = ctx.fit_info()
     ms_step = 1e3 * T / args.steps
     iters = fit["iterations"]["beta_iters"].cpu().numpy().astype(np.int64)
-    th_it = fit["iterations"]["theta_iters"].cpu().numpy().astype(np.int64)
+    th_it = fit["theta_passes"].cpu().numpy().astype(np.int64)
     sum_it = int(iters.sum())
     kname = "irls_mixed" if args.precision == "mixed" else "irls_fp64"
     nl, kms, kbytes = kern[kname]
     avg_ms = kms / max(nl, 1)
     achieved = kbytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
-    # whole-fit algorithmic bytes: init (2 reads: mean, variance) + IRLS +
-    # dispersion passes + bread/meat, 4 bytes per count per pass
+    # whole-fit algorithmic bytes: init + IRLS + dispersion passes +
+    # bread/meat, one read of 4 bytes per count per pass
     disp_passes = int(th_it.sum()) if w.od == "MLE" else w.G
-    fit_bytes = 4.0 * w.n * (2 * w.G + sum_it + disp_passes + w.G)
+    fit_bytes = 4.0 * w.n * (w.G + sum_it + disp_passes + w.G)
+    # algorithmic MFMA work of the IRLS contractions
+    fit_flops = 2.0 * w.n * sum_it * (w.p * (w.p + 1) / 2 + 2 * w.p)
     fit_ms = sum(stages[k] for k in ("init", "irls", "dispersion", "bread_meat"))
     value = w.G * w.n * args.steps / T
     if kname == "irls_mixed" and info.get("mixed_kernel") == "bf16":
@@ -343,10 +429,25 @@ def main():
         "stages_ms": stages,
         "fit_info": info,
         "fit_alg_GBps": fit_bytes / (fit_ms * 1e-3) / 1e9,
+        "fit_alg_frac_of_copy": fit_bytes / (fit_ms * 1e-3) / 1e9 / hbm_copy,
+        "hbm_copy_GBps": hbm_copy,
+        "mfma_util": {"flops_alg_per_step": fit_flops,
+                      "of_fit": fit_flops / (ms_step * 1e-3) / BF16_PEAK_FLOPS,
+                      "of_irls_stage": fit_flops / (stages["irls"] * 1e-3) / BF16_PEAK_FLOPS
+                      if stages["irls"] > 0 else 0.0,
+                      "peak": "bf16 dense MFMA 2.5 PFLOP/s"},
         "sum_beta_iters": sum_it,
         "sum_theta_passes": int(th_it.sum()),
         "max_beta_iters": int(iters.max()),
     }
+    nm, mms, mbytes = kern["mle"]
+    if nm > 0:
+        out["roofline_mle"] = {"bound": "hbm", "kernel": f"k_mle_accum<{w.p}> (FP64 MFMA)",
+                               "achieved": mbytes / (mms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                               "unit": "GB/s", "frac": mbytes / (mms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                               "launches_per_step": nm / args.steps,
+                               "avg_launch_ms": mms / nm,
+                               "algorithmic_bytes_per_launch": mbytes / nm}
     if args.precision == "mixed" and args.fp64_steps > 0:
         T64, st64, kern64, f64, de64 = timed("fp64", args.fp64_steps)
         ms64 = 1e3 * T64 / args.fp64_steps
@@ -357,8 +458,12 @@ def main():
                        "irls_kernel_GBps": kb64 / (kms64 * 1e-3) / 1e9 if kms64 > 0 else 0.0}
         out["mixed_vs_fp64"] = compare(devil_amd, fit, f64, de, de64)
     if not args.no_cpu_baseline:
-        n_threads = max(1, min(16, os.cpu_count() or 1))
-        cb, check = cpu_baseline(w, Yd, fit, n_threads, args.cpu_genes)
+        # the host's CPU share for one GPU is 16 threads
+        n_threads = max(1, min(16, host_cpu()["affinity"]))
+        fits = {args.precision: (fit, de)}
+        if args.precision == "mixed" and args.fp64_steps > 0:
+            fits["fp64"] = (f64, de64)
+        cb, check = cpu_baseline(w, Yd, fits, n_threads, args.cpu_genes)
         out["cpu_baseline"] = cb
         out["cpu_check"] = check
     print(json.dumps(out), flush=True)

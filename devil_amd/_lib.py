@@ -103,7 +103,7 @@ EXPORTED = ["devil_create", "devil_destroy", "devil_strerror", "devil_last_error
             "devil_kernel_stats", "devil_set_tuning", "devil_fit_info"]
 
 STAGES = ["init", "irls", "dispersion", "bread_meat", "test_de"]
-KERNELS = ["irls_fp64", "irls_mixed"]
+KERNELS = ["irls_fp64", "irls_mixed", "mle"]
 TUNE_MIXED_KERNEL, TUNE_CHUNK_CELLS, TUNE_CONCURRENCY = 0, 1, 2
 MIXED_KERNELS = {"f32": 0, "bf16": 1, "auto": 2, "bf16_ws": 3, "bf16_ws2": 4}
 

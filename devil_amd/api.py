@@ -68,6 +68,11 @@ class _Job:
                  init_overdispersion=None, init_beta_rough: bool = False, offset: float = 0,
                  size_factors=None, max_iter: int = 200, tolerance: float = 1e-3,
                  precision: str = "fp64", init_mode: str | None = None):
+        # a pandas DataFrame carries the gene names as its index (R's rownames)
+        self.gene_names = None
+        if hasattr(x, "index") and hasattr(x, "to_numpy"):
+            self.gene_names = list(x.index)
+            x = x.to_numpy()
         self.on_device = _is_torch(x) and x.is_cuda
         x_dev = _is_torch(design_matrix) and design_matrix.is_cuda and self.on_device
         sf_dev = _is_torch(size_factors) and size_factors.is_cuda and self.on_device
@@ -188,13 +193,25 @@ class _Job:
             beta = o["beta"]
             bread = o["bread"].transpose(0, 2, 1)
             sw = None if o["sw"] is None else o["sw"].transpose(0, 2, 1)
+        # theta_iters: 0 for MOM / no overdispersion, NA for MLE (cpu_fit,
+        # R/main.R:431-468); the MLE score passes are kept as theta_passes
+        if self.od == L.OD_MLE:
+            if self.on_device:
+                import torch
+                tit = torch.full((G,), float("nan"), dtype=torch.float64, device=o["tit"].device)
+            else:
+                tit = np.full(G, np.nan)
+        else:
+            tit = o["tit"]
         res = {
             "beta": beta,
             "beta_sandwiches_null": bread,
             "beta_sandwiches": sw,
             "overdispersion": o["theta"],
-            "iterations": {"beta_iters": o["bit"], "theta_iters": o["tit"]},
+            "iterations": {"beta_iters": o["bit"], "theta_iters": tit},
+            "theta_passes": o["tit"],
             "status": o["st"],
+            "gene_names": self.gene_names,
         }
         res.update(self.meta)
         return res
@@ -276,6 +293,6 @@ def test_de(devil_fit, contrast, clusters=None, pval_adjust_method: str = "BH",
     ptr = lambda a: None if a is None else a.data_ptr()
     ctx.check(L.lib().devil_test_de(ctx.h, ptr(bcol), ptr(bread), ptr(swd), G, p, n, ptr(c),
                                     float(max_lfc), ptr(pval), ptr(logp), ptr(adj), ptr(lfc)))
-    name = getattr(devil_fit["beta"], "index", None)
+    name = devil_fit.get("gene_names")
     return {"name": name, "pval": pval.cpu().numpy(), "adj_pval": adj.cpu().numpy(),
             "lfc": lfc.cpu().numpy(), "log_pval": logp.cpu().numpy()}

@@ -20,7 +20,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libdevil_hip.so")
 ROOT = os.path.dirname(HERE)
 
-SOURCES = ["kernels_wave.hip", "kernels_tile.hip", "kernels_mle.hip", "kernels_post.hip",
+SOURCES = ["kernels_epilogue.hip", "kernels_tile.hip", "kernels_mle.hip", "kernels_post.hip",
            "kernels_design.hip", "devil_hip.cpp"]
 
 COMMON = [

@@ -80,9 +80,9 @@ struct devil_ctx {
   };
   std::vector<hipEvent_t> ev_pool;
   std::vector<Launch> launches;
-  int64_t k_n[DEVIL_N_KERNELS] = {0, 0};
-  double k_ms[DEVIL_N_KERNELS] = {0, 0};
-  double k_bytes[DEVIL_N_KERNELS] = {0, 0};
+  int64_t k_n[DEVIL_N_KERNELS] = {};
+  double k_ms[DEVIL_N_KERNELS] = {};
+  double k_bytes[DEVIL_N_KERNELS] = {};
   // IRLS round counts read back without a stream synchronisation
   int32_t* round_counts = nullptr;  // pinned host slots
   hipEvent_t round_ev[2] = {nullptr, nullptr};
@@ -1144,9 +1144,13 @@ int run_mle_tiled(devil_ctx* ctx, const dv::TileDesign& td, dv::Counts c, int64_
   int32_t nA = 0;
   DV_HIP(hipMemcpyAsync(&nA, ncur, sizeof(int32_t), hipMemcpyDeviceToHost, s));
   DV_HIP(hipStreamSynchronize(s));
+  const double row_bytes = (double)count_elem_size(c.dtype) * (double)n;
   for (int round = 0; round <= max_iter && nA > 0; ++round) {
-    DV_HIP(dv::launch_mle_accum(s, td, c, p, cur, ncur, nA, beta, ms.x, ms.tab_n,
-                                part.as<double>()));
+    {
+      LaunchTimer lt(ctx, DEVIL_KERNEL_MLE, row_bytes * (double)nA);
+      DV_HIP(dv::launch_mle_accum(s, td, c, p, cur, ncur, nA, beta, ms.x, ms.tab_n,
+                                  part.as<double>()));
+    }
     DV_HIP(dv::launch_fill_i32(s, nnxt, 1, 0));
     DV_HIP(dv::launch_mle_update(s, td, p, cur, ncur, nA, part.as<double>(), ms, n, max_iter,
                                  nxt, nnxt));
@@ -1157,6 +1161,7 @@ int run_mle_tiled(devil_ctx* ctx, const dv::TileDesign& td, dv::Counts c, int64_
     std::swap(cur, nxt);
     std::swap(ncur, nnxt);
   }
+  collect_launches(ctx);
   return DEVIL_OK;
 }
 
@@ -1573,6 +1578,7 @@ int devil_fit(devil_ctx* ctx, const devil_problem* probs, devil_outputs* outs,
   });
   std::atomic<int> next{0};
   std::vector<int> rcs(n_problems, DEVIL_OK);
+  std::vector<std::string> errs(n_problems);
   std::vector<std::thread> th;
   for (int t = 0; t < S; ++t) {
     devil_ctx* sub = ctx->subs[t];
@@ -1589,7 +1595,9 @@ int devil_fit(devil_ctx* ctx, const devil_problem* probs, devil_outputs* outs,
         }
         double st[DEVIL_N_STAGES] = {0, 0, 0, 0, 0};
         for (int k = 0; k < DEVIL_N_STAGES; ++k) st[k] = sub->stage_ms[k];
+        sub->err.clear();
         rcs[j] = fit_one(sub, &probs[j], &outs[j]);
+        if (rcs[j] != DEVIL_OK) errs[j] = sub->err;
         sub->replay_sum += sub->replayed;
         sub->polish_sum += sub->polished;
         // fit_one resets the sub-context's stage and kernel figures: keep sums
@@ -1620,10 +1628,10 @@ int devil_fit(devil_ctx* ctx, const devil_problem* probs, devil_outputs* outs,
     sub->replay_sum = 0;
     sub->polish_sum = 0;
   }
+  // the first failing problem (in problem order) names the error
   for (int j = 0; j < n_problems; ++j)
     if (rcs[j] != DEVIL_OK) {
-      for (devil_ctx* sub : ctx->subs)
-        if (!sub->err.empty()) ctx->err = "problem " + std::to_string(j) + ": " + sub->err;
+      ctx->err = "problem " + std::to_string(j) + ": " + errs[j];
       return rcs[j];
     }
   return DEVIL_OK;

@@ -12,23 +12,8 @@ struct Counts {
   int64_t ld;  // elements between genes
 };
 
-// init modes for launch_init (beta part); -1 = no beta0
+// init modes of the init pass (beta part); -1 = no beta0
 enum { kInitNone = -1, kInitQR = 0, kInitRough = 1, kInitUnivariate = 2 };
-
-struct InitParams {
-  int want_alpha;           // write alpha0
-  double moi;               // 1 / mean(exp(offset))
-  int init_mode;            // kInit*
-  const double* eo;         // n: exp(offset)
-  const double* Q;          // n x p column-major thin Q of X (QR mode)
-  const double* R;          // p x p column-major upper R (QR mode)
-  const double* X;          // n x p (univariate mode)
-  const double* logsf;      // n: log(sf) (univariate mode)
-  const double* xmean;      // p (univariate mode)
-  const double* denom;      // p (univariate mode)
-  int intercept_first;      // univariate mode
-  double sum_sf;            // univariate mode
-};
 
 // Design laid out for the MFMA IRLS kernels (kernels_tile.hip).
 #define DEVIL_NCP_MAX 80  // 16 * ceil((10*11/2 + 10) / 16)
@@ -231,29 +216,8 @@ hipError_t launch_mle_update(hipStream_t s, const TileDesign& td, int p, const i
                              const MleState& ms, int64_t n, int max_iter, int32_t* next,
                              int32_t* nnext);
 
-hipError_t launch_init(hipStream_t s, Counts c, int64_t G, int64_t n, int p,
-                       const InitParams& ip, double* alpha0, double* beta0);
-
 // k: per-gene k as src/beta.cpp holds it (float(1/float(alpha0)), widened)
 hipError_t launch_k_from_alpha(hipStream_t s, const double* alpha0, int64_t G, double* k);
-
-hipError_t launch_irls_fp64(hipStream_t s, Counts c, int64_t G, int64_t n, int p,
-                            const double* X, const double* off, const double* k,
-                            double* beta, int max_iter, double eps, int32_t* iters,
-                            int32_t* status);
-
-hipError_t launch_mom(hipStream_t s, Counts c, int64_t G, int64_t n, int p,
-                      const double* X, const double* beta, const double* eo,
-                      double* theta, int32_t* status);
-
-hipError_t launch_bread_meat(hipStream_t s, Counts c, int64_t G, int64_t n, int p,
-                             const double* X, const double* beta, const double* theta,
-                             const double* eo, const int32_t* ends, int C, double* bread,
-                             double* sandwich, int32_t* status);
-
-hipError_t launch_mle(hipStream_t s, Counts c, int64_t G, int64_t n, int p,
-                      const double* X, const double* beta, const double* off,
-                      int max_iter, double* theta, int32_t* iters, int32_t* status);
 
 hipError_t launch_test_de(hipStream_t s, const double* beta, const double* bread,
                           const double* sandwich, int64_t G, int p, double df,

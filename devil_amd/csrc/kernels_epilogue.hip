@@ -1,0 +1,224 @@
+// gfx950 per-gene epilogue kernels: k as the reference's float holds it, the
+// Wald test with its t tail in log space, Benjamini-Hochberg by a device radix
+// sort and suffix-minimum scan, and small layout helpers (transposes, fills).
+// None of them reads the count matrix.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <hipcub/hipcub.hpp>
+
+#include "../../include/devil_hip.h"
+#include "device_common.hpp"
+#include "kernels.hpp"
+
+namespace dv {
+
+// k as src/beta.cpp:25-28 holds it: the float argument k = (float)alpha0,
+// then k = 1.0 / k stored back into the float.
+__global__ void k_k_from_alpha(const double* __restrict__ alpha0, int64_t G,
+                               double* __restrict__ k) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= G) return;
+  const float kf = (float)alpha0[g];
+  const float kk = (float)(1.0 / (double)kf);
+  k[g] = (double)kk;
+}
+
+// ---------------------------------------------------------------------------
+// test_de (R/test.R:94-133), one thread per gene, then Benjamini-Hochberg.
+// ---------------------------------------------------------------------------
+__global__ void k_test_de(const double* __restrict__ beta, const double* __restrict__ bread,
+                          const double* __restrict__ sandwich, int64_t G, int p, double df,
+                          const double* __restrict__ contrast, double max_lfc,
+                          double* __restrict__ pval, double* __restrict__ logp,
+                          double* __restrict__ lfc) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= G) return;
+  double l = 0.0;
+  for (int j = 0; j < p; ++j) l += beta[g * p + j] * contrast[j];
+  const double* H = bread + g * p * p;
+  double v0 = 0.0;
+  for (int a = 0; a < p; ++a)
+    for (int bb = 0; bb < p; ++bb) v0 += contrast[a] * H[a + bb * p] * contrast[bb];
+  double lp = log_pt2_upper(fabs(l) / sqrt(v0), df);
+  if (sandwich) {
+    const double* S = sandwich + g * p * p;
+    double v1 = 0.0;
+    for (int a = 0; a < p; ++a)
+      for (int bb = 0; bb < p; ++bb) v1 += contrast[a] * S[a + bb * p] * contrast[bb];
+    const double lp1 = log_pt2_upper(fabs(l) / sqrt(v1), df);
+    if (isnan(lp) || isnan(lp1)) lp = __builtin_nan("");
+    else if (lp1 > lp) lp = lp1;
+  }
+  pval[g] = exp(lp);
+  if (logp) logp[g] = lp;
+  double lf = l / log(2.0);
+  if (lf >= max_lfc) lf = max_lfc;
+  if (lf <= -max_lfc) lf = -max_lfc;
+  lfc[g] = lf;
+}
+
+// BH by its step-up definition: rank_j = #{k : p_k <= p_j} over non-NaN;
+// adj_i = min(1, min_{j : p_j >= p_i} (m / rank_j) * p_j), which equals
+// stats::p.adjust's pmin(1, cummin(n / i * p[o])) including ties.
+// Benjamini-Hochberg over the non-NaN p-values (m of them, stats::p.adjust's
+// lazy n): sorted ascending, adj(k) = min over k' >= k of m / (k' + 1) p(k'),
+// capped at 1 (R/test.R:133 -> stats::p.adjust "BH").  Ties need no care:
+// the minimum over a tied group's positions is taken at its last one, which
+// is the m / rank p of the step-up definition.  NaN p-values sort last (key
+// 2) and keep NaN.
+__global__ void k_bh_keys(const double* __restrict__ p, int64_t G, double* __restrict__ key,
+                          int32_t* __restrict__ idx) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= G) return;
+  const double v = p[i];
+  key[i] = isnan(v) ? 2.0 : v;
+  idx[i] = (int32_t)i;
+}
+
+__global__ void k_fill_i32_bh(int32_t* m) { *m = 0; }
+
+// m = the number of sorted keys below 2, written by the thread at the last one
+__global__ void k_bh_count(const double* __restrict__ ks, int64_t G, int32_t* __restrict__ m) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= G) return;
+  if (ks[k] < 1.5 && (k + 1 == G || !(ks[k + 1] < 1.5))) *m = (int32_t)(k + 1);
+}
+
+// vrev[G - 1 - k] = m / (k + 1) p(k) for k < m, +inf beyond (so an inclusive
+// min-scan of vrev is the suffix minimum)
+__global__ void k_bh_terms(const double* __restrict__ ks, int64_t G, const int32_t* __restrict__ m,
+                           double* __restrict__ vrev) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= G) return;
+  const int32_t mm = *m;
+  vrev[G - 1 - k] = k < mm ? (double)mm / (double)(k + 1) * ks[k] : __builtin_huge_val();
+}
+
+__global__ void k_bh_scatter(const double* __restrict__ p, const int32_t* __restrict__ is,
+                             const double* __restrict__ scan, int64_t G,
+                             const int32_t* __restrict__ m, double* __restrict__ adj) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= G) return;
+  const int32_t i = is[k];
+  adj[i] = k < *m ? fmin(scan[G - 1 - k], 1.0) : p[i];
+}
+
+namespace {
+struct MinOp {
+  __device__ __forceinline__ double operator()(double a, double b) const { return fmin(a, b); }
+};
+}  // namespace
+
+size_t bh_scratch_bytes(int64_t G) {
+  if (G <= 0) return 0;
+  size_t sort_b = 0, scan_b = 0;
+  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, sort_b, (const double*)nullptr,
+                                           (double*)nullptr, (const int32_t*)nullptr,
+                                           (int32_t*)nullptr, (int)G);
+  (void)hipcub::DeviceScan::InclusiveScan(nullptr, scan_b, (const double*)nullptr,
+                                          (double*)nullptr, MinOp(), (int)G);
+  const size_t tmp = std::max(sort_b, scan_b);
+  // the blocks launch_bh takes, each rounded to 256 bytes: keys, sorted keys,
+  // vrev, scan (doubles); idx, sorted idx (int32); m; the library's scratch
+  auto r256 = [](size_t v) { return ((v + 255) / 256) * 256; };
+  return 4 * r256(8 * (size_t)G) + 2 * r256(4 * (size_t)G) + r256(4) + r256(tmp);
+}
+
+hipError_t launch_bh(hipStream_t s, const double* pval, int64_t G, double* adj, void* scratch,
+                     size_t scratch_bytes) {
+  if (G <= 0) return hipSuccess;
+  char* b = static_cast<char*>(scratch);
+  auto take = [&](size_t bytes) {
+    char* q = b;
+    b += ((bytes + 255) / 256) * 256;
+    return q;
+  };
+  double* key = reinterpret_cast<double*>(take(8 * G));
+  double* ks = reinterpret_cast<double*>(take(8 * G));
+  double* vrev = reinterpret_cast<double*>(take(8 * G));
+  double* scan = reinterpret_cast<double*>(take(8 * G));
+  int32_t* idx = reinterpret_cast<int32_t*>(take(4 * G));
+  int32_t* is = reinterpret_cast<int32_t*>(take(4 * G));
+  int32_t* m = reinterpret_cast<int32_t*>(take(4));
+  void* tmp = take(0);
+  const size_t used = (size_t)(b - static_cast<char*>(scratch));
+  if (used > scratch_bytes) return hipErrorInvalidValue;
+  size_t tmp_bytes = scratch_bytes - used;
+  const dim3 grid((unsigned)((G + 255) / 256)), blk(256);
+  hipLaunchKernelGGL(k_bh_keys, grid, blk, 0, s, pval, G, key, idx);
+  hipError_t e = hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, key, ks, idx, is, (int)G, 0,
+                                                    64, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_fill_i32_bh, dim3(1), dim3(1), 0, s, m);
+  hipLaunchKernelGGL(k_bh_count, grid, blk, 0, s, ks, G, m);
+  hipLaunchKernelGGL(k_bh_terms, grid, blk, 0, s, ks, G, m, vrev);
+  tmp_bytes = scratch_bytes - used;
+  e = hipcub::DeviceScan::InclusiveScan(tmp, tmp_bytes, vrev, scan, MinOp(), (int)G, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_bh_scatter, grid, blk, 0, s, pval, is, scan, G, m, adj);
+  return hipGetLastError();
+}
+
+// G x p column-major (R layout) <-> gene-major G x p (row g holds its p values)
+__global__ void k_transpose_gp(const double* __restrict__ src, int64_t G, int p, int to_gene,
+                               double* __restrict__ dst) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= G * p) return;
+  const int64_t g = t / p;
+  const int j = (int)(t - g * p);
+  if (to_gene) dst[g * p + j] = src[g + j * G];
+  else dst[g + j * G] = src[g * p + j];
+}
+
+__global__ void k_fill_i32(int32_t* p, int64_t count, int32_t v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+
+
+hipError_t launch_k_from_alpha(hipStream_t s, const double* alpha0, int64_t G, double* k) {
+  if (G <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_k_from_alpha, dim3((unsigned)((G + 255) / 256)), dim3(256), 0, s,
+                     alpha0, G, k);
+  return hipGetLastError();
+}
+
+
+
+
+
+hipError_t launch_test_de(hipStream_t s, const double* beta, const double* bread,
+                          const double* sandwich, int64_t G, int p, double df,
+                          const double* contrast, double max_lfc, double* pval,
+                          double* logp, double* lfc, void* bh_scratch, size_t bh_bytes,
+                          double* adj) {
+  if (G <= 0) return hipSuccess;
+  const dim3 g1((unsigned)((G + 255) / 256));
+  hipLaunchKernelGGL(k_test_de, g1, dim3(256), 0, s, beta, bread, sandwich, G, p, df,
+                     contrast, max_lfc, pval, logp, lfc);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return launch_bh(s, pval, G, adj, bh_scratch, bh_bytes);
+}
+
+hipError_t launch_transpose_gp(hipStream_t s, const double* src, int64_t G, int p,
+                               int to_gene_major, double* dst) {
+  if (G <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_transpose_gp, dim3((unsigned)((G * p + 255) / 256)), dim3(256), 0, s,
+                     src, G, p, to_gene_major, dst);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_i32(hipStream_t s, int32_t* p, int64_t count, int32_t v) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, p,
+                     count, v);
+  return hipGetLastError();
+}
+
+}  // namespace dv

@@ -365,7 +365,10 @@ __device__ __forceinline__ void init_tile_chunk(const TileDesign& td, InitTile i
   const int64_t b0 = c0 / 16, b1 = c0 < c1 ? (c1 + 15) / 16 : b0;
   constexpr bool kInt = sizeof(CT) == 4 && CT(0.5) == CT(0);  // integer counts
   d4 az = d4{0.0, 0.0, 0.0, 0.0};
-  int64_t s1i = 0, s2i = 0;
+  // integer counts: sum y in int64, sum y^2 in 128 bits (lo, hi words; a
+  // block's four squares, each below 2^62, add up without overflow in 64)
+  int64_t s1i = 0;
+  uint64_t s2lo = 0, s2hi = 0;
   double s1d = 0.0, s2d = 0.0;
   struct Blk {
     CT y[4];
@@ -392,6 +395,7 @@ __device__ __forceinline__ void init_tile_chunk(const TileDesign& td, InitTile i
     constexpr bool kInner = decltype(inner)::value;
     const int64_t cb = b * 16 + 4 * h;
     double z[4];
+    uint64_t sq = 0;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const bool in = kInner || (cb + r >= c0 && cb + r < c1);
@@ -406,11 +410,15 @@ __device__ __forceinline__ void init_tile_chunk(const TileDesign& td, InitTile i
       if constexpr (kInt) {
         const int64_t yi = in ? (int64_t)o.y[r] : 0;
         s1i += yi;
-        s2i += yi * yi;
+        sq += (uint64_t)(yi * yi);
       } else {
         s1d += y;
         s2d += y * y;
       }
+    }
+    if constexpr (kInt) {
+      s2lo += sq;
+      s2hi += s2lo < sq ? 1 : 0;
     }
     if constexpr (ZM != 0) {
 #pragma unroll
@@ -425,7 +433,9 @@ __device__ __forceinline__ void init_tile_chunk(const TileDesign& td, InitTile i
 #pragma unroll
   for (int o = 16; o <= 32; o <<= 1) {
     s1i += __shfl_xor(s1i, o, 64);
-    s2i += __shfl_xor(s2i, o, 64);
+    const uint64_t lo = __shfl_xor(s2lo, o, 64), hi = __shfl_xor(s2hi, o, 64);
+    s2lo += lo;
+    s2hi += hi + (s2lo < lo ? 1 : 0);
     s1d += __shfl_xor(s1d, o, 64);
     s2d += __shfl_xor(s2d, o, 64);
   }
@@ -436,9 +446,10 @@ __device__ __forceinline__ void init_tile_chunk(const TileDesign& td, InitTile i
   if (h == 0) {
     double* o = out + s * 20 + 16;
     o[0] = s1d;
-    o[1] = s2d;
+    if constexpr (kInt) reinterpret_cast<uint64_t*>(o)[1] = s2hi;
+    else o[1] = s2d;
     reinterpret_cast<int64_t*>(o)[2] = s1i;
-    reinterpret_cast<int64_t*>(o)[3] = s2i;
+    reinterpret_cast<uint64_t*>(o)[3] = s2lo;
   }
 }
 
@@ -463,8 +474,9 @@ __global__ __launch_bounds__(256) void k_init_accum(TileDesign td, InitTile it, 
   });
 }
 
-// n * s2 - s1^2 exactly (128-bit), then one rounding
-__device__ __forceinline__ double exact_ssd(int64_t s1, int64_t s2, int64_t n) {
+// n * s2 - s1^2 exactly (128-bit; s2 < n 2^62, so n * s2 fits for n < 2^32),
+// then one rounding
+__device__ __forceinline__ double exact_ssd(int64_t s1, unsigned __int128 s2, int64_t n) {
   const __int128 v = (__int128)n * (__int128)s2 - (__int128)s1 * (__int128)s1;
   return (double)v;
 }
@@ -479,7 +491,8 @@ __global__ __launch_bounds__(256) void k_init_finish(InitTile it, ChunkTable ct,
   if (g >= G) return;
   const int64_t tile = g / 16, s = g % 16;
   double w[P], s1d = 0.0, s2d = 0.0;
-  int64_t s1i = 0, s2i = 0;
+  int64_t s1i = 0;
+  unsigned __int128 s2i = 0;
 #pragma unroll
   for (int j = 0; j < P; ++j) w[j] = 0.0;
   for (int ch = 0; ch < ct.nchunk; ++ch) {
@@ -487,9 +500,14 @@ __global__ __launch_bounds__(256) void k_init_finish(InitTile it, ChunkTable ct,
 #pragma unroll
     for (int j = 0; j < P; ++j) w[j] += src[j];
     s1d += src[16];
-    s2d += src[17];
     s1i += reinterpret_cast<const int64_t*>(src)[18];
-    s2i += reinterpret_cast<const int64_t*>(src)[19];
+    if (integer_counts) {
+      const uint64_t hi = reinterpret_cast<const uint64_t*>(src)[17];
+      const uint64_t lo = reinterpret_cast<const uint64_t*>(src)[19];
+      s2i += ((unsigned __int128)hi << 64) | lo;
+    } else {
+      s2d += src[17];
+    }
   }
   double s1, ssd;  // sum y and sum (y - mean)^2
   if (integer_counts) {

@@ -212,9 +212,12 @@ enum { DEVIL_MIXED_F32 = 0,   /* eta and all contractions on v_mfma_f32_16x16x4f
                                    candidates */ };
 int devil_set_tuning(devil_ctx *ctx, int32_t key, int64_t value);
 
-/* Per-launch timing of the IRLS accumulate kernels (the streaming pass over
- * the counts), kept while timing is enabled and reset by each devil_fit. */
-enum { DEVIL_KERNEL_IRLS_FP64 = 0, DEVIL_KERNEL_IRLS_MIXED = 1, DEVIL_N_KERNELS = 2 };
+/* Per-launch timing of the count-streaming kernels (the IRLS accumulate
+ * passes and the MLE score pass), kept while timing is enabled and reset by
+ * each devil_fit. */
+enum { DEVIL_KERNEL_IRLS_FP64 = 0, DEVIL_KERNEL_IRLS_MIXED = 1,
+       DEVIL_KERNEL_MLE = 2,  /* the MLE score pass (k_mle_accum) */
+       DEVIL_N_KERNELS = 3 };
 /* launches: number of launches; ms: summed device milliseconds (HIP events
  * on the context's stream around each launch); bytes: summed algorithmic
  * bytes (count element size x n x active genes of each launch). */
