@@ -20,6 +20,7 @@ PREC_FP64, PREC_MIXED = 0, 1
 PMAX = 10
 
 GENE_MAXITER, GENE_NAN, GENE_SINGULAR, GENE_MOM_DEN0, GENE_MLE_ZERO = 1, 2, 4, 8, 16
+GENE_MLE_NOCONV = 32
 
 _vp = C.c_void_p
 
@@ -76,7 +77,7 @@ def lib():
         "devil_mom_dispersion": (C.c_int, [_vp, _vp, i32, i64, i64, i64, _vp, i32, _vp, _vp,
                                            _vp]),
         "devil_fit_dispersion_mle": (C.c_int, [_vp, _vp, i32, i64, i64, i64, _vp, i32, _vp,
-                                               _vp, i32, _vp, _vp]),
+                                               _vp, i32, d, _vp, _vp]),
         "devil_bread_meat": (C.c_int, [_vp, _vp, i32, i64, i64, i64, _vp, i32, _vp, _vp, _vp,
                                        _vp, i32, _vp, _vp, _vp]),
         "devil_test_de": (C.c_int, [_vp, _vp, _vp, _vp, i64, i32, i64, _vp, d, _vp, _vp, _vp,
@@ -104,7 +105,8 @@ EXPORTED = ["devil_create", "devil_destroy", "devil_strerror", "devil_last_error
 
 STAGES = ["init", "irls", "dispersion", "bread_meat", "test_de"]
 KERNELS = ["irls_fp64", "irls_mixed", "mle"]
-TUNE_MIXED_KERNEL, TUNE_CHUNK_CELLS, TUNE_CONCURRENCY = 0, 1, 2
+TUNE_MIXED_KERNEL, TUNE_CHUNK_CELLS, TUNE_CONCURRENCY, TUNE_MLE_DRIVER = 0, 1, 2, 3
+MLE_DRIVERS = {"nlminb": 0, "root": 1}
 MIXED_KERNELS = {"f32": 0, "bf16": 1, "auto": 2, "bf16_ws": 3, "bf16_ws2": 4}
 
 
@@ -138,6 +140,11 @@ class Context:
     def set_mixed_kernel(self, name: str):
         """Kernel of the mixed-precision IRLS: "f32", "bf16" or "auto"."""
         self.check(lib().devil_set_tuning(self.h, TUNE_MIXED_KERNEL, MIXED_KERNELS[name]))
+
+    def set_mle_driver(self, name: str):
+        """MLE overdispersion driver: "nlminb" (the reference's nlminb call,
+        stopping rules included; default) or "root" (root of the score)."""
+        self.check(lib().devil_set_tuning(self.h, TUNE_MLE_DRIVER, MLE_DRIVERS[name]))
 
     def set_tuning(self, key: int, value: int):
         self.check(lib().devil_set_tuning(self.h, int(key), int(value)))

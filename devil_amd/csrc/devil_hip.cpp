@@ -18,6 +18,7 @@
 
 #include "../../include/devil_hip.h"
 #include "kernels.hpp"
+#include "port_driver.hpp"
 
 // Device blocks of a context kept for reuse after the scope that used them
 // ends, so repeated fits do not pay hipMalloc / hipFree (which synchronises
@@ -59,7 +60,8 @@ struct devil_ctx {
   BlockCache cache;
   std::string err;
   int timing = 0;
-  int tune[DEVIL_N_TUNE] = {DEVIL_MIXED_AUTO, 0, 8};  // kernel, chunk cells (0 auto), streams
+  // kernel, chunk cells (0 auto), streams, MLE driver
+  int tune[DEVIL_N_TUNE] = {DEVIL_MIXED_AUTO, 0, 8, DEVIL_MLE_NLMINB};
   std::map<uint64_t, int> tuned;      // autotuner choices by problem shape
   std::vector<devil_ctx*> subs;       // per-stream contexts of multi-problem fits
   int64_t replayed = 0;                // genes refit in FP64 by the mixed path (last fit)
@@ -1104,18 +1106,21 @@ int run_irls_tiled(devil_ctx* ctx, const TileHost& tl, dv::Counts c, int64_t G, 
 // far-left check of R/overdispersion.R:40-46).  theta, iters are outputs;
 // status is or-ed.
 int run_mle_tiled(devil_ctx* ctx, const dv::TileDesign& td, dv::Counts c, int64_t G, int64_t n,
-                  int p, const double* beta, int max_iter, double* theta, int32_t* iters,
-                  int32_t* status) {
+                  int p, const double* beta, int max_iter, double tolerance, double* theta,
+                  int32_t* iters, int32_t* status) {
   hipStream_t s = ctx->stream;
   if (G == 0) return DEVIL_OK;
-  DevBuf dbl, ints, act0, act1, cnt, part, tab;
+  const int driver = ctx->tune[DEVIL_TUNE_MLE_DRIVER] == DEVIL_MLE_ROOT ? dv::kMleDriverRoot
+                                                                       : dv::kMleDriverNlminb;
+  DevBuf dbl, ints, act0, act1, cnt, part, tab, port;
   DV_ALLOC(dbl, sizeof(double) * G * 7);
   DV_ALLOC(ints, sizeof(int32_t) * G * 2);
   DV_ALLOC(tab, sizeof(int32_t) * G * dv::kMleTabCap * 2);
   DV_ALLOC(act0, sizeof(int32_t) * G);
   DV_ALLOC(act1, sizeof(int32_t) * G);
   DV_ALLOC(cnt, sizeof(int32_t) * 2);
-  const int W = 5 + 3 * td.ncp;
+  if (driver == dv::kMleDriverNlminb) DV_ALLOC(port, sizeof(dv::PortState) * G);
+  const int W = 8 + 3 * td.ncp;
   const int64_t ntile = (G + 15) / 16;
   DV_ALLOC(part, sizeof(double) * (size_t)ntile * td.nchunk * 16 * W);
   dv::MleState ms;
@@ -1134,6 +1139,8 @@ int run_mle_tiled(devil_ctx* ctx, const dv::TileDesign& td, dv::Counts c, int64_
   ms.iters = iters;
   ms.theta = theta;
   ms.status = status;
+  ms.port = port.as<dv::PortState>();
+  ms.driver = driver;
   DV_HIP(dv::launch_mle_setup(s, c, G, n, p, ms));
   int32_t* cur = act0.as<int32_t>();
   int32_t* nxt = act1.as<int32_t>();
@@ -1145,15 +1152,19 @@ int run_mle_tiled(devil_ctx* ctx, const dv::TileDesign& td, dv::Counts c, int64_
   DV_HIP(hipMemcpyAsync(&nA, ncur, sizeof(int32_t), hipMemcpyDeviceToHost, s));
   DV_HIP(hipStreamSynchronize(s));
   const double row_bytes = (double)count_elem_size(c.dtype) * (double)n;
-  for (int round = 0; round <= max_iter && nA > 0; ++round) {
+  // passes: the far-left check, then at most max_iter + 1 Newton passes (root
+  // driver) or eval.max = 200 objective evaluations (nlminb driver)
+  const int max_rounds = std::max(max_iter, 200) + 2;
+  for (int round = 0; round < max_rounds && nA > 0; ++round) {
     {
       LaunchTimer lt(ctx, DEVIL_KERNEL_MLE, row_bytes * (double)nA);
       DV_HIP(dv::launch_mle_accum(s, td, c, p, cur, ncur, nA, beta, ms.x, ms.tab_n,
+                                  driver == dv::kMleDriverNlminb && round > 0,
                                   part.as<double>()));
     }
     DV_HIP(dv::launch_fill_i32(s, nnxt, 1, 0));
     DV_HIP(dv::launch_mle_update(s, td, p, cur, ncur, nA, part.as<double>(), ms, n, max_iter,
-                                 nxt, nnxt));
+                                 tolerance, nxt, nnxt));
     int32_t h = 0;
     DV_HIP(hipMemcpyAsync(&h, nnxt, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     DV_HIP(hipStreamSynchronize(s));
@@ -1493,11 +1504,11 @@ int fit_one(devil_ctx* ctx, const devil_problem* pr, devil_outputs* out) {
                                   status.as<int32_t>() + g0));
       DV_HIP(dv::launch_fill_i32(s, thit.as<int32_t>() + g0, Gb, 0));
     } else if (pr->od_mode == DEVIL_OD_MLE) {
-      if ((rc = run_mle_tiled(ctx, tl.td, c, Gb, n, p, b0, pr->max_iter, th,
+      if ((rc = run_mle_tiled(ctx, tl.td, c, Gb, n, p, b0, pr->max_iter, pr->tolerance, th,
                               thit.as<int32_t>() + g0, status.as<int32_t>() + g0)))
         return rc;
     } else {
-      DV_HIP(hipMemsetAsync(th, 0, sizeof(double) * Gb, s));
+      DV_HIP(dv::launch_fill_f64(s, th, Gb, 0.0));
       DV_HIP(dv::launch_fill_i32(s, thit.as<int32_t>() + g0, Gb, 0));
     }
     tm.end(ev);
@@ -1658,6 +1669,8 @@ int devil_set_tuning(devil_ctx* ctx, int32_t key, int64_t value) {
     return fail(ctx, DEVIL_E_ARG, "chunk cells must be 0 (autotune) or in [256, 2^20]");
   if (key == DEVIL_TUNE_CONCURRENCY && (value < 1 || value > 16))
     return fail(ctx, DEVIL_E_ARG, "concurrency must be in [1, 16]");
+  if (key == DEVIL_TUNE_MLE_DRIVER && value != DEVIL_MLE_NLMINB && value != DEVIL_MLE_ROOT)
+    return fail(ctx, DEVIL_E_ARG, "unknown MLE driver");
   ctx->tune[key] = (int)value;
   return DEVIL_OK;
 }
@@ -1827,7 +1840,7 @@ int devil_mom_dispersion(devil_ctx* ctx, const void* counts, int32_t dtype, int6
 int devil_fit_dispersion_mle(devil_ctx* ctx, const void* counts, int32_t dtype, int64_t G,
                              int64_t n, int64_t ld, const double* X, int32_t p,
                              const double* beta, const double* offset, int32_t max_iter,
-                             double* theta, int32_t* iters) {
+                             double tolerance, double* theta, int32_t* iters) {
   int rc;
   if ((rc = set_device(ctx))) return rc;
   if ((rc = check_p(ctx, p))) return rc;
@@ -1844,8 +1857,8 @@ int devil_fit_dispersion_mle(devil_ctx* ctx, const void* counts, int32_t dtype, 
   dv::Counts c{counts, dtype, ld};
   TileHost th;
   if ((rc = tile_from_device(ctx, X, offset, n, p, th))) return rc;
-  if ((rc = run_mle_tiled(ctx, th.td, c, G, n, p, bg.as<double>(), max_iter, theta, iters,
-                          st.as<int32_t>())))
+  if ((rc = run_mle_tiled(ctx, th.td, c, G, n, p, bg.as<double>(), max_iter, tolerance, theta,
+                          iters, st.as<int32_t>())))
     return rc;
   DV_HIP(hipStreamSynchronize(s));
   return DEVIL_OK;

@@ -184,6 +184,8 @@ hipError_t launch_polish_select(hipStream_t s, int64_t G, int p, const double* b
                                 int32_t* list, int32_t* nlist);
 hipError_t launch_iota(hipStream_t s, int32_t* p, int n);
 
+struct PortState;  // port_driver.hpp
+
 // Per-gene state of the tiled MLE (kernels_mle.hip); gene-major device arrays.
 struct MleState {
   double* x;       // current log(theta)
@@ -200,8 +202,11 @@ struct MleState {
   int32_t* tab_n;  // distinct positive counts in the gene's table, -1 = none
   int32_t* tab_u;  // [g * kMleTabCap + k] value
   int32_t* tab_f;  // [g * kMleTabCap + k] frequency
+  PortState* port; // per-gene state of the nlminb driver
+  int driver;      // kMleDriver*
 };
 constexpr int kMleTabCap = 256;
+enum { kMleDriverNlminb = 0, kMleDriverRoot = 1 };
 
 hipError_t launch_mle_setup(hipStream_t s, Counts c, int64_t G, int64_t n, int p,
                             const MleState& ms);
@@ -210,11 +215,11 @@ hipError_t launch_mle_collect(hipStream_t s, const int32_t* mode, int G, int32_t
 hipError_t launch_mle_accum(hipStream_t s, const TileDesign& td, Counts c, int p,
                             const int32_t* active, const int32_t* nactive, int nA_bound,
                             const double* beta, const double* xs, const int32_t* tab_n,
-                            double* part);
+                            int want_f, double* part);
 hipError_t launch_mle_update(hipStream_t s, const TileDesign& td, int p, const int32_t* active,
                              const int32_t* nactive, int nA_bound, const double* part,
-                             const MleState& ms, int64_t n, int max_iter, int32_t* next,
-                             int32_t* nnext);
+                             const MleState& ms, int64_t n, int max_iter, double rel_tol,
+                             int32_t* next, int32_t* nnext);
 
 // k: per-gene k as src/beta.cpp holds it (float(1/float(alpha0)), widened)
 hipError_t launch_k_from_alpha(hipStream_t s, const double* alpha0, int64_t G, double* k);

@@ -1,9 +1,13 @@
 // gfx950 MLE overdispersion with Cox-Reid adjustment, tiled like the IRLS.
 //
-// R/overdispersion.R:25-115 runs nlminb on the score of
-// src/overdispersion.cpp:95-178 and its derivative (:186-242).  Here every
-// gene runs the safeguarded Newton iteration on score(log theta) = 0 (the
-// same driver as the FP64 oracle), all genes in lock-step rounds:
+// R/overdispersion.R:25-115 runs nlminb on the log-likelihood of
+// src/overdispersion.cpp:53-88, its score (:95-178) and the score's
+// derivative (:186-242).  Every gene runs one of two drivers, all genes in
+// lock-step rounds of one evaluation each:
+//   DEVIL_MLE_NLMINB (default)  nlminb's PORT iteration and stopping rules
+//                               restated for one parameter (port_driver.hpp)
+//   DEVIL_MLE_ROOT              the safeguarded Newton iteration on
+//                               score(log theta) = 0 to |dx| < 1e-9
 //
 //   k_mle_setup   one wavefront per gene: sum y, sum (y-1) y, max y, the
 //                 all-zero rule and the start value (var(y) - ybar) / ybar^2.
@@ -13,28 +17,65 @@
 //                 matrices X' diag(w) X, X' diag(-w^2) X, X' diag(2 w^3) X
 //                 with w = mu / (1 + mu theta) (v_mfma_f64_16x16x4f64 over the
 //                 Khatri-Rao columns, as in the IRLS).
-//   k_mle_update  one wavefront per active gene: forms score and derivative,
-//                 applies the far-left rule (score(log 1e-8) < 0 -> 0) on the
-//                 first round and one Newton / bracketing step after that.
+//   k_mle_update  one wavefront per active gene: forms the log-likelihood,
+//                 score and derivative, applies the far-left rule
+//                 (score(log 1e-8) < 0 -> 0) on the first round and one step
+//                 of the gene's driver after that.
 //
 // Per cell everything is written with r = 1 / (1 + mu theta):
 //   y / (mu + 1/theta) = y theta r,  (y - mu) / (mu + 1/theta) = (y - mu) theta r,
 //   (mu^2 theta + y) / (1 + mu theta)^2 = (mu mu theta + y) r^2,  w = mu r,
 // and the digamma / trigamma sums and log(1 + x) - x / (1 + x) are taken in
-// their cancellation-free forms (dpsi_diff, dpsi1_diff, nb_f).
+// their cancellation-free forms (dpsi_diff, dpsi1_diff, nb_f).  The
+// log-likelihood is taken as
+//   sum_i ldiff(1/theta, y_i) - sum_i (y_i + 1/theta) log1p(mu_i theta) + CR,
+// ldiff(a, y) = lgamma(a + y) - lgamma(a) - y log(a): the same value as the
+// reference's lgamma sums, whose y log(1/theta) and n/theta log(theta) terms
+// cancel exactly.
 #include <hip/hip_runtime.h>
 
 #include "../../include/devil_hip.h"
 #include "device_common.hpp"
 #include "kernels.hpp"
+#include "port_driver.hpp"
 #include "tile_common.hpp"
 
 namespace dv {
 
+// lgamma(a + y) - lgamma(a) - y log(a) for a > 0, y >= 0 without the
+// cancellation of the two lgamma values: a sum of log1p(j / a) for small
+// integer y, else Stirling's series differenced term by term after shifting
+// a to >= 10 (ldiff(a, y) = ldiff(a + 1, y) + y log1p(1/a) - log1p(y/a)).
+__device__ inline double ldiff(double a, double y) {
+  if (y == 0.0) return 0.0;
+  if (y <= 16.0 && y == floor(y)) {
+    double s = 0.0;
+    for (int j = 1; j < (int)y; ++j) s += log1p(j / a);
+    return s;
+  }
+  double s = 0.0;
+  while (a < 10.0) {
+    s += y * log1p(1.0 / a) - log1p(y / a);
+    a += 1.0;
+  }
+  // (a + y - 1/2) log1p(t) - y = a (log1p(t) - t) + (y - 1/2) log1p(t), t = y / a,
+  // with log1p(t) - t by its series below t = 0.1
+  const double t = y / a;
+  double lm;
+  if (t < 0.1) {
+    double q = 0.0;
+    for (int k = 24; k >= 2; --k) q = ((k & 1) ? 1.0 : -1.0) / k + t * q;
+    lm = t * t * q;
+  } else {
+    lm = log1p(t) - t;
+  }
+  return s + a * lm + (y - 0.5) * log1p(t) + lgamma_corr(a + y) - lgamma_corr(a);
+}
+
 enum { kMleFarLeft = 0, kMleNewton = 1, kMleDone = 2 };
 
-// columns of a gene's partial row: 5 scalars then the three Gram tiles
-constexpr int kMleScal = 5;  // sc, l1, l2, dg, tg
+// columns of a gene's partial row: 8 scalars then the three Gram tiles
+constexpr int kMleScal = 8;  // sc, l1, l2, dg, tg, ld (cell ldiff), sum y L, sum L
 
 // Per-gene table of the distinct positive counts (value, frequency), the
 // make_table_if_small idea of src/overdispersion.cpp:32-46: the digamma and
@@ -139,7 +180,8 @@ __device__ __forceinline__ void mle_tile_chunk(const TileDesign& td, const CT* _
                                                int nA, const double* __restrict__ beta,
                                                const double* __restrict__ xs,
                                                const int32_t* __restrict__ tab_n, int tile,
-                                               int chunk, double* __restrict__ part) {
+                                               int chunk, bool want_f,
+                                               double* __restrict__ part) {
   constexpr int NJ = (P + 3) / 4;
   constexpr int NTMAX = ((P * (P + 1)) / 2 + P + 15) / 16;
   const int lane = threadIdx.x & 63;
@@ -166,7 +208,7 @@ __device__ __forceinline__ void mle_tile_chunk(const TileDesign& td, const CT* _
     a1[t] = a0[t];
     a2[t] = a0[t];
   }
-  double sc = 0.0, l1 = 0.0, l2 = 0.0, dg = 0.0, tg = 0.0;
+  double sc = 0.0, l1 = 0.0, l2 = 0.0, dg = 0.0, tg = 0.0, lg = 0.0, syl = 0.0, sl = 0.0;
   const int64_t b0 = (int64_t)chunk * td.chunk_blocks;
   const int64_t b1 = min((int64_t)(chunk + 1) * td.chunk_blocks, td.nblk);
   typedef TileBlock<NJ, NTMAX, CT> Blk;
@@ -193,9 +235,12 @@ __device__ __forceinline__ void mle_tile_chunk(const TileDesign& td, const CT* _
         sc += f + y * theta * rr;
         l1 += L + (y - mu) * theta * rr;
         l2 += (mu * mt + y) * rr * rr;
+        syl += y * L;
+        sl += L;
         if (cell_psi && y != 0.0) {
           dg += dpsi_diff(tn1, y);
           tg += dpsi1_diff(tn1, y);
+          if (want_f) lg += ldiff(tn1, y);
         }
       }
     }
@@ -226,6 +271,9 @@ __device__ __forceinline__ void mle_tile_chunk(const TileDesign& td, const CT* _
     l2 += __shfl_xor(l2, o, 64);
     dg += __shfl_xor(dg, o, 64);
     tg += __shfl_xor(tg, o, 64);
+    lg += __shfl_xor(lg, o, 64);
+    syl += __shfl_xor(syl, o, 64);
+    sl += __shfl_xor(sl, o, 64);
   }
   const int W = kMleScal + 3 * td.ncp;
   double* out = part + ((int64_t)tile * td.nchunk + chunk) * 16 * W;
@@ -236,6 +284,9 @@ __device__ __forceinline__ void mle_tile_chunk(const TileDesign& td, const CT* _
     o[2] = l2;
     o[3] = dg;
     o[4] = tg;
+    o[5] = lg;
+    o[6] = syl;
+    o[7] = sl;
   }
 #pragma unroll
   for (int t = 0; t < NTMAX; ++t) {
@@ -258,7 +309,7 @@ __global__ __launch_bounds__(256) void k_mle_accum(TileDesign td, Counts c,
                                                    const double* __restrict__ beta,
                                                    const double* __restrict__ xs,
                                                    const int32_t* __restrict__ tab_n,
-                                                   double* __restrict__ part) {
+                                                   int want_f, double* __restrict__ part) {
   const int nA = *nactive;
   const int ntile = (nA + 15) / 16;
   const int wave = threadIdx.x >> 6;
@@ -269,7 +320,7 @@ __global__ __launch_bounds__(256) void k_mle_accum(TileDesign td, Counts c,
   with_counts(c, [&](auto Y, auto vec) {
     typedef std::remove_cv_t<std::remove_pointer_t<decltype(Y)>> CT;
     mle_tile_chunk<P, CT, decltype(vec)::value>(td, Y, c.ld, active, nA, beta, xs, tab_n, tile,
-                                                chunk, part);
+                                                chunk, want_f != 0, part);
   });
 }
 
@@ -277,7 +328,7 @@ template <int P>
 __global__ __launch_bounds__(64) void k_mle_update(TileDesign td, const int32_t* __restrict__ active,
                                                    const int32_t* __restrict__ nactive,
                                                    const double* __restrict__ part, MleState ms,
-                                                   int64_t n, int max_iter,
+                                                   int64_t n, int max_iter, PortParams pp,
                                                    int32_t* __restrict__ next,
                                                    int32_t* __restrict__ nnext) {
   __shared__ double acc[kMleScal + 3 * DEVIL_NCP_MAX];
@@ -298,17 +349,22 @@ __global__ __launch_bounds__(64) void k_mle_update(TileDesign td, const int32_t*
   const double x = ms.x[g];
   const double theta = exp(x);
   const double tn1 = 1.0 / theta;
-  // digamma / trigamma difference sums over the distinct-count table
-  double tdg = 0.0, ttg = 0.0;
+  const int mode0 = ms.mode[g];
+  const bool port = ms.driver == kMleDriverNlminb && mode0 == kMleNewton;
+  // digamma / trigamma (and, for the nlminb driver, lgamma) difference sums
+  // over the distinct-count table
+  double tdg = 0.0, ttg = 0.0, tld = 0.0;
   const int tn = ms.tab_n[g];
   for (int k = lane; k < tn; k += 64) {
     const double u = (double)ms.tab_u[(int64_t)g * kMleTabCap + k];
     const double f = (double)ms.tab_f[(int64_t)g * kMleTabCap + k];
     tdg += f * dpsi_diff(tn1, u);
     ttg += f * dpsi1_diff(tn1, u);
+    if (port) tld += f * ldiff(tn1, u);
   }
   tdg = wave_sum(tdg);
   ttg = wave_sum(ttg);
+  if (port) tld = wave_sum(tld);
   if (lane != 0) return;
   const double sc = acc[0], l1 = acc[1], l2 = acc[2];
   const double dg = acc[3] + tdg, tg = acc[4] + ttg;
@@ -342,7 +398,7 @@ __global__ __launch_bounds__(64) void k_mle_update(TileDesign td, const int32_t*
   const double corr = tn1 > 1e5 ? spy / (2 * tn1) : 0.0;
   const double dterm = (max_y * 1e6 < tn1) ? sum_y - corr : fmin(dg * tn1, sum_y - corr);
   const double score = sc * tn1 - dterm + cr_s * theta;
-  int mode = ms.mode[g];
+  const int mode = mode0;
   int it = ms.iters[g] + 1;
   ms.iters[g] = it;
   if (mode == kMleFarLeft) {
@@ -352,11 +408,19 @@ __global__ __launch_bounds__(64) void k_mle_update(TileDesign td, const int32_t*
       ms.status[g] |= DEVIL_GENE_MLE_ZERO;
       return;
     }
-    const double lo = log(1e-8), hi = log(1e16);
+    // the root driver brackets in [log 1e-8, log 1e16]; nlminb's box is
+    // [log 1e-16, log 1e16] (R/overdispersion.R:72)
+    const double lo = ms.driver == kMleDriverNlminb ? pp.lo : log(1e-8), hi = log(1e16);
     ms.lo[g] = lo;
     ms.hi[g] = hi;
     ms.x[g] = fmin(fmax(log(ms.start[g]), lo), hi);
     ms.mode[g] = kMleNewton;
+    if (ms.driver == kMleDriverNlminb) {
+      PortState& st = ms.port[g];
+      st.phase = kPortStart;
+      st.xe = ms.x[g];
+      st.code = 0;
+    }
     if (max_iter > 0) {
       next[atomicAdd(nnext, 1)] = g;
     } else {
@@ -392,6 +456,43 @@ __global__ __launch_bounds__(64) void k_mle_update(TileDesign td, const int32_t*
   const double tn2 = 1.0 / (theta * theta);
   const double ll = -2 * tn1 * (l1 - dg) + (l2 + tn2 * tg);
   const double d = ll + cr_t * (theta * theta) + (l1 - dg) * tn1 + cr_t2 * theta;
+  if (port) {
+    // log-likelihood (src/overdispersion.cpp:53-88): the Cox-Reid term from
+    // the LU factors (partial pivoting) of X' diag(w) X without the ridge,
+    // each pivot's log clamped below at log(1e-50)
+    double Lm[P][P];
+    int perm[P];
+#pragma unroll
+    for (int a = 0; a < P; ++a)
+#pragma unroll
+      for (int bb = a; bb < P; ++bb) {
+        const int pi = td.kidx_pair[tri<P>(a, bb)];
+        const double v0 = pi >= 0 ? B0[pi] : 0.0;
+        Lm[a][bb] = v0;
+        Lm[bb][a] = v0;
+      }
+    lu_factor<P>(Lm, perm);
+    double ldet = 0.0;
+#pragma unroll
+    for (int a = 0; a < P; ++a) {
+      const double e = Lm[a][a];
+      ldet += e < 1e-50 ? log(1e-50) : log(e);
+    }
+    const double loglik = acc[5] + tld - (acc[6] + tn1 * acc[7]) - 0.5 * ldet * 0.99;
+    PortState st = ms.port[g];
+    const int done = port_advance(st, -loglik, -score, -d, pp);
+    ms.port[g] = st;
+    if (done || it - 1 >= kPtEvalMax + 1) {
+      ms.mode[g] = kMleDone;
+      ms.theta[g] = exp(st.x);
+      ms.x[g] = st.x;
+      if (!(st.code >= 3 && st.code <= 6)) ms.status[g] |= DEVIL_GENE_MLE_NOCONV;
+    } else {
+      ms.x[g] = st.xe;
+      next[atomicAdd(nnext, 1)] = g;
+    }
+    return;
+  }
   // one safeguarded Newton step (same driver as the oracle)
   bool done = false;
   double xn = x;
@@ -450,22 +551,29 @@ hipError_t launch_mle_collect(hipStream_t s, const int32_t* mode, int G, int32_t
 hipError_t launch_mle_accum(hipStream_t s, const TileDesign& td, Counts c, int p,
                             const int32_t* active, const int32_t* nactive, int nA_bound,
                             const double* beta, const double* xs, const int32_t* tab_n,
-                            double* part) {
+                            int want_f, double* part) {
   if (nA_bound <= 0) return hipSuccess;
   const int ntile = (nA_bound + 15) / 16;
   const int groups = (ntile + 3) / 4;
   const dim3 grid((unsigned)((int64_t)groups * td.nchunk));
-  DV_MLE_P(p, k_mle_accum, grid, dim3(256), s, td, c, active, nactive, beta, xs, tab_n, part);
+  DV_MLE_P(p, k_mle_accum, grid, dim3(256), s, td, c, active, nactive, beta, xs, tab_n, want_f,
+           part);
   return hipGetLastError();
 }
 
 hipError_t launch_mle_update(hipStream_t s, const TileDesign& td, int p, const int32_t* active,
                              const int32_t* nactive, int nA_bound, const double* part,
-                             const MleState& ms, int64_t n, int max_iter, int32_t* next,
-                             int32_t* nnext) {
+                             const MleState& ms, int64_t n, int max_iter, double rel_tol,
+                             int32_t* next, int32_t* nnext) {
   if (nA_bound <= 0) return hipSuccess;
+  PortParams pp;
+  pp.lo = log(1e-16);
+  pp.up = log(1e16);
+  pp.rfctol = rel_tol;
+  pp.sctol = rel_tol;
+  pp.iter_max = max_iter;
   DV_MLE_P(p, k_mle_update, dim3((unsigned)nA_bound), dim3(64), s, td, active, nactive, part,
-           ms, n, max_iter, next, nnext);
+           ms, n, max_iter, pp, next, nnext);
   return hipGetLastError();
 }
 

@@ -59,6 +59,11 @@ extern "C" {
 #define DEVIL_GENE_SINGULAR 4  /* a p x p factorisation met an exactly zero pivot */
 #define DEVIL_GENE_MOM_DEN0 8  /* MOM denominator <= 0, theta = 0 */
 #define DEVIL_GENE_MLE_ZERO 16 /* MLE special case returned 0 (all-zero gene or score(1e-8) < 0) */
+#define DEVIL_GENE_MLE_NOCONV 32 /* the nlminb driver ended with convergence != 0 (singular or
+                                    false convergence, evaluation or iteration limit): the
+                                    reference reruns nlminb without the Hessian
+                                    (R/overdispersion.R:75-110), which this library does not
+                                    restate; theta is the first run's par */
 
 #define DEVIL_PMAX 10          /* largest design width this library supports */
 
@@ -150,12 +155,15 @@ int devil_mom_dispersion(devil_ctx *ctx, const void *counts, int32_t counts_dtyp
                          double *theta);
 
 /* R/overdispersion.R:25-115 with Cox-Reid adjustment: special cases and start
- * value as the reference; root of the score in log(theta) by safeguarded
- * Newton (the reference drives nlminb).  iters: score passes per gene. */
+ * value as the reference, then the context's MLE driver (DEVIL_TUNE_MLE_DRIVER):
+ * by default stats::nlminb(rel.tol = tolerance, iter.max = max_iter)'s PORT
+ * iteration restated (its stopping rules included), or the root of the score.
+ * iters: passes over the counts per gene. */
 int devil_fit_dispersion_mle(devil_ctx *ctx, const void *counts, int32_t counts_dtype,
                              int64_t G, int64_t n, int64_t counts_ld, const double *X,
                              int32_t p, const double *beta, const double *offset,
-                             int32_t max_iter, double *theta, int32_t *iters);
+                             int32_t max_iter, double tolerance, double *theta,
+                             int32_t *iters);
 
 /* src/variance.cpp:22-47 (bread) and :186-228 (meat) in one pass; sandwich =
  * bread * meat * bread * n (R/main.R:491) when cluster_ends != NULL. */
@@ -192,7 +200,14 @@ int devil_stage_times(const devil_ctx *ctx, double *ms);
 enum { DEVIL_TUNE_MIXED_KERNEL = 0,  /* DEVIL_MIXED_* (default AUTO) */
        DEVIL_TUNE_CHUNK_CELLS = 1,   /* cells per chunk of the tile passes, 0 = autotune */
        DEVIL_TUNE_CONCURRENCY = 2,   /* streams for multi-problem fits (default 8) */
-       DEVIL_N_TUNE = 3 };
+       DEVIL_TUNE_MLE_DRIVER = 3,    /* DEVIL_MLE_* (default NLMINB) */
+       DEVIL_N_TUNE = 4 };
+/* MLE overdispersion drivers.  NLMINB: the reference's stats::nlminb call
+ * (R/overdispersion.R:53-73) restated for one parameter -- its trust-region
+ * Newton steps and its stopping rules (relative function convergence at
+ * rel.tol = tolerance), so theta is where nlminb stops, not the root of the
+ * score.  ROOT: safeguarded Newton on score(log theta) = 0 to |dx| < 1e-9. */
+enum { DEVIL_MLE_NLMINB = 0, DEVIL_MLE_ROOT = 1 };
 enum { DEVIL_MIXED_F32 = 0,   /* eta and all contractions on v_mfma_f32_16x16x4f32 */
        DEVIL_MIXED_BF16 = 1,  /* eta, design columns (six products of three-piece
                                  splits) and pair columns (three products) on

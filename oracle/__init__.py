@@ -35,6 +35,7 @@ class FitArgs(C.Structure):
         ("n_clusters", C.c_int32), ("disp_init", _dp), ("init_mode", C.c_int32),
         ("od_mode", C.c_int32), ("max_iter", C.c_int32), ("tolerance", C.c_double),
         ("n_threads", C.c_int32), ("genes", _lp), ("n_genes_subset", C.c_int64),
+        ("mle_driver", C.c_int32),
     ]
 
 
@@ -81,6 +82,11 @@ def lib():
             "orc_nb_dscore_acc": (d, [_dp, _dp, i64, d, _dp, C.c_int, C.c_int]),
             "orc_fit_dispersion": (d, [_dp, _dp, _dp, _dp, i64, C.c_int, d, C.c_int,
                                        C.c_int, C.POINTER(C.c_int)]),
+            "orc_fit_dispersion_port": (d, [_dp, _dp, _dp, _dp, i64, C.c_int, d, C.c_int,
+                                            C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                            C.POINTER(C.c_int)]),
+            "orc_lgamma_diff": (d, [d, d]),
+            "orc_nb_loglik_acc": (d, [_dp, _dp, i64, d, _dp, C.c_int, C.c_int]),
             "orc_estimate_dispersion": (None, [_dp, i64, i64, i64, _dp, _dp]),
             "orc_init_beta_qr": (C.c_int, [_dp, i64, i64, i64, _dp, C.c_int, _dp, _dp]),
             "orc_init_beta_rough": (None, [_dp, i64, i64, i64, C.c_int, _dp]),
@@ -249,6 +255,30 @@ def fit_dispersion(beta, X, y, off, tol=1e-3, max_iter=200, cr=True):
     return th, it.value
 
 
+def fit_dispersion_port(beta, X, y, off, tol=1e-3, max_iter=200, cr=True):
+    """fit_dispersion with nlminb's PORT driver restated: (theta, iterations,
+    PORT return code, objective evaluations)."""
+    X = np.asfortranarray(X, dtype=np.float64)
+    y = _f64(y)
+    it, code, nf = C.c_int(), C.c_int(), C.c_int()
+    th = lib().orc_fit_dispersion_port(_d(_f64(beta)), X.ctypes.data_as(_dp), _d(y),
+                                       _d(_f64(off)), y.size, X.shape[1], float(tol),
+                                       int(max_iter), int(cr), C.byref(it), C.byref(code),
+                                       C.byref(nf))
+    return th, it.value, code.value, nf.value
+
+
+def lgamma_diff(a, y):
+    return lib().orc_lgamma_diff(float(a), float(y))
+
+
+def nb_loglik_acc(y, mu, log_theta, X, cr=True):
+    X = np.asfortranarray(X, dtype=np.float64)
+    y = _f64(y)
+    return lib().orc_nb_loglik_acc(_d(y), _d(_f64(mu)), y.size, float(log_theta),
+                                   X.ctypes.data_as(_dp), X.shape[1], int(cr))
+
+
 def estimate_dispersion(Y, off):
     Y, off = _f64(Y), _f64(off)
     G, n = Y.shape
@@ -290,7 +320,8 @@ def init_beta_univariate(Y, X, sf):
 
 
 def cpu_fit(Y, X, offset, *, clusters_ends=None, disp_init=None, init_mode=INIT_QR,
-            od_mode=OD_MOM, max_iter=200, tolerance=1e-3, n_threads=0, genes=None):
+            od_mode=OD_MOM, max_iter=200, tolerance=1e-3, n_threads=0, genes=None,
+            mle_driver="nlminb"):
     """R/main.R:375-523 cpu_fit.  Returns a dict of numpy arrays.
 
     ``Y`` is G x n (any numeric dtype; converted to float64 rows).  ``genes``
@@ -311,7 +342,7 @@ def cpu_fit(Y, X, offset, *, clusters_ends=None, disp_init=None, init_mode=INIT_
                 init_mode=init_mode, od_mode=od_mode, max_iter=max_iter,
                 tolerance=tolerance, n_threads=n_threads,
                 genes=gidx.ctypes.data_as(_lp) if gidx is not None else None,
-                n_genes_subset=Gs)
+                n_genes_subset=Gs, mle_driver={"nlminb": 0, "root": 1}[mle_driver])
     out = dict(beta=np.zeros((Gs, p), order="F"), theta=np.zeros(Gs),
                beta_iters=np.zeros(Gs, np.int32), theta_iters=np.zeros(Gs, np.int32),
                bread=np.zeros((Gs, p, p)), alpha0=np.zeros(Gs),

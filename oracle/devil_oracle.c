@@ -900,6 +900,407 @@ double orc_fit_dispersion(const double *beta, const double *X, const double *y,
 }
 
 /* ===================================================================== */
+/* R's stats::nlminb driver of fit_dispersion, restated for one parameter  */
+/* ===================================================================== */
+
+/* lgamma(a + y) - lgamma(a) - y log(a) for a > 0, y >= 0, without the
+ * cancellation of the two lgamma values: a sum of log1p(j / a) for small
+ * integer y, else Stirling's series differenced term by term after shifting
+ * a to >= 10 (ldiff(a, y) = ldiff(a + 1, y) + y log1p(1/a) - log1p(y/a)). */
+double orc_lgamma_diff(double a, double y) {
+  if (y == 0.0) return 0.0;
+  if (y <= 16.0 && y == floor(y)) {
+    double s = 0.0;
+    for (int j = 1; j < (int)y; ++j) s += log1p(j / a);
+    return s;
+  }
+  double s = 0.0;
+  while (a < 10.0) {
+    s += y * log1p(1.0 / a) - log1p(y / a);
+    a += 1.0;
+  }
+  /* (a + y - 1/2) log1p(t) - y = a (log1p(t) - t) + (y - 1/2) log1p(t),
+   * t = y / a, with log1p(t) - t by its series below t = 0.1 */
+  const double t = y / a;
+  double lm;
+  if (t < 0.1) {
+    double q = 0.0;
+    for (int k = 24; k >= 2; --k) q = ((k & 1) ? 1.0 : -1.0) / k + t * q;
+    lm = t * t * q;
+  } else {
+    lm = log1p(t) - t;
+  }
+  return s + a * lm + (y - 0.5) * log1p(t) + lgcor(a + y) - lgcor(a);
+}
+
+/* conventional_loglikelihood_fast (src/overdispersion.cpp:53-88) in the
+ * equivalent form
+ *   sum_i ldiff(1/theta, y_i) - sum_i (y_i + 1/theta) log1p(mu_i theta) + CR
+ * (lgamma(y + 1/theta) - lgamma(1/theta) = y log(1/theta) + ldiff, and
+ * log(mu + 1/theta) = log(1/theta) + log1p(mu theta): the y log(1/theta)
+ * and the n/theta log(theta) terms cancel exactly).  The Cox-Reid term is the
+ * reference's: LU with partial pivoting of X' diag(1/(1/mu + theta)) X, log
+ * of each pivot clamped below at 1e-50. */
+double orc_nb_loglik_acc(const double *y, const double *mu, int64_t n, double log_theta,
+                         const double *X, int p, int cr) {
+  const double theta = exp(log_theta);
+  const double tn1 = 1.0 / theta;
+  double cr_term = 0.0;
+  if (cr) {
+    double *w = (double *)malloc(sizeof(double) * (size_t)n);
+    for (int64_t i = 0; i < n; ++i) w[i] = 1.0 / (1.0 / mu[i] + theta);
+    double b[PMAX * PMAX];
+    int perm[PMAX];
+    gram(X, w, n, p, b);
+    free(w);
+    lu_pp(b, p, perm);
+    double ld = 0.0;
+    for (int a = 0; a < p; ++a) {
+      const double e = b[a + a * p];
+      ld += e < 1e-50 ? log(1e-50) : log(e);
+    }
+    cr_term = -0.5 * ld * cr_correction_factor;
+  }
+  double s = 0.0;
+  for (int64_t i = 0; i < n; ++i)
+    s += orc_lgamma_diff(tn1, y[i]) - (y[i] + tn1) * log1p(mu[i] * theta);
+  return s + cr_term;
+}
+
+/* PORT constants: R's nlminb control defaults (eval.max 200, abs.tol 0,
+ * x.tol 1.5e-8, xf.tol 2.2e-14, step.min = step.max = 1, sing.tol =
+ * rel.tol) and the PORT library's defaults for the unconstrained/bounded
+ * Newton driver (DMNH / DMNHB: tuning constants of the step assessment,
+ * radius factors, trust-region acceptance band). */
+#define PT_LMAX0 1.0
+#define PT_LMAXS 1.0
+#define PT_XCTOL 1.5e-8
+#define PT_XFTOL 2.2e-14
+#define PT_TUNER1 0.1
+#define PT_TUNER2 1e-4
+#define PT_TUNER3 0.75
+#define PT_TUNER4 0.5
+#define PT_TUNER5 0.75
+#define PT_INCFAC 2.0
+#define PT_DECFAC 0.5
+#define PT_RDFCMN 0.1
+#define PT_RDFCMX 4.0
+#define PT_PHMXFC 0.1
+#define PT_EVALMAX 200
+
+typedef struct {
+  const double *y, *mu, *X;
+  int64_t n;
+  int p, cr;
+} mle_obj;
+
+/* objective, gradient and Hessian of nlminb's call: -loglik, -score,
+ * -dscore in log theta (R/overdispersion.R:53-72) */
+static void mle_fgh(const mle_obj *o, double x, double *f, double *g, double *h) {
+  *f = -orc_nb_loglik_acc(o->y, o->mu, o->n, x, o->X, o->p, o->cr);
+  *g = -orc_nb_score_acc(o->y, o->mu, o->n, x, o->X, o->p, o->cr);
+  *h = -orc_nb_dscore_acc(o->y, o->mu, o->n, x, o->X, o->p, o->cr);
+}
+
+/* The trust-region step for one parameter from x0 with gradient g, Hessian
+ * h and radius rad inside [lo, up]: the Newton step -g/h when h > 0 and
+ * |g/h| <= (1 + PHMXFC) rad, else the step of length rad downhill (what the
+ * locally constrained step becomes in one dimension), cut at the bounds.
+ * nreduc = g^2 / (2h) (the Newton step's predicted reduction, dst0 = its
+ * length) when h > 0; dst0 = -1 otherwise. */
+typedef struct {
+  double s, preduc, nreduc, dst0, dstnrm, gtstep;
+  int newton; /* stppar == 0 */
+} port_step;
+
+static port_step port_make_step(double x0, double g, double h, double rad, double lo,
+                                double up) {
+  port_step st;
+  st.newton = 0;
+  if (h > 0.0) {
+    const double sn = -g / h;
+    st.dst0 = fabs(sn);
+    st.nreduc = 0.5 * g * g / h;
+    if (st.dst0 <= (1.0 + PT_PHMXFC) * rad) {
+      st.s = sn;
+      st.newton = 1;
+    } else {
+      st.s = g > 0 ? -rad : rad;
+    }
+  } else {
+    st.dst0 = -1.0;
+    st.nreduc = 0.0;
+    st.s = g > 0 ? -rad : (g < 0 ? rad : 0.0);
+  }
+  if (x0 + st.s > up) {
+    st.s = up - x0;
+    st.newton = 0;
+  } else if (x0 + st.s < lo) {
+    st.s = lo - x0;
+    st.newton = 0;
+  }
+  st.dstnrm = fabs(st.s);
+  st.gtstep = g * st.s;
+  st.preduc = -(g * st.s + 0.5 * h * st.s * st.s);
+  if (st.newton) st.preduc = st.nreduc;
+  return st;
+}
+
+/* One-parameter restatement of nlminb's PORT driver with analytic gradient
+ * and Hessian and box bounds (R/overdispersion.R:53-73 -> stats::nlminb ->
+ * PORT DMNHB), following the PORT documentation of its step assessment
+ * (Gay, ACM TOMS 9 (1983) 503-524; PORT usage summary): per iteration a
+ * trust-region Newton step from the accepted point x0; the step is accepted
+ * when f decreased by more than TUNER2 times the predicted reduction, the
+ * radius then scales by Fletcher's factors, else the step is recomputed
+ * with a smaller radius; after each assessment
+ *   relative function convergence: f(x0) - f <= 2 preduc and the Newton
+ *       step's predicted reduction nreduc <= rel.tol |f(x0)|
+ *   x-convergence: a Newton step with |dx| / (|x| + |x0|) <= x.tol
+ *   singular convergence: predicted reduction of a step of length step.min
+ *       <= sing.tol |f(x0)| when no Newton step is taken
+ *   false convergence: no decrease with |dx| / (|x| + |x0|) <= xf.tol.
+ * Returns the PORT return code: 3 x-, 4 relative function, 5 both, 6
+ * absolute function, 7 singular, 8 false convergence, 9 evaluation limit,
+ * 10 iteration limit, 63 objective not finite at the start.  R's nlminb
+ * reports convergence = 0 for codes 3-6 only; fit_dispersion reruns
+ * nlminb without the Hessian otherwise (R/overdispersion.R:75-110), which is
+ * not restated here.  *xout is nlminb's par; *niter its iterations, *nfev
+ * its objective evaluations. */
+int orc_port_min1(const mle_obj *o, double xstart, double lo, double up, double rel_tol,
+                  int iter_max, double *xout, int *niter_out, int *nfev_out) {
+  const double rfctol = rel_tol, sctol = rel_tol;
+  double x = fmin(fmax(xstart, lo), up);
+  double f, g, h;
+  mle_fgh(o, x, &f, &g, &h);
+  int nf = 1, niter = 0, code = 0;
+  if (!isfinite(f)) {
+    *xout = x;
+    *niter_out = 0;
+    *nfev_out = nf;
+    return 63;
+  }
+  /* the first radius keeps the first step within step.max: the trust-region
+   * step may be (1 + PHMXFC) times the radius long */
+  double radfac = 1.0, last_step = 0.0, radius = PT_LMAX0 / (1.0 + PT_PHMXFC);
+  int acc_irc = 0;
+  for (;;) {
+    if (niter >= iter_max) { code = 10; break; }
+    niter++;
+    const double x0 = x, f0 = f, g0 = g, h0 = h;
+    if (niter > 1) radius = radfac * last_step;
+    /* DA7SST's state for this iteration (one model: STGLIM = 1) */
+    int irc_in = 4, radinc = 0, restor = 0;
+    double flstgd = f0, xlstgd = x0, plstgd = 0.0, gthstg = 0.0, dstsav = 0.0;
+    double xacc = x0, facc = f0;
+    int accepted = 0, converged = 0;
+    for (;;) {
+      port_step st = port_make_step(x0, g0, h0, radius, lo, up);
+      double xt = x0 + st.s, ft = f0;
+      int toobig = 0;
+      if (st.dstnrm > 0.0) {
+        if (nf >= PT_EVALMAX) { code = 9; goto done; }
+        double gt, ht;
+        mle_fgh(o, xt, &ft, &gt, &ht);
+        nf++;
+        toobig = !isfinite(ft);
+      }
+      if (toobig) {  /* objective not computable: halve the radius */
+        radius = PT_DECFAC * st.dstnrm;
+        radinc--;
+        irc_in = 5;
+        continue;
+      }
+      const double den = fabs(xt) + fabs(x0);
+      const double reldx = den > 0.0 ? st.dstnrm / den : 0.0;
+      /* ---- assess the candidate (DA7SST) ---- */
+      double fc = ft, xc = xt, preduc = st.preduc, gtstep = st.gtstep, dstnrm = st.dstnrm;
+      int goodx = 1, irc = 0;
+      double rfac1 = 1.0, radf = 1.0;
+      if (irc_in == 5 && !(fc < flstgd) && flstgd < f0) {
+        /* the recomputed step is worse than the saved good step: restore it */
+        rfac1 = dstnrm / dstsav;
+        fc = flstgd;
+        xc = xlstgd;
+        preduc = plstgd;
+        gtstep = gthstg;
+        dstnrm = dstsav;
+        goodx = 0;
+        restor = 1;
+      }
+      double fdif = f0 - fc;
+      if (!(fdif > PT_TUNER2 * preduc) && radinc <= 0) {
+        /* no or only a trivial decrease: recompute with a smaller radius */
+        if (!(fc < f0)) {
+          flstgd = fc;
+          fc = f0;
+          xc = x0;
+          restor = 1;
+        }
+        irc = 5;
+        radinc--;
+        const double emax = gtstep + fdif;
+        radf = 0.5 * rfac1;
+        if (emax < gtstep) radf = rfac1 * fmax(PT_RDFCMN, 0.5 * gtstep / emax);
+        if (reldx <= PT_XFTOL) {
+          irc = 12;
+        } else if (fc < f0) {
+          flstgd = fc;
+          xlstgd = xc;
+          if (restor != 1) restor = 2;
+          dstsav = dstnrm;
+          plstgd = preduc;
+          gthstg = gtstep;
+        }
+      } else if (!(fdif > preduc * PT_TUNER1)) {
+        /* decrease much less than predicted: accept, smaller radius */
+        irc = 4;
+        const double emax = gtstep + fdif;
+        radf = 0.5 * rfac1;
+        if (emax < gtstep) radf = rfac1 * fmax(PT_RDFCMN, 0.5 * gtstep / emax);
+        if (reldx <= PT_XFTOL) irc = 12;
+      } else if (fdif < -PT_TUNER3 * gtstep || radinc < 0 || restor == 1) {
+        /* good decrease: accept with the radius unchanged */
+        radf = 1.0;
+        irc = 3;
+      } else {
+        /* very good decrease: a longer step looks worthwhile */
+        radf = PT_RDFCMX;
+        if (fdif < (0.5 / radf - 1.0) * gtstep)
+          radf = fmax(PT_INCFAC, 0.5 * gtstep / (gtstep + fdif));
+        irc = 4;
+        if (!st.newton && !(st.dst0 >= 0.0 && (st.dst0 < 2.0 * dstnrm ||
+                                               st.nreduc < 1.2 * fdif))) {
+          irc = 5;
+          radinc++;
+          flstgd = fc;
+          xlstgd = xc;
+          if (restor != 1) restor = 2;
+          dstsav = dstnrm;
+          plstgd = preduc;
+          gthstg = gtstep;
+        }
+      }
+      if (restor == 1 && flstgd < f0) {
+        restor = 3;
+        xc = xlstgd;
+        fc = flstgd;
+      }
+      /* ---- convergence tests ---- */
+      if (!(0.5 * fdif > preduc)) {
+        const double emax = rfctol * fabs(f0), emaxs = sctol * fabs(f0);
+        if (preduc <= emaxs && (dstnrm > PT_LMAXS || st.newton)) irc = 11;
+        if (st.dst0 >= 0.0) {
+          int i = 0;
+          if ((st.nreduc > 0.0 && st.nreduc <= emax) || (st.nreduc == 0.0 && preduc == 0.0))
+            i = 2;
+          if (st.newton && reldx <= PT_XCTOL && goodx) i += 1;
+          if (i > 0) irc = i + 6;
+        }
+        if (!(irc > 5 && irc != 12) && !st.newton) {
+          /* singular convergence test with a step of length step.min */
+          int test = 0;
+          if (dstnrm > PT_LMAXS) {
+            const double xm = PT_LMAXS / dstnrm;
+            test = !(0.5 * dstnrm <= PT_LMAXS) && !(xm * (2.0 - xm) * preduc >= emaxs);
+          } else {
+            test = !(preduc >= emaxs) && (st.dst0 <= 0.0 || !(0.5 * st.dst0 <= PT_LMAXS));
+          }
+          if (test) {
+            port_step s1 = port_make_step(x0, g0, h0, PT_LMAXS, lo, up);
+            if (s1.preduc <= emaxs) irc = 11;
+          }
+        }
+      }
+      if (irc >= 7 && irc <= 12) {
+        /* converged (or false convergence): nlminb's par is the better of
+         * the candidate and x0 */
+        xacc = fc < f0 ? xc : x0;
+        facc = fc < f0 ? fc : f0;
+        code = irc - 4;
+        converged = 1;
+        break;
+      }
+      if (irc == 3 || irc == 4) {
+        xacc = xc;
+        facc = fc;
+        radfac = radf;
+        acc_irc = irc;
+        accepted = 1;
+        break;
+      }
+      /* irc 5: recompute the step from x0 with the radius scaled */
+      radius = radf * dstnrm;
+      irc_in = 5;
+    }
+    x = xacc;
+    f = facc;
+    if (converged) break;
+    if (accepted) {
+      double fn;
+      mle_fgh(o, x, &fn, &g, &h);
+      /* gradient tests of an IRC = 3 acceptance: a poor model of the
+       * gradient along a step that kept pointing downhill doubles the
+       * radius */
+      const double s = x - x0;
+      if (acc_irc == 3) {
+        const double temp1 = h0 * s + g0 - g;
+        if (fabs(temp1) > PT_TUNER4 * fabs(g0) && g * s < PT_TUNER5 * (g0 * s))
+          radfac = PT_INCFAC;
+      }
+      last_step = fabs(s);
+    }
+  }
+done:
+  *xout = x;
+  *niter_out = niter;
+  *nfev_out = nf;
+  return code;
+}
+
+/* fit_dispersion (R/overdispersion.R:25-115) with nlminb restated: special
+ * cases and start value as orc_fit_dispersion, then orc_port_min1 on
+ * -loglik over [log 1e-16, log 1e16] from log(start) with rel.tol =
+ * tolerance and iter.max = max_iter.  *code receives the PORT return code
+ * (0 for the special cases); *nfev the objective evaluations. */
+double orc_fit_dispersion_port(const double *beta, const double *X, const double *y,
+                               const double *off, int64_t n, int p, double tol, int max_iter,
+                               int cr, int *iters, int *code, int *nfev) {
+  *iters = 0;
+  *code = 0;
+  *nfev = 0;
+  int allzero = 1;
+  for (int64_t i = 0; i < n; ++i)
+    if (y[i] != 0) { allzero = 0; break; }
+  if (allzero) return 0.0;
+  double *mu = (double *)malloc(sizeof(double) * (size_t)n);
+  double ybar = 0.0;
+  for (int64_t i = 0; i < n; ++i) {
+    double eta = 0.0;
+    for (int j = 0; j < p; ++j) eta += X[i + j * n] * beta[j];
+    mu[i] = exp(eta + off[i]);
+    if (mu[i] == 0.0) mu[i] = 1e-6;
+    ybar += y[i];
+  }
+  ybar /= (double)n;
+  double v = 0.0;
+  for (int64_t i = 0; i < n; ++i) v += (y[i] - ybar) * (y[i] - ybar);
+  v /= (double)(n - 1);
+  double start = (v - ybar) / (ybar * ybar);
+  if (isnan(start) || start <= 0) start = 0.5;
+  const double far_left = orc_nb_score_acc(y, mu, n, log(1e-8), X, p, cr);
+  if (far_left < 0) {
+    free(mu);
+    return 0.0;
+  }
+  mle_obj o = {y, mu, X, n, p, cr};
+  double x;
+  *code = orc_port_min1(&o, log(start), log(1e-16), log(1e16), tol, max_iter, &x, iters, nfev);
+  free(mu);
+  return exp(x);
+}
+
+/* ===================================================================== */
 /* matrix-level restatements                                              */
 /* ===================================================================== */
 
@@ -1118,9 +1519,14 @@ int orc_cpu_fit(const orc_fit_args *a, orc_fit_out *o) {
                             a->max_iter, (float)a->tolerance);
       double theta = alpha0[s];
       int th_it = 0;
-      if (a->od_mode == ORC_OD_MLE) {
+      if (a->od_mode == ORC_OD_MLE && a->mle_driver == ORC_MLE_ROOT) {
         theta = orc_fit_dispersion(beta, a->X, y, a->offset, n, p, a->tolerance,
                                    a->max_iter, 1, &th_it);
+      } else if (a->od_mode == ORC_OD_MLE) {
+        int code, nfev;
+        theta = orc_fit_dispersion_port(beta, a->X, y, a->offset, n, p, a->tolerance,
+                                        a->max_iter, 1, &th_it, &code, &nfev);
+        th_it = nfev + 1;  /* passes: the far-left check and each evaluation */
       } else if (a->od_mode == ORC_OD_MOM) {
         theta = orc_mom_dispersion(y, n, a->X, p, beta, eo);
       } else {

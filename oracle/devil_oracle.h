@@ -108,6 +108,17 @@ double orc_nb_dscore_acc(const double *y, const double *mu, int64_t n, double lo
 double orc_fit_dispersion(const double *beta, const double *X, const double *y,
                           const double *off, int64_t n, int p, double tol,
                           int max_iter, int cr, int *iters);
+/* lgamma(a + y) - lgamma(a) - y log(a) without cancellation */
+double orc_lgamma_diff(double a, double y);
+/* conventional_loglikelihood_fast's value, cancellation-free */
+double orc_nb_loglik_acc(const double *y, const double *mu, int64_t n, double log_theta,
+                         const double *X, int p, int cr);
+/* fit_dispersion with stats::nlminb's PORT driver restated (one parameter,
+ * analytic gradient and Hessian, bounds [log 1e-16, log 1e16]); code = the
+ * PORT return code (3-6 converged; 0 special case). */
+double orc_fit_dispersion_port(const double *beta, const double *X, const double *y,
+                               const double *off, int64_t n, int p, double tol, int max_iter,
+                               int cr, int *iters, int *code, int *nfev);
 
 /* ---------------- matrix-level restatements ---------------- */
 /* R/overdispersion.R:129-135 estimate_dispersion */
@@ -145,13 +156,15 @@ typedef struct {
   int32_t n_threads;          /* OpenMP threads, <=0: runtime default */
   const int64_t *genes;       /* optional subset of gene indices (NULL: all) */
   int64_t n_genes_subset;
+  int32_t mle_driver;         /* ORC_MLE_NLMINB (stats::nlminb restated) or ORC_MLE_ROOT */
 } orc_fit_args;
+enum { ORC_MLE_NLMINB = 0, ORC_MLE_ROOT = 1 };
 
 typedef struct {
   double *beta;        /* Gs x p column-major (Gs = G or subset size) */
   double *theta;       /* Gs */
   int32_t *beta_iters; /* Gs */
-  int32_t *theta_iters;/* Gs */
+  int32_t *theta_iters;/* Gs: MLE passes (1 far-left + evaluations), 0 otherwise */
   double *bread;       /* Gs x p x p */
   double *sandwich;    /* Gs x p x p, or NULL */
   double *alpha0;      /* Gs (optional, may be NULL): dispersion_init */

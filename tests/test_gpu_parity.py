@@ -314,11 +314,23 @@ def test_stage_entry_points(dv, orc):
     assert_rel(thn, ref_th, 1e-9, 1e-9, "theta MOM")
     thm = torch.empty(G, dtype=torch.float64, device=dev)
     itm = torch.empty(G, dtype=torch.int32, device=dev)
+    # default driver: nlminb's iteration and stopping rules restated
     ctx.check(lib.devil_fit_dispersion_mle(ctx.h, Yd.data_ptr(), L.COUNTS_I32, G, n, n,
                                            Xd.data_ptr(), p, beta.data_ptr(), off.data_ptr(),
-                                           200, thm.data_ptr(), itm.data_ptr()))
+                                           200, 1e-3, thm.data_ptr(), itm.data_ptr()))
+    ref_mle = [orc.fit_dispersion_port(bn[g], w.X, Yf[g], w.offset)[0] for g in range(G)]
+    assert_rel(thm.cpu().numpy(), ref_mle, 1e-8, 1e-8, "theta MLE (nlminb)")
+    # the root of the score
+    ctx.set_mle_driver("root")
+    try:
+        ctx.check(lib.devil_fit_dispersion_mle(ctx.h, Yd.data_ptr(), L.COUNTS_I32, G, n, n,
+                                               Xd.data_ptr(), p, beta.data_ptr(),
+                                               off.data_ptr(), 200, 1e-3, thm.data_ptr(),
+                                               itm.data_ptr()))
+    finally:
+        ctx.set_mle_driver("nlminb")
     ref_mle = [orc.fit_dispersion(bn[g], w.X, Yf[g], w.offset)[0] for g in range(G)]
-    assert_rel(thm.cpu().numpy(), ref_mle, 1e-8, 1e-8, "theta MLE")
+    assert_rel(thm.cpu().numpy(), ref_mle, 1e-8, 1e-8, "theta MLE (root)")
     ends = t(np.array([1000, 2000, 3000], np.int32), torch.int32)
     br = torch.empty(G * p * p, dtype=torch.float64, device=dev)
     sw = torch.empty(G * p * p, dtype=torch.float64, device=dev)
@@ -376,3 +388,71 @@ def test_estimate_dispersion_large_counts(dv, orc):
                                                  offd.data_ptr(), a0.data_ptr()))
     ref = orc.estimate_dispersion(Y.astype(np.float64), off)
     assert_rel(a0.cpu().numpy(), ref, 1e-10, 1e-12, "alpha0")
+
+
+def _mle_one(dv, y, X, beta, tol, max_iter):
+    """devil_fit_dispersion_mle on one gene (device pointers)."""
+    from devil_amd import _lib as L
+    ctx = dv.context(0)
+    n, p = X.shape
+    t = lambda a, dt=torch.float64: torch.as_tensor(np.ascontiguousarray(a), dtype=dt,
+                                                    device="cuda")
+    Yd = t(np.asarray(y)[None, :], torch.float64)
+    Xd = t(np.asarray(X, np.float64).T.reshape(-1))
+    bd = t(np.asarray(beta, np.float64).reshape(1, p).T.reshape(-1))
+    off = t(np.zeros(n))
+    th = torch.empty(1, dtype=torch.float64, device="cuda")
+    it = torch.empty(1, dtype=torch.int32, device="cuda")
+    ctx.check(dv.lib().devil_fit_dispersion_mle(ctx.h, Yd.data_ptr(), L.COUNTS_F64, 1, n, n,
+                                                Xd.data_ptr(), p, bd.data_ptr(),
+                                                off.data_ptr(), max_iter, tol, th.data_ptr(),
+                                                it.data_ptr()))
+    return float(th.item())
+
+
+def test_mle_reference_scenarios(dv, orc):
+    """test-overdispersion.R:47-152 through the C ABI (both drivers), against
+    the oracle; numpy draws of the same distributions."""
+    import math
+    say("[case] MLE reference scenarios")
+    ctx = dv.context(0)
+    rng = np.random.default_rng(10)
+    n = 200
+    y = rng.poisson(10, n).astype(float)
+    X = np.ones((n, 1))
+    b = [math.log(10)]
+    for drv in ("nlminb", "root"):
+        ctx.set_mle_driver(drv)
+        try:
+            th = _mle_one(dv, np.zeros(20), np.ones((20, 1)), [0.0], 1e-8, 50)
+            assert th == 0.0
+            th = _mle_one(dv, y, X, b, 1e-8, 100)
+            assert np.isfinite(th) and 0 <= th < 0.2
+            ref = (orc.fit_dispersion_port(b, X, y, np.zeros(n), 1e-8, 100)[0] if drv == "nlminb"
+                   else orc.fit_dispersion(b, X, y, np.zeros(n), 1e-8, 100)[0])
+            assert abs(th - ref) <= 1e-8 * max(ref, 1e-8)
+            r2 = np.random.default_rng(11)
+            yp = r2.poisson(10, 400).astype(float)
+            ynb = r2.poisson(r2.gamma(2.0, 5.0, 400)).astype(float)
+            X4 = np.ones((400, 1))
+            dp = _mle_one(dv, yp, X4, b, 1e-8, 100)
+            dnb = _mle_one(dv, ynb, X4, b, 1e-8, 100)
+            assert np.isfinite(dnb) and dnb > dp and abs(dnb - 0.5) <= 0.15
+        finally:
+            ctx.set_mle_driver("nlminb")
+
+
+def test_fit_mle_root_driver(dv, orc):
+    """The root-of-the-score driver through fit_devil against the oracle's."""
+    from devil_amd import synth
+    w = synth.make_workload("C2", G=64, n=3000, seed=21)
+    Y = synth.counts_host(w)
+    ctx = dv.context(0)
+    ctx.set_mle_driver("root")
+    try:
+        fit = dv.fit_devil(Y, w.X, overdispersion="MLE", size_factors=w.sf)
+    finally:
+        ctx.set_mle_driver("nlminb")
+    ref = orc.cpu_fit(Y.astype(np.float64), w.X, w.offset, od_mode=orc.OD_MLE,
+                      mle_driver="root")
+    check_fit(fit, ref, theta_tol=1e-8)
