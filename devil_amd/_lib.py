@@ -82,6 +82,11 @@ def lib():
                                        _vp, i32, _vp, _vp, _vp]),
         "devil_test_de": (C.c_int, [_vp, _vp, _vp, _vp, i64, i32, i64, _vp, d, _vp, _vp, _vp,
                                     _vp]),
+        "devil_sandwich_hc0": (C.c_int, [_vp, _vp, i32, i64, i64, i64, _vp, i32, _vp, _vp,
+                                         _vp, _vp, _vp, _vp]),
+        "devil_size_factors": (C.c_int, [_vp, _vp, i32, i64, i64, i64, i32, _vp]),
+        "devil_group_cells": (C.c_int, [_vp, _vp, i32, i64, i64, i64, _vp, _vp, i64]),
+        "devil_csc_to_dense": (C.c_int, [_vp, _vp, _vp, _vp, i64, i64, i32, _vp, i64]),
         "devil_set_timing": (C.c_int, [_vp, C.c_int]),
         "devil_stage_times": (C.c_int, [_vp, C.POINTER(C.c_double)]),
         "devil_set_tuning": (C.c_int, [_vp, C.c_int32, C.c_int64]),
@@ -100,13 +105,17 @@ def lib():
 EXPORTED = ["devil_create", "devil_destroy", "devil_strerror", "devil_last_error",
             "devil_version", "devil_fit", "devil_estimate_dispersion", "devil_init_beta",
             "devil_beta_fit", "devil_mom_dispersion", "devil_fit_dispersion_mle",
-            "devil_bread_meat", "devil_test_de", "devil_set_timing", "devil_stage_times",
+            "devil_bread_meat", "devil_test_de",
+            "devil_sandwich_hc0", "devil_size_factors", "devil_group_cells",
+            "devil_csc_to_dense",
+            "devil_set_timing", "devil_stage_times",
             "devil_kernel_stats", "devil_set_tuning", "devil_fit_info"]
 
 STAGES = ["init", "irls", "dispersion", "bread_meat", "test_de"]
 KERNELS = ["irls_fp64", "irls_mixed", "mle"]
 TUNE_MIXED_KERNEL, TUNE_CHUNK_CELLS, TUNE_CONCURRENCY, TUNE_MLE_DRIVER = 0, 1, 2, 3
 MLE_DRIVERS = {"nlminb": 0, "root": 1}
+SF_METHODS = {"normed_sum": 0, "psinorm": 1, "edgeR": 2}
 MIXED_KERNELS = {"f32": 0, "bf16": 1, "auto": 2, "bf16_ws": 3, "bf16_ws2": 4}
 
 

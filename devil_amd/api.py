@@ -52,6 +52,128 @@ def cluster_blocks(clusters, n: int) -> np.ndarray:
     return ends
 
 
+def _is_sparse(x) -> bool:
+    return type(x).__module__.startswith("scipy.sparse")
+
+
+def densify(x, device: int = 0):
+    """A scipy.sparse G x n count matrix (R's dgCMatrix: CSC, column = cell)
+    as a dense gene-major torch tensor on the GPU, built by the library's
+    scatter kernel (devil_csc_to_dense).  int32 when every value is integral
+    and fits, else float64."""
+    import torch
+    csc = x.tocsc()
+    G, n = csc.shape
+    dev = torch.device("cuda", device)
+    data = np.asarray(csc.data, dtype=np.float64)
+    integral = bool(np.all(np.floor(data) == data)) and (
+        data.size == 0 or (data.min() >= -2**31 and data.max() < 2**31))
+    out_dtype = L.COUNTS_I32 if integral else L.COUNTS_F64
+    out = torch.empty((G, n), dtype=torch.int32 if integral else torch.float64, device=dev)
+    colptr = torch.as_tensor(np.asarray(csc.indptr, dtype=np.int64), device=dev)
+    rowind = torch.as_tensor(np.asarray(csc.indices, dtype=np.int32), device=dev)
+    vals = torch.as_tensor(data, device=dev)
+    ctx = L.context(device)
+    ctx.check(L.lib().devil_csc_to_dense(ctx.h, colptr.data_ptr(), rowind.data_ptr(),
+                                         vals.data_ptr(), G, n, out_dtype, out.data_ptr(), n))
+    return out
+
+
+def calculate_sf(Y, method: str = "normed_sum", verbose: bool = False, device: int = 0):
+    """R/size_factors.R:37-127 on the GPU (devil_size_factors): "normed_sum"
+    or "psinorm"; "edgeR" raises as the reference does without edgeR.
+    ``Y`` is a genes x cells torch CUDA tensor (int32, float32 or float64);
+    returns a float64 tensor of n size factors."""
+    import torch
+    if method not in L.SF_METHODS:
+        raise ValueError(f"'arg' should be one of \"normed_sum\", \"psinorm\", \"edgeR\"")
+    if verbose:
+        print(f"Calculating size factors using method: {method}")
+    if not Y.is_contiguous() and Y.stride(1) != 1:
+        Y = Y.contiguous()
+    G, n = (int(v) for v in Y.shape)
+    sf = torch.empty(n, dtype=torch.float64, device=Y.device)
+    ctx = L.context(Y.device.index or 0)
+    ctx.check(L.lib().devil_size_factors(ctx.h, Y.data_ptr(), _counts_dtype(Y.dtype), G, n,
+                                         Y.stride(0), L.SF_METHODS[method], sf.data_ptr()))
+    return sf
+
+
+def group_data(input_matrix, design_matrix, clusters, device: int = 0):
+    """R/group_data.R:38-69: cells reordered so that each cluster is one
+    contiguous block (clusters coded by first appearance, stable order).  The
+    count matrix is permuted on the GPU (devil_group_cells); returns
+    dict(input_matrix, design_matrix, clusters) like the reference's list."""
+    import torch
+    n = int(input_matrix.shape[1])
+    if int(design_matrix.shape[0]) != n:
+        raise ValueError("Dimension mismatch: ncol(input_matrix) must equal nrow(design_matrix)")
+    lab = np.asarray(clusters.cpu() if _is_torch(clusters) else clusters)
+    if lab.shape[0] != n:
+        raise ValueError("Dimension mismatch: length(clusters) must match the number of cells")
+    _, first, inv = np.unique(lab, return_index=True, return_inverse=True)
+    rank = np.empty(len(first), np.int64)
+    rank[np.argsort(first)] = np.arange(len(first))
+    codes = rank[inv] + 1
+    order = np.argsort(codes, kind="stable")
+    Y = input_matrix
+    on_host = not (_is_torch(Y) and Y.is_cuda)
+    dev = torch.device("cuda", device) if on_host else Y.device
+    if on_host:
+        Y = torch.as_tensor(np.ascontiguousarray(Y), device=dev)
+    if not Y.is_contiguous() and Y.stride(1) != 1:
+        Y = Y.contiguous()
+    G = int(Y.shape[0])
+    out = torch.empty((G, n), dtype=Y.dtype, device=dev)
+    ordd = torch.as_tensor(order, device=dev)
+    ctx = L.context(dev.index or 0)
+    ctx.check(L.lib().devil_group_cells(ctx.h, Y.data_ptr(), _counts_dtype(Y.dtype), G, n,
+                                        Y.stride(0), ordd.data_ptr(), out.data_ptr(), n))
+    if _is_torch(design_matrix):
+        X = design_matrix[torch.as_tensor(order, device=design_matrix.device)]
+    else:
+        X = np.asarray(design_matrix)[order]
+    return {"input_matrix": out.cpu().numpy() if on_host else out, "design_matrix": X,
+            "clusters": codes[order].astype(np.float64)}
+
+
+def compute_sandwich(design_matrix, y, beta, overdispersion, size_factors, clusters=None,
+                     device: int = 0):
+    """R/variance.R:16-26 for every gene at once on the GPU: bread meat bread n
+    with the clustered meat (contiguous patient blocks, as fit_devil requires)
+    or, without clusters, the HC0 meat (1/n) sum_i s_i s_i' of compute_meat.
+    ``y`` is genes x cells, ``beta`` genes x p, ``overdispersion`` per gene.
+    The bread uses theta as fit_devil's does.  Returns a genes x p x p tensor."""
+    import torch
+    dev = torch.device("cuda", device)
+    Y = y if (_is_torch(y) and y.is_cuda) else torch.as_tensor(np.ascontiguousarray(y),
+                                                               device=dev)
+    if not Y.is_contiguous() and Y.stride(1) != 1:
+        Y = Y.contiguous()
+    G, n = (int(v) for v in Y.shape)
+    f64 = lambda a: (a.to(device=dev, dtype=torch.float64) if _is_torch(a)
+                     else torch.as_tensor(np.asarray(a, dtype=np.float64), device=dev))
+    X = f64(design_matrix)
+    p = int(X.shape[1])
+    Xd = X.t().contiguous().reshape(-1)
+    bcol = f64(beta).reshape(G, p).t().contiguous().reshape(-1)
+    th = f64(overdispersion).reshape(-1).contiguous()
+    off = torch.log(f64(size_factors).reshape(-1)).contiguous()
+    bread = torch.empty(G * p * p, dtype=torch.float64, device=dev)
+    sw = torch.empty(G * p * p, dtype=torch.float64, device=dev)
+    ctx = L.context(device)
+    lib = L.lib()
+    args = (ctx.h, Y.data_ptr(), _counts_dtype(Y.dtype), G, n, Y.stride(0), Xd.data_ptr(), p,
+            bcol.data_ptr(), th.data_ptr(), off.data_ptr())
+    if clusters is None:
+        ctx.check(lib.devil_sandwich_hc0(*args, bread.data_ptr(), sw.data_ptr(), None))
+    else:
+        ends = torch.as_tensor(cluster_blocks(clusters, n), dtype=torch.int32, device=dev)
+        ctx.check(lib.devil_bread_meat(*args, ends.data_ptr(), int(ends.numel()),
+                                       bread.data_ptr(), sw.data_ptr(), None))
+    return sw.view(G, p, p).transpose(1, 2)
+
+
 def compute_offset_vector(off, n: int, size_factors) -> np.ndarray:
     """R/size_factors.R:172-184: rep(off, n) + log(size_factors)."""
     v = np.full(n, float(off))
@@ -67,12 +189,22 @@ class _Job:
     def __init__(self, x, design_matrix, clusters=None, overdispersion: Any = "MOM",
                  init_overdispersion=None, init_beta_rough: bool = False, offset: float = 0,
                  size_factors=None, max_iter: int = 200, tolerance: float = 1e-3,
-                 precision: str = "fp64", init_mode: str | None = None):
+                 precision: str = "fp64", init_mode: str | None = None, device: int = 0):
         # a pandas DataFrame carries the gene names as its index (R's rownames)
         self.gene_names = None
         if hasattr(x, "index") and hasattr(x, "to_numpy"):
             self.gene_names = list(x.index)
             x = x.to_numpy()
+        # sparse assays (R/main.R:1-9): densified on the GPU, gene-major
+        if _is_sparse(x):
+            x = densify(x, device)
+        # size factors estimated from the counts (R/main.R:215-224) need the
+        # counts on the GPU; the fit then reads that copy
+        if isinstance(size_factors, str):
+            if not (_is_torch(x) and x.is_cuda):
+                import torch
+                x = torch.as_tensor(np.ascontiguousarray(x), device=torch.device("cuda", device))
+            size_factors = calculate_sf(x, size_factors)
         self.on_device = _is_torch(x) and x.is_cuda
         x_dev = _is_torch(design_matrix) and design_matrix.is_cuda and self.on_device
         sf_dev = _is_torch(size_factors) and size_factors.is_cuda and self.on_device
@@ -96,9 +228,6 @@ class _Job:
             od = _OD_MODES[overdispersion]
         else:
             raise ValueError(f"Unknown overdispersion mode: {overdispersion}")
-        if isinstance(size_factors, str):
-            raise NotImplementedError("size-factor estimation is outside the fitted path; "
-                                      "pass numeric size_factors")
         if sf_dev:
             import torch
             sf = size_factors.to(torch.float64).reshape(-1)
@@ -236,7 +365,7 @@ def fit_devil(x, design_matrix, clusters=None, overdispersion: Any = "MOM",
     job = _Job(x, design_matrix, clusters=clusters, overdispersion=overdispersion,
                init_overdispersion=init_overdispersion, init_beta_rough=init_beta_rough,
                offset=offset, size_factors=size_factors, max_iter=max_iter,
-               tolerance=tolerance, precision=precision, init_mode=init_mode)
+               tolerance=tolerance, precision=precision, init_mode=init_mode, device=device)
     ctx.check(L.lib().devil_fit(ctx.h, C.byref(job.prob), C.byref(job.outs), 1))
     return job.result()
 
@@ -250,7 +379,7 @@ def fit_devil_many(problems, device: int = 0, **common):
     Returns the list of fit_devil results in the same order.
     """
     ctx = L.context(device)
-    jobs = [_Job(**{**common, **pr}) for pr in problems]
+    jobs = [_Job(**{**common, **pr, "device": device}) for pr in problems]
     if not jobs:
         return []
     probs = (L.Problem * len(jobs))(*[j.prob for j in jobs])

@@ -21,7 +21,7 @@ LIB = os.path.join(HERE, "libdevil_hip.so")
 ROOT = os.path.dirname(HERE)
 
 SOURCES = ["kernels_epilogue.hip", "kernels_tile.hip", "kernels_mle.hip", "kernels_post.hip",
-           "kernels_design.hip", "devil_hip.cpp"]
+           "kernels_design.hip", "kernels_prep.hip", "devil_hip.cpp"]
 
 COMMON = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall",
@@ -37,8 +37,18 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found (ROCm toolchain required to build devil_amd)")
 
 
-def _compile(src: str, verbose: bool) -> str:
+def _headers_mtime() -> float:
+    paths = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hpp")]
+    paths.append(os.path.join(ROOT, "include", "devil_hip.h"))
+    return max(os.path.getmtime(q) for q in paths)
+
+
+def _compile(src: str, verbose: bool, incremental: bool = False) -> str:
     obj = os.path.join(BUILD, os.path.basename(src) + ".o")
+    if incremental and os.path.exists(obj):
+        t = os.path.getmtime(obj)
+        if t > os.path.getmtime(os.path.join(CSRC, src)) and t > _headers_mtime():
+            return obj
     cmd = [hipcc(), "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj] + COMMON
     if verbose:
         print(" ".join(cmd), flush=True)
@@ -52,11 +62,12 @@ def _compile(src: str, verbose: bool) -> str:
     return obj
 
 
-def build(verbose: bool = False) -> str:
-    """Compile every HIP source and link libdevil_hip.so; returns its path."""
+def build(verbose: bool = False, incremental: bool = False) -> str:
+    """Compile every HIP source and link libdevil_hip.so; returns its path.
+    incremental: keep objects newer than their source and every header."""
     os.makedirs(BUILD, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
-        objs = list(ex.map(lambda s: _compile(s, verbose), SOURCES))
+        objs = list(ex.map(lambda s: _compile(s, verbose, incremental), SOURCES))
     tmp = LIB + ".tmp"
     cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
     if verbose:
@@ -69,4 +80,4 @@ def build(verbose: bool = False) -> str:
 
 
 if __name__ == "__main__":
-    print(build(verbose=True))
+    print(build(verbose=True, incremental="--incremental" in sys.argv))

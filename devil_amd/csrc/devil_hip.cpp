@@ -825,8 +825,11 @@ int build_chunks(devil_ctx* ctx, int64_t n, const int32_t* ends, int C, ChunkHos
   return DEVIL_OK;
 }
 
-size_t post_part_doubles(const dv::TileDesign& td, const ChunkHost& ch, int64_t G) {
-  return (size_t)((G + 15) / 16) * ch.ct.nchunk * 16 * (td.ncp + 16);
+// partial sums of the post-IRLS passes; hc0: the bread pass also keeps the
+// HC0 meat columns
+size_t post_part_doubles(const dv::TileDesign& td, const ChunkHost& ch, int64_t G,
+                         bool hc0 = false) {
+  return (size_t)((G + 15) / 16) * ch.ct.nchunk * 16 * (td.ncp + 16 + (hc0 ? td.ncp : 0));
 }
 
 // IRLS over genes [0, G) of the counts view: rounds of accumulate + update
@@ -1152,18 +1155,33 @@ int run_mle_tiled(devil_ctx* ctx, const dv::TileDesign& td, dv::Counts c, int64_
   DV_HIP(hipMemcpyAsync(&nA, ncur, sizeof(int32_t), hipMemcpyDeviceToHost, s));
   DV_HIP(hipStreamSynchronize(s));
   const double row_bytes = (double)count_elem_size(c.dtype) * (double)n;
+  const size_t part_cap = (size_t)ntile * td.nchunk * 16 * W;
   // passes: the far-left check, then at most max_iter + 1 Newton passes (root
   // driver) or eval.max = 200 objective evaluations (nlminb driver)
   const int max_rounds = std::max(max_iter, 200) + 2;
   for (int round = 0; round < max_rounds && nA > 0; ++round) {
+    // rounds of few genes get shorter chunks (more workgroups), within the
+    // partial buffer
+    dv::TileDesign tr = td;
+    {
+      const int64_t nt = (nA + 15) / 16, groups = (nt + 3) / 4;
+      const int64_t cap = (int64_t)(part_cap / ((size_t)nt * 16 * W));
+      const int64_t want = std::min({(kRoundWorkgroups + groups - 1) / groups, kMaxRoundChunks,
+                                     cap});
+      if (groups * td.nchunk < kRoundWorkgroups && want > td.nchunk) {
+        dv::TileDesign t2 = td;
+        set_chunks(t2, std::max(kMinRoundBlocks, (td.nblk + want - 1) / want));
+        if (t2.nchunk > td.nchunk && t2.nchunk <= cap) tr = t2;
+      }
+    }
     {
       LaunchTimer lt(ctx, DEVIL_KERNEL_MLE, row_bytes * (double)nA);
-      DV_HIP(dv::launch_mle_accum(s, td, c, p, cur, ncur, nA, beta, ms.x, ms.tab_n,
+      DV_HIP(dv::launch_mle_accum(s, tr, c, p, cur, ncur, nA, beta, ms.x, ms.tab_n,
                                   driver == dv::kMleDriverNlminb && round > 0,
                                   part.as<double>()));
     }
     DV_HIP(dv::launch_fill_i32(s, nnxt, 1, 0));
-    DV_HIP(dv::launch_mle_update(s, td, p, cur, ncur, nA, part.as<double>(), ms, n, max_iter,
+    DV_HIP(dv::launch_mle_update(s, tr, p, cur, ncur, nA, part.as<double>(), ms, n, max_iter,
                                  tolerance, nxt, nnxt));
     int32_t h = 0;
     DV_HIP(hipMemcpyAsync(&h, nnxt, sizeof(int32_t), hipMemcpyDeviceToHost, s));
@@ -1705,6 +1723,104 @@ int devil_kernel_stats(const devil_ctx* ctx, int32_t kernel, int64_t* launches, 
 // stage entry points (device pointers)
 // ---------------------------------------------------------------------------
 
+int devil_size_factors(devil_ctx* ctx, const void* counts, int32_t dtype, int64_t G,
+                       int64_t n, int64_t ld, int32_t method, double* sf) {
+  int rc;
+  if ((rc = set_device(ctx))) return rc;
+  if ((rc = check_counts_args(ctx, counts, dtype, G, n, ld))) return rc;
+  if (n > 0 && !sf) return fail(ctx, DEVIL_E_ARG, "NULL pointer");
+  if (method == DEVIL_SF_EDGER)
+    return fail(ctx, DEVIL_E_UNSUPPORTED,
+                "the \"edgeR\" method (TMM with singleton pairing) needs the edgeR package; "
+                "use normed_sum or psinorm");
+  if (method != DEVIL_SF_NORMED_SUM && method != DEVIL_SF_PSINORM)
+    return fail(ctx, DEVIL_E_ARG, "unknown size factor method");
+  if (n == 0) return DEVIL_OK;
+  hipStream_t s = ctx->stream;
+  if (G <= 1) {  // R/size_factors.R:38-42: unit factors
+    DV_HIP(dv::launch_fill_f64(s, sf, n, 1.0));
+    DV_HIP(hipStreamSynchronize(s));
+    return DEVIL_OK;
+  }
+  // enough (cell block, gene slice) workgroups to fill the GPU
+  const int64_t cell_blocks = (n + 1023) / 1024;
+  const int slices = (int)std::max<int64_t>(
+      1, std::min<int64_t>({(4096 + cell_blocks - 1) / cell_blocks, G, 256}));
+  DevBuf part, dmin, logsf, flag;
+  DV_ALLOC(part, sizeof(double) * (size_t)slices * n);
+  if (method == DEVIL_SF_PSINORM) DV_ALLOC(dmin, sizeof(double) * (size_t)slices * n);
+  DV_ALLOC(logsf, sizeof(double) * n);
+  DV_ALLOC(flag, sizeof(int32_t));
+  DV_HIP(dv::launch_fill_i32(s, flag.as<int32_t>(), 1, 0));
+  dv::Counts c{counts, dtype, ld};
+  DV_HIP(dv::launch_size_factors(s, c, G, n, method, slices, part.as<int64_t>(),
+                                 part.as<double>(), dmin.as<double>(), sf, logsf.as<double>(),
+                                 flag.as<int32_t>()));
+  int32_t zero = 0;
+  DV_HIP(hipMemcpyAsync(&zero, flag.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  DV_HIP(hipStreamSynchronize(s));
+  if (zero)
+    return fail(ctx, DEVIL_E_ARG,
+                "At least one column (sample) contains all zeros, unable to compute size "
+                "factor. Please filter out empty samples from input matrix.");
+  return DEVIL_OK;
+}
+
+int devil_group_cells(devil_ctx* ctx, const void* counts, int32_t dtype, int64_t G, int64_t n,
+                      int64_t ld, const int64_t* order, void* out, int64_t out_ld) {
+  int rc;
+  if ((rc = set_device(ctx))) return rc;
+  if ((rc = check_counts_args(ctx, counts, dtype, G, n, ld))) return rc;
+  if (G > 0 && n > 0 && (!order || !out)) return fail(ctx, DEVIL_E_ARG, "NULL pointer");
+  if (out_ld < n) return fail(ctx, DEVIL_E_ARG, "out_ld < n");
+  if (G == 0 || n == 0) return DEVIL_OK;
+  hipStream_t s = ctx->stream;
+  DevBuf flag;
+  DV_ALLOC(flag, sizeof(int32_t));
+  DV_HIP(dv::launch_fill_i32(s, flag.as<int32_t>(), 1, 0));
+  DV_HIP(dv::launch_check_order(s, order, n, flag.as<int32_t>()));
+  int32_t bad = 0;
+  DV_HIP(hipMemcpyAsync(&bad, flag.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  DV_HIP(hipStreamSynchronize(s));
+  if (bad) return fail(ctx, DEVIL_E_ARG, "order holds a cell index outside [0, n)");
+  dv::Counts c{counts, dtype, ld};
+  DV_HIP(dv::launch_group_cells(s, c, G, n, order, out, out_ld));
+  DV_HIP(hipStreamSynchronize(s));
+  return DEVIL_OK;
+}
+
+int devil_csc_to_dense(devil_ctx* ctx, const int64_t* colptr, const int32_t* rowind,
+                       const double* values, int64_t G, int64_t n, int32_t out_dtype,
+                       void* out, int64_t out_ld) {
+  int rc;
+  if ((rc = set_device(ctx))) return rc;
+  if (G < 0 || n < 0) return fail(ctx, DEVIL_E_ARG, "negative dimension");
+  if (out_dtype != DEVIL_COUNTS_I32 && out_dtype != DEVIL_COUNTS_F32 &&
+      out_dtype != DEVIL_COUNTS_F64)
+    return fail(ctx, DEVIL_E_ARG, "unknown output dtype");
+  if (G == 0 || n == 0) return DEVIL_OK;
+  if (!colptr || !out) return fail(ctx, DEVIL_E_ARG, "NULL pointer");
+  if (out_ld < n) return fail(ctx, DEVIL_E_ARG, "out_ld < n");
+  hipStream_t s = ctx->stream;
+  int64_t nnz = 0;
+  DV_HIP(hipMemcpyAsync(&nnz, colptr + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  DV_HIP(hipStreamSynchronize(s));
+  if (nnz < 0) return fail(ctx, DEVIL_E_ARG, "colptr[n] < 0");
+  if (nnz > 0 && (!rowind || !values)) return fail(ctx, DEVIL_E_ARG, "NULL pointer");
+  DevBuf flag;
+  DV_ALLOC(flag, sizeof(int32_t));
+  DV_HIP(dv::launch_fill_i32(s, flag.as<int32_t>(), 1, 0));
+  DV_HIP(dv::launch_check_csc(s, colptr, rowind, G, n, nnz, flag.as<int32_t>()));
+  int32_t bad = 0;
+  DV_HIP(hipMemcpyAsync(&bad, flag.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  DV_HIP(hipStreamSynchronize(s));
+  if (bad & 1) return fail(ctx, DEVIL_E_ARG, "colptr not non-decreasing from 0 to nnz");
+  if (bad & 2) return fail(ctx, DEVIL_E_ARG, "a row index lies outside [0, G)");
+  DV_HIP(dv::launch_csc_to_dense(s, colptr, rowind, values, G, n, out_dtype, out, out_ld));
+  DV_HIP(hipStreamSynchronize(s));
+  return DEVIL_OK;
+}
+
 int devil_estimate_dispersion(devil_ctx* ctx, const void* counts, int32_t dtype, int64_t G,
                               int64_t n, int64_t ld, const double* offset, double* alpha0) {
   int rc;
@@ -1900,6 +2016,33 @@ int devil_bread_meat(devil_ctx* ctx, const void* counts, int32_t dtype, int64_t 
   DV_HIP(dv::launch_bread_tiled(s, th.td, chunks.ct, c, G, n, p, bg.as<double>(), theta,
                                 n_clusters, post.as<double>(), bread,
                                 n_clusters > 0 ? sandwich : nullptr, status));
+  DV_HIP(hipStreamSynchronize(s));
+  return DEVIL_OK;
+}
+
+int devil_sandwich_hc0(devil_ctx* ctx, const void* counts, int32_t dtype, int64_t G, int64_t n,
+                       int64_t ld, const double* X, int32_t p, const double* beta,
+                       const double* theta, const double* offset, double* bread,
+                       double* sandwich, int32_t* status) {
+  int rc;
+  if ((rc = set_device(ctx))) return rc;
+  if ((rc = check_p(ctx, p))) return rc;
+  if ((rc = check_counts_args(ctx, counts, dtype, G, n, ld))) return rc;
+  if (!X || !offset || (G > 0 && (!beta || !theta || !bread || !sandwich)))
+    return fail(ctx, DEVIL_E_ARG, "NULL pointer");
+  if (G == 0) return DEVIL_OK;
+  hipStream_t s = ctx->stream;
+  TileHost th;
+  if ((rc = tile_from_device(ctx, X, offset, n, p, th))) return rc;
+  ChunkHost chunks;
+  if ((rc = build_chunks(ctx, n, nullptr, 0, chunks))) return rc;
+  DevBuf bg, post;
+  DV_ALLOC(bg, sizeof(double) * G * p);
+  DV_ALLOC(post, sizeof(double) * post_part_doubles(th.td, chunks, G, true));
+  DV_HIP(dv::launch_transpose_gp(s, beta, G, p, 1, bg.as<double>()));
+  dv::Counts c{counts, dtype, ld};
+  DV_HIP(dv::launch_bread_tiled(s, th.td, chunks.ct, c, G, n, p, bg.as<double>(), theta, -1,
+                                post.as<double>(), bread, sandwich, status));
   DV_HIP(hipStreamSynchronize(s));
   return DEVIL_OK;
 }

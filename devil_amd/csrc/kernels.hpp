@@ -205,7 +205,7 @@ struct MleState {
   PortState* port; // per-gene state of the nlminb driver
   int driver;      // kMleDriver*
 };
-constexpr int kMleTabCap = 256;
+constexpr int kMleTabCap = 4096;  // one entry per integer count below 4096
 enum { kMleDriverNlminb = 0, kMleDriverRoot = 1 };
 
 hipError_t launch_mle_setup(hipStream_t s, Counts c, int64_t G, int64_t n, int p,
@@ -241,5 +241,18 @@ hipError_t launch_transpose_gp(hipStream_t s, const double* src, int64_t G, int 
                                int to_gene_major, double* dst);
 
 hipError_t launch_fill_i32(hipStream_t s, int32_t* p, int64_t count, int32_t v);
+
+// count-matrix preparation (kernels_prep.hip)
+hipError_t launch_size_factors(hipStream_t s, Counts c, int64_t G, int64_t n, int method,
+                               int slices, int64_t* isum, double* dsum, double* dmin,
+                               double* sf, double* logsf, int32_t* zero_flag);
+hipError_t launch_group_cells(hipStream_t s, Counts c, int64_t G, int64_t n,
+                              const int64_t* ord, void* out, int64_t ld_out);
+hipError_t launch_csc_to_dense(hipStream_t s, const int64_t* colptr, const int32_t* rowind,
+                               const double* values, int64_t G, int64_t n, int out_dtype,
+                               void* out, int64_t ld);
+hipError_t launch_check_order(hipStream_t s, const int64_t* ord, int64_t n, int32_t* flag);
+hipError_t launch_check_csc(hipStream_t s, const int64_t* colptr, const int32_t* rowind,
+                            int64_t G, int64_t n, int64_t nnz, int32_t* flag);
 
 }  // namespace dv

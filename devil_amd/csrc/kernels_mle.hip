@@ -78,69 +78,128 @@ enum { kMleFarLeft = 0, kMleNewton = 1, kMleDone = 2 };
 constexpr int kMleScal = 8;  // sc, l1, l2, dg, tg, ld (cell ldiff), sum y L, sum L
 
 // Per-gene table of the distinct positive counts (value, frequency), the
-// make_table_if_small idea of src/overdispersion.cpp:32-46: the digamma and
-// trigamma sums then cost O(#distinct) per Newton round instead of O(n).
-// Counts of 4096 or more, fractional counts, or more than kMleTabCap distinct
-// values leave the gene on the per-cell path (tab_n = -1).
+// make_table_if_small idea of src/overdispersion.cpp:32-46: the digamma,
+// trigamma and lgamma sums then cost O(#distinct) per pass instead of O(n).
+// Every integer count below kMleHist has its bin, so only genes with a count
+// of 4096 or more (or fractional counts) stay on the per-cell path
+// (tab_n = -1).
 constexpr int kMleHist = 4096;
+static_assert(kMleTabCap == kMleHist, "one table entry per possible bin");
+constexpr int kSetupThreads = 256;
 
+// One workgroup of 256 threads per gene, one read of its counts: the
+// histogram in LDS, sum y, sum y^2 (integer counts: exact, 64 and 128 bits),
+// max y and the zero count; float counts take a second read for the
+// variance about the mean.
 template <int P>
-__global__ __launch_bounds__(64) void k_mle_setup(Counts c, int64_t G, int64_t n,
-                                                  MleState ms) {
+__global__ __launch_bounds__(kSetupThreads) void k_mle_setup(Counts c, int64_t G, int64_t n,
+                                                             MleState ms) {
   __shared__ int32_t hist[kMleHist];
   __shared__ int32_t big;
+  __shared__ int64_t w1[kSetupThreads / 64], wz[kSetupThreads / 64];
+  __shared__ uint64_t wlo[kSetupThreads / 64], whi[kSetupThreads / 64];
+  __shared__ double wd1[kSetupThreads / 64], wmx[kSetupThreads / 64];
+  __shared__ int32_t wcnt[kSetupThreads / 64];
   const int64_t g = blockIdx.x;
   if (g >= G) return;
-  const int lane = threadIdx.x;
-  for (int k = lane; k < kMleHist; k += 64) hist[k] = 0;
-  if (lane == 0) big = 0;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  constexpr int NW = kSetupThreads / 64;
+  for (int k = t; k < kMleHist; k += kSetupThreads) hist[k] = 0;
+  if (t == 0) big = 0;
   __syncthreads();
   const void* row = count_row(c.ptr, c.dtype, g, c.ld);
-  double s1 = 0.0, spy = 0.0, my = 0.0, nz = 0.0;
-  for (int64_t i = lane; i < n; i += kWave) {
-    const double y = load_count(row, c.dtype, i);
-    s1 += y;
-    spy += (y - 1) * y;
+  const bool integer = c.dtype == DEVIL_COUNTS_I32;
+  int64_t s1i = 0, nz = 0;
+  uint64_t s2lo = 0, s2hi = 0;
+  double s1d = 0.0, my = 0.0;
+  for (int64_t i = t; i < n; i += kSetupThreads) {
+    double y;
+    if (integer) {
+      const int64_t yi = static_cast<const int32_t*>(row)[i];
+      s1i += yi;
+      const uint64_t q = (uint64_t)(yi * yi);
+      s2lo += q;
+      s2hi += s2lo < q ? 1 : 0;
+      y = (double)yi;
+    } else {
+      y = load_count(row, c.dtype, i);
+      s1d += y;
+    }
     my = fmax(my, y);
-    nz += (y == 0.0) ? 1.0 : 0.0;
+    nz += y == 0.0 ? 1 : 0;
     if (y > 0.0) {
       if (y < (double)kMleHist && y == floor(y)) atomicAdd(&hist[(int)y], 1);
       else big = 1;
     }
   }
-  s1 = wave_sum(s1);
-  spy = wave_sum(spy);
-  my = wave_max(my);
-  nz = wave_sum(nz);
+  // wave reductions, then across the four waves
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    s1i += __shfl_xor(s1i, o, 64);
+    nz += __shfl_xor(nz, o, 64);
+    const uint64_t lo = __shfl_xor(s2lo, o, 64), hi = __shfl_xor(s2hi, o, 64);
+    s2lo += lo;
+    s2hi += hi + (s2lo < lo ? 1 : 0);
+    s1d += __shfl_xor(s1d, o, 64);
+    my = fmax(my, __shfl_xor(my, o, 64));
+  }
+  if (lane == 0) {
+    w1[wave] = s1i;
+    wz[wave] = nz;
+    wlo[wave] = s2lo;
+    whi[wave] = s2hi;
+    wd1[wave] = s1d;
+    wmx[wave] = my;
+  }
   __syncthreads();
+  s1i = 0;
+  nz = 0;
+  s1d = 0.0;
+  my = 0.0;
+  unsigned __int128 s2 = 0;
+#pragma unroll
+  for (int w = 0; w < NW; ++w) {
+    s1i += w1[w];
+    nz += wz[w];
+    s2 += ((unsigned __int128)whi[w] << 64) | wlo[w];
+    s1d += wd1[w];
+    my = fmax(my, wmx[w]);
+  }
   // compact the non-zero bins (ascending value) into the gene's table
   {
-    constexpr int per = kMleHist / 64;
+    constexpr int per = kMleHist / kSetupThreads;
     int cnt = 0;
-    for (int k = lane * per; k < (lane + 1) * per; ++k) cnt += hist[k] > 0 ? 1 : 0;
-    int pre = cnt;  // inclusive prefix sum over lanes
+    for (int k = t * per; k < (t + 1) * per; ++k) cnt += hist[k] > 0 ? 1 : 0;
+    int pre = cnt;  // inclusive prefix sum within the wave
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
       const int v = __shfl_up(pre, o, 64);
       if (lane >= o) pre += v;
     }
-    const int total = __shfl(pre, 63, 64);
+    if (lane == 63) wcnt[wave] = pre;
+    __syncthreads();
+    int base = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      if (w < wave) base += wcnt[w];
+      total += wcnt[w];
+    }
     const bool ok = big == 0 && total <= kMleTabCap;
     if (ok) {
-      int pos = pre - cnt;
+      int pos = base + pre - cnt;
       int32_t* tu = ms.tab_u + g * kMleTabCap;
       int32_t* tf = ms.tab_f + g * kMleTabCap;
-      for (int k = lane * per; k < (lane + 1) * per; ++k)
+      for (int k = t * per; k < (t + 1) * per; ++k)
         if (hist[k] > 0) {
           tu[pos] = k;
           tf[pos] = hist[k];
           ++pos;
         }
     }
-    if (lane == 0) ms.tab_n[g] = ok ? total : -1;
+    if (t == 0) ms.tab_n[g] = ok ? total : -1;
   }
-  if (nz == (double)n) {  // all(y == 0) -> 0 (R/overdispersion.R:28-30)
-    if (lane == 0) {
+  if (nz == n) {  // all(y == 0) -> 0 (R/overdispersion.R:28-30)
+    if (t == 0) {
       ms.mode[g] = kMleDone;
       ms.theta[g] = 0.0;
       ms.iters[g] = 0;
@@ -148,16 +207,44 @@ __global__ __launch_bounds__(64) void k_mle_setup(Counts c, int64_t G, int64_t n
     }
     return;
   }
-  const double ybar = s1 / (double)n;
-  double v = 0.0;
-  for (int64_t i = lane; i < n; i += kWave) {
-    const double d = load_count(row, c.dtype, i) - ybar;
-    v += d * d;
+  double s1, v, spy;
+  if (integer) {
+    // var with n - 1 from the exact sums: (n s2 - s1^2) / (n (n - 1))
+    s1 = (double)s1i;
+    const __int128 num = (__int128)n * (__int128)s2 - (__int128)s1i * (__int128)s1i;
+    v = ((double)num / (double)n) / (double)(n - 1);
+    spy = (double)(s2 - (unsigned __int128)s1i);  // sum (y - 1) y
+  } else {
+    s1 = s1d;
+    const double ybar = s1 / (double)n;
+    double vv = 0.0, sp = 0.0;
+    for (int64_t i = t; i < n; i += kSetupThreads) {
+      const double y = load_count(row, c.dtype, i);
+      vv += (y - ybar) * (y - ybar);
+      sp += (y - 1) * y;
+    }
+    vv = wave_sum(vv);
+    sp = wave_sum(sp);
+    __syncthreads();
+    if (lane == 0) {
+      wd1[wave] = vv;
+      wmx[wave] = sp;
+    }
+    __syncthreads();
+    vv = 0.0;
+    sp = 0.0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      vv += wd1[w];
+      sp += wmx[w];
+    }
+    v = vv / (double)(n - 1);
+    spy = sp;
   }
-  v = wave_sum(v) / (double)(n - 1);
+  const double ybar = s1 / (double)n;
   double start = (v - ybar) / (ybar * ybar);
   if (isnan(start) || start <= 0) start = 0.5;
-  if (lane == 0) {
+  if (t == 0) {
     ms.mode[g] = kMleFarLeft;
     ms.x[g] = log(1e-8);
     ms.start[g] = start;
@@ -536,7 +623,7 @@ __global__ __launch_bounds__(64) void k_mle_update(TileDesign td, const int32_t*
 hipError_t launch_mle_setup(hipStream_t s, Counts c, int64_t G, int64_t n, int p,
                             const MleState& ms) {
   if (G <= 0) return hipSuccess;
-  DV_MLE_P(p, k_mle_setup, dim3((unsigned)G), dim3(64), s, c, G, n, ms);
+  DV_MLE_P(p, k_mle_setup, dim3((unsigned)G), dim3(kSetupThreads), s, c, G, n, ms);
   return hipGetLastError();
 }
 

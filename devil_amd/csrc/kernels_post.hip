@@ -92,7 +92,9 @@ __device__ __forceinline__ void mom_tile_chunk(const TileDesign& td, ChunkTable 
   }
 }
 
-template <int P, typename CT, bool VEC>
+// HC: also the HC0 meat sum, sum_i r_i^2 x_i x_i' with r = (y - mu) / (1 + theta mu)
+// (src/variance.cpp:64-85, 157-178), in ncp more partial columns.
+template <int P, typename CT, bool VEC, bool HC>
 __device__ __forceinline__ void bread_tile_chunk(const TileDesign& td, ChunkTable ct,
                                                  const CT* __restrict__ Y, int64_t ld,
                                                  int64_t G, const double* __restrict__ beta,
@@ -116,9 +118,11 @@ __device__ __forceinline__ void bread_tile_chunk(const TileDesign& td, ChunkTabl
   const double th = live ? theta[g] : 0.0;
   const CT* yrow = Y + g * ld;
   const CT* yh = yrow + 4 * h;
-  d4 aw[NTMAX], ar;
+  d4 aw[NTMAX], ah[HC ? NTMAX : 1], ar;
 #pragma unroll
   for (int t = 0; t < NTMAX; ++t) aw[t] = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int t = 0; t < (HC ? NTMAX : 1); ++t) ah[t] = d4{0.0, 0.0, 0.0, 0.0};
   ar = d4{0.0, 0.0, 0.0, 0.0};
   const int64_t c0 = ct.c0[chunk], c1 = ct.c1[chunk];
   const int64_t b0 = c0 / 16, b1 = c0 < c1 ? (c1 + 15) / 16 : b0;
@@ -146,6 +150,15 @@ __device__ __forceinline__ void bread_tile_chunk(const TileDesign& td, ChunkTabl
         for (int r = 0; r < 4; ++r) aw[t] = mfma16x16x4(w[r], cur.kb[t][r], aw[t]);
       }
     }
+    if constexpr (HC) {
+#pragma unroll
+      for (int t = 0; t < NTMAX; ++t) {
+        if (t < NT) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) ah[t] = mfma16x16x4(rr[r] * rr[r], cur.kb[t][r], ah[t]);
+        }
+      }
+    }
 #pragma unroll
     for (int r = 0; r < 4; ++r) ar = mfma16x16x4(rr[r], cur.kb[0][r], ar);
   };
@@ -157,7 +170,7 @@ __device__ __forceinline__ void bread_tile_chunk(const TileDesign& td, ChunkTabl
   };
   pipeline<Blk>(b0, min(b1, td.n / 16), b1, load, load_tail, body);
   // register r holds gene slot h + 4 r, column 16 t + s
-  const int W = td.ncp + 16;
+  const int W = td.ncp + 16 + (HC ? td.ncp : 0);
   double* out = part + ((int64_t)tile * ct.nchunk + chunk) * 16 * W;
 #pragma unroll
   for (int t = 0; t < NTMAX; ++t) {
@@ -168,6 +181,15 @@ __device__ __forceinline__ void bread_tile_chunk(const TileDesign& td, ChunkTabl
   }
 #pragma unroll
   for (int r = 0; r < 4; ++r) out[(h + 4 * r) * W + td.ncp + s] = ar[r];
+  if constexpr (HC) {
+#pragma unroll
+    for (int t = 0; t < NTMAX; ++t) {
+      if (t < NT) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) out[(h + 4 * r) * W + td.ncp + 16 + 16 * t + s] = ah[t][r];
+      }
+    }
+  }
 }
 
 template <int P>
@@ -213,7 +235,7 @@ __global__ __launch_bounds__(256) void k_mom_finish(ChunkTable ct, int64_t G, in
   if (status && !(den > 0.0)) status[g] |= DEVIL_GENE_MOM_DEN0;
 }
 
-template <int P>
+template <int P, bool HC>
 __global__ __launch_bounds__(256) void k_bread_accum(TileDesign td, ChunkTable ct, Counts c,
                                                      int64_t G, const double* __restrict__ beta,
                                                      const double* __restrict__ theta,
@@ -228,12 +250,13 @@ __global__ __launch_bounds__(256) void k_bread_accum(TileDesign td, ChunkTable c
   if ((int64_t)tile * 16 >= G) return;
   with_counts(c, [&](auto Y, auto vec) {
     typedef std::remove_cv_t<std::remove_pointer_t<decltype(Y)>> CT;
-    bread_tile_chunk<P, CT, decltype(vec)::value>(td, ct, Y, c.ld, G, beta, theta, tile, chunk,
-                                                  part, e2);
+    bread_tile_chunk<P, CT, decltype(vec)::value, HC>(td, ct, Y, c.ld, G, beta, theta, tile,
+                                                      chunk, part, e2);
   });
 }
 
-// One wavefront per gene.  C = number of clusters (0: no sandwich).
+// One wavefront per gene.  C = number of clusters (0: no sandwich; -1: the
+// HC0 meat, sum_i r_i^2 x_i x_i' / n, from the partials' last ncp columns).
 template <int P>
 __global__ __launch_bounds__(64) void k_bread_finish(TileDesign td, ChunkTable ct, int64_t G,
                                                      int64_t n, int C,
@@ -243,17 +266,23 @@ __global__ __launch_bounds__(64) void k_bread_finish(TileDesign td, ChunkTable c
                                                      int32_t* __restrict__ status) {
   constexpr int NPAIR = P * (P + 1) / 2;
   __shared__ double accw[DEVIL_NCP_MAX];
+  __shared__ double acch[DEVIL_NCP_MAX];
   __shared__ double sc[16];
   __shared__ double meat[NPAIR];
   const int64_t g = blockIdx.x;
   if (g >= G) return;
   const int lane = threadIdx.x;
-  const int W = td.ncp + 16;
+  const bool hc0 = C < 0;
+  const int W = td.ncp + 16 + (hc0 ? td.ncp : 0);
   const double* src = part + (g / 16) * ct.nchunk * 16 * W + (g % 16) * W;
   for (int col = lane; col < td.ncp; col += 64) {
-    double a = 0.0;
-    for (int ch = 0; ch < ct.nchunk; ++ch) a += src[(int64_t)ch * 16 * W + col];
+    double a = 0.0, m = 0.0;
+    for (int ch = 0; ch < ct.nchunk; ++ch) {
+      a += src[(int64_t)ch * 16 * W + col];
+      if (hc0) m += src[(int64_t)ch * 16 * W + td.ncp + 16 + col];
+    }
     accw[col] = a;
+    if (hc0) acch[col] = m;
   }
   // lane t < NPAIR owns meat entry t = tri(a, b)
   int ma = 0, mb = 0;
@@ -306,15 +335,21 @@ __global__ __launch_bounds__(64) void k_bread_finish(TileDesign td, ChunkTable c
 #pragma unroll
     for (int bb = 0; bb < P; ++bb) out[a + bb * P] = Bi[a][bb];
   if (status && sing) status[g] |= DEVIL_GENE_SINGULAR;
-  if (C > 0 && sandwich) {
+  if (C != 0 && sandwich) {
     const double adj = (C > 1) ? (double)C / (C - 1.0) : 1.0;
-    const double scl = adj / (double)n;
+    const double scl = (hc0 ? 1.0 : adj) / (double)n;
     double Me[P][P], T1[P][P];
 #pragma unroll
     for (int a = 0; a < P; ++a)
 #pragma unroll
       for (int bb = a; bb < P; ++bb) {
-        const double v = scl * meat[tri<P>(a, bb)];
+        double v;
+        if (hc0) {
+          const int pi = td.kidx_pair[tri<P>(a, bb)];
+          v = scl * (pi >= 0 ? acch[pi] : 0.0);
+        } else {
+          v = scl * meat[tri<P>(a, bb)];
+        }
         Me[a][bb] = v;
         Me[bb][a] = v;
       }
@@ -561,6 +596,27 @@ __global__ __launch_bounds__(256) void k_init_finish(InitTile it, ChunkTable ct,
     default: return hipErrorInvalidValue;                                           \
   }
 
+// the same dispatch for kernels with a second (bool) template argument
+#define DV_POST_P2_CASE(P_, KERNEL, B, GRID, BLOCK, STREAM, ...)    \
+  case P_: {                                                        \
+    auto k_ = KERNEL<P_, B>;                                        \
+    hipLaunchKernelGGL(k_, GRID, BLOCK, 0, STREAM, __VA_ARGS__);    \
+  } break;
+#define DV_POST_P2(p, KERNEL, B, GRID, BLOCK, STREAM, ...)                     \
+  switch (p) {                                                                \
+    DV_POST_P2_CASE(1, KERNEL, B, GRID, BLOCK, STREAM, __VA_ARGS__)           \
+    DV_POST_P2_CASE(2, KERNEL, B, GRID, BLOCK, STREAM, __VA_ARGS__)           \
+    DV_POST_P2_CASE(3, KERNEL, B, GRID, BLOCK, STREAM, __VA_ARGS__)           \
+    DV_POST_P2_CASE(4, KERNEL, B, GRID, BLOCK, STREAM, __VA_ARGS__)           \
+    DV_POST_P2_CASE(5, KERNEL, B, GRID, BLOCK, STREAM, __VA_ARGS__)           \
+    DV_POST_P2_CASE(6, KERNEL, B, GRID, BLOCK, STREAM, __VA_ARGS__)           \
+    DV_POST_P2_CASE(7, KERNEL, B, GRID, BLOCK, STREAM, __VA_ARGS__)           \
+    DV_POST_P2_CASE(8, KERNEL, B, GRID, BLOCK, STREAM, __VA_ARGS__)           \
+    DV_POST_P2_CASE(9, KERNEL, B, GRID, BLOCK, STREAM, __VA_ARGS__)           \
+    DV_POST_P2_CASE(10, KERNEL, B, GRID, BLOCK, STREAM, __VA_ARGS__)          \
+    default: return hipErrorInvalidValue;                                     \
+  }
+
 hipError_t launch_mom_tiled(hipStream_t s, const TileDesign& td, const ChunkTable& ct, Counts c,
                             int64_t G, int64_t n, int p, const double* beta, double* part,
                             double* theta, int32_t* status) {
@@ -579,8 +635,13 @@ hipError_t launch_bread_tiled(hipStream_t s, const TileDesign& td, const ChunkTa
                               double* sandwich, int32_t* status) {
   if (G <= 0) return hipSuccess;
   const int64_t groups = ((G + 15) / 16 + 3) / 4;
-  DV_POST_P(p, k_bread_accum, dim3((unsigned)(groups * ct.nchunk)), dim3(256), s, td, ct, c, G,
-            beta, theta, part);
+  if (C < 0) {
+    DV_POST_P2(p, k_bread_accum, true, dim3((unsigned)(groups * ct.nchunk)), dim3(256), s, td,
+               ct, c, G, beta, theta, part);
+  } else {
+    DV_POST_P2(p, k_bread_accum, false, dim3((unsigned)(groups * ct.nchunk)), dim3(256), s, td,
+               ct, c, G, beta, theta, part);
+  }
   DV_POST_P(p, k_bread_finish, dim3((unsigned)G), dim3(64), s, td, ct, G, n, C, part, bread,
             sandwich, status);
   return hipGetLastError();

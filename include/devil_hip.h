@@ -20,6 +20,11 @@
  *                             compute_clustered_meat_fast src/variance.cpp:186-228
  *                             sandwich assembly          R/main.R:481-492
  *   devil_test_de             test_de                    R/test.R:85-134
+ *   devil_sandwich_hc0        compute_sandwich (HC0)     R/variance.R:16-26,
+ *                             compute_meat               src/variance.cpp:157-178
+ *   devil_size_factors        calculate_sf               R/size_factors.R:37-127
+ *   devil_group_cells         group_data (counts)        R/group_data.R:38-69
+ *   devil_csc_to_dense        sparse assays of fit_devil R/main.R:1-9
  *
  * Conventions
  *   - Every function returns DEVIL_OK (0) or a DEVIL_E_* code; nothing calls
@@ -128,6 +133,35 @@ int devil_fit(devil_ctx *ctx, const devil_problem *probs, devil_outputs *outs,
 /* Stage entry points (device pointers; counts as in devil_problem).         */
 /* ------------------------------------------------------------------------ */
 
+/* Preparation of the count matrix (device pointers). */
+
+/* calculate_sf (R/size_factors.R:37-127): sf[i] for each cell i.
+ * NORMED_SUM: column sums over the genes divided by their geometric mean
+ * (DEVIL_E_ARG with the reference's message when a column sums to 0);
+ * PSINORM: mean_g log1p(y) - min_g log1p(y), divided by its geometric mean;
+ * EDGER: DEVIL_E_UNSUPPORTED (the reference calls edgeR's calcNormFactors).
+ * G <= 1 gives unit factors, as the reference. */
+enum { DEVIL_SF_NORMED_SUM = 0, DEVIL_SF_PSINORM = 1, DEVIL_SF_EDGER = 2 };
+int devil_size_factors(devil_ctx *ctx, const void *counts, int32_t counts_dtype, int64_t G,
+                       int64_t n, int64_t counts_ld, int32_t method, double *sf);
+
+/* group_data (R/group_data.R:38-69), the count part: out[g, i] = counts[g,
+ * order[i]] for the 0-based cell order (R's order() of the cluster codes);
+ * out has the counts' dtype and out_ld elements per gene.  An order entry
+ * outside [0, n) returns DEVIL_E_ARG before anything is read. */
+int devil_group_cells(devil_ctx *ctx, const void *counts, int32_t counts_dtype, int64_t G,
+                      int64_t n, int64_t counts_ld, const int64_t *order, void *out,
+                      int64_t out_ld);
+
+/* A dgCMatrix / CSC G x n count matrix (R/main.R:1-9 accepts sparse assays:
+ * column = cell; colptr n + 1 entries from 0, rowind 0-based gene indices,
+ * values as R stores them) into the dense gene-major matrix the fit streams
+ * (out_dtype DEVIL_COUNTS_*; values converted by a C cast).  The structure is
+ * checked on the device first (DEVIL_E_ARG, nothing written). */
+int devil_csc_to_dense(devil_ctx *ctx, const int64_t *colptr, const int32_t *rowind,
+                       const double *values, int64_t G, int64_t n, int32_t out_dtype,
+                       void *out, int64_t out_ld);
+
 /* R/overdispersion.R:129-135: alpha0[g] = (var_g - ybar_g/mean(exp(off))) / ybar_g^2,
  * var with n-1; NaN or negative -> 0.01. */
 int devil_estimate_dispersion(devil_ctx *ctx, const void *counts, int32_t counts_dtype,
@@ -172,6 +206,17 @@ int devil_bread_meat(devil_ctx *ctx, const void *counts, int32_t counts_dtype, i
                      const double *beta, const double *theta, const double *offset,
                      const int32_t *cluster_ends, int32_t n_clusters, double *bread,
                      double *sandwich, int32_t *status);
+
+/* The unclustered sandwich of compute_sandwich (R/variance.R:16-26) with the
+ * HC0 meat of compute_meat / compute_scores (src/variance.cpp:64-85, 157-178):
+ * meat = (1/n) sum_i s_i s_i', s_i = x_i (y_i - mu_i) / (1 + theta mu_i);
+ * sandwich = bread meat bread n.  The bread is compute_hessian with theta as
+ * devil_bread_meat (and cpu_fit) use it; the reference's compute_sandwich
+ * passes 1/overdispersion there, which its callers do not rely on. */
+int devil_sandwich_hc0(devil_ctx *ctx, const void *counts, int32_t counts_dtype, int64_t G,
+                       int64_t n, int64_t counts_ld, const double *X, int32_t p,
+                       const double *beta, const double *theta, const double *offset,
+                       double *bread, double *sandwich, int32_t *status);
 
 /* R/test.R:85-134: p = 2*pt(|c'b| / sqrt(c'Vc), n-2, upper); clustered:
  * max(p_sandwich, p_bread); BH over non-NaN p; lfc = c'b/log(2) clipped to

@@ -95,6 +95,7 @@ def lib():
             "orc_test_de": (None, [_dp, _dp, _dp, i64, C.c_int, i64, _dp, d, _dp, _dp,
                                    _dp, _dp]),
             "orc_p_adjust_bh": (None, [_dp, i64, _dp]),
+            "orc_size_factors": (C.c_int, [_dp, i64, i64, i64, C.c_int, _dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -382,3 +383,27 @@ def de_test(beta, bread, sandwich, n, contrast, max_lfc=10.0):
     lib().orc_test_de(beta.ctypes.data_as(_dp), _d(br), _d(sw), G, p, int(n), _d(c),
                       float(max_lfc), _d(pval), _d(lpv), _d(adj), _d(lfc))
     return dict(pval=pval, log_pval=lpv, adj_pval=adj, lfc=lfc)
+
+
+def size_factors(Y, method="normed_sum"):
+    """R/size_factors.R:37-127 calculate_sf (normed_sum or psinorm)."""
+    Y = _f64(Y)
+    G, n = Y.shape
+    sf = np.zeros(n)
+    rc = lib().orc_size_factors(_d(Y), G, n, n, {"normed_sum": 0, "psinorm": 1}[method],
+                                _d(sf))
+    if rc:
+        raise ValueError("At least one column (sample) contains all zeros")
+    return sf
+
+
+def group_data(Y, X, clusters):
+    """R/group_data.R:38-69: cells ordered by cluster code (codes by first
+    appearance, stable), returned as (Y[:, ord], X[ord], codes[ord], ord)."""
+    lab = np.asarray(clusters)
+    _, first, inv = np.unique(lab, return_index=True, return_inverse=True)
+    rank = np.empty(len(first), np.int64)
+    rank[np.argsort(first)] = np.arange(len(first))
+    codes = rank[inv] + 1
+    ord_ = np.argsort(codes, kind="stable")
+    return np.asarray(Y)[:, ord_], np.asarray(X)[ord_], codes[ord_].astype(float), ord_

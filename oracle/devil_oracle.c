@@ -14,6 +14,10 @@
  *   orc_make_table          src/overdispersion.cpp:32-46
  *   orc_nb_loglik/score/dscore  src/overdispersion.cpp:53-88,95-178,186-242
  *   orc_fit_dispersion      R/overdispersion.R:25-115 (special cases, start)
+ *   orc_fit_dispersion_port R/overdispersion.R:25-115 with stats::nlminb's
+ *                           PORT driver (DMNHB) restated for one parameter
+ *   orc_nb_loglik_acc       src/overdispersion.cpp:53-88, cancellation-free
+ *   orc_size_factors        R/size_factors.R:37-127 (normed_sum, psinorm)
  *   orc_estimate_dispersion R/overdispersion.R:129-135
  *   orc_init_beta_qr        R/beta.R:13-20
  *   orc_init_beta_rough     R/main.R:404-406
@@ -46,11 +50,23 @@
  *                           19-39), column scaling (test-beta.R:79-100)
  *   test_de                 scipy t.sf; BH against a direct restatement of
  *                           p.adjust; lfc clipping (test-test.R:32-62)
- *   orc_fit_dispersion      PARITY UNPINNED against the reference: R's
- *                           nlminb(rel.tol = tolerance) trajectory cannot be
- *                           run here.  What is pinned: the 0 special cases
+ *   orc_fit_dispersion      (the root of the score) the 0 special cases
  *                           (test-overdispersion.R:47-65), score(root) = 0
  *                           and the root maximising orc_nb_loglik.
+ *   orc_fit_dispersion_port tests/test_oracle_mle.py: converges to the root
+ *                           as rel.tol -> 0; on Newton-only paths par equals
+ *                           the documented stopping rule's iterate; the
+ *                           reference's MLE scenarios (test-overdispersion.R:
+ *                           47-152).  PARITY UNPINNED beyond that: R's nlminb
+ *                           cannot run here, so its radius updates, step
+ *                           retries and singular / false convergence paths
+ *                           are restated from the PORT documentation only
+ *                           (4 of 395 fitted genes of a C2-shaped sample take a
+ *                           path other than accepted Newton steps).
+ *   orc_lgamma_diff /       mpmath at 50 digits; equal to orc_nb_loglik where
+ *   orc_nb_loglik_acc       that is well conditioned; score = its derivative
+ *   orc_size_factors        the definitions (numpy), scale and gene-order
+ *                           invariance
  */
 #include "devil_oracle.h"
 
@@ -1640,4 +1656,44 @@ void orc_test_de(const double *beta, const double *bread, const double *sandwich
     lfc[g] = lf;
   }
   orc_p_adjust_bh(pval, G, adj_pval);
+}
+
+/* ===================================================================== */
+/* size factors and cell grouping (R/size_factors.R, R/group_data.R)      */
+/* ===================================================================== */
+
+/* calculate_sf (R/size_factors.R:37-127) on a G x n gene-major matrix:
+ * method 0 normed_sum: s_i = colSums(Y)_i / exp(mean(log colSums(Y)));
+ *   returns 1 (and leaves sf) when a column sums to 0 (the reference stops);
+ * method 1 psinorm: s_i = mean_g log1p(y_gi) - min_g log1p(y_gi), then
+ *   divided by exp(mean(log s)) (log 0 = -inf propagates, as in R).
+ * One row gives unit factors (R/size_factors.R:38-42). */
+int orc_size_factors(const double *Y, int64_t G, int64_t n, int64_t ld, int method,
+                     double *sf) {
+  if (G <= 1) {
+    for (int64_t i = 0; i < n; ++i) sf[i] = 1.0;
+    return 0;
+  }
+  for (int64_t i = 0; i < n; ++i) {
+    if (method == 0) {
+      double s = 0.0;
+      for (int64_t g = 0; g < G; ++g) s += Y[g * ld + i];
+      if (s == 0.0) return 1;
+      sf[i] = s;
+    } else {
+      double mn = INFINITY, s = 0.0;
+      for (int64_t g = 0; g < G; ++g) {
+        const double l = log1p(Y[g * ld + i]);
+        s += l;
+        if (l < mn) mn = l;
+      }
+      sf[i] = s / (double)G - mn;
+    }
+  }
+  double ml = 0.0;
+  for (int64_t i = 0; i < n; ++i) ml += log(sf[i]);
+  ml /= (double)n;
+  const double gm = exp(ml);
+  for (int64_t i = 0; i < n; ++i) sf[i] /= gm;
+  return 0;
 }

@@ -184,6 +184,12 @@ void orc_test_de(const double *beta, const double *bread, const double *sandwich
 /* stats::p.adjust(p, "BH") with NA (NaN) entries left NaN */
 void orc_p_adjust_bh(const double *p, int64_t G, double *adj);
 
+/* ---------------- R/size_factors.R:37-127 calculate_sf ---------------- */
+/* method 0 normed_sum, 1 psinorm; returns 1 if a column sums to 0
+ * (normed_sum), else 0. */
+int orc_size_factors(const double *Y, int64_t G, int64_t n, int64_t ld, int method,
+                     double *sf);
+
 #ifdef __cplusplus
 }
 #endif

@@ -130,3 +130,43 @@ def test_test_de_clustered_takes_max(orc):
     assert np.all(r1["pval"] >= r0["pval"])
     r2 = orc.de_test(beta, 5 * b1, b1, n, [0, 1])  # max() is symmetric
     np.testing.assert_array_equal(r1["pval"], r2["pval"])
+
+
+def test_size_factors_definitions():
+    """calculate_sf (R/size_factors.R:37-127) by its definitions: column sums
+    over their geometric mean; mean minus minimum of log1p over genes, over
+    its geometric mean; scale-free and gene-permutation-free."""
+    import oracle
+    rng = np.random.default_rng(8)
+    Y = rng.poisson(3.0, (40, 25)).astype(float) + (rng.random((40, 25)) < 0.05)
+    Y[:, Y.sum(axis=0) == 0] = 1.0
+    cs = Y.sum(axis=0)
+    np.testing.assert_allclose(oracle.size_factors(Y), cs / np.exp(np.mean(np.log(cs))),
+                               rtol=1e-14)
+    lg = np.log1p(Y)
+    ps = lg.mean(axis=0) - lg.min(axis=0)
+    np.testing.assert_allclose(oracle.size_factors(Y, "psinorm"),
+                               ps / np.exp(np.mean(np.log(ps))), rtol=1e-13)
+    np.testing.assert_allclose(oracle.size_factors(7.0 * Y), oracle.size_factors(Y), rtol=1e-13)
+    perm = rng.permutation(40)
+    np.testing.assert_allclose(oracle.size_factors(Y[perm], "psinorm"),
+                               oracle.size_factors(Y, "psinorm"), rtol=1e-13)
+    assert np.all(oracle.size_factors(Y[:1]) == 1.0)
+    Y0 = Y.copy()
+    Y0[:, 3] = 0
+    with pytest.raises(ValueError):
+        oracle.size_factors(Y0)
+
+
+def test_group_data_definition():
+    """group_data (R/group_data.R:38-69): codes by first appearance, a stable
+    order, and every array permuted alike."""
+    import oracle
+    Y = np.arange(2 * 7).reshape(2, 7)
+    X = np.arange(7)[:, None] * 1.0
+    cl = ["b", "a", "b", "c", "a", "b", "c"]
+    Yr, Xr, cr, order = oracle.group_data(Y, X, cl)
+    np.testing.assert_array_equal(order, [0, 2, 5, 1, 4, 3, 6])
+    np.testing.assert_array_equal(cr, [1, 1, 1, 2, 2, 3, 3])
+    np.testing.assert_array_equal(Yr, Y[:, order])
+    np.testing.assert_array_equal(Xr[:, 0], order)
