@@ -82,6 +82,8 @@ def lib():
                                        _vp, i32, _vp, _vp, _vp]),
         "devil_test_de": (C.c_int, [_vp, _vp, _vp, _vp, i64, i32, i64, _vp, d, _vp, _vp, _vp,
                                     _vp]),
+        "devil_test_de_noalloc": (C.c_int, [_vp, _vp, _vp, _vp, i64, i32, i64, _vp, d, _vp,
+                                            _vp, _vp, _vp]),
         "devil_sandwich_hc0": (C.c_int, [_vp, _vp, i32, i64, i64, i64, _vp, i32, _vp, _vp,
                                          _vp, _vp, _vp, _vp]),
         "devil_size_factors": (C.c_int, [_vp, _vp, i32, i64, i64, i64, i32, _vp]),
@@ -105,7 +107,7 @@ def lib():
 EXPORTED = ["devil_create", "devil_destroy", "devil_strerror", "devil_last_error",
             "devil_version", "devil_fit", "devil_estimate_dispersion", "devil_init_beta",
             "devil_beta_fit", "devil_mom_dispersion", "devil_fit_dispersion_mle",
-            "devil_bread_meat", "devil_test_de",
+            "devil_bread_meat", "devil_test_de", "devil_test_de_noalloc",
             "devil_sandwich_hc0", "devil_size_factors", "devil_group_cells",
             "devil_csc_to_dense",
             "devil_set_timing", "devil_stage_times",

@@ -12,6 +12,7 @@
 #include <map>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -57,6 +58,7 @@ struct BlockCache {
 struct devil_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  void* scratch = nullptr;  // the device's no-allocation scratch array (its address)
   BlockCache cache;
   std::string err;
   int timing = 0;
@@ -1292,6 +1294,34 @@ int set_device(devil_ctx* ctx) {
 
 }  // namespace
 
+// Scratch of devil_test_de_noalloc: a fixed device array, one copy per GPU,
+// held until the process ends.  In host code the name is only a stand-in for
+// the device symbol: its address on each device is looked up once, with that
+// device current (devil_create), and only that address is handed to kernels.
+static constexpr size_t kNoAllocScratchBytes = (size_t)16 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// the address of this device's scratch array (the device must be current)
+int scratch_address(int device, void** ptr) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (device < 0 || device >= kMaxDevices) return DEVIL_E_NODEVICE;
+  if (!g_scratch_ptr[device]) {
+    void* q = nullptr;
+    if (hipGetSymbolAddress(&q, HIP_SYMBOL(scratch)) != hipSuccess || !q) return DEVIL_E_HIP;
+    g_scratch_ptr[device] = q;
+  }
+  *ptr = g_scratch_ptr[device];
+  return DEVIL_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* devil_version(void) { return "devil_hip 0.1.0 gfx950"; }
@@ -1322,6 +1352,11 @@ int devil_create(devil_ctx** out, int hip_device) {
   devil_ctx* ctx = new (std::nothrow) devil_ctx();
   if (!ctx) return DEVIL_E_NOMEM;
   ctx->device = hip_device;
+  int rc = scratch_address(hip_device, &ctx->scratch);
+  if (rc != DEVIL_OK) {
+    delete ctx;
+    return rc;
+  }
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
     delete ctx;
     return DEVIL_E_HIP;
@@ -2072,6 +2107,36 @@ int devil_test_de(devil_ctx* ctx, const double* beta, const double* bread,
   tm.end(ev);
   DV_HIP(hipStreamSynchronize(s));
   tm.collect();
+  return DEVIL_OK;
+}
+
+// devil_test_de with its scratch (the gene-major beta and the BH sort
+// buffers) in the device's fixed scratch array: no allocation, no events.
+int devil_test_de_noalloc(devil_ctx* ctx, const double* beta, const double* bread,
+                          const double* sandwich, int64_t G, int32_t p, int64_t n,
+                          const double* contrast, double max_lfc, double* pval,
+                          double* log_pval, double* adj_pval, double* lfc) {
+  int rc;
+  if ((rc = set_device(ctx))) return rc;
+  if ((rc = check_p(ctx, p))) return rc;
+  if (G < 0 || (G > 0 && (!beta || !bread || !contrast || !pval || !adj_pval || !lfc)))
+    return fail(ctx, DEVIL_E_ARG, "NULL pointer or G < 0");
+  if (G == 0) return DEVIL_OK;
+  if (!ctx->scratch) return fail(ctx, DEVIL_E_HIP, "no scratch array on this device");
+  const size_t bg_bytes = ((sizeof(double) * (size_t)G * (size_t)p + 255) / 256) * 256;
+  const size_t bh_bytes = dv::bh_scratch_bytes(G);
+  if (bg_bytes + bh_bytes > kNoAllocScratchBytes)
+    return fail(ctx, DEVIL_E_ARG,
+                "G = " + std::to_string(G) + " needs " + std::to_string(bg_bytes + bh_bytes) +
+                    " scratch bytes; devil_test_de_noalloc holds " +
+                    std::to_string(kNoAllocScratchBytes));
+  hipStream_t s = ctx->stream;
+  unsigned char* base = static_cast<unsigned char*>(ctx->scratch);
+  double* bg = reinterpret_cast<double*>(base);
+  DV_HIP(dv::launch_transpose_gp(s, beta, G, p, 1, bg));
+  DV_HIP(dv::launch_test_de(s, bg, bread, sandwich, G, p, (double)n - 2.0, contrast, max_lfc,
+                            pval, log_pval, lfc, base + bg_bytes, bh_bytes, adj_pval));
+  DV_HIP(hipStreamSynchronize(s));
   return DEVIL_OK;
 }
 

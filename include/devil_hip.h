@@ -227,6 +227,21 @@ int devil_test_de(devil_ctx *ctx, const double *beta, const double *bread,
                   const double *contrast, double max_lfc, double *pval,
                   double *log_pval, double *adj_pval, double *lfc);
 
+/* devil_test_de without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Arguments and results are devil_test_de's.  Its scratch (the gene-major
+ * beta, 8 G p bytes, and the Benjamini-Hochberg sort buffers, about 40 G
+ * bytes plus the sort's temporary storage) lives in a fixed 16 MiB device
+ * array per GPU, held until the process ends: G up to about 100000 genes at
+ * p = 10; larger G returns DEVIL_E_ARG.  Every call on a GPU shares that
+ * GPU's array, so calls that overlap on one GPU (several contexts or host
+ * threads) race. */
+int devil_test_de_noalloc(devil_ctx *ctx, const double *beta, const double *bread,
+                          const double *sandwich, int64_t G, int32_t p, int64_t n,
+                          const double *contrast, double max_lfc, double *pval,
+                          double *log_pval, double *adj_pval, double *lfc);
+
 /* ------------------------------------------------------------------------ */
 /* Instrumentation (no counterpart in the reference).                        */
 /* ------------------------------------------------------------------------ */
