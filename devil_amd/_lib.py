@@ -115,7 +115,7 @@ EXPORTED = ["devil_create", "devil_destroy", "devil_strerror", "devil_last_error
 
 STAGES = ["init", "irls", "dispersion", "bread_meat", "test_de"]
 KERNELS = ["irls_fp64", "irls_mixed", "mle"]
-TUNE_MIXED_KERNEL, TUNE_CHUNK_CELLS, TUNE_CONCURRENCY, TUNE_MLE_DRIVER = 0, 1, 2, 3
+TUNE_MIXED_KERNEL, TUNE_CHUNK_CELLS, TUNE_CONCURRENCY, TUNE_MLE_DRIVER, TUNE_PAD_COPY = 0, 1, 2, 3, 4
 MLE_DRIVERS = {"nlminb": 0, "root": 1}
 SF_METHODS = {"normed_sum": 0, "psinorm": 1, "edgeR": 2}
 MIXED_KERNELS = {"f32": 0, "bf16": 1, "auto": 2, "bf16_ws": 3, "bf16_ws2": 4}
@@ -164,7 +164,8 @@ class Context:
         """Genes refit in FP64 by the last mixed fit, and its kernel and chunk
         size (the autotuner's choice when tuning was left open)."""
         out = {}
-        for k, name in enumerate(("replayed", "mixed_kernel", "chunk_cells", "polished")):
+        for k, name in enumerate(("replayed", "mixed_kernel", "chunk_cells", "polished",
+                                  "aligned_copy")):
             v = C.c_int64()
             self.check(lib().devil_fit_info(self.h, k, C.byref(v)))
             out[name] = int(v.value)

@@ -62,14 +62,19 @@ struct devil_ctx {
   BlockCache cache;
   std::string err;
   int timing = 0;
-  // kernel, chunk cells (0 auto), streams, MLE driver
-  int tune[DEVIL_N_TUNE] = {DEVIL_MIXED_AUTO, 0, 8, DEVIL_MLE_NLMINB};
+  // kernel, chunk cells (0 auto), streams, MLE driver, aligned copies allowed
+  int tune[DEVIL_N_TUNE] = {DEVIL_MIXED_AUTO, 0, 8, DEVIL_MLE_NLMINB, 1};
   std::map<uint64_t, int> tuned;      // autotuner choices by problem shape
   std::vector<devil_ctx*> subs;       // per-stream contexts of multi-problem fits
   int64_t replayed = 0;                // genes refit in FP64 by the mixed path (last fit)
   int64_t polished = 0;                // genes whose last mixed step was redone in FP64
   int64_t replay_sum = 0;
   int64_t polish_sum = 0;
+  int aligned_copy = 0;                // the last fit read an aligned copy of its counts
+  // the aligned copy's buffer, kept for the next fit (a fresh hipMalloc of
+  // tens of GB costs far more than the copy itself); freed by devil_destroy
+  void* pad_p = nullptr;
+  size_t pad_size = 0;
   int last_mk = 0;                     // mixed kernel and chunk cells of the last fit
   int64_t last_cc = 0;
   int64_t k_sum_n[DEVIL_N_KERNELS] = {0, 0};
@@ -1381,6 +1386,7 @@ void devil_destroy(devil_ctx* ctx) {
   for (hipEvent_t e : ctx->round_ev)
     if (e) (void)hipEventDestroy(e);
   if (ctx->round_counts) (void)hipHostFree(ctx->round_counts);
+  if (ctx->pad_p) (void)hipFree(ctx->pad_p);
   ctx->cache.clear();
   delete ctx;
 }
@@ -1483,18 +1489,48 @@ int fit_one(devil_ctx* ctx, const devil_problem* pr, devil_outputs* out) {
     DV_HIP(hipStreamSynchronize(s));
   }
 
-  // counts: device-resident as given, or streamed in gene batches
+  // counts: device-resident as given, or streamed in gene batches.  4-byte
+  // counts are read with 16-byte loads (and staged in LDS by the mixed
+  // kernels) only when every row starts 16-byte aligned: host batches land in
+  // rows padded to a multiple of 4 elements, and a mixed fit of misaligned
+  // device rows first copies them so (when HBM holds the copy)
   const size_t es = count_elem_size(pr->counts_dtype);
+  const int64_t ld_pad = es == 4 ? ((n + 3) & ~int64_t(3)) : pr->counts_ld;
   int64_t batch = G;
   DevBuf cbuf;
+  const void* dev_counts = pr->counts;
+  int64_t dev_ld = pr->counts_ld;
+  DevBuf stage;  // host rows needing padding land here first
+  const bool pad_host = mem == DEVIL_MEM_HOST && pr->counts_ld != ld_pad;
   if (mem == DEVIL_MEM_HOST) {
     size_t freeb = 0, totalb = 0;
     DV_HIP(hipMemGetInfo(&freeb, &totalb));
-    const size_t row_bytes = (size_t)pr->counts_ld * es;
+    const size_t row_bytes = (size_t)ld_pad * es + (pad_host ? (size_t)pr->counts_ld * es : 0);
     const size_t budget = std::min<size_t>(freeb / 2, (size_t)8 << 30);
     batch = std::max<int64_t>(1, std::min<int64_t>(G, (int64_t)(budget / row_bytes)));
-    DV_ALLOC(cbuf, row_bytes * batch);
+    DV_ALLOC(cbuf, (size_t)ld_pad * es * batch);
+    if (pad_host) DV_ALLOC(stage, (size_t)pr->counts_ld * es * batch);
+  } else if (mixed && es == 4 && ctx->tune[DEVIL_TUNE_PAD_COPY] &&
+             !counts_vec4(dv::Counts{pr->counts, pr->counts_dtype, pr->counts_ld})) {
+    const size_t bytes = (size_t)G * ld_pad * es;
+    if (bytes > ctx->pad_size) {
+      size_t freeb = 0, totalb = 0;
+      DV_HIP(hipMemGetInfo(&freeb, &totalb));
+      if (bytes + ctx->pad_size <= freeb / 2) {
+        if (ctx->pad_p) DV_HIP(hipFree(ctx->pad_p));
+        ctx->pad_p = nullptr;
+        ctx->pad_size = 0;
+        if (hipMalloc(&ctx->pad_p, bytes) == hipSuccess) ctx->pad_size = bytes;
+        else ctx->pad_p = nullptr;
+      }
+    }
+    if (bytes <= ctx->pad_size) {
+      DV_HIP(dv::launch_pad_rows(s, pr->counts, pr->counts_ld, G, n, ctx->pad_p, ld_pad));
+      dev_counts = ctx->pad_p;
+      dev_ld = ld_pad;
+    }
   }
+  ctx->aligned_copy = dev_counts != pr->counts;
   ChunkHost chunks;
   if ((rc = build_chunks(ctx, n, C > 0 ? ends_h.data() : nullptr, C, chunks))) return rc;
   DevBuf post;
@@ -1517,14 +1553,20 @@ int fit_one(devil_ctx* ctx, const devil_problem* pr, devil_outputs* out) {
     const int64_t Gb = std::min(batch, G - g0);
     dv::Counts c;
     c.dtype = pr->counts_dtype;
-    c.ld = pr->counts_ld;
     if (mem == DEVIL_MEM_HOST) {
-      DV_HIP(hipMemcpyAsync(cbuf.p,
-                            static_cast<const char*>(pr->counts) + (size_t)g0 * pr->counts_ld * es,
-                            (size_t)Gb * pr->counts_ld * es, hipMemcpyHostToDevice, s));
+      const char* src = static_cast<const char*>(pr->counts) + (size_t)g0 * pr->counts_ld * es;
+      if (!pad_host) {
+        DV_HIP(hipMemcpyAsync(cbuf.p, src, (size_t)Gb * ld_pad * es, hipMemcpyHostToDevice, s));
+      } else {
+        DV_HIP(hipMemcpyAsync(stage.p, src, (size_t)Gb * pr->counts_ld * es,
+                              hipMemcpyHostToDevice, s));
+        DV_HIP(dv::launch_pad_rows(s, stage.p, pr->counts_ld, Gb, n, cbuf.p, ld_pad));
+      }
       c.ptr = cbuf.p;
+      c.ld = ld_pad;
     } else {
-      c.ptr = static_cast<const char*>(pr->counts) + (size_t)g0 * pr->counts_ld * es;
+      c.ptr = static_cast<const char*>(dev_counts) + (size_t)g0 * dev_ld * es;
+      c.ld = dev_ld;
     }
     double* a0 = alpha0.as<double>() + g0;
     double* b0 = beta.as<double>() + g0 * p;
@@ -1691,6 +1733,10 @@ int devil_fit(devil_ctx* ctx, const devil_problem* probs, devil_outputs* outs,
     ctx->polished += sub->polish_sum;
     sub->replay_sum = 0;
     sub->polish_sum = 0;
+    // the streams' aligned copies are not kept (one per stream would add up)
+    if (sub->pad_p) (void)hipFree(sub->pad_p);
+    sub->pad_p = nullptr;
+    sub->pad_size = 0;
   }
   // the first failing problem (in problem order) names the error
   for (int j = 0; j < n_problems; ++j)
@@ -1724,6 +1770,14 @@ int devil_set_tuning(devil_ctx* ctx, int32_t key, int64_t value) {
     return fail(ctx, DEVIL_E_ARG, "concurrency must be in [1, 16]");
   if (key == DEVIL_TUNE_MLE_DRIVER && value != DEVIL_MLE_NLMINB && value != DEVIL_MLE_ROOT)
     return fail(ctx, DEVIL_E_ARG, "unknown MLE driver");
+  if (key == DEVIL_TUNE_PAD_COPY && value != 0 && value != 1)
+    return fail(ctx, DEVIL_E_ARG, "pad copy must be 0 or 1");
+  if (key == DEVIL_TUNE_PAD_COPY && value == 0 && ctx->pad_p) {
+    (void)hipSetDevice(ctx->device);
+    DV_HIP(hipFree(ctx->pad_p));
+    ctx->pad_p = nullptr;
+    ctx->pad_size = 0;
+  }
   ctx->tune[key] = (int)value;
   return DEVIL_OK;
 }
@@ -1741,6 +1795,7 @@ int devil_fit_info(const devil_ctx* ctx, int32_t key, int64_t* value) {
       return DEVIL_OK;
     case DEVIL_INFO_CHUNK_CELLS: *value = ctx->last_cc; return DEVIL_OK;
     case DEVIL_INFO_POLISHED: *value = ctx->polished; return DEVIL_OK;
+    case DEVIL_INFO_ALIGNED_COPY: *value = ctx->aligned_copy; return DEVIL_OK;
     default: return DEVIL_E_ARG;
   }
 }

@@ -251,6 +251,8 @@ hipError_t launch_group_cells(hipStream_t s, Counts c, int64_t G, int64_t n,
 hipError_t launch_csc_to_dense(hipStream_t s, const int64_t* colptr, const int32_t* rowind,
                                const double* values, int64_t G, int64_t n, int out_dtype,
                                void* out, int64_t ld);
+hipError_t launch_pad_rows(hipStream_t s, const void* src, int64_t ld_src, int64_t G, int64_t n,
+                           void* dst, int64_t ld_dst);
 hipError_t launch_check_order(hipStream_t s, const int64_t* ord, int64_t n, int32_t* flag);
 hipError_t launch_check_csc(hipStream_t s, const int64_t* colptr, const int32_t* rowind,
                             int64_t G, int64_t n, int64_t nnz, int32_t* flag);

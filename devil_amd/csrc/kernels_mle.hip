@@ -304,7 +304,6 @@ __device__ __forceinline__ void mle_tile_chunk(const TileDesign& td, const CT* _
 #pragma unroll
     for (int j = 0; j < NJ; ++j) eta = mfma16x16x4(cur.xa[j], bpos[j], eta);
     const int64_t c0 = b * 16 + 4 * h;
-    double w0[4], w1[4], w2[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       double mu = exp_f64(eta[r] + cur.off[r]);
@@ -315,9 +314,18 @@ __device__ __forceinline__ void mle_tile_chunk(const TileDesign& td, const CT* _
       const double L = log1p_f64(mt);
       const double f = (mt < 1e-2) ? nb_f(mt) : L - mt * rr;
       const double w = mu * rr;
-      w0[r] = w;
-      w1[r] = -1.0 * w * w;
-      w2[r] = -2.0 * w1[r] * w;
+      const double w1 = -1.0 * w * w;
+      const double w2 = -2.0 * w1 * w;
+      // the three Gram sums take this cell row at once (each accumulator still
+      // adds r = 0..3 in order), so no per-row weights stay live
+#pragma unroll
+      for (int t = 0; t < NTMAX; ++t) {
+        if (t < NT) {
+          a0[t] = mfma16x16x4(w, cur.kb[t][r], a0[t]);
+          a1[t] = mfma16x16x4(w1, cur.kb[t][r], a1[t]);
+          a2[t] = mfma16x16x4(w2, cur.kb[t][r], a2[t]);
+        }
+      }
       if (c0 + r < td.n) {
         sc += f + y * theta * rr;
         l1 += L + (y - mu) * theta * rr;
@@ -328,17 +336,6 @@ __device__ __forceinline__ void mle_tile_chunk(const TileDesign& td, const CT* _
           dg += dpsi_diff(tn1, y);
           tg += dpsi1_diff(tn1, y);
           if (want_f) lg += ldiff(tn1, y);
-        }
-      }
-    }
-#pragma unroll
-    for (int t = 0; t < NTMAX; ++t) {
-      if (t < NT) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          a0[t] = mfma16x16x4(w0[r], cur.kb[t][r], a0[t]);
-          a1[t] = mfma16x16x4(w1[r], cur.kb[t][r], a1[t]);
-          a2[t] = mfma16x16x4(w2[r], cur.kb[t][r], a2[t]);
         }
       }
     }

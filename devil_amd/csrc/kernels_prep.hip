@@ -244,6 +244,36 @@ hipError_t launch_csc_to_dense(hipStream_t s, const int64_t* colptr, const int32
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// rows of 4-byte counts copied into rows of ld_dst elements (a multiple of 4,
+// so every destination row starts 16-byte aligned); grid (cell blocks of
+// 1024, genes)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_pad_rows(const uint32_t* __restrict__ src,
+                                                  int64_t ld_src, int64_t G, int64_t n,
+                                                  uint32_t* __restrict__ dst, int64_t ld_dst) {
+  const int64_t g = blockIdx.y + (int64_t)blockIdx.z * 65535;
+  if (g >= G) return;
+  const uint32_t* s = src + g * ld_src;
+  uint32_t* d = dst + g * ld_dst;
+  const int64_t i0 = (int64_t)blockIdx.x * 1024 + threadIdx.x;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int64_t i = i0 + 256 * k;
+    if (i < n) d[i] = s[i];
+  }
+}
+
+hipError_t launch_pad_rows(hipStream_t s, const void* src, int64_t ld_src, int64_t G, int64_t n,
+                           void* dst, int64_t ld_dst) {
+  if (G <= 0 || n <= 0) return hipSuccess;
+  const dim3 grid((unsigned)((n + 1023) / 1024), (unsigned)std::min<int64_t>(G, 65535),
+                  (unsigned)((G + 65534) / 65535));
+  hipLaunchKernelGGL(k_pad_rows, grid, dim3(256), 0, s, static_cast<const uint32_t*>(src),
+                     ld_src, G, n, static_cast<uint32_t*>(dst), ld_dst);
+  return hipGetLastError();
+}
+
 // argument checks on the device: bad_flag |= 1 when an index is out of range
 __global__ void k_check_order(const int64_t* __restrict__ ord, int64_t n,
                               int32_t* __restrict__ bad_flag) {

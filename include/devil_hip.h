@@ -261,7 +261,11 @@ enum { DEVIL_TUNE_MIXED_KERNEL = 0,  /* DEVIL_MIXED_* (default AUTO) */
        DEVIL_TUNE_CHUNK_CELLS = 1,   /* cells per chunk of the tile passes, 0 = autotune */
        DEVIL_TUNE_CONCURRENCY = 2,   /* streams for multi-problem fits (default 8) */
        DEVIL_TUNE_MLE_DRIVER = 3,    /* DEVIL_MLE_* (default NLMINB) */
-       DEVIL_N_TUNE = 4 };
+       DEVIL_TUNE_PAD_COPY = 4,      /* 1 (default): a mixed fit of device counts whose rows
+                                        are not 16-byte aligned reads a copy with padded rows,
+                                        kept by the context for the next fit (freed by
+                                        devil_destroy or by setting 0); 0: never */
+       DEVIL_N_TUNE = 5 };
 /* MLE overdispersion drivers.  NLMINB: the reference's stats::nlminb call
  * (R/overdispersion.R:53-73) restated for one parameter -- its trust-region
  * Newton steps and its stopping rules (relative function convergence at
@@ -303,7 +307,9 @@ int devil_kernel_stats(const devil_ctx *ctx, int32_t kernel, int64_t *launches, 
  * stopping step came within 1e-3 eps of the threshold, or was NaN), and the
  * mixed kernel (DEVIL_MIXED_*, -1 for FP64) and chunk size it ran with. */
 enum { DEVIL_INFO_REPLAYED = 0, DEVIL_INFO_MIXED_KERNEL = 1, DEVIL_INFO_CHUNK_CELLS = 2,
-       DEVIL_INFO_POLISHED = 3 /* genes whose last mixed IRLS step was redone in FP64 */ };
+       DEVIL_INFO_POLISHED = 3, /* genes whose last mixed IRLS step was redone in FP64 */
+       DEVIL_INFO_ALIGNED_COPY = 4 /* 1 if the mixed fit read a copy of device counts whose
+                                      rows it padded to 16-byte alignment */ };
 int devil_fit_info(const devil_ctx *ctx, int32_t key, int64_t *value);
 
 #ifdef __cplusplus

@@ -225,3 +225,28 @@ def test_mixed_small_coefficients_c5(dv):
         eb = relmax(mix["beta"].cpu().numpy(), f64["beta"].cpu().numpy(), BETA_FLOOR)
         say(f"  beta {eb:.2e}")
         assert eb <= TOL
+
+
+def test_mixed_misaligned_device_rows(dv):
+    """Device counts whose rows are not 16-byte aligned (n odd) run the
+    LDS-staged kernel on the library's aligned copy: results equal the same
+    counts fitted from host memory (padded rows) bit for bit."""
+    from devil_amd import synth
+    w = synth.make_workload("C3", G=256, n=20011, seed=31)
+    Y = synth.counts_host(w)
+    say("[case] mixed, misaligned device rows, n=20011")
+    ctx = dv.context(0)
+    ctx.set_mixed_kernel("bf16_ws")
+    try:
+        fh = dv.fit_devil(Y, w.X, size_factors=w.sf, precision="mixed")
+        ih = ctx.fit_info()
+        fd = dv.fit_devil(torch.as_tensor(Y, device="cuda"), w.X, size_factors=w.sf,
+                          precision="mixed")
+        idv = ctx.fit_info()
+    finally:
+        ctx.set_mixed_kernel("auto")
+    assert ih["mixed_kernel"] == "bf16_ws" and idv["mixed_kernel"] == "bf16_ws"
+    assert ih["aligned_copy"] == 0 and idv["aligned_copy"] == 1
+    np.testing.assert_array_equal(np.asarray(fh["beta"]), fd["beta"].cpu().numpy())
+    np.testing.assert_array_equal(np.asarray(fh["iterations"]["beta_iters"]),
+                                  fd["iterations"]["beta_iters"].cpu().numpy())
