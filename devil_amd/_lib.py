@@ -15,12 +15,12 @@ DEVIL_OK = 0
 MEM_HOST, MEM_DEVICE = 0, 1
 COUNTS_I32, COUNTS_F32, COUNTS_F64 = 0, 1, 2
 INIT_QR, INIT_ROUGH, INIT_UNIVARIATE = 0, 1, 2
-OD_NONE, OD_MOM, OD_MLE = 0, 1, 2
+OD_NONE, OD_MOM, OD_MLE, OD_NEW = 0, 1, 2, 3
 PREC_FP64, PREC_MIXED = 0, 1
 PMAX = 10
 
 GENE_MAXITER, GENE_NAN, GENE_SINGULAR, GENE_MOM_DEN0, GENE_MLE_ZERO = 1, 2, 4, 8, 16
-GENE_MLE_NOCONV = 32
+GENE_MLE_NOCONV, GENE_OD_FAILED = 32, 64
 
 _vp = C.c_void_p
 

@@ -16,7 +16,8 @@ import numpy as np
 
 from . import _lib as L
 
-_OD_MODES = {"MOM": L.OD_MOM, "MLE": L.OD_MLE, "old": L.OD_MLE}
+_OD_MODES = {"MOM": L.OD_MOM, "MLE": L.OD_MLE, "old": L.OD_MLE, "new": L.OD_NEW,
+             "I": L.OD_NEW}
 
 
 def _is_torch(x) -> bool:

@@ -21,7 +21,8 @@ LIB = os.path.join(HERE, "libdevil_hip.so")
 ROOT = os.path.dirname(HERE)
 
 SOURCES = ["kernels_epilogue.hip", "kernels_tile.hip", "kernels_mle.hip", "kernels_post.hip",
-           "kernels_design.hip", "kernels_prep.hip", "devil_hip.cpp"]
+           "kernels_design.hip", "kernels_prep.hip", "kernels_newod.hip",
+           "devil_hip.cpp"]
 
 COMMON = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall",

@@ -19,6 +19,7 @@
 
 #include "../../include/devil_hip.h"
 #include "kernels.hpp"
+#include "newod_driver.hpp"
 #include "port_driver.hpp"
 
 // Device blocks of a context kept for reuse after the scope that used them
@@ -1133,7 +1134,7 @@ int run_mle_tiled(devil_ctx* ctx, const dv::TileDesign& td, dv::Counts c, int64_
   const int W = 8 + 3 * td.ncp;
   const int64_t ntile = (G + 15) / 16;
   DV_ALLOC(part, sizeof(double) * (size_t)ntile * td.nchunk * 16 * W);
-  dv::MleState ms;
+  dv::MleState ms{};  // raw_start stays NULL: the MLE uses the fallback start
   double* d = dbl.as<double>();
   ms.x = d;
   ms.lo = d + G;
@@ -1198,6 +1199,103 @@ int run_mle_tiled(devil_ctx* ctx, const dv::TileDesign& td, dv::Counts c, int64_
     std::swap(ncur, nnxt);
   }
   collect_launches(ctx);
+  return DEVIL_OK;
+}
+
+// "new" / "I" overdispersion over genes [0, G): the setup pass (count table,
+// moment start), then rounds of one F_k evaluation per active gene until
+// every gene's driver has finished.  theta, iters (kappa_iters) are outputs;
+// status is or-ed.
+int run_newod_tiled(devil_ctx* ctx, const dv::TileDesign& td, dv::Counts c, int64_t G,
+                    int64_t n, int p, const double* beta, int max_iter, double tolerance,
+                    double* theta, int32_t* iters, int32_t* status) {
+  hipStream_t s = ctx->stream;
+  if (G == 0) return DEVIL_OK;
+  DevBuf dbl, ints, tab, act0, act1, cnt, part, st, kreq, scr;
+  DV_ALLOC(dbl, sizeof(double) * G * 8);
+  DV_ALLOC(ints, sizeof(int32_t) * G * 2);
+  DV_ALLOC(tab, sizeof(int32_t) * G * dv::kMleTabCap * 2);
+  DV_ALLOC(act0, sizeof(int32_t) * G);
+  DV_ALLOC(act1, sizeof(int32_t) * G);
+  DV_ALLOC(cnt, sizeof(int32_t) * 2);
+  DV_ALLOC(st, sizeof(dv::NewodState) * G);
+  DV_ALLOC(kreq, sizeof(double) * G);
+  DV_ALLOC(scr, sizeof(int32_t) * G * 2 + sizeof(double) * G);
+  const int64_t ntile = (G + 15) / 16;
+  const size_t part_cap = (size_t)ntile * td.nchunk * 16 * 4;
+  DV_ALLOC(part, sizeof(double) * part_cap);
+  // the setup pass of the MLE: count table and (var y - ybar) / ybar^2; its
+  // MLE special cases land in scratch (the "new" driver runs every gene)
+  dv::MleState ms{};
+  double* d = dbl.as<double>();
+  ms.x = d;
+  ms.lo = d + G;
+  ms.hi = d + 2 * G;
+  ms.start = d + 3 * G;
+  ms.sum_y = d + 4 * G;
+  ms.sum_py = d + 5 * G;
+  ms.max_y = d + 6 * G;
+  ms.raw_start = d + 7 * G;
+  ms.mode = ints.as<int32_t>();
+  ms.tab_n = ints.as<int32_t>() + G;
+  ms.tab_u = tab.as<int32_t>();
+  ms.tab_f = tab.as<int32_t>() + G * dv::kMleTabCap;
+  ms.iters = scr.as<int32_t>();
+  ms.status = scr.as<int32_t>() + G;
+  ms.theta = reinterpret_cast<double*>(scr.as<int32_t>() + 2 * G);
+  ms.port = nullptr;
+  ms.driver = dv::kMleDriverRoot;
+  DV_HIP(dv::launch_fill_f64(s, ms.raw_start, G, std::nan("")));
+  DV_HIP(dv::launch_fill_i32(s, ms.status, G, 0));
+  DV_HIP(dv::launch_mle_setup(s, c, G, n, p, ms));
+  // the newod kernels write the real outputs
+  ms.iters = iters;
+  ms.status = status;
+  ms.theta = theta;
+  int32_t* cur = act0.as<int32_t>();
+  int32_t* nxt = act1.as<int32_t>();
+  int32_t* ncur = cnt.as<int32_t>();
+  int32_t* nnxt = cnt.as<int32_t>() + 1;
+  DV_HIP(dv::launch_fill_i32(s, ncur, 1, 0));
+  DV_HIP(dv::launch_newod_init(s, G, ms.raw_start, st.as<dv::NewodState>(), kreq.as<double>(),
+                               cur, ncur, max_iter, tolerance));
+  int32_t nA = (int32_t)G;
+  const double row_bytes = (double)count_elem_size(c.dtype) * (double)n;
+  // every evaluation is one round; the driver's evaluations are bounded by
+  // its loops (Newton start, Aitken steps with backtracking, the bracket
+  // search and at most 60 Brent steps)
+  const int64_t max_rounds = 16 + (int64_t)std::max(max_iter, 1) * 40 + 100;
+  for (int64_t round = 0; round < max_rounds && nA > 0; ++round) {
+    dv::TileDesign tr = td;
+    {
+      const int64_t nt = (nA + 15) / 16, groups = (nt + 3) / 4;
+      const int64_t cap = (int64_t)(part_cap / ((size_t)nt * 16 * 4));
+      const int64_t want = std::min({(kRoundWorkgroups + groups - 1) / groups, kMaxRoundChunks,
+                                     cap});
+      if (groups * td.nchunk < kRoundWorkgroups && want > td.nchunk) {
+        dv::TileDesign t2 = td;
+        set_chunks(t2, std::max(kMinRoundBlocks, (td.nblk + want - 1) / want));
+        if (t2.nchunk > td.nchunk && t2.nchunk <= cap) tr = t2;
+      }
+    }
+    {
+      LaunchTimer lt(ctx, DEVIL_KERNEL_MLE, row_bytes * (double)nA);
+      DV_HIP(dv::launch_newod_accum(s, tr, c, p, cur, ncur, nA, beta, kreq.as<double>(),
+                                    ms.tab_n, part.as<double>()));
+    }
+    DV_HIP(dv::launch_fill_i32(s, nnxt, 1, 0));
+    DV_HIP(dv::launch_newod_update(s, tr.nchunk, cur, ncur, nA, part.as<double>(), ms,
+                                   st.as<dv::NewodState>(), kreq.as<double>(), n, max_iter,
+                                   tolerance, nxt, nnxt));
+    int32_t h = 0;
+    DV_HIP(hipMemcpyAsync(&h, nnxt, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    DV_HIP(hipStreamSynchronize(s));
+    nA = h;
+    std::swap(cur, nxt);
+    std::swap(ncur, nnxt);
+  }
+  collect_launches(ctx);
+  if (nA > 0) return fail(ctx, DEVIL_E_HIP, "new overdispersion: evaluation rounds exhausted");
   return DEVIL_OK;
 }
 
@@ -1410,7 +1508,7 @@ int fit_one(devil_ctx* ctx, const devil_problem* pr, devil_outputs* out) {
   if (!pr->X || !pr->offset) return fail(ctx, DEVIL_E_ARG, "X or offset is NULL");
   if (pr->init_mode < DEVIL_INIT_QR || pr->init_mode > DEVIL_INIT_UNIVARIATE)
     return fail(ctx, DEVIL_E_ARG, "unknown init_mode");
-  if (pr->od_mode < DEVIL_OD_NONE || pr->od_mode > DEVIL_OD_MLE)
+  if (pr->od_mode < DEVIL_OD_NONE || pr->od_mode > DEVIL_OD_NEW)
     return fail(ctx, DEVIL_E_ARG, "unknown od_mode (R/main.R:467)");
   if (pr->max_iter < 0) return fail(ctx, DEVIL_E_ARG, "max_iter < 0");
   if (pr->precision != DEVIL_PREC_FP64 && pr->precision != DEVIL_PREC_MIXED)
@@ -1601,6 +1699,10 @@ int fit_one(devil_ctx* ctx, const devil_problem* pr, devil_outputs* out) {
     } else if (pr->od_mode == DEVIL_OD_MLE) {
       if ((rc = run_mle_tiled(ctx, tl.td, c, Gb, n, p, b0, pr->max_iter, pr->tolerance, th,
                               thit.as<int32_t>() + g0, status.as<int32_t>() + g0)))
+        return rc;
+    } else if (pr->od_mode == DEVIL_OD_NEW) {
+      if ((rc = run_newod_tiled(ctx, tl.td, c, Gb, n, p, b0, pr->max_iter, pr->tolerance, th,
+                                thit.as<int32_t>() + g0, status.as<int32_t>() + g0)))
         return rc;
     } else {
       DV_HIP(dv::launch_fill_f64(s, th, Gb, 0.0));

@@ -184,26 +184,28 @@ hipError_t launch_polish_select(hipStream_t s, int64_t G, int p, const double* b
                                 int32_t* list, int32_t* nlist);
 hipError_t launch_iota(hipStream_t s, int32_t* p, int n);
 
-struct PortState;  // port_driver.hpp
+struct PortState;   // port_driver.hpp
+struct NewodState;  // newod_driver.hpp
 
 // Per-gene state of the tiled MLE (kernels_mle.hip); gene-major device arrays.
 struct MleState {
-  double* x;       // current log(theta)
-  double* lo;      // Newton bracket
-  double* hi;
-  double* start;   // (var(y) - ybar) / ybar^2, or 0.5
-  double* sum_y;
-  double* sum_py;  // sum (y - 1) y
-  double* max_y;
-  int32_t* mode;   // 0 far-left pass pending, 1 Newton, 2 done
-  int32_t* iters;  // score passes
-  double* theta;   // result
-  int32_t* status; // DEVIL_GENE_* bits (or-ed)
-  int32_t* tab_n;  // distinct positive counts in the gene's table, -1 = none
-  int32_t* tab_u;  // [g * kMleTabCap + k] value
-  int32_t* tab_f;  // [g * kMleTabCap + k] frequency
-  PortState* port; // per-gene state of the nlminb driver
-  int driver;      // kMleDriver*
+  double* x = nullptr;       // current log(theta)
+  double* lo = nullptr;      // Newton bracket
+  double* hi = nullptr;
+  double* start = nullptr;   // (var(y) - ybar) / ybar^2, or 0.5
+  double* sum_y = nullptr;
+  double* sum_py = nullptr;  // sum (y - 1) y
+  double* max_y = nullptr;
+  int32_t* mode = nullptr;   // 0 far-left pass pending, 1 Newton, 2 done
+  int32_t* iters = nullptr;  // score passes
+  double* theta = nullptr;   // result
+  int32_t* status = nullptr; // DEVIL_GENE_* bits (or-ed)
+  int32_t* tab_n = nullptr;  // distinct positive counts in the gene's table, -1 = none
+  int32_t* tab_u = nullptr;  // [g * kMleTabCap + k] value
+  int32_t* tab_f = nullptr;  // [g * kMleTabCap + k] frequency
+  PortState* port = nullptr; // per-gene state of the nlminb driver
+  int driver = 0;            // kMleDriver*
+  double* raw_start = nullptr;  // (var(y) - ybar) / ybar^2 before any fallback, or NULL
 };
 constexpr int kMleTabCap = 4096;  // one entry per integer count below 4096
 enum { kMleDriverNlminb = 0, kMleDriverRoot = 1 };
@@ -223,6 +225,19 @@ hipError_t launch_mle_update(hipStream_t s, const TileDesign& td, int p, const i
 
 // k: per-gene k as src/beta.cpp holds it (float(1/float(alpha0)), widened)
 hipError_t launch_k_from_alpha(hipStream_t s, const double* alpha0, int64_t G, double* k);
+
+// "new" / "I" overdispersion (kernels_newod.hip)
+hipError_t launch_newod_init(hipStream_t s, int64_t G, const double* raw_start,
+                             NewodState* st, double* kreq, int32_t* act, int32_t* nact,
+                             int max_iter, double eps);
+hipError_t launch_newod_accum(hipStream_t s, const TileDesign& td, Counts c, int p,
+                              const int32_t* active, const int32_t* nactive, int nA_bound,
+                              const double* beta, const double* kreq, const int32_t* tab_n,
+                              double* part);
+hipError_t launch_newod_update(hipStream_t s, int nchunk, const int32_t* active,
+                               const int32_t* nactive, int nA_bound, const double* part,
+                               const MleState& ms, NewodState* st, double* kreq, int64_t n,
+                               int max_iter, double eps, int32_t* next, int32_t* nnext);
 
 hipError_t launch_test_de(hipStream_t s, const double* beta, const double* bread,
                           const double* sandwich, int64_t G, int p, double df,

@@ -243,6 +243,7 @@ __global__ __launch_bounds__(kSetupThreads) void k_mle_setup(Counts c, int64_t G
   }
   const double ybar = s1 / (double)n;
   double start = (v - ybar) / (ybar * ybar);
+  if (t == 0 && ms.raw_start) ms.raw_start[g] = start;
   if (isnan(start) || start <= 0) start = 0.5;
   if (t == 0) {
     ms.mode[g] = kMleFarLeft;

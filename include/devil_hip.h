@@ -70,12 +70,17 @@ extern "C" {
                                     (R/overdispersion.R:75-110), which this library does not
                                     restate; theta is the first run's par */
 
+#define DEVIL_GENE_OD_FAILED 64 /* "new" overdispersion: the reference stops with an error here
+                                     (C1 <= n, a Laplace mode or quadrature failure, f not finite
+                                     at the start); theta = NaN */
+
 #define DEVIL_PMAX 10          /* largest design width this library supports */
 
 enum { DEVIL_MEM_HOST = 0, DEVIL_MEM_DEVICE = 1 };
 enum { DEVIL_COUNTS_I32 = 0, DEVIL_COUNTS_F32 = 1, DEVIL_COUNTS_F64 = 2 };
 enum { DEVIL_INIT_QR = 0, DEVIL_INIT_ROUGH = 1, DEVIL_INIT_UNIVARIATE = 2 };
-enum { DEVIL_OD_NONE = 0, DEVIL_OD_MOM = 1, DEVIL_OD_MLE = 2 };
+enum { DEVIL_OD_NONE = 0, DEVIL_OD_MOM = 1, DEVIL_OD_MLE = 2,
+       DEVIL_OD_NEW = 3 /* "new" / "I": fit_overdispersion_cppp, src/new_overdispersion.cpp */ };
 enum { DEVIL_PREC_FP64 = 0, DEVIL_PREC_MIXED = 1 };
 
 typedef struct devil_ctx devil_ctx;
@@ -116,7 +121,7 @@ typedef struct {                /* caller-allocated, same memory space as the pr
   double *beta;                 /* G x p column-major */
   double *theta;                /* G: overdispersion (MOM / MLE / 0) */
   int32_t *beta_iters;          /* G */
-  int32_t *theta_iters;         /* G: MLE score passes; 0 for MOM / none */
+  int32_t *theta_iters;         /* G: MLE passes; kappa_iters for "new"; 0 for MOM / none */
   double *bread;                /* G x p x p: beta_sandwiches_null */
   double *sandwich;             /* G x p x p, or NULL: beta_sandwiches (clusters only) */
   int32_t *status;              /* G, or NULL: DEVIL_GENE_* bits */

@@ -24,7 +24,7 @@ _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
 _lp = C.POINTER(C.c_int64)
 
-OD_NONE, OD_MOM, OD_MLE = 0, 1, 2
+OD_NONE, OD_MOM, OD_MLE, OD_NEW = 0, 1, 2, 3
 INIT_QR, INIT_ROUGH, INIT_UNIVARIATE = 0, 1, 2
 
 
@@ -96,6 +96,11 @@ def lib():
                                    _dp, _dp]),
             "orc_p_adjust_bh": (None, [_dp, i64, _dp]),
             "orc_size_factors": (C.c_int, [_dp, i64, i64, i64, C.c_int, _dp]),
+            "orc_fit_overdispersion_new": (d, [_dp, _dp, _dp, _dp, i64, C.c_int, d, C.c_int, d,
+                                               C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                               C.POINTER(C.c_int), C.POINTER(C.c_int64)]),
+            "orc_h_log_gh_pmhalf": (C.c_int, [d, d, d, _dp, _dp]),
+            "orc_gh16": (None, [_dp, _dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -407,3 +412,29 @@ def group_data(Y, X, clusters):
     codes = rank[inv] + 1
     ord_ = np.argsort(codes, kind="stable")
     return np.asarray(Y)[:, ord_], np.asarray(X)[ord_], codes[ord_].astype(float), ord_
+
+
+def fit_overdispersion_new(y, X, beta, off, kappa=1.0, max_iter=200, eps_theta=1e-3):
+    """src/new_overdispersion.cpp:195-481 for one gene: dict(kappa, kappa_iters,
+    kappa_converged, err, evals); err != 0 where the reference stops."""
+    X = np.asfortranarray(X, dtype=np.float64)
+    y = _f64(y)
+    it, conv, err, ev = C.c_int(), C.c_int(), C.c_int(), C.c_int64()
+    k = lib().orc_fit_overdispersion_new(_d(y), X.ctypes.data_as(_dp), _d(_f64(beta)),
+                                         _d(_f64(off)), y.size, X.shape[1], float(kappa),
+                                         int(max_iter), float(eps_theta), C.byref(it),
+                                         C.byref(conv), C.byref(err), C.byref(ev))
+    return dict(kappa=k, kappa_iters=it.value, kappa_converged=bool(conv.value),
+                err=err.value, evals=ev.value)
+
+
+def h_log_gh_pmhalf(s, t, tol=1e-3):
+    lp, lm = C.c_double(), C.c_double()
+    ok = lib().orc_h_log_gh_pmhalf(float(s), float(t), float(tol), C.byref(lp), C.byref(lm))
+    return (lp.value, lm.value) if ok else None
+
+
+def gh16():
+    x, w = np.zeros(16), np.zeros(16)
+    lib().orc_gh16(_d(x), _d(w))
+    return x, w

@@ -1543,6 +1543,12 @@ int orc_cpu_fit(const orc_fit_args *a, orc_fit_out *o) {
         theta = orc_fit_dispersion_port(beta, a->X, y, a->offset, n, p, a->tolerance,
                                         a->max_iter, 1, &th_it, &code, &nfev);
         th_it = nfev + 1;  /* passes: the far-left check and each evaluation */
+      } else if (a->od_mode == ORC_OD_NEW) {
+        int conv, err;
+        int64_t evals;
+        theta = orc_fit_overdispersion_new(y, a->X, beta, a->offset, n, p, alpha0[s],
+                                           a->max_iter, a->tolerance, &th_it, &conv, &err,
+                                           &evals);
       } else if (a->od_mode == ORC_OD_MOM) {
         theta = orc_mom_dispersion(y, n, a->X, p, beta, eo);
       } else {
@@ -1696,4 +1702,369 @@ int orc_size_factors(const double *Y, int64_t G, int64_t n, int64_t ld, int meth
   const double gm = exp(ml);
   for (int64_t i = 0; i < n; ++i) sf[i] /= gm;
   return 0;
+}
+
+/* ===================================================================== */
+/* "new" / "I" overdispersion: src/new_overdispersion.cpp:20-481          */
+/* ===================================================================== */
+
+/* Gauss-Hermite rule of order 16 (weight exp(-x^2)): the nodes and weights
+ * the reference computes by Golub-Welsch (src/new_overdispersion.cpp:24-48;
+ * gh_order = newton_max = 16 in F_k, :287).  Pinned against
+ * numpy.polynomial.hermite.hermgauss and an eigendecomposition of the
+ * Jacobi matrix (tests/test_oracle_newod.py). */
+static const double gh16_x[16] = {
+    -4.688738939305819, -3.869447904860123, -3.176999161979956, -2.5462021578474814,
+    -1.9517879909162539, -1.3802585391988809, -0.8229514491446559, -0.27348104613815244,
+    0.27348104613815244, 0.8229514491446559, 1.3802585391988809, 1.9517879909162539,
+    2.5462021578474814, 3.176999161979956, 3.869447904860123, 4.688738939305819};
+static const double gh16_w[16] = {
+    2.6548074740111673e-10, 2.3209808448652032e-07, 2.7118600925378892e-05,
+    0.0009322840086241807, 0.012880311535509989, 0.08381004139898583, 0.2806474585285337,
+    0.5079294790166137, 0.5079294790166137, 0.2806474585285337, 0.08381004139898583,
+    0.012880311535509989, 0.0009322840086241807, 2.7118600925378892e-05,
+    2.3209808448652032e-07, 2.6548074740111673e-10};
+
+void orc_gh16(double *x, double *w) {
+  for (int i = 0; i < 16; ++i) {
+    x[i] = gh16_x[i];
+    w[i] = gh16_w[i];
+  }
+}
+
+/* h(x) = p log x + g(s, t, x), g = s (x log x - lgamma x) - t x (:52-66) */
+static double nod_g(double s, double t, double x) { return s * (x * log(x) - lgam(x)) - t * x; }
+static double nod_g1(double s, double t, double x) {
+  return s * (log(x) + 1.0 - orc_digamma(x)) - t;
+}
+static double nod_g2(double s, double x) { return s * (1.0 / x - orc_trigamma(x)); }
+
+/* find_mode (:70-93): Newton on h'(x) = p/x + g1; 0 when h'' at the mode is
+ * not negative or not finite (the reference stops with an error) */
+static int nod_find_mode(double p, double s, double t, int max_newton, double tol, double x0,
+                         double *mu0, double *sigma0, double *h_mu0) {
+  double x = fmax(1e-9, x0);
+  for (int i = 0; i < max_newton; ++i) {
+    const double g1v = p / x + nod_g1(s, t, x);
+    if (fabs(g1v) < tol) break;
+    const double g2v = -p / (x * x) + nod_g2(s, x);
+    const double step = g1v / g2v;
+    double xn = x - step;
+    if (!(xn > 0.0) || !isfinite(xn)) xn = 0.5 * x;
+    if (fabs(xn - x) < tol * (1.0 + fabs(x))) {
+      x = xn;
+      break;
+    }
+    x = xn;
+  }
+  const double curv = -(-p / (x * x) + nod_g2(s, x));
+  if (!(curv > 0.0) || !isfinite(curv)) return 0;
+  *mu0 = fmax(1e-12, x);
+  *sigma0 = sqrt(1.0 / curv);
+  *h_mu0 = p * log(*mu0) + nod_g(s, t, *mu0);
+  return 1;
+}
+
+/* log_integral_at (:95-123): log of the Laplace-centred Gauss-Hermite sum
+ * (two-pass log-sum-exp, long double accumulator); 0 on underflow */
+static int nod_log_integral(double p, double s, double t, double mu0, double sigma0,
+                            double h_mu0, double *out) {
+  const double scale = sigma0 * sqrt(2.0);
+  double max_lt = -INFINITY;
+  for (int i = 0; i < 16; ++i) {
+    const double u = gh16_x[i], xx = mu0 + scale * u;
+    if (xx <= 0.0 || !isfinite(xx)) continue;
+    const double r = (p * log(xx) + nod_g(s, t, xx)) - h_mu0 + u * u;
+    const double lt = log(gh16_w[i]) + r;
+    if (lt > max_lt) max_lt = lt;
+  }
+  long double acc = 0.0L;
+  for (int i = 0; i < 16; ++i) {
+    const double u = gh16_x[i], xx = mu0 + scale * u;
+    if (xx <= 0.0 || !isfinite(xx)) continue;
+    const double r = (p * log(xx) + nod_g(s, t, xx)) - h_mu0 + u * u;
+    const double lt = log(gh16_w[i]) + r;
+    acc += expl((long double)(lt - max_lt));
+  }
+  if (!(acc > 0.0L)) return 0;
+  *out = h_mu0 + log(scale) + (max_lt + log((double)acc));
+  return 1;
+}
+
+/* H_log_gh_pmhalf (:141-158) with gh_order = max_newton = 16 */
+int orc_h_log_gh_pmhalf(double s, double t, double tol, double *logH_p, double *logH_m) {
+  if (!(t > s)) return 0;
+  const double x0_1 = fmax(1e-3, (0.5 + s) / (t - s));
+  double m1, s1, h1, m2, s2, h2;
+  if (!nod_find_mode(0.5, s, t, 16, tol, x0_1, &m1, &s1, &h1)) return 0;
+  if (!nod_find_mode(-0.5, s, t, 16, tol, m1, &m2, &s2, &h2)) return 0;
+  if (!nod_log_integral(0.5, s, t, m1, s1, h1, logH_p)) return 0;
+  if (!nod_log_integral(-0.5, s, t, m2, s2, h2, logH_m)) return 0;
+  return 1;
+}
+
+typedef struct {
+  const double *y, *w;      /* counts, w_q = exp(-(X beta + off)) */
+  const double *keys, *cnt; /* frequency table (nu > 0) */
+  int64_t n, nu;
+  double xbeta_sum, eps;
+  int err;                  /* set when the reference would stop() */
+  int64_t evals;
+} nod_ctx;
+
+/* F_k (:270-291): the fixed-point map exp(logH_p - logH_m) at k */
+static double nod_F(nod_ctx *c, double kk) {
+  c->evals++;
+  double sum_digamma = 0.0;
+  if (c->nu > 0) {
+    for (int64_t j = 0; j < c->nu; ++j) sum_digamma += c->cnt[j] * orc_digamma(kk + c->keys[j]);
+  } else {
+    for (int64_t i = 0; i < c->n; ++i) sum_digamma += orc_digamma(kk + c->y[i]);
+  }
+  double sum_log1p = 0.0, sum_mu_g_wq = 0.0;
+  for (int64_t i = 0; i < c->n; ++i) {
+    const double wi = c->w[i], yi = c->y[i];
+    const double kw = kk * wi, den = 1.0 + kw;
+    sum_log1p += log1p(kw);
+    sum_mu_g_wq += (kk + yi) * wi / den;
+  }
+  const double C1 = c->xbeta_sum - (sum_digamma - sum_log1p) + sum_mu_g_wq;
+  if (!(C1 > (double)c->n)) {
+    c->err = 1;
+    return NAN;
+  }
+  double lp, lm;
+  if (!orc_h_log_gh_pmhalf((double)c->n, C1, c->eps, &lp, &lm)) {
+    c->err = 2;
+    return NAN;
+  }
+  return exp(lp - lm);
+}
+
+static double nod_f(nod_ctx *c, double kk) {
+  const double kn = nod_F(c, kk);
+  return log(kn) - log(kk);
+}
+
+/* brent (:319-348) on f_of_k; fa, fb as the caller passes them */
+static double nod_brent(nod_ctx *c, double a, double b, double fa, double fb) {
+  double cc = a, fc = fa, d = b - a, e = d;
+  const double tol = c->eps, eps_m = DBL_EPSILON;
+  for (int it = 0; it < 60; ++it) {
+    if (fabs(fc) < fabs(fb)) {
+      a = b; b = cc; cc = a;
+      fa = fb; fb = fc; fc = fa;
+    }
+    const double tol1 = 2.0 * eps_m * fabs(b) + 0.5 * tol;
+    const double xm = 0.5 * (cc - b);
+    if (fabs(xm) <= tol1 || fb == 0.0) return b;
+    double s, p, q;
+    if (fabs(e) >= tol1 && fabs(fa) > fabs(fb)) {
+      s = fb / fa;
+      if (a == cc) {
+        p = 2.0 * xm * s;
+        q = 1.0 - s;
+      } else {
+        const double r = fb / fc, t = fa / fc;
+        p = s * (2.0 * xm * t * (t - r) - (b - a) * (r - 1.0));
+        q = (t - 1.0) * (r - 1.0) * (s - 1.0);
+      }
+      if (p > 0) q = -q;
+      p = fabs(p);
+      if (2.0 * p < fmin(3.0 * xm * q - fabs(tol1 * q), fabs(e * q))) {
+        e = d;
+        d = p / q;
+      } else {
+        d = xm;
+        e = d;
+      }
+    } else {
+      d = xm;
+      e = d;
+    }
+    a = b;
+    fa = fb;
+    b += (fabs(d) > tol1 ? d : (xm > 0 ? tol1 : -tol1));
+    fb = nod_f(c, b);
+    if ((fb > 0 && fc > 0) || (fb < 0 && fc < 0)) {
+      cc = a;
+      fc = fa;
+      e = d = b - a;
+    }
+  }
+  return b;
+}
+
+/* fit_overdispersion_cppp (src/new_overdispersion.cpp:195-481) for one gene:
+ * returns kappa (1/k); *iters = kappa_iters, *converged = kappa_converged,
+ * *err = 1 or 2 where the reference stops with an error (C1 <= n, a failed
+ * mode or quadrature), 3 when f is not finite at the start; *evals counts
+ * the F_k evaluations (passes over the gene's counts). */
+double orc_fit_overdispersion_new(const double *y, const double *X, const double *beta,
+                                  const double *off, int64_t n, int p, double kappa,
+                                  int max_iter, double eps_theta, int *iters, int *converged,
+                                  int *err, int64_t *evals) {
+  (void)kappa; /* overwritten by the moment estimate (:233-249), as in the reference */
+  double *w = (double *)malloc(sizeof(double) * (size_t)n);
+  double *keys = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+  double *cnt = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+  double xbeta_sum = 0.0;
+  for (int64_t i = 0; i < n; ++i) {
+    double eta = 0.0;
+    for (int j = 0; j < p; ++j) eta += X[i + j * n] * beta[j];
+    xbeta_sum += eta;
+    w[i] = exp(-(eta + off[i]));
+  }
+  int64_t nu = orc_make_table(y, n, 4096, keys, cnt);
+  if (nu < 0) nu = 0;
+  nod_ctx c = {y, w, keys, cnt, n, nu, xbeta_sum, eps_theta, 0, 0};
+  double ym = 0.0;
+  for (int64_t i = 0; i < n; ++i) ym += y[i];
+  ym /= (double)n;
+  double yv = 0.0;
+  if (n > 1) {
+    for (int64_t i = 0; i < n; ++i) yv += (y[i] - ym) * (y[i] - ym);
+    yv /= (double)(n - 1);
+  }
+  double kappa_mom = (yv - ym) / (ym * ym);
+  if (!isfinite(kappa_mom) || kappa_mom <= 0.0) kappa_mom = 0.01;
+  const double k_min = 1e-4, k_max = 1e4, jump_factor = 10;
+  double k_mom = 1.0 / kappa_mom;
+  if (!isfinite(k_mom)) k_mom = k_min;
+  double k = fmax(k_min, fmin(k_max, k_mom));
+  int kappa_converged = 0, it_kappa = 0, bad_accel = 0;
+
+  double k0 = fmax(k_min, fmin(k_max, k));
+  double f0 = nod_f(&c, k0);
+  if (c.err || !isfinite(f0)) {
+    if (!c.err) c.err = 3;
+    goto out;
+  }
+  {
+    /* one Newton step with a forward-difference derivative (:296-299) */
+    const double h = fmax(1e-8, 1e-6 * k0);
+    const double fp0 = (nod_f(&c, k0 + h) - f0) / h;
+    if (c.err) goto out;
+    if (isfinite(fp0) && fabs(fp0) > 1e-10) {
+      double k_newton = k0 - f0 / fp0;
+      k_newton = fmax(k_min, fmin(k_max, k_newton));
+      const double f_newton = nod_f(&c, k_newton);
+      if (c.err) goto out;
+      if (isfinite(f_newton) && fabs(f_newton) < fabs(f0)) {
+        k0 = k_newton;
+        f0 = f_newton;
+      }
+    }
+  }
+  {
+    double k1 = nod_F(&c, k0);
+    if (c.err) goto out;
+    k1 = fmin(k1, jump_factor * fmax(k0, 1.0));
+    k1 = fmax(k_min, fmin(k_max, k1));
+    double f1 = nod_f(&c, k1);
+    if (c.err) goto out;
+    if (f0 * f1 < 0) {
+      k = nod_brent(&c, fmin(k0, k1), fmax(k0, k1), f0, f1);
+      kappa_converged = 1;
+    } else {
+      for (it_kappa = 0; it_kappa < max_iter; ++it_kappa) {
+        const double adaptive_jump = jump_factor / (1.0 + 0.1 * it_kappa);
+        double k2 = nod_F(&c, k1);
+        if (c.err) goto out;
+        k2 = fmin(k2, adaptive_jump * fmax(k1, 1.0));
+        k2 = fmax(k_min, fmin(k_max, k2));
+        const double f2 = nod_f(&c, k2);
+        if (c.err) goto out;
+        const double d1 = k1 - k0, d2 = k2 - k1, denom = d2 - d1;
+        double k_acc = (fabs(denom) > 1e-14) ? k0 - (d1 * d1) / denom : k2;
+        if (!(k_acc > 0) || !isfinite(k_acc)) k_acc = k2;
+        k_acc = fmin(k_acc, adaptive_jump * fmax(k2, 1.0));
+        k_acc = fmax(k_min, fmin(k_max, k_acc));
+        double k_prop = k_acc;
+        double f_prop = nod_f(&c, k_prop);
+        if (c.err) goto out;
+        int bt = 0;
+        while ((fabs(f_prop) >= fabs(f1) || !isfinite(f_prop)) && bt < 8) {
+          k_prop = 0.5 * (k_prop + k1);
+          f_prop = nod_f(&c, k_prop);
+          if (c.err) goto out;
+          ++bt;
+        }
+        if (!(fabs(f_prop) < fabs(f1))) {
+          bad_accel++;
+          if (bad_accel == 1 && fabs(f1 - f0) > 1e-14) {
+            double k_sec = k1 - f1 * (k1 - k0) / (f1 - f0);
+            k_sec = fmax(k_min, fmin(k_max, k_sec));
+            const double f_sec = nod_f(&c, k_sec);
+            if (c.err) goto out;
+            if (isfinite(f_sec) && fabs(f_sec) < fabs(f1)) {
+              k_prop = k_sec;
+              f_prop = f_sec;
+              bad_accel = 0;
+            }
+          }
+          if (bad_accel >= 1) {
+            const double a = fmin(k1, k2), b = fmax(k1, k2);
+            const double fa = f1, fb = f2;
+            if (fa * fb < 0) {
+              k = nod_brent(&c, a, b, fa, fb);
+              kappa_converged = 1;
+              break;
+            }
+          }
+          if (bad_accel >= 1) {
+            /* find_bracket (:300-316) around k1 */
+            double blo = -1.0, bhi = -1.0;
+            const double f_center = nod_f(&c, k1);
+            if (c.err) goto out;
+            for (int expand = 1; expand <= 10; ++expand) {
+              const double factor = pow(2.0, expand);
+              const double lo = fmax(k_min, k1 / factor);
+              const double hi = fmin(k_max, k1 * factor);
+              const double flo = nod_f(&c, lo);
+              if (c.err) goto out;
+              if (flo * f_center < 0) { blo = lo; bhi = k1; break; }
+              const double fhi = nod_f(&c, hi);
+              if (c.err) goto out;
+              if (f_center * fhi < 0) { blo = k1; bhi = hi; break; }
+              if (flo * fhi < 0) { blo = lo; bhi = hi; break; }
+            }
+            if (blo > 0) {
+              const double flo = nod_f(&c, blo);
+              if (c.err) goto out;
+              const double fhi = nod_f(&c, bhi);
+              if (c.err) goto out;
+              k = nod_brent(&c, blo, bhi, flo, fhi);
+              kappa_converged = 1;
+              break;
+            }
+          }
+          k_prop = k2;
+          f_prop = f2;
+        } else {
+          bad_accel = 0;
+        }
+        if (fabs(k_prop - k1) < eps_theta * (1.0 + k1)) {
+          k = k_prop;
+          kappa_converged = 1;
+          break;
+        }
+        k0 = k1;
+        f0 = f1;
+        k1 = k_prop;
+        f1 = f_prop;
+      }
+      if (!kappa_converged) k = fmax(k_min, k1);
+    }
+  }
+out:
+  if (c.err) k = NAN;
+  *iters = it_kappa + 1;
+  *converged = kappa_converged;
+  *err = c.err;
+  *evals = c.evals;
+  free(w);
+  free(keys);
+  free(cnt);
+  return 1.0 / k;
 }

@@ -137,7 +137,7 @@ void orc_init_beta_univariate(const double *Y, int64_t G, int64_t n, int64_t ld,
                               double *beta);
 
 /* ---------------- pipeline: R/main.R:375-523 cpu_fit ---------------- */
-enum { ORC_OD_NONE = 0, ORC_OD_MOM = 1, ORC_OD_MLE = 2 };
+enum { ORC_OD_NONE = 0, ORC_OD_MOM = 1, ORC_OD_MLE = 2, ORC_OD_NEW = 3 };
 enum { ORC_INIT_QR = 0, ORC_INIT_ROUGH = 1, ORC_INIT_UNIVARIATE = 2 };
 
 typedef struct {
@@ -183,6 +183,18 @@ void orc_test_de(const double *beta, const double *bread, const double *sandwich
                  double *adj_pval, double *lfc);
 /* stats::p.adjust(p, "BH") with NA (NaN) entries left NaN */
 void orc_p_adjust_bh(const double *p, int64_t G, double *adj);
+
+/* ---------------- "new" / "I" overdispersion ---------------- */
+/* fit_overdispersion_cppp (src/new_overdispersion.cpp:195-481) for one
+ * gene; err: 1 C1 <= n, 2 mode / quadrature failure, 3 f not finite at the
+ * start (where the reference stops with an error); evals: F_k evaluations */
+double orc_fit_overdispersion_new(const double *y, const double *X, const double *beta,
+                                  const double *off, int64_t n, int p, double kappa,
+                                  int max_iter, double eps_theta, int *iters, int *converged,
+                                  int *err, int64_t *evals);
+/* H_log_gh_pmhalf (src/new_overdispersion.cpp:141-158), gh order 16 */
+int orc_h_log_gh_pmhalf(double s, double t, double tol, double *logH_p, double *logH_m);
+void orc_gh16(double *x, double *w);
 
 /* ---------------- R/size_factors.R:37-127 calculate_sf ---------------- */
 /* method 0 normed_sum, 1 psinorm; returns 1 if a column sums to 0
