@@ -925,11 +925,13 @@ constexpr int kSplitTiles = 16;
 constexpr int kSplitSlices = 64;
 
 dv::TileDesign round_shape(const dv::TileDesign& td, int nA, int tiles_per_wg,
-                           size_t part_cap) {
+                           size_t part_cap, int width = 0) {
   const int64_t ntile = (nA + 15) / 16;
   const int64_t groups = (ntile + tiles_per_wg - 1) / tiles_per_wg;
   if (ntile == 0 || groups * td.nchunk >= kRoundWorkgroups) return td;
-  const int64_t cap = (int64_t)(part_cap / ((size_t)ntile * 16 * td.ncp));
+  // doubles per gene and chunk in the partial buffer (the Gram columns)
+  const int64_t w = width > 0 ? width : td.ncp;
+  const int64_t cap = (int64_t)(part_cap / ((size_t)ntile * 16 * w));
   const int64_t want = std::min({(kRoundWorkgroups + groups - 1) / groups, kMaxRoundChunks, cap});
   if (want <= td.nchunk) return td;
   dv::TileDesign tr = td;
@@ -1031,6 +1033,60 @@ int irls_rounds(devil_ctx* ctx, const TileHost& tl, bool mixed, dv::Counts c, in
     if (count[j] == 0) ctx->launches[rec[j]].kind = -1;
   }
   collect_launches(ctx);
+  return DEVIL_OK;
+}
+
+// Lock-step rounds of an overdispersion driver (MLE, "new") over the active
+// list, with the count reads of irls_rounds: the host never waits for the
+// round it has just queued, so the short rounds of the last genes run back to
+// back.  accum(td, bound, k, cur, ncur) queues round k's accumulate pass
+// (timed as `kind`), update(td, bound, k, cur, ncur, nxt, nnxt) its update,
+// which appends the genes still running to nxt.  Lists only shrink.  *left =
+// genes still active after max_rounds rounds.
+template <class Accum, class Update>
+int driver_rounds(devil_ctx* ctx, const dv::TileDesign& tfull, size_t part_cap, int width,
+                  int kind, double row_bytes, int32_t* cur, int32_t* nxt, int32_t* ncur,
+                  int32_t* nnxt, int nA, int64_t max_rounds, Accum&& accum, Update&& update,
+                  int* left) {
+  hipStream_t s = ctx->stream;
+  int rc;
+  if ((rc = round_sync_setup(ctx))) return rc;
+  volatile int32_t* slot = ctx->round_counts;
+  std::vector<int> count{nA};  // count[k]: genes of round k's list
+  std::vector<long> rec;       // round k's launch record (-1: none)
+  int64_t k = 0;
+  for (; k < max_rounds; ++k) {
+    if (k >= 2) {
+      DV_HIP(hipEventSynchronize(ctx->round_ev[k & 1]));
+      count.push_back((int)slot[k & 1]);  // round k - 2's update: round k - 1's list
+    }
+    const int bound = k == 0 ? nA : count[k - 1];
+    if (bound <= 0) break;
+    const dv::TileDesign td = round_shape(tfull, bound, 4, part_cap, width);
+    const size_t nrec = ctx->launches.size();
+    {
+      LaunchTimer lt(ctx, kind, 0.0);
+      if ((rc = accum(td, bound, k, cur, ncur))) return rc;
+    }
+    rec.push_back(ctx->launches.size() > nrec ? (long)nrec : -1);
+    DV_HIP(dv::launch_fill_i32(s, nnxt, 1, 0));
+    if ((rc = update(td, bound, k, cur, ncur, nxt, nnxt))) return rc;
+    DV_HIP(hipMemcpyAsync(ctx->round_counts + (k & 1), nnxt, sizeof(int32_t),
+                          hipMemcpyDeviceToHost, s));
+    DV_HIP(hipEventRecord(ctx->round_ev[k & 1], s));
+    std::swap(cur, nxt);
+    std::swap(ncur, nnxt);
+  }
+  DV_HIP(hipStreamSynchronize(s));
+  for (int64_t j = std::max<int64_t>(0, k - 2); j < k; ++j)
+    if ((int64_t)count.size() == j + 1) count.push_back((int)slot[j & 1]);
+  for (size_t j = 0; j < rec.size(); ++j) {
+    if (rec[j] < 0) continue;
+    ctx->launches[rec[j]].bytes = row_bytes * (double)count[j];
+    if (count[j] == 0) ctx->launches[rec[j]].kind = -1;
+  }
+  collect_launches(ctx);
+  *left = (int64_t)count.size() > k ? count[k] : 0;
   return DEVIL_OK;
 }
 
@@ -1167,39 +1223,22 @@ int run_mle_tiled(devil_ctx* ctx, const dv::TileDesign& td, dv::Counts c, int64_
   // passes: the far-left check, then at most max_iter + 1 Newton passes (root
   // driver) or eval.max = 200 objective evaluations (nlminb driver)
   const int max_rounds = std::max(max_iter, 200) + 2;
-  for (int round = 0; round < max_rounds && nA > 0; ++round) {
-    // rounds of few genes get shorter chunks (more workgroups), within the
-    // partial buffer
-    dv::TileDesign tr = td;
-    {
-      const int64_t nt = (nA + 15) / 16, groups = (nt + 3) / 4;
-      const int64_t cap = (int64_t)(part_cap / ((size_t)nt * 16 * W));
-      const int64_t want = std::min({(kRoundWorkgroups + groups - 1) / groups, kMaxRoundChunks,
-                                     cap});
-      if (groups * td.nchunk < kRoundWorkgroups && want > td.nchunk) {
-        dv::TileDesign t2 = td;
-        set_chunks(t2, std::max(kMinRoundBlocks, (td.nblk + want - 1) / want));
-        if (t2.nchunk > td.nchunk && t2.nchunk <= cap) tr = t2;
-      }
-    }
-    {
-      LaunchTimer lt(ctx, DEVIL_KERNEL_MLE, row_bytes * (double)nA);
-      DV_HIP(dv::launch_mle_accum(s, tr, c, p, cur, ncur, nA, beta, ms.x, ms.tab_n,
-                                  driver == dv::kMleDriverNlminb && round > 0,
-                                  part.as<double>()));
-    }
-    DV_HIP(dv::launch_fill_i32(s, nnxt, 1, 0));
-    DV_HIP(dv::launch_mle_update(s, tr, p, cur, ncur, nA, part.as<double>(), ms, n, max_iter,
-                                 tolerance, nxt, nnxt));
-    int32_t h = 0;
-    DV_HIP(hipMemcpyAsync(&h, nnxt, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    DV_HIP(hipStreamSynchronize(s));
-    nA = h;
-    std::swap(cur, nxt);
-    std::swap(ncur, nnxt);
-  }
-  collect_launches(ctx);
-  return DEVIL_OK;
+  int left = 0;
+  return driver_rounds(
+      ctx, td, part_cap, W, DEVIL_KERNEL_MLE, row_bytes, cur, nxt, ncur, nnxt, nA, max_rounds,
+      [&](const dv::TileDesign& tr, int bound, int64_t k, int32_t* a, int32_t* na) -> int {
+        DV_HIP(dv::launch_mle_accum(s, tr, c, p, a, na, bound, beta, ms.x, ms.tab_n,
+                                    driver == dv::kMleDriverNlminb && k > 0,
+                                    part.as<double>()));
+        return DEVIL_OK;
+      },
+      [&](const dv::TileDesign& tr, int bound, int64_t, int32_t* a, int32_t* na, int32_t* b,
+          int32_t* nb) -> int {
+        DV_HIP(dv::launch_mle_update(s, tr, p, a, na, bound, part.as<double>(), ms, n, max_iter,
+                                     tolerance, b, nb));
+        return DEVIL_OK;
+      },
+      &left);
 }
 
 // "new" / "I" overdispersion over genes [0, G): the setup pass (count table,
@@ -1265,37 +1304,24 @@ int run_newod_tiled(devil_ctx* ctx, const dv::TileDesign& td, dv::Counts c, int6
   // its loops (Newton start, Aitken steps with backtracking, the bracket
   // search and at most 60 Brent steps)
   const int64_t max_rounds = 16 + (int64_t)std::max(max_iter, 1) * 40 + 100;
-  for (int64_t round = 0; round < max_rounds && nA > 0; ++round) {
-    dv::TileDesign tr = td;
-    {
-      const int64_t nt = (nA + 15) / 16, groups = (nt + 3) / 4;
-      const int64_t cap = (int64_t)(part_cap / ((size_t)nt * 16 * 4));
-      const int64_t want = std::min({(kRoundWorkgroups + groups - 1) / groups, kMaxRoundChunks,
-                                     cap});
-      if (groups * td.nchunk < kRoundWorkgroups && want > td.nchunk) {
-        dv::TileDesign t2 = td;
-        set_chunks(t2, std::max(kMinRoundBlocks, (td.nblk + want - 1) / want));
-        if (t2.nchunk > td.nchunk && t2.nchunk <= cap) tr = t2;
-      }
-    }
-    {
-      LaunchTimer lt(ctx, DEVIL_KERNEL_MLE, row_bytes * (double)nA);
-      DV_HIP(dv::launch_newod_accum(s, tr, c, p, cur, ncur, nA, beta, kreq.as<double>(),
-                                    ms.tab_n, part.as<double>()));
-    }
-    DV_HIP(dv::launch_fill_i32(s, nnxt, 1, 0));
-    DV_HIP(dv::launch_newod_update(s, tr.nchunk, cur, ncur, nA, part.as<double>(), ms,
-                                   st.as<dv::NewodState>(), kreq.as<double>(), n, max_iter,
-                                   tolerance, nxt, nnxt));
-    int32_t h = 0;
-    DV_HIP(hipMemcpyAsync(&h, nnxt, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    DV_HIP(hipStreamSynchronize(s));
-    nA = h;
-    std::swap(cur, nxt);
-    std::swap(ncur, nnxt);
-  }
-  collect_launches(ctx);
-  if (nA > 0) return fail(ctx, DEVIL_E_HIP, "new overdispersion: evaluation rounds exhausted");
+  int left = 0;
+  int rc = driver_rounds(
+      ctx, td, part_cap, 4, DEVIL_KERNEL_MLE, row_bytes, cur, nxt, ncur, nnxt, nA, max_rounds,
+      [&](const dv::TileDesign& tr, int bound, int64_t, int32_t* a, int32_t* na) -> int {
+        DV_HIP(dv::launch_newod_accum(s, tr, c, p, a, na, bound, beta, kreq.as<double>(),
+                                      ms.tab_n, part.as<double>()));
+        return DEVIL_OK;
+      },
+      [&](const dv::TileDesign& tr, int bound, int64_t, int32_t* a, int32_t* na, int32_t* b,
+          int32_t* nb) -> int {
+        DV_HIP(dv::launch_newod_update(s, tr.nchunk, a, na, bound, part.as<double>(), ms,
+                                       st.as<dv::NewodState>(), kreq.as<double>(), n, max_iter,
+                                       tolerance, b, nb));
+        return DEVIL_OK;
+      },
+      &left);
+  if (rc) return rc;
+  if (left > 0) return fail(ctx, DEVIL_E_HIP, "new overdispersion: evaluation rounds exhausted");
   return DEVIL_OK;
 }
 

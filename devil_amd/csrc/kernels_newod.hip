@@ -134,7 +134,11 @@ __global__ __launch_bounds__(256) void k_newod_accum(TileDesign td, Counts c,
   });
 }
 
-__global__ __launch_bounds__(64) void k_newod_update(int nchunk, const int32_t* __restrict__ active,
+// threads of the update: the count-table digamma sum of one gene spreads over
+// four waves (a lone gene's long tail of evaluations is latency-bound)
+constexpr int kNodUpdThreads = 256;
+
+__global__ __launch_bounds__(kNodUpdThreads) void k_newod_update(int nchunk, const int32_t* __restrict__ active,
                                                      const int32_t* __restrict__ nactive,
                                                      const double* __restrict__ part,
                                                      const int32_t* __restrict__ tab_n,
@@ -151,34 +155,60 @@ __global__ __launch_bounds__(64) void k_newod_update(int nchunk, const int32_t* 
   const int nA = *nactive;
   const int i = blockIdx.x;
   if (i >= nA) return;
-  const int lane = threadIdx.x;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int g = active[i];
   const int tile = i >> 4, s = i & 15;
+  __shared__ double red[2][kNodUpdThreads / 64];
+  // the chunks' partials (wave 0): 16 lanes per scalar, then a reduction
+  // within each group of 16
+  static_assert(kNodScal * 16 == 64, "one 16-lane group per scalar");
   double a = 0.0;
-  if (lane < kNodScal) {
-    const double* src = part + ((int64_t)tile * nchunk * 16 + s) * kNodScal + lane;
-    for (int ch = 0; ch < nchunk; ++ch) a += src[(int64_t)ch * 16 * kNodScal];
+  if (wave == 0) {
+    const int q = lane >> 4, l = lane & 15;
+    const double* src = part + ((int64_t)tile * nchunk * 16 + s) * kNodScal + q;
+    for (int ch = l; ch < nchunk; ch += 16) a += src[(int64_t)ch * 16 * kNodScal];
+#pragma unroll
+    for (int o = 8; o >= 1; o >>= 1) a += __shfl_xor(a, o, 64);
   }
-  const double sa = __shfl(a, 0, 64), sb = __shfl(a, 1, 64), se = __shfl(a, 2, 64);
-  double sd = __shfl(a, 3, 64);
+  const double sa = __shfl(a, 0, 64), sb = __shfl(a, 16, 64), se = __shfl(a, 32, 64);
+  double sd = __shfl(a, 48, 64);
   const double k = kreq[g];
   // digamma(k + y) over the count table; cells with y = 0 give digamma(k)
   const int tn = tab_n[g];
-  if (tn >= 0) {
-    double t = 0.0, nz = 0.0;
-    for (int j = lane; j < tn; j += 64) {
+  if (tn >= 0) {  // uniform over the block
+    double tp = 0.0, nz = 0.0;
+    for (int j = t; j < tn; j += kNodUpdThreads) {
       const double f = (double)tab_f[(int64_t)g * kMleTabCap + j];
-      t += f * digamma(k + (double)tab_u[(int64_t)g * kMleTabCap + j]);
+      tp += f * digamma(k + (double)tab_u[(int64_t)g * kMleTabCap + j]);
       nz += f;
     }
-    t = wave_sum(t);
+    tp = wave_sum(tp);
     nz = wave_sum(nz);
-    sd = t + ((double)n - nz) * digamma(k);
+    if (lane == 0) {
+      red[0][wave] = tp;
+      red[1][wave] = nz;
+    }
+    __syncthreads();
+    if (wave == 0) {
+      tp = 0.0;
+      nz = 0.0;
+#pragma unroll
+      for (int w = 0; w < kNodUpdThreads / 64; ++w) {
+        tp += red[0][w];
+        nz += red[1][w];
+      }
+      sd = tp + ((double)n - nz) * digamma(k);
+    }
   }
+  if (wave != 0) return;
+  // wave 0: C1 and F_k (the same values in every lane), then lane 0 steps the
+  // driver
+  const double C1 = se - (sd - sa) + sb;
+  int ferr = 0;
+  const double F = nod_F_from_C1_wave((double)n, C1, eps, ferr);
   if (lane != 0) return;
   NewodState sg = st[g];
-  const double C1 = se - (sd - sa) + sb;
-  const double F = nod_F_from_C1((double)n, C1, eps, sg.err);
+  if (ferr) sg.err = ferr;
   const int done = newod_advance(sg, F, max_iter, eps);
   st[g] = sg;
   if (done) {
@@ -233,8 +263,8 @@ hipError_t launch_newod_update(hipStream_t s, int nchunk, const int32_t* active,
                                const MleState& ms, NewodState* st, double* kreq, int64_t n,
                                int max_iter, double eps, int32_t* next, int32_t* nnext) {
   if (nA_bound <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_newod_update, dim3((unsigned)nA_bound), dim3(64), 0, s, nchunk, active,
-                     nactive, part, ms.tab_n, ms.tab_u, ms.tab_f, st, kreq, n, max_iter, eps,
+  hipLaunchKernelGGL(k_newod_update, dim3((unsigned)nA_bound), dim3(kNodUpdThreads), 0, s,
+                     nchunk, active, nactive, part, ms.tab_n, ms.tab_u, ms.tab_f, st, kreq, n, max_iter, eps,
                      ms.theta, ms.iters, ms.status, next, nnext);
   return hipGetLastError();
 }

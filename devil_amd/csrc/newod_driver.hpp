@@ -66,41 +66,44 @@ __device__ inline bool nod_find_mode(double p, double s, double t, double tol, d
   return true;
 }
 
-// log_integral_at (:95-123), two-pass log-sum-exp
-__device__ inline bool nod_log_integral(double p, double s, double t, double mu0, double sigma0,
-                                        double h_mu0, double& out) {
-  const double scale = sigma0 * sqrt(2.0);
-  double max_lt = -INFINITY;
-  double lt[16];
-#pragma unroll
-  for (int i = 0; i < 16; ++i) {
-    const double u = kGh16X[i], xx = mu0 + scale * u;
-    lt[i] = -INFINITY;
-    if (xx <= 0.0 || !isfinite(xx)) continue;
-    lt[i] = log(kGh16W[i]) + ((p * log(xx) + nod_g(s, t, xx)) - h_mu0 + u * u);
-    if (lt[i] > max_lt) max_lt = lt[i];
-  }
-  double acc = 0.0;
-#pragma unroll
-  for (int i = 0; i < 16; ++i)
-    if (lt[i] > -INFINITY) acc += exp(lt[i] - max_lt);
-  if (!(acc > 0.0)) return false;
-  out = h_mu0 + log(scale) + (max_lt + log(acc));
-  return true;
-}
-
-// F_k from C1 (H_log_gh_pmhalf with s = n, t = C1); NaN where the reference stops
-__device__ inline double nod_F_from_C1(double n, double C1, double tol, int& err) {
+// F_k from C1 (H_log_gh_pmhalf with s = n, t = C1; NaN where the reference
+// stops), by one wavefront: C1 is the same in every lane and every lane gets
+// F.  Both modes are found on every lane; then log_integral_at (:95-123)
+// takes one Gauss-Hermite node per lane (lanes 0-15 the p = +1/2 integral,
+// 16-31 the p = -1/2 one), with the two-pass log-sum-exp reduced within each
+// 16-lane group.
+__device__ inline double nod_F_from_C1_wave(double n, double C1, double tol, int& err) {
   if (!(C1 > n)) {
     err = 1;
     return __builtin_nan("");
   }
-  double m1, s1, h1, m2, s2, h2, lp, lm;
+  double m1, s1, h1, m2, s2, h2;
   const double x0 = fmax(1e-3, (0.5 + n) / (C1 - n));
   if (!nod_find_mode(0.5, n, C1, tol, x0, m1, s1, h1) ||
-      !nod_find_mode(-0.5, n, C1, tol, m1, m2, s2, h2) ||
-      !nod_log_integral(0.5, n, C1, m1, s1, h1, lp) ||
-      !nod_log_integral(-0.5, n, C1, m2, s2, h2, lm)) {
+      !nod_find_mode(-0.5, n, C1, tol, m1, m2, s2, h2)) {
+    err = 2;
+    return __builtin_nan("");
+  }
+  const int lane = threadIdx.x & 63;
+  const int node = lane & 15;
+  const bool second = (lane & 16) != 0;
+  const double p = second ? -0.5 : 0.5;
+  const double mu0 = second ? m2 : m1, sigma0 = second ? s2 : s1, h_mu0 = second ? h2 : h1;
+  const double scale = sigma0 * sqrt(2.0);
+  const double u = kGh16X[node], xx = mu0 + scale * u;
+  double lt = -INFINITY;
+  if (lane < 32 && xx > 0.0 && isfinite(xx))
+    lt = log(kGh16W[node]) + ((p * log(xx) + nod_g(n, C1, xx)) - h_mu0 + u * u);
+  double mx = lt;
+#pragma unroll
+  for (int o = 8; o >= 1; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+  double acc = lt > -INFINITY ? exp(lt - mx) : 0.0;
+#pragma unroll
+  for (int o = 8; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  const double li = h_mu0 + log(scale) + (mx + log(acc));
+  const double acc1 = __shfl(acc, 0, 64), acc2 = __shfl(acc, 16, 64);
+  const double lp = __shfl(li, 0, 64), lm = __shfl(li, 16, 64);
+  if (!(acc1 > 0.0) || !(acc2 > 0.0)) {
     err = 2;
     return __builtin_nan("");
   }
