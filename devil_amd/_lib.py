@@ -84,6 +84,7 @@ def lib():
                                     _vp]),
         "devil_test_de_noalloc": (C.c_int, [_vp, _vp, _vp, _vp, i64, i32, i64, _vp, d, _vp,
                                             _vp, _vp, _vp]),
+        "devil_p_adjust": (C.c_int, [_vp, _vp, i64, i32, _vp]),
         "devil_sandwich_hc0": (C.c_int, [_vp, _vp, i32, i64, i64, i64, _vp, i32, _vp, _vp,
                                          _vp, _vp, _vp, _vp]),
         "devil_size_factors": (C.c_int, [_vp, _vp, i32, i64, i64, i64, i32, _vp]),
@@ -107,7 +108,7 @@ def lib():
 EXPORTED = ["devil_create", "devil_destroy", "devil_strerror", "devil_last_error",
             "devil_version", "devil_fit", "devil_estimate_dispersion", "devil_init_beta",
             "devil_beta_fit", "devil_mom_dispersion", "devil_fit_dispersion_mle",
-            "devil_bread_meat", "devil_test_de", "devil_test_de_noalloc",
+            "devil_bread_meat", "devil_test_de", "devil_test_de_noalloc", "devil_p_adjust",
             "devil_sandwich_hc0", "devil_size_factors", "devil_group_cells",
             "devil_csc_to_dense",
             "devil_set_timing", "devil_stage_times",
@@ -118,6 +119,9 @@ KERNELS = ["irls_fp64", "irls_mixed", "mle"]
 TUNE_MIXED_KERNEL, TUNE_CHUNK_CELLS, TUNE_CONCURRENCY, TUNE_MLE_DRIVER, TUNE_PAD_COPY = 0, 1, 2, 3, 4
 MLE_DRIVERS = {"nlminb": 0, "root": 1}
 SF_METHODS = {"normed_sum": 0, "psinorm": 1, "edgeR": 2}
+# stats::p.adjust.methods
+P_ADJUST = {"holm": 2, "hochberg": 3, "hommel": 4, "bonferroni": 5, "BH": 0, "BY": 1,
+            "fdr": 0, "none": 6}
 MIXED_KERNELS = {"f32": 0, "bf16": 1, "auto": 2, "bf16_ws": 3, "bf16_ws2": 4}
 
 

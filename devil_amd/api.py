@@ -20,6 +20,15 @@ _OD_MODES = {"MOM": L.OD_MOM, "MLE": L.OD_MLE, "old": L.OD_MLE, "new": L.OD_NEW,
              "I": L.OD_NEW}
 
 
+def _inputs_ready(device: int) -> None:
+    """The library's stream does not wait for other streams: finish the work
+    torch has queued on its current stream (device inputs just made) before a
+    library call reads those tensors.  Outputs are complete when a call
+    returns."""
+    import torch
+    torch.cuda.current_stream(device).synchronize()
+
+
 def _is_torch(x) -> bool:
     return type(x).__module__.startswith("torch")
 
@@ -75,6 +84,7 @@ def densify(x, device: int = 0):
     rowind = torch.as_tensor(np.asarray(csc.indices, dtype=np.int32), device=dev)
     vals = torch.as_tensor(data, device=dev)
     ctx = L.context(device)
+    _inputs_ready(ctx.device)
     ctx.check(L.lib().devil_csc_to_dense(ctx.h, colptr.data_ptr(), rowind.data_ptr(),
                                          vals.data_ptr(), G, n, out_dtype, out.data_ptr(), n))
     return out
@@ -95,6 +105,7 @@ def calculate_sf(Y, method: str = "normed_sum", verbose: bool = False, device: i
     G, n = (int(v) for v in Y.shape)
     sf = torch.empty(n, dtype=torch.float64, device=Y.device)
     ctx = L.context(Y.device.index or 0)
+    _inputs_ready(ctx.device)
     ctx.check(L.lib().devil_size_factors(ctx.h, Y.data_ptr(), _counts_dtype(Y.dtype), G, n,
                                          Y.stride(0), L.SF_METHODS[method], sf.data_ptr()))
     return sf
@@ -128,6 +139,7 @@ def group_data(input_matrix, design_matrix, clusters, device: int = 0):
     out = torch.empty((G, n), dtype=Y.dtype, device=dev)
     ordd = torch.as_tensor(order, device=dev)
     ctx = L.context(dev.index or 0)
+    _inputs_ready(ctx.device)
     ctx.check(L.lib().devil_group_cells(ctx.h, Y.data_ptr(), _counts_dtype(Y.dtype), G, n,
                                         Y.stride(0), ordd.data_ptr(), out.data_ptr(), n))
     if _is_torch(design_matrix):
@@ -167,9 +179,11 @@ def compute_sandwich(design_matrix, y, beta, overdispersion, size_factors, clust
     args = (ctx.h, Y.data_ptr(), _counts_dtype(Y.dtype), G, n, Y.stride(0), Xd.data_ptr(), p,
             bcol.data_ptr(), th.data_ptr(), off.data_ptr())
     if clusters is None:
+        _inputs_ready(ctx.device)
         ctx.check(lib.devil_sandwich_hc0(*args, bread.data_ptr(), sw.data_ptr(), None))
     else:
         ends = torch.as_tensor(cluster_blocks(clusters, n), dtype=torch.int32, device=dev)
+        _inputs_ready(ctx.device)
         ctx.check(lib.devil_bread_meat(*args, ends.data_ptr(), int(ends.numel()),
                                        bread.data_ptr(), sw.data_ptr(), None))
     return sw.view(G, p, p).transpose(1, 2)
@@ -367,6 +381,7 @@ def fit_devil(x, design_matrix, clusters=None, overdispersion: Any = "MOM",
                init_overdispersion=init_overdispersion, init_beta_rough=init_beta_rough,
                offset=offset, size_factors=size_factors, max_iter=max_iter,
                tolerance=tolerance, precision=precision, init_mode=init_mode, device=device)
+    _inputs_ready(ctx.device)
     ctx.check(L.lib().devil_fit(ctx.h, C.byref(job.prob), C.byref(job.outs), 1))
     return job.result()
 
@@ -385,6 +400,7 @@ def fit_devil_many(problems, device: int = 0, **common):
         return []
     probs = (L.Problem * len(jobs))(*[j.prob for j in jobs])
     outs = (L.Outputs * len(jobs))(*[j.outs for j in jobs])
+    _inputs_ready(ctx.device)
     ctx.check(L.lib().devil_fit(ctx.h, probs, outs, len(jobs)))
     return [j.result() for j in jobs]
 
@@ -398,8 +414,8 @@ def test_de(devil_fit, contrast, clusters=None, pval_adjust_method: str = "BH",
         raise ValueError("The fit object was created with an older version of devil and is "
                          "no longer supported. Please re-fit your model with the current "
                          "version of fit_devil().")
-    if pval_adjust_method not in ("BH", "fdr"):
-        raise NotImplementedError("only the BH adjustment is implemented")
+    if pval_adjust_method not in L.P_ADJUST:
+        raise ValueError(f"'arg' should be one of {', '.join(map(repr, L.P_ADJUST))}")
     ctx = L.context(device)
     dev = torch.device("cuda", device)
 
@@ -421,8 +437,13 @@ def test_de(devil_fit, contrast, clusters=None, pval_adjust_method: str = "BH",
         raise ValueError("contrast length must match the number of coefficients")
     pval, logp, adj, lfc = (torch.empty(G, dtype=torch.float64, device=dev) for _ in range(4))
     ptr = lambda a: None if a is None else a.data_ptr()
+    _inputs_ready(ctx.device)
     ctx.check(L.lib().devil_test_de(ctx.h, ptr(bcol), ptr(bread), ptr(swd), G, p, n, ptr(c),
                                     float(max_lfc), ptr(pval), ptr(logp), ptr(adj), ptr(lfc)))
+    method = L.P_ADJUST[pval_adjust_method]
+    if method != L.P_ADJUST["BH"]:  # devil_test_de applies BH
+        _inputs_ready(ctx.device)
+        ctx.check(L.lib().devil_p_adjust(ctx.h, ptr(pval), G, method, ptr(adj)))
     name = devil_fit.get("gene_names")
     return {"name": name, "pval": pval.cpu().numpy(), "adj_pval": adj.cpu().numpy(),
             "lfc": lfc.cpu().numpy(), "log_pval": logp.cpu().numpy()}

@@ -2293,6 +2293,36 @@ int devil_test_de(devil_ctx* ctx, const double* beta, const double* bread,
   return DEVIL_OK;
 }
 
+int devil_p_adjust(devil_ctx* ctx, const double* pval, int64_t G, int32_t method,
+                   double* adj) {
+  int rc;
+  if ((rc = set_device(ctx))) return rc;
+  if (G < 0 || (G > 0 && (!pval || !adj))) return fail(ctx, DEVIL_E_ARG, "NULL pointer or G < 0");
+  if (method < DEVIL_PADJ_BH || method > DEVIL_PADJ_NONE)
+    return fail(ctx, DEVIL_E_ARG, "unknown p-value adjustment method " + std::to_string(method));
+  if (G > INT32_MAX) return fail(ctx, DEVIL_E_ARG, "G above 2^31 - 1");
+  if (G == 0) return DEVIL_OK;
+  hipStream_t s = ctx->stream;
+  DevBuf bh;
+  const size_t bh_bytes = dv::bh_scratch_bytes(G);
+  DV_ALLOC(bh, bh_bytes);
+  dv::PadjBufs b{};
+  DV_HIP(dv::launch_padj_sort(s, pval, G, bh.p, bh_bytes, b));
+  int32_t m = 0;
+  DV_HIP(hipMemcpyAsync(&m, b.m, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  DV_HIP(hipStreamSynchronize(s));
+  // BY's q = sum(1 / (1:n)), accumulated in long double as R's sum does
+  double q = 1.0;
+  if (method == DEVIL_PADJ_BY) {
+    long double acc = 0.0L;
+    for (int32_t k = 1; k <= m; ++k) acc += (long double)(1.0 / (double)k);
+    q = (double)acc;
+  }
+  DV_HIP(dv::launch_padj_finish(s, pval, G, m, method, q, b, adj));
+  DV_HIP(hipStreamSynchronize(s));
+  return DEVIL_OK;
+}
+
 // devil_test_de with its scratch (the gene-major beta and the BH sort
 // buffers) in the device's fixed scratch array: no allocation, no events.
 int devil_test_de_noalloc(devil_ctx* ctx, const double* beta, const double* bread,

@@ -256,6 +256,25 @@ hipError_t launch_transpose_gp(hipStream_t s, const double* src, int64_t G, int 
                                int to_gene_major, double* dst);
 
 hipError_t launch_fill_i32(hipStream_t s, int32_t* p, int64_t count, int32_t v);
+// stats::p.adjust methods (DEVIL_PADJ_* values) on the device
+enum { kPadjBH = 0, kPadjBY = 1, kPadjHolm = 2, kPadjHochberg = 3, kPadjHommel = 4,
+       kPadjBonferroni = 5, kPadjNone = 6 };
+struct PadjBufs {
+  double* ks;       // ascending p (NaN last)
+  int32_t* is;      // their genes
+  int32_t* m;       // non-NaN count (device)
+  double* t;        // terms / hommel q1
+  double* scan;     // scanned terms / hommel pa0
+  void* tmp;
+  size_t tmp_bytes;
+};
+// sort into b (scratch of bh_scratch_bytes(G)), then, with m read back,
+// finish one method into adj
+hipError_t launch_padj_sort(hipStream_t s, const double* pval, int64_t G, void* scratch,
+                            size_t scratch_bytes, PadjBufs& b);
+hipError_t launch_padj_finish(hipStream_t s, const double* pval, int64_t G, int m, int method,
+                              double q, const PadjBufs& b, double* adj);
+
 
 // count-matrix preparation (kernels_prep.hip)
 hipError_t launch_size_factors(hipStream_t s, Counts c, int64_t G, int64_t n, int method,

@@ -160,6 +160,151 @@ hipError_t launch_bh(hipStream_t s, const double* pval, int64_t G, double* adj, 
   return hipGetLastError();
 }
 
+// stats::p.adjust for the other methods test_de can pass (R/test.R:127).
+// The p-values are sorted ascending once (NaN last, as in launch_bh); with m
+// the number of non-NaN ones and k the 0-based ascending rank, in R's order
+// of operations:
+//   holm        (m - k) p(k), running maximum from k = 0, capped at 1
+//   hochberg    (m - k) p(k), minimum over k' >= k, capped at 1
+//   BY          q m / (k + 1) p(k), minimum over k' >= k, capped at 1
+//   bonferroni  m p, capped at 1
+//   hommel      pa(k) = max(min_k' m p(k') / (k' + 1),
+//                           max_{j=2..m-1} min(j p(min(k, m - j)), q1(j))),
+//               q1(j) = min_{i=2..j} j p(m - j + i - 1) / i, then max(pa, p)
+// (the loop of the R function unrolled: its q is rewritten in full for each
+// j, so pa is the maximum of the per-j vectors).
+__global__ void k_padj_terms(const double* __restrict__ ks, int m, int method, double q,
+                             double* __restrict__ t) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m) return;
+  const double pk = ks[k];
+  double v;
+  if (method == kPadjHolm) {
+    t[k] = (double)(m - k) * pk;  // forward order, max-scanned
+    return;
+  }
+  if (method == kPadjHochberg) v = (double)(m - k) * pk;
+  else v = q * (double)m / (double)(k + 1) * pk;  // BY (q = 1 for BH)
+  t[m - 1 - k] = v;  // reversed, so a min-scan is the suffix minimum
+}
+
+__global__ void k_padj_scatter(const double* __restrict__ p, const double* __restrict__ ks,
+                               const int32_t* __restrict__ is, const double* __restrict__ scan,
+                               int64_t G, int m, int method, double* __restrict__ adj) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= G) return;
+  const int32_t i = is[k];
+  if (k >= m || m <= 1 || method == kPadjNone) {
+    adj[i] = p[i];
+    return;
+  }
+  double v;
+  if (method == kPadjBonferroni) v = (double)m * ks[k];
+  else if (method == kPadjHolm) v = scan[k];
+  else v = scan[m - 1 - k];
+  adj[i] = fmin(v, 1.0);
+}
+
+// hommel: min_k m p(k) / (k + 1) by one workgroup, into *pa0
+__global__ __launch_bounds__(1024) void k_hommel_pa0(const double* __restrict__ ks, int m,
+                                                     double* __restrict__ pa0) {
+  __shared__ double red[16];
+  double v = __builtin_huge_val();
+  for (int k = threadIdx.x; k < m; k += 1024) v = fmin(v, (double)m * ks[k] / (double)(k + 1));
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 16; ++w) v = fmin(v, red[w]);
+    *pa0 = v;
+  }
+}
+
+// hommel: q1[j] for j = 2 .. m - 1
+__global__ void k_hommel_q1(const double* __restrict__ ks, int m, double* __restrict__ q1) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x + 2;
+  if (j >= m) return;
+  const double dj = (double)j;
+  double v = __builtin_huge_val();
+  for (int i = 2; i <= j; ++i) v = fmin(v, dj * ks[m - j + i - 1] / (double)i);
+  q1[j] = v;
+}
+
+// hommel: adjusted p of ascending rank k, scattered to the gene
+__global__ void k_hommel_pa(const double* __restrict__ ks, const int32_t* __restrict__ is, int m,
+                            const double* __restrict__ q1, const double* __restrict__ pa0,
+                            double* __restrict__ adj) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m) return;
+  double pa = *pa0;
+  for (int j = 2; j < m; ++j) pa = fmax(pa, fmin((double)j * ks[min(k, m - j)], q1[j]));
+  adj[is[k]] = fmax(pa, ks[k]);
+}
+
+namespace {
+struct MaxOp {
+  __device__ __forceinline__ double operator()(double a, double b) const { return fmax(a, b); }
+};
+}  // namespace
+
+hipError_t launch_padj_sort(hipStream_t s, const double* pval, int64_t G, void* scratch,
+                            size_t scratch_bytes, PadjBufs& b) {
+  char* c = static_cast<char*>(scratch);
+  auto take = [&](size_t bytes) {
+    char* q = c;
+    c += ((bytes + 255) / 256) * 256;
+    return q;
+  };
+  double* key = reinterpret_cast<double*>(take(8 * G));
+  b.ks = reinterpret_cast<double*>(take(8 * G));
+  b.t = reinterpret_cast<double*>(take(8 * G));
+  b.scan = reinterpret_cast<double*>(take(8 * G));
+  int32_t* idx = reinterpret_cast<int32_t*>(take(4 * G));
+  b.is = reinterpret_cast<int32_t*>(take(4 * G));
+  b.m = reinterpret_cast<int32_t*>(take(4));
+  b.tmp = take(0);
+  const size_t used = (size_t)(c - static_cast<char*>(scratch));
+  if (used > scratch_bytes) return hipErrorInvalidValue;
+  b.tmp_bytes = scratch_bytes - used;
+  const dim3 grid((unsigned)((G + 255) / 256)), blk(256);
+  hipLaunchKernelGGL(k_bh_keys, grid, blk, 0, s, pval, G, key, idx);
+  size_t tb = b.tmp_bytes;
+  hipError_t e = hipcub::DeviceRadixSort::SortPairs(b.tmp, tb, key, b.ks, idx, b.is, (int)G, 0,
+                                                    64, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_fill_i32_bh, dim3(1), dim3(1), 0, s, b.m);
+  hipLaunchKernelGGL(k_bh_count, grid, blk, 0, s, b.ks, G, b.m);
+  return hipGetLastError();
+}
+
+hipError_t launch_padj_finish(hipStream_t s, const double* pval, int64_t G, int m, int method,
+                              double q, const PadjBufs& b, double* adj) {
+  const dim3 grid((unsigned)((G + 255) / 256)), blk(256);
+  if (method == kPadjHommel && m > 2) {
+    const dim3 gm((unsigned)((m + 255) / 256));
+    hipLaunchKernelGGL(k_hommel_pa0, dim3(1), dim3(1024), 0, s, b.ks, m, b.scan);
+    hipLaunchKernelGGL(k_hommel_q1, gm, blk, 0, s, b.ks, m, b.t);
+    hipLaunchKernelGGL(k_padj_scatter, grid, blk, 0, s, pval, b.ks, b.is, b.scan, G, 0,
+                       kPadjNone, adj);  // NaN entries (and all, overwritten below)
+    hipLaunchKernelGGL(k_hommel_pa, gm, blk, 0, s, b.ks, b.is, m, b.t, b.scan, adj);
+    return hipGetLastError();
+  }
+  if (method == kPadjHommel) method = kPadjHochberg;  // m == 2, as p.adjust
+  if (m > 1 && method != kPadjNone && method != kPadjBonferroni) {
+    const dim3 gm((unsigned)((m + 255) / 256));
+    hipLaunchKernelGGL(k_padj_terms, gm, blk, 0, s, b.ks, m, method, q, b.t);
+    size_t tb = b.tmp_bytes;
+    hipError_t e = method == kPadjHolm
+                       ? hipcub::DeviceScan::InclusiveScan(b.tmp, tb, b.t, b.scan, MaxOp(), m, s)
+                       : hipcub::DeviceScan::InclusiveScan(b.tmp, tb, b.t, b.scan, MinOp(), m, s);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_padj_scatter, grid, blk, 0, s, pval, b.ks, b.is, b.scan, G, m, method,
+                     adj);
+  return hipGetLastError();
+}
+
 // G x p column-major (R layout) <-> gene-major G x p (row g holds its p values)
 __global__ void k_transpose_gp(const double* __restrict__ src, int64_t G, int p, int to_gene,
                                double* __restrict__ dst) {

@@ -232,6 +232,16 @@ int devil_test_de(devil_ctx *ctx, const double *beta, const double *bread,
                   const double *contrast, double max_lfc, double *pval,
                   double *log_pval, double *adj_pval, double *lfc);
 
+/* stats::p.adjust(pval, method) on the device, for the pval_adjust_method
+ * values test_de passes through (R/test.R:85,127; stats' own p.adjust.R).
+ * NaN p-values stay NaN and the adjustment runs over the others (n = their
+ * count); adj may alias pval.  devil_test_de itself applies "BH" (its
+ * default). */
+enum { DEVIL_PADJ_BH = 0, DEVIL_PADJ_BY = 1, DEVIL_PADJ_HOLM = 2, DEVIL_PADJ_HOCHBERG = 3,
+       DEVIL_PADJ_HOMMEL = 4, DEVIL_PADJ_BONFERRONI = 5, DEVIL_PADJ_NONE = 6 };
+int devil_p_adjust(devil_ctx *ctx, const double *pval, int64_t G, int32_t method,
+                   double *adj);
+
 /* devil_test_de without any device allocation.  No counterpart in the
  * reference: for callers that cannot allocate device memory (code under HIP
  * graph capture, tight serving loops, harnesses that forbid hipMalloc).

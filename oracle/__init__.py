@@ -95,6 +95,7 @@ def lib():
             "orc_test_de": (None, [_dp, _dp, _dp, i64, C.c_int, i64, _dp, d, _dp, _dp,
                                    _dp, _dp]),
             "orc_p_adjust_bh": (None, [_dp, i64, _dp]),
+            "orc_p_adjust": (None, [_dp, i64, i32, _dp]),
             "orc_size_factors": (C.c_int, [_dp, i64, i64, i64, C.c_int, _dp]),
             "orc_fit_overdispersion_new": (d, [_dp, _dp, _dp, _dp, i64, C.c_int, d, C.c_int, d,
                                                C.POINTER(C.c_int), C.POINTER(C.c_int),
@@ -366,6 +367,18 @@ def cpu_fit(Y, X, offset, *, clusters_ends=None, disp_init=None, init_mode=INIT_
     out["bread"] = out["bread"].transpose(0, 2, 1).copy()
     if out["sandwich"] is not None:
         out["sandwich"] = out["sandwich"].transpose(0, 2, 1).copy()
+    return out
+
+
+P_ADJUST = {"BH": 0, "fdr": 0, "BY": 1, "holm": 2, "hochberg": 3, "hommel": 4,
+            "bonferroni": 5, "none": 6}
+
+
+def p_adjust(p, method="BH"):
+    """stats::p.adjust(p, method); NaN entries stay NaN."""
+    p = _f64(p)
+    out = np.zeros_like(p)
+    lib().orc_p_adjust(_d(p), p.size, P_ADJUST[method], _d(out))
     return out
 
 

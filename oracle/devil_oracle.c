@@ -1606,22 +1606,116 @@ static int cmp_desc(const void *a, const void *b) {
 /* stats::p.adjust(method = "BH"): NaN entries are removed first, so n is the
  * number of non-NaN p-values; o = order(p, decreasing = TRUE);
  * adj[o] = pmin(1, cummin(n / (n:1) * p[o])). */
-void orc_p_adjust_bh(const double *p, int64_t G, double *adj) {
+static int cmp_asc(const void *a, const void *b) {
+  const pidx *x = (const pidx *)a, *y = (const pidx *)b;
+  if (x->p < y->p) return -1;
+  if (x->p > y->p) return 1;
+  return (x->i > y->i) - (x->i < y->i);
+}
+
+/* stats::p.adjust (R's stats package, src/library/stats/R/p.adjust.R, the
+ * same since R 2.x; not part of the reference, which calls it from
+ * R/test.R:127).  NA (NaN) entries stay NaN and n is the number of the
+ * others (p.adjust's lazy default n = length(p) is evaluated after the NAs
+ * are dropped); n <= 1 returns p; hommel with n == 2 is hochberg.  Each
+ * method in R's order of operations:
+ *   bonferroni  pmin(1, n p)
+ *   holm        ascending order, pmin(1, cummax((n + 1 - i) p[o]))
+ *   hochberg    descending order, i = n..1, pmin(1, cummin((n + 1 - i) p[o]))
+ *   BH          descending order, pmin(1, cummin(n / i p[o]))
+ *   BY          q = sum(1 / (1:n)), pmin(1, cummin(q n / i p[o]))
+ *   hommel      the loop over m = n - 1 .. 2 of the R function (below)
+ *   none        p */
+void orc_p_adjust(const double *p, int64_t G, int method, double *adj) {
   pidx *v = (pidx *)malloc(sizeof(pidx) * (size_t)(G > 0 ? G : 1));
-  int64_t m = 0;
+  int64_t n = 0;
   for (int64_t g = 0; g < G; ++g) {
-    if (isnan(p[g])) adj[g] = NAN;
-    else v[m++] = (pidx){p[g], g};
+    adj[g] = p[g];
+    if (!isnan(p[g])) v[n++] = (pidx){p[g], g};
   }
-  qsort(v, (size_t)m, sizeof(pidx), cmp_desc);
-  double run = INFINITY;
-  for (int64_t r = 0; r < m; ++r) {
-    double i = (double)(m - r);
-    double val = (double)m / i * v[r].p;
-    if (val < run) run = val;
-    adj[v[r].i] = run < 1.0 ? run : 1.0;
+  if (n <= 1 || method == ORC_PADJ_NONE) {
+    free(v);
+    return;
+  }
+  if (method == ORC_PADJ_HOMMEL && n == 2) method = ORC_PADJ_HOCHBERG;
+  const double dn = (double)n;
+  switch (method) {
+    case ORC_PADJ_BONFERRONI:
+      for (int64_t r = 0; r < n; ++r) adj[v[r].i] = fmin(1.0, dn * v[r].p);
+      break;
+    case ORC_PADJ_HOLM: {
+      qsort(v, (size_t)n, sizeof(pidx), cmp_asc);
+      double run = -INFINITY;
+      for (int64_t r = 0; r < n; ++r) {
+        const double i = (double)(r + 1);
+        const double val = (dn + 1.0 - i) * v[r].p;
+        if (val > run) run = val;
+        adj[v[r].i] = fmin(1.0, run);
+      }
+      break;
+    }
+    case ORC_PADJ_HOCHBERG:
+    case ORC_PADJ_BH:
+    case ORC_PADJ_BY: {
+      qsort(v, (size_t)n, sizeof(pidx), cmp_desc);
+      double q = 0.0;
+      if (method == ORC_PADJ_BY) {
+        /* R's sum accumulates in long double */
+        long double acc = 0.0L;
+        for (int64_t k = 1; k <= n; ++k) acc += (long double)(1.0 / (double)k);
+        q = (double)acc;
+      }
+      double run = INFINITY;
+      for (int64_t r = 0; r < n; ++r) {
+        const double i = (double)(n - r);
+        double val;
+        if (method == ORC_PADJ_HOCHBERG) val = (dn + 1.0 - i) * v[r].p;
+        else if (method == ORC_PADJ_BH) val = dn / i * v[r].p;
+        else val = q * dn / i * v[r].p;
+        if (val < run) run = val;
+        adj[v[r].i] = fmin(1.0, run);
+      }
+      break;
+    }
+    case ORC_PADJ_HOMMEL: {
+      /* p sorted ascending (1-based in the comments, as in R):
+       *   q = pa = min(n p / i)
+       *   for m = n - 1 .. 2:
+       *     q1 = min(m p[i2] / (2:m)),    i2 = n - m + 2 .. n
+       *     q[i1] = pmin(m p[i1], q1),    i1 = 1 .. n - m + 1
+       *     q[i2] = q[n - m + 1]
+       *     pa = pmax(pa, q)
+       *   pmax(pa, p) */
+      qsort(v, (size_t)n, sizeof(pidx), cmp_asc);
+      double *ps = (double *)malloc(sizeof(double) * (size_t)n);
+      double *q = (double *)malloc(sizeof(double) * (size_t)n);
+      double *pa = (double *)malloc(sizeof(double) * (size_t)n);
+      for (int64_t r = 0; r < n; ++r) ps[r] = v[r].p;
+      double mn = INFINITY;
+      for (int64_t r = 0; r < n; ++r) mn = fmin(mn, dn * ps[r] / (double)(r + 1));
+      for (int64_t r = 0; r < n; ++r) q[r] = pa[r] = mn;
+      for (int64_t m = n - 1; m >= 2; --m) {
+        const double dm = (double)m;
+        double q1 = INFINITY;
+        for (int64_t k = 2; k <= m; ++k) q1 = fmin(q1, dm * ps[n - m + k - 1] / (double)k);
+        for (int64_t i = 1; i <= n - m + 1; ++i) q[i - 1] = fmin(dm * ps[i - 1], q1);
+        for (int64_t i = n - m + 2; i <= n; ++i) q[i - 1] = q[n - m];
+        for (int64_t r = 0; r < n; ++r) pa[r] = fmax(pa[r], q[r]);
+      }
+      for (int64_t r = 0; r < n; ++r) adj[v[r].i] = fmax(pa[r], ps[r]);
+      free(ps);
+      free(q);
+      free(pa);
+      break;
+    }
+    default:
+      break;
   }
   free(v);
+}
+
+void orc_p_adjust_bh(const double *p, int64_t G, double *adj) {
+  orc_p_adjust(p, G, ORC_PADJ_BH, adj);
 }
 
 void orc_test_de(const double *beta, const double *bread, const double *sandwich,

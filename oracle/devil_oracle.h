@@ -181,7 +181,10 @@ void orc_test_de(const double *beta, const double *bread, const double *sandwich
                  int64_t G, int p, int64_t n, const double *contrast,
                  double max_lfc, double *pval, double *log_pval,
                  double *adj_pval, double *lfc);
-/* stats::p.adjust(p, "BH") with NA (NaN) entries left NaN */
+/* stats::p.adjust(p, method) with NA (NaN) entries left NaN */
+enum { ORC_PADJ_BH = 0, ORC_PADJ_BY = 1, ORC_PADJ_HOLM = 2, ORC_PADJ_HOCHBERG = 3,
+       ORC_PADJ_HOMMEL = 4, ORC_PADJ_BONFERRONI = 5, ORC_PADJ_NONE = 6 };
+void orc_p_adjust(const double *p, int64_t G, int method, double *adj);
 void orc_p_adjust_bh(const double *p, int64_t G, double *adj);
 
 /* ---------------- "new" / "I" overdispersion ---------------- */

@@ -295,6 +295,7 @@ def test_stage_entry_points(dv, orc):
     b0n = b0.cpu().numpy().reshape(p, G).T
     assert_rel(b0n, orc.init_beta_qr(Yf, w.X, w.offset), 1e-10, 1e-8, "beta0")
     beta = b0.clone()
+    torch.cuda.synchronize()  # the clone runs on torch's stream, the calls on the library's
     it = torch.empty(G, dtype=torch.int32, device=dev)
     ctx.check(lib.devil_beta_fit(ctx.h, Yd.data_ptr(), L.COUNTS_I32, G, n, n, Xd.data_ptr(), p,
                                  beta.data_ptr(), off.data_ptr(), a0.data_ptr(), 200, 1e-3,
@@ -335,6 +336,7 @@ def test_stage_entry_points(dv, orc):
     br = torch.empty(G * p * p, dtype=torch.float64, device=dev)
     sw = torch.empty(G * p * p, dtype=torch.float64, device=dev)
     st = torch.zeros(G, dtype=torch.int32, device=dev)
+    torch.cuda.synchronize()
     ctx.check(lib.devil_bread_meat(ctx.h, Yd.data_ptr(), L.COUNTS_I32, G, n, n, Xd.data_ptr(),
                                    p, beta.data_ptr(), th.data_ptr(), off.data_ptr(),
                                    ends.data_ptr(), 3, br.data_ptr(), sw.data_ptr(),
@@ -514,3 +516,42 @@ def test_new_overdispersion_reference_scenario(dv, orc):
     assert th.shape == (3,) and np.all(np.isfinite(th))
     assert np.all(np.asarray(fit["iterations"]["theta_iters"]) >= 0)
     assert_rel(th, ref["theta"], 1e-3, 1e-8, "theta")
+
+
+@pytest.mark.parametrize("G", [1, 2, 3, 4097, 20000])
+def test_p_adjust_methods(dv, orc, G):
+    """devil_p_adjust (stats::p.adjust, R/test.R:127) against the oracle,
+    bit for bit, with ties and NaN p-values."""
+    from devil_amd import _lib as L
+    say(f"[case] p_adjust G={G}")
+    rng = np.random.default_rng(G)
+    p = rng.uniform(0, 1, G) ** 4
+    if G > 3:
+        p[rng.integers(0, G, G // 10)] = p[0]
+        p[rng.integers(0, G, G // 20)] = np.nan
+    ctx = dv.context(0)
+    pd = torch.as_tensor(p, device="cuda")
+    for name, code in L.P_ADJUST.items():
+        adj = torch.empty_like(pd)
+        torch.cuda.synchronize()
+        ctx.check(dv.lib().devil_p_adjust(ctx.h, pd.data_ptr(), G, code, adj.data_ptr()))
+        np.testing.assert_array_equal(adj.cpu().numpy(), orc.p_adjust(p, name), err_msg=name)
+    inplace = pd.clone()
+    torch.cuda.synchronize()  # the clone runs on torch's stream, the call on its own
+    ctx.check(dv.lib().devil_p_adjust(ctx.h, inplace.data_ptr(), G, L.P_ADJUST["holm"],
+                                      inplace.data_ptr()))
+    np.testing.assert_array_equal(inplace.cpu().numpy(), orc.p_adjust(p, "holm"))
+
+
+def test_test_de_adjust_methods(dv, orc):
+    """test_de(pval_adjust_method=...) applies p.adjust to its p-values."""
+    from devil_amd import synth
+    say("[case] test_de adjust methods")
+    w = synth.make_workload("C1", G=300, n=2000)
+    Y = synth.counts_host(w)
+    fit = dv.fit_devil(Y, w.X, overdispersion="MOM", size_factors=w.sf)
+    for m in ("holm", "hochberg", "hommel", "bonferroni", "BY", "none", "fdr"):
+        r = dv.test_de(fit, [0, 1], pval_adjust_method=m)
+        np.testing.assert_array_equal(r["adj_pval"], orc.p_adjust(r["pval"], m), err_msg=m)
+    with pytest.raises(ValueError):
+        dv.test_de(fit, [0, 1], pval_adjust_method="sidak")

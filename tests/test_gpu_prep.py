@@ -95,6 +95,7 @@ def test_group_data_matches_oracle(dv, orc):
     Yd = torch.as_tensor(Y, device="cuda")
     bad = torch.as_tensor(np.r_[np.arange(n - 1), n + 5].astype(np.int64), device="cuda")
     outb = torch.empty_like(Yd)
+    torch.cuda.synchronize()  # torch's stream before the library's
     rc = dv.lib().devil_group_cells(ctx.h, Yd.data_ptr(), 0, G, n, n, bad.data_ptr(),
                                     outb.data_ptr(), n)
     assert rc == 1
@@ -120,6 +121,7 @@ def test_csc_densify_exact(dv):
     rowind = torch.as_tensor(np.array([0, 9], np.int32), device="cuda")
     vals = torch.ones(2, dtype=torch.float64, device="cuda")
     out = torch.full((3, 2), 5, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()  # torch's stream before the library's
     rc = dv.lib().devil_csc_to_dense(ctx.h, colptr.data_ptr(), rowind.data_ptr(),
                                      vals.data_ptr(), 3, 2, 0, out.data_ptr(), 2)
     assert rc == 1 and torch.all(out == 5)

@@ -48,6 +48,7 @@ def _run(dv, fn, beta, bread, sw, n, contrast):
     c = t(contrast)
     out = [torch.full((G,), -7.0, dtype=torch.float64, device="cuda") for _ in range(4)]
     ptr = lambda a: None if a is None else a.data_ptr()
+    torch.cuda.synchronize()  # the fills above run on torch's stream, the call on its own
     rc = getattr(dv.lib(), fn)(ctx.h, bcol.data_ptr(), br.data_ptr(), ptr(swd), G, p, n,
                                c.data_ptr(), 10.0, out[0].data_ptr(), out[1].data_ptr(),
                                out[2].data_ptr(), out[3].data_ptr())
