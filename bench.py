@@ -222,6 +222,12 @@ def bench_many(args):
     torch.cuda.synchronize()
     ctx = devil_amd.context(0)
     ctx.set_timing(True)
+    if args.concurrency:
+        from devil_amd import _lib
+        ctx.set_tuning(_lib.TUNE_CONCURRENCY, args.concurrency)
+    if args.pad_copy >= 0:
+        from devil_amd import _lib
+        ctx.set_tuning(_lib.TUNE_PAD_COPY, args.pad_copy)
     # every input resident in HBM before timing (design column-major, as read)
     jobs = [dict(x=Y, design_matrix=torch.as_tensor(np.ascontiguousarray(w.X.T), device=dev).t(),
                  size_factors=torch.as_tensor(np.asarray(w.sf, dtype=np.float64), device=dev))
@@ -302,6 +308,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--chunk-cells", type=int, default=0,
                     help="cells per chunk of the tile passes (0: autotuned)")
+    ap.add_argument("--pad-copy", type=int, default=-1, choices=[-1, 0, 1],
+                    help="mixed fits of misaligned device rows: 1 read a padded copy, 0 read in "
+                         "place (-1: library default)")
+    ap.add_argument("--concurrency", type=int, default=0,
+                    help="C5: host threads / streams of the many-problem call (0: library "
+                         "default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -336,6 +348,9 @@ def main():
     if args.chunk_cells:
         from devil_amd import _lib
         ctx.set_tuning(_lib.TUNE_CHUNK_CELLS, args.chunk_cells)
+    if args.pad_copy >= 0:
+        from devil_amd import _lib
+        ctx.set_tuning(_lib.TUNE_PAD_COPY, args.pad_copy)
 
     # every input resident in HBM before timing: counts, the design matrix
     # (column-major, as the library reads it) and the size factors

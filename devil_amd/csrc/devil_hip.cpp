@@ -498,11 +498,14 @@ bool counts_vec4(const dv::Counts& c) {
          (reinterpret_cast<uintptr_t>(c.ptr) & 15) == 0;
 }
 
+// 4-byte counts (what the LDS-staged kernels read, 16 or 4 bytes per load)
+bool counts_ws(const dv::Counts& c) { return c.dtype != DEVIL_COUNTS_F64; }
+
 // 16-gene tiles per workgroup of a round's accumulate kernel.
 int accum_tiles_per_wg(int kind, int p, const dv::Counts& c) {
-  if (!counts_vec4(c)) return 4;
+  if (!counts_ws(c)) return 4;
   if (kind == kMixedBf16Ws) return dv::ws_tiles_per_workgroup(p, 1);
-  if (kind == kMixedBf16Ws2) return dv::ws_tiles_per_workgroup(p, 2);
+  if (kind == kMixedBf16Ws2) return dv::ws_tiles_per_workgroup(p, counts_vec4(c) ? 2 : 1);
   return 4;
 }
 
@@ -1739,7 +1742,7 @@ int fit_one(devil_ctx* ctx, const devil_problem* pr, devil_outputs* out) {
     // mixed precision: the bread on the bf16 tables (FP32 element math, FP64
     // sums and inverse) where there is no meat to build
     bool bread_done = false;
-    if (mixed && bf16_layout(tl.mixed) && C == 0 && counts_vec4(c)) {
+    if (mixed && bf16_layout(tl.mixed) && C == 0 && counts_ws(c)) {
       if ((rc = bread_mixed(ctx, tl, c, Gb, p, b0, th, bread.as<double>() + g0 * p * p,
                             status.as<int32_t>() + g0, bread_done)))
         return rc;

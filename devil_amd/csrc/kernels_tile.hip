@@ -689,6 +689,11 @@ __device__ __forceinline__ void lds_dma16(const void* g, unsigned char* l) {
   __builtin_amdgcn_global_load_lds(g, (lds_ptr_t)l, 16, 0, 0);
 }
 
+// 4 bytes per lane, written to l + 4 lane
+__device__ __forceinline__ void lds_dma4(const void* g, unsigned char* l) {
+  __builtin_amdgcn_global_load_lds(g, (lds_ptr_t)l, 4, 0, 0);
+}
+
 template <int NE, int NTH>
 __device__ __forceinline__ void ws_loader(const TileDesign& td, int64_t s0, int64_t s1,
                                           unsigned char* ring) {
@@ -740,8 +745,14 @@ __device__ __forceinline__ void ws_loader(const TileDesign& td, int64_t s0, int6
 
 // MODE 0: IRLS accumulate (gpar = k, residuals r); MODE 1: bread (gpar =
 // theta, weights w): the same contractions of a per-cell scalar with the
-// design columns and the pair columns.
-template <int P, int MODE, int TPW, typename CT>
+// design columns and the pair columns.  A16: every count row starts 16-byte
+// aligned, so a lane's four cells arrive by one 16-byte direct-to-LDS load
+// (lane l's gene l % 16, cells 4 (l / 16) .. + 3, at byte 16 l of each KB);
+// otherwise four 4-byte direct-to-LDS loads per KB, load e covering genes
+// 4 e .. 4 e + 3 with 16 lanes (64 contiguous bytes) each, so gene s's 16
+// cells sit at dwords 16 s .. 16 s + 15 and lane (s, h) reads its four at
+// byte 64 s + 16 h.
+template <int P, int MODE, int TPW, bool A16, typename CT>
 __device__ __forceinline__ void ws_compute(const TileDesign& td, const CT* __restrict__ Y,
                                            int64_t ld, const int32_t* __restrict__ active,
                                            int nA, const double* __restrict__ beta,
@@ -777,6 +788,22 @@ __device__ __forceinline__ void ws_compute(const TileDesign& td, const CT* __res
     yrow[u] = Y + (int64_t)g[u] * ld;
     yh[u] = yrow[u] + 4 * h;
   }
+  // 4-byte loads: in load e, lane l fetches cell l % 16 of the gene of lane
+  // 4 e + l / 16 (lanes 0..15 hold the 16 genes in lane order)
+  const CT* ydw[TPW][4];
+#pragma unroll
+  for (int u = 0; u < TPW; ++u) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      if constexpr (A16) {
+        ydw[u][e] = yh[u];
+      } else {
+        const int gs = __shfl(g[u], 4 * e + (lane >> 4), 64);
+        ydw[u][e] = Y + (int64_t)gs * ld + (lane & 15);
+      }
+    }
+  }
+  constexpr int kLoadsPerStep = A16 ? 2 : 8;  // count loads per tile and step
   const int64_t nsteps = s1 - s0;
   f4 ag0[TPW];
   double ag64[TPW][4];
@@ -800,8 +827,16 @@ __device__ __forceinline__ void ws_compute(const TileDesign& td, const CT* __res
 #pragma unroll
     for (int u = 0; u < TPW; ++u) {
       unsigned char* yr = yring + u * Lay::kYWave + q * Lay::kYSlot;
-      lds_dma16(yh[u] + st * 32, yr);
-      lds_dma16(yh[u] + st * 32 + 16, yr + 1024);
+      if constexpr (A16) {
+        lds_dma16(yh[u] + st * 32, yr);
+        lds_dma16(yh[u] + st * 32 + 16, yr + 1024);
+      } else {
+#pragma unroll
+        for (int half = 0; half < 2; ++half)
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            lds_dma4(ydw[u][e] + st * 32 + 16 * half, yr + 1024 * half + 256 * e);
+      }
     }
   };
   auto ready = [&](int q, Cnt (&o)[TPW]) {
@@ -809,8 +844,9 @@ __device__ __forceinline__ void ws_compute(const TileDesign& td, const CT* __res
 #pragma unroll
     for (int u = 0; u < TPW; ++u) {
       const unsigned char* yr = yring + u * Lay::kYWave + q * Lay::kYSlot;
-      const V4 a = reinterpret_cast<const V4*>(yr)[lane];
-      const V4 b = reinterpret_cast<const V4*>(yr + 1024)[lane];
+      const int at = A16 ? lane : 4 * s + h;
+      const V4 a = reinterpret_cast<const V4*>(yr)[at];
+      const V4 b = reinterpret_cast<const V4*>(yr + 1024)[at];
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         o[u].y[r] = a[r];
@@ -863,8 +899,9 @@ __device__ __forceinline__ void ws_compute(const TileDesign& td, const CT* __res
   // steps ahead: s0 .. s0 + 2 before barrier 0, then in step k the load of
   // step s0 + k + 3 (clamped to s1f - 1) into count slot (k + 3) % 4, whose
   // step k - 1 was read.  This wave's vector-memory instructions are these
-  // loads alone (two per tile and step), so vmcnt(6 TPW) after issuing step
-  // k + 3 leaves steps k + 1 .. k + 3 in flight and step k in LDS.  A row's
+  // loads alone (kLoadsPerStep per tile and step), so vmcnt(3 kLoadsPerStep
+  // TPW) after issuing step k + 3 leaves steps k + 1 .. k + 3 in flight and
+  // step k in LDS.  A row's
   // partial last step comes after them and loads its counts when it runs.
   const int64_t s1f = max(s0, min(s1, td.n / 32));
   const int64_t nf = s1f - s0;
@@ -879,7 +916,7 @@ __device__ __forceinline__ void ws_compute(const TileDesign& td, const CT* __res
     for (int u = 0; u < 4; ++u) {
       if (wave_live) {
         issuey(min(s0 + kk + u + 3, s1f - 1), (u + 3) & 3);
-        wait_vmcnt<6 * TPW>();
+        wait_vmcnt<3 * kLoadsPerStep * TPW>();
         Cnt yc[TPW];
         ready(u, yc);
         step(kk + u, yc);
@@ -971,12 +1008,24 @@ __device__ __forceinline__ void ws_body(const TileDesign& td, const Counts& c,
     return;
   }
   const int tile0 = (group * kWsWaves + wave) * TPW;
-  if (c.dtype == 0)
-    ws_compute<P, MODE, TPW>(td, static_cast<const int32_t*>(c.ptr), c.ld, active, nA, beta, gpar,
-                             tile0, ntile, chunk, s0, s1, ring, part);
-  else
-    ws_compute<P, MODE, TPW>(td, static_cast<const float*>(c.ptr), c.ld, active, nA, beta, gpar,
-                             tile0, ntile, chunk, s0, s1, ring, part);
+  const bool a16 = (c.ld & 3) == 0 && (reinterpret_cast<uintptr_t>(c.ptr) & 15) == 0;
+  if (a16) {
+    if (c.dtype == 0)
+      ws_compute<P, MODE, TPW, true>(td, static_cast<const int32_t*>(c.ptr), c.ld, active, nA,
+                                     beta, gpar, tile0, ntile, chunk, s0, s1, ring, part);
+    else
+      ws_compute<P, MODE, TPW, true>(td, static_cast<const float*>(c.ptr), c.ld, active, nA,
+                                     beta, gpar, tile0, ntile, chunk, s0, s1, ring, part);
+  } else if constexpr (TPW == 1) {
+    // (two tiles per wave would keep 64 loads in flight, past what vmcnt
+    // counts: the launchers run misaligned rows with one tile per wave)
+    if (c.dtype == 0)
+      ws_compute<P, MODE, TPW, false>(td, static_cast<const int32_t*>(c.ptr), c.ld, active, nA,
+                                      beta, gpar, tile0, ntile, chunk, s0, s1, ring, part);
+    else
+      ws_compute<P, MODE, TPW, false>(td, static_cast<const float*>(c.ptr), c.ld, active, nA,
+                                      beta, gpar, tile0, ntile, chunk, s0, s1, ring, part);
+  }
 }
 
 // Counts must be 4-byte elements in 16-byte aligned rows (the launchers fall
@@ -1288,8 +1337,13 @@ hipError_t launch_irls_accum_ws(hipStream_t s, const TileDesign& td, Counts c, i
                                 const double* beta, const double* kvec, double* part, int tpw) {
   if (nA_bound <= 0) return hipSuccess;
   const int ntile = (nA_bound + 15) / 16;
-  const bool vec4 = c.dtype != 2 && (c.ld & 3) == 0 && (reinterpret_cast<uintptr_t>(c.ptr) & 15) == 0;
-  if (!vec4) return launch_irls_accum_bf16(s, td, c, p, active, nactive, nA_bound, beta, kvec, part);
+  // 8-byte counts have no direct-to-LDS form here: the register kernel
+  if (c.dtype == 2)
+    return launch_irls_accum_bf16(s, td, c, p, active, nactive, nA_bound, beta, kvec, part);
+  // 4-byte counts must start 4-byte aligned (every row then does); rows not
+  // 16-byte aligned run with one tile per wave
+  if ((reinterpret_cast<uintptr_t>(c.ptr) & 3) != 0) return hipErrorInvalidValue;
+  if ((c.ld & 3) != 0 || (reinterpret_cast<uintptr_t>(c.ptr) & 15) != 0) tpw = 1;
   const int nw = ws_waves(p, tpw);
   const int per_group = nw * tpw;
   const int wgroups = (ntile + per_group - 1) / per_group;
@@ -1319,8 +1373,7 @@ hipError_t launch_bread_mixed(hipStream_t s, const TileDesign& td, Counts c, int
                               const double* beta, const double* theta, double* part,
                               double* bread, int32_t* singular, int32_t* fallback) {
   if (G <= 0) return hipSuccess;
-  const bool vec4 = c.dtype != 2 && (c.ld & 3) == 0 && (reinterpret_cast<uintptr_t>(c.ptr) & 15) == 0;
-  if (!vec4) return hipErrorInvalidValue;
+  if (c.dtype == 2 || (reinterpret_cast<uintptr_t>(c.ptr) & 3) != 0) return hipErrorInvalidValue;
   const int ntile = (int)((G + 15) / 16);
   const int nw = ws_waves(p);
   const int wgroups = (ntile + nw - 1) / nw;

@@ -4,8 +4,9 @@ register kernel computes -- same operands, same products, same summation
 order -- so the two fits must agree bit for bit.  The shapes cover a chunk
 that holds only a row's partial last step (n = 5128 with 1024-cell chunks),
 rows without a partial step (n = 6144), a partial step inside a longer chunk
-(n = 5000), 16-byte-misaligned rows (n = 5001: the launcher falls back to the
-bf16 kernel), workgroups with idle tiles (G not a multiple of the tiles per
+(n = 5000), 16-byte-misaligned rows (n = 5001: read in place by 4-byte
+direct-to-LDS loads, or from the padded copy a mixed fit makes by default),
+workgroups with idle tiles (G not a multiple of the tiles per
 workgroup; 17 genes leave the second tile of a two-tile wave idle), float
 counts, and p = 4 and 8, for the kernel with one and with two 16-gene tiles per
 compute wave.  The same kernel computes the
@@ -38,33 +39,40 @@ def dv():
     ctx.set_tuning(devil_amd._lib.TUNE_CHUNK_CELLS, 0)
 
 
-def fit_with(dv, kernel, Y, w):
+def fit_with(dv, kernel, Y, w, pad=1):
     ctx = dv.context(0)
     ctx.set_mixed_kernel(kernel)
     ctx.set_tuning(dv._lib.TUNE_CHUNK_CELLS, 1024)
-    fit = dv.fit_devil(Y, w.X, size_factors=w.sf, precision="mixed")
+    ctx.set_tuning(dv._lib.TUNE_PAD_COPY, pad)
+    try:
+        fit = dv.fit_devil(Y, w.X, size_factors=w.sf, precision="mixed")
+    finally:
+        ctx.set_tuning(dv._lib.TUNE_PAD_COPY, 1)
     info = ctx.fit_info()
     return fit, info
 
 
 @pytest.mark.parametrize("kernel", ["bf16_ws", "bf16_ws2"])
-@pytest.mark.parametrize("config,G,n,dtype", [
-    ("C3", 487, 5128, "int32"),
-    ("C3", 487, 6144, "int32"),
-    ("C2", 300, 5000, "int32"),
-    ("C3", 250, 5001, "int32"),
-    ("C2", 487, 5128, "float32"),
-    ("C3", 17, 6144, "int32"),
+@pytest.mark.parametrize("config,G,n,dtype,pad", [
+    ("C3", 487, 5128, "int32", 1),
+    ("C3", 487, 6144, "int32", 1),
+    ("C2", 300, 5000, "int32", 1),
+    ("C3", 250, 5001, "int32", 1),
+    ("C3", 250, 5001, "int32", 0),
+    ("C2", 301, 4097, "float32", 0),
+    ("C2", 487, 5128, "float32", 1),
+    ("C3", 17, 6144, "int32", 1),
 ])
-def test_ws_equals_bf16(dv, config, G, n, dtype, kernel):
+def test_ws_equals_bf16(dv, config, G, n, dtype, pad, kernel):
     from devil_amd import synth
     w = synth.make_workload(config, G=G, n=n, seed=4242 + n)
     Y = synth.counts_device(w, device="cuda")
     if dtype == "float32":
         Y = Y.float()
-    say(f"[case] {kernel} {config} G={G} n={n} p={w.p} {dtype}")
-    a, ia = fit_with(dv, "bf16", Y, w)
-    b, ib = fit_with(dv, kernel, Y, w)
+    say(f"[case] {kernel} {config} G={G} n={n} p={w.p} {dtype} pad copy {pad}")
+    a, ia = fit_with(dv, "bf16", Y, w, pad)
+    b, ib = fit_with(dv, kernel, Y, w, pad)
+    assert ib["aligned_copy"] == (1 if pad and n % 4 else 0)
     say(f"  kernels {ia['mixed_kernel']} / {ib['mixed_kernel']}, replayed {ia['replayed']} / "
         f"{ib['replayed']}")
     assert ia["mixed_kernel"] == "bf16" and ib["mixed_kernel"] == kernel
@@ -97,3 +105,19 @@ def test_mixed_bread_matches_fp64(dv):
     assert e.max() < 1e-5
     st = fit["status"].cpu().numpy() & 4  # DEVIL_GENE_SINGULAR
     assert np.array_equal(st, ref["status"].cpu().numpy() & 4)
+
+
+def test_mixed_bread_misaligned_rows(dv):
+    """The mixed bread read from misaligned rows in place (4-byte
+    direct-to-LDS loads) equals the one read from the padded copy."""
+    from devil_amd import synth
+    w = synth.make_workload("C3", G=300, n=6143, seed=778)
+    Y = synth.counts_device(w, device="cuda")
+    say("[case] mixed bread, n = 6143, in place vs padded copy")
+    a, ia = fit_with(dv, "bf16_ws", Y, w, 0)
+    b, ib = fit_with(dv, "bf16_ws", Y, w, 1)
+    assert ia["aligned_copy"] == 0 and ib["aligned_copy"] == 1
+    for key in ("beta", "overdispersion", "beta_sandwiches_null"):
+        x, y = a[key].cpu().numpy(), b[key].cpu().numpy()
+        same = (x == y) | (np.isnan(x) & np.isnan(y))
+        assert same.all(), f"{key}: {int((~same).sum())} entries differ"
