@@ -250,7 +250,9 @@ def bench_many(args):
         fits = many()
     torch.cuda.synchronize()
     T = time.perf_counter() - t0
-    ks = ctx.kernel_stats()["irls_mixed"]
+    kall = ctx.kernel_stats()
+    ks = kall["irls_mixed"]
+    ctx_fp64_launches = kall["irls_fp64"][0]
     info = ctx.fit_info()
     log(f"[bench] mixed, {len(jobs)} problems per call: {1e3 * T / args.steps:.1f} ms/step")
     out = {
@@ -271,6 +273,27 @@ def bench_many(args):
                      "traffic": None, "launches_last_step": ks[0]},
         "fit_info": info,
     }
+    # the same call with joint IRLS rounds (one accumulate and one update launch
+    # per round for each unit of problems of one design width)
+    from devil_amd import _lib
+    ctx.set_tuning(_lib.TUNE_JOINT, 1)
+    try:
+        many()  # tunes the joint rounds' chunk sizes
+        torch.cuda.synchronize()
+        tj = time.perf_counter()
+        for _ in range(args.steps):
+            many()
+        torch.cuda.synchronize()
+        TJ = (time.perf_counter() - tj) / args.steps
+        kj = ctx.kernel_stats()
+        out["joint_rounds"] = {
+            "ms_per_step": 1e3 * TJ, "value": units / TJ, "problems_joined": ctx.fit_info()["joined"],
+            "launches_mixed_accumulate": kj["irls_mixed"][0],
+            "launches_fp64_accumulate": kj["irls_fp64"][0],
+            "launches_own_rounds": {"mixed": ks[0], "fp64": ctx_fp64_launches}}
+    finally:
+        ctx.set_tuning(_lib.TUNE_JOINT, 0)
+    log(f"[bench] joint rounds: {1e3 * TJ:.1f} ms/step")
     if args.fp64_steps > 0:
         torch.cuda.synchronize()
         t1 = time.perf_counter()

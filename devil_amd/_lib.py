@@ -117,6 +117,7 @@ EXPORTED = ["devil_create", "devil_destroy", "devil_strerror", "devil_last_error
 STAGES = ["init", "irls", "dispersion", "bread_meat", "test_de"]
 KERNELS = ["irls_fp64", "irls_mixed", "mle"]
 TUNE_MIXED_KERNEL, TUNE_CHUNK_CELLS, TUNE_CONCURRENCY, TUNE_MLE_DRIVER, TUNE_PAD_COPY = 0, 1, 2, 3, 4
+TUNE_JOINT = 5
 MLE_DRIVERS = {"nlminb": 0, "root": 1}
 SF_METHODS = {"normed_sum": 0, "psinorm": 1, "edgeR": 2}
 # stats::p.adjust.methods
@@ -169,7 +170,7 @@ class Context:
         size (the autotuner's choice when tuning was left open)."""
         out = {}
         for k, name in enumerate(("replayed", "mixed_kernel", "chunk_cells", "polished",
-                                  "aligned_copy")):
+                                  "aligned_copy", "joined")):
             v = C.c_int64()
             self.check(lib().devil_fit_info(self.h, k, C.byref(v)))
             out[name] = int(v.value)

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <memory>
 #include <atomic>
 #include <map>
 #include <cmath>
@@ -64,8 +65,9 @@ struct devil_ctx {
   std::string err;
   int timing = 0;
   // kernel, chunk cells (0 auto), streams, MLE driver, aligned copies allowed
-  int tune[DEVIL_N_TUNE] = {DEVIL_MIXED_AUTO, 0, 8, DEVIL_MLE_NLMINB, 1};
+  int tune[DEVIL_N_TUNE] = {DEVIL_MIXED_AUTO, 0, 8, DEVIL_MLE_NLMINB, 1, 0};
   std::map<uint64_t, int> tuned;      // autotuner choices by problem shape
+  std::map<uint64_t, int> tuned_joint;  // the same for the joint rounds (kernel fixed)
   std::vector<devil_ctx*> subs;       // per-stream contexts of multi-problem fits
   int64_t replayed = 0;                // genes refit in FP64 by the mixed path (last fit)
   int64_t polished = 0;                // genes whose last mixed step was redone in FP64
@@ -96,6 +98,10 @@ struct devil_ctx {
   // IRLS round counts read back without a stream synchronisation
   int32_t* round_counts = nullptr;  // pinned host slots
   hipEvent_t round_ev[2] = {nullptr, nullptr};
+  int64_t joined = 0;                 // problems of the last fit in the joint rounds
+  // joint rounds of many problems: pinned counts and round descriptors
+  void* joint_host = nullptr;
+  size_t joint_host_bytes = 0;
 };
 
 namespace {
@@ -1513,6 +1519,7 @@ void devil_destroy(devil_ctx* ctx) {
   for (hipEvent_t e : ctx->round_ev)
     if (e) (void)hipEventDestroy(e);
   if (ctx->round_counts) (void)hipHostFree(ctx->round_counts);
+  if (ctx->joint_host) (void)hipHostFree(ctx->joint_host);
   if (ctx->pad_p) (void)hipFree(ctx->pad_p);
   ctx->cache.clear();
   delete ctx;
@@ -1523,12 +1530,45 @@ void devil_destroy(devil_ctx* ctx) {
 namespace {
 
 // One problem on ctx's stream (the body of devil_fit for n_problems == 1).
-int fit_one(devil_ctx* ctx, const devil_problem* pr, devil_outputs* out) {
+// One problem of a fit across its phases: set-up (validation, design tables,
+// buffers), per gene batch the init pass, the IRLS and the stages after it,
+// then the outputs.  fit_one runs them in order; devil_fit with several
+// problems runs the IRLS of the eligible ones jointly (run_irls_joint).
+struct ProbRun {
+  const devil_problem* pr = nullptr;
+  devil_outputs* out = nullptr;
+  int64_t G = 0, n = 0;
+  int p = 0, mem = 0, C = 0, init_mode = 0, mk = kMixedNone;
+  bool mixed = false, autotune = false, want_sw = false, pad_host = false, run_init = false;
+  int64_t cc = 2048, batch = 0, ld_pad = 0, dev_ld = 0;
+  uint64_t key = 0;
+  size_t es = 4;
+  double eps = 0.0;
+  std::vector<int32_t> ends_h;
+  Design d;
+  DeviceDesign dd;
+  TileHost tl;
+  ChunkHost chunks;
+  InitHost ih;
+  DevBuf alpha0, kvec, beta, theta, iters, thit, status, bread, sandwich, cbuf, stage, post;
+  const void* dev_counts = nullptr;
+  int64_t g0 = 0, Gb = 0;  // the current gene batch
+  dv::Counts c{};
+};
+
+// Validation, design tables, per-gene buffers, counts layout.  G == 0 leaves
+// r.G = 0 (nothing else to do).
+int run_setup(devil_ctx* ctx, const devil_problem* pr, devil_outputs* out, ProbRun& r) {
   int rc;
   if ((rc = set_device(ctx))) return rc;
   if (!pr || !out) return fail(ctx, DEVIL_E_ARG, "NULL problem or outputs");
+  r.pr = pr;
+  r.out = out;
   const int64_t G = pr->G, n = pr->n;
   const int p = pr->p;
+  r.G = G;
+  r.n = n;
+  r.p = p;
   if ((rc = check_p(ctx, p))) return rc;
   if ((rc = check_counts_args(ctx, pr->counts, pr->counts_dtype, G, n, pr->counts_ld)))
     return rc;
@@ -1550,57 +1590,56 @@ int fit_one(devil_ctx* ctx, const devil_problem* pr, devil_outputs* out) {
   if (G == 0) return DEVIL_OK;
   const int mem = pr->mem;
   const int C = pr->n_clusters;
+  r.mem = mem;
+  r.C = C;
 
-  std::vector<int32_t> ends_h(C);
+  r.ends_h.assign(C, 0);
   if (C > 0) {
     if (mem == DEVIL_MEM_DEVICE)
-      DV_HIP(hipMemcpy(ends_h.data(), pr->cluster_ends, sizeof(int32_t) * C,
+      DV_HIP(hipMemcpy(r.ends_h.data(), pr->cluster_ends, sizeof(int32_t) * C,
                        hipMemcpyDeviceToHost));
     else
-      std::memcpy(ends_h.data(), pr->cluster_ends, sizeof(int32_t) * C);
-    if ((rc = check_clusters(ctx, ends_h.data(), C, n))) return rc;
+      std::memcpy(r.ends_h.data(), pr->cluster_ends, sizeof(int32_t) * C);
+    if ((rc = check_clusters(ctx, r.ends_h.data(), C, n))) return rc;
   }
-  const int init_mode = pr->beta_init ? dv::kInitNone : pr->init_mode;
-  Design d;
-  DeviceDesign dd;
-  if ((rc = prepare_design(ctx, pr->X, pr->offset, n, p, mem, init_mode, d, dd))) return rc;
+  r.init_mode = pr->beta_init ? dv::kInitNone : pr->init_mode;
+  if ((rc = prepare_design(ctx, pr->X, pr->offset, n, p, mem, r.init_mode, r.d, r.dd)))
+    return rc;
   // mixed kernel and chunk size: fixed by the context, remembered from an
   // earlier fit of this shape, or tuned after the first batch's init pass
   // mixed precision pays where the counts stream is long: problems with fewer
   // than kMixedMinCells cells run the FP64 kernels (exact; their cost is small)
-  const bool mixed = pr->precision == DEVIL_PREC_MIXED && n >= kMixedMinCells;
-  int mk = mixed ? mixed_kernel(ctx) : kMixedNone;
-  int64_t cc = ctx->tune[DEVIL_TUNE_CHUNK_CELLS] > 0 ? ctx->tune[DEVIL_TUNE_CHUNK_CELLS] : 2048;
-  const uint64_t key = tune_key(n, p, pr->counts_dtype);
-  bool autotune = mixed && (ctx->tune[DEVIL_TUNE_MIXED_KERNEL] == DEVIL_MIXED_AUTO ||
-                            ctx->tune[DEVIL_TUNE_CHUNK_CELLS] == 0);
-  if (autotune && ctx->tuned.count(key)) {
-    const int v = ctx->tuned[key];
-    if (ctx->tune[DEVIL_TUNE_MIXED_KERNEL] == DEVIL_MIXED_AUTO) mk = v & 0xff;
-    if (ctx->tune[DEVIL_TUNE_CHUNK_CELLS] == 0) cc = (int64_t)(v >> 8);
-    autotune = false;
+  r.mixed = pr->precision == DEVIL_PREC_MIXED && n >= kMixedMinCells;
+  r.mk = r.mixed ? mixed_kernel(ctx) : kMixedNone;
+  r.cc = ctx->tune[DEVIL_TUNE_CHUNK_CELLS] > 0 ? ctx->tune[DEVIL_TUNE_CHUNK_CELLS] : 2048;
+  r.key = tune_key(n, p, pr->counts_dtype);
+  r.autotune = r.mixed && (ctx->tune[DEVIL_TUNE_MIXED_KERNEL] == DEVIL_MIXED_AUTO ||
+                           ctx->tune[DEVIL_TUNE_CHUNK_CELLS] == 0);
+  if (r.autotune && ctx->tuned.count(r.key)) {
+    const int v = ctx->tuned[r.key];
+    if (ctx->tune[DEVIL_TUNE_MIXED_KERNEL] == DEVIL_MIXED_AUTO) r.mk = v & 0xff;
+    if (ctx->tune[DEVIL_TUNE_CHUNK_CELLS] == 0) r.cc = (int64_t)(v >> 8);
+    r.autotune = false;
   }
-  TileHost tl;
-  if ((rc = make_tile(ctx, dd.x, dd.off.as<double>(), dd.eo.as<double>(), n, p,
-                      autotune ? kMixedNone : mk, tl, cc)))
+  if ((rc = make_tile(ctx, r.dd.x, r.dd.off.as<double>(), r.dd.eo.as<double>(), n, p,
+                      r.autotune ? kMixedNone : r.mk, r.tl, r.cc)))
     return rc;
 
   // per-gene device state (gene-major)
-  DevBuf alpha0, kvec, beta, theta, iters, thit, status, bread, sandwich;
-  DV_ALLOC(alpha0, sizeof(double) * G);
-  DV_ALLOC(kvec, sizeof(double) * G);
-  DV_ALLOC(beta, sizeof(double) * G * p);
-  DV_ALLOC(theta, sizeof(double) * G);
-  DV_ALLOC(iters, sizeof(int32_t) * G);
-  DV_ALLOC(thit, sizeof(int32_t) * G);
-  DV_ALLOC(status, sizeof(int32_t) * G);
-  DV_ALLOC(bread, sizeof(double) * G * p * p);
-  const bool want_sw = C > 0 && out->sandwich;
-  if (want_sw) DV_ALLOC(sandwich, sizeof(double) * G * p * p);
+  DV_ALLOC(r.alpha0, sizeof(double) * G);
+  DV_ALLOC(r.kvec, sizeof(double) * G);
+  DV_ALLOC(r.beta, sizeof(double) * G * p);
+  DV_ALLOC(r.theta, sizeof(double) * G);
+  DV_ALLOC(r.iters, sizeof(int32_t) * G);
+  DV_ALLOC(r.thit, sizeof(int32_t) * G);
+  DV_ALLOC(r.status, sizeof(int32_t) * G);
+  DV_ALLOC(r.bread, sizeof(double) * G * p * p);
+  r.want_sw = C > 0 && out->sandwich;
+  if (r.want_sw) DV_ALLOC(r.sandwich, sizeof(double) * G * p * p);
 
   hipStream_t s = ctx->stream;
   if (pr->disp_init) {
-    DV_HIP(hipMemcpyAsync(alpha0.p, pr->disp_init, sizeof(double) * G,
+    DV_HIP(hipMemcpyAsync(r.alpha0.p, pr->disp_init, sizeof(double) * G,
                           mem == DEVIL_MEM_DEVICE ? hipMemcpyDeviceToDevice
                                                   : hipMemcpyHostToDevice,
                           s));
@@ -1612,7 +1651,7 @@ int fit_one(devil_ctx* ctx, const devil_problem* pr, devil_outputs* out) {
                           mem == DEVIL_MEM_DEVICE ? hipMemcpyDeviceToDevice
                                                   : hipMemcpyHostToDevice,
                           s));
-    DV_HIP(dv::launch_transpose_gp(s, tmp.as<double>(), G, p, 1, beta.as<double>()));
+    DV_HIP(dv::launch_transpose_gp(s, tmp.as<double>(), G, p, 1, r.beta.as<double>()));
     DV_HIP(hipStreamSynchronize(s));
   }
 
@@ -1621,23 +1660,24 @@ int fit_one(devil_ctx* ctx, const devil_problem* pr, devil_outputs* out) {
   // kernels) only when every row starts 16-byte aligned: host batches land in
   // rows padded to a multiple of 4 elements, and a mixed fit of misaligned
   // device rows first copies them so (when HBM holds the copy)
-  const size_t es = count_elem_size(pr->counts_dtype);
-  const int64_t ld_pad = es == 4 ? ((n + 3) & ~int64_t(3)) : pr->counts_ld;
-  int64_t batch = G;
-  DevBuf cbuf;
-  const void* dev_counts = pr->counts;
-  int64_t dev_ld = pr->counts_ld;
-  DevBuf stage;  // host rows needing padding land here first
-  const bool pad_host = mem == DEVIL_MEM_HOST && pr->counts_ld != ld_pad;
+  r.es = count_elem_size(pr->counts_dtype);
+  const size_t es = r.es;
+  r.ld_pad = es == 4 ? ((n + 3) & ~int64_t(3)) : pr->counts_ld;
+  const int64_t ld_pad = r.ld_pad;
+  r.batch = G;
+  r.dev_counts = pr->counts;
+  r.dev_ld = pr->counts_ld;
+  r.pad_host = mem == DEVIL_MEM_HOST && pr->counts_ld != ld_pad;
   if (mem == DEVIL_MEM_HOST) {
     size_t freeb = 0, totalb = 0;
     DV_HIP(hipMemGetInfo(&freeb, &totalb));
-    const size_t row_bytes = (size_t)ld_pad * es + (pad_host ? (size_t)pr->counts_ld * es : 0);
+    const size_t row_bytes =
+        (size_t)ld_pad * es + (r.pad_host ? (size_t)pr->counts_ld * es : 0);
     const size_t budget = std::min<size_t>(freeb / 2, (size_t)8 << 30);
-    batch = std::max<int64_t>(1, std::min<int64_t>(G, (int64_t)(budget / row_bytes)));
-    DV_ALLOC(cbuf, (size_t)ld_pad * es * batch);
-    if (pad_host) DV_ALLOC(stage, (size_t)pr->counts_ld * es * batch);
-  } else if (mixed && es == 4 && ctx->tune[DEVIL_TUNE_PAD_COPY] &&
+    r.batch = std::max<int64_t>(1, std::min<int64_t>(G, (int64_t)(budget / row_bytes)));
+    DV_ALLOC(r.cbuf, (size_t)ld_pad * es * r.batch);
+    if (r.pad_host) DV_ALLOC(r.stage, (size_t)pr->counts_ld * es * r.batch);
+  } else if (r.mixed && es == 4 && ctx->tune[DEVIL_TUNE_PAD_COPY] &&
              !counts_vec4(dv::Counts{pr->counts, pr->counts_dtype, pr->counts_ld})) {
     const size_t bytes = (size_t)G * ld_pad * es;
     if (bytes > ctx->pad_size) {
@@ -1653,20 +1693,159 @@ int fit_one(devil_ctx* ctx, const devil_problem* pr, devil_outputs* out) {
     }
     if (bytes <= ctx->pad_size) {
       DV_HIP(dv::launch_pad_rows(s, pr->counts, pr->counts_ld, G, n, ctx->pad_p, ld_pad));
-      dev_counts = ctx->pad_p;
-      dev_ld = ld_pad;
+      r.dev_counts = ctx->pad_p;
+      r.dev_ld = ld_pad;
     }
   }
-  ctx->aligned_copy = dev_counts != pr->counts;
-  ChunkHost chunks;
-  if ((rc = build_chunks(ctx, n, C > 0 ? ends_h.data() : nullptr, C, chunks))) return rc;
-  DevBuf post;
-  DV_ALLOC(post, sizeof(double) * post_part_doubles(tl.td, chunks, batch));
-  const double eps = (double)(float)pr->tolerance;  // src/beta.cpp:25 float eps
-  InitHost ih;
-  if ((rc = build_init_tile(ctx, d, dd, tl.td, pr->disp_init ? 0 : 1, init_mode, ih)))
+  ctx->aligned_copy = r.dev_counts != pr->counts;
+  if ((rc = build_chunks(ctx, n, C > 0 ? r.ends_h.data() : nullptr, C, r.chunks))) return rc;
+  DV_ALLOC(r.post, sizeof(double) * post_part_doubles(r.tl.td, r.chunks, r.batch));
+  r.eps = (double)(float)pr->tolerance;  // src/beta.cpp:25 float eps
+  if ((rc = build_init_tile(ctx, r.d, r.dd, r.tl.td, pr->disp_init ? 0 : 1, r.init_mode, r.ih)))
     return rc;
-  const bool run_init = ih.it.want_alpha || init_mode != dv::kInitNone;
+  r.run_init = r.ih.it.want_alpha || r.init_mode != dv::kInitNone;
+  return DEVIL_OK;
+}
+
+// Gene batch [g0, g0 + Gb): its counts (uploaded when host-resident), the init
+// pass (alpha0, beta0, k) and, on the first batch of an untuned shape, the
+// mixed-kernel autotuner.
+int run_batch_init(devil_ctx* ctx, ProbRun& r, int64_t g0, StageTimer& tm) {
+  const devil_problem* pr = r.pr;
+  hipStream_t s = ctx->stream;
+  const int64_t Gb = std::min(r.batch, r.G - g0);
+  const size_t es = r.es;
+  r.g0 = g0;
+  r.Gb = Gb;
+  dv::Counts& c = r.c;
+  c.dtype = pr->counts_dtype;
+  if (r.mem == DEVIL_MEM_HOST) {
+    const char* src = static_cast<const char*>(pr->counts) + (size_t)g0 * pr->counts_ld * es;
+    if (!r.pad_host) {
+      DV_HIP(hipMemcpyAsync(r.cbuf.p, src, (size_t)Gb * r.ld_pad * es, hipMemcpyHostToDevice,
+                            s));
+    } else {
+      DV_HIP(hipMemcpyAsync(r.stage.p, src, (size_t)Gb * pr->counts_ld * es,
+                            hipMemcpyHostToDevice, s));
+      DV_HIP(dv::launch_pad_rows(s, r.stage.p, pr->counts_ld, Gb, r.n, r.cbuf.p, r.ld_pad));
+    }
+    c.ptr = r.cbuf.p;
+    c.ld = r.ld_pad;
+  } else {
+    c.ptr = static_cast<const char*>(r.dev_counts) + (size_t)g0 * r.dev_ld * es;
+    c.ld = r.dev_ld;
+  }
+  double* a0 = r.alpha0.as<double>() + g0;
+  double* b0 = r.beta.as<double>() + g0 * r.p;
+  hipEvent_t ev = tm.begin(DEVIL_STAGE_INIT);
+  if (r.run_init)
+    DV_HIP(dv::launch_init_tiled(s, r.tl.td, r.ih.it, r.chunks.ct, c, Gb, r.n, r.p,
+                                 r.post.as<double>(), a0, b0));
+  DV_HIP(dv::launch_k_from_alpha(s, a0, Gb, r.kvec.as<double>() + g0));
+  tm.end(ev);
+  int rc;
+  if (r.autotune) {
+    if ((rc = autotune_mixed(ctx, r.dd, r.n, r.p, c, Gb, b0, r.kvec.as<double>() + g0, r.mk,
+                             r.cc)))
+      return rc;
+    ctx->tuned[r.key] = r.mk | (int)(r.cc << 8);
+    r.autotune = false;
+    r.tl = TileHost();
+    if ((rc = make_tile(ctx, r.dd.x, r.dd.off.as<double>(), r.dd.eo.as<double>(), r.n, r.p, r.mk,
+                        r.tl, r.cc)))
+      return rc;
+  }
+  ctx->last_mk = r.mixed ? r.mk : kMixedNone;
+  ctx->last_cc = r.cc;
+  return DEVIL_OK;
+}
+
+int run_batch_irls(devil_ctx* ctx, ProbRun& r, StageTimer& tm) {
+  hipEvent_t ev = tm.begin(DEVIL_STAGE_IRLS);
+  int rc;
+  if ((rc = run_irls_tiled(ctx, r.tl, r.c, r.Gb, r.p, r.beta.as<double>() + r.g0 * r.p,
+                           r.kvec.as<double>() + r.g0, r.pr->max_iter, r.eps,
+                           r.iters.as<int32_t>() + r.g0, r.status.as<int32_t>() + r.g0)))
+    return rc;
+  tm.end(ev);
+  return DEVIL_OK;
+}
+
+// Overdispersion, bread (+ meat and sandwich) of the current gene batch.
+int run_batch_post(devil_ctx* ctx, ProbRun& r, StageTimer& tm) {
+  const devil_problem* pr = r.pr;
+  hipStream_t s = ctx->stream;
+  const int64_t g0 = r.g0, Gb = r.Gb, n = r.n;
+  const int p = r.p, C = r.C;
+  const dv::Counts& c = r.c;
+  double* b0 = r.beta.as<double>() + g0 * p;
+  double* th = r.theta.as<double>() + g0;
+  int32_t* st = r.status.as<int32_t>() + g0;
+  int32_t* tit = r.thit.as<int32_t>() + g0;
+  int rc;
+  hipEvent_t ev = tm.begin(DEVIL_STAGE_DISPERSION);
+  if (pr->od_mode == DEVIL_OD_MOM) {
+    DV_HIP(dv::launch_mom_tiled(s, r.tl.td, r.chunks.ct, c, Gb, n, p, b0, r.post.as<double>(),
+                                th, st));
+    DV_HIP(dv::launch_fill_i32(s, tit, Gb, 0));
+  } else if (pr->od_mode == DEVIL_OD_MLE) {
+    if ((rc = run_mle_tiled(ctx, r.tl.td, c, Gb, n, p, b0, pr->max_iter, pr->tolerance, th, tit,
+                            st)))
+      return rc;
+  } else if (pr->od_mode == DEVIL_OD_NEW) {
+    if ((rc = run_newod_tiled(ctx, r.tl.td, c, Gb, n, p, b0, pr->max_iter, pr->tolerance, th,
+                              tit, st)))
+      return rc;
+  } else {
+    DV_HIP(dv::launch_fill_f64(s, th, Gb, 0.0));
+    DV_HIP(dv::launch_fill_i32(s, tit, Gb, 0));
+  }
+  tm.end(ev);
+  ev = tm.begin(DEVIL_STAGE_BREAD_MEAT);
+  // mixed precision: the bread on the bf16 tables (FP32 element math, FP64
+  // sums and inverse) where there is no meat to build
+  bool bread_done = false;
+  if (r.mixed && bf16_layout(r.tl.mixed) && C == 0 && counts_ws(c)) {
+    if ((rc = bread_mixed(ctx, r.tl, c, Gb, p, b0, th, r.bread.as<double>() + g0 * p * p, st,
+                          bread_done)))
+      return rc;
+  }
+  if (!bread_done)
+    DV_HIP(dv::launch_bread_tiled(s, r.tl.td, r.chunks.ct, c, Gb, n, p, b0, th, C,
+                                  r.post.as<double>(), r.bread.as<double>() + g0 * p * p,
+                                  r.want_sw ? r.sandwich.as<double>() + g0 * p * p : nullptr,
+                                  st));
+  tm.end(ev);
+  return DEVIL_OK;
+}
+
+// Outputs in the caller's layout (beta back to R's column-major G x p); the
+// stream is synchronised on return.
+int run_outputs(devil_ctx* ctx, ProbRun& r) {
+  const devil_problem* pr = r.pr;
+  devil_outputs* out = r.out;
+  hipStream_t s = ctx->stream;
+  const int64_t G = r.G;
+  const int p = r.p;
+  DevBuf bcol;
+  DV_ALLOC(bcol, sizeof(double) * G * p);
+  DV_HIP(dv::launch_transpose_gp(s, r.beta.as<double>(), G, p, 0, bcol.as<double>()));
+  const hipMemcpyKind kind =
+      pr->mem == DEVIL_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+  DV_HIP(hipMemcpyAsync(out->beta, bcol.p, sizeof(double) * G * p, kind, s));
+  DV_HIP(hipMemcpyAsync(out->theta, r.theta.p, sizeof(double) * G, kind, s));
+  DV_HIP(hipMemcpyAsync(out->beta_iters, r.iters.p, sizeof(int32_t) * G, kind, s));
+  DV_HIP(hipMemcpyAsync(out->theta_iters, r.thit.p, sizeof(int32_t) * G, kind, s));
+  DV_HIP(hipMemcpyAsync(out->bread, r.bread.p, sizeof(double) * G * p * p, kind, s));
+  if (r.want_sw)
+    DV_HIP(hipMemcpyAsync(out->sandwich, r.sandwich.p, sizeof(double) * G * p * p, kind, s));
+  if (out->status)
+    DV_HIP(hipMemcpyAsync(out->status, r.status.p, sizeof(int32_t) * G, kind, s));
+  DV_HIP(hipStreamSynchronize(s));
+  return DEVIL_OK;
+}
+
+void reset_fit_stats(devil_ctx* ctx) {
   for (int k = 0; k < DEVIL_STAGE_TEST_DE; ++k) ctx->stage_ms[k] = 0.0;
   for (int k = 0; k < DEVIL_N_KERNELS; ++k) {
     ctx->k_n[k] = 0;
@@ -1674,104 +1853,492 @@ int fit_one(devil_ctx* ctx, const devil_problem* pr, devil_outputs* out) {
   }
   ctx->replayed = 0;
   ctx->polished = 0;
-  StageTimer tm(ctx);
-  hipEvent_t ev;
-  for (int64_t g0 = 0; g0 < G; g0 += batch) {
-    const int64_t Gb = std::min(batch, G - g0);
-    dv::Counts c;
-    c.dtype = pr->counts_dtype;
-    if (mem == DEVIL_MEM_HOST) {
-      const char* src = static_cast<const char*>(pr->counts) + (size_t)g0 * pr->counts_ld * es;
-      if (!pad_host) {
-        DV_HIP(hipMemcpyAsync(cbuf.p, src, (size_t)Gb * ld_pad * es, hipMemcpyHostToDevice, s));
-      } else {
-        DV_HIP(hipMemcpyAsync(stage.p, src, (size_t)Gb * pr->counts_ld * es,
-                              hipMemcpyHostToDevice, s));
-        DV_HIP(dv::launch_pad_rows(s, stage.p, pr->counts_ld, Gb, n, cbuf.p, ld_pad));
+}
+
+// ---------------------------------------------------------------------------
+// Joint IRLS of many problems (devil_fit with n_problems > 1): every round is
+// one accumulate launch per (design width, kernel) and one update launch per
+// design width for all problems, instead of a pair of launches per problem.
+// A problem's share of a round is a JointItem (its round shape, lists,
+// outputs and ranges of the two grids); the items of a round travel to the
+// device in one copy.  Rounds are queued one ahead of the count reads, as in
+// irls_rounds.  The sequence per problem is run_irls_tiled's: mixed (or
+// FP64) rounds, the FP64 refit of the genes the mixed rounds set aside, the
+// FP64 last step of small coefficients.
+struct JointMember {
+  ProbRun* r = nullptr;
+  bool mixed = false;
+  int kind = dv::kJointFp64;  // this phase's accumulate kernel
+  dv::TileDesign tfull{};     // this phase's full design
+  int p = 0;
+  size_t part_cap = 0;
+  double row_bytes = 0.0;
+  DevBuf act0, act1, part, rep, beta0, bprev;
+  int32_t* cur = nullptr;     // this phase's active list and its spare
+  int32_t* nxt = nullptr;
+  int nA = 0;                 // genes of the list at the phase's start
+  double band = 0.0;
+  int polish = 0;
+  const double* abeta = nullptr;  // beta the accumulate reads
+};
+
+int joint_host_setup(devil_ctx* ctx, size_t bytes) {
+  if (ctx->joint_host_bytes >= bytes) return DEVIL_OK;
+  if (ctx->joint_host) DV_HIP(hipHostFree(ctx->joint_host));
+  ctx->joint_host = nullptr;
+  ctx->joint_host_bytes = 0;
+  DV_HIP(hipHostMalloc(&ctx->joint_host, bytes, hipHostMallocDefault));
+  ctx->joint_host_bytes = bytes;
+  return DEVIL_OK;
+}
+
+// Rounds over the members until every list is empty (at most max_rounds).
+// cnt: 2 M device counters (A = the current lists' counts, B = the next);
+// nrep: M device counters of the mixed members' set-aside genes.
+int joint_rounds(devil_ctx* ctx, std::vector<JointMember*>& ms, int max_rounds, int32_t* cnt,
+                 int32_t* nrep) {
+  hipStream_t s = ctx->stream;
+  const int M = (int)ms.size();
+  if (M == 0) return DEVIL_OK;
+  int rc;
+  if ((rc = round_sync_setup(ctx))) return rc;
+  // pinned host: 2 M count slots, then 3 staging slots of M items
+  const size_t slot_items = (size_t)M;
+  const size_t count_bytes = ((sizeof(int32_t) * 2 * M + 255) / 256) * 256;
+  const size_t items_bytes = sizeof(dv::JointItem) * slot_items;
+  if ((rc = joint_host_setup(ctx, count_bytes + 3 * items_bytes))) return rc;
+  volatile int32_t* slot = static_cast<int32_t*>(ctx->joint_host);
+  dv::JointItem* stage = reinterpret_cast<dv::JointItem*>(static_cast<char*>(ctx->joint_host) +
+                                                          count_bytes);
+  DevBuf ditems;
+  DV_ALLOC(ditems, 3 * items_bytes);
+  int32_t* A = cnt;
+  int32_t* B = cnt + M;
+  // round k's list counts per member; count[k][m]
+  std::vector<std::vector<int>> count;
+  count.emplace_back(M);
+  for (int m = 0; m < M; ++m) count[0][m] = ms[m]->nA;
+  struct Rec {
+    long idx;
+    std::vector<int> members;
+  };
+  std::vector<std::vector<Rec>> recs;  // per round, per accumulate launch
+  std::vector<int32_t*> cur(M), nxt(M);
+  for (int m = 0; m < M; ++m) {
+    cur[m] = ms[m]->cur;
+    nxt[m] = ms[m]->nxt;
+  }
+  // order of the items: by design width, ws kernel before FP64 within one
+  std::vector<int> order(M);
+  for (int m = 0; m < M; ++m) order[m] = m;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    if (ms[a]->p != ms[b]->p) return ms[a]->p < ms[b]->p;
+    return ms[a]->kind > ms[b]->kind;
+  });
+  int k = 0;
+  for (; k < max_rounds; ++k) {
+    if (k >= 2) {
+      DV_HIP(hipEventSynchronize(ctx->round_ev[k & 1]));
+      std::vector<int> ck(M);
+      for (int m = 0; m < M; ++m) ck[m] = (int)slot[(k & 1) * M + m];
+      count.push_back(ck);  // round k - 2's update: round k - 1's lists
+    }
+    const std::vector<int>& bound = k == 0 ? count[0] : count[k - 1];
+    int live = 0;
+    for (int m = 0; m < M; ++m) live += bound[m] > 0 ? 1 : 0;
+    if (live == 0) break;
+    // the round's items, grouped by width (and kernel within a width)
+    dv::JointItem* st = stage + (size_t)(k % 3) * slot_items;
+    dv::JointItem* dev = reinterpret_cast<dv::JointItem*>(ditems.as<char>() +
+                                                         (size_t)(k % 3) * items_bytes);
+    struct Launch {
+      int p, kind, first, n, total;
+    };
+    std::vector<Launch> acc, upd;
+    std::vector<int> item_member;
+    int ni = 0;
+    for (int oi = 0; oi < M; ++oi) {
+      const int m = order[oi];
+      if (bound[m] <= 0) continue;
+      JointMember& J = *ms[m];
+      ProbRun& r = *J.r;
+      const int tpw = J.kind == dv::kJointWs ? dv::ws_tiles_per_workgroup(J.p, 1) : 4;
+      dv::JointItem& it = st[ni];
+      std::memset(&it, 0, sizeof(it));
+      it.td = round_shape(J.tfull, bound[m], tpw, J.part_cap);
+      it.c = r.c;
+      it.active = cur[m];
+      it.nactive = A + m;
+      it.beta = const_cast<double*>(J.abeta);
+      it.kvec = r.kvec.as<double>();
+      it.part = J.part.as<double>();
+      it.ubeta = r.beta.as<double>();
+      it.iters = r.iters.as<int32_t>();
+      it.status = r.status.as<int32_t>();
+      it.next = nxt[m];
+      it.nnext = B + m;
+      it.band = J.band;
+      it.recheck = J.band > 0.0 ? J.rep.as<int32_t>() : nullptr;
+      it.nrecheck = J.band > 0.0 ? nrep + m : nullptr;
+      it.bprev = (J.band > 0.0 || J.polish) ? J.bprev.as<double>() : nullptr;
+      it.eps = r.eps;
+      it.max_iter = r.pr->max_iter;
+      it.polish = J.polish;
+      const int ntile = (bound[m] + 15) / 16;
+      const int groups = (ntile + tpw - 1) / tpw;
+      it.nwg = groups * it.td.nchunk;
+      it.nub = bound[m];
+      // accumulate launch of this (width, kernel)
+      if (acc.empty() || acc.back().p != J.p || acc.back().kind != J.kind)
+        acc.push_back({J.p, J.kind, ni, 0, 0});
+      Launch& a = acc.back();
+      it.wg0 = a.total;
+      a.total += ((it.nwg + 7) / 8) * 8;
+      a.n += 1;
+      if (upd.empty() || upd.back().p != J.p) upd.push_back({J.p, 0, ni, 0, 0});
+      Launch& u = upd.back();
+      it.ub0 = u.total;
+      u.total += it.nub;
+      u.n += 1;
+      item_member.push_back(m);
+      ++ni;
+    }
+    DV_HIP(hipMemcpyAsync(dev, st, sizeof(dv::JointItem) * ni, hipMemcpyHostToDevice, s));
+    DV_HIP(dv::launch_fill_i32(s, B, M, 0));
+    std::vector<Rec> rk;
+    for (const Launch& a : acc) {
+      const size_t nrec = ctx->launches.size();
+      {
+        LaunchTimer lt(ctx, a.kind == dv::kJointWs ? DEVIL_KERNEL_IRLS_MIXED
+                                                   : DEVIL_KERNEL_IRLS_FP64,
+                       0.0);
+        DV_HIP(dv::launch_irls_accum_joint(s, a.kind, a.p, dev + a.first, a.n, a.total));
       }
-      c.ptr = cbuf.p;
-      c.ld = ld_pad;
-    } else {
-      c.ptr = static_cast<const char*>(dev_counts) + (size_t)g0 * dev_ld * es;
-      c.ld = dev_ld;
+      Rec rec{ctx->launches.size() > nrec ? (long)nrec : -1, {}};
+      for (int q = a.first; q < a.first + a.n; ++q) rec.members.push_back(item_member[q]);
+      rk.push_back(rec);
     }
-    double* a0 = alpha0.as<double>() + g0;
-    double* b0 = beta.as<double>() + g0 * p;
-    ev = tm.begin(DEVIL_STAGE_INIT);
-    if (run_init)
-      DV_HIP(dv::launch_init_tiled(s, tl.td, ih.it, chunks.ct, c, Gb, n, p, post.as<double>(),
-                                   a0, b0));
-    DV_HIP(dv::launch_k_from_alpha(s, a0, Gb, kvec.as<double>() + g0));
-    tm.end(ev);
-    if (autotune) {
-      if ((rc = autotune_mixed(ctx, dd, n, p, c, Gb, b0, kvec.as<double>() + g0, mk, cc)))
-        return rc;
-      ctx->tuned[key] = mk | (int)(cc << 8);
-      autotune = false;
-      tl = TileHost();
-      if ((rc = make_tile(ctx, dd.x, dd.off.as<double>(), dd.eo.as<double>(), n, p, mk, tl, cc)))
-        return rc;
+    recs.push_back(rk);
+    for (const Launch& u : upd)
+      DV_HIP(dv::launch_irls_update_joint(s, u.p, dev + u.first, u.n, u.total));
+    DV_HIP(hipMemcpyAsync(const_cast<int32_t*>(slot) + (k & 1) * M, B, sizeof(int32_t) * M,
+                          hipMemcpyDeviceToHost, s));
+    DV_HIP(hipEventRecord(ctx->round_ev[k & 1], s));
+    std::swap(A, B);
+    for (int m = 0; m < M; ++m) std::swap(cur[m], nxt[m]);
+  }
+  DV_HIP(hipStreamSynchronize(s));
+  for (int j = std::max(0, k - 2); j < k; ++j)
+    if ((int)count.size() == j + 1) {
+      std::vector<int> ck(M);
+      for (int m = 0; m < M; ++m) ck[m] = (int)slot[(j & 1) * M + m];
+      count.push_back(ck);
     }
-    ctx->last_mk = mixed ? mk : kMixedNone;
-    ctx->last_cc = cc;
-    ev = tm.begin(DEVIL_STAGE_IRLS);
-    if ((rc = run_irls_tiled(ctx, tl, c, Gb, p, b0, kvec.as<double>() + g0, pr->max_iter,
-                             eps, iters.as<int32_t>() + g0, status.as<int32_t>() + g0)))
-      return rc;
-    tm.end(ev);
-    double* th = theta.as<double>() + g0;
-    ev = tm.begin(DEVIL_STAGE_DISPERSION);
-    if (pr->od_mode == DEVIL_OD_MOM) {
-      DV_HIP(dv::launch_mom_tiled(s, tl.td, chunks.ct, c, Gb, n, p, b0, post.as<double>(), th,
-                                  status.as<int32_t>() + g0));
-      DV_HIP(dv::launch_fill_i32(s, thit.as<int32_t>() + g0, Gb, 0));
-    } else if (pr->od_mode == DEVIL_OD_MLE) {
-      if ((rc = run_mle_tiled(ctx, tl.td, c, Gb, n, p, b0, pr->max_iter, pr->tolerance, th,
-                              thit.as<int32_t>() + g0, status.as<int32_t>() + g0)))
-        return rc;
-    } else if (pr->od_mode == DEVIL_OD_NEW) {
-      if ((rc = run_newod_tiled(ctx, tl.td, c, Gb, n, p, b0, pr->max_iter, pr->tolerance, th,
-                                thit.as<int32_t>() + g0, status.as<int32_t>() + g0)))
-        return rc;
-    } else {
-      DV_HIP(dv::launch_fill_f64(s, th, Gb, 0.0));
-      DV_HIP(dv::launch_fill_i32(s, thit.as<int32_t>() + g0, Gb, 0));
+  // exact bytes per launch; launches whose lists had all emptied are dropped
+  for (size_t j = 0; j < recs.size(); ++j)
+    for (const Rec& rec : recs[j]) {
+      if (rec.idx < 0) continue;
+      double bytes = 0.0;
+      for (int m : rec.members) bytes += ms[m]->row_bytes * (double)count[j][m];
+      ctx->launches[rec.idx].bytes = bytes;
+      if (bytes == 0.0) ctx->launches[rec.idx].kind = -1;
     }
-    tm.end(ev);
-    ev = tm.begin(DEVIL_STAGE_BREAD_MEAT);
-    // mixed precision: the bread on the bf16 tables (FP32 element math, FP64
-    // sums and inverse) where there is no meat to build
-    bool bread_done = false;
-    if (mixed && bf16_layout(tl.mixed) && C == 0 && counts_ws(c)) {
-      if ((rc = bread_mixed(ctx, tl, c, Gb, p, b0, th, bread.as<double>() + g0 * p * p,
-                            status.as<int32_t>() + g0, bread_done)))
-        return rc;
+  collect_launches(ctx);
+  return DEVIL_OK;
+}
+
+// Whether a problem's IRLS can join: one gene batch (device counts, or host
+// counts in one batch), the FP64 kernels or the LDS-staged mixed kernel.
+bool joint_eligible(const ProbRun& r) {
+  if (r.G == 0 || r.batch < r.G || r.pr->max_iter <= 0) return false;
+  if (!r.mixed) return true;
+  return r.tl.mixed == kMixedBf16Ws && counts_ws(r.c);
+}
+
+int run_irls_joint(devil_ctx* ctx, std::vector<ProbRun*>& runs) {
+  hipStream_t s = ctx->stream;
+  const int M = (int)runs.size();
+  if (M == 0) return DEVIL_OK;
+  std::vector<std::unique_ptr<JointMember>> mem(M);
+  std::vector<JointMember*> all;
+  for (int m = 0; m < M; ++m) {
+    mem[m].reset(new (std::nothrow) JointMember());
+    if (!mem[m]) return fail(ctx, DEVIL_E_NOMEM, "joint member");
+    JointMember& J = *mem[m];
+    ProbRun& r = *runs[m];
+    J.r = &r;
+    J.mixed = r.mixed;
+    J.p = r.p;
+    const int64_t G = r.G;
+    J.row_bytes = (double)count_elem_size(r.c.dtype) * (double)r.n;
+    const int64_t ntile = (G + 15) / 16;
+    size_t per_tile = (size_t)r.tl.td.nchunk * 16 * r.tl.td.ncp;
+    if (J.mixed) per_tile = std::max(per_tile, (size_t)r.tl.tdm.nchunk * 16 * r.tl.tdm.ncp);
+    J.part_cap = (size_t)ntile * per_tile;
+    DV_ALLOC(J.act0, sizeof(int32_t) * G);
+    DV_ALLOC(J.act1, sizeof(int32_t) * G);
+    DV_ALLOC(J.part, sizeof(double) * J.part_cap);
+    DV_HIP(dv::launch_fill_i32(s, r.iters.as<int32_t>(), G, 0));
+    DV_HIP(dv::launch_fill_i32(s, r.status.as<int32_t>(), G, 0));
+    DV_HIP(dv::launch_iota(s, J.act0.as<int32_t>(), (int)G));
+    if (J.mixed) {
+      DV_ALLOC(J.rep, sizeof(int32_t) * G);
+      DV_ALLOC(J.beta0, sizeof(double) * G * r.p);
+      DV_ALLOC(J.bprev, sizeof(double) * G * r.p);
+      DV_HIP(hipMemcpyAsync(J.beta0.p, r.beta.p, sizeof(double) * G * r.p,
+                            hipMemcpyDeviceToDevice, s));
+      DV_HIP(dv::launch_fill_f64(s, J.bprev.as<double>(), G * r.p, std::nan("")));
     }
-    if (!bread_done)
-      DV_HIP(dv::launch_bread_tiled(s, tl.td, chunks.ct, c, Gb, n, p, b0, th, C,
-                                    post.as<double>(), bread.as<double>() + g0 * p * p,
-                                    want_sw ? sandwich.as<double>() + g0 * p * p : nullptr,
-                                    status.as<int32_t>() + g0));
-    tm.end(ev);
+    J.kind = J.mixed ? dv::kJointWs : dv::kJointFp64;
+    J.tfull = J.mixed ? r.tl.tdm : r.tl.td;
+    J.cur = J.act0.as<int32_t>();
+    J.nxt = J.act1.as<int32_t>();
+    J.nA = (int)G;
+    J.band = J.mixed ? kMixedBand : 0.0;
+    J.abeta = r.beta.as<double>();
+    all.push_back(&J);
+  }
+  // counters: 2 M list counts, M set-aside counts
+  DevBuf cnt;
+  DV_ALLOC(cnt, sizeof(int32_t) * 3 * M);
+  int32_t* nrep = cnt.as<int32_t>() + 2 * M;
+  std::vector<int32_t> h(M);
+  for (int m = 0; m < M; ++m) h[m] = (int32_t)all[m]->nA;
+  DV_HIP(hipMemcpyAsync(cnt.p, h.data(), sizeof(int32_t) * M, hipMemcpyHostToDevice, s));
+  DV_HIP(dv::launch_fill_i32(s, nrep, M, 0));
+  int max_rounds = 0;
+  for (auto* J : all) max_rounds = std::max(max_rounds, J->r->pr->max_iter);
+  int rc;
+  if ((rc = joint_rounds(ctx, all, max_rounds, cnt.as<int32_t>(), nrep))) return rc;
+
+  // FP64 refit of the genes the mixed rounds set aside, from their starting beta
+  std::vector<int32_t> nr(M, 0);
+  DV_HIP(hipMemcpyAsync(nr.data(), nrep, sizeof(int32_t) * M, hipMemcpyDeviceToHost, s));
+  DV_HIP(hipStreamSynchronize(s));
+  std::vector<JointMember*> refit;
+  for (int m = 0; m < M; ++m) {
+    JointMember& J = *all[m];
+    if (!J.mixed || nr[m] <= 0) continue;
+    ProbRun& r = *J.r;
+    ctx->replayed += nr[m];
+    DV_HIP(dv::launch_restart_genes(s, J.rep.as<int32_t>(), nr[m], r.p, J.beta0.as<double>(),
+                                    r.beta.as<double>(), r.iters.as<int32_t>(),
+                                    r.status.as<int32_t>(), J.bprev.as<double>()));
+    J.kind = dv::kJointFp64;
+    J.tfull = r.tl.td;
+    J.cur = J.rep.as<int32_t>();
+    J.nxt = J.act1.as<int32_t>();
+    J.nA = nr[m];
+    J.band = 0.0;
+    refit.push_back(&J);
+  }
+  if (!refit.empty()) {
+    const int R = (int)refit.size();
+    for (int q = 0; q < R; ++q) h[q] = (int32_t)refit[q]->nA;
+    DV_HIP(hipMemcpyAsync(cnt.p, h.data(), sizeof(int32_t) * R, hipMemcpyHostToDevice, s));
+    int mr = 0;
+    for (auto* J : refit) mr = std::max(mr, J->r->pr->max_iter);
+    if ((rc = joint_rounds(ctx, refit, mr, cnt.as<int32_t>(), nrep))) return rc;
   }
 
-  // outputs: beta back to R's column-major layout
-  DevBuf bcol;
-  DV_ALLOC(bcol, sizeof(double) * G * p);
-  DV_HIP(dv::launch_transpose_gp(s, beta.as<double>(), G, p, 0, bcol.as<double>()));
-  const hipMemcpyKind kind =
-      mem == DEVIL_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
-  DV_HIP(hipMemcpyAsync(out->beta, bcol.p, sizeof(double) * G * p, kind, s));
-  DV_HIP(hipMemcpyAsync(out->theta, theta.p, sizeof(double) * G, kind, s));
-  DV_HIP(hipMemcpyAsync(out->beta_iters, iters.p, sizeof(int32_t) * G, kind, s));
-  DV_HIP(hipMemcpyAsync(out->theta_iters, thit.p, sizeof(int32_t) * G, kind, s));
-  DV_HIP(hipMemcpyAsync(out->bread, bread.p, sizeof(double) * G * p * p, kind, s));
-  if (want_sw)
-    DV_HIP(hipMemcpyAsync(out->sandwich, sandwich.p, sizeof(double) * G * p * p, kind, s));
-  if (out->status)
-    DV_HIP(hipMemcpyAsync(out->status, status.p, sizeof(int32_t) * G, kind, s));
+  // The mixed trajectory leaves ~5e-7 absolute on beta: genes with a small
+  // coefficient get their last step redone in FP64 from the beta it started at
+  std::vector<JointMember*> mixed;
+  for (auto* J : all)
+    if (J->mixed) mixed.push_back(J);
+  if (mixed.empty()) return DEVIL_OK;
+  const int X = (int)mixed.size();
+  DV_HIP(dv::launch_fill_i32(s, cnt.as<int32_t>(), X, 0));
+  for (int q = 0; q < X; ++q) {
+    JointMember& J = *mixed[q];
+    ProbRun& r = *J.r;
+    DV_HIP(dv::launch_polish_select(s, r.G, r.p, r.beta.as<double>(), J.bprev.as<double>(),
+                                    r.status.as<int32_t>(), kPolishBelow, J.act0.as<int32_t>(),
+                                    cnt.as<int32_t>() + q));
+  }
+  std::vector<int32_t> np(X, 0);
+  DV_HIP(hipMemcpyAsync(np.data(), cnt.p, sizeof(int32_t) * X, hipMemcpyDeviceToHost, s));
   DV_HIP(hipStreamSynchronize(s));
+  std::vector<JointMember*> pol;
+  for (int q = 0; q < X; ++q) {
+    JointMember& J = *mixed[q];
+    if (np[q] <= 0) continue;
+    ctx->polished += np[q];
+    J.kind = dv::kJointFp64;
+    J.tfull = J.r->tl.td;
+    J.cur = J.act0.as<int32_t>();
+    J.nxt = J.act1.as<int32_t>();
+    J.nA = np[q];
+    J.band = 0.0;
+    J.polish = 1;
+    J.abeta = J.bprev.as<double>();
+    pol.push_back(&J);
+  }
+  if (pol.empty()) return DEVIL_OK;
+  const int Pn = (int)pol.size();
+  for (int q = 0; q < Pn; ++q) h[q] = (int32_t)pol[q]->nA;
+  DV_HIP(hipMemcpyAsync(cnt.p, h.data(), sizeof(int32_t) * Pn, hipMemcpyHostToDevice, s));
+  return joint_rounds(ctx, pol, 1, cnt.as<int32_t>(), nrep);
+}
+
+int fit_one(devil_ctx* ctx, const devil_problem* pr, devil_outputs* out) {
+  ProbRun r;
+  int rc;
+  if ((rc = run_setup(ctx, pr, out, r))) return rc;
+  if (r.G == 0) return DEVIL_OK;
+  reset_fit_stats(ctx);
+  StageTimer tm(ctx);
+  for (int64_t g0 = 0; g0 < r.G; g0 += r.batch) {
+    if ((rc = run_batch_init(ctx, r, g0, tm))) return rc;
+    if ((rc = run_batch_irls(ctx, r, tm))) return rc;
+    if ((rc = run_batch_post(ctx, r, tm))) return rc;
+  }
+  if ((rc = run_outputs(ctx, r))) return rc;
   tm.collect();
+  return DEVIL_OK;
+}
+
+// Per-problem figures of a sub-context folded into its sums (each phase of a
+// problem starts from reset_fit_stats).
+void fold_sub_stats(devil_ctx* sub, const double* st_before) {
+  sub->replay_sum += sub->replayed;
+  sub->polish_sum += sub->polished;
+  for (int k = 0; k < DEVIL_N_STAGES; ++k) sub->stage_ms[k] += st_before[k];
+  for (int k = 0; k < DEVIL_N_KERNELS; ++k) {
+    sub->k_sum_n[k] += sub->k_n[k];
+    sub->k_sum_ms[k] += sub->k_ms[k];
+    sub->k_sum_bytes[k] += sub->k_bytes[k];
+  }
+}
+
+// devil_fit of several problems with joint IRLS rounds.  The problems are
+// cut into units of equal design width (at most about n_problems / 2S
+// problems each, largest first); S host threads, one sub-context and stream
+// each, take units in turn.  Per unit: set-up and init pass of its problems
+// (a problem that cannot join -- several gene batches, a mixed kernel other
+// than the LDS-staged one -- is fitted whole there), the joint IRLS of the
+// others, then their overdispersion, bread and outputs.  A unit's state lives
+// in its thread's sub-context only.
+int fit_many_joint(devil_ctx* ctx, const devil_problem* probs, devil_outputs* outs,
+                   int n_problems, int S, const std::vector<int>& order) {
+  // units: problems of one width, in decreasing size, cut to at most U each
+  const int U = ctx->tune[DEVIL_TUNE_JOINT] > 1
+                    ? ctx->tune[DEVIL_TUNE_JOINT]
+                    : std::max(1, (n_problems + 2 * S - 1) / (2 * S));
+  std::vector<std::vector<int>> units;
+  {
+    std::map<int, std::vector<int>> byp;
+    for (int j : order) byp[probs[j].p].push_back(j);
+    for (auto& kv : byp)
+      for (size_t a = 0; a < kv.second.size(); a += U)
+        units.emplace_back(kv.second.begin() + a,
+                           kv.second.begin() + std::min(kv.second.size(), a + U));
+    std::stable_sort(units.begin(), units.end(), [&](const std::vector<int>& a,
+                                                     const std::vector<int>& b) {
+      auto work = [&](const std::vector<int>& v) {
+        double w = 0.0;
+        for (int j : v) w += (double)probs[j].G * (double)probs[j].n;
+        return w;
+      };
+      return work(a) > work(b);
+    });
+  }
+  std::vector<int> rcs(n_problems, DEVIL_OK);
+  std::vector<std::string> errs(n_problems);
+  std::atomic<int> next{0};
+  std::atomic<int64_t> joined_total{0};
+  for (int t = 0; t < S; ++t) {
+    devil_ctx* sub = ctx->subs[t];
+    std::memcpy(sub->tune, ctx->tune, sizeof(ctx->tune));
+    // the joint mixed rounds run the LDS-staged kernel (the autotuner then
+    // picks its chunk size only)
+    if (sub->tune[DEVIL_TUNE_MIXED_KERNEL] == DEVIL_MIXED_AUTO)
+      sub->tune[DEVIL_TUNE_MIXED_KERNEL] = DEVIL_MIXED_BF16_WS;
+    sub->tuned = ctx->tune[DEVIL_TUNE_MIXED_KERNEL] == DEVIL_MIXED_AUTO ? ctx->tuned_joint
+                                                                          : ctx->tuned;
+    sub->timing = ctx->timing;
+    for (int k = 0; k < DEVIL_N_STAGES; ++k) sub->stage_ms[k] = 0.0;
+  }
+  std::vector<std::thread> th;
+  for (int t = 0; t < S; ++t) {
+    devil_ctx* sub = ctx->subs[t];
+    th.emplace_back([&, sub] {
+      for (int u; (u = next.fetch_add(1)) < (int)units.size();) {
+        const std::vector<int>& unit = units[u];
+        double st[DEVIL_N_STAGES];
+        for (int k = 0; k < DEVIL_N_STAGES; ++k) st[k] = sub->stage_ms[k];
+        reset_fit_stats(sub);
+        StageTimer tm(sub);
+        std::vector<std::unique_ptr<ProbRun>> runs;
+        std::vector<int> jids;
+        std::vector<ProbRun*> jr;
+        int rc = DEVIL_OK, bad = -1;
+        for (int j : unit) {
+          sub->err.clear();
+          // a unit's problems stay alive together: only its first (largest)
+          // may read the sub-context's padded copy
+          sub->tune[DEVIL_TUNE_PAD_COPY] = j == unit.front() ? ctx->tune[DEVIL_TUNE_PAD_COPY] : 0;
+          std::unique_ptr<ProbRun> r(new (std::nothrow) ProbRun());
+          if (!r) rc = fail(sub, DEVIL_E_NOMEM, "problem state");
+          else rc = run_setup(sub, &probs[j], &outs[j], *r);
+          if (rc == DEVIL_OK && r->G > 0) {
+            if (r->batch < r->G) {
+              for (int64_t g0 = 0; g0 < r->G && rc == DEVIL_OK; g0 += r->batch) {
+                if ((rc = run_batch_init(sub, *r, g0, tm))) break;
+                if ((rc = run_batch_irls(sub, *r, tm))) break;
+                rc = run_batch_post(sub, *r, tm);
+              }
+              if (rc == DEVIL_OK) rc = run_outputs(sub, *r);
+            } else if ((rc = run_batch_init(sub, *r, 0, tm)) == DEVIL_OK) {
+              if (joint_eligible(*r)) {
+                jids.push_back(j);
+                jr.push_back(r.get());
+                runs.push_back(std::move(r));
+              } else if ((rc = run_batch_irls(sub, *r, tm)) == DEVIL_OK &&
+                         (rc = run_batch_post(sub, *r, tm)) == DEVIL_OK) {
+                rc = run_outputs(sub, *r);
+              }
+            }
+          }
+          if (rc != DEVIL_OK) {
+            bad = j;
+            break;
+          }
+        }
+        if (rc == DEVIL_OK && !jr.empty()) {
+          joined_total += (int64_t)jr.size();
+          hipEvent_t ev = tm.begin(DEVIL_STAGE_IRLS);
+          sub->err.clear();
+          rc = run_irls_joint(sub, jr);
+          tm.end(ev);
+          if (rc != DEVIL_OK) bad = jids.front();
+          for (size_t q = 0; q < jr.size() && rc == DEVIL_OK; ++q) {
+            sub->err.clear();
+            if ((rc = run_batch_post(sub, *jr[q], tm)) == DEVIL_OK) rc = run_outputs(sub, *jr[q]);
+            if (rc != DEVIL_OK) bad = jids[q];
+          }
+        }
+        if (rc == DEVIL_OK && hipStreamSynchronize(sub->stream) != hipSuccess)
+          rc = fail(sub, DEVIL_E_HIP, "hipStreamSynchronize");
+        if (rc == DEVIL_OK) tm.collect();
+        fold_sub_stats(sub, st);
+        runs.clear();
+        if (rc != DEVIL_OK) {
+          if (bad < 0) bad = unit.front();
+          rcs[bad] = rc;
+          errs[bad] = sub->err;
+          return;
+        }
+      }
+    });
+  }
+  for (auto& x : th) x.join();
+  ctx->joined = joined_total.load();
+  for (int j = 0; j < n_problems; ++j)
+    if (rcs[j] != DEVIL_OK) {
+      ctx->err = "problem " + std::to_string(j) + ": " + errs[j];
+      return rcs[j];
+    }
   return DEVIL_OK;
 }
 
@@ -1796,6 +2363,7 @@ int devil_fit(devil_ctx* ctx, const devil_problem* probs, devil_outputs* outs,
   }
   ctx->replayed = 0;
   ctx->polished = 0;
+  ctx->joined = 0;
   if (n_problems == 1) return fit_one(ctx, probs, outs);
   const int S = std::max(1, std::min<int>(n_problems, ctx->tune[DEVIL_TUNE_CONCURRENCY]));
   while ((int)ctx->subs.size() < S) {
@@ -1813,6 +2381,32 @@ int devil_fit(devil_ctx* ctx, const devil_problem* probs, devil_outputs* outs,
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
     return probs[a].G * probs[a].n > probs[b].G * probs[b].n;
   });
+  if (ctx->tune[DEVIL_TUNE_JOINT]) {
+    const int jrc = fit_many_joint(ctx, probs, outs, n_problems, S, order);
+    for (int t = 0; t < S; ++t) {
+      devil_ctx* sub = ctx->subs[t];
+      for (int k = 0; k < DEVIL_STAGE_TEST_DE; ++k) ctx->stage_ms[k] += sub->stage_ms[k];
+      for (int k = 0; k < DEVIL_N_KERNELS; ++k) {
+        ctx->k_n[k] += sub->k_sum_n[k];
+        ctx->k_ms[k] += sub->k_sum_ms[k];
+        ctx->k_bytes[k] += sub->k_sum_bytes[k];
+        sub->k_sum_n[k] = 0;
+        sub->k_sum_ms[k] = sub->k_sum_bytes[k] = 0.0;
+      }
+      // choices made with the kernel fixed to the joint rounds' stay apart
+      auto& into = ctx->tune[DEVIL_TUNE_MIXED_KERNEL] == DEVIL_MIXED_AUTO ? ctx->tuned_joint
+                                                                          : ctx->tuned;
+      for (auto& kv : sub->tuned) into[kv.first] = kv.second;
+      ctx->replayed += sub->replay_sum;
+      ctx->polished += sub->polish_sum;
+      sub->replay_sum = 0;
+      sub->polish_sum = 0;
+      if (sub->pad_p) (void)hipFree(sub->pad_p);
+      sub->pad_p = nullptr;
+      sub->pad_size = 0;
+    }
+    return jrc;
+  }
   std::atomic<int> next{0};
   std::vector<int> rcs(n_problems, DEVIL_OK);
   std::vector<std::string> errs(n_problems);
@@ -1903,6 +2497,8 @@ int devil_set_tuning(devil_ctx* ctx, int32_t key, int64_t value) {
     return fail(ctx, DEVIL_E_ARG, "unknown MLE driver");
   if (key == DEVIL_TUNE_PAD_COPY && value != 0 && value != 1)
     return fail(ctx, DEVIL_E_ARG, "pad copy must be 0 or 1");
+  if (key == DEVIL_TUNE_JOINT && (value < 0 || value > 4096))
+    return fail(ctx, DEVIL_E_ARG, "joint must be 0 (off), 1 (auto) or a unit size <= 4096");
   if (key == DEVIL_TUNE_PAD_COPY && value == 0 && ctx->pad_p) {
     (void)hipSetDevice(ctx->device);
     DV_HIP(hipFree(ctx->pad_p));
@@ -1927,6 +2523,7 @@ int devil_fit_info(const devil_ctx* ctx, int32_t key, int64_t* value) {
     case DEVIL_INFO_CHUNK_CELLS: *value = ctx->last_cc; return DEVIL_OK;
     case DEVIL_INFO_POLISHED: *value = ctx->polished; return DEVIL_OK;
     case DEVIL_INFO_ALIGNED_COPY: *value = ctx->aligned_copy; return DEVIL_OK;
+    case DEVIL_INFO_JOINED: *value = ctx->joined; return DEVIL_OK;
     default: return DEVIL_E_ARG;
   }
 }

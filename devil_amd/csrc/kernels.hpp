@@ -184,6 +184,36 @@ hipError_t launch_polish_select(hipStream_t s, int64_t G, int p, const double* b
                                 int32_t* list, int32_t* nlist);
 hipError_t launch_iota(hipStream_t s, int32_t* p, int n);
 
+// One problem's share of a joint round (kernels_tile.hip): its round shape,
+// counts, active list and outputs, its workgroups of the accumulate grid and
+// its blocks of the update grid.
+struct JointItem {
+  TileDesign td;
+  Counts c;
+  const int32_t* active;
+  const int32_t* nactive;
+  double* beta;         // beta the accumulate reads (polish: beta before the last step)
+  const double* kvec;
+  double* part;
+  double* ubeta;        // beta the update writes
+  int32_t* iters;
+  int32_t* status;
+  int32_t* next;
+  int32_t* nnext;
+  int32_t* recheck;
+  int32_t* nrecheck;
+  double* bprev;
+  double eps, band;
+  int max_iter, polish;
+  int wg0, nwg;         // accumulate workgroups [wg0, wg0 + nwg)
+  int ub0, nub;         // update blocks [ub0, ub0 + nub)
+};
+enum { kJointFp64 = 0, kJointWs = 1 };
+hipError_t launch_irls_accum_joint(hipStream_t s, int kind, int p, const JointItem* items,
+                                   int nitems, int total_wg);
+hipError_t launch_irls_update_joint(hipStream_t s, int p, const JointItem* items, int nitems,
+                                    int total_ub);
+
 struct PortState;   // port_driver.hpp
 struct NewodState;  // newod_driver.hpp
 

@@ -988,7 +988,7 @@ __device__ __forceinline__ void ws_body(const TileDesign& td, const Counts& c,
                                         const int32_t* __restrict__ nactive,
                                         const double* __restrict__ beta,
                                         const double* __restrict__ gpar,
-                                        double* __restrict__ part) {
+                                        double* __restrict__ part, int wb, int nwb) {
   constexpr int NE = eta_blocks(P);
   constexpr int NTH = ((P * (P + 1)) / 2 + 15) / 16;
   typedef WsLayout<NE, NTH> Lay;
@@ -999,7 +999,7 @@ __device__ __forceinline__ void ws_body(const TileDesign& td, const Counts& c,
   const int ntile = (nA + 15) / 16;
   const int wave = threadIdx.x >> 6;
   int chunk, group;
-  tile_coords(td.nchunk, chunk, group);
+  tile_coords_at(wb, nwb, td.nchunk, chunk, group);
   if (group * kWsWaves * TPW >= ntile) return;  // the whole workgroup
   const int64_t s0 = (int64_t)chunk * td.chunk_blocks;
   const int64_t s1 = max(s0, min((int64_t)(chunk + 1) * td.chunk_blocks, td.nblk));
@@ -1035,7 +1035,7 @@ __global__ __launch_bounds__(64 * (ws_waves(P) + 1)) void k_irls_accum_ws(
     TileDesign td, Counts c, const int32_t* __restrict__ active,
     const int32_t* __restrict__ nactive, const double* __restrict__ beta,
     const double* __restrict__ kvec, double* __restrict__ part) {
-  ws_body<P, 0, 1>(td, c, active, nactive, beta, kvec, part);
+  ws_body<P, 0, 1>(td, c, active, nactive, beta, kvec, part, blockIdx.x, gridDim.x);
 }
 
 // The same with two 16-gene tiles per compute wave: half the waves, each
@@ -1046,7 +1046,7 @@ __global__ __launch_bounds__(64 * (ws_waves(P, 2) + 1)) void k_irls_accum_ws2(
     TileDesign td, Counts c, const int32_t* __restrict__ active,
     const int32_t* __restrict__ nactive, const double* __restrict__ beta,
     const double* __restrict__ kvec, double* __restrict__ part) {
-  ws_body<P, 0, 2>(td, c, active, nactive, beta, kvec, part);
+  ws_body<P, 0, 2>(td, c, active, nactive, beta, kvec, part, blockIdx.x, gridDim.x);
 }
 
 // Mixed-precision bread partials: per (16-gene tile, chunk) the sums of
@@ -1057,7 +1057,7 @@ __global__ __launch_bounds__(64 * (ws_waves(P) + 1)) void k_bread_accum_ws(
     TileDesign td, Counts c, const int32_t* __restrict__ genes,
     const int32_t* __restrict__ ngenes, const double* __restrict__ beta,
     const double* __restrict__ theta, double* __restrict__ part) {
-  ws_body<P, 1, 1>(td, c, genes, ngenes, beta, theta, part);
+  ws_body<P, 1, 1>(td, c, genes, ngenes, beta, theta, part, blockIdx.x, gridDim.x);
 }
 
 // bread = H^-1 with H[a][b] = the sum over chunks of the pair column of
@@ -1116,6 +1116,30 @@ __global__ void k_merge_flags(int32_t* __restrict__ status, const int32_t* __res
   if (g < G && flag[g]) status[g] |= bit;
 }
 
+// FP64 accumulate of workgroup wb of nwb (4 tiles, one per wave); e2: the
+// exponential table, filled by the caller
+template <int P>
+__device__ __forceinline__ void irls_accum_body(const TileDesign& td, const Counts& c,
+                                                const int32_t* __restrict__ active,
+                                                const int32_t* __restrict__ nactive,
+                                                const double* __restrict__ beta,
+                                                const double* __restrict__ kvec,
+                                                double* __restrict__ part, const double* e2,
+                                                int wb, int nwb) {
+  const int nA = *nactive;
+  const int ntile = (nA + 15) / 16;
+  const int wave = threadIdx.x >> 6;
+  int chunk, group;
+  tile_coords_at(wb, nwb, td.nchunk, chunk, group);
+  const int tile = group * 4 + wave;
+  if (tile >= ntile) return;
+  with_counts(c, [&](auto Y, auto vec) {
+    typedef std::remove_cv_t<std::remove_pointer_t<decltype(Y)>> CT;
+    irls_tile_chunk<P, CT, decltype(vec)::value>(td, Y, c.ld, active, nA, beta, kvec, tile,
+                                                 chunk, part, e2);
+  });
+}
+
 template <int P>
 __global__ __launch_bounds__(256) void k_irls_accum(TileDesign td, Counts c,
                                                     const int32_t* __restrict__ active,
@@ -1126,18 +1150,7 @@ __global__ __launch_bounds__(256) void k_irls_accum(TileDesign td, Counts c,
   __shared__ double e2[64];
   exp_tab_fill(e2, threadIdx.x, blockDim.x);
   __syncthreads();
-  const int nA = *nactive;
-  const int ntile = (nA + 15) / 16;
-  const int wave = threadIdx.x >> 6;
-  int chunk, group;
-  tile_coords(td.nchunk, chunk, group);
-  const int tile = group * 4 + wave;
-  if (tile >= ntile) return;
-  with_counts(c, [&](auto Y, auto vec) {
-    typedef std::remove_cv_t<std::remove_pointer_t<decltype(Y)>> CT;
-    irls_tile_chunk<P, CT, decltype(vec)::value>(td, Y, c.ld, active, nA, beta, kvec, tile,
-                                                 chunk, part, e2);
-  });
+  irls_accum_body<P>(td, c, active, nactive, beta, kvec, part, e2, blockIdx.x, gridDim.x);
 }
 
 // One workgroup per active gene.  Its threads sum the chunk partials in
@@ -1147,25 +1160,24 @@ __global__ __launch_bounds__(256) void k_irls_accum(TileDesign td, Counts c,
 constexpr int kUpdThreads = 256;
 constexpr int kUpdSlices = 16;
 
+// The update of active-list entry i (one workgroup of kUpdThreads).
 template <int P>
-__global__ __launch_bounds__(kUpdThreads) void k_irls_update(TileDesign td,
-                                                    const int32_t* __restrict__ active,
-                                                    const int32_t* __restrict__ nactive,
-                                                    const double* __restrict__ part,
-                                                    double* __restrict__ beta,
-                                                    const double* __restrict__ kvec,
-                                                    int max_iter, double eps,
-                                                    int32_t* __restrict__ iters,
-                                                    int32_t* __restrict__ status,
-                                                    int32_t* __restrict__ next,
-                                                    int32_t* __restrict__ nnext, double band,
-                                                    int32_t* __restrict__ recheck,
-                                                    int32_t* __restrict__ nrecheck,
-                                                    double* __restrict__ bprev, int polish) {
+__device__ __forceinline__ void irls_update_body(const TileDesign& td, int i,
+                                                 const int32_t* __restrict__ active,
+                                                 const int32_t* __restrict__ nactive,
+                                                 const double* __restrict__ part,
+                                                 double* __restrict__ beta,
+                                                 const double* __restrict__ kvec, int max_iter,
+                                                 double eps, int32_t* __restrict__ iters,
+                                                 int32_t* __restrict__ status,
+                                                 int32_t* __restrict__ next,
+                                                 int32_t* __restrict__ nnext, double band,
+                                                 int32_t* __restrict__ recheck,
+                                                 int32_t* __restrict__ nrecheck,
+                                                 double* __restrict__ bprev, int polish) {
   __shared__ double acc[DEVIL_NCP_MAX];
   __shared__ double slice[kUpdSlices][DEVIL_NCP_MAX];
   const int nA = *nactive;
-  const int i = blockIdx.x;
   if (i >= nA) return;
   const int lane = threadIdx.x;
   const int g = active[i];
@@ -1254,6 +1266,73 @@ __global__ __launch_bounds__(kUpdThreads) void k_irls_update(TileDesign td,
   if (!conv && it >= max_iter) st |= DEVIL_GENE_MAXITER;
   status[g] = st;
   if (!conv && it < max_iter) next[atomicAdd(nnext, 1)] = g;
+}
+
+template <int P>
+__global__ __launch_bounds__(kUpdThreads) void k_irls_update(TileDesign td,
+                                                    const int32_t* __restrict__ active,
+                                                    const int32_t* __restrict__ nactive,
+                                                    const double* __restrict__ part,
+                                                    double* __restrict__ beta,
+                                                    const double* __restrict__ kvec,
+                                                    int max_iter, double eps,
+                                                    int32_t* __restrict__ iters,
+                                                    int32_t* __restrict__ status,
+                                                    int32_t* __restrict__ next,
+                                                    int32_t* __restrict__ nnext, double band,
+                                                    int32_t* __restrict__ recheck,
+                                                    int32_t* __restrict__ nrecheck,
+                                                    double* __restrict__ bprev, int polish) {
+  irls_update_body<P>(td, blockIdx.x, active, nactive, part, beta, kvec, max_iter, eps, iters,
+                      status, next, nnext, band, recheck, nrecheck, bprev, polish);
+}
+
+// ---------------------------------------------------------------------------
+// Joint rounds: one accumulate launch (per design width and kernel) and one
+// update launch (per design width) per round for many problems.  Item q owns
+// workgroups [wg0, wg0 + nwg) of the accumulate grid (wg0 a multiple of 8, so
+// the XCD walk of tile_coords holds within each problem) and blocks
+// [ub0, ub0 + nub) of the update grid; items are in increasing wg0 and ub0.
+__device__ __forceinline__ int joint_find(const JointItem* __restrict__ it, int n, int b,
+                                          bool upd) {
+  int q = 0;
+  while (q + 1 < n && (upd ? it[q + 1].ub0 : it[q + 1].wg0) <= b) ++q;
+  return q;
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void k_irls_accum_joint(const JointItem* __restrict__ items,
+                                                          int nitems) {
+  __shared__ double e2[64];
+  exp_tab_fill(e2, threadIdx.x, blockDim.x);
+  __syncthreads();
+  const int b = blockIdx.x;
+  const JointItem& J = items[joint_find(items, nitems, b, false)];
+  const int wb = b - J.wg0;
+  if (wb >= J.nwg) return;  // padding to the next multiple of 8
+  irls_accum_body<P>(J.td, J.c, J.active, J.nactive, J.beta, J.kvec, J.part, e2, wb, J.nwg);
+}
+
+template <int P>
+__global__ __launch_bounds__(64 * (ws_waves(P) + 1)) void k_irls_accum_ws_joint(
+    const JointItem* __restrict__ items, int nitems) {
+  const int b = blockIdx.x;
+  const JointItem& J = items[joint_find(items, nitems, b, false)];
+  const int wb = b - J.wg0;
+  if (wb >= J.nwg) return;
+  ws_body<P, 0, 1>(J.td, J.c, J.active, J.nactive, J.beta, J.kvec, J.part, wb, J.nwg);
+}
+
+template <int P>
+__global__ __launch_bounds__(kUpdThreads) void k_irls_update_joint(
+    const JointItem* __restrict__ items, int nitems) {
+  const int b = blockIdx.x;
+  const JointItem& J = items[joint_find(items, nitems, b, true)];
+  const int i = b - J.ub0;
+  if (i >= J.nub) return;
+  irls_update_body<P>(J.td, i, J.active, J.nactive, J.part, J.ubeta, J.kvec, J.max_iter, J.eps,
+                      J.iters, J.status, J.next, J.nnext, J.band, J.recheck, J.nrecheck,
+                      J.bprev, J.polish);
 }
 
 // genes list[0..n): beta <- beta0, iteration count and status cleared
@@ -1359,6 +1438,27 @@ hipError_t launch_irls_accum_ws(hipStream_t s, const TileDesign& td, Counts c, i
 }
 
 int ws_tiles_per_workgroup(int p, int tpw) { return ws_waves(p, tpw) * tpw; }
+
+hipError_t launch_irls_accum_joint(hipStream_t s, int kind, int p, const JointItem* items,
+                                   int nitems, int total_wg) {
+  if (nitems <= 0 || total_wg <= 0) return hipSuccess;
+  if (kind == kJointWs) {
+    const int nw = ws_waves(p, 1);
+    DV_TILE_P(p, k_irls_accum_ws_joint, dim3((unsigned)total_wg), dim3(64 * (nw + 1)), s, items,
+              nitems);
+  } else {
+    DV_TILE_P(p, k_irls_accum_joint, dim3((unsigned)total_wg), dim3(256), s, items, nitems);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_irls_update_joint(hipStream_t s, int p, const JointItem* items, int nitems,
+                                    int total_ub) {
+  if (nitems <= 0 || total_ub <= 0) return hipSuccess;
+  DV_TILE_P(p, k_irls_update_joint, dim3((unsigned)total_ub), dim3(kUpdThreads), s, items,
+            nitems);
+  return hipGetLastError();
+}
 
 hipError_t launch_merge_flags(hipStream_t s, int32_t* status, const int32_t* flag, int64_t G,
                               int32_t bit) {

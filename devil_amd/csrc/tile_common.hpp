@@ -16,9 +16,11 @@ namespace dv {
 // the XCD b % 8 takes the chunks congruent to it and walks them chunk-major,
 // so the workgroups resident on one XCD share one chunk's operand tables in
 // that XCD's L2.
-__device__ __forceinline__ void tile_coords(int nchunk, int& chunk, int& group) {
-  const int b = blockIdx.x;
-  const int groups = gridDim.x / nchunk;
+// tile_coords_at: the same for workgroup b of nb (a range of a larger grid
+// that starts at a multiple of 8, as the joint launches lay them out).
+__device__ __forceinline__ void tile_coords_at(int b, int nb, int nchunk, int& chunk,
+                                               int& group) {
+  const int groups = nb / nchunk;
   if ((nchunk & 7) == 0) {
     const int x = b & 7, i = b >> 3;
     chunk = x + 8 * (i / groups);
@@ -27,6 +29,10 @@ __device__ __forceinline__ void tile_coords(int nchunk, int& chunk, int& group) 
     chunk = b % nchunk;
     group = b / nchunk;
   }
+}
+
+__device__ __forceinline__ void tile_coords(int nchunk, int& chunk, int& group) {
+  tile_coords_at(blockIdx.x, gridDim.x, nchunk, chunk, group);
 }
 
 // Counts are loaded in their storage type and converted where they are used:

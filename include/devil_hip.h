@@ -280,7 +280,14 @@ enum { DEVIL_TUNE_MIXED_KERNEL = 0,  /* DEVIL_MIXED_* (default AUTO) */
                                         are not 16-byte aligned reads a copy with padded rows,
                                         kept by the context for the next fit (freed by
                                         devil_destroy or by setting 0); 0: never */
-       DEVIL_N_TUNE = 5 };
+       DEVIL_TUNE_JOINT = 5,         /* 0 (default): devil_fit with several problems runs
+                                        each problem's IRLS rounds on its own (the problems
+                                        overlap on the streams); 1: jointly, in units of
+                                        problems of one design width (about n_problems / 2
+                                        streams problems each), one accumulate launch per
+                                        kernel and one update launch per round and unit;
+                                        N > 1: units of at most N problems */
+       DEVIL_N_TUNE = 6 };
 /* MLE overdispersion drivers.  NLMINB: the reference's stats::nlminb call
  * (R/overdispersion.R:53-73) restated for one parameter -- its trust-region
  * Newton steps and its stopping rules (relative function convergence at
@@ -323,8 +330,10 @@ int devil_kernel_stats(const devil_ctx *ctx, int32_t kernel, int64_t *launches, 
  * mixed kernel (DEVIL_MIXED_*, -1 for FP64) and chunk size it ran with. */
 enum { DEVIL_INFO_REPLAYED = 0, DEVIL_INFO_MIXED_KERNEL = 1, DEVIL_INFO_CHUNK_CELLS = 2,
        DEVIL_INFO_POLISHED = 3, /* genes whose last mixed IRLS step was redone in FP64 */
-       DEVIL_INFO_ALIGNED_COPY = 4 /* 1 if the mixed fit read a copy of device counts whose
-                                      rows it padded to 16-byte alignment */ };
+       DEVIL_INFO_ALIGNED_COPY = 4, /* 1 if the mixed fit read a copy of device counts whose
+                                       rows it padded to 16-byte alignment */
+       DEVIL_INFO_JOINED = 5 /* problems of the last multi-problem fit whose IRLS ran in the
+                                joint rounds */ };
 int devil_fit_info(const devil_ctx *ctx, int32_t key, int64_t *value);
 
 #ifdef __cplusplus
