@@ -65,7 +65,7 @@ struct devil_ctx {
   std::string err;
   int timing = 0;
   // kernel, chunk cells (0 auto), streams, MLE driver, aligned copies allowed
-  int tune[DEVIL_N_TUNE] = {DEVIL_MIXED_AUTO, 0, 8, DEVIL_MLE_NLMINB, 1, 0};
+  int tune[DEVIL_N_TUNE] = {DEVIL_MIXED_AUTO, 0, 8, DEVIL_MLE_NLMINB, 1, 0, 0};
   std::map<uint64_t, int> tuned;      // autotuner choices by problem shape
   std::map<uint64_t, int> tuned_joint;  // the same for the joint rounds (kernel fixed)
   std::vector<devil_ctx*> subs;       // per-stream contexts of multi-problem fits
@@ -972,6 +972,7 @@ int round_sync_setup(devil_ctx* ctx) {
 // the GPU always has the next round queued.  A round queued for a list that
 // has meanwhile emptied exits at once (the kernels read the count on the
 // device).  The launch records get the exact count of their round.
+
 int irls_rounds(devil_ctx* ctx, const TileHost& tl, bool mixed, dv::Counts c, int p,
                 double* beta, const double* kvec, int max_iter, double eps, int32_t* iters,
                 int32_t* status, int32_t* act, int32_t* act_spare, int32_t* ncnt,
@@ -979,6 +980,7 @@ int irls_rounds(devil_ctx* ctx, const TileHost& tl, bool mixed, dv::Counts c, in
                 int32_t* nreplay, double* bprev = nullptr) {
   const dv::TileDesign& tfull = mixed ? tl.tdm : tl.td;
   const int tiles_per_wg = accum_tiles_per_wg(mixed ? tl.mixed : kMixedNone, p, c);
+  const int kind = mixed ? DEVIL_KERNEL_IRLS_MIXED : DEVIL_KERNEL_IRLS_FP64;
   hipStream_t s = ctx->stream;
   int rc;
   if ((rc = round_sync_setup(ctx))) return rc;
@@ -990,9 +992,40 @@ int irls_rounds(devil_ctx* ctx, const TileHost& tl, bool mixed, dv::Counts c, in
   volatile int32_t* slot = ctx->round_counts;
   DevBuf red;
   DV_ALLOC(red, sizeof(double) * kSplitTiles * kSplitSlices * 16 * tfull.ncp);
+  // one round over list (a, na) into (b, nb), its next count copied to *out
+  auto queue_round = [&](const dv::TileDesign& td, int bound, int32_t* a, int32_t* na,
+                         int32_t* b, int32_t* nb, int32_t* out) -> int {
+    if (!mixed)
+      DV_HIP(dv::launch_irls_accum(s, td, c, p, a, na, bound, beta, kvec, part));
+    else
+      DV_HIP(launch_mixed_accum(tl.mixed, s, td, c, p, a, na, bound, beta, kvec, part));
+    DV_HIP(dv::launch_fill_i32(s, nb, 1, 0));
+    const int ntile_b = (bound + 15) / 16;
+    if (ntile_b <= kSplitTiles && td.nchunk > kSplitSlices) {
+      DV_HIP(dv::launch_irls_reduce(s, td.nchunk, td.ncp, kSplitSlices, ntile_b, na, part,
+                                    red.as<double>()));
+      dv::TileDesign tu = td;
+      tu.nchunk = kSplitSlices;
+      DV_HIP(dv::launch_irls_update(s, tu, p, a, na, bound, red.as<double>(), beta, kvec,
+                                    max_iter, eps, iters, status, b, nb,
+                                    mixed ? kMixedBand : 0.0, replay, nreplay, bprev));
+    } else {
+      DV_HIP(dv::launch_irls_update(s, td, p, a, na, bound, part, beta, kvec, max_iter, eps,
+                                    iters, status, b, nb, mixed ? kMixedBand : 0.0, replay,
+                                    nreplay, bprev));
+    }
+    DV_HIP(hipMemcpyAsync(out, nb, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    return DEVIL_OK;
+  };
   std::vector<int> count{nA};       // count[k]: genes of round k's list
   std::vector<long> rec;            // round k's launch record (-1: none)
+  struct GraphRec {
+    long idx;
+    int r0;                         // its first round
+  };
+  std::vector<GraphRec> grec;       // graph replays: two rounds each
   int k = 0;
+  bool graphed = false;
   for (; k < max_iter; ++k) {
     if (k >= 2) {
       DV_HIP(hipEventSynchronize(ctx->round_ev[k & 1]));
@@ -1000,10 +1033,82 @@ int irls_rounds(devil_ctx* ctx, const TileHost& tl, bool mixed, dv::Counts c, in
     }
     const int bound = k == 0 ? nA : count[k - 1];
     if (bound <= 0) break;
+    // Once the list fits one workgroup column of tiles, round_shape no longer
+    // depends on its length: the rest of the rounds can replay one graph.
+    const int graph_genes = 16 * tiles_per_wg;
+    if (k >= 2 && ctx->tune[DEVIL_TUNE_GRAPH] && bound <= graph_genes && max_iter - k >= 4) {
+      // The tail: rounds k, k + 1, ... as replays of two captured rounds
+      // (the list and counter buffers swap twice per replay, so one graph
+      // serves every replay), with the grid and chunking of round k.  Two
+      // graphs that differ only in their count slots alternate, so a
+      // replay's counts are read while the next one runs.
+      DV_HIP(hipEventSynchronize(ctx->round_ev[(k - 1) & 1]));
+      count.push_back((int)slot[(k - 1) & 1]);  // round k's list
+      const int gb = count[k];
+      if (gb <= 0) break;
+      const dv::TileDesign td = round_shape(tfull, gb, tiles_per_wg, part_cap);
+      hipGraph_t g[2] = {nullptr, nullptr};
+      hipGraphExec_t ge[2] = {nullptr, nullptr};
+      auto cleanup = [&]() {
+        for (int q = 0; q < 2; ++q) {
+          if (ge[q]) (void)hipGraphExecDestroy(ge[q]);
+          if (g[q]) (void)hipGraphDestroy(g[q]);
+        }
+      };
+      for (int q = 0; q < 2 && rc == DEVIL_OK; ++q) {
+        int32_t* sl = ctx->round_counts + 2 + 2 * q;
+        if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+          rc = fail(ctx, DEVIL_E_HIP, "hipStreamBeginCapture");
+          break;
+        }
+        int qrc = queue_round(td, gb, cur, ncur, nxt, nnxt, sl);
+        if (qrc == DEVIL_OK) qrc = queue_round(td, gb, nxt, nnxt, cur, ncur, sl + 1);
+        const hipError_t e = hipStreamEndCapture(s, &g[q]);
+        if (qrc != DEVIL_OK) rc = qrc;
+        else if (e != hipSuccess) rc = fail(ctx, DEVIL_E_HIP, "hipStreamEndCapture");
+        else if (hipGraphInstantiate(&ge[q], g[q], nullptr, nullptr, 0) != hipSuccess)
+          rc = fail(ctx, DEVIL_E_HIP, "hipGraphInstantiate");
+      }
+      if (rc != DEVIL_OK) {
+        cleanup();
+        return rc;
+      }
+      auto launch = [&](int j) -> int {
+        const size_t nrec = ctx->launches.size();
+        {
+          LaunchTimer lt(ctx, kind, 0.0);
+          DV_HIP(hipGraphLaunch(ge[j & 1], s));
+        }
+        grec.push_back({ctx->launches.size() > nrec ? (long)nrec : -1, k + 2 * j});
+        DV_HIP(hipEventRecord(ctx->round_ev[j & 1], s));
+        return DEVIL_OK;
+      };
+      // replay j runs rounds k + 2j, k + 2j + 1; its slots hold the counts of
+      // rounds k + 2j + 1 and k + 2j + 2
+      const int nrep_max = (max_iter - k + 1) / 2;
+      int launched = 0;
+      for (; launched < std::min(2, nrep_max) && rc == DEVIL_OK; ++launched) rc = launch(launched);
+      for (int j = 0; rc == DEVIL_OK && j < launched; ++j) {
+        if (hipEventSynchronize(ctx->round_ev[j & 1]) != hipSuccess) {
+          rc = fail(ctx, DEVIL_E_HIP, "hipEventSynchronize");
+          break;
+        }
+        const int32_t* sl = ctx->round_counts + 2 + 2 * (j & 1);
+        count.push_back((int)sl[0]);
+        count.push_back((int)sl[1]);
+        if (sl[1] > 0 && launched < nrep_max) rc = launch(launched++);
+      }
+      if (rc == DEVIL_OK && hipStreamSynchronize(s) != hipSuccess)
+        rc = fail(ctx, DEVIL_E_HIP, "hipStreamSynchronize");
+      cleanup();
+      if (rc != DEVIL_OK) return rc;
+      graphed = true;
+      break;
+    }
     const dv::TileDesign td = round_shape(tfull, bound, tiles_per_wg, part_cap);
     const size_t nrec = ctx->launches.size();
     {
-      LaunchTimer lt(ctx, mixed ? DEVIL_KERNEL_IRLS_MIXED : DEVIL_KERNEL_IRLS_FP64, 0.0);
+      LaunchTimer lt(ctx, kind, 0.0);
       if (!mixed)
         DV_HIP(dv::launch_irls_accum(s, td, c, p, cur, ncur, bound, beta, kvec, part));
       else
@@ -1032,14 +1137,25 @@ int irls_rounds(devil_ctx* ctx, const TileHost& tl, bool mixed, dv::Counts c, in
     std::swap(ncur, nnxt);
   }
   DV_HIP(hipStreamSynchronize(s));
-  // counts of the lists the loop did not read (rounds k - 2 and k - 1)
-  for (int j = std::max(0, k - 2); j < k; ++j)
-    if ((int)count.size() == j + 1) count.push_back((int)slot[j & 1]);
+  if (!graphed) {
+    // counts of the lists the loop did not read (rounds k - 2 and k - 1)
+    for (int j = std::max(0, k - 2); j < k; ++j)
+      if ((int)count.size() == j + 1) count.push_back((int)slot[j & 1]);
+  }
   // exact bytes per launch; launches over an emptied list are dropped
   for (size_t j = 0; j < rec.size(); ++j) {
     if (rec[j] < 0) continue;
     ctx->launches[rec[j]].bytes = row_bytes * (double)count[j];
     if (count[j] == 0) ctx->launches[rec[j]].kind = -1;
+  }
+  // a graph replay: its two rounds' accumulate bytes over the whole replay
+  // (updates included in its time)
+  for (const GraphRec& g : grec) {
+    if (g.idx < 0) continue;
+    const int a = g.r0 < (int)count.size() ? count[g.r0] : 0;
+    const int b = g.r0 + 1 < (int)count.size() ? count[g.r0 + 1] : 0;
+    ctx->launches[g.idx].bytes = row_bytes * (double)(a + b);
+    if (a + b == 0) ctx->launches[g.idx].kind = -1;
   }
   collect_launches(ctx);
   return DEVIL_OK;
@@ -2497,6 +2613,8 @@ int devil_set_tuning(devil_ctx* ctx, int32_t key, int64_t value) {
     return fail(ctx, DEVIL_E_ARG, "unknown MLE driver");
   if (key == DEVIL_TUNE_PAD_COPY && value != 0 && value != 1)
     return fail(ctx, DEVIL_E_ARG, "pad copy must be 0 or 1");
+  if (key == DEVIL_TUNE_GRAPH && value != 0 && value != 1)
+    return fail(ctx, DEVIL_E_ARG, "graph must be 0 or 1");
   if (key == DEVIL_TUNE_JOINT && (value < 0 || value > 4096))
     return fail(ctx, DEVIL_E_ARG, "joint must be 0 (off), 1 (auto) or a unit size <= 4096");
   if (key == DEVIL_TUNE_PAD_COPY && value == 0 && ctx->pad_p) {

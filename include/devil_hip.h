@@ -287,7 +287,13 @@ enum { DEVIL_TUNE_MIXED_KERNEL = 0,  /* DEVIL_MIXED_* (default AUTO) */
                                         streams problems each), one accumulate launch per
                                         kernel and one update launch per round and unit;
                                         N > 1: units of at most N problems */
-       DEVIL_N_TUNE = 6 };
+       DEVIL_TUNE_GRAPH = 6,         /* 1: the IRLS rounds whose active genes fit one
+                                        column of workgroups (16 x tiles per workgroup) run
+                                        as replays of a captured two-round hipGraph
+                                        (accumulate, clear, update, count copy, twice);
+                                        0 (default): launched one by one, queued ahead of the
+                                        count reads (the same speed; per-kernel timing) */
+       DEVIL_N_TUNE = 7 };
 /* MLE overdispersion drivers.  NLMINB: the reference's stats::nlminb call
  * (R/overdispersion.R:53-73) restated for one parameter -- its trust-region
  * Newton steps and its stopping rules (relative function convergence at

@@ -555,3 +555,32 @@ def test_test_de_adjust_methods(dv, orc):
         np.testing.assert_array_equal(r["adj_pval"], orc.p_adjust(r["pval"], m), err_msg=m)
     with pytest.raises(ValueError):
         dv.test_de(fit, [0, 1], pval_adjust_method="sidak")
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_graph_tail_matches_launched_rounds(dv, orc, precision):
+    """With DEVIL_TUNE_GRAPH the IRLS rounds whose active genes fit one column
+    of workgroups replay a captured two-round hipGraph: the same kernels and
+    round shapes as rounds launched one by one, so the same results."""
+    from devil_amd import _lib as L
+    from devil_amd import synth
+    w = synth.make_workload("C3", G=600, n=6000, seed=33)
+    Y = synth.counts_host(w)
+    say(f"[case] graph tail vs launched rounds, {precision}")
+    ctx = dv.context(0)
+    fits = {}
+    try:
+        for gv in (1, 0):
+            ctx.set_tuning(L.TUNE_GRAPH, gv)
+            fits[gv] = dv.fit_devil(Y, w.X, size_factors=w.sf, precision=precision)
+    finally:
+        ctx.set_tuning(L.TUNE_GRAPH, 0)
+    np.testing.assert_array_equal(fits[1]["iterations"]["beta_iters"],
+                                  fits[0]["iterations"]["beta_iters"])
+    tol = 1e-10 if precision == "fp64" else 1e-4
+    floor = 1e-6 if precision == "fp64" else 1e-3
+    assert_rel(fits[1]["beta"], fits[0]["beta"], tol, floor, "beta")
+    if precision == "fp64":
+        ref = orc.cpu_fit(Y.astype(np.float64), w.X, w.offset)
+        np.testing.assert_array_equal(fits[1]["iterations"]["beta_iters"], ref["beta_iters"])
+        assert_rel(fits[1]["beta"], ref["beta"], 1e-8, 1e-6, "beta vs oracle")
