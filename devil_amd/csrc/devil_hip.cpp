@@ -97,10 +97,12 @@ struct devil_ctx {
   double k_bytes[DEVIL_N_KERNELS] = {};
   // IRLS round counts read back without a stream synchronisation
   int32_t* round_counts = nullptr;  // pinned host slots
+  int32_t* round_counts_dev = nullptr;  // the same slots as the device addresses them
   hipEvent_t round_ev[2] = {nullptr, nullptr};
   int64_t joined = 0;                 // problems of the last fit in the joint rounds
   // joint rounds of many problems: pinned counts and round descriptors
   void* joint_host = nullptr;
+  void* joint_host_dev = nullptr;
   size_t joint_host_bytes = 0;
 };
 
@@ -476,7 +478,9 @@ struct StageTimer {
   hipEvent_t begin(int stage) {
     if (!ctx->timing) return nullptr;
     hipEvent_t a = nullptr, b = nullptr;
-    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return nullptr;
+    if (hipEventCreateWithFlags(&a, hipEventDisableSystemFence) != hipSuccess ||
+        hipEventCreateWithFlags(&b, hipEventDisableSystemFence) != hipSuccess)
+      return nullptr;
     (void)hipEventRecord(a, ctx->stream);
     marks.push_back({stage, {a, b}});
     return b;
@@ -861,8 +865,10 @@ hipEvent_t pool_event(devil_ctx* ctx) {
     ctx->ev_pool.pop_back();
     return e;
   }
+  // timing only: no system-scope fence (no L2 writeback at every record,
+  // which would also lengthen the measured kernels)
   hipEvent_t e = nullptr;
-  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return nullptr;
   return e;
 }
 
@@ -955,6 +961,9 @@ int round_sync_setup(devil_ctx* ctx) {
     void* h = nullptr;
     DV_HIP(hipHostMalloc(&h, 64, hipHostMallocDefault));
     ctx->round_counts = static_cast<int32_t*>(h);
+    void* d = nullptr;
+    DV_HIP(hipHostGetDevicePointer(&d, h, 0));
+    ctx->round_counts_dev = static_cast<int32_t*>(d);
   }
   for (hipEvent_t& e : ctx->round_ev)
     if (!e) DV_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -1014,7 +1023,7 @@ int irls_rounds(devil_ctx* ctx, const TileHost& tl, bool mixed, dv::Counts c, in
                                     iters, status, b, nb, mixed ? kMixedBand : 0.0, replay,
                                     nreplay, bprev));
     }
-    DV_HIP(hipMemcpyAsync(out, nb, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    DV_HIP(dv::launch_copy_counts(s, nb, 1, ctx->round_counts_dev + (out - ctx->round_counts)));
     return DEVIL_OK;
   };
   std::vector<int> count{nA};       // count[k]: genes of round k's list
@@ -1130,8 +1139,7 @@ int irls_rounds(devil_ctx* ctx, const TileHost& tl, bool mixed, dv::Counts c, in
                                     eps, iters, status, nxt, nnxt, mixed ? kMixedBand : 0.0,
                                     replay, nreplay, bprev));
     }
-    DV_HIP(hipMemcpyAsync(ctx->round_counts + (k & 1), nnxt, sizeof(int32_t),
-                          hipMemcpyDeviceToHost, s));
+    DV_HIP(dv::launch_copy_counts(s, nnxt, 1, ctx->round_counts_dev + (k & 1)));
     DV_HIP(hipEventRecord(ctx->round_ev[k & 1], s));
     std::swap(cur, nxt);
     std::swap(ncur, nnxt);
@@ -1196,8 +1204,7 @@ int driver_rounds(devil_ctx* ctx, const dv::TileDesign& tfull, size_t part_cap, 
     rec.push_back(ctx->launches.size() > nrec ? (long)nrec : -1);
     DV_HIP(dv::launch_fill_i32(s, nnxt, 1, 0));
     if ((rc = update(td, bound, k, cur, ncur, nxt, nnxt))) return rc;
-    DV_HIP(hipMemcpyAsync(ctx->round_counts + (k & 1), nnxt, sizeof(int32_t),
-                          hipMemcpyDeviceToHost, s));
+    DV_HIP(dv::launch_copy_counts(s, nnxt, 1, ctx->round_counts_dev + (k & 1)));
     DV_HIP(hipEventRecord(ctx->round_ev[k & 1], s));
     std::swap(cur, nxt);
     std::swap(ncur, nnxt);
@@ -1491,14 +1498,17 @@ int bread_mixed(devil_ctx* ctx, const TileHost& tl, dv::Counts c, int64_t G, int
 int autotune_mixed(devil_ctx* ctx, const DeviceDesign& dd, int64_t n, int p, dv::Counts c,
                    int64_t G, const double* beta, const double* kvec, int& mk, int64_t& cc) {
   hipStream_t s = ctx->stream;
-  // enough genes that every candidate fills the GPU as the full fit does
-  const int64_t Gs = std::min<int64_t>(G, 4096);
+  // enough genes that every candidate fills the GPU much as the full fit's
+  // first rounds do
+  const int64_t Gs = std::min<int64_t>(G, 16384);
+  // the usual winners first: a later candidate must be 2% faster to replace
+  // the choice, so timing noise does not flip it
   std::vector<int> kinds;
   if (ctx->tune[DEVIL_TUNE_MIXED_KERNEL] == DEVIL_MIXED_AUTO)
-    kinds = {kMixedF32, kMixedBf16, kMixedBf16Ws};  // two-tile waves measured slower on C3
+    kinds = {kMixedBf16Ws, kMixedBf16, kMixedF32};  // two-tile waves measured slower on C3
   else kinds = {mixed_kernel(ctx)};
   std::vector<int64_t> cells;
-  if (ctx->tune[DEVIL_TUNE_CHUNK_CELLS] == 0) cells = {1024, 2048, 4096};
+  if (ctx->tune[DEVIL_TUNE_CHUNK_CELLS] == 0) cells = {2048, 1024, 4096};
   else cells = {(int64_t)ctx->tune[DEVIL_TUNE_CHUNK_CELLS]};
   DevBuf act, cnt;
   DV_ALLOC(act, sizeof(int32_t) * Gs);
@@ -1529,7 +1539,7 @@ int autotune_mixed(devil_ctx* ctx, const DeviceDesign& dd, int64_t n, int p, dv:
       DV_HIP(hipEventSynchronize(e1));
       float ms = 0.f;
       DV_HIP(hipEventElapsedTime(&ms, e0, e1));
-      if (ms < best) {
+      if (ms < 0.98f * best) {
         best = ms;
         mk = kind;
         cc = cl;
@@ -2002,8 +2012,10 @@ int joint_host_setup(devil_ctx* ctx, size_t bytes) {
   if (ctx->joint_host_bytes >= bytes) return DEVIL_OK;
   if (ctx->joint_host) DV_HIP(hipHostFree(ctx->joint_host));
   ctx->joint_host = nullptr;
+  ctx->joint_host_dev = nullptr;
   ctx->joint_host_bytes = 0;
   DV_HIP(hipHostMalloc(&ctx->joint_host, bytes, hipHostMallocDefault));
+  DV_HIP(hipHostGetDevicePointer(&ctx->joint_host_dev, ctx->joint_host, 0));
   ctx->joint_host_bytes = bytes;
   return DEVIL_OK;
 }
@@ -2137,8 +2149,8 @@ int joint_rounds(devil_ctx* ctx, std::vector<JointMember*>& ms, int max_rounds, 
     recs.push_back(rk);
     for (const Launch& u : upd)
       DV_HIP(dv::launch_irls_update_joint(s, u.p, dev + u.first, u.n, u.total));
-    DV_HIP(hipMemcpyAsync(const_cast<int32_t*>(slot) + (k & 1) * M, B, sizeof(int32_t) * M,
-                          hipMemcpyDeviceToHost, s));
+    DV_HIP(dv::launch_copy_counts(s, B, M,
+                                  static_cast<int32_t*>(ctx->joint_host_dev) + (k & 1) * M));
     DV_HIP(hipEventRecord(ctx->round_ev[k & 1], s));
     std::swap(A, B);
     for (int m = 0; m < M; ++m) std::swap(cur[m], nxt[m]);

@@ -286,6 +286,7 @@ hipError_t launch_transpose_gp(hipStream_t s, const double* src, int64_t G, int 
                                int to_gene_major, double* dst);
 
 hipError_t launch_fill_i32(hipStream_t s, int32_t* p, int64_t count, int32_t v);
+hipError_t launch_copy_counts(hipStream_t s, const int32_t* src, int n, int32_t* dst);
 // stats::p.adjust methods (DEVIL_PADJ_* values) on the device
 enum { kPadjBH = 0, kPadjBY = 1, kPadjHolm = 2, kPadjHochberg = 3, kPadjHommel = 4,
        kPadjBonferroni = 5, kPadjNone = 6 };

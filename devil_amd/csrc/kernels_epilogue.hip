@@ -305,6 +305,19 @@ hipError_t launch_padj_finish(hipStream_t s, const double* pval, int64_t G, int 
   return hipGetLastError();
 }
 
+// dst[i] = src[i], i < n: device counts into pinned host slots (a kernel in
+// the stream rather than a copy-engine transfer, so the next round's kernels
+// need not wait for a copy engine)
+__global__ void k_copy_counts(const int32_t* __restrict__ src, int n, int32_t* __restrict__ dst) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
+}
+
+hipError_t launch_copy_counts(hipStream_t s, const int32_t* src, int n, int32_t* dst) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_copy_counts, dim3(1), dim3(n < 256 ? 64 : 256), 0, s, src, n, dst);
+  return hipGetLastError();
+}
+
 // G x p column-major (R layout) <-> gene-major G x p (row g holds its p values)
 __global__ void k_transpose_gp(const double* __restrict__ src, int64_t G, int p, int to_gene,
                                double* __restrict__ dst) {
