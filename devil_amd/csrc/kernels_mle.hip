@@ -409,28 +409,42 @@ __global__ __launch_bounds__(256) void k_mle_accum(TileDesign td, Counts c,
   });
 }
 
+// The update runs on kMleUpdThreads threads per gene: the chunk partials in
+// kMleUpdSlices interleaved slices per column and the count-table sums spread
+// over every thread (a lone gene's last rounds are latency-bound), then one
+// lane steps the driver.
+constexpr int kMleUpdThreads = 256;
+constexpr int kMleUpdSlices = 4;
+
 template <int P>
-__global__ __launch_bounds__(64) void k_mle_update(TileDesign td, const int32_t* __restrict__ active,
-                                                   const int32_t* __restrict__ nactive,
-                                                   const double* __restrict__ part, MleState ms,
-                                                   int64_t n, int max_iter, PortParams pp,
-                                                   int32_t* __restrict__ next,
-                                                   int32_t* __restrict__ nnext) {
+__global__ __launch_bounds__(kMleUpdThreads) void k_mle_update(
+    TileDesign td, const int32_t* __restrict__ active, const int32_t* __restrict__ nactive,
+    const double* __restrict__ part, MleState ms, int64_t n, int max_iter, PortParams pp,
+    int32_t* __restrict__ next, int32_t* __restrict__ nnext) {
   __shared__ double acc[kMleScal + 3 * DEVIL_NCP_MAX];
+  __shared__ double slice[kMleUpdSlices][kMleScal + 3 * DEVIL_NCP_MAX];
+  __shared__ double red[3][kMleUpdThreads / 64];
   const int nA = *nactive;
   const int i = blockIdx.x;
-  if (i >= nA) return;
-  const int lane = threadIdx.x;
+  if (i >= nA) return;  // uniform over the block
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int g = active[i];
   const int tile = i >> 4, s = i & 15;
   const int W = kMleScal + 3 * td.ncp;
   const double* src = part + (int64_t)tile * td.nchunk * 16 * W + (int64_t)s * W;
-  for (int col = lane; col < W; col += 64) {
+  for (int idx = t; idx < kMleUpdSlices * W; idx += kMleUpdThreads) {
+    const int col = idx % W, q = idx / W;
     double a = 0.0;
-    for (int ch = 0; ch < td.nchunk; ++ch) a += src[(int64_t)ch * 16 * W + col];
-    acc[col] = a;
+    for (int ch = q; ch < td.nchunk; ch += kMleUpdSlices) a += src[(int64_t)ch * 16 * W + col];
+    slice[q][col] = a;
   }
   __syncthreads();
+  for (int col = t; col < W; col += kMleUpdThreads) {
+    double a = 0.0;
+#pragma unroll
+    for (int q = 0; q < kMleUpdSlices; ++q) a += slice[q][col];
+    acc[col] = a;
+  }
   const double x = ms.x[g];
   const double theta = exp(x);
   const double tn1 = 1.0 / theta;
@@ -440,7 +454,7 @@ __global__ __launch_bounds__(64) void k_mle_update(TileDesign td, const int32_t*
   // over the distinct-count table
   double tdg = 0.0, ttg = 0.0, tld = 0.0;
   const int tn = ms.tab_n[g];
-  for (int k = lane; k < tn; k += 64) {
+  for (int k = t; k < tn; k += kMleUpdThreads) {
     const double u = (double)ms.tab_u[(int64_t)g * kMleTabCap + k];
     const double f = (double)ms.tab_f[(int64_t)g * kMleTabCap + k];
     tdg += f * dpsi_diff(tn1, u);
@@ -449,8 +463,21 @@ __global__ __launch_bounds__(64) void k_mle_update(TileDesign td, const int32_t*
   }
   tdg = wave_sum(tdg);
   ttg = wave_sum(ttg);
-  if (port) tld = wave_sum(tld);
-  if (lane != 0) return;
+  tld = wave_sum(tld);
+  if (lane == 0) {
+    red[0][wave] = tdg;
+    red[1][wave] = ttg;
+    red[2][wave] = tld;
+  }
+  __syncthreads();
+  if (t != 0) return;
+  tdg = ttg = tld = 0.0;
+#pragma unroll
+  for (int w = 0; w < kMleUpdThreads / 64; ++w) {
+    tdg += red[0][w];
+    ttg += red[1][w];
+    tld += red[2][w];
+  }
   const double sc = acc[0], l1 = acc[1], l2 = acc[2];
   const double dg = acc[3] + tdg, tg = acc[4] + ttg;
   const double* B0 = acc + kMleScal;
@@ -657,8 +684,8 @@ hipError_t launch_mle_update(hipStream_t s, const TileDesign& td, int p, const i
   pp.rfctol = rel_tol;
   pp.sctol = rel_tol;
   pp.iter_max = max_iter;
-  DV_MLE_P(p, k_mle_update, dim3((unsigned)nA_bound), dim3(64), s, td, active, nactive, part,
-           ms, n, max_iter, pp, next, nnext);
+  DV_MLE_P(p, k_mle_update, dim3((unsigned)nA_bound), dim3(kMleUpdThreads), s, td, active,
+           nactive, part, ms, n, max_iter, pp, next, nnext);
   return hipGetLastError();
 }
 
