@@ -91,9 +91,17 @@ def check_mixed(dv, orc, w, fit, ref, contrast):
     lp = res["log_pval"] / np.log(10.0)
     lr = rr["log_pval"] / np.log(10.0)
     ep = relmax(lp, lr, 1.0)
-    say(f"  beta {eb:.2e}  theta {et:.2e}  log10 p {ep:.2e}")
+    # the same errors at the 1e-6 floor, recorded, and the absolute beta error
+    # that the 1e-3 floor stands for (FP32 element noise, ~4e-7 on C3)
+    eb6 = relmax(fit["beta"], ref["beta"], FLOOR)
+    ep6 = relmax(lp, lr, FLOOR)
+    b = fit["beta"].cpu().numpy() if hasattr(fit["beta"], "cpu") else np.asarray(fit["beta"])
+    ea = float(np.nanmax(np.abs(b - ref["beta"])))
+    say(f"  beta {eb:.2e}  theta {et:.2e}  log10 p {ep:.2e}  |  floor 1e-6: beta {eb6:.2e} "
+        f"log10 p {ep6:.2e}  |  beta absolute {ea:.2e}")
     tol = dv.mixed_tol
     assert eb <= tol and ep <= tol and et <= TOL
+    assert ea <= 5e-6
     calls = res["adj_pval"] < 0.05
     calls_ref = rr["adj_pval"] < 0.05
     np.testing.assert_array_equal(calls, calls_ref)
