@@ -24,6 +24,11 @@
  *   orc_init_beta_univariate src/beta_init_univariate.cpp:13-95
  *   orc_cpu_fit             R/main.R:375-523
  *   orc_test_de             R/test.R:85-134 (stats::pt, stats::p.adjust "BH")
+ *   orc_p_adjust            stats::p.adjust (R's stats package, p.adjust.R;
+ *                           called from R/test.R:127), every method
+ *   orc_fit_overdispersion_new  src/new_overdispersion.cpp:195-481
+ *                           (fit_overdispersion_cppp), H_log_gh_pmhalf :141-158,
+ *                           the Gauss-Hermite rule :24-66
  *
  * How each part is pinned (tests/test_oracle_*.py):
  *   special functions       scipy.special digamma/polygamma/betaln/betainc,
@@ -67,6 +72,21 @@
  *   orc_nb_loglik_acc       that is well conditioned; score = its derivative
  *   orc_size_factors        the definitions (numpy), scale and gene-order
  *                           invariance
+ *   orc_p_adjust            tests/test_oracle_padjust.py: each method's
+ *                           decision rule at many levels, Holm and Hommel by
+ *                           brute-force closed testing (Bonferroni / Simes
+ *                           local tests over every subset), BH and BY against
+ *                           scipy.stats.false_discovery_control, p.adjust's
+ *                           NA / n <= 1 / hommel-with-two rules
+ *   orc_fit_overdispersion_new  tests/test_oracle_newod.py: the 16-node rule
+ *                           against numpy's hermgauss and the Golub-Welsch
+ *                           eigenproblem, H_log_gh_pmhalf against adaptive
+ *                           quadrature, kappa a fixed point of F_k, the
+ *                           reference's scenario (test-main.R:216-252), and
+ *                           the conditioning of the reference's own iteration
+ *                           (its Newton mode search stops at eps_theta)
+ *   selftest.c              every entry point under AddressSanitizer and
+ *                           UndefinedBehaviorSanitizer (make sanitize)
  */
 #include "devil_oracle.h"
 
