@@ -47,17 +47,18 @@ def _counts_dtype(dt) -> int:
 def cluster_blocks(clusters, n: int) -> np.ndarray:
     """R/main.R:190-196: codes by first appearance, must be sorted, then the
     cumulative run-length ends (1-based)."""
-    lab = np.asarray(clusters)
+    lab = np.asarray(clusters.cpu() if _is_torch(clusters) else clusters)
     if lab.shape[0] != n:
         raise ValueError("clusters must have one entry per cell")
-    _, first = np.unique(lab, return_index=True)
-    order = np.argsort(first)
-    code_of = {lab[first[k]]: i + 1 for i, k in enumerate(order)}
-    codes = np.array([code_of[v] for v in lab], dtype=np.int64)
-    if np.any(np.diff(codes) < 0):
+    if n == 0:
+        return np.zeros(0, np.int32)
+    # codes by first appearance never decrease exactly when every label's
+    # cells form one run: the runs' labels are then all distinct
+    change = np.flatnonzero(lab[1:] != lab[:-1])
+    run_labels = lab[np.concatenate([[0], change + 1])]
+    if np.unique(run_labels).shape[0] != run_labels.shape[0]:
         raise ValueError("Input data should be grouped in blocks of patients. "
                          "Use the `group_data` function to group them.")
-    change = np.flatnonzero(np.diff(codes) != 0)
     ends = np.concatenate([change + 1, [n]]).astype(np.int32)
     return ends
 
