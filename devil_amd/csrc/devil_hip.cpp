@@ -1467,31 +1467,49 @@ uint64_t tune_key(int64_t n, int p, int dtype) {
 }
 
 
-// Mixed-precision bread of genes [0, G) on the bf16 operand tables (no
-// clusters).  ok = false when some gene's FP32 element range was exceeded:
-// the caller then runs the FP64 bread for the problem.
-int bread_mixed(devil_ctx* ctx, const TileHost& tl, dv::Counts c, int64_t G, int p,
-                const double* beta, const double* theta, double* bread, int32_t* status,
-                bool& ok) {
+// Mixed-precision bread of genes [0, G) on the bf16 operand tables; with C >
+// 0 patients (chunk table ch, aligned to the patient blocks) the same pass
+// also sums each patient's scores and the clustered sandwich follows.  ok =
+// false when some gene's FP32 element range was exceeded: the caller then runs
+// the FP64 bread (and meat) for the problem.
+int bread_mixed(devil_ctx* ctx, const TileHost& tl, const ChunkHost& ch, dv::Counts c,
+                int64_t G, int64_t n, int p, int C, const double* beta, const double* theta,
+                double* bread, double* sandwich, int32_t* status, bool& ok) {
   hipStream_t s = ctx->stream;
   const dv::TileDesign& tm = tl.tdm;
-  DevBuf genes, cnt, sing, part;
+  const bool meat = C > 0;
+  const int nchunk = meat ? ch.ct.nchunk : tm.nchunk;
+  const size_t tiles = (size_t)((G + 15) / 16);
+  DevBuf genes, cnt, sing, part, spart;
   DV_ALLOC(genes, sizeof(int32_t) * G);
   DV_ALLOC(sing, sizeof(int32_t) * G);
   DV_ALLOC(cnt, sizeof(int32_t) * 2);
-  DV_ALLOC(part, sizeof(double) * (size_t)((G + 15) / 16) * tm.nchunk * 16 * tm.ncp);
+  DV_ALLOC(part, sizeof(double) * tiles * nchunk * 16 * tm.ncp);
+  if (meat) DV_ALLOC(spart, sizeof(double) * tiles * nchunk * 16 * 16);
   DV_HIP(dv::launch_iota(s, genes.as<int32_t>(), (int)G));
   DV_HIP(dv::launch_fill_i32(s, cnt.as<int32_t>(), 1, (int32_t)G));
   DV_HIP(dv::launch_fill_i32(s, cnt.as<int32_t>() + 1, 1, 0));
-  DV_HIP(dv::launch_bread_mixed(s, tm, c, p, genes.as<int32_t>(), cnt.as<int32_t>(), G, beta,
-                                theta, part.as<double>(), bread, sing.as<int32_t>(),
-                                cnt.as<int32_t>() + 1));
+  if (meat)
+    DV_HIP(dv::launch_bread_meat_mixed(s, tm, ch.ct, c, p, genes.as<int32_t>(),
+                                       cnt.as<int32_t>(), G, beta, theta, part.as<double>(),
+                                       spart.as<double>(), bread, sing.as<int32_t>(),
+                                       cnt.as<int32_t>() + 1));
+  else
+    DV_HIP(dv::launch_bread_mixed(s, tm, c, p, genes.as<int32_t>(), cnt.as<int32_t>(), G, beta,
+                                  theta, part.as<double>(), bread, sing.as<int32_t>(),
+                                  cnt.as<int32_t>() + 1));
   int32_t nbad = 0;
   DV_HIP(hipMemcpyAsync(&nbad, cnt.as<int32_t>() + 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
   DV_HIP(hipStreamSynchronize(s));
   ok = nbad == 0;
-  if (ok)
+  if (ok) {
     DV_HIP(dv::launch_merge_flags(s, status, sing.as<int32_t>(), G, DEVIL_GENE_SINGULAR));
+    if (meat)
+      DV_HIP(dv::launch_meat_finish(s, tm, ch.ct, G, n, p, C, spart.as<double>(), bread,
+                                    sandwich));
+  }
+  // part and spart are freed on return
+  DV_HIP(hipStreamSynchronize(s));
   return DEVIL_OK;
 }
 
@@ -1929,10 +1947,13 @@ int run_batch_post(devil_ctx* ctx, ProbRun& r, StageTimer& tm) {
   tm.end(ev);
   ev = tm.begin(DEVIL_STAGE_BREAD_MEAT);
   // mixed precision: the bread on the bf16 tables (FP32 element math, FP64
-  // sums and inverse) where there is no meat to build
+  // sums and inverse) and, with patients, each patient's score sums from the
+  // same pass (the HC0 meat stays on the FP64 pass)
   bool bread_done = false;
-  if (r.mixed && bf16_layout(r.tl.mixed) && C == 0 && counts_ws(c)) {
-    if ((rc = bread_mixed(ctx, r.tl, c, Gb, p, b0, th, r.bread.as<double>() + g0 * p * p, st,
+  if (r.mixed && bf16_layout(r.tl.mixed) && counts_ws(c) && C >= 0) {
+    if ((rc = bread_mixed(ctx, r.tl, r.chunks, c, Gb, n, p, r.want_sw ? C : 0, b0, th,
+                          r.bread.as<double>() + g0 * p * p,
+                          r.want_sw ? r.sandwich.as<double>() + g0 * p * p : nullptr, st,
                           bread_done)))
       return rc;
   }

@@ -121,6 +121,11 @@ hipError_t launch_bread_tiled(hipStream_t s, const TileDesign& td, const ChunkTa
                               Counts c, int64_t G, int64_t n, int p, const double* beta,
                               const double* theta, int C, double* part, double* bread,
                               double* sandwich, int32_t* status);
+// meat and sandwich from per-patient score partials spart[tile][chunk][16][16]
+// (design column a at column a) and a bread already in `bread`
+hipError_t launch_meat_finish(hipStream_t s, const TileDesign& td, const ChunkTable& ct,
+                              int64_t G, int64_t n, int p, int C, const double* spart,
+                              const double* bread, double* sandwich);
 
 hipError_t launch_irls_accum(hipStream_t s, const TileDesign& td, Counts c, int p,
                              const int32_t* active, const int32_t* nactive, int nA_bound,
@@ -157,6 +162,13 @@ hipError_t launch_bread_mixed(hipStream_t s, const TileDesign& td, Counts c, int
                               const int32_t* genes, const int32_t* ngenes, int64_t G,
                               const double* beta, const double* theta, double* part,
                               double* bread, int32_t* singular, int32_t* fallback);
+// The same over the patient-aligned chunks of ct, also writing the per-patient
+// score partials (spart, for launch_meat_finish).
+hipError_t launch_bread_meat_mixed(hipStream_t s, const TileDesign& td, const ChunkTable& ct,
+                                   Counts c, int p, const int32_t* genes, const int32_t* ngenes,
+                                   int64_t G, const double* beta, const double* theta,
+                                   double* part, double* spart, double* bread, int32_t* singular,
+                                   int32_t* fallback);
 // status[g] |= bit where flag[g] != 0.
 hipError_t launch_merge_flags(hipStream_t s, int32_t* status, const int32_t* flag, int64_t G,
                               int32_t bit);

@@ -257,13 +257,16 @@ __global__ __launch_bounds__(256) void k_bread_accum(TileDesign td, ChunkTable c
 
 // One wavefront per gene.  C = number of clusters (0: no sandwich; -1: the
 // HC0 meat, sum_i r_i^2 x_i x_i' / n, from the partials' last ncp columns).
+// bread_given: the bread is already in `bread` (the mixed path's): only the
+// meat and the sandwich are formed.
 template <int P>
 __global__ __launch_bounds__(64) void k_bread_finish(TileDesign td, ChunkTable ct, int64_t G,
                                                      int64_t n, int C,
                                                      const double* __restrict__ part,
                                                      double* __restrict__ bread,
                                                      double* __restrict__ sandwich,
-                                                     int32_t* __restrict__ status) {
+                                                     int32_t* __restrict__ status,
+                                                     int bread_given) {
   constexpr int NPAIR = P * (P + 1) / 2;
   __shared__ double accw[DEVIL_NCP_MAX];
   __shared__ double acch[DEVIL_NCP_MAX];
@@ -275,7 +278,7 @@ __global__ __launch_bounds__(64) void k_bread_finish(TileDesign td, ChunkTable c
   const bool hc0 = C < 0;
   const int W = td.ncp + 16 + (hc0 ? td.ncp : 0);
   const double* src = part + (g / 16) * ct.nchunk * 16 * W + (g % 16) * W;
-  for (int col = lane; col < td.ncp; col += 64) {
+  for (int col = lane; col < td.ncp && !bread_given; col += 64) {
     double a = 0.0, m = 0.0;
     for (int ch = 0; ch < ct.nchunk; ++ch) {
       a += src[(int64_t)ch * 16 * W + col];
@@ -319,22 +322,29 @@ __global__ __launch_bounds__(64) void k_bread_finish(TileDesign td, ChunkTable c
   __syncthreads();
   if (lane != 0) return;
   double Hm[P][P], Bi[P][P];
-#pragma unroll
-  for (int a = 0; a < P; ++a)
-#pragma unroll
-    for (int bb = a; bb < P; ++bb) {
-      const int pi = td.kidx_pair[tri<P>(a, bb)];
-      const double v = pi >= 0 ? accw[pi] : 0.0;
-      Hm[a][bb] = v;
-      Hm[bb][a] = v;
-    }
-  const int sing = lu_inverse<P>(Hm, Bi);
   double* out = bread + g * P * P;
+  if (bread_given) {
 #pragma unroll
-  for (int a = 0; a < P; ++a)
+    for (int a = 0; a < P; ++a)
 #pragma unroll
-    for (int bb = 0; bb < P; ++bb) out[a + bb * P] = Bi[a][bb];
-  if (status && sing) status[g] |= DEVIL_GENE_SINGULAR;
+      for (int bb = 0; bb < P; ++bb) Bi[a][bb] = out[a + bb * P];
+  } else {
+#pragma unroll
+    for (int a = 0; a < P; ++a)
+#pragma unroll
+      for (int bb = a; bb < P; ++bb) {
+        const int pi = td.kidx_pair[tri<P>(a, bb)];
+        const double v = pi >= 0 ? accw[pi] : 0.0;
+        Hm[a][bb] = v;
+        Hm[bb][a] = v;
+      }
+    const int sing = lu_inverse<P>(Hm, Bi);
+#pragma unroll
+    for (int a = 0; a < P; ++a)
+#pragma unroll
+      for (int bb = 0; bb < P; ++bb) out[a + bb * P] = Bi[a][bb];
+    if (status && sing) status[g] |= DEVIL_GENE_SINGULAR;
+  }
   if (C != 0 && sandwich) {
     const double adj = (C > 1) ? (double)C / (C - 1.0) : 1.0;
     const double scl = (hc0 ? 1.0 : adj) / (double)n;
@@ -643,7 +653,20 @@ hipError_t launch_bread_tiled(hipStream_t s, const TileDesign& td, const ChunkTa
                ct, c, G, beta, theta, part);
   }
   DV_POST_P(p, k_bread_finish, dim3((unsigned)G), dim3(64), s, td, ct, G, n, C, part, bread,
-            sandwich, status);
+            sandwich, status, 0);
+  return hipGetLastError();
+}
+
+hipError_t launch_meat_finish(hipStream_t s, const TileDesign& td, const ChunkTable& ct,
+                              int64_t G, int64_t n, int p, int C, const double* spart,
+                              const double* bread, double* sandwich) {
+  if (G <= 0) return hipSuccess;
+  if (C <= 0 || !sandwich) return hipErrorInvalidValue;
+  // the score partials alone: 16 columns per row, design column a at a
+  TileDesign tc = td;
+  tc.ncp = 0;
+  DV_POST_P(p, k_bread_finish, dim3((unsigned)G), dim3(64), s, tc, ct, G, n, C, spart,
+            const_cast<double*>(bread), sandwich, nullptr, 1);
   return hipGetLastError();
 }
 

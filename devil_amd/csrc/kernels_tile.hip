@@ -368,6 +368,18 @@ struct BfCounts {
 // of |r x|, under the rounding of an FP32 product.  Without GRAD3 (the
 // bread, whose design columns hold pair products too) and in the pair
 // tiles: r1 K1 + r2 K1 + r1 K2 (about 16 significant bits per product).
+template <typename S>
+__device__ __forceinline__ void bf_grad3(const float (&rr)[8], const S& o, f4& ag) {
+  bf8 r1, r2, r3;
+  bf_split8_3(rr, r1, r2, r3);
+  ag = mfma_bf(r3, o.kg[0], ag);
+  ag = mfma_bf(r2, o.kg[1], ag);
+  ag = mfma_bf(r1, o.kg[2], ag);
+  ag = mfma_bf(r2, o.kg[0], ag);
+  ag = mfma_bf(r1, o.kg[1], ag);
+  ag = mfma_bf(r1, o.kg[0], ag);
+}
+
 template <int NTH, bool GRAD3, typename S>
 __device__ __forceinline__ void bf_accumulate(const float (&rr)[8], const S& o, int nth, f4& ag,
                                               f4 (&ah)[NTH]) {
@@ -525,6 +537,29 @@ __device__ __forceinline__ void bf_bread_weights(const S& o, const CT (&y)[8],
     const float mu = __builtin_amdgcn_exp2f(t2);
     const float rd = __builtin_amdgcn_rcpf(fmaf(th, mu, 1.0f));
     ww[r] = fmaf((float)y[r], th, 1.0f) * (mu * rd) * rd;
+  }
+}
+
+// The bread weights and the score residuals r = (y - mu) / (1 + th mu)
+// (src/variance.cpp:186-228) of the step's 8 cells from one mu.
+template <int NE, typename CT, typename S>
+__device__ __forceinline__ void bf_bread_score(const S& o, const CT (&y)[8], const bf8 (&bb)[NE],
+                                               float th, float& tmax, float (&ww)[8],
+                                               float (&rs)[8]) {
+  f4 eta0 = f4{0.f, 0.f, 0.f, 0.f}, eta1 = eta0;
+#pragma unroll
+  for (int j = 0; j < NE; ++j) eta0 = mfma_bf(o.xa[j], bb[j], eta0);
+#pragma unroll
+  for (int j = 0; j < NE; ++j) eta1 = mfma_bf(o.xa[NE + j], bb[j], eta1);
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const float t2 = (r < 4 ? eta0[r] : eta1[r - 4]) + (r < 4 ? o.off0[r] : o.off1[r - 4]);
+    tmax = fmaxf(tmax, t2);
+    const float mu = __builtin_amdgcn_exp2f(t2);
+    const float rd = __builtin_amdgcn_rcpf(fmaf(th, mu, 1.0f));
+    const float yf = (float)y[r];
+    ww[r] = fmaf(yf, th, 1.0f) * (mu * rd) * rd;
+    rs[r] = (yf - mu) * rd;
   }
 }
 
@@ -745,7 +780,10 @@ __device__ __forceinline__ void ws_loader(const TileDesign& td, int64_t s0, int6
 
 // MODE 0: IRLS accumulate (gpar = k, residuals r); MODE 1: bread (gpar =
 // theta, weights w): the same contractions of a per-cell scalar with the
-// design columns and the pair columns.  A16: every count row starts 16-byte
+// design columns and the pair columns; MODE 2: the bread and, against the
+// design columns, the score residuals of the cells [c0, c1) of one patient
+// (cells of the chunk's first and last step outside the range are masked)
+// into spart.  nchunk: chunks of the partial layout.  A16: every count row starts 16-byte
 // aligned, so a lane's four cells arrive by one 16-byte direct-to-LDS load
 // (lane l's gene l % 16, cells 4 (l / 16) .. + 3, at byte 16 l of each KB);
 // otherwise four 4-byte direct-to-LDS loads per KB, load e covering genes
@@ -757,8 +795,10 @@ __device__ __forceinline__ void ws_compute(const TileDesign& td, const CT* __res
                                            int64_t ld, const int32_t* __restrict__ active,
                                            int nA, const double* __restrict__ beta,
                                            const double* __restrict__ kvec, int tile0,
-                                           int ntile, int chunk, int64_t s0, int64_t s1,
-                                           unsigned char* ring, double* __restrict__ part) {
+                                           int ntile, int chunk, int nchunk, int64_t s0,
+                                           int64_t s1, int64_t c0, int64_t c1,
+                                           unsigned char* ring, double* __restrict__ part,
+                                           double* __restrict__ spart) {
   constexpr int NE = eta_blocks(P);
   constexpr int NTH = ((P * (P + 1)) / 2 + 15) / 16;
   constexpr int kSpan = 4;
@@ -809,12 +849,15 @@ __device__ __forceinline__ void ws_compute(const TileDesign& td, const CT* __res
   double ag64[TPW][4];
   f4 ah[TPW][NTH];
   float tmin[TPW];
+  f4 as0[TPW];         // MODE 2: score sums of the FP32 span
+  double as64[TPW][4];
 #pragma unroll
   for (int u = 0; u < TPW; ++u) {
     ag0[u] = f4{0.f, 0.f, 0.f, 0.f};
+    as0[u] = f4{0.f, 0.f, 0.f, 0.f};
     tmin[u] = 0.0f;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) ag64[u][r] = 0.0;
+    for (int r = 0; r < 4; ++r) ag64[u][r] = as64[u][r] = 0.0;
 #pragma unroll
     for (int t = 0; t < NTH; ++t) ah[u][t] = f4{0.f, 0.f, 0.f, 0.f};
   }
@@ -879,10 +922,23 @@ __device__ __forceinline__ void ws_compute(const TileDesign& td, const CT* __res
 #pragma unroll
     for (int u = 0; u < TPW; ++u) {
       float rr[8];
-      if constexpr (MODE == 0)
+      if constexpr (MODE == 0) {
         bf_residuals<NE, CT>(o, yc[u].y, bb[u], k[u], tmin[u], rr);
-      else
+      } else if constexpr (MODE == 1) {
         bf_bread_weights<NE, CT>(o, yc[u].y, bb[u], k[u], tmin[u], rr);
+      } else {
+        float rs[8];
+        bf_bread_score<NE, CT>(o, yc[u].y, bb[u], k[u], tmin[u], rr, rs);
+        const int64_t cell0 = (s0 + kk) * 32;
+        if (cell0 < c0 || cell0 + 32 > c1) {
+#pragma unroll
+          for (int r = 0; r < 8; ++r) {
+            const int64_t cell = cell0 + (r < 4 ? 4 * h + r : 16 + 4 * h + r - 4);
+            if (cell < c0 || cell >= c1) rr[r] = rs[r] = 0.0f;
+          }
+        }
+        bf_grad3(rs, o, as0[u]);
+      }
       bf_accumulate<NTH, MODE == 0>(rr, o, nth, ag0[u], ah[u]);
     }
     if (++span == kSpan) {
@@ -892,6 +948,11 @@ __device__ __forceinline__ void ws_compute(const TileDesign& td, const CT* __res
 #pragma unroll
         for (int r = 0; r < 4; ++r) ag64[u][r] += (double)ag0[u][r];
         ag0[u] = f4{0.f, 0.f, 0.f, 0.f};
+        if constexpr (MODE == 2) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) as64[u][r] += (double)as0[u][r];
+          as0[u] = f4{0.f, 0.f, 0.f, 0.f};
+        }
       }
     }
   };
@@ -952,7 +1013,10 @@ __device__ __forceinline__ void ws_compute(const TileDesign& td, const CT* __res
   for (int u = 0; u < TPW; ++u) {
     if (!tile_live[u]) continue;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) ag64[u][r] += (double)ag0[u][r];
+    for (int r = 0; r < 4; ++r) {
+      ag64[u][r] += (double)ag0[u][r];
+      as64[u][r] += (double)as0[u][r];
+    }
     // IRLS: tmin is the smallest log2(mu), poison below -1024 (FP64's
     // exp(-eta - off) overflows there); bread: tmin holds the largest,
     // poison from 126 on (FP32 mu overflows)
@@ -969,7 +1033,7 @@ __device__ __forceinline__ void ws_compute(const TileDesign& td, const CT* __res
 #pragma unroll
       for (int r = 0; r < 4; ++r) poison[r] = !(__shfl(tm, 4 * h + r, 64) < 126.0f);
     }
-    double* out = part + ((int64_t)(tile0 + u) * td.nchunk + chunk) * 16 * td.ncp;
+    double* out = part + ((int64_t)(tile0 + u) * nchunk + chunk) * 16 * td.ncp;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const double nanv = __builtin_nan("");
@@ -978,6 +1042,9 @@ __device__ __forceinline__ void ws_compute(const TileDesign& td, const CT* __res
       for (int t = 0; t < NTH; ++t)
         if (t < nth)
           out[(4 * h + r) * td.ncp + 16 * (t + 1) + s] = poison[r] ? nanv : (double)ah[u][t][r];
+      if constexpr (MODE == 2)
+        spart[(((int64_t)(tile0 + u) * nchunk + chunk) * 16 + 4 * h + r) * 16 + s] =
+            poison[r] ? nanv : as64[u][r];
     }
   }
 }
@@ -988,7 +1055,9 @@ __device__ __forceinline__ void ws_body(const TileDesign& td, const Counts& c,
                                         const int32_t* __restrict__ nactive,
                                         const double* __restrict__ beta,
                                         const double* __restrict__ gpar,
-                                        double* __restrict__ part, int wb, int nwb) {
+                                        double* __restrict__ part, int wb, int nwb,
+                                        const ChunkTable& ct = ChunkTable{},
+                                        double* __restrict__ spart = nullptr) {
   constexpr int NE = eta_blocks(P);
   constexpr int NTH = ((P * (P + 1)) / 2 + 15) / 16;
   typedef WsLayout<NE, NTH> Lay;
@@ -999,10 +1068,20 @@ __device__ __forceinline__ void ws_body(const TileDesign& td, const Counts& c,
   const int ntile = (nA + 15) / 16;
   const int wave = threadIdx.x >> 6;
   int chunk, group;
-  tile_coords_at(wb, nwb, td.nchunk, chunk, group);
+  const int nchunk = MODE == 2 ? ct.nchunk : td.nchunk;
+  tile_coords_at(wb, nwb, nchunk, chunk, group);
   if (group * kWsWaves * TPW >= ntile) return;  // the whole workgroup
-  const int64_t s0 = (int64_t)chunk * td.chunk_blocks;
-  const int64_t s1 = max(s0, min((int64_t)(chunk + 1) * td.chunk_blocks, td.nblk));
+  int64_t s0, s1, c0 = 0, c1 = 0;
+  if constexpr (MODE == 2) {
+    // the steps holding the chunk's cells [c0, c1)
+    c0 = ct.c0[chunk];
+    c1 = ct.c1[chunk];
+    s0 = c0 / 32;
+    s1 = c0 < c1 ? min((c1 + 31) / 32, td.nblk) : s0;
+  } else {
+    s0 = (int64_t)chunk * td.chunk_blocks;
+    s1 = max(s0, min((int64_t)(chunk + 1) * td.chunk_blocks, td.nblk));
+  }
   if (wave == kWsWaves) {
     ws_loader<NE, NTH>(td, s0, s1, ring);
     return;
@@ -1012,19 +1091,23 @@ __device__ __forceinline__ void ws_body(const TileDesign& td, const Counts& c,
   if (a16) {
     if (c.dtype == 0)
       ws_compute<P, MODE, TPW, true>(td, static_cast<const int32_t*>(c.ptr), c.ld, active, nA,
-                                     beta, gpar, tile0, ntile, chunk, s0, s1, ring, part);
+                                     beta, gpar, tile0, ntile, chunk, nchunk, s0, s1, c0, c1,
+                                     ring, part, spart);
     else
       ws_compute<P, MODE, TPW, true>(td, static_cast<const float*>(c.ptr), c.ld, active, nA,
-                                     beta, gpar, tile0, ntile, chunk, s0, s1, ring, part);
+                                     beta, gpar, tile0, ntile, chunk, nchunk, s0, s1, c0, c1,
+                                     ring, part, spart);
   } else if constexpr (TPW == 1) {
     // (two tiles per wave would keep 64 loads in flight, past what vmcnt
     // counts: the launchers run misaligned rows with one tile per wave)
     if (c.dtype == 0)
       ws_compute<P, MODE, TPW, false>(td, static_cast<const int32_t*>(c.ptr), c.ld, active, nA,
-                                      beta, gpar, tile0, ntile, chunk, s0, s1, ring, part);
+                                      beta, gpar, tile0, ntile, chunk, nchunk, s0, s1, c0, c1,
+                                      ring, part, spart);
     else
       ws_compute<P, MODE, TPW, false>(td, static_cast<const float*>(c.ptr), c.ld, active, nA,
-                                      beta, gpar, tile0, ntile, chunk, s0, s1, ring, part);
+                                      beta, gpar, tile0, ntile, chunk, nchunk, s0, s1, c0, c1,
+                                      ring, part, spart);
   }
 }
 
@@ -1058,6 +1141,18 @@ __global__ __launch_bounds__(64 * (ws_waves(P) + 1)) void k_bread_accum_ws(
     const int32_t* __restrict__ ngenes, const double* __restrict__ beta,
     const double* __restrict__ theta, double* __restrict__ part) {
   ws_body<P, 1, 1>(td, c, genes, ngenes, beta, theta, part, blockIdx.x, gridDim.x);
+}
+
+// The same with the per-patient score sums of the clustered meat: chunks are
+// the patient-aligned cell ranges of ct; spart[tile][chunk][16][16] holds the
+// sums of r x_a (design column a at column a) with FP32 element math, bf16x3
+// products and FP64 sums over FP32 spans of four steps.
+template <int P>
+__global__ __launch_bounds__(64 * (ws_waves(P) + 1)) void k_bread_meat_ws(
+    TileDesign td, ChunkTable ct, Counts c, const int32_t* __restrict__ genes,
+    const int32_t* __restrict__ ngenes, const double* __restrict__ beta,
+    const double* __restrict__ theta, double* __restrict__ part, double* __restrict__ spart) {
+  ws_body<P, 2, 1>(td, c, genes, ngenes, beta, theta, part, blockIdx.x, gridDim.x, ct, spart);
 }
 
 // bread = H^-1 with H[a][b] = the sum over chunks of the pair column of
@@ -1481,6 +1576,27 @@ hipError_t launch_bread_mixed(hipStream_t s, const TileDesign& td, Counts c, int
   DV_TILE_P(p, k_bread_accum_ws, grid, dim3(64 * (nw + 1)), s, td, c, genes, ngenes, beta, theta,
             part);
   DV_TILE_P(p, k_bread_finish_tile, dim3((unsigned)G), dim3(64), s, td, G, part, bread, singular,
+            fallback);
+  return hipGetLastError();
+}
+
+hipError_t launch_bread_meat_mixed(hipStream_t s, const TileDesign& td, const ChunkTable& ct,
+                                   Counts c, int p, const int32_t* genes, const int32_t* ngenes,
+                                   int64_t G, const double* beta, const double* theta,
+                                   double* part, double* spart, double* bread, int32_t* singular,
+                                   int32_t* fallback) {
+  if (G <= 0) return hipSuccess;
+  if (c.dtype == 2 || (reinterpret_cast<uintptr_t>(c.ptr) & 3) != 0 || ct.nchunk <= 0)
+    return hipErrorInvalidValue;
+  const int ntile = (int)((G + 15) / 16);
+  const int nw = ws_waves(p);
+  const int wgroups = (ntile + nw - 1) / nw;
+  const dim3 grid((unsigned)((int64_t)wgroups * ct.nchunk));
+  DV_TILE_P(p, k_bread_meat_ws, grid, dim3(64 * (nw + 1)), s, td, ct, c, genes, ngenes, beta,
+            theta, part, spart);
+  TileDesign tc = td;  // the bread partials are laid out over ct's chunks
+  tc.nchunk = ct.nchunk;
+  DV_TILE_P(p, k_bread_finish_tile, dim3((unsigned)G), dim3(64), s, tc, G, part, bread, singular,
             fallback);
   return hipGetLastError();
 }

@@ -138,6 +138,14 @@ def test_mixed_clustered_c4(dv, orc):
     Y = synth.counts_host(w)
     fit, ref = fit_and_oracle(dv, orc, w, Y, clusters=w.clusters)
     check_mixed(dv, orc, w, fit, ref, [0, 1, 0, 0, 0, 0])
+    # the clustered sandwich of the mixed path (bread on the bf16 tables, FP64
+    # per-patient score sums): normwise per gene against the FP64 oracle
+    for key, rk in (("beta_sandwiches", "sandwich"), ("beta_sandwiches_null", "bread")):
+        a = np.asarray(fit[key])
+        b = ref[rk]
+        e = np.abs(a - b).reshape(len(b), -1).max(axis=1) / np.abs(b).reshape(len(b), -1).max(axis=1)
+        say(f"  {key} normwise rel err {e.max():.2e}")
+        assert e.max() <= 1e-4, key
 
 
 @pytest.mark.parametrize("p,n", [(1, 4099), (5, 5001), (10, 6007)])
