@@ -10,7 +10,10 @@ workgroups with idle tiles (G not a multiple of the tiles per
 workgroup; 17 genes leave the second tile of a two-tile wave idle), float
 counts, and p = 4 and 8, for the kernel with one and with two 16-gene tiles per
 compute wave.  The same kernel computes the
-mixed path's bread where there are no clusters (test_mixed_bread_matches_fp64).
+mixed path's bread where there are no clusters (test_mixed_bread_matches_fp64)
+and, with patients, the bread and each patient's score sums in one pass
+(misaligned rows, patients ending inside a step, patients shorter than a
+step, a single patient).
 """
 import sys
 
@@ -121,3 +124,78 @@ def test_mixed_bread_misaligned_rows(dv):
         x, y = a[key].cpu().numpy(), b[key].cpu().numpy()
         same = (x == y) | (np.isnan(x) & np.isnan(y))
         assert same.all(), f"{key}: {int((~same).sum())} entries differ"
+
+
+def fit_clustered(dv, Y, w, clusters, precision, pad=1, kernel="bf16_ws"):
+    ctx = dv.context(0)
+    ctx.set_mixed_kernel(kernel)
+    ctx.set_tuning(dv._lib.TUNE_PAD_COPY, pad)
+    try:
+        fit = dv.fit_devil(Y, w.X, clusters=clusters, size_factors=w.sf, precision=precision)
+    finally:
+        ctx.set_tuning(dv._lib.TUNE_PAD_COPY, 1)
+    return fit, ctx.fit_info()
+
+
+def normwise(a, b):
+    a = a.cpu().numpy() if hasattr(a, "cpu") else np.asarray(a)
+    b = b.cpu().numpy() if hasattr(b, "cpu") else np.asarray(b)
+    a = a.reshape(len(a), -1)
+    b = b.reshape(len(b), -1)
+    return np.abs(a - b).max(axis=1) / np.abs(b).max(axis=1)
+
+
+def test_mixed_clustered_misaligned_rows(dv):
+    """The mixed bread + per-patient score pass read from misaligned rows in
+    place (4-byte direct-to-LDS loads) equals the one read from the padded
+    copy, bread and sandwich bit for bit."""
+    from devil_amd import synth
+    w = synth.make_workload("C4", G=200, n=6001, seed=779)
+    Y = synth.counts_device(w, device="cuda")
+    say("[case] mixed clustered, n = 6001, in place vs padded copy")
+    a, ia = fit_clustered(dv, Y, w, w.clusters, "mixed", pad=0)
+    b, ib = fit_clustered(dv, Y, w, w.clusters, "mixed", pad=1)
+    assert ia["aligned_copy"] == 0 and ib["aligned_copy"] == 1
+    for key in ("beta", "overdispersion", "beta_sandwiches_null", "beta_sandwiches"):
+        x, y = a[key].cpu().numpy(), b[key].cpu().numpy()
+        same = (x == y) | (np.isnan(x) & np.isnan(y))
+        assert same.all(), f"{key}: {int((~same).sum())} entries differ"
+
+
+@pytest.mark.parametrize("sizes", ["one patient", "tiny patients", "ragged patients"])
+def test_mixed_clustered_patient_shapes(dv, sizes):
+    """Patient blocks that end inside a 32-cell step, patients of 1-40 cells
+    (chunks of less than one step), and a single patient (adj = 1): the mixed
+    sandwich within 1e-4 normwise of the FP64 path's, which the parity tests
+    hold to the oracle.  A single patient's score sum is the whole gradient
+    at the fitted beta, zero up to the IRLS tolerance, so its sandwich is
+    that residual's outer product: there the bar is 1e-4 of the bread's
+    largest entry (the scale of a sandwich from more than one patient)."""
+    from devil_amd import synth
+    w = synth.make_workload("C4", G=96, n=5003, seed=780)
+    n = w.n  # the synthetic patient sizes round the cell count
+    Y = synth.counts_device(w, device="cuda")
+    rng = np.random.default_rng(781)
+    if sizes == "one patient":
+        lab = np.zeros(n, np.int64)
+    elif sizes == "tiny patients":
+        cuts = np.cumsum(rng.integers(1, 41, size=n))
+        lab = np.searchsorted(cuts, np.arange(n), side="right")
+    else:
+        cuts = np.sort(rng.choice(np.arange(1, n), size=24, replace=False))
+        lab = np.searchsorted(cuts, np.arange(n), side="right")
+    C = int(np.unique(lab).shape[0])
+    say(f"[case] mixed clustered, {sizes}: {C} patients over {n} cells")
+    ref, _ = fit_clustered(dv, Y, w, lab, "fp64")
+    fit, info = fit_clustered(dv, Y, w, lab, "mixed")
+    assert info["mixed_kernel"] == "bf16_ws"
+    for key in ("beta_sandwiches_null", "beta_sandwiches"):
+        e = normwise(fit[key], ref[key])
+        say(f"  {key} normwise rel err max {e.max():.2e} median {np.median(e):.2e}")
+        if key == "beta_sandwiches" and C == 1:
+            a = fit[key].cpu().numpy().reshape(w.G, -1)
+            b = ref[key].cpu().numpy().reshape(w.G, -1)
+            scale = np.abs(ref["beta_sandwiches_null"].cpu().numpy()).reshape(w.G, -1).max(axis=1)
+            e = np.abs(a - b).max(axis=1) / scale
+            say(f"  {key} error / largest bread entry: max {e.max():.2e}")
+        assert e.max() <= 1e-4, key
