@@ -11,6 +11,13 @@ constexpr int kWave = 64;
 
 // Butterfly sum over the 64 lanes; every lane ends with the same bits
 // (IEEE addition is commutative), so the result is deterministic.
+// This thread's wavefront index in its workgroup as a wave-uniform value, so
+// that branches on a wave's tiles compile to scalar branches rather than
+// exec-mask regions (blocks are whole wavefronts of 64 threads).
+__device__ __forceinline__ int wave_index() {
+  return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, kWave);

@@ -397,7 +397,7 @@ __global__ __launch_bounds__(256) void k_mle_accum(TileDesign td, Counts c,
                                                    int want_f, double* __restrict__ part) {
   const int nA = *nactive;
   const int ntile = (nA + 15) / 16;
-  const int wave = threadIdx.x >> 6;
+  const int wave = wave_index();
   int chunk, group;
   tile_coords(td.nchunk, chunk, group);
   const int tile = group * 4 + wave;

@@ -199,7 +199,7 @@ __global__ __launch_bounds__(256) void k_mom_accum(TileDesign td, ChunkTable ct,
   __shared__ double e2[64];
   exp_tab_fill(e2, threadIdx.x, blockDim.x);
   __syncthreads();
-  const int wave = threadIdx.x >> 6;
+  const int wave = wave_index();
   int chunk, group;
   tile_coords(ct.nchunk, chunk, group);
   const int tile = group * 4 + wave;
@@ -243,7 +243,7 @@ __global__ __launch_bounds__(256) void k_bread_accum(TileDesign td, ChunkTable c
   __shared__ double e2[64];
   exp_tab_fill(e2, threadIdx.x, blockDim.x);
   __syncthreads();
-  const int wave = threadIdx.x >> 6;
+  const int wave = wave_index();
   int chunk, group;
   tile_coords(ct.nchunk, chunk, group);
   const int tile = group * 4 + wave;
@@ -507,7 +507,7 @@ __global__ __launch_bounds__(256) void k_init_accum(TileDesign td, InitTile it, 
     log_tab_fill(ltab, threadIdx.x, blockDim.x);
     __syncthreads();
   }
-  const int wave = threadIdx.x >> 6;
+  const int wave = wave_index();
   int chunk, group;
   tile_coords(ct.nchunk, chunk, group);
   const int tile = group * 4 + wave;

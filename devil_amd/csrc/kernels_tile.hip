@@ -264,7 +264,7 @@ __global__ __launch_bounds__(256) void k_irls_accum_mixed(TileDesign td, Counts 
                                                           double* __restrict__ part) {
   const int nA = *nactive;
   const int ntile = (nA + 15) / 16;
-  const int wave = threadIdx.x >> 6;
+  const int wave = wave_index();
   int chunk, group;
   tile_coords(td.nchunk, chunk, group);
   const int tile = group * 4 + wave;
@@ -649,7 +649,7 @@ __global__ __launch_bounds__(256) void k_irls_accum_bf16(TileDesign td, Counts c
                                                          double* __restrict__ part) {
   const int nA = *nactive;
   const int ntile = (nA + 15) / 16;
-  const int wave = threadIdx.x >> 6;
+  const int wave = wave_index();
   int chunk, group;
   tile_coords(td.nchunk, chunk, group);
   const int tile = group * 4 + wave;
@@ -863,7 +863,7 @@ __device__ __forceinline__ void ws_compute(const TileDesign& td, const CT* __res
   }
   int span = 0;
   typedef BfCounts<CT> Cnt;
-  unsigned char* yring = ring + Lay::kY + (threadIdx.x >> 6) * (TPW * Lay::kYWave);
+  unsigned char* yring = ring + Lay::kY + wave_index() * (TPW * Lay::kYWave);
   // counts of step st into count slot q of every tile (lane l's 16 bytes at
   // l * 16)
   auto issuey = [&](int64_t st, int q) {
@@ -1066,7 +1066,7 @@ __device__ __forceinline__ void ws_body(const TileDesign& td, const Counts& c,
   unsigned char ring[Lay::kY + kWsWaves * TPW * Lay::kYWave];
   const int nA = *nactive;
   const int ntile = (nA + 15) / 16;
-  const int wave = threadIdx.x >> 6;
+  const int wave = wave_index();
   int chunk, group;
   const int nchunk = MODE == 2 ? ct.nchunk : td.nchunk;
   tile_coords_at(wb, nwb, nchunk, chunk, group);
@@ -1223,7 +1223,7 @@ __device__ __forceinline__ void irls_accum_body(const TileDesign& td, const Coun
                                                 int wb, int nwb) {
   const int nA = *nactive;
   const int ntile = (nA + 15) / 16;
-  const int wave = threadIdx.x >> 6;
+  const int wave = wave_index();
   int chunk, group;
   tile_coords_at(wb, nwb, td.nchunk, chunk, group);
   const int tile = group * 4 + wave;
