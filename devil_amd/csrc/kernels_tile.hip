@@ -29,6 +29,13 @@
 // product are permuted (row m = cell 4 (m % 4) + m / 4) so that lane l ends
 // up with the four consecutive cells 4 (l / 16) + r of gene slot l % 16:
 // its counts are one 16-byte load and its offsets four consecutive doubles.
+//
+// The file also holds the mixed-precision accumulates on the same tiling
+// (k_irls_accum_mixed: FP32 MFMA; k_irls_accum_bf16: bf16 MFMA from split
+// operands; k_irls_accum_ws / _ws2: the bf16 arithmetic with operands staged
+// in LDS by direct-to-LDS loads), the mixed bread (k_bread_accum_ws) and,
+// with patients, the bread and per-patient score sums in one pass
+// (k_bread_meat_ws), and the joint launches that serve many problems.
 #include <hip/hip_runtime.h>
 
 #include "../../include/devil_hip.h"
