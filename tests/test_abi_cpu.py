@@ -70,3 +70,16 @@ def test_api_argument_errors():
     import devil_amd
     with pytest.raises(NotImplementedError):
         devil_amd.fit_devil(np.ones((2, 4)), np.ones((4, 1)), CUDA=False)
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    """Without the built library every entry into the package raises: there
+    is no CPU path to fall back to."""
+    from devil_amd import _lib
+    monkeypatch.setattr(_lib, "_lib", None)
+    monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "libdevil_hip.so"))
+    with pytest.raises(RuntimeError, match="no CPU fallback"):
+        _lib.lib()
+    import devil_amd
+    with pytest.raises(RuntimeError, match="no CPU fallback"):
+        devil_amd.fit_devil(np.ones((4, 8), np.int32), np.ones((8, 1)))
