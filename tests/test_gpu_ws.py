@@ -126,12 +126,13 @@ def test_mixed_bread_misaligned_rows(dv):
         assert same.all(), f"{key}: {int((~same).sum())} entries differ"
 
 
-def fit_clustered(dv, Y, w, clusters, precision, pad=1, kernel="bf16_ws"):
+def fit_clustered(dv, Y, w, clusters, precision, pad=1, kernel="bf16_ws", od="MOM"):
     ctx = dv.context(0)
     ctx.set_mixed_kernel(kernel)
     ctx.set_tuning(dv._lib.TUNE_PAD_COPY, pad)
     try:
-        fit = dv.fit_devil(Y, w.X, clusters=clusters, size_factors=w.sf, precision=precision)
+        fit = dv.fit_devil(Y, w.X, clusters=clusters, size_factors=w.sf, precision=precision,
+                           overdispersion=od)
     finally:
         ctx.set_tuning(dv._lib.TUNE_PAD_COPY, 1)
     return fit, ctx.fit_info()
@@ -198,4 +199,34 @@ def test_mixed_clustered_patient_shapes(dv, sizes):
             scale = np.abs(ref["beta_sandwiches_null"].cpu().numpy()).reshape(w.G, -1).max(axis=1)
             e = np.abs(a - b).max(axis=1) / scale
             say(f"  {key} error / largest bread entry: max {e.max():.2e}")
+        assert e.max() <= 1e-4, key
+
+
+@pytest.mark.parametrize("od,dtype", [("MLE", "int32"), ("MOM", "float32")])
+def test_mixed_clustered_mle_and_float_counts(dv, od, dtype):
+    """The mixed bread + per-patient score pass after the MLE overdispersion
+    (theta from the FP64 nlminb driver) and on float32 counts: the same
+    iteration counts as the FP64 path, beta within 1e-4 (floor 1e-3), theta
+    within 1e-4 and the bread and sandwich within 1e-4 normwise of the FP64
+    path's."""
+    from devil_amd import synth
+    w = synth.make_workload("C4", G=128, n=6000, seed=782)
+    Y = synth.counts_device(w, device="cuda")
+    if dtype == "float32":
+        Y = Y.float()
+    say(f"[case] mixed clustered, {od}, {dtype} counts, G={w.G} n={w.n}")
+    ref, _ = fit_clustered(dv, Y, w, w.clusters, "fp64", od=od)
+    fit, info = fit_clustered(dv, Y, w, w.clusters, "mixed", od=od)
+    assert info["mixed_kernel"] == "bf16_ws"
+    np.testing.assert_array_equal(fit["iterations"]["beta_iters"].cpu().numpy(),
+                                  ref["iterations"]["beta_iters"].cpu().numpy())
+    b, rb = fit["beta"].cpu().numpy(), ref["beta"].cpu().numpy()
+    eb = float(np.max(np.abs(b - rb) / np.maximum(np.abs(rb), 1e-3)))
+    t, rt = fit["overdispersion"].cpu().numpy(), ref["overdispersion"].cpu().numpy()
+    et = float(np.max(np.abs(t - rt) / np.maximum(np.abs(rt), 1e-8)))
+    say(f"  beta {eb:.2e}  theta {et:.2e}")
+    assert eb <= 1e-4 and et <= 1e-4
+    for key in ("beta_sandwiches_null", "beta_sandwiches"):
+        e = normwise(fit[key], ref[key])
+        say(f"  {key} normwise rel err max {e.max():.2e}")
         assert e.max() <= 1e-4, key
