@@ -230,3 +230,18 @@ def test_mixed_clustered_mle_and_float_counts(dv, od, dtype):
         e = normwise(fit[key], ref[key])
         say(f"  {key} normwise rel err max {e.max():.2e}")
         assert e.max() <= 1e-4, key
+
+
+def test_mixed_clustered_deterministic(dv):
+    """Two mixed clustered fits of the same problem agree bit for bit (the
+    partial sums have a fixed order; no floating-point atomics)."""
+    from devil_amd import synth
+    w = synth.make_workload("C4", G=100, n=6001, seed=783)
+    Y = synth.counts_device(w, device="cuda")
+    say("[case] mixed clustered, twice")
+    a, _ = fit_clustered(dv, Y, w, w.clusters, "mixed")
+    b, _ = fit_clustered(dv, Y, w, w.clusters, "mixed")
+    for key in ("beta", "overdispersion", "beta_sandwiches_null", "beta_sandwiches"):
+        x, y = a[key].cpu().numpy(), b[key].cpu().numpy()
+        same = (x == y) | (np.isnan(x) & np.isnan(y))
+        assert same.all(), f"{key}: {int((~same).sum())} entries differ"
