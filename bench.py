@@ -58,10 +58,17 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def measured_traffic(kdesc, alg_bytes_per_launch):
+# committed rocprofv3 FETCH_SIZE / WRITE_SIZE summaries, by workload
+TRAFFIC_FILES = {"C3": "traffic_irls_C3.json", "C4": "traffic_C4_mixed_clustered.json"}
+
+
+def measured_traffic(config, kdesc, alg_bytes_per_launch):
     """HBM bytes per launch of the kernel from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE measurement (traffic per algorithmic byte), or None."""
-    path = os.path.join(ROOT, "profiles", "traffic_irls_C3.json")
+    WRITE_SIZE measurement of the same workload (traffic per algorithmic byte), or
+    None."""
+    if config not in TRAFFIC_FILES:
+        return None
+    path = os.path.join(ROOT, "profiles", TRAFFIC_FILES[config])
     try:
         with open(path) as f:
             meas = json.load(f)["kernels"]
@@ -460,7 +467,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": kdesc,
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(kdesc, kbytes / max(nl, 1)),
+                     "traffic": measured_traffic(args.config, kdesc, kbytes / max(nl, 1)),
                      "launches_per_step": nl / args.steps,
                      "avg_launch_ms": avg_ms,
                      "algorithmic_bytes_per_launch": kbytes / max(nl, 1)},
