@@ -245,3 +245,34 @@ def test_mixed_clustered_deterministic(dv):
         x, y = a[key].cpu().numpy(), b[key].cpu().numpy()
         same = (x == y) | (np.isnan(x) & np.isnan(y))
         assert same.all(), f"{key}: {int((~same).sum())} entries differ"
+
+
+def test_mixed_new_overdispersion(dv):
+    """overdispersion="new" after the mixed IRLS, against the FP64 path (which
+    the parity tests hold to the oracle): identical beta iteration counts,
+    beta within 1e-4 (floor 1e-3), the same genes failed, and theta within the
+    method's own tolerance (1e-3, src/new_overdispersion.cpp:284): its Laplace
+    mode is found to that tolerance, so its F_k, and theta, jump with the
+    Newton count when beta moves by FP32 rounding."""
+    from devil_amd import synth, _lib as L
+    w = synth.make_workload("C3", G=256, n=6144, seed=784)
+    Y = synth.counts_device(w, device="cuda")
+    say("[case] mixed, new overdispersion")
+    ctx = dv.context(0)
+    ctx.set_mixed_kernel("bf16_ws")
+    ref = dv.fit_devil(Y, w.X, size_factors=w.sf, overdispersion="new", precision="fp64")
+    fit = dv.fit_devil(Y, w.X, size_factors=w.sf, overdispersion="new", precision="mixed")
+    np.testing.assert_array_equal(fit["iterations"]["beta_iters"].cpu().numpy(),
+                                  ref["iterations"]["beta_iters"].cpu().numpy())
+    b, rb = fit["beta"].cpu().numpy(), ref["beta"].cpu().numpy()
+    eb = float(np.max(np.abs(b - rb) / np.maximum(np.abs(rb), 1e-3)))
+    failed = (fit["status"].cpu().numpy() & L.GENE_OD_FAILED) != 0
+    rfailed = (ref["status"].cpu().numpy() & L.GENE_OD_FAILED) != 0
+    np.testing.assert_array_equal(failed, rfailed)
+    t, rt = fit["overdispersion"].cpu().numpy(), ref["overdispersion"].cpu().numpy()
+    ok = ~rfailed
+    et = np.abs(t[ok] - rt[ok]) / np.maximum(np.abs(rt[ok]), 1e-8)
+    say(f"  beta {eb:.2e}  theta rel err median {np.median(et):.2e} p90 "
+        f"{np.quantile(et, 0.9):.2e} max {et.max():.2e} ({ok.sum()} genes)")
+    assert eb <= 1e-4
+    assert et.max() <= 1e-3
