@@ -117,7 +117,7 @@ EXPORTED = ["devil_create", "devil_destroy", "devil_strerror", "devil_last_error
 STAGES = ["init", "irls", "dispersion", "bread_meat", "test_de"]
 KERNELS = ["irls_fp64", "irls_mixed", "mle"]
 TUNE_MIXED_KERNEL, TUNE_CHUNK_CELLS, TUNE_CONCURRENCY, TUNE_MLE_DRIVER, TUNE_PAD_COPY = 0, 1, 2, 3, 4
-TUNE_JOINT, TUNE_GRAPH = 5, 6
+TUNE_JOINT, TUNE_GRAPH, TUNE_BATCH_GENES = 5, 6, 7
 MLE_DRIVERS = {"nlminb": 0, "root": 1}
 SF_METHODS = {"normed_sum": 0, "psinorm": 1, "edgeR": 2}
 # stats::p.adjust.methods

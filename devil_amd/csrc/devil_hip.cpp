@@ -65,7 +65,7 @@ struct devil_ctx {
   std::string err;
   int timing = 0;
   // kernel, chunk cells (0 auto), streams, MLE driver, aligned copies allowed
-  int tune[DEVIL_N_TUNE] = {DEVIL_MIXED_AUTO, 0, 8, DEVIL_MLE_NLMINB, 1, 0, 0};
+  int tune[DEVIL_N_TUNE] = {DEVIL_MIXED_AUTO, 0, 8, DEVIL_MLE_NLMINB, 1, 0, 0, 0};
   std::map<uint64_t, int> tuned;      // autotuner choices by problem shape
   std::map<uint64_t, int> tuned_joint;  // the same for the joint rounds (kernel fixed)
   std::vector<devil_ctx*> subs;       // per-stream contexts of multi-problem fits
@@ -1819,6 +1819,8 @@ int run_setup(devil_ctx* ctx, const devil_problem* pr, devil_outputs* out, ProbR
         (size_t)ld_pad * es + (r.pad_host ? (size_t)pr->counts_ld * es : 0);
     const size_t budget = std::min<size_t>(freeb / 2, (size_t)8 << 30);
     r.batch = std::max<int64_t>(1, std::min<int64_t>(G, (int64_t)(budget / row_bytes)));
+    if (ctx->tune[DEVIL_TUNE_BATCH_GENES] > 0)
+      r.batch = std::min<int64_t>(r.batch, ctx->tune[DEVIL_TUNE_BATCH_GENES]);
     DV_ALLOC(r.cbuf, (size_t)ld_pad * es * r.batch);
     if (r.pad_host) DV_ALLOC(r.stage, (size_t)pr->counts_ld * es * r.batch);
   } else if (r.mixed && es == 4 && ctx->tune[DEVIL_TUNE_PAD_COPY] &&
@@ -2648,6 +2650,8 @@ int devil_set_tuning(devil_ctx* ctx, int32_t key, int64_t value) {
     return fail(ctx, DEVIL_E_ARG, "pad copy must be 0 or 1");
   if (key == DEVIL_TUNE_GRAPH && value != 0 && value != 1)
     return fail(ctx, DEVIL_E_ARG, "graph must be 0 or 1");
+  if (key == DEVIL_TUNE_BATCH_GENES && (value < 0 || value > INT32_MAX))
+    return fail(ctx, DEVIL_E_ARG, "batch genes must be 0 (no cap) or positive");
   if (key == DEVIL_TUNE_JOINT && (value < 0 || value > 4096))
     return fail(ctx, DEVIL_E_ARG, "joint must be 0 (off), 1 (auto) or a unit size <= 4096");
   if (key == DEVIL_TUNE_PAD_COPY && value == 0 && ctx->pad_p) {

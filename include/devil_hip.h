@@ -293,7 +293,10 @@ enum { DEVIL_TUNE_MIXED_KERNEL = 0,  /* DEVIL_MIXED_* (default AUTO) */
                                         (accumulate, clear, update, count copy, twice);
                                         0 (default): launched one by one, queued ahead of the
                                         count reads (the same speed; per-kernel timing) */
-       DEVIL_N_TUNE = 7 };
+       DEVIL_TUNE_BATCH_GENES = 7,   /* host-resident counts are uploaded and fitted in gene
+                                        batches sized by free device memory (up to 8 GiB);
+                                        N > 0 caps a batch at N genes, 0 (default): no cap */
+       DEVIL_N_TUNE = 8 };
 /* MLE overdispersion drivers.  NLMINB: the reference's stats::nlminb call
  * (R/overdispersion.R:53-73) restated for one parameter -- its trust-region
  * Newton steps and its stopping rules (relative function convergence at

@@ -584,3 +584,44 @@ def test_graph_tail_matches_launched_rounds(dv, orc, precision):
         ref = orc.cpu_fit(Y.astype(np.float64), w.X, w.offset)
         np.testing.assert_array_equal(fits[1]["iterations"]["beta_iters"], ref["beta_iters"])
         assert_rel(fits[1]["beta"], ref["beta"], 1e-8, 1e-6, "beta vs oracle")
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_host_counts_in_gene_batches(dv, precision):
+    """Host-resident counts (R's path) are uploaded and fitted in gene
+    batches; with the batch capped at 100 genes (5 batches, the last one
+    partial) and rows that need padding (n = 6001), the per-gene outputs
+    equal the single-batch fit of the same counts on the device up to the
+    order of the FP64 chunk sums (rounds of fewer genes get shorter chunks):
+    identical iteration counts, beta / theta / bread / sandwich within 1e-10
+    of each gene's largest entry, clustered sandwich included.  In the mixed
+    path a last-bit change of beta moves the next rounds' FP32 roundings, so
+    there the bar is the mixed path's own noise level, 1e-6."""
+    from devil_amd import _lib as L, synth
+    w = synth.make_workload("C4", G=437, n=6001, seed=785)
+    Y = synth.counts_host(w)
+    say(f"[case] host counts in batches of 100 genes, {precision}")
+    ctx = dv.context(0)
+    if precision == "mixed":
+        ctx.set_mixed_kernel("bf16_ws")
+        ctx.set_tuning(L.TUNE_CHUNK_CELLS, 2048)
+    try:
+        ctx.set_tuning(L.TUNE_BATCH_GENES, 100)
+        a = dv.fit_devil(Y, w.X, clusters=w.clusters, size_factors=w.sf, precision=precision)
+        ctx.set_tuning(L.TUNE_BATCH_GENES, 0)
+        b = dv.fit_devil(torch.as_tensor(Y, device="cuda"), w.X, clusters=w.clusters,
+                         size_factors=w.sf, precision=precision)
+    finally:
+        ctx.set_tuning(L.TUNE_BATCH_GENES, 0)
+        ctx.set_tuning(L.TUNE_CHUNK_CELLS, 0)
+        ctx.set_mixed_kernel("auto")
+    np.testing.assert_array_equal(np.asarray(a["iterations"]["beta_iters"]),
+                                  b["iterations"]["beta_iters"].cpu().numpy())
+    for key in ("beta", "overdispersion", "beta_sandwiches_null", "beta_sandwiches"):
+        x = np.asarray(a[key]).reshape(w.G, -1)
+        y = b[key].cpu().numpy().reshape(w.G, -1)
+        scale = np.maximum(np.abs(y).max(axis=1, keepdims=True), 1e-300)
+        e = np.nanmax(np.abs(x - y) / scale)
+        say(f"  {key}: max difference / gene's largest entry {e:.2e}")
+        bar = 1e-10 if precision == "fp64" else 1e-6
+        assert np.array_equal(np.isnan(x), np.isnan(y)) and e <= bar, key
