@@ -587,7 +587,8 @@ def test_graph_tail_matches_launched_rounds(dv, orc, precision):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_host_counts_in_gene_batches(dv, precision):
+@pytest.mark.parametrize("dtype", ["int32", "float64"])
+def test_host_counts_in_gene_batches(dv, precision, dtype):
     """Host-resident counts (R's path) are uploaded and fitted in gene
     batches; with the batch capped at 100 genes (5 batches, the last one
     partial) and rows that need padding (n = 6001), the per-gene outputs
@@ -599,8 +600,8 @@ def test_host_counts_in_gene_batches(dv, precision):
     there the bar is the mixed path's own noise level, 1e-6."""
     from devil_amd import _lib as L, synth
     w = synth.make_workload("C4", G=437, n=6001, seed=785)
-    Y = synth.counts_host(w)
-    say(f"[case] host counts in batches of 100 genes, {precision}")
+    Y = synth.counts_host(w).astype(dtype)  # float64: an R double matrix
+    say(f"[case] host {dtype} counts in batches of 100 genes, {precision}")
     ctx = dv.context(0)
     if precision == "mixed":
         ctx.set_mixed_kernel("bf16_ws")
